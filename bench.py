@@ -1,0 +1,220 @@
+#!/usr/bin/env python3
+"""Benchmark: steady SIMPLE iterations per second on one MI355X.
+
+Workload (default): 3D channel, box 10 x 2 x 2, 200 x 40 x 40 hexahedra
+(320 000 cells), interior vertices jittered +-10% h (seed 0), uniform inlet
+u = 1 (-X), p = 0 outlet (+X), no-slip walls, nu = 0.01 (Re = 200 on the
+height), upwind convection, linear + skew face interpolation,
+alpha_u = 0.7 / alpha_p = 0.3, the reference's Krylov settings (BiCGSTAB for
+momentum, CG for pressure, Jacobi, atol 1e-5, maxiter 500, check every 10).
+One step = one SIMPLE outer iteration.  Inputs are resident in HBM before
+the timed region.
+
+Prints ONE JSON line.  Besides the contract fields it carries
+``roofline`` (the dominant kernel's algorithmic bytes per launch over its
+average launch time, HIP events recorded by the library around every launch
+during a profiled pass of the same number of steps) and ``cpu_baseline``
+(the CPU oracle, single thread, on a bounded sample of the same workload).
+
+    python bench.py [--steps K] [--warmup W] [--workload channel|cavity]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    'channel': dict(desc='channel 10x2x2, 200x40x40 hex (320000 cells), interior vertices jittered '
+                         '+-10% h (seed 0), inlet u=1, outlet p=0, no-slip walls, Re=200 (nu=0.01), '
+                         'steady SIMPLE, upwind (implicit convective outflow), alpha_u 0.7 / alpha_p 0.3, '
+                         'reference Krylov settings', kind='channel'),
+    'cavity': dict(desc='lid-driven cavity 64x64x1 hex (4096 cells), Re=100, steady SIMPLE, upwind, '
+                        'alpha_u 0.7 / alpha_p 0.3, reference Krylov settings', kind='cavity'),
+}
+
+
+def build_case(workload):
+    from warpcfd_amd.preprocess import Mesh, define_boundary_walls, channel_mesh, cavity_mesh
+    if workload == 'channel':
+        nodes, cells = channel_mesh(200, 40, 40, (10.0, 2.0, 2.0), jitter=0.1, seed=0)
+    else:
+        nodes, cells = cavity_mesh(64)
+    return nodes, cells
+
+
+def apply_bcs(b, kind):
+    if kind == 'channel':
+        b.velocity_BC('-X', u=1.0, v=0.0, w=0.0)
+        b.pressure_BC('+X', p=0.0)
+        for w in ('-Y', '+Y', '-Z', '+Z'):
+            b.no_slip_wall(w)
+    else:
+        for w in ('-X', '+X', '-Y'):
+            b.no_slip_wall(w)
+        b.velocity_BC('+Y', u=1.0, v=0.0, w=0.0)
+        b.slip_wall('+Z'); b.slip_wall('-Z')
+
+
+def algorithmic_bytes(kernel, C, F, Fi, Fb, nnz):
+    """Compulsory bytes of one launch: every input element read once, every
+    output element written once (float32 / int32 = 4 B)."""
+    rp = 4 * (C + 1)
+    table = {
+        # q = A (z + beta p) gathered on the fly, p_out, q; reads r, inv, p_in
+        'cg_1': 8 * nnz + rp + 12 * C + 8 * C,
+        # x += a p, r -= a q: reads x p r q inv, writes x r
+        'cg_2': 28 * C,
+        'bicgstab_1': 8 * nnz + rp + 4 * C + 36 * C + 24 * C + 12 * C,
+        'bicgstab_2': 8 * nnz + rp + 4 * C + 24 * C + 24 * C,
+        'bicgstab_3': 64 * C + 24 * C,
+        # owner/neigh, psi, skew, normal, area, dmag, bc; cell values + grads; face values + flux
+        'face_interp': 4 * F + 4 * Fi + 4 * Fi + 12 * Fi + 12 * F + 4 * F + 4 * Fb + 20 * Fb
+                       + 16 * C + 48 * C + 16 * F + 4 * F,
+        # slots, sides, sn, volume; face values 4 fields; grads out; matrix + rhs; flux; bc
+        'assemble_momentum': 24 * C + 6 * C + 72 * C + 4 * C + 16 * F + 48 * C + 4 * nnz + 12 * C
+                             + 4 * F + 24 * C + 4 * C + 20 * Fb + 16 * C,
+        'assemble_pressure': 24 * C + 6 * C + 72 * C + 4 * C + 4 * F + 12 * C + 4 * nnz + 4 * C
+                             + 8 * F + 24 * C + 4 * C + 8 * Fb,
+    }
+    return table.get(kernel)
+
+
+def cpu_baseline(nodes, cells, kind, budget_s=20.0, max_iters=6):
+    """CPU oracle (numpy restatement of the reference path), one thread."""
+    try:
+        from threadpoolctl import threadpool_limits
+        limiter = threadpool_limits(limits=1)
+    except Exception:
+        limiter = None
+    sys.path.insert(0, os.path.join(ROOT, 'tests'))
+    from oracle.mesh import OracleMesh
+    from oracle.fv import OracleFVM
+    from oracle.simple import OracleSolver
+    om = OracleMesh(nodes, cells)
+    om.define_boundary_walls()
+    of = OracleFVM(om)
+    if kind == 'cavity':
+        of.set_reference_pressure(0, 0.0)
+    apply_bcs(of.boundary, kind)
+    of.boundary.finalize()
+    s = OracleSolver(of, 0.7, 0.3, nu=0.01, implicit_outflow=(kind == 'channel'))
+    s.outer_iteration()                     # warm-up (untimed)
+    t0 = time.perf_counter()
+    n = 0
+    while n < max_iters:
+        s.outer_iteration()
+        n += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    if limiter is not None:
+        limiter.unregister() if hasattr(limiter, 'unregister') else None
+    return {'value': n / dt, 'unit': 'iter/s', 'cores': 1, 'kind': 'port',
+            'sample': f'{n} SIMPLE iterations (after 1 untimed) of the same workload, oracle '
+                      f'(numpy/scipy restatement of the reference path), 1 thread, {dt:.1f} s'}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--workload', default='channel', choices=sorted(WORKLOADS))
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-profile', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py runs on one GPU only: this build has no multi-GPU path '
+              '(cases shard across GPUs with no communication; see the design notes).',
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from warpcfd_amd import _lib
+    from warpcfd_amd.preprocess import Mesh, define_boundary_walls
+    from warpcfd_amd.FV import FVM
+    from warpcfd_amd.FV.implicit_Solvers import IncompressibleSolver
+
+    wl = WORKLOADS[args.workload]
+    nodes, cells = build_case(args.workload)
+    m = Mesh(nodes, cells)
+    define_boundary_walls(m)
+    model = FVM(m)
+    model.material.create_incompressible_newtonian_fluid(viscosity=0.01, density=1.0)
+    if wl['kind'] == 'cavity':
+        model.set_reference_pressure(0, 0.0)
+    apply_bcs(model.boundary, wl['kind'])
+    model.finalize()
+    solver = IncompressibleSolver(model, 0.7, 0.3, implicit_outflow=(wl['kind'] == 'channel'))
+    dm = model.dm
+    C, F, Fi, Fb, nnz = dm.n_cells, dm.n_faces, dm.n_internal, dm.n_boundary, dm.nnz
+
+    for _ in range(args.warmup):
+        solver.outer_iteration()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        solver.outer_iteration()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    hist = solver.history[-args.steps:]
+
+    roofline = None
+    kernels = None
+    if not args.no_profile:
+        with _lib.Profiler(capacity=200000) as prof:
+            for _ in range(args.steps):
+                solver.outer_iteration()
+            torch.cuda.synchronize()
+        kernels = {k: {'total_ms': round(v[0], 3), 'launches': v[1], 'avg_us': round(1e3 * v[0] / v[1], 3)}
+                   for k, v in sorted(prof.result.items(), key=lambda kv: -kv[1][0])}
+        dominant = max(prof.result.items(), key=lambda kv: kv[1][0])[0]
+        ms, cnt = prof.result[dominant]
+        byts = algorithmic_bytes(dominant, C, F, Fi, Fb, nnz)
+        if byts is not None:
+            avg_s = ms / cnt / 1e3
+            ach = byts / avg_s / 1e9
+            roofline = {'bound': 'hbm', 'kernel': dominant, 'achieved': round(ach, 1),
+                        'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(ach / HBM_PEAK_GBS, 4),
+                        'traffic': None, 'bytes_per_launch': byts,
+                        'avg_launch_us': round(avg_s * 1e6, 3)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(nodes, cells, wl['kind'])
+
+    out = {
+        'metric': 'SIMPLE iter/s + %HBM roofline; served solves/s, p50/p99 at equal residual',
+        'value': round(args.steps / elapsed, 3),
+        'unit': 'iter/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': round(1e3 * elapsed / args.steps, 3),
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32',
+        'data': 'synthetic',
+        'config': {'workload': wl['desc'], 'cells': C, 'faces': F, 'nnz': nnz,
+                   'reductions': 'f64', 'krylov': {'atol': 1e-5, 'maxiter': 500, 'check_every': 10}},
+        'roofline': roofline,
+        'cpu_baseline': cpu,
+        'continuity_residual': hist[-1]['continuity'],
+        'krylov_iters_per_step': {'momentum': float(np.mean([h['u_iters'] for h in hist])),
+                                  'pressure': float(np.mean([h['p_iters'] for h in hist]))},
+        'kernels': kernels,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

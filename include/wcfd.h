@@ -1,0 +1,274 @@
+/*
+ * wcfd.h -- C ABI of libwcfd.so, the gfx950 HIP implementation of the
+ * SIMPLE/PISO inner loop of JohnCSu/WarpCFD.
+ *
+ * Conventions
+ *  - Every pointer is a device pointer unless marked [host].  The caller owns
+ *    all memory; the library allocates nothing.  Scratch space comes from a
+ *    caller-provided workspace whose size wcfd_*_workspace_bytes() returns.
+ *  - Arrays are structure-of-arrays in the device layout: cells renumbered
+ *    (reverse Cuthill-McKee), internal faces first then boundary faces.
+ *    "[4][C]" means 4 contiguous rows of C floats (u, v, w, p).
+ *  - Every function returns WCFD_OK or an error code; on WCFD_EHIP the HIP
+ *    error is kept per thread and readable with wcfd_last_hip_error().
+ *  - `stream` is a hipStream_t passed as void* (0 = the null stream).
+ *  - Functions keep no global mutable state (the opt-in profiler below
+ *    aside); calls on different streams with different workspaces may run
+ *    concurrently.
+ *
+ * Each entry point cites the reference function it replaces
+ * (paths relative to the JohnCSu/WarpCFD repository root).
+ */
+#ifndef WCFD_H
+#define WCFD_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WCFD_OK 0
+#define WCFD_EINVAL 1
+#define WCFD_EHIP 2
+#define WCFD_ENOTCONV 3
+#define WCFD_ENAN 4
+
+#define WCFD_DIRICHLET 1
+#define WCFD_NEUMANN 2
+
+/* scheme flags */
+#define WCFD_SCHEME_CORRECTED 1        /* over-relaxed non-orthogonal Laplacian
+                                          (laplacianScheme.py:39-66) */
+#define WCFD_SCHEME_IMPLICIT_OUTFLOW 2 /* not in the reference: implicit upwind
+                                          convective outflow on Neumann faces */
+
+/* Read-only mesh description (replaces the Cell/Face struct arrays of
+ * warp_cfd/FV/mesh_structs.py:5-64 built by init_structs 141-251). */
+typedef struct wcfd_mesh {
+  int32_t n_cells;      /* C */
+  int32_t n_faces;      /* F = Fi + Fb */
+  int32_t n_internal;   /* Fi, faces 0..Fi-1 */
+  int32_t n_boundary;   /* Fb, faces Fi..F-1 */
+  int32_t nnz;          /* nnz of the C x C CSR pattern */
+  int32_t reserved;
+  const int32_t* owner;     /* [F]    owner cell (the reference's owner) */
+  const int32_t* neigh;     /* [Fi]   neighbour cell */
+  const float* psi;         /* [Fi]   |x_f - x_O| / |x_O - x_N| (mesh_structs.py:206) */
+  const float* skew;        /* [3][Fi] x_f - (psi x_O + (1-psi) x_N) */
+  const float* area;        /* [F]    owner-side face area (mesh_structs.py:200) */
+  const float* normal;      /* [3][F] owner-side normal */
+  const float* dist;        /* [3][Fi] x_N - x_O */
+  const float* dmag;        /* [F]    |x_N - x_O| (internal), |x_f - x_P| (boundary) */
+  const float* volume;      /* [C] */
+  const int32_t* slot_face; /* [6][C] face of local face k of cell c */
+  const int8_t* slot_side;  /* [6][C] 0 if c owns that face, 1 otherwise */
+  const float* slot_sn;     /* [3][6][C] A_f * n_{c,k} (the cell's own normal) */
+  const int32_t* slot_pos;  /* [6][C] CSR index of (c, neighbour) or -1 */
+  const int32_t* diag_pos;  /* [C]    CSR index of (c, c) */
+  const int32_t* row_ptr;   /* [C+1] */
+  const int32_t* col;       /* [nnz]  columns sorted within a row */
+} wcfd_mesh;
+
+/* Boundary conditions per boundary face (warp_cfd/FV/boundary/boundary.py). */
+typedef struct wcfd_bc {
+  const uint8_t* type;  /* [4][Fb] 1 Dirichlet, 2 Neumann */
+  const float* value;   /* [4][Fb] value or normal gradient */
+} wcfd_bc;
+
+/* ---------------------------------------------------------------- errors */
+int wcfd_last_hip_error(void);
+const char* wcfd_error_string(int code); /* [host] static string */
+int wcfd_abi_version(void);
+
+/* ------------------------------------------------------------ face/cell ops */
+
+/* FVM.set_boundary_conditions (warp_cfd/FV/model.py:206-208,
+ * FV/boundary/conditions.py:6-19): Dirichlet face values for the fields in
+ * field_mask (bit q = field q). */
+int wcfd_set_boundary_conditions(const wcfd_mesh* m, wcfd_bc bc, float* face_values,
+                                 int field_mask, void* stream);
+
+/* FVM.face_interpolation (model.py:210-229; interpolation_kernels.py:5-54;
+ * skewness_functions.py:7-20): internal faces linear in psi plus skew
+ * correction with cell_grad, Neumann boundary faces |x_f-x_P| g + phi_P.
+ * skew_mask selects the fields whose skew term is evaluated (fields whose
+ * gradients are known to be zero may be skipped).  If mass_flux != NULL the
+ * face flux (u_f . n_O) A_f is written too (model_kernels.py:46-60); if
+ * rua_cell != NULL the face value of rUA (solver_kernels.py:22-36) is
+ * written to rua_face. */
+int wcfd_face_interpolation(const wcfd_mesh* m, wcfd_bc bc, const float* cell_values,
+                            const float* cell_grad, float* face_values, int field_mask,
+                            int skew_mask, float* mass_flux, const float* rua_cell,
+                            float* rua_face, void* stream);
+
+/* FVM.calculate_gradients (model.py:232-239; model_kernels.py:72-90):
+ * Green-Gauss gradient of the fields in field_mask; when zero_others != 0 the
+ * gradients of the other fields are zeroed, as the reference's zero_() does. */
+int wcfd_green_gauss(const wcfd_mesh* m, const float* face_values, float* cell_grad,
+                     int field_mask, int zero_others, void* stream);
+
+/* FVM.calculate_mass_flux (model.py:264-266; model_kernels.py:46-60). */
+int wcfd_mass_flux(const wcfd_mesh* m, const float* face_values, float* mass_flux,
+                   void* stream);
+
+/* FVM.divFlux (model.py:269-275; model_kernels.py:27-34). */
+int wcfd_divergence(const wcfd_mesh* m, const float* mass_flux, float* div, void* stream);
+
+/* Momentum system, fused: ConvectionTerm upwind (convection.py:44-88,
+ * convectionScheme.py:28-52) + DiffusionTerm (diffusion.py:113-197,
+ * laplacianScheme.py:14-66) + GradTerm (grad.py:22-27) [+ DdtTerm
+ * (ddt.py:25-33) when dt > 0] -> Equation.form_system (equation.py:99-124,
+ * equation_kernels.py:52-122) -> Equation.relax (equation_kernels.py:128-152)
+ * when alpha > 0.  One C x C value array shared by u, v, w (requires the same
+ * BC type for u, v, w on every face); three right-hand sides rhs[3][C].
+ * If compute_gradients != 0 the Green-Gauss gradients of all 4 fields are
+ * computed first (model_kernels.py:72-90) into cell_grad and used for the p
+ * term.  scheme_flags: WCFD_SCHEME_* bits. */
+int wcfd_assemble_momentum(const wcfd_mesh* m, wcfd_bc bc, const float* cell_values,
+                           const float* face_values, const float* mass_flux,
+                           float* cell_grad, int compute_gradients, float nu,
+                           float inv_density, float alpha_u, float dt, int scheme_flags,
+                           float* values, float* rhs, void* stream);
+
+/* Pressure equation, fused: divFlux (model_kernels.py:27-34) + DiffusionTerm
+ * with per-face viscosity rUA_f (diffusion.py:40-49) -> form_system + add_RHS
+ * (equation.py:99-124,169-170) -> replace_row (equation_kernels.py:155-175)
+ * when ref_cell >= 0.  If compute_grad_p != 0 the Green-Gauss gradient of p
+ * is computed into cell_grad (other gradients untouched).  div_out may be
+ * NULL. */
+int wcfd_assemble_pressure(const wcfd_mesh* m, wcfd_bc bc, const float* face_values,
+                           const float* mass_flux, const float* rua_face, float* cell_grad,
+                           int compute_grad_p, int correction, int ref_cell, float ref_value,
+                           float* values, float* rhs, float* div_out, void* stream);
+
+/* calculate_rUA + get_HbyA + replace_cell_values (solver_kernels.py:7-16,
+ * 39-43; model_kernels.py:104-109): rUA = V / (a_PP rho), HbyA += rUA grad p,
+ * cell_values[u,v,w] = HbyA.  HbyA is [3][C]. */
+int wcfd_rua_hbya(const wcfd_mesh* m, const float* values, float density,
+                  const float* cell_grad, float* hbya, float* cell_values, float* rua,
+                  void* stream);
+
+/* FVM.relax (model.py:256-262,381-384): p = alpha p_new + (1-alpha) p;
+ * alpha <= 0 means replace (model.py:147). */
+int wcfd_relax_pressure(const wcfd_mesh* m, const float* p_new, float alpha,
+                        float* cell_values, void* stream);
+
+/* Corrector tail (incompressible.py:150-156): face_interpolation('p'),
+ * calculate_gradients('p') (zeroes u,v,w gradients, T3), HbyA -= rUA grad p,
+ * cell_values[u,v,w] = HbyA.  Also the float64 continuity residual
+ * sum_c|sum_f +-F_c| / sum_c sum_f |F_c| of the corrected flux
+ * F_c = F - rUA_f A_f (p_N - p_O)/|d|, written to resid[0..1]
+ * (numerator, denominator) if resid != NULL.  Needs a workspace of
+ * wcfd_reduce_workspace_bytes(F) bytes, zero-filled before its first use
+ * (the library leaves its counters at zero after every call). */
+int wcfd_correct_velocity(const wcfd_mesh* m, wcfd_bc bc, float* cell_values,
+                          float* cell_grad, float* face_values, const float* mass_flux,
+                          const float* rua, const float* rua_face, float* hbya,
+                          double* resid, void* workspace, void* stream);
+
+size_t wcfd_reduce_workspace_bytes(int32_t n_faces);
+
+/* ------------------------------------------------------------------ Krylov */
+
+/* y[q] = A x[q] for q < nvec, CSR (row_ptr, col, values) of size n. */
+int wcfd_spmv(int32_t n, const int32_t* row_ptr, const int32_t* col, const float* values,
+              int nvec, const float* x, float* y, void* stream);
+
+/* float64 dot product of two float32 vectors, result to out[0]. */
+int wcfd_dot_f64(int32_t n, const float* a, const float* b, double* out, void* workspace,
+                 void* stream);
+
+size_t wcfd_krylov_workspace_bytes(int32_t n, int nvec);
+
+/* Result of a Krylov solve (host struct). */
+typedef struct wcfd_solve_info {
+  int32_t iterations;
+  int32_t converged;
+  double residual;   /* ||r||_2 over all components */
+  double threshold;  /* max(tol ||b||, atol) */
+} wcfd_solve_info;
+
+/* warp.optim.linear.cg with the Jacobi preconditioner, as called by
+ * Equation.solve_Axb (equation.py:195-200; incompressible.py:32,142).
+ * x is updated in place (warm start).  Stops when ||r|| <= max(tol ||b||,
+ * atol), checked before the first iteration and every check_every
+ * iterations, or at maxiter.  info is [host]. */
+int wcfd_cg(int32_t n, const int32_t* row_ptr, const int32_t* col, const float* values,
+            const int32_t* diag_pos, const float* b, float* x, double atol, double tol,
+            int maxiter, int check_every, void* workspace, wcfd_solve_info* info,
+            void* stream);
+
+/* warp.optim.linear.bicgstab, right Jacobi preconditioning, as called by
+ * Equation.solve_Axb for the momentum system (equation.py:195-200;
+ * incompressible.py:116).  nvec right-hand sides b[nvec][n] share the matrix
+ * and all scalars (the reference's single interleaved 3C solve). */
+int wcfd_bicgstab(int32_t n, const int32_t* row_ptr, const int32_t* col, const float* values,
+                  const int32_t* diag_pos, int nvec, const float* b, float* x, double atol,
+                  double tol, int maxiter, int check_every, void* workspace,
+                  wcfd_solve_info* info, void* stream);
+
+/* ------------------------------------------------------ whole SIMPLE step */
+
+typedef struct wcfd_simple_params {
+  float nu;            /* kinematic viscosity mu/rho (incompressible.py:89) */
+  float density;
+  float alpha_u;       /* <= 0: no momentum relaxation */
+  float alpha_p;       /* <= 0: replace p */
+  float dt;            /* > 0: transient (DdtTerm); 0: steady */
+  int32_t scheme_flags; /* WCFD_SCHEME_* bits */
+  int32_t ref_cell;    /* device cell id of the reference-pressure row, -1 none */
+  float ref_value;
+  int32_t n_inner;     /* NUM_INNER_LOOPS */
+  int32_t n_ortho;     /* NUM_ORTHOGONAL_CORRECTORS */
+  double atol, tol;    /* Krylov tolerances */
+  int32_t maxiter, check_every;
+  int32_t first_iteration; /* 1: u,v,w gradients may be non-zero on entry */
+  int32_t reserved;
+} wcfd_simple_params;
+
+/* Device state of one case. */
+typedef struct wcfd_simple_state {
+  float* cell_values;  /* [4][C] */
+  float* cell_grad;    /* [4][3][C] */
+  float* face_values;  /* [4][F] */
+  float* mass_flux;    /* [F] */
+  float* hbya;         /* [3][C] momentum solution / HbyA */
+  float* p_x;          /* [C] pressure solution (warm start) */
+  float* rua;          /* [C] */
+  float* rua_face;     /* [F] */
+  float* mom_values;   /* [nnz] */
+  float* mom_rhs;      /* [3][C] */
+  float* p_values;     /* [nnz] */
+  float* p_rhs;        /* [C] */
+  double* resid;       /* [2] continuity numerator / denominator */
+} wcfd_simple_state;
+
+typedef struct wcfd_simple_info {
+  wcfd_solve_info momentum;
+  wcfd_solve_info pressure; /* last corrector */
+  double continuity;        /* numerator / denominator */
+} wcfd_simple_info;
+
+size_t wcfd_simple_workspace_bytes(int32_t n_cells, int32_t n_faces);
+
+/* One outer iteration of IncompressibleSolver.run (incompressible.py:95-156)
+ * on the device.  info is [host] and may be NULL (then no host sync at the
+ * end; Krylov checks still sync every check_every iterations). */
+int wcfd_simple_iteration(const wcfd_mesh* m, wcfd_bc bc, const wcfd_simple_params* prm,
+                          const wcfd_simple_state* st, void* workspace,
+                          wcfd_simple_info* info, void* stream);
+
+/* ---------------------------------------------------------------- profiler
+ * No counterpart in the reference (it has no timers).  While active, every
+ * kernel launch of the library records a start/stop hipEvent pair on its
+ * stream (up to `capacity` launches); wcfd_profile_end() waits for them and
+ * returns per-kernel total milliseconds and launch counts, indexed by kernel
+ * id (names from wcfd_kernel_name). */
+int wcfd_kernel_count(void);
+const char* wcfd_kernel_name(int id); /* [host] static string */
+int wcfd_profile_begin(int capacity);
+int wcfd_profile_end(double* total_ms, int32_t* counts, int n_ids); /* [host] arrays */
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WCFD_H */

@@ -1,0 +1,186 @@
+"""Oracle restatement of IncompressibleSolver.run (test infrastructure).
+
+Follows FV/implicit_Solvers/incompressible.py:11-172 step by step, with the
+reference quirks T1-T14 (face-interpolation weight psi, dropped internal
+explicit terms, gradient zeroing by calculate_gradients('p'), the identity
+reference-pressure row, warm-started Krylov solves, no early stop).  The
+pressure-correction stage also returns the continuity residual
+
+    r = sum_c |sum_f +-F_c| / sum_c sum_f |F_c|     (float64)
+
+of the corrected conservative flux F_c = F_HbyA - rUA_f A_f (p_N - p_O)/|d|
+(boundary: the Dirichlet-p correction rUA_f A_f (p_b - p_P)/|x_f - x_P|),
+evaluated with the relaxed pressure.  The reference itself reports the L1
+divergence of the pre-correction flux (model.py:286; Convergence.py:55), which
+does not tend to zero; ours is the stop test the batched/served solvers use.
+"""
+import numpy as np
+import scipy.sparse as sp
+
+from .fv import (OracleFVM, OracleEquation, convection_weights, diffusion_weights,
+                 grad_term_weights, ddt_weights, NEUMANN, DIRICHLET)
+from . import krylov
+
+
+class OracleSolver:
+    def __init__(self, fvm, u_relax=0.7, p_relax=0.3, correction=False, steady_state=True,
+                 nu=0.01, rho=1.0, atol=None, maxiter=500, check_every=10,
+                 implicit_outflow=False):
+        self.fvm = fvm
+        self.u_relax = u_relax
+        self.p_relax = p_relax
+        self.correction = correction
+        self.steady_state = steady_state
+        self.nu = nu                 # kinematic viscosity = mu / rho (incompressible.py:89)
+        self.rho = rho
+        d = fvm.dtype
+        self.atol = atol if atol is not None else (1e-5 if d == np.float32 else 1e-6)
+        self.maxiter = maxiter
+        self.check_every = check_every
+        self.implicit_outflow = implicit_outflow
+        C = fvm.m.num_cells
+        self.vel_eq = OracleEquation(fvm, [0, 1, 2])
+        self.p_eq = OracleEquation(fvm, [3])
+        self.HbyA = np.zeros(3 * C, d)
+        self.rUA = np.zeros(C, d)
+        self.rUA_faces = np.zeros(fvm.m.num_faces, d)
+        self.n_outer = 1
+        self.n_inner = 1
+        self.n_ortho = 1
+        self.max_time_steps = 1
+        self.dt = 1.0
+        self.history = []            # per outer iteration: dict of diagnostics
+
+    def set_iterations(self, n_outer, n_inner=1, n_ortho=1):
+        self.n_outer, self.n_inner, self.n_ortho = n_outer, n_inner, n_ortho
+
+    def set_time_stepping(self, max_time_steps, dt):
+        self.max_time_steps, self.dt = max_time_steps, dt
+
+    # solver_kernels.py:7-16, 22-36
+    def _rua(self, Ap):
+        d = self.fvm.dtype
+        self.rUA[:] = self.fvm.volumes / (Ap[0::3] * d.type(self.rho))
+        m = self.fvm.m
+        own = m.adjacent_cells[:, 0]
+        nb = m.adjacent_cells[:, 1]
+        internal = m.is_boundary == 0
+        rf = self.rUA[own].copy()
+        rf[internal] = (self.fvm.psi[internal] * self.rUA[own[internal]]
+                        + self.fvm.psi2[internal] * self.rUA[nb[internal]])
+        self.rUA_faces[:] = rf
+
+    def continuity_residual(self, p):
+        """float64 normalised L1 imbalance of the corrected conservative flux."""
+        fvm = self.fvm
+        m = fvm.m
+        F = fvm.mass_fluxes.astype(np.float64)
+        Fc = F.copy()
+        fi = m.internal_face_ids
+        own, nb = m.adjacent_cells[fi, 0], m.adjacent_cells[fi, 1]
+        d = fvm.cell_distance[fi].astype(np.float64)
+        dmag = np.sqrt((d * d).sum(axis=1))
+        coef = self.rUA_faces[fi].astype(np.float64) * fvm.face_area[fi] / dmag
+        Fc[fi] = F[fi] - coef * (p[nb].astype(np.float64) - p[own])
+        bnd = fvm.boundary
+        b = bnd.boundary_ids
+        dirp = bnd.types[:, 3] == DIRICHLET
+        ob, kb = m.adjacent_cells[b, 0], m.cell_face_index[b, 0]
+        c2f = fvm.c2f[ob, kb].astype(np.float64)
+        dm = np.sqrt((c2f * c2f).sum(axis=1))
+        coef_b = self.rUA_faces[b].astype(np.float64) * fvm.face_area[b] / dm
+        pb = fvm.face_values[b, 3].astype(np.float64)
+        Fc[b[dirp]] = F[b[dirp]] - coef_b[dirp] * (bnd.values[dirp, 3].astype(np.float64) - p[ob[dirp]])
+        num = np.zeros(m.num_cells)
+        den = np.zeros(m.num_cells)
+        for k in range(6):
+            f = m.cell_faces[:, k]
+            s = np.where(m.face_side[:, k] == 0, 1.0, -1.0)
+            num += s * Fc[f]
+            den += np.abs(Fc[f])
+        tot = den.sum()
+        return float(np.abs(num).sum() / tot) if tot > 0 else 0.0
+
+    def outer_iteration(self, dt_step=None):
+        fvm = self.fvm
+        d = fvm.dtype
+        inv_density = 1.0 / self.rho
+        info = {}
+        fvm.set_boundary_conditions()
+        fvm.face_interpolation()
+        fvm.calculate_gradients()
+        fvm.calculate_mass_flux()
+        wc = convection_weights(fvm, (0, 1, 2), self.implicit_outflow)
+        wd = diffusion_weights(fvm, self.nu, (0, 1, 2), self.correction)
+        wg = grad_term_weights(fvm, 3)
+        implicit = [('face', wc, 1.0), ('face', wd, -1.0)]
+        if not self.steady_state:
+            implicit = [('cell', ddt_weights(fvm, dt_step), 1.0)] + implicit
+        self.vel_eq.form_system(implicit, explicit=[(wg, -inv_density)])
+        if self.u_relax is not None:
+            self.vel_eq.relax(self.u_relax)
+        Ap = self.vel_eq.diagonal
+        A = self.vel_eq.matrix()
+        inv = (1.0 / Ap).astype(d)
+        it_u, res_u = krylov.bicgstab(A, self.vel_eq.rhs, self.HbyA, inv, self.atol,
+                                      maxiter=self.maxiter, check_every=self.check_every)
+        info['u_iters'] = it_u
+        p_grad = fvm.get_gradient(3).reshape(-1)
+        self._rua(Ap)
+        self.HbyA += p_grad * np.repeat(self.rUA, 3)
+        fvm.replace_cell_values([0, 1, 2], self.HbyA)
+        p_iters = []
+        for _ in range(self.n_inner):
+            for _ in range(self.n_ortho):
+                fvm.face_interpolation()
+                fvm.calculate_gradients()
+                fvm.calculate_mass_flux()
+                div_u = fvm.divFlux()
+                wp_ = diffusion_weights(fvm, self.rUA_faces, (3,), self.correction)
+                self.p_eq.form_system([('face', wp_, 1.0)])
+                self.p_eq.add_RHS(div_u)
+                if fvm.reference_pressure_cell_id is not None:
+                    self.p_eq.replace_row(fvm.reference_pressure_cell_id, fvm.reference_pressure)
+                Pd = self.p_eq.diagonal
+                invp = (1.0 / Pd).astype(d)
+                it_p, res_p = krylov.cg(self.p_eq.matrix(), self.p_eq.rhs, self.p_eq.x, invp,
+                                        self.atol, maxiter=self.maxiter, check_every=self.check_every)
+                p_iters.append(it_p)
+                p_new = self.p_eq.x
+                if self.p_relax is not None:
+                    fvm.relax(p_new, self.p_relax, 3)
+                else:
+                    fvm.replace_cell_values([3], p_new)
+                info['continuity'] = self.continuity_residual(fvm.cell_values[:, 3])
+                info['div_hbya_l1'] = float(np.abs(div_u.astype(np.float64)).sum())
+                fvm.face_interpolation((3,))
+                fvm.calculate_gradients((3,))
+                vel_corr = fvm.get_gradient(3, self.rUA).reshape(-1)
+                self.HbyA -= vel_corr
+                fvm.replace_cell_values([0, 1, 2], self.HbyA)
+        info['p_iters'] = p_iters
+        self.history.append(info)
+        return info
+
+    def run(self):
+        """incompressible.py:91-172 without printing (no early stop, T10)."""
+        for _ in range(self.max_time_steps):
+            for _ in range(self.n_outer):
+                self.outer_iteration(self.dt)
+        return self.history
+
+
+def lid_driven_cavity(n, dtype=np.float32, nu=0.01, dz=0.1, mesh_builder=None):
+    """The LDC set-up of examples/LDC/LDC.py:24-48 on an n x n x 1 hex mesh."""
+    from .mesh import OracleMesh
+    nodes, cells = mesh_builder(n)
+    m = OracleMesh(nodes, cells)
+    m.define_boundary_walls()
+    fvm = OracleFVM(m, dtype)
+    fvm.set_reference_pressure(0, 0.0)
+    b = fvm.boundary
+    b.no_slip_wall('-X'); b.no_slip_wall('+X'); b.no_slip_wall('-Y')
+    b.velocity_BC('+Y', u=1.0, v=0.0, w=0.0)
+    b.slip_wall('+Z'); b.slip_wall('-Z')
+    b.finalize()
+    return fvm

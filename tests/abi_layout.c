@@ -1,0 +1,20 @@
+/* Prints sizeof/offsetof of the C-ABI structs (compiled by gcc in
+ * test_abi.py to check the ctypes mirrors in warpcfd_amd/_lib.py). */
+#include <stddef.h>
+#include <stdio.h>
+#include "wcfd.h"
+#define P(T, F) printf(#T "." #F " %zu\n", offsetof(T, F))
+int main(void) {
+  printf("wcfd_mesh %zu\n", sizeof(wcfd_mesh));
+  P(wcfd_mesh, owner); P(wcfd_mesh, col); P(wcfd_mesh, slot_sn);
+  printf("wcfd_bc %zu\n", sizeof(wcfd_bc));
+  printf("wcfd_solve_info %zu\n", sizeof(wcfd_solve_info));
+  P(wcfd_solve_info, residual);
+  printf("wcfd_simple_params %zu\n", sizeof(wcfd_simple_params));
+  P(wcfd_simple_params, atol); P(wcfd_simple_params, maxiter); P(wcfd_simple_params, first_iteration);
+  printf("wcfd_simple_state %zu\n", sizeof(wcfd_simple_state));
+  P(wcfd_simple_state, resid);
+  printf("wcfd_simple_info %zu\n", sizeof(wcfd_simple_info));
+  P(wcfd_simple_info, continuity);
+  return 0;
+}

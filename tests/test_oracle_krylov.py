@@ -1,0 +1,63 @@
+"""Pins oracle.krylov (our statement of warp.optim.linear.cg / bicgstab) by
+the true residual and scipy's direct solver, and fixes the stopping rule the
+HIP solvers follow: check before the first iteration and every check_every
+iterations, threshold max(tol ||b||, atol)."""
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from oracle import krylov
+
+
+def _laplacian(n, seed=0, nonsym=0.0):
+    rng = np.random.default_rng(seed)
+    g = sp.random(n, n, density=4.0 / n, random_state=seed).tocsr()
+    g.data[:] = 0.1 + rng.random(g.nnz)
+    g = (g + g.T).tocsr()
+    d = np.asarray(g.sum(axis=1)).ravel() + 0.5
+    A = (sp.diags(d) - g).tocsr()
+    if nonsym:
+        Nn = sp.random(n, n, density=2.0 / n, random_state=seed + 1) * nonsym
+        A = (A - Nn + sp.diags(np.asarray(Nn.sum(axis=1)).ravel())).tocsr()
+    return A
+
+
+def test_cg_matches_direct_solve_and_negative_definite():
+    A = _laplacian(300)
+    b = np.random.default_rng(1).standard_normal(300)
+    for sign in (1.0, -1.0):           # the reference's pressure matrix is negative definite
+        As = (sign * A).astype(np.float64)
+        x = np.zeros(300)
+        it, res = krylov.cg(As, sign * b, x, 1.0 / As.diagonal(), atol=1e-12, maxiter=2000)
+        xs = spla.spsolve(As.tocsc(), sign * b)
+        assert it % 10 == 0 and it > 0
+        assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-9
+        assert np.linalg.norm(sign * b - As @ x) <= 1e-12 * 1.0001 * max(1.0, np.linalg.norm(b))
+
+
+def test_bicgstab_matches_direct_solve():
+    A = _laplacian(300, seed=3, nonsym=0.5)
+    b = np.random.default_rng(2).standard_normal(900)
+    K = sp.kron(A, sp.eye(3)).tocsr()        # interleaved 3-component system
+    x = np.zeros(900)
+    it, res = krylov.bicgstab(K, b, x, 1.0 / K.diagonal(), atol=1e-11, maxiter=2000)
+    xs = spla.spsolve(K.tocsc(), b)
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-8
+    assert it % 10 == 0
+
+
+def test_stopping_rule():
+    A = _laplacian(200, seed=5)
+    b = np.random.default_rng(3).standard_normal(200)
+    x = spla.spsolve(A.tocsc(), b)
+    # converged start: zero iterations, x untouched
+    x0 = x.copy()
+    it, res = krylov.cg(A, b, x0, 1.0 / A.diagonal(), atol=1e-6)
+    assert it == 0 and np.array_equal(x0, x)
+    # threshold is max(tol ||b||, atol) with tol = atol by default
+    assert krylov.threshold(b, 1e-5) == max(1e-5 * np.linalg.norm(b), 1e-5)
+    assert krylov.threshold(b * 1e-9, 1e-5) == 1e-5
+    # maxiter caps the count (multiple of check_every)
+    x1 = np.zeros(200)
+    it, res = krylov.cg(A, b, x1, 1.0 / A.diagonal(), atol=1e-30, maxiter=30)
+    assert it == 30

@@ -1,0 +1,260 @@
+"""FVM: the finite-volume model, device-resident.
+
+Mirrors warp_cfd/FV/model.py:51-384.  The state lives on the GPU in the
+device layout (structure-of-arrays, RCM-renumbered cells, face-ordered);
+every operator is a HIP kernel of libwcfd.so reached through the C ABI.
+Properties such as ``cell_values`` return tensors in the *reference* order
+(cell i = the i-th cell of the input mesh), as the reference's arrays are.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from .. import _lib
+from ..material import Material
+from ..preprocess.layout import DeviceLayout
+from .boundary import Boundary
+from .device import DeviceMesh, ptr, require_gpu, stream_ptr
+from .field import Field
+
+
+class FVM:
+    def __init__(self, mesh, output_variables=None, float_dtype=np.float32, int_dtype=np.int32,
+                 device='cuda', renumber=True):
+        if np.dtype(float_dtype) != np.float32 and float_dtype is not torch.float32:
+            raise NotImplementedError('the device path computes in float32 (float64 reductions)')
+        self.material = Material()
+        self.dimension = mesh.dimension
+        self.float_dtype = np.float32
+        self.int_dtype = np.int32
+        self.mesh = mesh
+        self.skew_correction = True
+        self.face_properties = mesh.face_properties
+        self.cell_properties = mesh.cell_properties
+        self.input_variables = ['x', 'y', 'z', 't']
+        self.output_variables = output_variables or ['u', 'v', 'w', 'p']
+        if list(self.output_variables) != ['u', 'v', 'w', 'p']:
+            raise NotImplementedError("output_variables must be ['u','v','w','p']")
+        self.fields = {o: Field(o, i) for i, o in enumerate(self.output_variables)}
+        self.boundary = Boundary(self.fields, mesh.groups,
+                                 boundary_ids=self.face_properties.boundary_face_ids)
+        self.vars_mapping = {v: i for i, v in enumerate(self.output_variables)}
+        self.num_outputs = 4
+        self.num_nodes = mesh.nodes.shape[0]
+        self.num_cells = mesh.cells.shape[0]
+        self.num_faces = self.face_properties.num_faces
+        self.nodes_per_cell = 8
+        self.faces_per_cell = 6
+        self.nodes_per_face = 4
+        self.reference_pressure_cell_id = None
+        self.reference_pressure = 0.0
+        self.finalized = False
+        self._device = device
+        self._renumber = renumber
+        # u,v,w gradients are known to be zero (after calculate_gradients('p'))
+        self._uvw_grad_zero = True
+
+    # ------------------------------------------------------------ set-up
+    def finalize(self):
+        """model.py:121-139: upload the mesh, check and upload the BCs,
+        allocate the field arrays."""
+        if self.finalized:
+            return
+        dev = require_gpu(self._device)
+        self.boundary.finalize()
+        self.layout = DeviceLayout(self.mesh, renumber=self._renumber)
+        self.dm = DeviceMesh(self.layout, dev)
+        self.bc_type, self.bc_value = self.boundary.device_arrays(self.layout, dev)
+        self.bc = _lib.WcfdBC(self.bc_type.data_ptr(), self.bc_value.data_ptr())
+        C, F = self.num_cells, self.num_faces
+        z = lambda *s: torch.zeros(*s, dtype=torch.float32, device=dev)
+        self._cv = z(4, C)
+        self._cg = z(12, C)
+        self._fv = z(4, F)
+        self._flux = z(F)
+        self.lib = _lib.load()
+        self._reduce_ws = torch.zeros(self.lib.wcfd_reduce_workspace_bytes(F), dtype=torch.uint8,
+                                      device=dev)
+        self.finalized = True
+
+    def _need(self):
+        if not self.finalized:
+            raise ValueError('Please call method model.finalize() first')
+
+    def set_initial_conditions(self, IC, output_indices=None):
+        """model.py:143-147 (IC in reference cell order)."""
+        self._need()
+        ic = torch.as_tensor(np.asarray(IC) if not torch.is_tensor(IC) else IC,
+                             dtype=torch.float32, device=self._cv.device)
+        if output_indices is None:
+            output_indices = list(range(ic.shape[-1]))
+        for o in self._indices(output_indices):
+            self._cv[o] = self.dm.cells_to_device(ic[:, o].contiguous())
+        self._uvw_grad_zero = bool(torch.all(self._cg[:9] == 0))
+
+    def set_reference_pressure(self, cell_id, value=0.):
+        self.reference_pressure_cell_id = cell_id
+        self.reference_pressure = float(value)
+
+    @property
+    def reference_cell_device(self):
+        if self.reference_pressure_cell_id is None:
+            return -1
+        return int(self.layout.old2new[self.reference_pressure_cell_id])
+
+    def _indices(self, output_indices):
+        if output_indices is None:
+            return [0, 1, 2, 3]
+        if isinstance(output_indices, (int, np.integer)):
+            return [int(output_indices)]
+        if isinstance(output_indices, str):
+            assert output_indices in self.output_variables
+            return [self.fields[output_indices].index]
+        out = []
+        for o in output_indices:
+            out.append(self.fields[o].index if isinstance(o, str) else int(o))
+        return out
+
+    def get_output_indices(self, output_indices):
+        return np.asarray(self._indices(output_indices), np.int32)
+
+    @staticmethod
+    def _mask(idx):
+        m = 0
+        for o in idx:
+            m |= 1 << o
+        return m
+
+    def _s(self):
+        return stream_ptr()
+
+    # ------------------------------------------------ reference-order views
+    @property
+    def cell_values(self):
+        """(C,4) float32 tensor in reference cell order (a copy)."""
+        self._need()
+        return self.dm.cells_to_reference(self._cv).T.contiguous()
+
+    @property
+    def cell_gradients(self):
+        """(C,4,3) in reference cell order (a copy)."""
+        self._need()
+        return self.dm.cells_to_reference(self._cg).T.reshape(self.num_cells, 4, 3).contiguous()
+
+    @property
+    def face_values(self):
+        self._need()
+        return self.dm.faces_to_reference(self._fv).T.contiguous()
+
+    @property
+    def mass_fluxes(self):
+        self._need()
+        return self.dm.faces_to_reference(self._flux).contiguous()
+
+    @property
+    def face_gradients(self):
+        """(F,4): the Neumann boundary gradients (the only entries the
+        reference ever writes, conditions.py:17-18)."""
+        fg = np.zeros((self.num_faces, 4), np.float32)
+        b = self.boundary
+        n = b.boundary_type == 2
+        rows, cols = np.nonzero(n)
+        fg[b.boundary_ids[rows], cols] = b.boundary_values[rows, cols]
+        return torch.from_numpy(fg)
+
+    # ----------------------------------------------------------- operators
+    def set_boundary_conditions(self):
+        """model.py:206-208."""
+        self._need()
+        _lib.check(self.lib.wcfd_set_boundary_conditions(self.dm.desc_ref, self.bc, ptr(self._fv),
+                                                         15, self._s()), 'set_boundary_conditions')
+
+    def face_interpolation(self, output_indices=None, upwind=False):
+        """model.py:210-229 (linear + skew correction; ``upwind`` is not used
+        by the reference solver and is not provided)."""
+        self._need()
+        if upwind:
+            raise NotImplementedError('upwind face interpolation is not on the solver path')
+        idx = self._indices(output_indices)
+        mask = self._mask(idx)
+        skew = mask if self.skew_correction else 0
+        if self._uvw_grad_zero:
+            skew &= 8
+        _lib.check(self.lib.wcfd_face_interpolation(self.dm.desc_ref, self.bc, ptr(self._cv),
+                                                    ptr(self._cg), ptr(self._fv), mask, skew, None,
+                                                    None, None, self._s()), 'face_interpolation')
+
+    def calculate_gradients(self, output_indices=None):
+        """model.py:232-239 (zeroes every gradient first, as the reference)."""
+        self._need()
+        idx = self._indices(output_indices)
+        mask = self._mask(idx)
+        _lib.check(self.lib.wcfd_green_gauss(self.dm.desc_ref, ptr(self._fv), ptr(self._cg), mask, 1,
+                                             self._s()), 'calculate_gradients')
+        self._uvw_grad_zero = (mask & 7) == 0
+
+    def calculate_mass_flux(self):
+        self._need()
+        _lib.check(self.lib.wcfd_mass_flux(self.dm.desc_ref, ptr(self._fv), ptr(self._flux),
+                                           self._s()), 'calculate_mass_flux')
+
+    def divFlux(self, arr=None):
+        """model.py:269-275; returns a (C,) tensor in reference order."""
+        self._need()
+        d = torch.empty(self.num_cells, dtype=torch.float32, device=self._cv.device)
+        _lib.check(self.lib.wcfd_divergence(self.dm.desc_ref, ptr(self._flux), ptr(d), self._s()),
+                   'divFlux')
+        out = self.dm.cells_to_reference(d)
+        if arr is not None:
+            arr.copy_(out)
+            return arr
+        return out
+
+    def get_gradient(self, output_index, coeff=1.):
+        """model.py:242-254: coeff * grad(phi), (C,3) in reference order."""
+        self._need()
+        o = self._indices(output_index)[0]
+        g = self.cell_gradients[:, o]
+        if isinstance(coeff, (float, int)):
+            return (torch.tensor(coeff, dtype=torch.float32, device=g.device) * g)
+        c = torch.as_tensor(coeff, dtype=torch.float32, device=g.device).reshape(-1, 1)
+        return c * g
+
+    def relax(self, new_value, alpha, output_index):
+        """model.py:256-262 (new_value in reference order)."""
+        self._need()
+        o = self._indices(output_index)[0]
+        nv = self.dm.cells_to_device(torch.as_tensor(new_value, dtype=torch.float32,
+                                                     device=self._cv.device))
+        a = torch.tensor(alpha, dtype=torch.float32)
+        self._cv[o] = a.item() * nv + (1.0 - a).item() * self._cv[o]
+
+    def replace_cell_values(self, output_indices, value):
+        """model.py:318-325: value is interleaved (C*n,) in reference order."""
+        self._need()
+        idx = self._indices(output_indices)
+        v = torch.as_tensor(value, dtype=torch.float32, device=self._cv.device).reshape(-1, len(idx))
+        for k, o in enumerate(idx):
+            self._cv[o] = self.dm.cells_to_device(v[:, k].contiguous())
+
+    def update_cell_values(self, field_idx, value, scale=1.):
+        """model.py:329-349."""
+        self._need()
+        idx = self._indices(field_idx)
+        v = torch.as_tensor(value, dtype=torch.float32, device=self._cv.device)
+        for k, o in enumerate(idx):
+            if v.numel() == 1:
+                self._cv[o] += scale * v.reshape(())
+            else:
+                vv = v.reshape(self.num_cells, -1)[:, k] if v.ndim > 1 else v
+                self._cv[o] += scale * self.dm.cells_to_device(vv.contiguous())
+
+    def struct_member_to_array(self, member='centroid', struct='cells'):
+        """model.py:177-190 for the members the examples read."""
+        cp, fp = self.cell_properties, self.face_properties
+        table = {('centroid', 'cells'): cp.centroids, ('volume', 'cells'): cp.volumes,
+                 ('centroid', 'faces'): fp.centroid}
+        if (member, struct) not in table:
+            raise ValueError(f'{member} of {struct} is not exported')
+        return np.asarray(table[(member, struct)])

@@ -1,0 +1,160 @@
+"""ctypes binding of libwcfd.so (C ABI declared in include/wcfd.h).
+
+The product path has no CPU fallback: if the library is missing or no GPU is
+present, the first call that needs it raises.
+"""
+import ctypes
+import os
+import re
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_NAME = 'libwcfd.so'
+LIB_PATH = os.path.join(_HERE, LIB_NAME)
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), 'include', 'wcfd.h')
+
+WCFD_OK, WCFD_EINVAL, WCFD_EHIP, WCFD_ENOTCONV, WCFD_ENAN = 0, 1, 2, 3, 4
+DIRICHLET, NEUMANN = 1, 2
+SCHEME_CORRECTED, SCHEME_IMPLICIT_OUTFLOW = 1, 2
+
+c_p = ctypes.c_void_p
+c_i32 = ctypes.c_int32
+c_f = ctypes.c_float
+c_d = ctypes.c_double
+
+
+class WcfdMesh(ctypes.Structure):
+    _fields_ = [('n_cells', c_i32), ('n_faces', c_i32), ('n_internal', c_i32),
+                ('n_boundary', c_i32), ('nnz', c_i32), ('reserved', c_i32),
+                ('owner', c_p), ('neigh', c_p), ('psi', c_p), ('skew', c_p), ('area', c_p),
+                ('normal', c_p), ('dist', c_p), ('dmag', c_p), ('volume', c_p),
+                ('slot_face', c_p), ('slot_side', c_p), ('slot_sn', c_p), ('slot_pos', c_p),
+                ('diag_pos', c_p), ('row_ptr', c_p), ('col', c_p)]
+
+
+class WcfdBC(ctypes.Structure):
+    _fields_ = [('type', c_p), ('value', c_p)]
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [('iterations', c_i32), ('converged', c_i32), ('residual', c_d),
+                ('threshold', c_d)]
+
+    def as_tuple(self):
+        return (self.iterations, self.residual, self.threshold)
+
+
+class SimpleParams(ctypes.Structure):
+    _fields_ = [('nu', c_f), ('density', c_f), ('alpha_u', c_f), ('alpha_p', c_f), ('dt', c_f),
+                ('scheme_flags', c_i32), ('ref_cell', c_i32), ('ref_value', c_f),
+                ('n_inner', c_i32), ('n_ortho', c_i32), ('atol', c_d), ('tol', c_d),
+                ('maxiter', c_i32), ('check_every', c_i32), ('first_iteration', c_i32),
+                ('reserved', c_i32)]
+
+
+class SimpleState(ctypes.Structure):
+    _fields_ = [(n, c_p) for n in ('cell_values', 'cell_grad', 'face_values', 'mass_flux', 'hbya',
+                                   'p_x', 'rua', 'rua_face', 'mom_values', 'mom_rhs', 'p_values',
+                                   'p_rhs', 'resid')]
+
+
+class SimpleInfo(ctypes.Structure):
+    _fields_ = [('momentum', SolveInfo), ('pressure', SolveInfo), ('continuity', c_d)]
+
+
+_PM = ctypes.POINTER(WcfdMesh)
+_SIGS = {
+    'wcfd_last_hip_error': (ctypes.c_int, []),
+    'wcfd_error_string': (ctypes.c_char_p, [ctypes.c_int]),
+    'wcfd_abi_version': (ctypes.c_int, []),
+    'wcfd_set_boundary_conditions': (ctypes.c_int, [_PM, WcfdBC, c_p, ctypes.c_int, c_p]),
+    'wcfd_face_interpolation': (ctypes.c_int, [_PM, WcfdBC, c_p, c_p, c_p, ctypes.c_int,
+                                               ctypes.c_int, c_p, c_p, c_p, c_p]),
+    'wcfd_green_gauss': (ctypes.c_int, [_PM, c_p, c_p, ctypes.c_int, ctypes.c_int, c_p]),
+    'wcfd_mass_flux': (ctypes.c_int, [_PM, c_p, c_p, c_p]),
+    'wcfd_divergence': (ctypes.c_int, [_PM, c_p, c_p, c_p]),
+    'wcfd_assemble_momentum': (ctypes.c_int, [_PM, WcfdBC, c_p, c_p, c_p, c_p, ctypes.c_int, c_f,
+                                              c_f, c_f, c_f, ctypes.c_int, c_p, c_p, c_p]),
+    'wcfd_assemble_pressure': (ctypes.c_int, [_PM, WcfdBC, c_p, c_p, c_p, c_p, ctypes.c_int,
+                                              ctypes.c_int, ctypes.c_int, c_f, c_p, c_p, c_p, c_p]),
+    'wcfd_rua_hbya': (ctypes.c_int, [_PM, c_p, c_f, c_p, c_p, c_p, c_p, c_p]),
+    'wcfd_relax_pressure': (ctypes.c_int, [_PM, c_p, c_f, c_p, c_p]),
+    'wcfd_correct_velocity': (ctypes.c_int, [_PM, WcfdBC, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p,
+                                             c_p, c_p]),
+    'wcfd_reduce_workspace_bytes': (ctypes.c_size_t, [c_i32]),
+    'wcfd_spmv': (ctypes.c_int, [c_i32, c_p, c_p, c_p, ctypes.c_int, c_p, c_p, c_p]),
+    'wcfd_dot_f64': (ctypes.c_int, [c_i32, c_p, c_p, c_p, c_p, c_p]),
+    'wcfd_krylov_workspace_bytes': (ctypes.c_size_t, [c_i32, ctypes.c_int]),
+    'wcfd_cg': (ctypes.c_int, [c_i32, c_p, c_p, c_p, c_p, c_p, c_p, c_d, c_d, ctypes.c_int,
+                               ctypes.c_int, c_p, ctypes.POINTER(SolveInfo), c_p]),
+    'wcfd_bicgstab': (ctypes.c_int, [c_i32, c_p, c_p, c_p, c_p, ctypes.c_int, c_p, c_p, c_d, c_d,
+                                     ctypes.c_int, ctypes.c_int, c_p, ctypes.POINTER(SolveInfo),
+                                     c_p]),
+    'wcfd_simple_workspace_bytes': (ctypes.c_size_t, [c_i32, c_i32]),
+    'wcfd_kernel_count': (ctypes.c_int, []),
+    'wcfd_kernel_name': (ctypes.c_char_p, [ctypes.c_int]),
+    'wcfd_profile_begin': (ctypes.c_int, [ctypes.c_int]),
+    'wcfd_profile_end': (ctypes.c_int, [ctypes.POINTER(c_d), ctypes.POINTER(c_i32), ctypes.c_int]),
+    'wcfd_simple_iteration': (ctypes.c_int, [_PM, WcfdBC, ctypes.POINTER(SimpleParams),
+                                             ctypes.POINTER(SimpleState), c_p,
+                                             ctypes.POINTER(SimpleInfo), c_p]),
+}
+
+_lib = None
+
+
+class WcfdError(RuntimeError):
+    pass
+
+
+def header_symbols(path=HEADER_PATH):
+    """Function names declared in include/wcfd.h."""
+    text = open(path).read()
+    return sorted(set(re.findall(r'^[A-Za-z_][\w\s\*]*?\b(wcfd_\w+)\s*\(', text, re.M)))
+
+
+def load(path=LIB_PATH):
+    """Load libwcfd.so and declare every signature; raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise WcfdError(f'{LIB_NAME} is not built ({path}); run __graft_entry__.build() first')
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(code, what=''):
+    if code != WCFD_OK:
+        lib = load()
+        msg = lib.wcfd_error_string(code).decode()
+        if code == WCFD_EHIP:
+            msg += f' (hipError {lib.wcfd_last_hip_error()})'
+        if code == WCFD_ENAN:
+            raise ValueError(f'{what}: {msg}')
+        raise WcfdError(f'{what}: {msg}')
+
+
+class Profiler:
+    """Per-kernel HIP-event timings of every libwcfd launch in a region."""
+
+    def __init__(self, capacity=65536):
+        self.capacity = capacity
+        self.result = None
+
+    def __enter__(self):
+        check(load().wcfd_profile_begin(self.capacity), 'profile_begin')
+        return self
+
+    def __exit__(self, *exc):
+        lib = load()
+        n = lib.wcfd_kernel_count()
+        ms = (c_d * n)()
+        cnt = (c_i32 * n)()
+        check(lib.wcfd_profile_end(ms, cnt, n), 'profile_end')
+        self.result = {lib.wcfd_kernel_name(i).decode(): (ms[i], cnt[i]) for i in range(n) if cnt[i]}
+        return False

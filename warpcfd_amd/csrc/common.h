@@ -1,0 +1,157 @@
+// Shared helpers for the wcfd HIP kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/wcfd.h"
+
+namespace wcfd {
+
+constexpr int kBlock = 256;           // threads per workgroup for streaming kernels
+constexpr int kWave = 64;             // CDNA wavefront
+constexpr int kMaxReduceBlocks = 2048;  // partials the last workgroup folds
+
+// per-thread HIP error of the last failing call
+extern thread_local int g_last_hip_error;
+
+// --------------------------------------------------------- opt-in profiler
+// Every launch site is bracketed by a ProfScope.  While wcfd_profile_begin()
+// is active each launch records a start and a stop hipEvent on its own
+// stream; wcfd_profile_end() folds them into per-kernel totals.
+enum KernelId {
+  K_SET_BC, K_FACE_INTERP, K_GREEN_GAUSS, K_MASS_FLUX, K_DIVERGENCE, K_ASM_MOM, K_ASM_P,
+  K_RUA_HBYA, K_RELAX_P, K_FACE_P_FC, K_GRAD_P_CORR, K_INV_DIAG, K_SPMV, K_DOT, K_CG_INIT,
+  K_CG_1, K_CG_2, K_BI_INIT, K_BI_1, K_BI_2, K_BI_3, K_COUNT
+};
+bool prof_active();
+void prof_record(int id, hipStream_t s, bool start);
+struct ProfScope {
+  int id;
+  hipStream_t s;
+  bool on;
+  ProfScope(int id_, hipStream_t s_) : id(id_), s(s_), on(prof_active()) {
+    if (on) prof_record(id, s, true);
+  }
+  ~ProfScope() {
+    if (on) prof_record(id, s, false);
+  }
+};
+
+// launch KERNEL<<<GRID, kBlock, 0, S>>>(args...) inside a profiler scope
+#define WCFD_LAUNCH(ID, S, KERNEL, GRID, ...)          \
+  do {                                                \
+    ::wcfd::ProfScope wcfd_ps_((ID), (S));            \
+    KERNEL<<<(GRID), kBlock, 0, (S)>>>(__VA_ARGS__);  \
+  } while (0)
+
+inline int check_hip(hipError_t e) {
+  if (e != hipSuccess) {
+    g_last_hip_error = static_cast<int>(e);
+    return WCFD_EHIP;
+  }
+  return WCFD_OK;
+}
+
+inline int check_launch() { return check_hip(hipGetLastError()); }
+
+inline unsigned grid_for(int64_t n, int block = kBlock) {
+  int64_t g = (n + block - 1) / block;
+  return static_cast<unsigned>(g < 1 ? 1 : g);
+}
+
+inline unsigned reduce_grid(int64_t n, int block = kBlock) {
+  int64_t g = (n + block - 1) / block;
+  if (g > kMaxReduceBlocks) g = kMaxReduceBlocks;
+  return static_cast<unsigned>(g < 1 ? 1 : g);
+}
+
+inline size_t align256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+
+// Carves a caller-provided workspace into aligned pieces.
+struct Carver {
+  char* base;
+  size_t off = 0;
+  explicit Carver(void* p) : base(static_cast<char*>(p)) {}
+  template <class T>
+  T* take(size_t count) {
+    T* p = reinterpret_cast<T*>(base ? base + off : nullptr);
+    off += align256(count * sizeof(T));
+    return p;
+  }
+};
+
+// ------------------------------------------------------------ reductions
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// Block sum of ND values per thread; result valid in thread 0.  Order is fixed
+// by (blockDim, thread index), so the result is deterministic.
+template <int ND>
+__device__ inline void block_sum(double (&v)[ND], double* lds /* [ND][kBlock/kWave] */) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = threadIdx.x / kWave;
+  const int nw = blockDim.x / kWave;
+#pragma unroll
+  for (int i = 0; i < ND; ++i) {
+    double s = wave_sum(v[i]);
+    if (lane == 0) lds[i * (kBlock / kWave) + wid] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+      double s = 0.0;
+      for (int w = 0; w < nw; ++w) s += lds[i * (kBlock / kWave) + w];
+      v[i] = s;
+    }
+  }
+  __syncthreads();
+}
+
+// Grid-wide deterministic sum: every workgroup stores its partials, the last
+// workgroup to arrive (atomic ticket) folds them in block order and returns
+// true with the totals in `out` (valid in all its threads).  The ticket is
+// reset by that workgroup so the next launch starts from zero.
+template <int ND>
+__device__ inline bool grid_sum(double (&v)[ND], double* partials /* [ND][gridDim] */,
+                                unsigned* ticket, double (&out)[ND]) {
+  __shared__ double lds[ND * (kBlock / kWave)];
+  __shared__ bool last;
+  block_sum<ND>(v, lds);
+  const unsigned nb = gridDim.x;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < ND; ++i) partials[i * nb + blockIdx.x] = v[i];
+    __threadfence();
+    unsigned t = atomicAdd(ticket, 1u);
+    last = (t == nb - 1);
+  }
+  __syncthreads();
+  if (!last) return false;
+  __threadfence();
+  double acc[ND];
+#pragma unroll
+  for (int i = 0; i < ND; ++i) {
+    double s = 0.0;
+    for (unsigned b = threadIdx.x; b < nb; b += blockDim.x)
+      s += partials[i * nb + b];
+    acc[i] = s;
+  }
+  block_sum<ND>(acc, lds);
+  __shared__ double bc[ND];
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < ND; ++i) bc[i] = acc[i];
+    *ticket = 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < ND; ++i) out[i] = bc[i];
+  return true;
+}
+
+}  // namespace wcfd

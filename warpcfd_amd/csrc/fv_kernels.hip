@@ -1,0 +1,651 @@
+// Face and cell kernels of the finite-volume operators (gfx950).
+//
+// Every face->cell reduction is a cell-centric gather over the cell's six
+// "slots" (its local faces in the reference's order), so no atomics are used
+// and results are bitwise reproducible run to run.  The reference
+// accumulates with wp.atomic_add in launch order (model_kernels.py:32-34,90;
+// equation_kernels.py:75,79,101); the sums here run over the same terms in
+// local-face order.
+#include "fv_kernels.h"
+
+namespace wcfd {
+
+thread_local int g_last_hip_error = 0;
+
+// ------------------------------------------------------------------ BCs
+__global__ void k_set_bc(int F, int Fi, int Fb, const uint8_t* __restrict__ type,
+                         const float* __restrict__ value, float* __restrict__ fv, int mask) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= Fb) return;
+  int f = Fi + j;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    if ((mask >> q) & 1)
+      if (type[q * Fb + j] == WCFD_DIRICHLET) fv[q * F + f] = value[q * Fb + j];
+}
+
+// --------------------------------------------------- face interpolation
+// interpolation_kernels.py:5-54, interpolation_functions.py:6-14,
+// skewness_functions.py:7-20, model_kernels.py:46-60, solver_kernels.py:22-36.
+__global__ void k_face_interp(wcfd_mesh m, wcfd_bc bc, const float* __restrict__ cv,
+                              const float* __restrict__ cg, float* __restrict__ fv,
+                              int mask, int skew_mask, float* __restrict__ flux,
+                              const float* __restrict__ rua, float* __restrict__ rua_f,
+                              int apply_dirichlet) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  const int C = m.n_cells, F = m.n_faces, Fi = m.n_internal, Fb = m.n_boundary;
+  if (f >= F) return;
+  float val[4];
+  if (f < Fi) {
+    const int o = m.owner[f], n = m.neigh[f];
+    const float psi = m.psi[f];
+    const float psi2 = 1.0f - psi;
+    const float s0 = m.skew[f], s1 = m.skew[Fi + f], s2 = m.skew[2 * Fi + f];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if ((mask >> q) & 1) {
+        float v = cv[q * C + o] * psi + psi2 * cv[q * C + n];
+        if ((skew_mask >> q) & 1) {
+          const float* g = cg + (size_t)(3 * q) * C;
+          float g0 = g[o] * psi + psi2 * g[n];
+          float g1 = g[C + o] * psi + psi2 * g[C + n];
+          float g2 = g[2 * C + o] * psi + psi2 * g[2 * C + n];
+          v = v + (g0 * s0 + g1 * s1 + g2 * s2);
+        }
+        fv[q * F + f] = v;
+        val[q] = v;
+      } else {
+        val[q] = (q < 3 && flux) ? fv[q * F + f] : 0.0f;
+      }
+    }
+    if (rua_f) rua_f[f] = psi * rua[o] + psi2 * rua[n];
+  } else {
+    const int j = f - Fi;
+    const int P = m.owner[f];
+    const float d = m.dmag[f];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const bool want = ((mask >> q) & 1) || (q < 3 && flux);
+      if (!want) continue;
+      float v;
+      const uint8_t t = bc.type[q * Fb + j];
+      if (((mask >> q) & 1) && t == WCFD_NEUMANN) {
+        v = d * bc.value[q * Fb + j] + cv[q * C + P];
+        fv[q * F + f] = v;
+      } else if (((mask >> q) & 1) && apply_dirichlet && t == WCFD_DIRICHLET) {
+        v = bc.value[q * Fb + j];
+        fv[q * F + f] = v;
+      } else {
+        v = fv[q * F + f];
+      }
+      val[q] = v;
+    }
+    if (rua_f) rua_f[f] = rua[P];
+  }
+  if (flux) {
+    const float n0 = m.normal[f], n1 = m.normal[F + f], n2 = m.normal[2 * F + f];
+    flux[f] = (val[0] * n0 + val[1] * n1 + val[2] * n2) * m.area[f];
+  }
+}
+
+// ------------------------------------------------------------ Green-Gauss
+// model_kernels.py:72-90: grad_c = sum_k phi_f A_f n_{c,k} / V_c.
+__device__ inline void gg_field(const wcfd_mesh& m, int c, const int (&fk)[6],
+                                const float* __restrict__ phi_f, float V, float (&g)[3]) {
+  const int C = m.n_cells;
+  float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const float phi = phi_f[fk[k]];
+    a0 = a0 + (phi * m.slot_sn[(0 * 6 + k) * C + c]) / V;
+    a1 = a1 + (phi * m.slot_sn[(1 * 6 + k) * C + c]) / V;
+    a2 = a2 + (phi * m.slot_sn[(2 * 6 + k) * C + c]) / V;
+  }
+  g[0] = a0; g[1] = a1; g[2] = a2;
+}
+
+__global__ void k_green_gauss(wcfd_mesh m, const float* __restrict__ fv, float* __restrict__ cg,
+                              int mask, int zero_others) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int C = m.n_cells, F = m.n_faces;
+  if (c >= C) return;
+  int fk[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) fk[k] = m.slot_face[k * C + c];
+  const float V = m.volume[c];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if ((mask >> q) & 1) {
+      float g[3];
+      gg_field(m, c, fk, fv + (size_t)q * F, V, g);
+      cg[(3 * q + 0) * C + c] = g[0];
+      cg[(3 * q + 1) * C + c] = g[1];
+      cg[(3 * q + 2) * C + c] = g[2];
+    } else if (zero_others) {
+      cg[(3 * q + 0) * C + c] = 0.f;
+      cg[(3 * q + 1) * C + c] = 0.f;
+      cg[(3 * q + 2) * C + c] = 0.f;
+    }
+  }
+}
+
+__global__ void k_mass_flux(wcfd_mesh m, const float* __restrict__ fv, float* __restrict__ flux) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  const int F = m.n_faces;
+  if (f >= F) return;
+  flux[f] = (fv[f] * m.normal[f] + fv[F + f] * m.normal[F + f] + fv[2 * F + f] * m.normal[2 * F + f]) *
+            m.area[f];
+}
+
+__device__ inline float cell_div(const wcfd_mesh& m, int c, const int (&fk)[6],
+                                 const float* __restrict__ flux) {
+  const int C = m.n_cells;
+  float acc = 0.f;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const float fl = flux[fk[k]];
+    acc = acc + (m.slot_side[k * C + c] == 0 ? fl : -fl);
+  }
+  return acc;
+}
+
+__global__ void k_divergence(wcfd_mesh m, const float* __restrict__ flux, float* __restrict__ div) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int C = m.n_cells;
+  if (c >= C) return;
+  int fk[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) fk[k] = m.slot_face[k * C + c];
+  div[c] = cell_div(m, c, fk, flux);
+}
+
+// Laplacian face coefficient factors (laplacianScheme.py:14-66): returns
+// (1/|d|) or, with the over-relaxed correction, |n1|/|d| where
+// n1 = d / (n_f . d).
+__device__ inline float lap_factor(const wcfd_mesh& m, int f, int correction) {
+  const float dm = m.dmag[f];
+  if (!correction) return 1.0f / dm;
+  const int Fi = m.n_internal, F = m.n_faces;
+  const float d0 = m.dist[f], d1 = m.dist[Fi + f], d2 = m.dist[2 * Fi + f];
+  const float n0 = m.normal[f], n1 = m.normal[F + f], n2 = m.normal[2 * F + f];
+  const float inv_nd = 1.0f / (n0 * d0 + n1 * d1 + n2 * d2);
+  const float e0 = d0 * inv_nd, e1 = d1 * inv_nd, e2 = d2 * inv_nd;
+  const float n1m = sqrtf(e0 * e0 + e1 * e1 + e2 * e2);
+  return 1.0f / dm * n1m;
+}
+
+// ------------------------------------------------------ momentum assembly
+__global__ void k_assemble_momentum(wcfd_mesh m, wcfd_bc bc, const float* __restrict__ cv,
+                                    const float* __restrict__ fv, const float* __restrict__ flux,
+                                    float* __restrict__ cg, int compute_grad, float nu,
+                                    float inv_density, float alpha, float dt, int flags,
+                                    float* __restrict__ values, float* __restrict__ rhs) {
+  const int correction = flags & WCFD_SCHEME_CORRECTED;
+  const bool implicit_outflow = (flags & WCFD_SCHEME_IMPLICIT_OUTFLOW) != 0;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int C = m.n_cells, F = m.n_faces, Fi = m.n_internal, Fb = m.n_boundary;
+  if (c >= C) return;
+  int fk[6];
+  int side[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    fk[k] = m.slot_face[k * C + c];
+    side[k] = m.slot_side[k * C + c];
+  }
+  const float V = m.volume[c];
+  float gp[3];
+  if (compute_grad) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      float g[3];
+      gg_field(m, c, fk, fv + (size_t)q * F, V, g);
+      cg[(3 * q + 0) * C + c] = g[0];
+      cg[(3 * q + 1) * C + c] = g[1];
+      cg[(3 * q + 2) * C + c] = g[2];
+      if (q == 3) { gp[0] = g[0]; gp[1] = g[1]; gp[2] = g[2]; }
+    }
+  } else {
+    gp[0] = cg[9 * C + c]; gp[1] = cg[10 * C + c]; gp[2] = cg[11 * C + c];
+  }
+  float diag = 0.f;
+  float b[3] = {0.f, 0.f, 0.f};
+  float off[6];
+  // DdtTerm first (incompressible.py:110): diag += V/dt, b += phi V/dt
+  if (dt > 0.f) {
+    const float vdt = V / dt;
+    diag = diag + vdt;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) b[q] = b[q] + (-(-cv[q * C + c] * vdt));
+  }
+  // ConvectionTerm (scale +1), upwind, tie -> neighbour (T6).  Boundary
+  // faces go to b with the lagged face value (convection.py:63-67); with
+  // WCFD_SCHEME_IMPLICIT_OUTFLOW a Neumann outflow face puts F_b on the
+  // diagonal and keeps only F_b |x_f - x_P| g explicit.
+  unsigned outflow = 0u;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int f = fk[k];
+    const float Fl = flux[f];
+    off[k] = 0.f;
+    if (f < Fi) {
+      const bool up = Fl > 0.f;
+      if (side[k] == 0) {
+        diag = diag + (up ? Fl : 0.f);
+        off[k] = up ? 0.f : Fl;
+      } else {
+        diag = diag + (up ? 0.f : -Fl);
+        off[k] = up ? -Fl : 0.f;
+      }
+    } else if (implicit_outflow && Fl > 0.f && bc.type[f - Fi] == WCFD_NEUMANN) {
+      diag = diag + Fl;
+      outflow |= 1u << k;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int f = fk[k];
+    if (f >= Fi) {
+      const float Fl = flux[f];
+      if ((outflow >> k) & 1u) {
+        const int j = f - Fi;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) b[q] = b[q] + (-(Fl * (m.dmag[f] * bc.value[q * Fb + j])));
+      } else {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) b[q] = b[q] + (-(Fl * fv[q * F + f]));
+      }
+    }
+  }
+  // DiffusionTerm (scale -1)
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int f = fk[k];
+    if (f < Fi) {
+      const float w = lap_factor(m, f, correction) * (nu * m.area[f]);
+      diag = diag + w;
+      off[k] = off[k] + (-w);
+    } else {
+      const int j = f - Fi;
+      if (bc.type[j] == WCFD_NEUMANN) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) b[q] = b[q] + (nu * bc.value[q * Fb + j]) * m.area[f];
+      } else {
+        const float w = (nu * m.area[f]) / m.dmag[f];
+        diag = diag + w;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) b[q] = b[q] + w * fv[q * F + f];
+      }
+    }
+  }
+  // explicit -(1/rho) grad(p) V (incompressible.py:107, grad.py:22-27)
+#pragma unroll
+  for (int q = 0; q < 3; ++q) b[q] = b[q] + (gp[q] * V) * (-inv_density);
+  // implicit under-relaxation (equation_kernels.py:128-152)
+  if (alpha > 0.f) {
+    const float r = (1.0f - alpha) / alpha;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) b[q] = b[q] + r * diag * cv[q * C + c];
+    diag = diag / alpha;
+  }
+  values[m.diag_pos[c]] = diag;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int p = m.slot_pos[k * C + c];
+    if (p >= 0) values[p] = off[k];
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) rhs[q * C + c] = b[q];
+}
+
+// ------------------------------------------------------ pressure assembly
+__global__ void k_assemble_pressure(wcfd_mesh m, wcfd_bc bc, const float* __restrict__ fv,
+                                    const float* __restrict__ flux,
+                                    const float* __restrict__ rua_f, float* __restrict__ cg,
+                                    int compute_grad_p, int correction, int ref_cell,
+                                    float ref_value, float* __restrict__ values,
+                                    float* __restrict__ rhs, float* __restrict__ div_out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int C = m.n_cells, F = m.n_faces, Fi = m.n_internal, Fb = m.n_boundary;
+  if (c >= C) return;
+  int fk[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) fk[k] = m.slot_face[k * C + c];
+  if (compute_grad_p) {
+    float g[3];
+    gg_field(m, c, fk, fv + (size_t)3 * F, m.volume[c], g);
+    cg[9 * C + c] = g[0];
+    cg[10 * C + c] = g[1];
+    cg[11 * C + c] = g[2];
+  }
+  const float div = cell_div(m, c, fk, flux);
+  if (div_out) div_out[c] = div;
+  float diag = 0.f, b = 0.f;
+  float off[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int f = fk[k];
+    off[k] = 0.f;
+    if (f < Fi) {
+      const float w = lap_factor(m, f, correction) * (rua_f[f] * m.area[f]);
+      diag = diag + (-w);
+      off[k] = w;
+    } else {
+      const int j = f - Fi;
+      if (bc.type[3 * Fb + j] == WCFD_NEUMANN) {
+        b = b + (-((rua_f[f] * bc.value[3 * Fb + j]) * m.area[f]));
+      } else {
+        const float w = (rua_f[f] * m.area[f]) / m.dmag[f];
+        diag = diag + (-w);
+        b = b + (-(w * fv[3 * F + f]));
+      }
+    }
+  }
+  b = b + div;
+  if (c == ref_cell) {
+    diag = 1.0f;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) off[k] = 0.f;
+    b = ref_value;
+  }
+  values[m.diag_pos[c]] = diag;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int p = m.slot_pos[k * C + c];
+    if (p >= 0) values[p] = off[k];
+  }
+  rhs[c] = b;
+}
+
+// ------------------------------------------------------------ rUA, HbyA
+__global__ void k_rua_hbya(wcfd_mesh m, const float* __restrict__ values, float density,
+                           const float* __restrict__ cg, float* __restrict__ hbya,
+                           float* __restrict__ cv, float* __restrict__ rua) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int C = m.n_cells;
+  if (c >= C) return;
+  const float r = m.volume[c] / (values[m.diag_pos[c]] * density);
+  rua[c] = r;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const float h = hbya[q * C + c] + cg[(9 + q) * C + c] * r;
+    hbya[q * C + c] = h;
+    cv[q * C + c] = h;
+  }
+}
+
+__global__ void k_relax_p(int C, const float* __restrict__ pn, float alpha, float* __restrict__ cv) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  float* p = cv + (size_t)3 * C;
+  p[c] = alpha > 0.f ? alpha * pn[c] + (1.0f - alpha) * p[c] : pn[c];
+}
+
+// face_interpolation('p') + the corrected conservative flux F_c per face.
+__global__ void k_face_p_fc(wcfd_mesh m, wcfd_bc bc, const float* __restrict__ cv,
+                            const float* __restrict__ cg, float* __restrict__ fv,
+                            const float* __restrict__ flux, const float* __restrict__ rua_f,
+                            float* __restrict__ fc) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  const int C = m.n_cells, F = m.n_faces, Fi = m.n_internal, Fb = m.n_boundary;
+  if (f >= F) return;
+  const float* p = cv + (size_t)3 * C;
+  if (f < Fi) {
+    const int o = m.owner[f], n = m.neigh[f];
+    const float psi = m.psi[f];
+    const float psi2 = 1.0f - psi;
+    const float* g = cg + (size_t)9 * C;
+    float v = p[o] * psi + psi2 * p[n];
+    float g0 = g[o] * psi + psi2 * g[n];
+    float g1 = g[C + o] * psi + psi2 * g[C + n];
+    float g2 = g[2 * C + o] * psi + psi2 * g[2 * C + n];
+    v = v + (g0 * m.skew[f] + g1 * m.skew[Fi + f] + g2 * m.skew[2 * Fi + f]);
+    fv[3 * F + f] = v;
+    if (fc) fc[f] = flux[f] - (rua_f[f] * m.area[f] / m.dmag[f]) * (p[n] - p[o]);
+  } else {
+    const int j = f - Fi;
+    const int P = m.owner[f];
+    const uint8_t t = bc.type[3 * Fb + j];
+    if (t == WCFD_NEUMANN) fv[3 * F + f] = m.dmag[f] * bc.value[3 * Fb + j] + p[P];
+    if (fc) {
+      float F_ = flux[f];
+      if (t == WCFD_DIRICHLET)
+        F_ = F_ - (rua_f[f] * m.area[f] / m.dmag[f]) * (bc.value[3 * Fb + j] - p[P]);
+      fc[f] = F_;
+    }
+  }
+}
+
+// calculate_gradients('p') + HbyA -= rUA grad p + replace_cell_values, plus
+// the float64 continuity partial sums of the corrected flux (grid-stride so
+// the reduction grid stays within kMaxReduceBlocks).
+__global__ void k_grad_p_correct(wcfd_mesh m, const float* __restrict__ fv,
+                                 float* __restrict__ cg, float* __restrict__ cv,
+                                 float* __restrict__ hbya, const float* __restrict__ rua,
+                                 const float* __restrict__ fc, double* __restrict__ partials,
+                                 unsigned* ticket, double* __restrict__ resid) {
+  const int C = m.n_cells, F = m.n_faces;
+  double acc[2] = {0.0, 0.0};
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
+    int fk[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) fk[k] = m.slot_face[k * C + c];
+    float g[3];
+    gg_field(m, c, fk, fv + (size_t)3 * F, m.volume[c], g);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) cg[i * C + c] = 0.f;
+    cg[9 * C + c] = g[0];
+    cg[10 * C + c] = g[1];
+    cg[11 * C + c] = g[2];
+    const float r = rua[c];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const float h = hbya[q * C + c] - r * g[q];
+      hbya[q * C + c] = h;
+      cv[q * C + c] = h;
+    }
+    if (fc) {
+      double num = 0.0, den = 0.0;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        const double v = (double)fc[fk[k]];
+        num += m.slot_side[k * C + c] == 0 ? v : -v;
+        den += fabs(v);
+      }
+      acc[0] += fabs(num);
+      acc[1] += den;
+    }
+  }
+  if (!fc) return;
+  double tot[2];
+  if (grid_sum<2>(acc, partials, ticket, tot)) {
+    if (threadIdx.x == 0) { resid[0] = tot[0]; resid[1] = tot[1]; }
+  }
+}
+
+// ========================================================== host launchers
+int launch_set_bc(const wcfd_mesh& m, wcfd_bc bc, float* fv, int mask, hipStream_t s) {
+  if (m.n_boundary == 0) return WCFD_OK;
+  WCFD_LAUNCH(K_SET_BC, s, k_set_bc, grid_for(m.n_boundary), m.n_faces, m.n_internal, m.n_boundary, bc.type,
+                                                     bc.value, fv, mask);
+  return check_launch();
+}
+
+int launch_face_interp(const wcfd_mesh& m, wcfd_bc bc, const float* cv, const float* cg, float* fv,
+                       int mask, int skew_mask, float* flux, const float* rua, float* rua_f,
+                       int apply_dirichlet, hipStream_t s) {
+  WCFD_LAUNCH(K_FACE_INTERP, s, k_face_interp, grid_for(m.n_faces), m, bc, cv, cg, fv, mask, skew_mask, flux,
+                                                       rua, rua_f, apply_dirichlet);
+  return check_launch();
+}
+
+int launch_green_gauss(const wcfd_mesh& m, const float* fv, float* cg, int mask, int zero_others,
+                       hipStream_t s) {
+  WCFD_LAUNCH(K_GREEN_GAUSS, s, k_green_gauss, grid_for(m.n_cells), m, fv, cg, mask, zero_others);
+  return check_launch();
+}
+
+int launch_assemble_momentum(const wcfd_mesh& m, wcfd_bc bc, const float* cv, const float* fv,
+                             const float* flux, float* cg, int compute_grad, float nu,
+                             float inv_density, float alpha, float dt, int flags,
+                             float* values, float* rhs, hipStream_t s) {
+  WCFD_LAUNCH(K_ASM_MOM, s, k_assemble_momentum, grid_for(m.n_cells),
+      m, bc, cv, fv, flux, cg, compute_grad, nu, inv_density, alpha, dt, flags, values, rhs);
+  return check_launch();
+}
+
+int launch_assemble_pressure(const wcfd_mesh& m, wcfd_bc bc, const float* fv, const float* flux,
+                             const float* rua_f, float* cg, int compute_grad_p, int correction,
+                             int ref_cell, float ref_value, float* values, float* rhs,
+                             float* div_out, hipStream_t s) {
+  WCFD_LAUNCH(K_ASM_P, s, k_assemble_pressure, grid_for(m.n_cells), m, bc, fv, flux, rua_f, cg,
+                                                             compute_grad_p, correction, ref_cell,
+                                                             ref_value, values, rhs, div_out);
+  return check_launch();
+}
+
+int launch_rua_hbya(const wcfd_mesh& m, const float* values, float density, const float* cg,
+                    float* hbya, float* cv, float* rua, hipStream_t s) {
+  WCFD_LAUNCH(K_RUA_HBYA, s, k_rua_hbya, grid_for(m.n_cells), m, values, density, cg, hbya, cv, rua);
+  return check_launch();
+}
+
+int launch_relax_p(const wcfd_mesh& m, const float* pn, float alpha, float* cv, hipStream_t s) {
+  WCFD_LAUNCH(K_RELAX_P, s, k_relax_p, grid_for(m.n_cells), m.n_cells, pn, alpha, cv);
+  return check_launch();
+}
+
+int launch_correct_velocity(const wcfd_mesh& m, wcfd_bc bc, float* cv, float* cg, float* fv,
+                            const float* flux, const float* rua, const float* rua_f, float* hbya,
+                            double* resid, float* fc_scratch, double* partials, unsigned* ticket,
+                            hipStream_t s) {
+  float* fc = resid ? fc_scratch : nullptr;
+  WCFD_LAUNCH(K_FACE_P_FC, s, k_face_p_fc, grid_for(m.n_faces), m, bc, cv, cg, fv, flux, rua_f, fc);
+  int e = check_launch();
+  if (e) return e;
+  WCFD_LAUNCH(K_GRAD_P_CORR, s, k_grad_p_correct, reduce_grid(m.n_cells), m, fv, cg, cv, hbya, rua, fc,
+                                                             partials, ticket, resid);
+  return check_launch();
+}
+
+}  // namespace wcfd
+
+// ============================================================== C ABI
+using namespace wcfd;
+
+static bool mesh_ok(const wcfd_mesh* m) {
+  return m && m->n_cells > 0 && m->n_faces >= m->n_internal && m->n_internal >= 0 &&
+         m->n_boundary == m->n_faces - m->n_internal && m->owner && m->slot_face && m->volume;
+}
+
+extern "C" {
+
+int wcfd_last_hip_error(void) { return g_last_hip_error; }
+
+const char* wcfd_error_string(int code) {
+  switch (code) {
+    case WCFD_OK: return "ok";
+    case WCFD_EINVAL: return "invalid argument";
+    case WCFD_EHIP: return "HIP runtime error";
+    case WCFD_ENOTCONV: return "solver did not converge";
+    case WCFD_ENAN: return "non-finite residual";
+    default: return "unknown error";
+  }
+}
+
+int wcfd_abi_version(void) { return 1; }
+
+int wcfd_set_boundary_conditions(const wcfd_mesh* m, wcfd_bc bc, float* face_values,
+                                 int field_mask, void* stream) {
+  if (!mesh_ok(m) || !face_values || !bc.type || !bc.value) return WCFD_EINVAL;
+  return launch_set_bc(*m, bc, face_values, field_mask, (hipStream_t)stream);
+}
+
+int wcfd_face_interpolation(const wcfd_mesh* m, wcfd_bc bc, const float* cell_values,
+                            const float* cell_grad, float* face_values, int field_mask,
+                            int skew_mask, float* mass_flux, const float* rua_cell,
+                            float* rua_face, void* stream) {
+  if (!mesh_ok(m) || !cell_values || !cell_grad || !face_values) return WCFD_EINVAL;
+  if ((rua_cell == nullptr) != (rua_face == nullptr)) return WCFD_EINVAL;
+  return launch_face_interp(*m, bc, cell_values, cell_grad, face_values, field_mask & 15,
+                            skew_mask & 15, mass_flux, rua_cell, rua_face, 0, (hipStream_t)stream);
+}
+
+int wcfd_green_gauss(const wcfd_mesh* m, const float* face_values, float* cell_grad, int field_mask,
+                     int zero_others, void* stream) {
+  if (!mesh_ok(m) || !face_values || !cell_grad) return WCFD_EINVAL;
+  return launch_green_gauss(*m, face_values, cell_grad, field_mask & 15, zero_others,
+                            (hipStream_t)stream);
+}
+
+int wcfd_mass_flux(const wcfd_mesh* m, const float* face_values, float* mass_flux, void* stream) {
+  if (!mesh_ok(m) || !face_values || !mass_flux) return WCFD_EINVAL;
+  WCFD_LAUNCH(K_MASS_FLUX, (hipStream_t)stream, k_mass_flux, grid_for(m->n_faces), *m, face_values, mass_flux);
+  return check_launch();
+}
+
+int wcfd_divergence(const wcfd_mesh* m, const float* mass_flux, float* div, void* stream) {
+  if (!mesh_ok(m) || !mass_flux || !div) return WCFD_EINVAL;
+  WCFD_LAUNCH(K_DIVERGENCE, (hipStream_t)stream, k_divergence, grid_for(m->n_cells), *m, mass_flux, div);
+  return check_launch();
+}
+
+int wcfd_assemble_momentum(const wcfd_mesh* m, wcfd_bc bc, const float* cell_values,
+                           const float* face_values, const float* mass_flux, float* cell_grad,
+                           int compute_gradients, float nu, float inv_density, float alpha_u,
+                           float dt, int scheme_flags, float* values, float* rhs, void* stream) {
+  if (!mesh_ok(m) || !cell_values || !face_values || !mass_flux || !cell_grad || !values || !rhs)
+    return WCFD_EINVAL;
+  return launch_assemble_momentum(*m, bc, cell_values, face_values, mass_flux, cell_grad,
+                                  compute_gradients, nu, inv_density, alpha_u, dt, scheme_flags,
+                                  values, rhs, (hipStream_t)stream);
+}
+
+int wcfd_assemble_pressure(const wcfd_mesh* m, wcfd_bc bc, const float* face_values,
+                           const float* mass_flux, const float* rua_face, float* cell_grad,
+                           int compute_grad_p, int correction, int ref_cell, float ref_value,
+                           float* values, float* rhs, float* div_out, void* stream) {
+  if (!mesh_ok(m) || !face_values || !mass_flux || !rua_face || !values || !rhs) return WCFD_EINVAL;
+  if (compute_grad_p && !cell_grad) return WCFD_EINVAL;
+  if (ref_cell >= m->n_cells) return WCFD_EINVAL;
+  return launch_assemble_pressure(*m, bc, face_values, mass_flux, rua_face, cell_grad,
+                                  compute_grad_p, correction, ref_cell, ref_value, values, rhs,
+                                  div_out, (hipStream_t)stream);
+}
+
+int wcfd_rua_hbya(const wcfd_mesh* m, const float* values, float density, const float* cell_grad,
+                  float* hbya, float* cell_values, float* rua, void* stream) {
+  if (!mesh_ok(m) || !values || !cell_grad || !hbya || !cell_values || !rua) return WCFD_EINVAL;
+  return launch_rua_hbya(*m, values, density, cell_grad, hbya, cell_values, rua,
+                         (hipStream_t)stream);
+}
+
+int wcfd_relax_pressure(const wcfd_mesh* m, const float* p_new, float alpha, float* cell_values,
+                        void* stream) {
+  if (!mesh_ok(m) || !p_new || !cell_values) return WCFD_EINVAL;
+  return launch_relax_p(*m, p_new, alpha, cell_values, (hipStream_t)stream);
+}
+
+size_t wcfd_reduce_workspace_bytes(int32_t n_faces) {
+  Carver c(nullptr);
+  c.take<double>(2 * kMaxReduceBlocks);
+  c.take<unsigned>(64);
+  c.take<float>(n_faces > 0 ? n_faces : 1);
+  return c.off;
+}
+
+int wcfd_correct_velocity(const wcfd_mesh* m, wcfd_bc bc, float* cell_values, float* cell_grad,
+                          float* face_values, const float* mass_flux, const float* rua,
+                          const float* rua_face, float* hbya, double* resid, void* workspace,
+                          void* stream) {
+  if (!mesh_ok(m) || !cell_values || !cell_grad || !face_values || !rua || !hbya || !rua_face)
+    return WCFD_EINVAL;
+  if (resid && (!workspace || !mass_flux)) return WCFD_EINVAL;
+  Carver w(workspace);
+  double* partials = w.take<double>(2 * kMaxReduceBlocks);
+  unsigned* ticket = w.take<unsigned>(64);
+  float* fc = w.take<float>(m->n_faces);
+  return launch_correct_velocity(*m, bc, cell_values, cell_grad, face_values, mass_flux, rua,
+                                 rua_face, hbya, resid, fc, partials, ticket, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,97 @@
+// Opt-in per-launch HIP-event profiler (the only process-wide state of the
+// library; guarded by a mutex, off unless wcfd_profile_begin() is called).
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <mutex>
+#include <vector>
+
+#include "common.h"
+
+namespace wcfd {
+
+namespace {
+struct Rec {
+  int id;
+  hipEvent_t a, b;
+};
+std::mutex g_mu;
+std::atomic<bool> g_on{false};
+std::vector<hipEvent_t> g_pool;
+std::vector<Rec> g_recs;
+size_t g_next = 0;
+int g_open = -1;  // index of the record waiting for its stop event
+const char* kNames[K_COUNT] = {
+    "set_bc", "face_interp", "green_gauss", "mass_flux", "divergence", "assemble_momentum",
+    "assemble_pressure", "rua_hbya", "relax_p", "face_p_fc", "grad_p_correct", "inv_diag", "spmv",
+    "dot", "cg_init", "cg_1", "cg_2", "bicgstab_init", "bicgstab_1", "bicgstab_2", "bicgstab_3"};
+}  // namespace
+
+bool prof_active() { return g_on.load(std::memory_order_relaxed); }
+
+void prof_record(int id, hipStream_t s, bool start) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_on) return;
+  if (start) {
+    if (g_next + 2 > g_pool.size()) { g_open = -1; return; }
+    Rec r{id, g_pool[g_next], g_pool[g_next + 1]};
+    g_next += 2;
+    (void)hipEventRecord(r.a, s);
+    g_recs.push_back(r);
+    g_open = static_cast<int>(g_recs.size()) - 1;
+  } else if (g_open >= 0) {
+    (void)hipEventRecord(g_recs[g_open].b, s);
+    g_open = -1;
+  }
+}
+
+}  // namespace wcfd
+
+using namespace wcfd;
+
+extern "C" {
+
+int wcfd_kernel_count(void) { return K_COUNT; }
+
+const char* wcfd_kernel_name(int id) { return (id >= 0 && id < K_COUNT) ? kNames[id] : "?"; }
+
+int wcfd_profile_begin(int capacity) {
+  if (capacity <= 0) return WCFD_EINVAL;
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (auto e : g_pool) (void)hipEventDestroy(e);
+  g_pool.clear();
+  g_recs.clear();
+  g_next = 0;
+  g_open = -1;
+  g_pool.resize(2 * static_cast<size_t>(capacity));
+  for (auto& e : g_pool) {
+    int rc = check_hip(hipEventCreate(&e));
+    if (rc) return rc;
+  }
+  g_recs.reserve(capacity);
+  g_on = true;
+  return WCFD_OK;
+}
+
+int wcfd_profile_end(double* total_ms, int32_t* counts, int n_ids) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_on = false;
+  if (n_ids < K_COUNT || !total_ms || !counts) return WCFD_EINVAL;
+  for (int i = 0; i < K_COUNT; ++i) { total_ms[i] = 0.0; counts[i] = 0; }
+  for (auto& r : g_recs) {
+    int rc = check_hip(hipEventSynchronize(r.b));
+    if (rc) return rc;
+    float ms = 0.f;
+    rc = check_hip(hipEventElapsedTime(&ms, r.a, r.b));
+    if (rc) return rc;
+    total_ms[r.id] += ms;
+    counts[r.id] += 1;
+  }
+  for (auto e : g_pool) (void)hipEventDestroy(e);
+  g_pool.clear();
+  g_recs.clear();
+  g_next = 0;
+  return WCFD_OK;
+}
+
+}  // extern "C"

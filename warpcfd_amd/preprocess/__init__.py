@@ -1,0 +1,3 @@
+from .mesh import Mesh
+from .grid import define_boundary_walls, structured_hex, cavity_mesh, channel_mesh, cylinder_mesh
+from .layout import DeviceLayout

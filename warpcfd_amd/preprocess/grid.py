@@ -1,0 +1,101 @@
+"""Synthetic hexahedral mesh generators and boundary wall tagging.
+
+The reference meshes with gmsh (preprocess/grid.py:8-159), which is absent;
+these generators produce the same kind of input -- raw ``(nodes, cells)``
+arrays of VTK hexahedra -- for the cavity, channel and cylinder workloads.
+``define_boundary_walls`` follows preprocess/grid.py:165-194.
+"""
+import numpy as np
+
+
+def structured_hex(counts, lengths=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0),
+                   jitter=0.0, seed=0):
+    """Box of nx*ny*nz hexahedra, nodes on a lattice, cells lexicographic
+    (x fastest), VTK node order (0-3 bottom counter-clockwise, 4-7 top).
+
+    ``jitter`` moves every *interior* node by U(-jitter*h, +jitter*h) per axis
+    (h = the cell size along that axis) from ``numpy.random.default_rng(seed)``;
+    boundary nodes stay on the box so boundary normals stay axis-aligned.
+    Returns (nodes (Nn,3) float64, cells (C,8) int32).
+    """
+    nx, ny, nz = (int(c) for c in counts)
+    Lx, Ly, Lz = lengths
+    ox, oy, oz = origin
+    xs = ox + np.linspace(0.0, Lx, nx + 1)
+    ys = oy + np.linspace(0.0, Ly, ny + 1)
+    zs = oz + np.linspace(0.0, Lz, nz + 1)
+    Z, Y, X = np.meshgrid(zs, ys, xs, indexing='ij')
+    nodes = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    if jitter > 0.0:
+        rng = np.random.default_rng(seed)
+        h = np.array([Lx / nx, Ly / ny, Lz / nz])
+        d = rng.uniform(-jitter, jitter, size=nodes.shape) * h
+        ii = np.arange(nx + 1)
+        jj = np.arange(ny + 1)
+        kk = np.arange(nz + 1)
+        K, J, I = np.meshgrid(kk, jj, ii, indexing='ij')
+        interior = ((I > 0) & (I < nx) & (J > 0) & (J < ny) & (K > 0) & (K < nz)).ravel()
+        nodes[interior] += d[interior]
+    i = np.arange(nx)
+    j = np.arange(ny)
+    k = np.arange(nz)
+    K, J, I = np.meshgrid(k, j, i, indexing='ij')
+    I, J, K = I.ravel(), J.ravel(), K.ravel()
+    sx, sy = 1, nx + 1
+    sz = (nx + 1) * (ny + 1)
+    n0 = I * sx + J * sy + K * sz
+    cells = np.stack([n0, n0 + sx, n0 + sx + sy, n0 + sy,
+                      n0 + sz, n0 + sx + sz, n0 + sx + sy + sz, n0 + sy + sz], axis=1)
+    return nodes, cells.astype(np.int32)
+
+
+def cavity_mesh(n, dz=0.1):
+    """Unit-square n x n x 1 cavity, dz thick (examples/LDC/LDC.py:24-29)."""
+    return structured_hex((n, n, 1), (1.0, 1.0, dz))
+
+
+def channel_mesh(nx=200, ny=40, nz=40, lengths=(10.0, 2.0, 2.0), jitter=0.1, seed=0):
+    """Box channel with +-jitter*h interior-node jitter."""
+    return structured_hex((nx, ny, nz), lengths, jitter=jitter, seed=seed)
+
+
+def cylinder_mesh(h=0.05, lengths=(20.0, 8.0, 1.0), centre=(5.0, 4.0), radius=0.5,
+                  jitter=0.1, seed=0):
+    """Voxel-carved cylinder in a box: cells whose centroid lies inside the
+    circle are removed; unused nodes are dropped and the rest renumbered.
+    Interior nodes away from the carved surface are jittered."""
+    nx, ny, nz = (int(round(L / h)) for L in lengths)
+    nodes, cells = structured_hex((nx, ny, nz), lengths, jitter=0.0)
+    cen = nodes[cells].mean(axis=1)
+    keep = (cen[:, 0] - centre[0]) ** 2 + (cen[:, 1] - centre[1]) ** 2 > radius ** 2
+    cells = cells[keep]
+    used = np.zeros(nodes.shape[0], bool)
+    used[cells.ravel()] = True
+    # nodes touching a removed cell lie on the carved surface: keep them fixed
+    removed = np.zeros(nodes.shape[0], bool)
+    full_cells = structured_hex((nx, ny, nz), lengths, jitter=0.0)[1]
+    removed[full_cells[~keep].ravel()] = True
+    if jitter > 0.0:
+        rng = np.random.default_rng(seed)
+        hv = np.array([lengths[0] / nx, lengths[1] / ny, lengths[2] / nz])
+        d = rng.uniform(-jitter, jitter, size=nodes.shape) * hv
+        eps = 1e-9
+        lo = np.array([0.0, 0.0, 0.0]); hi = np.array(lengths, float)
+        interior = np.all((nodes > lo + eps) & (nodes < hi - eps), axis=1) & ~removed
+        nodes = nodes + d * interior[:, None]
+    remap = -np.ones(nodes.shape[0], np.int64)
+    remap[used] = np.arange(used.sum())
+    return nodes[used], remap[cells].astype(np.int32)
+
+
+def define_boundary_walls(mesh):
+    """preprocess/grid.py:165-194: group boundary faces by their owner-side
+    normal, |n.e - 1| < 1e-3 for e = +-x, +-y, +-z."""
+    b = mesh.boundary_face_ids
+    n = mesh.boundary_normals()
+    walls = {}
+    for name, e in (("+X", [1, 0, 0]), ("-X", [-1, 0, 0]), ("+Y", [0, 1, 0]),
+                    ("-Y", [0, -1, 0]), ("+Z", [0, 0, 1]), ("-Z", [0, 0, -1])):
+        walls[name] = b[np.abs(n @ np.array(e, dtype=n.dtype) - 1) < 1e-3]
+    mesh.add_groups(walls, 'face')
+    return walls
