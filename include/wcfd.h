@@ -158,8 +158,7 @@ int wcfd_relax_pressure(const wcfd_mesh* m, const float* p_new, float alpha,
  * sum_c|sum_f +-F_c| / sum_c sum_f |F_c| of the corrected flux
  * F_c = F - rUA_f A_f (p_N - p_O)/|d|, written to resid[0..1]
  * (numerator, denominator) if resid != NULL.  Needs a workspace of
- * wcfd_reduce_workspace_bytes(F) bytes, zero-filled before its first use
- * (the library leaves its counters at zero after every call). */
+ * wcfd_reduce_workspace_bytes(F) bytes. */
 int wcfd_correct_velocity(const wcfd_mesh* m, wcfd_bc bc, float* cell_values,
                           float* cell_grad, float* face_values, const float* mass_flux,
                           const float* rua, const float* rua_face, float* hbya,
@@ -173,7 +172,8 @@ size_t wcfd_reduce_workspace_bytes(int32_t n_faces);
 int wcfd_spmv(int32_t n, const int32_t* row_ptr, const int32_t* col, const float* values,
               int nvec, const float* x, float* y, void* stream);
 
-/* float64 dot product of two float32 vectors, result to out[0]. */
+/* float64 dot product of two float32 vectors, result to out[0] (device).
+ * Workspace: wcfd_reduce_workspace_bytes(0) bytes. */
 int wcfd_dot_f64(int32_t n, const float* a, const float* b, double* out, void* workspace,
                  void* stream);
 

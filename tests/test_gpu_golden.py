@@ -1,0 +1,49 @@
+"""GPU runs against the committed oracle fixtures (tests/golden)."""
+import os
+
+import numpy as np
+import pytest
+
+from cases import device_case, rel_l2
+
+pytestmark = pytest.mark.gpu
+G = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden')
+
+
+def test_cavity24_converged_vs_fp64_oracle(gpu):
+    """Converged float32 GPU fields vs the float64 oracle: <= 1e-4."""
+    from warpcfd_amd.FV.implicit_Solvers import IncompressibleSolver
+    from warpcfd_amd.preprocess.grid import cavity_mesh
+    z = np.load(os.path.join(G, 'cavity24_converged.npz'))
+    nodes, cells = cavity_mesh(24)
+    model = device_case(nodes, cells, 'cavity')
+    sv = IncompressibleSolver(model, 0.7, 0.3)
+    sv.tol = 1e-9
+    for _ in range(400):
+        if sv.outer_iteration()['continuity'] < 1e-7:
+            break
+    got = model.cell_values.cpu().numpy()
+    for q in (0, 1, 3):
+        assert rel_l2(got[:, q], z['f64'][:, q]) <= 1e-4, q
+        assert rel_l2(got[:, q], z['f32'][:, q]) <= 1e-4, q
+
+
+def test_cavity64_500_iterations_reference_settings(gpu):
+    """The 64x64 cavity, 500 iterations, reference solver settings
+    (examples/LDC/LDC.py set-up).  With atol = 1e-5 the pressure solves are
+    only fixed to that tolerance, so the trajectory is compared at 1e-3
+    (velocity) and 1e-2 (pressure); converged fields are compared at 1e-4
+    above."""
+    from warpcfd_amd.FV.implicit_Solvers import IncompressibleSolver
+    from warpcfd_amd.preprocess.grid import cavity_mesh
+    z = np.load(os.path.join(G, 'cavity64_ref500.npz'))
+    nodes, cells = cavity_mesh(64)
+    model = device_case(nodes, cells, 'cavity')
+    sv = IncompressibleSolver(model, 0.7, 0.3)
+    for _ in range(500):
+        sv.outer_iteration()
+    got = model.cell_values.cpu().numpy()
+    ref = z['cell_values']
+    print('cavity64 rel-L2 u,v,p:', [rel_l2(got[:, q], ref[:, q]) for q in (0, 1, 3)])
+    assert rel_l2(got[:, :2], ref[:, :2]) <= 1e-3
+    assert rel_l2(got[:, 3], ref[:, 3]) <= 1e-2

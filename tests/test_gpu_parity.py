@@ -337,3 +337,36 @@ def test_simple_is_deterministic(gpu):
             sv.outer_iteration()
         outs.append(model.cell_values.cpu().numpy())
     assert np.array_equal(outs[0], outs[1])
+
+
+def test_cg_after_bicgstab_in_one_workspace(gpu):
+    """Regression: CG and the 3-vector BiCGSTAB share the SIMPLE workspace;
+    the solver state must not overlap either solver's vectors."""
+    torch = _torch()
+    from warpcfd_amd._lib import SolveInfo
+    from warpcfd_amd.FV.device import ptr, stream_ptr
+    nodes, cells = jittered_box((24, 16, 12), 0.1, 21)
+    model = device_case(nodes, cells, 'channel')
+    A = _spd_system(model, 5)
+    n = model.layout.num_cells
+    dev = model._cv.device
+    lib = model.lib
+    vals = torch.from_numpy(A.data).to(dev)
+    negA = torch.from_numpy((-A).data.astype(np.float32)).to(dev)
+    rng = np.random.default_rng(6)
+    b3 = torch.from_numpy(rng.standard_normal((3, n)).astype(np.float32)).to(dev)
+    b1 = rng.standard_normal(n).astype(np.float32)
+    ws = torch.zeros(lib.wcfd_simple_workspace_bytes(n, model.num_faces), dtype=torch.uint8, device=dev)
+    args = (ptr(model.dm.row_ptr), ptr(model.dm.col))
+    for _ in range(2):
+        x3 = torch.zeros(3, n, dtype=torch.float32, device=dev)
+        info = SolveInfo()
+        assert lib.wcfd_bicgstab(n, *args, ptr(negA), ptr(model.dm.diag_pos), 3, ptr(b3), ptr(x3), 1e-6, 1e-6,
+                                 500, 10, ptr(ws), ctypes.byref(info), stream_ptr()) == 0
+        x1 = torch.zeros(n, dtype=torch.float32, device=dev)
+        info = SolveInfo()
+        assert lib.wcfd_cg(n, *args, ptr(vals), ptr(model.dm.diag_pos), ptr(torch.from_numpy(b1).to(dev)),
+                           ptr(x1), 1e-6, 1e-6, 1000, 10, ptr(ws), ctypes.byref(info), stream_ptr()) == 0
+        assert info.converged and info.iterations > 0
+        xs = spla.spsolve(A.astype(np.float64).tocsc(), b1.astype(np.float64))
+        assert rel_l2(x1.cpu().numpy(), xs) <= 1e-4

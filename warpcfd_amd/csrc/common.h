@@ -10,7 +10,8 @@ namespace wcfd {
 
 constexpr int kBlock = 256;           // threads per workgroup for streaming kernels
 constexpr int kWave = 64;             // CDNA wavefront
-constexpr int kMaxReduceBlocks = 2048;  // partials the last workgroup folds
+constexpr int kMaxReduceBlocks = 2048;  // upper bound on partials of one reduction
+constexpr int kKrylovGrid = 1024;       // workgroups of the Krylov kernels (grid-stride)
 
 // per-thread HIP error of the last failing call
 extern thread_local int g_last_hip_error;
@@ -22,7 +23,7 @@ extern thread_local int g_last_hip_error;
 enum KernelId {
   K_SET_BC, K_FACE_INTERP, K_GREEN_GAUSS, K_MASS_FLUX, K_DIVERGENCE, K_ASM_MOM, K_ASM_P,
   K_RUA_HBYA, K_RELAX_P, K_FACE_P_FC, K_GRAD_P_CORR, K_INV_DIAG, K_SPMV, K_DOT, K_CG_INIT,
-  K_CG_1, K_CG_2, K_BI_INIT, K_BI_1, K_BI_2, K_BI_3, K_COUNT
+  K_CG_1, K_CG_2, K_BI_INIT, K_BI_1, K_BI_2, K_BI_3, K_FOLD, K_COUNT
 };
 bool prof_active();
 void prof_record(int id, hipStream_t s, bool start);
@@ -110,48 +111,6 @@ __device__ inline void block_sum(double (&v)[ND], double* lds /* [ND][kBlock/kWa
     }
   }
   __syncthreads();
-}
-
-// Grid-wide deterministic sum: every workgroup stores its partials, the last
-// workgroup to arrive (atomic ticket) folds them in block order and returns
-// true with the totals in `out` (valid in all its threads).  The ticket is
-// reset by that workgroup so the next launch starts from zero.
-template <int ND>
-__device__ inline bool grid_sum(double (&v)[ND], double* partials /* [ND][gridDim] */,
-                                unsigned* ticket, double (&out)[ND]) {
-  __shared__ double lds[ND * (kBlock / kWave)];
-  __shared__ bool last;
-  block_sum<ND>(v, lds);
-  const unsigned nb = gridDim.x;
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int i = 0; i < ND; ++i) partials[i * nb + blockIdx.x] = v[i];
-    __threadfence();
-    unsigned t = atomicAdd(ticket, 1u);
-    last = (t == nb - 1);
-  }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  double acc[ND];
-#pragma unroll
-  for (int i = 0; i < ND; ++i) {
-    double s = 0.0;
-    for (unsigned b = threadIdx.x; b < nb; b += blockDim.x)
-      s += partials[i * nb + b];
-    acc[i] = s;
-  }
-  block_sum<ND>(acc, lds);
-  __shared__ double bc[ND];
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int i = 0; i < ND; ++i) bc[i] = acc[i];
-    *ticket = 0u;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < ND; ++i) out[i] = bc[i];
-  return true;
 }
 
 }  // namespace wcfd

@@ -23,8 +23,7 @@ int launch_rua_hbya(const wcfd_mesh& m, const float* values, float density, cons
 int launch_relax_p(const wcfd_mesh& m, const float* pn, float alpha, float* cv, hipStream_t s);
 int launch_correct_velocity(const wcfd_mesh& m, wcfd_bc bc, float* cv, float* cg, float* fv,
                             const float* flux, const float* rua, const float* rua_f, float* hbya,
-                            double* resid, float* fc_scratch, double* partials, unsigned* ticket,
-                            hipStream_t s);
+                            double* resid, float* fc_scratch, double* partials, hipStream_t s);
 
 // Krylov solvers (krylov.hip)
 struct CsrView {

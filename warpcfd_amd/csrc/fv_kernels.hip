@@ -421,8 +421,7 @@ __global__ void k_face_p_fc(wcfd_mesh m, wcfd_bc bc, const float* __restrict__ c
 __global__ void k_grad_p_correct(wcfd_mesh m, const float* __restrict__ fv,
                                  float* __restrict__ cg, float* __restrict__ cv,
                                  float* __restrict__ hbya, const float* __restrict__ rua,
-                                 const float* __restrict__ fc, double* __restrict__ partials,
-                                 unsigned* ticket, double* __restrict__ resid) {
+                                 const float* __restrict__ fc, double* __restrict__ partials) {
   const int C = m.n_cells, F = m.n_faces;
   double acc[2] = {0.0, 0.0};
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
@@ -456,9 +455,44 @@ __global__ void k_grad_p_correct(wcfd_mesh m, const float* __restrict__ fv,
     }
   }
   if (!fc) return;
-  double tot[2];
-  if (grid_sum<2>(acc, partials, ticket, tot)) {
-    if (threadIdx.x == 0) { resid[0] = tot[0]; resid[1] = tot[1]; }
+  // per-workgroup partials, folded by k_fold_resid (no cross-workgroup wait)
+  __shared__ double lds[2 * (kBlock / kWave)];
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const double s = wave_sum(acc[i]);
+    if (lane == 0) lds[i * (kBlock / kWave) + wid] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < 2; ++i) {
+      double t = 0.0;
+      for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += lds[i * (kBlock / kWave) + w];
+      partials[i * gridDim.x + blockIdx.x] = t;
+    }
+  }
+}
+
+// one workgroup: fold the continuity partials in block order
+__global__ void k_fold_resid(const double* __restrict__ partials, int G, double* __restrict__ resid) {
+  __shared__ double lds[2 * (kBlock / kWave)];
+  double acc[2] = {0.0, 0.0};
+  for (int b = threadIdx.x; b < G; b += blockDim.x) {
+    acc[0] += partials[b];
+    acc[1] += partials[G + b];
+  }
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  for (int i = 0; i < 2; ++i) {
+    const double s = wave_sum(acc[i]);
+    if (lane == 0) lds[i * (kBlock / kWave) + wid] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < 2; ++i) {
+      double t = 0.0;
+      for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += lds[i * (kBlock / kWave) + w];
+      resid[i] = t;
+    }
   }
 }
 
@@ -516,14 +550,14 @@ int launch_relax_p(const wcfd_mesh& m, const float* pn, float alpha, float* cv, 
 
 int launch_correct_velocity(const wcfd_mesh& m, wcfd_bc bc, float* cv, float* cg, float* fv,
                             const float* flux, const float* rua, const float* rua_f, float* hbya,
-                            double* resid, float* fc_scratch, double* partials, unsigned* ticket,
-                            hipStream_t s) {
+                            double* resid, float* fc_scratch, double* partials, hipStream_t s) {
   float* fc = resid ? fc_scratch : nullptr;
   WCFD_LAUNCH(K_FACE_P_FC, s, k_face_p_fc, grid_for(m.n_faces), m, bc, cv, cg, fv, flux, rua_f, fc);
   int e = check_launch();
   if (e) return e;
-  WCFD_LAUNCH(K_GRAD_P_CORR, s, k_grad_p_correct, reduce_grid(m.n_cells), m, fv, cg, cv, hbya, rua, fc,
-                                                             partials, ticket, resid);
+  const unsigned G = reduce_grid(m.n_cells);
+  WCFD_LAUNCH(K_GRAD_P_CORR, s, k_grad_p_correct, G, m, fv, cg, cv, hbya, rua, fc, partials);
+  if (resid) WCFD_LAUNCH(K_FOLD, s, k_fold_resid, 1, partials, (int)G, resid);
   return check_launch();
 }
 
@@ -628,7 +662,6 @@ int wcfd_relax_pressure(const wcfd_mesh* m, const float* p_new, float alpha, flo
 size_t wcfd_reduce_workspace_bytes(int32_t n_faces) {
   Carver c(nullptr);
   c.take<double>(2 * kMaxReduceBlocks);
-  c.take<unsigned>(64);
   c.take<float>(n_faces > 0 ? n_faces : 1);
   return c.off;
 }
@@ -642,10 +675,9 @@ int wcfd_correct_velocity(const wcfd_mesh* m, wcfd_bc bc, float* cell_values, fl
   if (resid && (!workspace || !mass_flux)) return WCFD_EINVAL;
   Carver w(workspace);
   double* partials = w.take<double>(2 * kMaxReduceBlocks);
-  unsigned* ticket = w.take<unsigned>(64);
   float* fc = w.take<float>(m->n_faces);
   return launch_correct_velocity(*m, bc, cell_values, cell_grad, face_values, mass_flux, rua,
-                                 rua_face, hbya, resid, fc, partials, ticket, (hipStream_t)stream);
+                                 rua_face, hbya, resid, fc, partials, (hipStream_t)stream);
 }
 
 }  // extern "C"

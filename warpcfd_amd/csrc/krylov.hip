@@ -18,13 +18,62 @@
 
 namespace wcfd {
 
+// Scalars of one solve.  Quantities that change every iteration sit in
+// two-slot rings indexed by iteration parity: block 0 of a kernel writes the
+// slot of the current iteration while the other blocks of the same kernel
+// read the previous one.
 struct KState {
-  double alpha, beta, omega, rho, rz, rr, ss, thr2;
-  double atol, tol, bb, pad;
-  unsigned ticket[8];
+  double rz[2], rho[2], alpha[2], omega[2];
+  double thr2, atol, tol, rr, bb, ss;
 };
 
 __device__ inline double safe_div(double a, double b) { return b != 0.0 ? a / b : 0.0; }
+
+// Every reduction writes one float64 partial per workgroup; the NEXT kernel
+// folds the partials in its prologue, every workgroup in the same fixed order,
+// so all of them hold bitwise the same scalar.  No kernel waits on another
+// workgroup: a cross-workgroup handshake inside a kernel costs an agent-scope
+// release (an L2 write-back on every XCD) per workgroup, which measured
+// 5-6x slower than this scheme on MI355X at 320 k rows.
+__device__ inline double fold(const double* __restrict__ part, int G) {
+  __shared__ double lds[kBlock / kWave];
+  __shared__ double res;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < G; b += blockDim.x) s += part[b];
+  s = wave_sum(s);
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  if (lane == 0) lds[wid] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += lds[w];
+    res = t;
+  }
+  __syncthreads();
+  return res;
+}
+
+// Block sum of ND values; thread 0 stores them as this block's partials
+// part[i * G + blockIdx.x].
+template <int ND>
+__device__ inline void store_partials(double (&v)[ND], double* __restrict__ part, int G) {
+  __shared__ double lds[ND * (kBlock / kWave)];
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int i = 0; i < ND; ++i) {
+    double s = wave_sum(v[i]);
+    if (lane == 0) lds[i * (kBlock / kWave) + wid] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+      double t = 0.0;
+      for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += lds[i * (kBlock / kWave) + w];
+      part[i * G + blockIdx.x] = t;
+    }
+  }
+}
 
 __global__ void k_inv_diag(int n, const float* __restrict__ values, const int32_t* __restrict__ dpos,
                            float* __restrict__ inv) {
@@ -54,19 +103,24 @@ __global__ void k_spmv(int n, const int32_t* __restrict__ rp, const int32_t* __r
 }
 
 __global__ void k_dot(int n, const float* __restrict__ a, const float* __restrict__ b,
-                      double* partials, unsigned* ticket, double* out) {
+                      double* __restrict__ part) {
   double acc[1] = {0.0};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     acc[0] += (double)a[i] * (double)b[i];
-  double tot[1];
-  if (grid_sum<1>(acc, partials, ticket, tot) && threadIdx.x == 0) out[0] = tot[0];
+  store_partials<1>(acc, part, gridDim.x);
+}
+
+__global__ void k_fold_to(const double* __restrict__ part, int G, double* out) {
+  const double v = fold(part, G);
+  if (threadIdx.x == 0) out[0] = v;
 }
 
 // ------------------------------------------------------------------ CG
+// partial slots: PA[3][G] = {r.z, r.r, b.b} (init, k_cg_2), PB[1][G] = {p.q} (k_cg_1)
 __global__ void k_cg_init(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                           const float* __restrict__ val, const float* __restrict__ b,
                           const float* __restrict__ x, float* __restrict__ r,
-                          const float* __restrict__ inv, double* partials, KState* st) {
+                          const float* __restrict__ inv, double* __restrict__ PA) {
   double acc[3] = {0.0, 0.0, 0.0};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     float ax = 0.f;
@@ -78,26 +132,38 @@ __global__ void k_cg_init(int n, const int32_t* __restrict__ rp, const int32_t* 
     acc[1] += (double)ri * (double)ri;
     acc[2] += (double)b[i] * (double)b[i];
   }
-  double tot[3];
-  if (grid_sum<3>(acc, partials, &st->ticket[0], tot) && threadIdx.x == 0) {
-    st->rz = tot[0];
-    st->rr = tot[1];
-    st->bb = tot[2];
-    const double thr = fmax(st->tol * sqrt(tot[2]), st->atol);
+  store_partials<3>(acc, PA, gridDim.x);
+}
+
+// one workgroup: fold the init partials, set the threshold and rz[0]
+__global__ void k_init_fold(const double* __restrict__ PA, int G, KState* st, int is_cg) {
+  const double a = fold(PA, G), rr = fold(PA + G, G), bb = fold(PA + 2 * G, G);
+  if (threadIdx.x == 0) {
+    if (is_cg) st->rz[0] = a; else st->rho[0] = a;
+    st->rr = rr;
+    st->bb = bb;
+    const double thr = fmax(st->tol * sqrt(bb), st->atol);
     st->thr2 = thr * thr;
-    st->beta = 0.0;
-    st->alpha = 0.0;
+    st->alpha[0] = st->alpha[1] = 0.0;
+    st->omega[0] = st->omega[1] = 0.0;
   }
 }
 
-__global__ void k_cg_1(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+__global__ void k_cg_1(int n, int it, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                        const float* __restrict__ val, const float* __restrict__ r,
                        const float* __restrict__ inv, const float* __restrict__ p_in,
-                       float* __restrict__ p_out, float* __restrict__ q, double* partials,
-                       KState* st) {
-  const double beta_d = st->beta;
+                       float* __restrict__ p_out, float* __restrict__ q,
+                       const double* __restrict__ PA, double* __restrict__ PB, KState* st) {
+  const int G = gridDim.x;
+  bool first = (it == 0);
+  double beta_d = 0.0;
+  if (!first) {
+    const double rz = fold(PA, G);                 // r.z of the previous iteration
+    beta_d = safe_div(rz, st->rz[(it - 1) & 1]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->rz[it & 1] = rz;
+    first = (beta_d == 0.0);
+  }
   const float beta = (float)beta_d;
-  const bool first = beta_d == 0.0;
   double acc[1] = {0.0};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     float qi = 0.f;
@@ -113,15 +179,17 @@ __global__ void k_cg_1(int n, const int32_t* __restrict__ rp, const int32_t* __r
     q[i] = qi;
     acc[0] += (double)pi * (double)qi;
   }
-  double tot[1];
-  if (grid_sum<1>(acc, partials, &st->ticket[1], tot) && threadIdx.x == 0)
-    st->alpha = safe_div(st->rz, tot[0]);
+  store_partials<1>(acc, PB, G);
 }
 
-__global__ void k_cg_2(int n, float* __restrict__ x, float* __restrict__ r,
+__global__ void k_cg_2(int n, int it, float* __restrict__ x, float* __restrict__ r,
                        const float* __restrict__ inv, const float* __restrict__ p,
-                       const float* __restrict__ q, double* partials, KState* st) {
-  const float alpha = (float)st->alpha;
+                       const float* __restrict__ q, const double* __restrict__ PB,
+                       double* __restrict__ PA, KState* st) {
+  const int G = gridDim.x;
+  const double pq = fold(PB, G);
+  const double rz = (it == 0) ? st->rz[0] : st->rz[it & 1];
+  const float alpha = (float)safe_div(rz, pq);
   double acc[2] = {0.0, 0.0};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     x[i] = x[i] + alpha * p[i];
@@ -131,21 +199,18 @@ __global__ void k_cg_2(int n, float* __restrict__ x, float* __restrict__ r,
     acc[0] += (double)ri * (double)zi;
     acc[1] += (double)ri * (double)ri;
   }
-  double tot[2];
-  if (grid_sum<2>(acc, partials, &st->ticket[2], tot) && threadIdx.x == 0) {
-    st->beta = safe_div(tot[0], st->rz);
-    st->rz = tot[0];
-    st->rr = tot[1];
-  }
+  store_partials<2>(acc, PA, G);
 }
 
 // ------------------------------------------------------------- BiCGSTAB
+// partial slots: PA[3][G] = {r0.r, r.r, b.b} (init, k_bi_3), PB[1][G] = {r0.v}
+// (k_bi_1), PC[3][G] = {t.s, t.t, s.s} (k_bi_2)
 template <int NV>
 __global__ void k_bi_init(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                           const float* __restrict__ val, const float* __restrict__ b,
                           const float* __restrict__ x, float* __restrict__ r,
-                          float* __restrict__ r0, double* partials, KState* st) {
-  double acc[2] = {0.0, 0.0};
+                          float* __restrict__ r0, double* __restrict__ PA) {
+  double acc[3] = {0.0, 0.0, 0.0};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     float ax[NV];
 #pragma unroll
@@ -164,32 +229,34 @@ __global__ void k_bi_init(int n, const int32_t* __restrict__ rp, const int32_t* 
       r[k] = ri;
       r0[k] = ri;
       acc[0] += (double)ri * (double)ri;
-      acc[1] += (double)bi * (double)bi;
+      acc[1] += (double)ri * (double)ri;
+      acc[2] += (double)bi * (double)bi;
     }
   }
-  double tot[2];
-  if (grid_sum<2>(acc, partials, &st->ticket[0], tot) && threadIdx.x == 0) {
-    st->rho = tot[0];
-    st->rr = tot[0];
-    st->bb = tot[1];
-    const double thr = fmax(st->tol * sqrt(tot[1]), st->atol);
-    st->thr2 = thr * thr;
-    st->beta = 0.0;
-    st->omega = 0.0;
-    st->alpha = 0.0;
-  }
+  store_partials<3>(acc, PA, gridDim.x);
 }
 
 template <int NV>
-__global__ void k_bi_1(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                       const float* __restrict__ val, const float* __restrict__ r,
-                       const float* __restrict__ inv, const float* __restrict__ p_in,
-                       const float* __restrict__ v_in, float* __restrict__ p_out,
-                       float* __restrict__ v_out, const float* __restrict__ r0,
-                       double* partials, KState* st) {
-  const bool first = st->beta == 0.0;
-  const float beta = (float)st->beta;
-  const float omega = (float)st->omega;
+__global__ void k_bi_1(int n, int it, const int32_t* __restrict__ rp,
+                       const int32_t* __restrict__ col, const float* __restrict__ val,
+                       const float* __restrict__ r, const float* __restrict__ inv,
+                       const float* __restrict__ p_in, const float* __restrict__ v_in,
+                       float* __restrict__ p_out, float* __restrict__ v_out,
+                       const float* __restrict__ r0, const double* __restrict__ PA,
+                       double* __restrict__ PB, KState* st) {
+  const int G = gridDim.x;
+  bool first = (it == 0);
+  double beta_d = 0.0, omega_d = 0.0;
+  if (!first) {
+    const int pv = (it - 1) & 1;
+    const double rho = fold(PA, G);               // r0.r after the previous iteration
+    omega_d = st->omega[pv];
+    beta_d = safe_div(rho, st->rho[pv]) * safe_div(st->alpha[pv], omega_d);
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->rho[it & 1] = rho;
+    first = (beta_d == 0.0);
+  }
+  const float beta = (float)beta_d;
+  const float omega = (float)omega_d;
   double acc[1] = {0.0};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     float v[NV];
@@ -215,18 +282,20 @@ __global__ void k_bi_1(int n, const int32_t* __restrict__ rp, const int32_t* __r
       acc[0] += (double)r0[k] * (double)v[q];
     }
   }
-  double tot[1];
-  if (grid_sum<1>(acc, partials, &st->ticket[1], tot) && threadIdx.x == 0)
-    st->alpha = safe_div(st->rho, tot[0]);
+  store_partials<1>(acc, PB, G);
 }
 
 template <int NV>
-__global__ void k_bi_2(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                       const float* __restrict__ val, const float* __restrict__ r,
-                       const float* __restrict__ inv, const float* __restrict__ v,
-                       float* __restrict__ s, float* __restrict__ t, double* partials,
-                       KState* st) {
-  const float alpha = (float)st->alpha;
+__global__ void k_bi_2(int n, int it, const int32_t* __restrict__ rp,
+                       const int32_t* __restrict__ col, const float* __restrict__ val,
+                       const float* __restrict__ r, const float* __restrict__ inv,
+                       const float* __restrict__ v, float* __restrict__ s, float* __restrict__ t,
+                       const double* __restrict__ PB, double* __restrict__ PC, KState* st) {
+  const int G = gridDim.x;
+  const double r0v = fold(PB, G);
+  const double alpha_d = safe_div(st->rho[it & 1], r0v);
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha[it & 1] = alpha_d;
+  const float alpha = (float)alpha_d;
   double acc[3] = {0.0, 0.0, 0.0};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     float tv[NV];
@@ -254,20 +323,21 @@ __global__ void k_bi_2(int n, const int32_t* __restrict__ rp, const int32_t* __r
       acc[2] += (double)si * (double)si;
     }
   }
-  double tot[3];
-  if (grid_sum<3>(acc, partials, &st->ticket[2], tot) && threadIdx.x == 0) {
-    st->ss = tot[2];
-    st->omega = tot[2] > st->thr2 ? safe_div(tot[0], tot[1]) : 0.0;
-  }
+  store_partials<3>(acc, PC, G);
 }
 
 template <int NV>
-__global__ void k_bi_3(int n, float* __restrict__ x, float* __restrict__ r,
+__global__ void k_bi_3(int n, int it, float* __restrict__ x, float* __restrict__ r,
                        const float* __restrict__ inv, const float* __restrict__ p,
                        const float* __restrict__ s, const float* __restrict__ t,
-                       const float* __restrict__ r0, double* partials, KState* st) {
-  const float alpha = (float)st->alpha;
-  const float omega = (float)st->omega;
+                       const float* __restrict__ r0, const double* __restrict__ PC,
+                       double* __restrict__ PA, KState* st) {
+  const int G = gridDim.x;
+  const double ts = fold(PC, G), tt = fold(PC + G, G), ss = fold(PC + 2 * G, G);
+  const double omega_d = ss > st->thr2 ? safe_div(ts, tt) : 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) { st->omega[it & 1] = omega_d; st->ss = ss; }
+  const float alpha = (float)st->alpha[it & 1];
+  const float omega = (float)omega_d;
   double acc[2] = {0.0, 0.0};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const float ii = inv[i];
@@ -283,24 +353,26 @@ __global__ void k_bi_3(int n, float* __restrict__ x, float* __restrict__ r,
       acc[1] += (double)rk * (double)rk;
     }
   }
-  double tot[2];
-  if (grid_sum<2>(acc, partials, &st->ticket[3], tot) && threadIdx.x == 0) {
-    st->beta = safe_div(tot[0], st->rho) * safe_div(st->alpha, st->omega);
-    st->rho = tot[0];
-    st->rr = tot[1];
-  }
+  store_partials<2>(acc, PA, G);
 }
 
 // ------------------------------------------------------------- host side
 struct KWork {
   float *r, *r0, *p0, *p1, *v0, *v1, *s, *t, *inv;
-  double* partials;
+  double *PA, *PB, *PC;
   KState* st;
 };
 
+// The scalar state and the reduction partials come first, at offsets that do
+// not depend on nvec: CG (1 vector) and BiCGSTAB (3 vectors) may share one
+// workspace.
 static KWork carve_krylov(void* ws, int32_t n, int nvec) {
   Carver c(ws);
   KWork w;
+  w.st = c.take<KState>(1);
+  w.PA = c.take<double>(3 * kMaxReduceBlocks);
+  w.PB = c.take<double>(kMaxReduceBlocks);
+  w.PC = c.take<double>(3 * kMaxReduceBlocks);
   const size_t nv = (size_t)n * nvec;
   w.r = c.take<float>(nv);
   w.r0 = c.take<float>(nv);
@@ -311,18 +383,18 @@ static KWork carve_krylov(void* ws, int32_t n, int nvec) {
   w.s = c.take<float>(nv);
   w.t = c.take<float>(nv);
   w.inv = c.take<float>(n);
-  w.partials = c.take<double>(4 * kMaxReduceBlocks);
-  w.st = c.take<KState>(1);
   return w;
 }
 
 size_t krylov_workspace_bytes(int32_t n, int nvec) {
   Carver c(nullptr);
+  c.take<KState>(1);
+  c.take<double>(3 * kMaxReduceBlocks);
+  c.take<double>(kMaxReduceBlocks);
+  c.take<double>(3 * kMaxReduceBlocks);
   const size_t nv = (size_t)n * nvec;
   for (int i = 0; i < 8; ++i) c.take<float>(nv);
   c.take<float>(n);
-  c.take<double>(4 * kMaxReduceBlocks);
-  c.take<KState>(1);
   return c.off;
 }
 
@@ -348,6 +420,15 @@ static void fill_info(wcfd_solve_info* info, int it, const KState& h) {
   info->threshold = sqrt(h.thr2);
 }
 
+// workgroups of the reduction kernels: a grid-stride grid, at most
+// kMaxReduceBlocks (the partials each workgroup folds in its prologue)
+static unsigned krylov_grid(int64_t n) {
+  int64_t g = (n + kBlock - 1) / kBlock;
+  const int64_t cap = kKrylovGrid;
+  if (g > cap) g = cap;
+  return static_cast<unsigned>(g < 1 ? 1 : g);
+}
+
 int solve_cg(const CsrView& A, const float* b, float* x, double atol, double tol, int maxiter,
              int check_every, void* workspace, wcfd_solve_info* info, hipStream_t s) {
   const int n = A.n;
@@ -355,29 +436,29 @@ int solve_cg(const CsrView& A, const float* b, float* x, double atol, double tol
   float* q = w.s;
   int e = set_tolerances(w.st, atol, tol, s);
   if (e) return e;
-  const unsigned g = grid_for(n), rg = reduce_grid(n);
+  const unsigned g = grid_for(n), G = krylov_grid(n);
   WCFD_LAUNCH(K_INV_DIAG, s, k_inv_diag, g, n, A.values, A.diag_pos, w.inv);
-  WCFD_LAUNCH(K_CG_INIT, s, k_cg_init, rg, n, A.row_ptr, A.col, A.values, b, x, w.r, w.inv, w.partials, w.st);
+  WCFD_LAUNCH(K_CG_INIT, s, k_cg_init, G, n, A.row_ptr, A.col, A.values, b, x, w.r, w.inv, w.PA);
+  WCFD_LAUNCH(K_FOLD, s, k_init_fold, 1, w.PA, (int)G, w.st, 1);
   if ((e = check_launch())) return e;
   KState h;
   if ((e = read_state(w.st, &h, s))) return e;
   int it = 0;
-  if (!(h.rr > h.thr2) || maxiter <= 0) {  // converged already (or NaN: reported by caller)
+  if (!(h.rr > h.thr2) || maxiter <= 0) {  // converged already (or NaN: reported below)
     fill_info(info, 0, h);
     return isfinite(h.rr) ? WCFD_OK : WCFD_ENAN;
   }
   float* pb[2] = {w.p0, w.p1};
-  int cur = 0;
   const int ce = check_every > 0 ? check_every : 1;
   while (true) {
-    for (int k = 0; k < ce; ++k) {
-      WCFD_LAUNCH(K_CG_1, s, k_cg_1, rg, n, A.row_ptr, A.col, A.values, w.r, w.inv, pb[cur], pb[1 - cur],
-                                   q, w.partials, w.st);
-      WCFD_LAUNCH(K_CG_2, s, k_cg_2, rg, n, x, w.r, w.inv, pb[1 - cur], q, w.partials, w.st);
-      cur = 1 - cur;
+    for (int k = 0; k < ce; ++k, ++it) {
+      const int cur = it & 1;
+      WCFD_LAUNCH(K_CG_1, s, k_cg_1, G, n, it, A.row_ptr, A.col, A.values, w.r, w.inv, pb[cur],
+                  pb[1 - cur], q, w.PA, w.PB, w.st);
+      WCFD_LAUNCH(K_CG_2, s, k_cg_2, G, n, it, x, w.r, w.inv, pb[1 - cur], q, w.PB, w.PA, w.st);
     }
+    WCFD_LAUNCH(K_FOLD, s, k_fold_to, 1, w.PA + G, (int)G, &w.st->rr);
     if ((e = check_launch())) return e;
-    it += ce;
     if ((e = read_state(w.st, &h, s))) return e;
     if (!isfinite(h.rr)) {
       fill_info(info, it, h);
@@ -397,10 +478,11 @@ static int bicg_impl(const CsrView& A, const float* b, float* x, double atol, do
   KWork w = carve_krylov(workspace, n, NV);
   int e = set_tolerances(w.st, atol, tol, s);
   if (e) return e;
-  const unsigned g = grid_for(n), rg = reduce_grid(n);
+  const unsigned g = grid_for(n), G = krylov_grid(n);
   WCFD_LAUNCH(K_INV_DIAG, s, k_inv_diag, g, n, A.values, A.diag_pos, w.inv);
-  WCFD_LAUNCH(K_BI_INIT, s, k_bi_init<NV>, rg, n, A.row_ptr, A.col, A.values, b, x, w.r, w.r0, w.partials,
-                                       w.st);
+  WCFD_LAUNCH(K_BI_INIT, s, k_bi_init<NV>, G, n, A.row_ptr, A.col, A.values, b, x, w.r, w.r0,
+              w.PA);
+  WCFD_LAUNCH(K_FOLD, s, k_init_fold, 1, w.PA, (int)G, w.st, 0);
   if ((e = check_launch())) return e;
   KState h;
   if ((e = read_state(w.st, &h, s))) return e;
@@ -411,20 +493,19 @@ static int bicg_impl(const CsrView& A, const float* b, float* x, double atol, do
   }
   float* pb[2] = {w.p0, w.p1};
   float* vb[2] = {w.v0, w.v1};
-  int cur = 0;
   const int ce = check_every > 0 ? check_every : 1;
   while (true) {
-    for (int k = 0; k < ce; ++k) {
-      WCFD_LAUNCH(K_BI_1, s, k_bi_1<NV>, rg, n, A.row_ptr, A.col, A.values, w.r, w.inv, pb[cur], vb[cur],
-                                        pb[1 - cur], vb[1 - cur], w.r0, w.partials, w.st);
-      WCFD_LAUNCH(K_BI_2, s, k_bi_2<NV>, rg, n, A.row_ptr, A.col, A.values, w.r, w.inv, vb[1 - cur], w.s,
-                                        w.t, w.partials, w.st);
-      WCFD_LAUNCH(K_BI_3, s, k_bi_3<NV>, rg, n, x, w.r, w.inv, pb[1 - cur], w.s, w.t, w.r0, w.partials,
-                                        w.st);
-      cur = 1 - cur;
+    for (int k = 0; k < ce; ++k, ++it) {
+      const int cur = it & 1;
+      WCFD_LAUNCH(K_BI_1, s, k_bi_1<NV>, G, n, it, A.row_ptr, A.col, A.values, w.r, w.inv, pb[cur],
+                  vb[cur], pb[1 - cur], vb[1 - cur], w.r0, w.PA, w.PB, w.st);
+      WCFD_LAUNCH(K_BI_2, s, k_bi_2<NV>, G, n, it, A.row_ptr, A.col, A.values, w.r, w.inv,
+                  vb[1 - cur], w.s, w.t, w.PB, w.PC, w.st);
+      WCFD_LAUNCH(K_BI_3, s, k_bi_3<NV>, G, n, it, x, w.r, w.inv, pb[1 - cur], w.s, w.t, w.r0, w.PC,
+                  w.PA, w.st);
     }
+    WCFD_LAUNCH(K_FOLD, s, k_fold_to, 1, w.PA + G, (int)G, &w.st->rr);
     if ((e = check_launch())) return e;
-    it += ce;
     if ((e = read_state(w.st, &h, s))) return e;
     if (!isfinite(h.rr)) {
       fill_info(info, it, h);
@@ -467,9 +548,11 @@ int wcfd_dot_f64(int32_t n, const float* a, const float* b, double* out, void* w
                  void* stream) {
   if (n <= 0 || !a || !b || !out || !workspace) return WCFD_EINVAL;
   Carver c(workspace);
-  double* partials = c.take<double>(2 * kMaxReduceBlocks);
-  unsigned* ticket = c.take<unsigned>(64);
-  WCFD_LAUNCH(K_DOT, (hipStream_t)stream, k_dot, reduce_grid(n), n, a, b, partials, ticket, out);
+  double* part = c.take<double>(kMaxReduceBlocks);
+  hipStream_t s = (hipStream_t)stream;
+  const unsigned G = reduce_grid(n);
+  WCFD_LAUNCH(K_DOT, s, k_dot, G, n, a, b, part);
+  WCFD_LAUNCH(K_FOLD, s, k_fold_to, 1, part, (int)G, out);
   return check_launch();
 }
 

@@ -18,7 +18,6 @@ namespace wcfd {
 struct SimpleWork {
   void* krylov;
   double* partials;
-  unsigned* ticket;
   float* fc;
 };
 
@@ -27,7 +26,6 @@ static SimpleWork carve_simple(void* ws, int32_t C, int32_t F) {
   SimpleWork w;
   w.krylov = c.take<char>(krylov_workspace_bytes(C, 3));
   w.partials = c.take<double>(2 * kMaxReduceBlocks);
-  w.ticket = c.take<unsigned>(64);
   w.fc = c.take<float>(F);
   return w;
 }
@@ -43,7 +41,6 @@ size_t wcfd_simple_workspace_bytes(int32_t n_cells, int32_t n_faces) {
   Carver c(nullptr);
   c.take<char>(krylov_workspace_bytes(n_cells, 3));
   c.take<double>(2 * kMaxReduceBlocks);
-  c.take<unsigned>(64);
   c.take<float>(n_faces);
   return c.off;
 }
@@ -98,7 +95,7 @@ int wcfd_simple_iteration(const wcfd_mesh* mp, wcfd_bc bc, const wcfd_simple_par
       if ((e = launch_relax_p(m, st->p_x, prm->alpha_p, st->cell_values, s))) return e;
       if ((e = launch_correct_velocity(m, bc, st->cell_values, st->cell_grad, st->face_values,
                                        st->mass_flux, st->rua, st->rua_face, st->hbya, st->resid,
-                                       w.fc, w.partials, w.ticket, s)))
+                                       w.fc, w.partials, s)))
         return e;
     }
   }
