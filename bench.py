@@ -66,12 +66,14 @@ def algorithmic_bytes(kernel, C, F, Fi, Fb, nnz):
     output element written once (float32 / int32 = 4 B)."""
     rp = 4 * (C + 1)
     table = {
-        # q = A (z + beta p) gathered on the fly, p_out, q; reads r, inv, p_in
-        'cg_1': 8 * nnz + rp + 12 * C + 8 * C,
+        # slot-major (ELL) matrix: diagonal + 6 coefficients (28 B) + int16
+        # neighbour offsets (12 B) per row; q = A (z + beta p) gathered on the
+        # fly; reads r, inv, p_in (12 B), writes p_out, q (8 B)
+        'cg_1': 40 * C + 12 * C + 8 * C,
         # x += a p, r -= a q: reads x p r q inv, writes x r
         'cg_2': 28 * C,
-        'bicgstab_1': 8 * nnz + rp + 4 * C + 36 * C + 24 * C + 12 * C,
-        'bicgstab_2': 8 * nnz + rp + 4 * C + 24 * C + 24 * C,
+        'bicgstab_1': 40 * C + 4 * C + 36 * C + 24 * C + 12 * C,
+        'bicgstab_2': 40 * C + 4 * C + 24 * C + 24 * C,
         'bicgstab_3': 64 * C + 24 * C,
         # owner/neigh, psi, skew, normal, area, dmag, bc; cell values + grads; face values + flux
         'face_interp': 4 * F + 4 * Fi + 4 * Fi + 12 * Fi + 12 * F + 4 * F + 4 * Fb + 20 * Fb
@@ -128,6 +130,9 @@ def main():
     ap.add_argument('--workload', default='channel', choices=sorted(WORKLOADS))
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-profile', action='store_true')
+    ap.add_argument('--no-tune', action='store_true')
+    ap.add_argument('--krylov-launch', default=None,
+                    help='GRIDxBLOCK: fixed Krylov launch shape instead of autotuning')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py runs on one GPU only: this build has no multi-GPU path '
@@ -154,6 +159,16 @@ def main():
     model.finalize()
     solver = IncompressibleSolver(model, 0.7, 0.3, implicit_outflow=(wl['kind'] == 'channel'))
     dm = model.dm
+    tuned = None
+    if args.krylov_launch:
+        g, b = (int(v) for v in args.krylov_launch.lower().split('x'))
+        _lib.check(_lib.load().wcfd_set_krylov_launch(g, b), 'set_krylov_launch')
+        tuned = {'grid': g, 'block': b, 'source': 'command line'}
+    elif not args.no_tune:
+        from warpcfd_amd.tune import autotune_krylov
+        t = autotune_krylov(dm)
+        tuned = {'grid': t['grid'], 'block': t['block'], 'cg_ms_per_60_it': round(t['cg_ms'], 3),
+                 'candidates': len(t['candidates'])}
     C, F, Fi, Fb, nnz = dm.n_cells, dm.n_faces, dm.n_internal, dm.n_boundary, dm.nnz
 
     for _ in range(args.warmup):
@@ -211,6 +226,7 @@ def main():
         'continuity_residual': hist[-1]['continuity'],
         'krylov_iters_per_step': {'momentum': float(np.mean([h['u_iters'] for h in hist])),
                                   'pressure': float(np.mean([h['p_iters'] for h in hist]))},
+        'krylov_launch': tuned,
         'kernels': kernels,
     }
     print(json.dumps(out))

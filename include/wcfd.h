@@ -12,9 +12,9 @@
  *  - Every function returns WCFD_OK or an error code; on WCFD_EHIP the HIP
  *    error is kept per thread and readable with wcfd_last_hip_error().
  *  - `stream` is a hipStream_t passed as void* (0 = the null stream).
- *  - Functions keep no global mutable state (the opt-in profiler below
- *    aside); calls on different streams with different workspaces may run
- *    concurrently.
+ *  - Functions keep no global mutable state (the opt-in profiler and the
+ *    Krylov launch shape below aside); calls on different streams with
+ *    different workspaces may run concurrently.
  *
  * Each entry point cites the reference function it replaces
  * (paths relative to the JohnCSu/WarpCFD repository root).
@@ -66,7 +66,15 @@ typedef struct wcfd_mesh {
   const int32_t* diag_pos;  /* [C]    CSR index of (c, c) */
   const int32_t* row_ptr;   /* [C+1] */
   const int32_t* col;       /* [nnz]  columns sorted within a row */
+  const int32_t* slot_nbr;  /* [6][C] neighbour in slot k (c itself at boundary slots) */
+  const int16_t* slot_dnb;  /* [6][C] slot_nbr - c, or NULL when the band exceeds int16 */
 } wcfd_mesh;
+
+/* Matrices of the device path (the assembly outputs and the *_ell solvers)
+ * are stored slot-major, "ELL": values[c] is the diagonal of row c and
+ * values[(1+k)*C + c] the coefficient of (c, slot_nbr[k][c]), 0 at boundary
+ * slots -- 7*C floats.  wcfd_ell_to_csr() writes the same matrix on the CSR
+ * pattern (row_ptr, col), which is the reference's pattern (bit-exact). */
 
 /* Boundary conditions per boundary face (warp_cfd/FV/boundary/boundary.py). */
 typedef struct wcfd_bc {
@@ -122,7 +130,7 @@ int wcfd_divergence(const wcfd_mesh* m, const float* mass_flux, float* div, void
  * BC type for u, v, w on every face); three right-hand sides rhs[3][C].
  * If compute_gradients != 0 the Green-Gauss gradients of all 4 fields are
  * computed first (model_kernels.py:72-90) into cell_grad and used for the p
- * term.  scheme_flags: WCFD_SCHEME_* bits. */
+ * term.  scheme_flags: WCFD_SCHEME_* bits.  values: [7][C] ELL. */
 int wcfd_assemble_momentum(const wcfd_mesh* m, wcfd_bc bc, const float* cell_values,
                            const float* face_values, const float* mass_flux,
                            float* cell_grad, int compute_gradients, float nu,
@@ -134,15 +142,19 @@ int wcfd_assemble_momentum(const wcfd_mesh* m, wcfd_bc bc, const float* cell_val
  * (equation.py:99-124,169-170) -> replace_row (equation_kernels.py:155-175)
  * when ref_cell >= 0.  If compute_grad_p != 0 the Green-Gauss gradient of p
  * is computed into cell_grad (other gradients untouched).  div_out may be
- * NULL. */
+ * NULL.  values: [7][C] ELL. */
 int wcfd_assemble_pressure(const wcfd_mesh* m, wcfd_bc bc, const float* face_values,
                            const float* mass_flux, const float* rua_face, float* cell_grad,
                            int compute_grad_p, int correction, int ref_cell, float ref_value,
                            float* values, float* rhs, float* div_out, void* stream);
 
+/* ELL values [7][C] -> CSR values [nnz] on (row_ptr, col). */
+int wcfd_ell_to_csr(const wcfd_mesh* m, const float* ell_values, float* csr_values, void* stream);
+
 /* calculate_rUA + get_HbyA + replace_cell_values (solver_kernels.py:7-16,
  * 39-43; model_kernels.py:104-109): rUA = V / (a_PP rho), HbyA += rUA grad p,
- * cell_values[u,v,w] = HbyA.  HbyA is [3][C]. */
+ * cell_values[u,v,w] = HbyA.  HbyA is [3][C]; values is the momentum ELL
+ * matrix (its diagonal a_PP). */
 int wcfd_rua_hbya(const wcfd_mesh* m, const float* values, float density,
                   const float* cell_grad, float* hbya, float* cell_values, float* rua,
                   void* stream);
@@ -177,6 +189,10 @@ int wcfd_spmv(int32_t n, const int32_t* row_ptr, const int32_t* col, const float
 int wcfd_dot_f64(int32_t n, const float* a, const float* b, double* out, void* workspace,
                  void* stream);
 
+/* y[q] = A x[q] for the ELL matrix `values` [7][C] of mesh m. */
+int wcfd_spmv_ell(const wcfd_mesh* m, const float* values, int nvec, const float* x, float* y,
+                  void* stream);
+
 size_t wcfd_krylov_workspace_bytes(int32_t n, int nvec);
 
 /* Result of a Krylov solve (host struct). */
@@ -205,6 +221,22 @@ int wcfd_bicgstab(int32_t n, const int32_t* row_ptr, const int32_t* col, const f
                   const int32_t* diag_pos, int nvec, const float* b, float* x, double atol,
                   double tol, int maxiter, int check_every, void* workspace,
                   wcfd_solve_info* info, void* stream);
+
+/* Launch shape of the Krylov kernels: `grid` workgroups (grid-stride,
+ * 1..2048; every workgroup of a kernel folds the previous kernel's `grid`
+ * partials) of `block` threads (64..1024, multiple of 64).  Process-wide;
+ * chosen by the autotuner (warpcfd_amd/tune.py).  No counterpart in the
+ * reference (Warp picks its own launch shapes). */
+int wcfd_set_krylov_launch(int grid, int block);
+int wcfd_get_krylov_launch(int* grid, int* block); /* [host] outputs */
+
+/* wcfd_cg / wcfd_bicgstab on the ELL matrix `values` [7][C] of mesh m. */
+int wcfd_cg_ell(const wcfd_mesh* m, const float* values, const float* b, float* x, double atol,
+                double tol, int maxiter, int check_every, void* workspace, wcfd_solve_info* info,
+                void* stream);
+int wcfd_bicgstab_ell(const wcfd_mesh* m, const float* values, int nvec, const float* b, float* x,
+                      double atol, double tol, int maxiter, int check_every, void* workspace,
+                      wcfd_solve_info* info, void* stream);
 
 /* ------------------------------------------------------ whole SIMPLE step */
 
@@ -235,9 +267,9 @@ typedef struct wcfd_simple_state {
   float* p_x;          /* [C] pressure solution (warm start) */
   float* rua;          /* [C] */
   float* rua_face;     /* [F] */
-  float* mom_values;   /* [nnz] */
+  float* mom_values;   /* [7][C] ELL */
   float* mom_rhs;      /* [3][C] */
-  float* p_values;     /* [nnz] */
+  float* p_values;     /* [7][C] ELL */
   float* p_rhs;        /* [C] */
   double* resid;       /* [2] continuity numerator / denominator */
 } wcfd_simple_state;

@@ -44,11 +44,18 @@ def _pair(kind='channel', n=(9, 7, 5), jitter=0.1, seed=1):
 
 
 def _dev_matrix_ref(model, values):
-    """Device CSR (device order) -> scipy CSR in reference cell order."""
+    """Device ELL matrix (device order) -> scipy CSR in reference cell order,
+    converted on the GPU by wcfd_ell_to_csr and checked against the host
+    conversion."""
+    torch = _torch()
+    from warpcfd_amd.FV.device import ptr, stream_ptr
     L = model.layout
     rp = L.row_ptr.astype(np.int64)
     rows = np.repeat(np.arange(L.num_cells), np.diff(rp))
-    v = values.cpu().numpy()
+    csr = torch.zeros(L.nnz, dtype=torch.float32, device=values.device)
+    assert model.lib.wcfd_ell_to_csr(model.dm.desc_ref, ptr(values), ptr(csr), stream_ptr()) == 0
+    v = csr.cpu().numpy()
+    assert np.array_equal(v, L.ell_to_csr(values.cpu().numpy()))
     r = L.new2old[rows]
     c = L.new2old[L.col]
     return sp.csr_matrix((v, (r, c)), shape=(L.num_cells, L.num_cells))
@@ -96,7 +103,7 @@ def test_momentum_assembly(gpu, correction, outflow):
     A_ref = eq.matrix().tocsr()[0::3, 0::3].toarray()
     C = of.m.num_cells
     dev = model._cv.device
-    vals = torch.zeros(model.dm.nnz, dtype=torch.float32, device=dev)
+    vals = torch.zeros(7, C, dtype=torch.float32, device=dev)
     rhs = torch.zeros(3, C, dtype=torch.float32, device=dev)
     from warpcfd_amd.FV.device import ptr, stream_ptr
     code = model.lib.wcfd_assemble_momentum(model.dm.desc_ref, model.bc, ptr(model._cv), ptr(model._fv),
@@ -128,7 +135,7 @@ def test_transient_momentum_assembly(gpu):
                    explicit=[(grad_term_weights(of), -1.0)])
     C = of.m.num_cells
     dev = model._cv.device
-    vals = torch.zeros(model.dm.nnz, dtype=torch.float32, device=dev)
+    vals = torch.zeros(7, C, dtype=torch.float32, device=dev)
     rhs = torch.zeros(3, C, dtype=torch.float32, device=dev)
     from warpcfd_amd.FV.device import ptr, stream_ptr
     code = model.lib.wcfd_assemble_momentum(model.dm.desc_ref, model.bc, ptr(model._cv), ptr(model._fv),
@@ -160,7 +167,7 @@ def test_pressure_assembly(gpu, kind, ref_cell):
         eq.replace_row(ref_cell, 0.0)
     dev = model._cv.device
     C = of.m.num_cells
-    vals = torch.zeros(model.dm.nnz, dtype=torch.float32, device=dev)
+    vals = torch.zeros(7, C, dtype=torch.float32, device=dev)
     rhs = torch.zeros(C, dtype=torch.float32, device=dev)
     rua_dev = model.dm.faces_to_device(torch.from_numpy(rua_f).to(dev)).contiguous()
     from warpcfd_amd.FV.device import ptr, stream_ptr
@@ -189,6 +196,60 @@ def _spd_system(model, seed):
     A = (A + sp.diags(d)).tocsr()
     A.sort_indices()
     return A.astype(np.float32)
+
+
+def _csr_to_ell(L, data):
+    ell = np.zeros((7, L.num_cells), np.float32)
+    ell[0] = data[L.diag_pos]
+    for k in range(6):
+        ok = L.slot_pos[k] >= 0
+        ell[k + 1][ok] = data[L.slot_pos[k][ok]]
+    return ell
+
+
+def test_ell_solvers_match_csr(gpu):
+    """The device path's slot-major matrices give the same SpMV and solves as
+    the CSR entry points (int16 neighbour offsets)."""
+    torch = _torch()
+    from warpcfd_amd._lib import SolveInfo
+    from warpcfd_amd.FV.device import ptr, stream_ptr
+    nodes, cells = jittered_box((14, 10, 8), 0.1, 31)
+    model = device_case(nodes, cells, 'channel')
+    L = model.layout
+    assert L.slot_dnb is not None
+    A = _spd_system(model, 7)
+    n = L.num_cells
+    dev = model._cv.device
+    lib = model.lib
+    ell = torch.from_numpy(_csr_to_ell(L, A.data)).to(dev)
+    csr = torch.from_numpy(A.data).to(dev)
+    rng = np.random.default_rng(8)
+    x = torch.from_numpy(rng.standard_normal((3, n)).astype(np.float32)).to(dev)
+    y1 = torch.zeros(3, n, dtype=torch.float32, device=dev)
+    y2 = torch.zeros(3, n, dtype=torch.float32, device=dev)
+    assert lib.wcfd_spmv_ell(model.dm.desc_ref, ptr(ell), 3, ptr(x), ptr(y1), stream_ptr()) == 0
+    assert lib.wcfd_spmv(n, ptr(model.dm.row_ptr), ptr(model.dm.col), ptr(csr), 3, ptr(x), ptr(y2),
+                         stream_ptr()) == 0
+    assert rel_l2(y1.cpu().numpy(), y2.cpu().numpy()) <= 1e-6
+    b = torch.from_numpy(rng.standard_normal(n).astype(np.float32)).to(dev)
+    ws = torch.zeros(lib.wcfd_krylov_workspace_bytes(n, 3), dtype=torch.uint8, device=dev)
+    xa = torch.zeros(n, dtype=torch.float32, device=dev)
+    xb = torch.zeros(n, dtype=torch.float32, device=dev)
+    ia, ib = SolveInfo(), SolveInfo()
+    assert lib.wcfd_cg_ell(model.dm.desc_ref, ptr(ell), ptr(b), ptr(xa), 1e-6, 1e-6, 1000, 10, ptr(ws),
+                           ctypes.byref(ia), stream_ptr()) == 0
+    assert lib.wcfd_cg(n, ptr(model.dm.row_ptr), ptr(model.dm.col), ptr(csr), ptr(model.dm.diag_pos), ptr(b),
+                       ptr(xb), 1e-6, 1e-6, 1000, 10, ptr(ws), ctypes.byref(ib), stream_ptr()) == 0
+    assert ia.iterations == ib.iterations and ia.converged
+    assert rel_l2(xa.cpu().numpy(), xb.cpu().numpy()) <= 1e-5
+    b3 = torch.from_numpy(rng.standard_normal((3, n)).astype(np.float32)).to(dev)
+    xa3 = torch.zeros(3, n, dtype=torch.float32, device=dev)
+    negell = -ell
+    assert lib.wcfd_bicgstab_ell(model.dm.desc_ref, ptr(negell), 3, ptr(b3), ptr(xa3), 1e-6, 1e-6, 1000, 10,
+                                 ptr(ws), ctypes.byref(ia), stream_ptr()) == 0
+    xs = np.stack([spla.spsolve((-A).astype(np.float64).tocsc(), b3[q].cpu().numpy().astype(np.float64))
+                   for q in range(3)])
+    assert ia.converged and rel_l2(xa3.cpu().numpy(), xs) <= 1e-5
 
 
 def test_spmv_and_cg(gpu):

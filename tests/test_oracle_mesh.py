@@ -160,6 +160,21 @@ def test_device_layout_consistency(renumber):
         ok = L.slot_pos[k] >= 0
         assert np.array_equal(rows[L.slot_pos[k][ok]], np.arange(C)[ok])
         assert np.array_equal(L.col[L.slot_pos[k][ok]], L.slot_nbr[k][ok])
+    # ELL neighbour tables: boundary slots point at the row itself, int16
+    # offsets equal the int32 neighbours, ELL <-> CSR conversion is lossless
+    for k in range(6):
+        bnd = L.slot_pos[k] < 0
+        assert np.array_equal(L.slot_nbr[k][bnd], np.arange(C)[bnd])
+    assert L.slot_dnb is not None
+    assert np.array_equal(L.slot_dnb.astype(np.int64) + np.arange(C), L.slot_nbr)
+    rng = np.random.default_rng(3)
+    csr = rng.standard_normal(L.nnz).astype(np.float32)
+    ell = np.zeros((7, C), np.float32)
+    ell[0] = csr[L.diag_pos]
+    for k in range(6):
+        ok = L.slot_pos[k] >= 0
+        ell[k + 1][ok] = csr[L.slot_pos[k][ok]]
+    assert np.array_equal(L.ell_to_csr(ell), csr)
     # rows sorted and a symmetric pattern
     for c in range(C):
         seg = L.col[L.row_ptr[c]:L.row_ptr[c + 1]]

@@ -50,6 +50,8 @@ class DeviceMesh:
         self.diag_pos = up(L.diag_pos, i32)
         self.row_ptr = up(L.row_ptr, i32)
         self.col = up(L.col, i32)
+        self.slot_nbr = up(L.slot_nbr, i32)
+        self.slot_dnb = up(L.slot_dnb, np.int16) if L.slot_dnb is not None else None
         self.old2new = torch.from_numpy(L.old2new).to(self.device)
         self.new2old = torch.from_numpy(L.new2old).to(self.device)
         self.face_old2new = torch.from_numpy(L.face_old2new).to(self.device)
@@ -60,8 +62,10 @@ class DeviceMesh:
         d.n_cells, d.n_faces, d.n_internal = L.num_cells, L.num_faces, L.n_internal
         d.n_boundary, d.nnz, d.reserved = L.n_boundary, L.nnz, 0
         for name in ('owner', 'neigh', 'psi', 'skew', 'area', 'normal', 'dist', 'dmag', 'volume',
-                     'slot_face', 'slot_side', 'slot_sn', 'slot_pos', 'diag_pos', 'row_ptr', 'col'):
+                     'slot_face', 'slot_side', 'slot_sn', 'slot_pos', 'diag_pos', 'row_ptr', 'col',
+                     'slot_nbr'):
             setattr(d, name, getattr(self, name).data_ptr())
+        d.slot_dnb = self.slot_dnb.data_ptr() if self.slot_dnb is not None else None
         self.desc = d
         self.desc_ref = ctypes.byref(d)
 

@@ -44,9 +44,9 @@ class IncompressibleSolver:
         self.p_x = z(C)
         self.rUA = z(C)
         self.rUA_faces = z(F)
-        self.mom_values = z(nnz)
+        self.mom_values = z(7, C)      # slot-major (ELL) matrices
         self.mom_rhs = z(3, C)
-        self.p_values = z(nnz)
+        self.p_values = z(7, C)
         self.p_rhs = z(C)
         self.resid = torch.zeros(2, dtype=torch.float64, device=dev)
         self.workspace = torch.zeros(self.lib.wcfd_simple_workspace_bytes(C, F), dtype=torch.uint8,

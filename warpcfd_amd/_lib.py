@@ -28,7 +28,8 @@ class WcfdMesh(ctypes.Structure):
                 ('owner', c_p), ('neigh', c_p), ('psi', c_p), ('skew', c_p), ('area', c_p),
                 ('normal', c_p), ('dist', c_p), ('dmag', c_p), ('volume', c_p),
                 ('slot_face', c_p), ('slot_side', c_p), ('slot_sn', c_p), ('slot_pos', c_p),
-                ('diag_pos', c_p), ('row_ptr', c_p), ('col', c_p)]
+                ('diag_pos', c_p), ('row_ptr', c_p), ('col', c_p), ('slot_nbr', c_p),
+                ('slot_dnb', c_p)]
 
 
 class WcfdBC(ctypes.Structure):
@@ -76,6 +77,12 @@ _SIGS = {
                                               c_f, c_f, c_f, ctypes.c_int, c_p, c_p, c_p]),
     'wcfd_assemble_pressure': (ctypes.c_int, [_PM, WcfdBC, c_p, c_p, c_p, c_p, ctypes.c_int,
                                               ctypes.c_int, ctypes.c_int, c_f, c_p, c_p, c_p, c_p]),
+    'wcfd_ell_to_csr': (ctypes.c_int, [_PM, c_p, c_p, c_p]),
+    'wcfd_spmv_ell': (ctypes.c_int, [_PM, c_p, ctypes.c_int, c_p, c_p, c_p]),
+    'wcfd_cg_ell': (ctypes.c_int, [_PM, c_p, c_p, c_p, c_d, c_d, ctypes.c_int, ctypes.c_int, c_p,
+                                   ctypes.POINTER(SolveInfo), c_p]),
+    'wcfd_bicgstab_ell': (ctypes.c_int, [_PM, c_p, ctypes.c_int, c_p, c_p, c_d, c_d, ctypes.c_int,
+                                         ctypes.c_int, c_p, ctypes.POINTER(SolveInfo), c_p]),
     'wcfd_rua_hbya': (ctypes.c_int, [_PM, c_p, c_f, c_p, c_p, c_p, c_p, c_p]),
     'wcfd_relax_pressure': (ctypes.c_int, [_PM, c_p, c_f, c_p, c_p]),
     'wcfd_correct_velocity': (ctypes.c_int, [_PM, WcfdBC, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p,
@@ -89,6 +96,8 @@ _SIGS = {
     'wcfd_bicgstab': (ctypes.c_int, [c_i32, c_p, c_p, c_p, c_p, ctypes.c_int, c_p, c_p, c_d, c_d,
                                      ctypes.c_int, ctypes.c_int, c_p, ctypes.POINTER(SolveInfo),
                                      c_p]),
+    'wcfd_set_krylov_launch': (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    'wcfd_get_krylov_launch': (ctypes.c_int, [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     'wcfd_simple_workspace_bytes': (ctypes.c_size_t, [c_i32, c_i32]),
     'wcfd_kernel_count': (ctypes.c_int, []),
     'wcfd_kernel_name': (ctypes.c_char_p, [ctypes.c_int]),

@@ -39,6 +39,12 @@ int solve_cg(const CsrView& A, const float* b, float* x, double atol, double tol
 int solve_bicgstab(const CsrView& A, int nvec, const float* b, float* x, double atol, double tol,
                    int maxiter, int check_every, void* workspace, wcfd_solve_info* info,
                    hipStream_t s);
+int solve_cg_ell(const wcfd_mesh& m, const float* values, const float* b, float* x, double atol,
+                 double tol, int maxiter, int check_every, void* workspace, wcfd_solve_info* info,
+                 hipStream_t s);
+int solve_bicgstab_ell(const wcfd_mesh& m, const float* values, int nvec, const float* b, float* x,
+                       double atol, double tol, int maxiter, int check_every, void* workspace,
+                       wcfd_solve_info* info, hipStream_t s);
 size_t krylov_workspace_bytes(int32_t n, int nvec);
 
 }  // namespace wcfd

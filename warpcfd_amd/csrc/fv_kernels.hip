@@ -15,7 +15,7 @@ thread_local int g_last_hip_error = 0;
 // ------------------------------------------------------------------ BCs
 __global__ void k_set_bc(int F, int Fi, int Fb, const uint8_t* __restrict__ type,
                          const float* __restrict__ value, float* __restrict__ fv, int mask) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = xcd_item();
   if (j >= Fb) return;
   int f = Fi + j;
 #pragma unroll
@@ -32,7 +32,7 @@ __global__ void k_face_interp(wcfd_mesh m, wcfd_bc bc, const float* __restrict__
                               int mask, int skew_mask, float* __restrict__ flux,
                               const float* __restrict__ rua, float* __restrict__ rua_f,
                               int apply_dirichlet) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = xcd_item();
   const int C = m.n_cells, F = m.n_faces, Fi = m.n_internal, Fb = m.n_boundary;
   if (f >= F) return;
   float val[4];
@@ -106,7 +106,7 @@ __device__ inline void gg_field(const wcfd_mesh& m, int c, const int (&fk)[6],
 
 __global__ void k_green_gauss(wcfd_mesh m, const float* __restrict__ fv, float* __restrict__ cg,
                               int mask, int zero_others) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = xcd_item();
   const int C = m.n_cells, F = m.n_faces;
   if (c >= C) return;
   int fk[6];
@@ -130,7 +130,7 @@ __global__ void k_green_gauss(wcfd_mesh m, const float* __restrict__ fv, float* 
 }
 
 __global__ void k_mass_flux(wcfd_mesh m, const float* __restrict__ fv, float* __restrict__ flux) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = xcd_item();
   const int F = m.n_faces;
   if (f >= F) return;
   flux[f] = (fv[f] * m.normal[f] + fv[F + f] * m.normal[F + f] + fv[2 * F + f] * m.normal[2 * F + f]) *
@@ -150,7 +150,7 @@ __device__ inline float cell_div(const wcfd_mesh& m, int c, const int (&fk)[6],
 }
 
 __global__ void k_divergence(wcfd_mesh m, const float* __restrict__ flux, float* __restrict__ div) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = xcd_item();
   const int C = m.n_cells;
   if (c >= C) return;
   int fk[6];
@@ -182,7 +182,7 @@ __global__ void k_assemble_momentum(wcfd_mesh m, wcfd_bc bc, const float* __rest
                                     float* __restrict__ values, float* __restrict__ rhs) {
   const int correction = flags & WCFD_SCHEME_CORRECTED;
   const bool implicit_outflow = (flags & WCFD_SCHEME_IMPLICIT_OUTFLOW) != 0;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = xcd_item();
   const int C = m.n_cells, F = m.n_faces, Fi = m.n_internal, Fb = m.n_boundary;
   if (c >= C) return;
   int fk[6];
@@ -287,12 +287,10 @@ __global__ void k_assemble_momentum(wcfd_mesh m, wcfd_bc bc, const float* __rest
     for (int q = 0; q < 3; ++q) b[q] = b[q] + r * diag * cv[q * C + c];
     diag = diag / alpha;
   }
-  values[m.diag_pos[c]] = diag;
+  // slot-major (ELL) matrix: diagonal, then the 6 slot coefficients
+  values[c] = diag;
 #pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    const int p = m.slot_pos[k * C + c];
-    if (p >= 0) values[p] = off[k];
-  }
+  for (int k = 0; k < 6; ++k) values[(k + 1) * C + c] = off[k];
 #pragma unroll
   for (int q = 0; q < 3; ++q) rhs[q * C + c] = b[q];
 }
@@ -304,7 +302,7 @@ __global__ void k_assemble_pressure(wcfd_mesh m, wcfd_bc bc, const float* __rest
                                     int compute_grad_p, int correction, int ref_cell,
                                     float ref_value, float* __restrict__ values,
                                     float* __restrict__ rhs, float* __restrict__ div_out) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = xcd_item();
   const int C = m.n_cells, F = m.n_faces, Fi = m.n_internal, Fb = m.n_boundary;
   if (c >= C) return;
   int fk[6];
@@ -347,12 +345,9 @@ __global__ void k_assemble_pressure(wcfd_mesh m, wcfd_bc bc, const float* __rest
     for (int k = 0; k < 6; ++k) off[k] = 0.f;
     b = ref_value;
   }
-  values[m.diag_pos[c]] = diag;
+  values[c] = diag;
 #pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    const int p = m.slot_pos[k * C + c];
-    if (p >= 0) values[p] = off[k];
-  }
+  for (int k = 0; k < 6; ++k) values[(k + 1) * C + c] = off[k];
   rhs[c] = b;
 }
 
@@ -360,10 +355,10 @@ __global__ void k_assemble_pressure(wcfd_mesh m, wcfd_bc bc, const float* __rest
 __global__ void k_rua_hbya(wcfd_mesh m, const float* __restrict__ values, float density,
                            const float* __restrict__ cg, float* __restrict__ hbya,
                            float* __restrict__ cv, float* __restrict__ rua) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = xcd_item();
   const int C = m.n_cells;
   if (c >= C) return;
-  const float r = m.volume[c] / (values[m.diag_pos[c]] * density);
+  const float r = m.volume[c] / (values[c] * density);
   rua[c] = r;
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
@@ -374,7 +369,7 @@ __global__ void k_rua_hbya(wcfd_mesh m, const float* __restrict__ values, float 
 }
 
 __global__ void k_relax_p(int C, const float* __restrict__ pn, float alpha, float* __restrict__ cv) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = xcd_item();
   if (c >= C) return;
   float* p = cv + (size_t)3 * C;
   p[c] = alpha > 0.f ? alpha * pn[c] + (1.0f - alpha) * p[c] : pn[c];
@@ -385,7 +380,7 @@ __global__ void k_face_p_fc(wcfd_mesh m, wcfd_bc bc, const float* __restrict__ c
                             const float* __restrict__ cg, float* __restrict__ fv,
                             const float* __restrict__ flux, const float* __restrict__ rua_f,
                             float* __restrict__ fc) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = xcd_item();
   const int C = m.n_cells, F = m.n_faces, Fi = m.n_internal, Fb = m.n_boundary;
   if (f >= F) return;
   const float* p = cv + (size_t)3 * C;
@@ -424,7 +419,8 @@ __global__ void k_grad_p_correct(wcfd_mesh m, const float* __restrict__ fv,
                                  const float* __restrict__ fc, double* __restrict__ partials) {
   const int C = m.n_cells, F = m.n_faces;
   double acc[2] = {0.0, 0.0};
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
+  const RowRange rr_ = my_rows(C);
+  for (int c = rr_.begin + (int)threadIdx.x; c < rr_.end; c += blockDim.x) {
     int fk[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) fk[k] = m.slot_face[k * C + c];
@@ -456,18 +452,18 @@ __global__ void k_grad_p_correct(wcfd_mesh m, const float* __restrict__ fv,
   }
   if (!fc) return;
   // per-workgroup partials, folded by k_fold_resid (no cross-workgroup wait)
-  __shared__ double lds[2 * (kBlock / kWave)];
+  __shared__ double lds[2 * kMaxWaves];
   const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
     const double s = wave_sum(acc[i]);
-    if (lane == 0) lds[i * (kBlock / kWave) + wid] = s;
+    if (lane == 0) lds[i * kMaxWaves + wid] = s;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int i = 0; i < 2; ++i) {
       double t = 0.0;
-      for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += lds[i * (kBlock / kWave) + w];
+      for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += lds[i * kMaxWaves + w];
       partials[i * gridDim.x + blockIdx.x] = t;
     }
   }
@@ -475,7 +471,7 @@ __global__ void k_grad_p_correct(wcfd_mesh m, const float* __restrict__ fv,
 
 // one workgroup: fold the continuity partials in block order
 __global__ void k_fold_resid(const double* __restrict__ partials, int G, double* __restrict__ resid) {
-  __shared__ double lds[2 * (kBlock / kWave)];
+  __shared__ double lds[2 * kMaxWaves];
   double acc[2] = {0.0, 0.0};
   for (int b = threadIdx.x; b < G; b += blockDim.x) {
     acc[0] += partials[b];
@@ -484,15 +480,28 @@ __global__ void k_fold_resid(const double* __restrict__ partials, int G, double*
   const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
   for (int i = 0; i < 2; ++i) {
     const double s = wave_sum(acc[i]);
-    if (lane == 0) lds[i * (kBlock / kWave) + wid] = s;
+    if (lane == 0) lds[i * kMaxWaves + wid] = s;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int i = 0; i < 2; ++i) {
       double t = 0.0;
-      for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += lds[i * (kBlock / kWave) + w];
+      for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += lds[i * kMaxWaves + w];
       resid[i] = t;
     }
+  }
+}
+
+// ELL (slot-major) values -> values on the CSR pattern (row_ptr, col).
+__global__ void k_ell_to_csr(wcfd_mesh m, const float* __restrict__ ell, float* __restrict__ csr) {
+  const int c = xcd_item();
+  const int C = m.n_cells;
+  if (c >= C) return;
+  csr[m.diag_pos[c]] = ell[c];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int p = m.slot_pos[k * C + c];
+    if (p >= 0) csr[p] = ell[(k + 1) * C + c];
   }
 }
 
@@ -657,6 +666,13 @@ int wcfd_relax_pressure(const wcfd_mesh* m, const float* p_new, float alpha, flo
                         void* stream) {
   if (!mesh_ok(m) || !p_new || !cell_values) return WCFD_EINVAL;
   return launch_relax_p(*m, p_new, alpha, cell_values, (hipStream_t)stream);
+}
+
+int wcfd_ell_to_csr(const wcfd_mesh* m, const float* ell_values, float* csr_values, void* stream) {
+  if (!mesh_ok(m) || !m->slot_pos || !m->diag_pos || !ell_values || !csr_values) return WCFD_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  WCFD_LAUNCH(K_ELL_TO_CSR, s, k_ell_to_csr, grid_for(m->n_cells), *m, ell_values, csr_values);
+  return check_launch();
 }
 
 size_t wcfd_reduce_workspace_bytes(int32_t n_faces) {

@@ -16,15 +16,26 @@
 
 #include <math.h>
 
+#include <atomic>
+
 namespace wcfd {
 
 // Scalars of one solve.  Quantities that change every iteration sit in
 // two-slot rings indexed by iteration parity: block 0 of a kernel writes the
 // slot of the current iteration while the other blocks of the same kernel
 // read the previous one.
+// Flags the host polls (copied to pinned memory after every check).
+struct KFlags {
+  int32_t done;   // set once converged, non-finite or out of iterations
+  int32_t iters;  // iterations run when done was set
+  double rr;      // ||r||^2 at the last check
+  double thr2;    // squared stopping threshold
+};
+
 struct KState {
+  KFlags f;
   double rz[2], rho[2], alpha[2], omega[2];
-  double thr2, atol, tol, rr, bb, ss;
+  double atol, tol, bb, ss;
 };
 
 __device__ inline double safe_div(double a, double b) { return b != 0.0 ? a / b : 0.0; }
@@ -36,7 +47,7 @@ __device__ inline double safe_div(double a, double b) { return b != 0.0 ? a / b 
 // release (an L2 write-back on every XCD) per workgroup, which measured
 // 5-6x slower than this scheme on MI355X at 320 k rows.
 __device__ inline double fold(const double* __restrict__ part, int G) {
-  __shared__ double lds[kBlock / kWave];
+  __shared__ double lds[kMaxWaves];
   __shared__ double res;
   double s = 0.0;
   for (int b = threadIdx.x; b < G; b += blockDim.x) s += part[b];
@@ -57,47 +68,43 @@ __device__ inline double fold(const double* __restrict__ part, int G) {
 // part[i * G + blockIdx.x].
 template <int ND>
 __device__ inline void store_partials(double (&v)[ND], double* __restrict__ part, int G) {
-  __shared__ double lds[ND * (kBlock / kWave)];
+  __shared__ double lds[ND * kMaxWaves];
   const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
 #pragma unroll
   for (int i = 0; i < ND; ++i) {
     double s = wave_sum(v[i]);
-    if (lane == 0) lds[i * (kBlock / kWave) + wid] = s;
+    if (lane == 0) lds[i * kMaxWaves + wid] = s;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int i = 0; i < ND; ++i) {
       double t = 0.0;
-      for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += lds[i * (kBlock / kWave) + w];
+      for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += lds[i * kMaxWaves + w];
       part[i * G + blockIdx.x] = t;
     }
   }
 }
 
-__global__ void k_inv_diag(int n, const float* __restrict__ values, const int32_t* __restrict__ dpos,
-                           float* __restrict__ inv) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+template <class M>
+__global__ void k_inv_diag(M A, int n, float* __restrict__ inv) {
+  const int i = xcd_item();
   if (i >= n) return;
-  const float d = values[dpos[i]];
+  const float d = A.diag(i);
   inv[i] = d != 0.0f ? 1.0f / d : 1.0f;
 }
 
-template <int NV>
-__global__ void k_spmv(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                       const float* __restrict__ val, const float* __restrict__ x,
-                       float* __restrict__ y) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+template <class M, int NV>
+__global__ void k_spmv(M A, int n, const float* __restrict__ x, float* __restrict__ y) {
+  const int i = xcd_item();
   if (i >= n) return;
   float acc[NV];
 #pragma unroll
   for (int q = 0; q < NV; ++q) acc[q] = 0.f;
-  for (int j = rp[i]; j < rp[i + 1]; ++j) {
-    const float a = val[j];
-    const int c = col[j];
+  A.row(i, [&](float a, int c) {
 #pragma unroll
     for (int q = 0; q < NV; ++q) acc[q] += a * x[(size_t)q * n + c];
-  }
+  });
 #pragma unroll
   for (int q = 0; q < NV; ++q) y[(size_t)q * n + i] = acc[q];
 }
@@ -105,7 +112,8 @@ __global__ void k_spmv(int n, const int32_t* __restrict__ rp, const int32_t* __r
 __global__ void k_dot(int n, const float* __restrict__ a, const float* __restrict__ b,
                       double* __restrict__ part) {
   double acc[1] = {0.0};
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+  const RowRange rr_ = my_rows(n);
+  for (int i = rr_.begin + (int)threadIdx.x; i < rr_.end; i += blockDim.x)
     acc[0] += (double)a[i] * (double)b[i];
   store_partials<1>(acc, part, gridDim.x);
 }
@@ -117,14 +125,15 @@ __global__ void k_fold_to(const double* __restrict__ part, int G, double* out) {
 
 // ------------------------------------------------------------------ CG
 // partial slots: PA[3][G] = {r.z, r.r, b.b} (init, k_cg_2), PB[1][G] = {p.q} (k_cg_1)
-__global__ void k_cg_init(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                          const float* __restrict__ val, const float* __restrict__ b,
-                          const float* __restrict__ x, float* __restrict__ r,
-                          const float* __restrict__ inv, double* __restrict__ PA) {
+template <class M>
+__global__ void k_cg_init(M A, int n, const float* __restrict__ b, const float* __restrict__ x,
+                          float* __restrict__ r, const float* __restrict__ inv,
+                          double* __restrict__ PA) {
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  const RowRange rr_ = my_rows(n);
+  for (int i = rr_.begin + (int)threadIdx.x; i < rr_.end; i += blockDim.x) {
     float ax = 0.f;
-    for (int j = rp[i]; j < rp[i + 1]; ++j) ax += val[j] * x[col[j]];
+    A.row(i, [&](float a, int c) { ax += a * x[c]; });
     const float ri = b[i] - ax;
     r[i] = ri;
     const float zi = ri * inv[i];
@@ -135,25 +144,45 @@ __global__ void k_cg_init(int n, const int32_t* __restrict__ rp, const int32_t* 
   store_partials<3>(acc, PA, gridDim.x);
 }
 
-// one workgroup: fold the init partials, set the threshold and rz[0]
-__global__ void k_init_fold(const double* __restrict__ PA, int G, KState* st, int is_cg) {
+// one workgroup: fold the init partials, set the threshold and rz[0] / rho[0]
+__global__ void k_init_fold(const double* __restrict__ PA, int G, KState* st, int is_cg,
+                            int maxiter) {
   const double a = fold(PA, G), rr = fold(PA + G, G), bb = fold(PA + 2 * G, G);
   if (threadIdx.x == 0) {
     if (is_cg) st->rz[0] = a; else st->rho[0] = a;
-    st->rr = rr;
     st->bb = bb;
     const double thr = fmax(st->tol * sqrt(bb), st->atol);
-    st->thr2 = thr * thr;
+    st->f.rr = rr;
+    st->f.thr2 = thr * thr;
+    st->f.iters = 0;
+    // converged already, non-finite, or no iterations allowed
+    st->f.done = (!(rr > thr * thr) || maxiter <= 0) ? 1 : 0;
     st->alpha[0] = st->alpha[1] = 0.0;
     st->omega[0] = st->omega[1] = 0.0;
   }
 }
 
-__global__ void k_cg_1(int n, int it, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                       const float* __restrict__ val, const float* __restrict__ r,
+// one workgroup, after every check_every iterations: fold ||r||^2 and set
+// the done flag the iteration kernels test at entry
+__global__ void k_check(const double* __restrict__ part, int G, KState* st, int it_end,
+                        int maxiter) {
+  if (st->f.done) return;
+  const double rr = fold(part, G);
+  if (threadIdx.x == 0) {
+    st->f.rr = rr;
+    if (!(rr > st->f.thr2) || it_end >= maxiter) {
+      st->f.done = 1;
+      st->f.iters = it_end;
+    }
+  }
+}
+
+template <class M>
+__global__ void k_cg_1(M A, int n, int it, const float* __restrict__ r,
                        const float* __restrict__ inv, const float* __restrict__ p_in,
                        float* __restrict__ p_out, float* __restrict__ q,
                        const double* __restrict__ PA, double* __restrict__ PB, KState* st) {
+  if (st->f.done) return;
   const int G = gridDim.x;
   bool first = (it == 0);
   double beta_d = 0.0;
@@ -165,14 +194,14 @@ __global__ void k_cg_1(int n, int it, const int32_t* __restrict__ rp, const int3
   }
   const float beta = (float)beta_d;
   double acc[1] = {0.0};
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  const RowRange rr_ = my_rows(n);
+  for (int i = rr_.begin + (int)threadIdx.x; i < rr_.end; i += blockDim.x) {
     float qi = 0.f;
-    for (int j = rp[i]; j < rp[i + 1]; ++j) {
-      const int c = col[j];
+    A.row(i, [&](float a, int c) {
       const float z = r[c] * inv[c];
       const float pc = first ? z : z + beta * p_in[c];
-      qi += val[j] * pc;
-    }
+      qi += a * pc;
+    });
     const float zi = r[i] * inv[i];
     const float pi = first ? zi : zi + beta * p_in[i];
     p_out[i] = pi;
@@ -186,12 +215,13 @@ __global__ void k_cg_2(int n, int it, float* __restrict__ x, float* __restrict__
                        const float* __restrict__ inv, const float* __restrict__ p,
                        const float* __restrict__ q, const double* __restrict__ PB,
                        double* __restrict__ PA, KState* st) {
+  if (st->f.done) return;
   const int G = gridDim.x;
   const double pq = fold(PB, G);
-  const double rz = (it == 0) ? st->rz[0] : st->rz[it & 1];
-  const float alpha = (float)safe_div(rz, pq);
+  const float alpha = (float)safe_div(st->rz[it & 1], pq);
   double acc[2] = {0.0, 0.0};
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  const RowRange rr_ = my_rows(n);
+  for (int i = rr_.begin + (int)threadIdx.x; i < rr_.end; i += blockDim.x) {
     x[i] = x[i] + alpha * p[i];
     const float ri = r[i] - alpha * q[i];
     r[i] = ri;
@@ -205,22 +235,19 @@ __global__ void k_cg_2(int n, int it, float* __restrict__ x, float* __restrict__
 // ------------------------------------------------------------- BiCGSTAB
 // partial slots: PA[3][G] = {r0.r, r.r, b.b} (init, k_bi_3), PB[1][G] = {r0.v}
 // (k_bi_1), PC[3][G] = {t.s, t.t, s.s} (k_bi_2)
-template <int NV>
-__global__ void k_bi_init(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                          const float* __restrict__ val, const float* __restrict__ b,
-                          const float* __restrict__ x, float* __restrict__ r,
-                          float* __restrict__ r0, double* __restrict__ PA) {
+template <class M, int NV>
+__global__ void k_bi_init(M A, int n, const float* __restrict__ b, const float* __restrict__ x,
+                          float* __restrict__ r, float* __restrict__ r0, double* __restrict__ PA) {
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  const RowRange rr_ = my_rows(n);
+  for (int i = rr_.begin + (int)threadIdx.x; i < rr_.end; i += blockDim.x) {
     float ax[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) ax[q] = 0.f;
-    for (int j = rp[i]; j < rp[i + 1]; ++j) {
-      const float a = val[j];
-      const int c = col[j];
+    A.row(i, [&](float a, int c) {
 #pragma unroll
       for (int q = 0; q < NV; ++q) ax[q] += a * x[(size_t)q * n + c];
-    }
+    });
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
       const size_t k = (size_t)q * n + i;
@@ -236,14 +263,13 @@ __global__ void k_bi_init(int n, const int32_t* __restrict__ rp, const int32_t* 
   store_partials<3>(acc, PA, gridDim.x);
 }
 
-template <int NV>
-__global__ void k_bi_1(int n, int it, const int32_t* __restrict__ rp,
-                       const int32_t* __restrict__ col, const float* __restrict__ val,
-                       const float* __restrict__ r, const float* __restrict__ inv,
-                       const float* __restrict__ p_in, const float* __restrict__ v_in,
-                       float* __restrict__ p_out, float* __restrict__ v_out,
-                       const float* __restrict__ r0, const double* __restrict__ PA,
-                       double* __restrict__ PB, KState* st) {
+template <class M, int NV>
+__global__ void k_bi_1(M A, int n, int it, const float* __restrict__ r,
+                       const float* __restrict__ inv, const float* __restrict__ p_in,
+                       const float* __restrict__ v_in, float* __restrict__ p_out,
+                       float* __restrict__ v_out, const float* __restrict__ r0,
+                       const double* __restrict__ PA, double* __restrict__ PB, KState* st) {
+  if (st->f.done) return;
   const int G = gridDim.x;
   bool first = (it == 0);
   double beta_d = 0.0, omega_d = 0.0;
@@ -258,13 +284,12 @@ __global__ void k_bi_1(int n, int it, const int32_t* __restrict__ rp,
   const float beta = (float)beta_d;
   const float omega = (float)omega_d;
   double acc[1] = {0.0};
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  const RowRange rr_ = my_rows(n);
+  for (int i = rr_.begin + (int)threadIdx.x; i < rr_.end; i += blockDim.x) {
     float v[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) v[q] = 0.f;
-    for (int j = rp[i]; j < rp[i + 1]; ++j) {
-      const float a = val[j];
-      const int c = col[j];
+    A.row(i, [&](float a, int c) {
       const float ic = inv[c];
 #pragma unroll
       for (int q = 0; q < NV; ++q) {
@@ -272,7 +297,7 @@ __global__ void k_bi_1(int n, int it, const int32_t* __restrict__ rp,
         const float pc = first ? r[k] : r[k] + beta * (p_in[k] - omega * v_in[k]);
         v[q] += a * (pc * ic);
       }
-    }
+    });
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
       const size_t k = (size_t)q * n + i;
@@ -285,25 +310,24 @@ __global__ void k_bi_1(int n, int it, const int32_t* __restrict__ rp,
   store_partials<1>(acc, PB, G);
 }
 
-template <int NV>
-__global__ void k_bi_2(int n, int it, const int32_t* __restrict__ rp,
-                       const int32_t* __restrict__ col, const float* __restrict__ val,
-                       const float* __restrict__ r, const float* __restrict__ inv,
-                       const float* __restrict__ v, float* __restrict__ s, float* __restrict__ t,
+template <class M, int NV>
+__global__ void k_bi_2(M A, int n, int it, const float* __restrict__ r,
+                       const float* __restrict__ inv, const float* __restrict__ v,
+                       float* __restrict__ s, float* __restrict__ t,
                        const double* __restrict__ PB, double* __restrict__ PC, KState* st) {
+  if (st->f.done) return;
   const int G = gridDim.x;
   const double r0v = fold(PB, G);
   const double alpha_d = safe_div(st->rho[it & 1], r0v);
   if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha[it & 1] = alpha_d;
   const float alpha = (float)alpha_d;
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  const RowRange rr_ = my_rows(n);
+  for (int i = rr_.begin + (int)threadIdx.x; i < rr_.end; i += blockDim.x) {
     float tv[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) tv[q] = 0.f;
-    for (int j = rp[i]; j < rp[i + 1]; ++j) {
-      const float a = val[j];
-      const int c = col[j];
+    A.row(i, [&](float a, int c) {
       const float ic = inv[c];
 #pragma unroll
       for (int q = 0; q < NV; ++q) {
@@ -311,7 +335,7 @@ __global__ void k_bi_2(int n, int it, const int32_t* __restrict__ rp,
         const float sc = r[k] - alpha * v[k];
         tv[q] += a * (sc * ic);
       }
-    }
+    });
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
       const size_t k = (size_t)q * n + i;
@@ -332,14 +356,16 @@ __global__ void k_bi_3(int n, int it, float* __restrict__ x, float* __restrict__
                        const float* __restrict__ s, const float* __restrict__ t,
                        const float* __restrict__ r0, const double* __restrict__ PC,
                        double* __restrict__ PA, KState* st) {
+  if (st->f.done) return;
   const int G = gridDim.x;
   const double ts = fold(PC, G), tt = fold(PC + G, G), ss = fold(PC + 2 * G, G);
-  const double omega_d = ss > st->thr2 ? safe_div(ts, tt) : 0.0;
+  const double omega_d = ss > st->f.thr2 ? safe_div(ts, tt) : 0.0;
   if (blockIdx.x == 0 && threadIdx.x == 0) { st->omega[it & 1] = omega_d; st->ss = ss; }
   const float alpha = (float)st->alpha[it & 1];
   const float omega = (float)omega_d;
   double acc[2] = {0.0, 0.0};
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  const RowRange rr_ = my_rows(n);
+  for (int i = rr_.begin + (int)threadIdx.x; i < rr_.end; i += blockDim.x) {
     const float ii = inv[i];
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
@@ -398,12 +424,6 @@ size_t krylov_workspace_bytes(int32_t n, int nvec) {
   return c.off;
 }
 
-static int read_state(KState* dev, KState* host, hipStream_t s) {
-  int e = check_hip(hipMemcpyAsync(host, dev, sizeof(KState), hipMemcpyDeviceToHost, s));
-  if (e) return e;
-  return check_hip(hipStreamSynchronize(s));
-}
-
 static int set_tolerances(KState* dev, double atol, double tol, hipStream_t s) {
   double t[2] = {atol, tol};
   int e = check_hip(hipMemcpyAsync(&dev->atol, t, sizeof(t), hipMemcpyHostToDevice, s));
@@ -412,117 +432,212 @@ static int set_tolerances(KState* dev, double atol, double tol, hipStream_t s) {
   return check_hip(hipStreamSynchronize(s));
 }
 
-static void fill_info(wcfd_solve_info* info, int it, const KState& h) {
+static void fill_info(wcfd_solve_info* info, const KFlags& h) {
   if (!info) return;
-  info->iterations = it;
+  info->iterations = h.iters;
   info->converged = h.rr <= h.thr2 ? 1 : 0;
   info->residual = sqrt(h.rr);
   info->threshold = sqrt(h.thr2);
 }
 
-// workgroups of the reduction kernels: a grid-stride grid, at most
-// kMaxReduceBlocks (the partials each workgroup folds in its prologue)
-static unsigned krylov_grid(int64_t n) {
-  int64_t g = (n + kBlock - 1) / kBlock;
-  const int64_t cap = kKrylovGrid;
+// Pinned host slots and events for polling the done flag without draining
+// the stream (one set per host thread, created on first use).
+constexpr int kSlots = 4;
+struct HostSync {
+  KFlags* slot = nullptr;
+  hipEvent_t ev[kSlots] = {};
+  hipEvent_t tail = nullptr;
+  bool tail_pending = false;
+};
+
+static HostSync* host_sync() {
+  static thread_local HostSync hs;
+  static thread_local bool ready = false;
+  if (!ready) {
+    if (check_hip(hipHostMalloc((void**)&hs.slot, sizeof(KFlags) * kSlots, hipHostMallocDefault)))
+      return nullptr;
+    for (auto& e : hs.ev)
+      if (check_hip(hipEventCreateWithFlags(&e, hipEventDisableTiming))) return nullptr;
+    if (check_hip(hipEventCreateWithFlags(&hs.tail, hipEventDisableTiming))) return nullptr;
+    ready = true;
+  }
+  return &hs;
+}
+
+// Runs the iteration groups of one solve.  Each group is `ce` iterations
+// followed by k_check; its flags are copied to a pinned slot and an event is
+// recorded.  The host keeps `depth` groups in flight and stops launching as
+// soon as a polled group reports done; groups already queued find the flag
+// set and exit at entry, so the result is the same as checking after every
+// group synchronously (the reference solver's check interval).
+template <class LaunchGroup>
+static int run_groups(KState* st, const double* rr_part, unsigned G, int maxiter, int ce,
+                      LaunchGroup&& launch_group, wcfd_solve_info* info, hipStream_t s) {
+  HostSync* hs = host_sync();
+  if (!hs) return WCFD_EHIP;
+  int e;
+  if (hs->tail_pending) {   // copies of an earlier solve may still target the slots
+    if ((e = check_hip(hipEventSynchronize(hs->tail)))) return e;
+    hs->tail_pending = false;
+  }
+  // flags after init (synchronous: one round trip per solve)
+  if ((e = check_hip(hipMemcpyAsync(&hs->slot[0], &st->f, sizeof(KFlags), hipMemcpyDeviceToHost, s))))
+    return e;
+  if ((e = check_hip(hipStreamSynchronize(s)))) return e;
+  KFlags last = hs->slot[0];
+  if (last.done) {
+    fill_info(info, last);
+    return isfinite(last.rr) ? WCFD_OK : WCFD_ENAN;
+  }
+  const int depth = 2;
+  int launched = 0, polled = 0, it = 0;
+  const int cap = maxiter;
+  while (true) {
+    if (it < cap && launched - polled < depth) {
+      launch_group(it);
+      it += ce;
+      WCFD_LAUNCH(K_FOLD, s, k_check, 1, rr_part, (int)G, st, it, cap);
+      const int sl = launched % kSlots;
+      if ((e = check_launch())) return e;
+      if ((e = check_hip(hipMemcpyAsync(&hs->slot[sl], &st->f, sizeof(KFlags),
+                                        hipMemcpyDeviceToHost, s))))
+        return e;
+      if ((e = check_hip(hipEventRecord(hs->ev[sl], s)))) return e;
+      ++launched;
+      continue;
+    }
+    if (polled == launched) break;   // nothing left (cannot happen: the last check sets done)
+    const int sl = polled % kSlots;
+    if ((e = check_hip(hipEventSynchronize(hs->ev[sl])))) return e;
+    last = hs->slot[sl];
+    ++polled;
+    if (last.done) break;
+  }
+  if (launched > polled) {
+    if ((e = check_hip(hipEventRecord(hs->tail, s)))) return e;
+    hs->tail_pending = true;
+  }
+  fill_info(info, last);
+  return isfinite(last.rr) ? WCFD_OK : WCFD_ENAN;
+}
+
+namespace {
+std::atomic<int> g_krylov_grid{kKrylovGrid};
+std::atomic<int> g_krylov_block{kBlock};
+}  // namespace
+
+KrylovLaunch krylov_launch() { return {g_krylov_grid.load(), g_krylov_block.load()}; }
+
+// workgroups of the reduction kernels: a grid-stride grid of at most the
+// tuned size (every workgroup of the next kernel folds that many partials)
+static unsigned krylov_grid(int64_t n, int block) {
+  int64_t g = (n + block - 1) / block;
+  const int64_t cap = krylov_launch().grid;
   if (g > cap) g = cap;
   return static_cast<unsigned>(g < 1 ? 1 : g);
 }
 
-int solve_cg(const CsrView& A, const float* b, float* x, double atol, double tol, int maxiter,
-             int check_every, void* workspace, wcfd_solve_info* info, hipStream_t s) {
-  const int n = A.n;
+template <class M>
+static int cg_impl(const M& A, int n, const float* b, float* x, double atol, double tol, int maxiter,
+                   int check_every, void* workspace, wcfd_solve_info* info, hipStream_t s) {
   KWork w = carve_krylov(workspace, n, 1);
   float* q = w.s;
   int e = set_tolerances(w.st, atol, tol, s);
   if (e) return e;
-  const unsigned g = grid_for(n), G = krylov_grid(n);
-  WCFD_LAUNCH(K_INV_DIAG, s, k_inv_diag, g, n, A.values, A.diag_pos, w.inv);
-  WCFD_LAUNCH(K_CG_INIT, s, k_cg_init, G, n, A.row_ptr, A.col, A.values, b, x, w.r, w.inv, w.PA);
-  WCFD_LAUNCH(K_FOLD, s, k_init_fold, 1, w.PA, (int)G, w.st, 1);
+  const int B = krylov_launch().block;
+  const unsigned g = grid_for(n), G = krylov_grid(n, B);
+  WCFD_LAUNCH(K_INV_DIAG, s, k_inv_diag<M>, g, A, n, w.inv);
+  WCFD_LAUNCH_B(K_CG_INIT, s, k_cg_init<M>, G, B, A, n, b, x, w.r, w.inv, w.PA);
+  WCFD_LAUNCH(K_FOLD, s, k_init_fold, 1, w.PA, (int)G, w.st, 1, maxiter);
   if ((e = check_launch())) return e;
-  KState h;
-  if ((e = read_state(w.st, &h, s))) return e;
-  int it = 0;
-  if (!(h.rr > h.thr2) || maxiter <= 0) {  // converged already (or NaN: reported below)
-    fill_info(info, 0, h);
-    return isfinite(h.rr) ? WCFD_OK : WCFD_ENAN;
-  }
   float* pb[2] = {w.p0, w.p1};
   const int ce = check_every > 0 ? check_every : 1;
-  while (true) {
-    for (int k = 0; k < ce; ++k, ++it) {
+  auto group = [&](int it0) {
+    for (int it = it0; it < it0 + ce; ++it) {
       const int cur = it & 1;
-      WCFD_LAUNCH(K_CG_1, s, k_cg_1, G, n, it, A.row_ptr, A.col, A.values, w.r, w.inv, pb[cur],
-                  pb[1 - cur], q, w.PA, w.PB, w.st);
-      WCFD_LAUNCH(K_CG_2, s, k_cg_2, G, n, it, x, w.r, w.inv, pb[1 - cur], q, w.PB, w.PA, w.st);
+      WCFD_LAUNCH_B(K_CG_1, s, k_cg_1<M>, G, B, A, n, it, w.r, w.inv, pb[cur], pb[1 - cur], q,
+                    w.PA, w.PB, w.st);
+      WCFD_LAUNCH_B(K_CG_2, s, k_cg_2, G, B, n, it, x, w.r, w.inv, pb[1 - cur], q, w.PB, w.PA,
+                    w.st);
     }
-    WCFD_LAUNCH(K_FOLD, s, k_fold_to, 1, w.PA + G, (int)G, &w.st->rr);
-    if ((e = check_launch())) return e;
-    if ((e = read_state(w.st, &h, s))) return e;
-    if (!isfinite(h.rr)) {
-      fill_info(info, it, h);
-      return WCFD_ENAN;
-    }
-    if (h.rr <= h.thr2 || it >= maxiter) break;
-  }
-  fill_info(info, it, h);
-  return WCFD_OK;
+  };
+  return run_groups(w.st, w.PA + G, G, maxiter, ce, group, info, s);
 }
 
-template <int NV>
-static int bicg_impl(const CsrView& A, const float* b, float* x, double atol, double tol,
+template <class M, int NV>
+static int bicg_impl(const M& A, int n, const float* b, float* x, double atol, double tol,
                      int maxiter, int check_every, void* workspace, wcfd_solve_info* info,
                      hipStream_t s) {
-  const int n = A.n;
   KWork w = carve_krylov(workspace, n, NV);
   int e = set_tolerances(w.st, atol, tol, s);
   if (e) return e;
-  const unsigned g = grid_for(n), G = krylov_grid(n);
-  WCFD_LAUNCH(K_INV_DIAG, s, k_inv_diag, g, n, A.values, A.diag_pos, w.inv);
-  WCFD_LAUNCH(K_BI_INIT, s, k_bi_init<NV>, G, n, A.row_ptr, A.col, A.values, b, x, w.r, w.r0,
-              w.PA);
-  WCFD_LAUNCH(K_FOLD, s, k_init_fold, 1, w.PA, (int)G, w.st, 0);
+  const int B = krylov_launch().block;
+  const unsigned g = grid_for(n), G = krylov_grid(n, B);
+  WCFD_LAUNCH(K_INV_DIAG, s, k_inv_diag<M>, g, A, n, w.inv);
+  WCFD_LAUNCH_B(K_BI_INIT, s, (k_bi_init<M, NV>), G, B, A, n, b, x, w.r, w.r0, w.PA);
+  WCFD_LAUNCH(K_FOLD, s, k_init_fold, 1, w.PA, (int)G, w.st, 0, maxiter);
   if ((e = check_launch())) return e;
-  KState h;
-  if ((e = read_state(w.st, &h, s))) return e;
-  int it = 0;
-  if (!(h.rr > h.thr2) || maxiter <= 0) {
-    fill_info(info, 0, h);
-    return isfinite(h.rr) ? WCFD_OK : WCFD_ENAN;
-  }
   float* pb[2] = {w.p0, w.p1};
   float* vb[2] = {w.v0, w.v1};
   const int ce = check_every > 0 ? check_every : 1;
-  while (true) {
-    for (int k = 0; k < ce; ++k, ++it) {
+  auto group = [&](int it0) {
+    for (int it = it0; it < it0 + ce; ++it) {
       const int cur = it & 1;
-      WCFD_LAUNCH(K_BI_1, s, k_bi_1<NV>, G, n, it, A.row_ptr, A.col, A.values, w.r, w.inv, pb[cur],
-                  vb[cur], pb[1 - cur], vb[1 - cur], w.r0, w.PA, w.PB, w.st);
-      WCFD_LAUNCH(K_BI_2, s, k_bi_2<NV>, G, n, it, A.row_ptr, A.col, A.values, w.r, w.inv,
-                  vb[1 - cur], w.s, w.t, w.PB, w.PC, w.st);
-      WCFD_LAUNCH(K_BI_3, s, k_bi_3<NV>, G, n, it, x, w.r, w.inv, pb[1 - cur], w.s, w.t, w.r0, w.PC,
-                  w.PA, w.st);
+      WCFD_LAUNCH_B(K_BI_1, s, (k_bi_1<M, NV>), G, B, A, n, it, w.r, w.inv, pb[cur], vb[cur],
+                    pb[1 - cur], vb[1 - cur], w.r0, w.PA, w.PB, w.st);
+      WCFD_LAUNCH_B(K_BI_2, s, (k_bi_2<M, NV>), G, B, A, n, it, w.r, w.inv, vb[1 - cur], w.s, w.t,
+                    w.PB, w.PC, w.st);
+      WCFD_LAUNCH_B(K_BI_3, s, k_bi_3<NV>, G, B, n, it, x, w.r, w.inv, pb[1 - cur], w.s, w.t, w.r0,
+                    w.PC, w.PA, w.st);
     }
-    WCFD_LAUNCH(K_FOLD, s, k_fold_to, 1, w.PA + G, (int)G, &w.st->rr);
-    if ((e = check_launch())) return e;
-    if ((e = read_state(w.st, &h, s))) return e;
-    if (!isfinite(h.rr)) {
-      fill_info(info, it, h);
-      return WCFD_ENAN;
-    }
-    if (h.rr <= h.thr2 || it >= maxiter) break;
-  }
-  fill_info(info, it, h);
-  return WCFD_OK;
+  };
+  return run_groups(w.st, w.PA + G, G, maxiter, ce, group, info, s);
+}
+
+template <class M>
+static int bicg_dispatch(const M& A, int n, int nvec, const float* b, float* x, double atol,
+                         double tol, int maxiter, int check_every, void* ws,
+                         wcfd_solve_info* info, hipStream_t s) {
+  if (nvec == 1) return bicg_impl<M, 1>(A, n, b, x, atol, tol, maxiter, check_every, ws, info, s);
+  if (nvec == 3) return bicg_impl<M, 3>(A, n, b, x, atol, tol, maxiter, check_every, ws, info, s);
+  return WCFD_EINVAL;
+}
+
+int solve_cg(const CsrView& A, const float* b, float* x, double atol, double tol, int maxiter,
+             int check_every, void* workspace, wcfd_solve_info* info, hipStream_t s) {
+  const CsrMat M{A.n, A.row_ptr, A.col, A.values, A.diag_pos};
+  return cg_impl(M, A.n, b, x, atol, tol, maxiter, check_every, workspace, info, s);
 }
 
 int solve_bicgstab(const CsrView& A, int nvec, const float* b, float* x, double atol, double tol,
                    int maxiter, int check_every, void* workspace, wcfd_solve_info* info,
                    hipStream_t s) {
-  if (nvec == 1) return bicg_impl<1>(A, b, x, atol, tol, maxiter, check_every, workspace, info, s);
-  if (nvec == 3) return bicg_impl<3>(A, b, x, atol, tol, maxiter, check_every, workspace, info, s);
-  return WCFD_EINVAL;
+  const CsrMat M{A.n, A.row_ptr, A.col, A.values, A.diag_pos};
+  return bicg_dispatch(M, A.n, nvec, b, x, atol, tol, maxiter, check_every, workspace, info, s);
+}
+
+int solve_cg_ell(const wcfd_mesh& m, const float* values, const float* b, float* x, double atol,
+                 double tol, int maxiter, int check_every, void* workspace, wcfd_solve_info* info,
+                 hipStream_t s) {
+  const int n = m.n_cells;
+  if (m.slot_dnb) {
+    const EllMat<true> M{n, values, m.slot_nbr, m.slot_dnb};
+    return cg_impl(M, n, b, x, atol, tol, maxiter, check_every, workspace, info, s);
+  }
+  const EllMat<false> M{n, values, m.slot_nbr, m.slot_dnb};
+  return cg_impl(M, n, b, x, atol, tol, maxiter, check_every, workspace, info, s);
+}
+
+int solve_bicgstab_ell(const wcfd_mesh& m, const float* values, int nvec, const float* b, float* x,
+                       double atol, double tol, int maxiter, int check_every, void* workspace,
+                       wcfd_solve_info* info, hipStream_t s) {
+  const int n = m.n_cells;
+  if (m.slot_dnb) {
+    const EllMat<true> M{n, values, m.slot_nbr, m.slot_dnb};
+    return bicg_dispatch(M, n, nvec, b, x, atol, tol, maxiter, check_every, workspace, info, s);
+  }
+  const EllMat<false> M{n, values, m.slot_nbr, m.slot_dnb};
+  return bicg_dispatch(M, n, nvec, b, x, atol, tol, maxiter, check_every, workspace, info, s);
 }
 
 }  // namespace wcfd
@@ -535,12 +650,31 @@ int wcfd_spmv(int32_t n, const int32_t* row_ptr, const int32_t* col, const float
               const float* x, float* y, void* stream) {
   if (n <= 0 || !row_ptr || !col || !values || !x || !y) return WCFD_EINVAL;
   hipStream_t s = (hipStream_t)stream;
+  const CsrMat A{n, row_ptr, col, values, nullptr};
   if (nvec == 1)
-    WCFD_LAUNCH(K_SPMV, s, k_spmv<1>, grid_for(n), n, row_ptr, col, values, x, y);
+    WCFD_LAUNCH(K_SPMV, s, (k_spmv<CsrMat, 1>), grid_for(n), A, n, x, y);
   else if (nvec == 3)
-    WCFD_LAUNCH(K_SPMV, s, k_spmv<3>, grid_for(n), n, row_ptr, col, values, x, y);
+    WCFD_LAUNCH(K_SPMV, s, (k_spmv<CsrMat, 3>), grid_for(n), A, n, x, y);
   else
     return WCFD_EINVAL;
+  return check_launch();
+}
+
+int wcfd_spmv_ell(const wcfd_mesh* m, const float* values, int nvec, const float* x, float* y,
+                  void* stream) {
+  if (!m || m->n_cells <= 0 || !m->slot_nbr || !values || !x || !y) return WCFD_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  const int n = m->n_cells;
+  if (nvec != 1 && nvec != 3) return WCFD_EINVAL;
+  if (m->slot_dnb) {
+    const EllMat<true> A{n, values, m->slot_nbr, m->slot_dnb};
+    if (nvec == 1) WCFD_LAUNCH(K_SPMV, s, (k_spmv<EllMat<true>, 1>), grid_for(n), A, n, x, y);
+    else WCFD_LAUNCH(K_SPMV, s, (k_spmv<EllMat<true>, 3>), grid_for(n), A, n, x, y);
+  } else {
+    const EllMat<false> A{n, values, m->slot_nbr, m->slot_dnb};
+    if (nvec == 1) WCFD_LAUNCH(K_SPMV, s, (k_spmv<EllMat<false>, 1>), grid_for(n), A, n, x, y);
+    else WCFD_LAUNCH(K_SPMV, s, (k_spmv<EllMat<false>, 3>), grid_for(n), A, n, x, y);
+  }
   return check_launch();
 }
 
@@ -556,6 +690,23 @@ int wcfd_dot_f64(int32_t n, const float* a, const float* b, double* out, void* w
   return check_launch();
 }
 
+int wcfd_set_krylov_launch(int grid, int block) {
+  if (grid < 1 || grid > kMaxReduceBlocks || block < kWave || block > kMaxWaves * kWave ||
+      block % kWave != 0)
+    return WCFD_EINVAL;
+  g_krylov_grid = grid;
+  g_krylov_block = block;
+  return WCFD_OK;
+}
+
+int wcfd_get_krylov_launch(int* grid, int* block) {
+  if (!grid || !block) return WCFD_EINVAL;
+  const KrylovLaunch k = krylov_launch();
+  *grid = k.grid;
+  *block = k.block;
+  return WCFD_OK;
+}
+
 size_t wcfd_krylov_workspace_bytes(int32_t n, int nvec) {
   if (n <= 0 || (nvec != 1 && nvec != 3)) return 0;
   return krylov_workspace_bytes(n, nvec);
@@ -568,6 +719,24 @@ int wcfd_cg(int32_t n, const int32_t* row_ptr, const int32_t* col, const float* 
     return WCFD_EINVAL;
   CsrView A{n, row_ptr, col, values, diag_pos};
   return solve_cg(A, b, x, atol, tol, maxiter, check_every, workspace, info, (hipStream_t)stream);
+}
+
+int wcfd_cg_ell(const wcfd_mesh* m, const float* values, const float* b, float* x, double atol,
+                double tol, int maxiter, int check_every, void* workspace, wcfd_solve_info* info,
+                void* stream) {
+  if (!m || m->n_cells <= 0 || !m->slot_nbr || !values || !b || !x || !workspace)
+    return WCFD_EINVAL;
+  return solve_cg_ell(*m, values, b, x, atol, tol, maxiter, check_every, workspace, info,
+                      (hipStream_t)stream);
+}
+
+int wcfd_bicgstab_ell(const wcfd_mesh* m, const float* values, int nvec, const float* b, float* x,
+                      double atol, double tol, int maxiter, int check_every, void* workspace,
+                      wcfd_solve_info* info, void* stream) {
+  if (!m || m->n_cells <= 0 || !m->slot_nbr || !values || !b || !x || !workspace)
+    return WCFD_EINVAL;
+  return solve_bicgstab_ell(*m, values, nvec, b, x, atol, tol, maxiter, check_every, workspace,
+                            info, (hipStream_t)stream);
 }
 
 int wcfd_bicgstab(int32_t n, const int32_t* row_ptr, const int32_t* col, const float* values,

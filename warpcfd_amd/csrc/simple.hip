@@ -48,14 +48,12 @@ size_t wcfd_simple_workspace_bytes(int32_t n_cells, int32_t n_faces) {
 int wcfd_simple_iteration(const wcfd_mesh* mp, wcfd_bc bc, const wcfd_simple_params* prm,
                           const wcfd_simple_state* st, void* workspace, wcfd_simple_info* info,
                           void* stream) {
-  if (!mp || !prm || !st || !workspace || !bc.type || !bc.value) return WCFD_EINVAL;
+  if (!mp || !prm || !st || !workspace || !bc.type || !bc.value || !mp->slot_nbr) return WCFD_EINVAL;
   const wcfd_mesh& m = *mp;
   if (m.n_cells <= 0 || m.n_faces < m.n_internal || prm->density == 0.f) return WCFD_EINVAL;
   if (prm->ref_cell >= m.n_cells) return WCFD_EINVAL;
   hipStream_t s = (hipStream_t)stream;
   SimpleWork w = carve_simple(workspace, m.n_cells, m.n_faces);
-  const CsrView Am{m.n_cells, m.row_ptr, m.col, st->mom_values, m.diag_pos};
-  const CsrView Ap{m.n_cells, m.row_ptr, m.col, st->p_values, m.diag_pos};
   int e;
   wcfd_simple_info local{};
   wcfd_simple_info* inf = info ? info : &local;
@@ -70,8 +68,8 @@ int wcfd_simple_iteration(const wcfd_mesh* mp, wcfd_bc bc, const wcfd_simple_par
                                     st->cell_grad, 1, prm->nu, 1.0f / prm->density, prm->alpha_u,
                                     prm->dt, prm->scheme_flags, st->mom_values, st->mom_rhs, s)))
     return e;
-  e = solve_bicgstab(Am, 3, st->mom_rhs, st->hbya, prm->atol, prm->tol, prm->maxiter,
-                     prm->check_every, w.krylov, &inf->momentum, s);
+  e = solve_bicgstab_ell(m, st->mom_values, 3, st->mom_rhs, st->hbya, prm->atol, prm->tol,
+                         prm->maxiter, prm->check_every, w.krylov, &inf->momentum, s);
   if (e) return e;
   if ((e = launch_rua_hbya(m, st->mom_values, prm->density, st->cell_grad, st->hbya,
                            st->cell_values, st->rua, s)))
@@ -89,8 +87,8 @@ int wcfd_simple_iteration(const wcfd_mesh* mp, wcfd_bc bc, const wcfd_simple_par
                                         prm->ref_cell,
                                         prm->ref_value, st->p_values, st->p_rhs, nullptr, s)))
         return e;
-      e = solve_cg(Ap, st->p_rhs, st->p_x, prm->atol, prm->tol, prm->maxiter, prm->check_every,
-                   w.krylov, &inf->pressure, s);
+      e = solve_cg_ell(m, st->p_values, st->p_rhs, st->p_x, prm->atol, prm->tol, prm->maxiter,
+                       prm->check_every, w.krylov, &inf->pressure, s);
       if (e) return e;
       if ((e = launch_relax_p(m, st->p_x, prm->alpha_p, st->cell_values, s))) return e;
       if ((e = launch_correct_velocity(m, bc, st->cell_values, st->cell_grad, st->face_values,

@@ -130,11 +130,31 @@ class DeviceLayout:
         slot_pos = np.where(nb_slot_new >= 0, row_ptr[:-1, None] + rank, -1)
         self.slot_pos = np.ascontiguousarray(slot_pos.T).astype(np.int32)    # (6,C)
         self.diag_pos = (row_ptr[:-1] + (srt < np.arange(C)[:, None]).sum(axis=1)).astype(np.int32)
-        # neighbour cell per slot (new ids, -1 at boundary)
-        self.slot_nbr = np.ascontiguousarray(nb_slot_new.T).astype(np.int32)
+
+        # neighbour cell per slot (new ids); boundary slots point at the cell
+        # itself (their matrix coefficient is 0), so SpMV needs no branch
+        own_id = np.arange(C)[:, None]
+        nb_self = np.where(nb_slot_new >= 0, nb_slot_new, own_id)
+        self.slot_nbr = np.ascontiguousarray(nb_self.T).astype(np.int32)
+        delta = nb_self - own_id
+        self.band = int(np.abs(delta).max()) if delta.size else 0
+        # int16 neighbour offsets when the RCM band allows it (halves index bytes)
+        self.slot_dnb = (np.ascontiguousarray(delta.T).astype(np.int16)
+                         if self.band <= np.iinfo(np.int16).max else None)
         # owner-side local index of each face and the neighbour-side one
         self.face_k0 = k0[fo].astype(np.int8)
         self.face_k1 = k1[fi_sorted].astype(np.int8)
+
+    def ell_to_csr(self, ell):
+        """Host conversion of ELL values [7][C] (diagonal, 6 slots) to values
+        on (row_ptr, col)."""
+        ell = np.asarray(ell).reshape(7, self.num_cells)
+        out = np.zeros(self.nnz, ell.dtype)
+        out[self.diag_pos] = ell[0]
+        for k in range(6):
+            ok = self.slot_pos[k] >= 0
+            out[self.slot_pos[k][ok]] = ell[k + 1][ok]
+        return out
 
     # ---------------------------------------------------------------- maps
     def cells_to_device(self, a):

@@ -1,0 +1,92 @@
+"""Autotuner for the Krylov launch shape (workgroups x threads).
+
+The CG and BiCGSTAB kernels are grid-stride loops whose workgroups each write
+one float64 partial per reduction; the next kernel folds all of them in its
+prologue.  More workgroups hide memory latency better but make every fold
+longer, so the best shape depends on the mesh size.  ``autotune_krylov``
+times fixed-iteration solves on a pressure-like matrix built on the mesh's
+own pattern for every candidate shape, installs the fastest with
+``wcfd_set_krylov_launch`` and caches it by cell count.
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+from .FV.device import ptr, stream_ptr
+
+GRIDS = (128, 256, 512, 1024, 2048)
+BLOCKS = (128, 256, 512)
+
+_cache = {}
+
+
+def _laplacian_ell(dm):
+    """Negative-definite ELL matrix with the geometric Laplacian weights
+    A_f / |d| on the mesh pattern (the pressure matrix with rUA = 1)."""
+    C = dm.n_cells
+    sf = dm.slot_face.long()
+    w = dm.area[sf] / dm.dmag[sf]
+    w = torch.where(dm.slot_pos >= 0, w, torch.zeros_like(w))
+    ell = torch.empty(7, C, dtype=torch.float32, device=w.device)
+    ell[1:] = w
+    ell[0] = -(w.sum(dim=0)) * 1.001
+    return ell.contiguous()
+
+
+def time_solvers(dm, grid, block, iters=60, reps=2):
+    """Milliseconds of `iters` CG iterations and `iters` 3-vector BiCGSTAB
+    iterations at launch shape (grid, block)."""
+    lib = _lib.load()
+    _lib.check(lib.wcfd_set_krylov_launch(grid, block), 'set_krylov_launch')
+    C = dm.n_cells
+    dev = dm.device
+    A = _laplacian_ell(dm)
+    g = torch.Generator(device='cpu').manual_seed(0)
+    b = torch.randn(C, generator=g).to(dev)
+    b3 = torch.randn(3, C, generator=g).to(dev)
+    ws = torch.zeros(lib.wcfd_krylov_workspace_bytes(C, 3), dtype=torch.uint8, device=dev)
+    info = _lib.SolveInfo()
+    best = [float('inf'), float('inf')]
+    for _ in range(reps):
+        for j, nv in enumerate((1, 3)):
+            x = torch.zeros(nv, C, dtype=torch.float32, device=dev)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            e0.record()
+            if nv == 1:
+                code = lib.wcfd_cg_ell(dm.desc_ref, ptr(A), ptr(b), ptr(x), 0.0, 0.0, iters, iters,
+                                       ptr(ws), ctypes.byref(info), stream_ptr())
+            else:
+                code = lib.wcfd_bicgstab_ell(dm.desc_ref, ptr(-A), 3, ptr(b3), ptr(x), 0.0, 0.0, iters,
+                                             iters, ptr(ws), ctypes.byref(info), stream_ptr())
+            e1.record()
+            torch.cuda.synchronize()
+            _lib.check(code, 'tuning solve')
+            best[j] = min(best[j], e0.elapsed_time(e1))
+    return best
+
+
+def autotune_krylov(dm, grids=GRIDS, blocks=BLOCKS, iters=60, cg_weight=1.0, bicg_weight=0.1):
+    """Pick and install the fastest (grid, block) for mesh ``dm``
+    (a DeviceMesh).  The weights reflect a SIMPLE iteration, where the
+    pressure CG runs far more iterations than the momentum BiCGSTAB."""
+    key = dm.n_cells
+    if key in _cache:
+        g, b = _cache[key]['grid'], _cache[key]['block']
+        _lib.check(_lib.load().wcfd_set_krylov_launch(g, b), 'set_krylov_launch')
+        return _cache[key]
+    results = []
+    for blk in blocks:
+        for grid in grids:
+            if grid * blk > 4 * dm.n_cells and grid > grids[0]:
+                continue            # more threads than rows: no gain
+            cg_ms, bi_ms = time_solvers(dm, grid, blk, iters)
+            results.append({'grid': grid, 'block': blk, 'cg_ms': cg_ms, 'bicg_ms': bi_ms,
+                            'score': cg_weight * cg_ms + bicg_weight * bi_ms})
+    best = min(results, key=lambda r: r['score'])
+    _lib.check(_lib.load().wcfd_set_krylov_launch(best['grid'], best['block']), 'set_krylov_launch')
+    out = dict(best, iters=iters, candidates=results)
+    _cache[key] = out
+    return out
