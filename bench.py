@@ -72,6 +72,9 @@ def algorithmic_bytes(kernel, C, F, Fi, Fb, nnz):
         'cg_1': 40 * C + 12 * C + 8 * C,
         # x += a p, r -= a q: reads x p r q inv, writes x r
         'cg_2': 28 * C,
+        # fused single-reduction CG iteration: ELL matrix 40 B, gathers of
+        # r, w, s, inv (16 B), own p, x (8 B), writes p, s, x, r, w (20 B)
+        'cg_1_single': 40 * C + 16 * C + 8 * C + 20 * C,
         'bicgstab_1': 40 * C + 4 * C + 36 * C + 24 * C + 12 * C,
         'bicgstab_2': 40 * C + 4 * C + 24 * C + 24 * C,
         'bicgstab_3': 64 * C + 24 * C,
@@ -131,6 +134,9 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-profile', action='store_true')
     ap.add_argument('--no-tune', action='store_true')
+    ap.add_argument('--cg', default='single', choices=['standard', 'single'],
+                    help='pressure CG form: standard (two reductions per iteration) or single '
+                         '(Chronopoulos-Gear, one fused kernel per iteration)')
     ap.add_argument('--krylov-launch', default=None,
                     help='GRIDxBLOCK: fixed Krylov launch shape instead of autotuning')
     args = ap.parse_args()
@@ -158,6 +164,7 @@ def main():
     apply_bcs(model.boundary, wl['kind'])
     model.finalize()
     solver = IncompressibleSolver(model, 0.7, 0.3, implicit_outflow=(wl['kind'] == 'channel'))
+    solver.cg_variant = _lib.CG_SINGLE_REDUCTION if args.cg == 'single' else _lib.CG_STANDARD
     dm = model.dm
     tuned = None
     if args.krylov_launch:
@@ -166,7 +173,7 @@ def main():
         tuned = {'grid': g, 'block': b, 'source': 'command line'}
     elif not args.no_tune:
         from warpcfd_amd.tune import autotune_krylov
-        t = autotune_krylov(dm)
+        t = autotune_krylov(dm, cg_variant=solver.cg_variant)
         tuned = {'grid': t['grid'], 'block': t['block'], 'cg_ms_per_60_it': round(t['cg_ms'], 3),
                  'candidates': len(t['candidates'])}
     C, F, Fi, Fb, nnz = dm.n_cells, dm.n_faces, dm.n_internal, dm.n_boundary, dm.nnz
@@ -193,7 +200,8 @@ def main():
                    for k, v in sorted(prof.result.items(), key=lambda kv: -kv[1][0])}
         dominant = max(prof.result.items(), key=lambda kv: kv[1][0])[0]
         ms, cnt = prof.result[dominant]
-        byts = algorithmic_bytes(dominant, C, F, Fi, Fb, nnz)
+        key = dominant + ('_single' if dominant == 'cg_1' and args.cg == 'single' else '')
+        byts = algorithmic_bytes(key, C, F, Fi, Fb, nnz)
         if byts is not None:
             avg_s = ms / cnt / 1e3
             ach = byts / avg_s / 1e9
@@ -220,7 +228,8 @@ def main():
         'dtype': 'f32',
         'data': 'synthetic',
         'config': {'workload': wl['desc'], 'cells': C, 'faces': F, 'nnz': nnz,
-                   'reductions': 'f64', 'krylov': {'atol': 1e-5, 'maxiter': 500, 'check_every': 10}},
+                   'reductions': 'f64', 'krylov': {'atol': 1e-5, 'maxiter': 500, 'check_every': 10,
+                                                   'pressure_cg': args.cg}},
         'roofline': roofline,
         'cpu_baseline': cpu,
         'continuity_residual': hist[-1]['continuity'],

@@ -35,6 +35,11 @@ extern "C" {
 #define WCFD_DIRICHLET 1
 #define WCFD_NEUMANN 2
 
+/* CG variants (same stopping rule and, in exact arithmetic, the same
+ * iterates) */
+#define WCFD_CG_STANDARD 0          /* two reductions, two kernels per iteration */
+#define WCFD_CG_SINGLE_REDUCTION 1  /* Chronopoulos-Gear: one fused kernel per iteration */
+
 /* scheme flags */
 #define WCFD_SCHEME_CORRECTED 1        /* over-relaxed non-orthogonal Laplacian
                                           (laplacianScheme.py:39-66) */
@@ -207,10 +212,10 @@ typedef struct wcfd_solve_info {
  * Equation.solve_Axb (equation.py:195-200; incompressible.py:32,142).
  * x is updated in place (warm start).  Stops when ||r|| <= max(tol ||b||,
  * atol), checked before the first iteration and every check_every
- * iterations, or at maxiter.  info is [host]. */
+ * iterations, or at maxiter.  variant: WCFD_CG_*.  info is [host]. */
 int wcfd_cg(int32_t n, const int32_t* row_ptr, const int32_t* col, const float* values,
-            const int32_t* diag_pos, const float* b, float* x, double atol, double tol,
-            int maxiter, int check_every, void* workspace, wcfd_solve_info* info,
+            const int32_t* diag_pos, int variant, const float* b, float* x, double atol,
+            double tol, int maxiter, int check_every, void* workspace, wcfd_solve_info* info,
             void* stream);
 
 /* warp.optim.linear.bicgstab, right Jacobi preconditioning, as called by
@@ -231,9 +236,9 @@ int wcfd_set_krylov_launch(int grid, int block);
 int wcfd_get_krylov_launch(int* grid, int* block); /* [host] outputs */
 
 /* wcfd_cg / wcfd_bicgstab on the ELL matrix `values` [7][C] of mesh m. */
-int wcfd_cg_ell(const wcfd_mesh* m, const float* values, const float* b, float* x, double atol,
-                double tol, int maxiter, int check_every, void* workspace, wcfd_solve_info* info,
-                void* stream);
+int wcfd_cg_ell(const wcfd_mesh* m, const float* values, int variant, const float* b, float* x,
+                double atol, double tol, int maxiter, int check_every, void* workspace,
+                wcfd_solve_info* info, void* stream);
 int wcfd_bicgstab_ell(const wcfd_mesh* m, const float* values, int nvec, const float* b, float* x,
                       double atol, double tol, int maxiter, int check_every, void* workspace,
                       wcfd_solve_info* info, void* stream);
@@ -254,7 +259,7 @@ typedef struct wcfd_simple_params {
   double atol, tol;    /* Krylov tolerances */
   int32_t maxiter, check_every;
   int32_t first_iteration; /* 1: u,v,w gradients may be non-zero on entry */
-  int32_t reserved;
+  int32_t cg_variant;      /* WCFD_CG_* for the pressure equation */
 } wcfd_simple_params;
 
 /* Device state of one case. */

@@ -99,3 +99,48 @@ def bicgstab(A, b, x, inv_diag, atol=1e-5, tol=None, maxiter=500, check_every=10
         it += check_every
         if rr <= thr2 or it >= maxiter:
             return it, np.sqrt(rr)
+
+
+def cg_single_reduction(A, b, x, inv_diag, atol=1e-5, tol=None, maxiter=500, check_every=10):
+    """Chronopoulos-Gear PCG: the same iterates as ``cg`` in exact arithmetic,
+    with both inner products of an iteration taken together (one global
+    reduction per iteration).  Restates the variant the HIP path can run as
+    one fused kernel per iteration; same stopping rule as ``cg``."""
+    dt = x.dtype.type
+    thr2 = threshold(b, atol, tol) ** 2
+    r = (b - A @ x).astype(x.dtype)
+    u = (r * inv_diag).astype(x.dtype)
+    w = (A @ u).astype(x.dtype)
+    gamma = _dot(r, u)
+    delta = _dot(w, u)
+    rr = _dot(r, r)
+    it = 0
+    if rr <= thr2 or maxiter == 0:
+        return it, np.sqrt(rr)
+    alpha = _safe_div(gamma, delta)
+    beta = 0.0
+    p = np.zeros_like(r)
+    s = np.zeros_like(r)
+    first = True
+    while True:
+        for _ in range(check_every):
+            if first:
+                p = u.copy()
+                s = w.copy()
+                first = False
+            else:
+                p = (u + dt(beta) * p).astype(x.dtype)
+                s = (w + dt(beta) * s).astype(x.dtype)
+            x += dt(alpha) * p
+            r = (r - dt(alpha) * s).astype(x.dtype)
+            u = (r * inv_diag).astype(x.dtype)
+            w = (A @ u).astype(x.dtype)
+            g_new = _dot(r, u)
+            d_new = _dot(w, u)
+            rr = _dot(r, r)
+            beta = _safe_div(g_new, gamma)
+            alpha = _safe_div(g_new, d_new - beta * _safe_div(g_new, alpha))
+            gamma = g_new
+        it += check_every
+        if rr <= thr2 or it >= maxiter:
+            return it, np.sqrt(rr)

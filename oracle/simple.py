@@ -25,7 +25,7 @@ from . import krylov
 class OracleSolver:
     def __init__(self, fvm, u_relax=0.7, p_relax=0.3, correction=False, steady_state=True,
                  nu=0.01, rho=1.0, atol=None, maxiter=500, check_every=10,
-                 implicit_outflow=False):
+                 implicit_outflow=False, cg_variant=0):
         self.fvm = fvm
         self.u_relax = u_relax
         self.p_relax = p_relax
@@ -38,6 +38,7 @@ class OracleSolver:
         self.maxiter = maxiter
         self.check_every = check_every
         self.implicit_outflow = implicit_outflow
+        self.cg = krylov.cg if cg_variant == 0 else krylov.cg_single_reduction
         C = fvm.m.num_cells
         self.vel_eq = OracleEquation(fvm, [0, 1, 2])
         self.p_eq = OracleEquation(fvm, [3])
@@ -143,7 +144,7 @@ class OracleSolver:
                     self.p_eq.replace_row(fvm.reference_pressure_cell_id, fvm.reference_pressure)
                 Pd = self.p_eq.diagonal
                 invp = (1.0 / Pd).astype(d)
-                it_p, res_p = krylov.cg(self.p_eq.matrix(), self.p_eq.rhs, self.p_eq.x, invp,
+                it_p, res_p = self.cg(self.p_eq.matrix(), self.p_eq.rhs, self.p_eq.x, invp,
                                         self.atol, maxiter=self.maxiter, check_every=self.check_every)
                 p_iters.append(it_p)
                 p_new = self.p_eq.x

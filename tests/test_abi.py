@@ -30,7 +30,7 @@ def test_library_exports_every_header_symbol():
 def test_invalid_arguments_are_rejected_without_gpu_work():
     lib = _lib.load()
     assert lib.wcfd_spmv(0, None, None, None, 1, None, None, None) == _lib.WCFD_EINVAL
-    assert lib.wcfd_cg(10, None, None, None, None, None, None, 1e-5, 1e-5, 10, 10, None, None,
+    assert lib.wcfd_cg(10, None, None, None, None, 0, None, None, 1e-5, 1e-5, 10, 10, None, None,
                        None) == _lib.WCFD_EINVAL
     m = _lib.WcfdMesh()
     assert lib.wcfd_green_gauss(ctypes.byref(m), None, None, 15, 1, None) == _lib.WCFD_EINVAL

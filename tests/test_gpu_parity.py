@@ -236,12 +236,15 @@ def test_ell_solvers_match_csr(gpu):
     xa = torch.zeros(n, dtype=torch.float32, device=dev)
     xb = torch.zeros(n, dtype=torch.float32, device=dev)
     ia, ib = SolveInfo(), SolveInfo()
-    assert lib.wcfd_cg_ell(model.dm.desc_ref, ptr(ell), ptr(b), ptr(xa), 1e-6, 1e-6, 1000, 10, ptr(ws),
-                           ctypes.byref(ia), stream_ptr()) == 0
-    assert lib.wcfd_cg(n, ptr(model.dm.row_ptr), ptr(model.dm.col), ptr(csr), ptr(model.dm.diag_pos), ptr(b),
-                       ptr(xb), 1e-6, 1e-6, 1000, 10, ptr(ws), ctypes.byref(ib), stream_ptr()) == 0
-    assert ia.iterations == ib.iterations and ia.converged
-    assert rel_l2(xa.cpu().numpy(), xb.cpu().numpy()) <= 1e-5
+    for variant in (0, 1):
+        xa.zero_(); xb.zero_()
+        assert lib.wcfd_cg_ell(model.dm.desc_ref, ptr(ell), variant, ptr(b), ptr(xa), 1e-6, 1e-6, 1000, 10,
+                               ptr(ws), ctypes.byref(ia), stream_ptr()) == 0
+        assert lib.wcfd_cg(n, ptr(model.dm.row_ptr), ptr(model.dm.col), ptr(csr), ptr(model.dm.diag_pos),
+                           variant, ptr(b), ptr(xb), 1e-6, 1e-6, 1000, 10, ptr(ws), ctypes.byref(ib),
+                           stream_ptr()) == 0
+        assert ia.iterations == ib.iterations and ia.converged
+        assert rel_l2(xa.cpu().numpy(), xb.cpu().numpy()) <= 1e-5
     b3 = torch.from_numpy(rng.standard_normal((3, n)).astype(np.float32)).to(dev)
     xa3 = torch.zeros(3, n, dtype=torch.float32, device=dev)
     negell = -ell
@@ -252,7 +255,8 @@ def test_ell_solvers_match_csr(gpu):
     assert ia.converged and rel_l2(xa3.cpu().numpy(), xs) <= 1e-5
 
 
-def test_spmv_and_cg(gpu):
+@pytest.mark.parametrize('variant', [0, 1])
+def test_spmv_and_cg(gpu, variant):
     torch = _torch()
     from warpcfd_amd._lib import SolveInfo
     from warpcfd_amd.FV.device import ptr, stream_ptr
@@ -278,14 +282,15 @@ def test_spmv_and_cg(gpu):
     ws = torch.zeros(lib.wcfd_krylov_workspace_bytes(n, 3), dtype=torch.uint8, device=dev)
     info = SolveInfo()
     code = lib.wcfd_cg(n, ptr(model.dm.row_ptr), ptr(model.dm.col), ptr(vals), ptr(model.dm.diag_pos),
-                       ptr(torch.from_numpy(b).to(dev)), ptr(xd), 1e-7, 1e-7, 2000, 10, ptr(ws),
+                       variant, ptr(torch.from_numpy(b).to(dev)), ptr(xd), 1e-7, 1e-7, 2000, 10, ptr(ws),
                        ctypes.byref(info), stream_ptr())
     assert code == 0 and info.converged
     assert rel_l2(xd.cpu().numpy(), xs) <= 1e-5
     # iteration count and residual follow the oracle's statement of the solver
     from oracle import krylov
     xo = np.zeros(n, np.float32)
-    it, res = krylov.cg(A, b, xo, (1.0 / A.diagonal()).astype(np.float32), 1e-7, maxiter=2000)
+    solve = krylov.cg if variant == 0 else krylov.cg_single_reduction
+    it, res = solve(A, b, xo, (1.0 / A.diagonal()).astype(np.float32), 1e-7, maxiter=2000)
     assert it == info.iterations
     assert rel_l2(xd.cpu().numpy(), xo) <= 1e-5
 
@@ -323,13 +328,15 @@ def test_bicgstab_three_vectors(gpu):
 
 
 def _run_pair(kind, nodes, cells, iters, ref_cell=0, u_relax=0.7, p_relax=0.3, tol=None,
-              implicit_outflow=False):
+              implicit_outflow=False, cg_variant=0):
     from oracle.simple import OracleSolver
     from warpcfd_amd.FV.implicit_Solvers import IncompressibleSolver
     of = oracle_case(nodes, cells, kind, ref_cell=ref_cell)
     model = device_case(nodes, cells, kind, ref_cell=ref_cell)
-    os_ = OracleSolver(of, u_relax, p_relax, nu=0.01, atol=tol, implicit_outflow=implicit_outflow)
+    os_ = OracleSolver(of, u_relax, p_relax, nu=0.01, atol=tol, implicit_outflow=implicit_outflow,
+                       cg_variant=cg_variant)
     sv = IncompressibleSolver(model, u_relax, p_relax, implicit_outflow=implicit_outflow)
+    sv.cg_variant = cg_variant
     if tol is not None:
         sv.tol = tol
     for _ in range(iters):
@@ -378,9 +385,11 @@ def test_simple_cavity_converged_vs_oracle(gpu):
         assert rel_l2(got[:, q], of.cell_values[:, q]) <= 1e-4, q
 
 
-def test_simple_jittered_channel_vs_oracle(gpu):
+@pytest.mark.parametrize('cg_variant', [0, 1])
+def test_simple_jittered_channel_vs_oracle(gpu, cg_variant):
     nodes, cells = jittered_box((12, 6, 5), 0.1, 13, lengths=(2.0, 1.0, 0.8))
-    of, model, os_, sv = _run_pair('channel', nodes, cells, 20, ref_cell=None, implicit_outflow=True)
+    of, model, os_, sv = _run_pair('channel', nodes, cells, 20, ref_cell=None, implicit_outflow=True,
+                                   cg_variant=cg_variant)
     got = model.cell_values.cpu().numpy()
     assert rel_l2(got[:, :3], of.cell_values[:, :3]) <= 1e-4
     assert rel_l2(got[:, 3], of.cell_values[:, 3]) <= 1e-4
@@ -426,7 +435,7 @@ def test_cg_after_bicgstab_in_one_workspace(gpu):
                                  500, 10, ptr(ws), ctypes.byref(info), stream_ptr()) == 0
         x1 = torch.zeros(n, dtype=torch.float32, device=dev)
         info = SolveInfo()
-        assert lib.wcfd_cg(n, *args, ptr(vals), ptr(model.dm.diag_pos), ptr(torch.from_numpy(b1).to(dev)),
+        assert lib.wcfd_cg(n, *args, ptr(vals), ptr(model.dm.diag_pos), 1, ptr(torch.from_numpy(b1).to(dev)),
                            ptr(x1), 1e-6, 1e-6, 1000, 10, ptr(ws), ctypes.byref(info), stream_ptr()) == 0
         assert info.converged and info.iterations > 0
         xs = spla.spsolve(A.astype(np.float64).tocsc(), b1.astype(np.float64))

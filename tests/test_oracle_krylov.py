@@ -61,3 +61,23 @@ def test_stopping_rule():
     x1 = np.zeros(200)
     it, res = krylov.cg(A, b, x1, 1.0 / A.diagonal(), atol=1e-30, maxiter=30)
     assert it == 30
+
+
+def test_single_reduction_cg_matches_cg():
+    """Chronopoulos-Gear CG: same solution and (in this well-conditioned
+    case) the same number of check intervals as standard CG."""
+    A = _laplacian(400, seed=7)
+    b = np.random.default_rng(5).standard_normal(400)
+    inv = 1.0 / A.diagonal()
+    x1 = np.zeros(400); x2 = np.zeros(400)
+    it1, _ = krylov.cg(A, b, x1, inv, atol=1e-10, maxiter=2000)
+    it2, _ = krylov.cg_single_reduction(A, b, x2, inv, atol=1e-10, maxiter=2000)
+    xs = spla.spsolve(A.tocsc(), b)
+    assert np.linalg.norm(x2 - xs) / np.linalg.norm(xs) < 1e-8
+    assert abs(it1 - it2) <= 10
+    # float32 vectors with float64 dots: still converges to the float32 floor
+    A32 = A.astype(np.float32)
+    x3 = np.zeros(400, np.float32)
+    it3, res3 = krylov.cg_single_reduction(A32, b.astype(np.float32), x3, inv.astype(np.float32),
+                                           atol=1e-4, maxiter=2000)
+    assert res3 <= 1e-4 * max(1.0, np.linalg.norm(b))

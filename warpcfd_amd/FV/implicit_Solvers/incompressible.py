@@ -55,6 +55,10 @@ class IncompressibleSolver:
         self.tol = 1e-5          # equation.py:36-39 (float32)
         self.rtol = None         # None: relative tolerance = atol, as the reference's call
         self.check_every = 10
+        # pressure CG: WCFD_CG_STANDARD (the reference's two-reduction form) or
+        # WCFD_CG_SINGLE_REDUCTION (same iterates in exact arithmetic, one
+        # fused kernel per iteration)
+        self.cg_variant = _lib.CG_STANDARD
         self.set_time_stepping(1, 1.)
         self.NUM_ORTHOGONAL_CORRECTORS = 1
         self.NUM_OUTER_LOOPS = 1
@@ -100,6 +104,7 @@ class IncompressibleSolver:
         p.maxiter = int(self.max_iter)
         p.check_every = int(self.check_every)
         p.first_iteration = 0 if m._uvw_grad_zero else 1
+        p.cg_variant = int(self.cg_variant)
         return p
 
     def outer_iteration(self, dt=None, stream=None):

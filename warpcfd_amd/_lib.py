@@ -15,6 +15,7 @@ HEADER_PATH = os.path.join(os.path.dirname(_HERE), 'include', 'wcfd.h')
 WCFD_OK, WCFD_EINVAL, WCFD_EHIP, WCFD_ENOTCONV, WCFD_ENAN = 0, 1, 2, 3, 4
 DIRICHLET, NEUMANN = 1, 2
 SCHEME_CORRECTED, SCHEME_IMPLICIT_OUTFLOW = 1, 2
+CG_STANDARD, CG_SINGLE_REDUCTION = 0, 1
 
 c_p = ctypes.c_void_p
 c_i32 = ctypes.c_int32
@@ -49,7 +50,7 @@ class SimpleParams(ctypes.Structure):
                 ('scheme_flags', c_i32), ('ref_cell', c_i32), ('ref_value', c_f),
                 ('n_inner', c_i32), ('n_ortho', c_i32), ('atol', c_d), ('tol', c_d),
                 ('maxiter', c_i32), ('check_every', c_i32), ('first_iteration', c_i32),
-                ('reserved', c_i32)]
+                ('cg_variant', c_i32)]
 
 
 class SimpleState(ctypes.Structure):
@@ -79,8 +80,8 @@ _SIGS = {
                                               ctypes.c_int, ctypes.c_int, c_f, c_p, c_p, c_p, c_p]),
     'wcfd_ell_to_csr': (ctypes.c_int, [_PM, c_p, c_p, c_p]),
     'wcfd_spmv_ell': (ctypes.c_int, [_PM, c_p, ctypes.c_int, c_p, c_p, c_p]),
-    'wcfd_cg_ell': (ctypes.c_int, [_PM, c_p, c_p, c_p, c_d, c_d, ctypes.c_int, ctypes.c_int, c_p,
-                                   ctypes.POINTER(SolveInfo), c_p]),
+    'wcfd_cg_ell': (ctypes.c_int, [_PM, c_p, ctypes.c_int, c_p, c_p, c_d, c_d, ctypes.c_int,
+                                   ctypes.c_int, c_p, ctypes.POINTER(SolveInfo), c_p]),
     'wcfd_bicgstab_ell': (ctypes.c_int, [_PM, c_p, ctypes.c_int, c_p, c_p, c_d, c_d, ctypes.c_int,
                                          ctypes.c_int, c_p, ctypes.POINTER(SolveInfo), c_p]),
     'wcfd_rua_hbya': (ctypes.c_int, [_PM, c_p, c_f, c_p, c_p, c_p, c_p, c_p]),
@@ -91,8 +92,8 @@ _SIGS = {
     'wcfd_spmv': (ctypes.c_int, [c_i32, c_p, c_p, c_p, ctypes.c_int, c_p, c_p, c_p]),
     'wcfd_dot_f64': (ctypes.c_int, [c_i32, c_p, c_p, c_p, c_p, c_p]),
     'wcfd_krylov_workspace_bytes': (ctypes.c_size_t, [c_i32, ctypes.c_int]),
-    'wcfd_cg': (ctypes.c_int, [c_i32, c_p, c_p, c_p, c_p, c_p, c_p, c_d, c_d, ctypes.c_int,
-                               ctypes.c_int, c_p, ctypes.POINTER(SolveInfo), c_p]),
+    'wcfd_cg': (ctypes.c_int, [c_i32, c_p, c_p, c_p, c_p, ctypes.c_int, c_p, c_p, c_d, c_d,
+                               ctypes.c_int, ctypes.c_int, c_p, ctypes.POINTER(SolveInfo), c_p]),
     'wcfd_bicgstab': (ctypes.c_int, [c_i32, c_p, c_p, c_p, c_p, ctypes.c_int, c_p, c_p, c_d, c_d,
                                      ctypes.c_int, ctypes.c_int, c_p, ctypes.POINTER(SolveInfo),
                                      c_p]),

@@ -34,14 +34,14 @@ struct CsrView {
   const int32_t* diag_pos;
 };
 
-int solve_cg(const CsrView& A, const float* b, float* x, double atol, double tol, int maxiter,
-             int check_every, void* workspace, wcfd_solve_info* info, hipStream_t s);
+int solve_cg(const CsrView& A, int variant, const float* b, float* x, double atol, double tol,
+             int maxiter, int check_every, void* workspace, wcfd_solve_info* info, hipStream_t s);
 int solve_bicgstab(const CsrView& A, int nvec, const float* b, float* x, double atol, double tol,
                    int maxiter, int check_every, void* workspace, wcfd_solve_info* info,
                    hipStream_t s);
-int solve_cg_ell(const wcfd_mesh& m, const float* values, const float* b, float* x, double atol,
-                 double tol, int maxiter, int check_every, void* workspace, wcfd_solve_info* info,
-                 hipStream_t s);
+int solve_cg_ell(const wcfd_mesh& m, const float* values, int variant, const float* b, float* x,
+                 double atol, double tol, int maxiter, int check_every, void* workspace,
+                 wcfd_solve_info* info, hipStream_t s);
 int solve_bicgstab_ell(const wcfd_mesh& m, const float* values, int nvec, const float* b, float* x,
                        double atol, double tol, int maxiter, int check_every, void* workspace,
                        wcfd_solve_info* info, hipStream_t s);

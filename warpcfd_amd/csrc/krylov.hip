@@ -232,6 +232,115 @@ __global__ void k_cg_2(int n, int it, float* __restrict__ x, float* __restrict__
   store_partials<2>(acc, PA, G);
 }
 
+// ------------------------------------------- CG with one reduction per iteration
+// Chronopoulos-Gear form of the same Jacobi-preconditioned CG: the two inner
+// products of an iteration (gamma = r.u, delta = w.u, u = M r, w = A u) are
+// taken together, so one fused kernel per iteration updates p, s (= A p), x
+// and r, forms the next w by SpMV and writes the partials of gamma, delta
+// and r.r.  The p and s recurrences of iteration i are applied at the start
+// of kernel i (they need beta_i, known only after the reduction of kernel
+// i-1); the neighbour values of r_{i+1} = r_i - alpha_i s_i are formed on the
+// fly.  Partials ping-pong between PA and PC: kernel i reads the buffer
+// kernel i-1 wrote.
+template <class M>
+__global__ void k_cgs_init1(M A, int n, const float* __restrict__ b, const float* __restrict__ x,
+                            float* __restrict__ r, double* __restrict__ PC) {
+  double acc[3] = {0.0, 0.0, 0.0};
+  const RowRange rr_ = my_rows(n);
+  for (int i = rr_.begin + (int)threadIdx.x; i < rr_.end; i += blockDim.x) {
+    float ax = 0.f;
+    A.row(i, [&](float a, int c) { ax += a * x[c]; });
+    const float ri = b[i] - ax;
+    r[i] = ri;
+    acc[1] += (double)ri * (double)ri;
+    acc[2] += (double)b[i] * (double)b[i];
+  }
+  store_partials<3>(acc, PC, gridDim.x);
+}
+
+template <class M>
+__global__ void k_cgs_init2(M A, int n, const float* __restrict__ r, const float* __restrict__ inv,
+                            float* __restrict__ w, double* __restrict__ PA) {
+  double acc[3] = {0.0, 0.0, 0.0};
+  const RowRange rr_ = my_rows(n);
+  for (int i = rr_.begin + (int)threadIdx.x; i < rr_.end; i += blockDim.x) {
+    float wi = 0.f;
+    A.row(i, [&](float a, int c) { wi += a * (r[c] * inv[c]); });
+    w[i] = wi;
+    const float ui = r[i] * inv[i];
+    acc[0] += (double)r[i] * (double)ui;
+    acc[1] += (double)wi * (double)ui;
+    acc[2] += (double)r[i] * (double)r[i];
+  }
+  store_partials<3>(acc, PA, gridDim.x);
+}
+
+__global__ void k_cgs_init_fold(const double* __restrict__ PA, const double* __restrict__ PC, int G,
+                                KState* st, int maxiter) {
+  const double gamma = fold(PA, G), delta = fold(PA + G, G), rr = fold(PA + 2 * G, G);
+  const double bb = fold(PC + 2 * G, G);
+  if (threadIdx.x == 0) {
+    st->bb = bb;
+    const double thr = fmax(st->tol * sqrt(bb), st->atol);
+    st->f.rr = rr;
+    st->f.thr2 = thr * thr;
+    st->f.iters = 0;
+    st->f.done = (!(rr > thr * thr) || maxiter <= 0) ? 1 : 0;
+    st->rz[0] = gamma;
+    st->alpha[0] = safe_div(gamma, delta);
+  }
+}
+
+template <class M>
+__global__ void k_cgs_iter(M A, int n, int it, float* __restrict__ x, const float* __restrict__ r_in,
+                           float* __restrict__ r_out, const float* __restrict__ w_in,
+                           float* __restrict__ w_out, const float* __restrict__ s_in,
+                           float* __restrict__ s_out, float* __restrict__ p,
+                           const float* __restrict__ inv, const double* __restrict__ P_in,
+                           double* __restrict__ P_out, KState* st) {
+  if (st->f.done) return;
+  const int G = gridDim.x;
+  const bool first = (it == 0);
+  double alpha_d, beta_d = 0.0;
+  if (first) {
+    alpha_d = st->alpha[0];
+  } else {
+    const int pv = (it - 1) & 1;
+    const double g = fold(P_in, G), d = fold(P_in + G, G);
+    beta_d = safe_div(g, st->rz[pv]);
+    alpha_d = safe_div(g, d - beta_d * safe_div(g, st->alpha[pv]));
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->rz[it & 1] = g;
+      st->alpha[it & 1] = alpha_d;
+    }
+  }
+  const float alpha = (float)alpha_d, beta = (float)beta_d;
+  double acc[3] = {0.0, 0.0, 0.0};
+  const RowRange rr_ = my_rows(n);
+  for (int i = rr_.begin + (int)threadIdx.x; i < rr_.end; i += blockDim.x) {
+    float wn = 0.f;
+    A.row(i, [&](float a, int c) {
+      const float sc = first ? w_in[c] : w_in[c] + beta * s_in[c];
+      wn += a * (inv[c] * (r_in[c] - alpha * sc));
+    });
+    const float ii = inv[i];
+    const float ui = r_in[i] * ii;
+    const float pi = first ? ui : ui + beta * p[i];
+    const float si = first ? w_in[i] : w_in[i] + beta * s_in[i];
+    x[i] = x[i] + alpha * pi;
+    const float rn = r_in[i] - alpha * si;
+    p[i] = pi;
+    s_out[i] = si;
+    r_out[i] = rn;
+    w_out[i] = wn;
+    const float un = rn * ii;
+    acc[0] += (double)rn * (double)un;
+    acc[1] += (double)wn * (double)un;
+    acc[2] += (double)rn * (double)rn;
+  }
+  store_partials<3>(acc, P_out, G);
+}
+
 // ------------------------------------------------------------- BiCGSTAB
 // partial slots: PA[3][G] = {r0.r, r.r, b.b} (init, k_bi_3), PB[1][G] = {r0.v}
 // (k_bi_1), PC[3][G] = {t.s, t.t, s.s} (k_bi_2)
@@ -470,8 +579,8 @@ static HostSync* host_sync() {
 // soon as a polled group reports done; groups already queued find the flag
 // set and exit at entry, so the result is the same as checking after every
 // group synchronously (the reference solver's check interval).
-template <class LaunchGroup>
-static int run_groups(KState* st, const double* rr_part, unsigned G, int maxiter, int ce,
+template <class LaunchGroup, class RrPart>
+static int run_groups(KState* st, RrPart&& rr_part, unsigned G, int maxiter, int ce,
                       LaunchGroup&& launch_group, wcfd_solve_info* info, hipStream_t s) {
   HostSync* hs = host_sync();
   if (!hs) return WCFD_EHIP;
@@ -496,7 +605,7 @@ static int run_groups(KState* st, const double* rr_part, unsigned G, int maxiter
     if (it < cap && launched - polled < depth) {
       launch_group(it);
       it += ce;
-      WCFD_LAUNCH(K_FOLD, s, k_check, 1, rr_part, (int)G, st, it, cap);
+      WCFD_LAUNCH(K_FOLD, s, k_check, 1, rr_part(it), (int)G, st, it, cap);
       const int sl = launched % kSlots;
       if ((e = check_launch())) return e;
       if ((e = check_hip(hipMemcpyAsync(&hs->slot[sl], &st->f, sizeof(KFlags),
@@ -561,7 +670,40 @@ static int cg_impl(const M& A, int n, const float* b, float* x, double atol, dou
                     w.st);
     }
   };
-  return run_groups(w.st, w.PA + G, G, maxiter, ce, group, info, s);
+  auto rr_part = [&](int) { return (const double*)(w.PA + G); };
+  return run_groups(w.st, rr_part, G, maxiter, ce, group, info, s);
+}
+
+// Chronopoulos-Gear CG: one fused kernel per iteration.
+template <class M>
+static int cgs_impl(const M& A, int n, const float* b, float* x, double atol, double tol,
+                    int maxiter, int check_every, void* workspace, wcfd_solve_info* info,
+                    hipStream_t s) {
+  KWork w = carve_krylov(workspace, n, 1);
+  int e = set_tolerances(w.st, atol, tol, s);
+  if (e) return e;
+  const int B = krylov_launch().block;
+  const unsigned g = grid_for(n), G = krylov_grid(n, B);
+  float* rb[2] = {w.r, w.r0};
+  float* wb[2] = {w.v0, w.v1};
+  float* sb[2] = {w.s, w.t};
+  double* pb[2] = {w.PA, w.PC};   // iteration i reads pb[i&1], writes pb[(i+1)&1]
+  WCFD_LAUNCH(K_INV_DIAG, s, k_inv_diag<M>, g, A, n, w.inv);
+  WCFD_LAUNCH_B(K_CG_INIT, s, k_cgs_init1<M>, G, B, A, n, b, x, rb[0], w.PC);
+  WCFD_LAUNCH_B(K_CG_INIT, s, k_cgs_init2<M>, G, B, A, n, rb[0], w.inv, wb[0], w.PA);
+  WCFD_LAUNCH(K_FOLD, s, k_cgs_init_fold, 1, w.PA, w.PC, (int)G, w.st, maxiter);
+  if ((e = check_launch())) return e;
+  const int ce = check_every > 0 ? check_every : 1;
+  auto group = [&](int it0) {
+    for (int it = it0; it < it0 + ce; ++it) {
+      const int cur = it & 1;
+      WCFD_LAUNCH_B(K_CG_1, s, k_cgs_iter<M>, G, B, A, n, it, x, rb[cur], rb[1 - cur], wb[cur],
+                    wb[1 - cur], sb[cur], sb[1 - cur], w.p0, w.inv, pb[cur], pb[1 - cur], w.st);
+    }
+  };
+  // the r.r partials of the last iteration (it_end - 1) are in pb[it_end & 1], slot 2
+  auto rr_part = [&](int it_end) { return (const double*)(pb[it_end & 1] + 2 * G); };
+  return run_groups(w.st, rr_part, G, maxiter, ce, group, info, s);
 }
 
 template <class M, int NV>
@@ -591,7 +733,8 @@ static int bicg_impl(const M& A, int n, const float* b, float* x, double atol, d
                     w.PC, w.PA, w.st);
     }
   };
-  return run_groups(w.st, w.PA + G, G, maxiter, ce, group, info, s);
+  auto rr_part = [&](int) { return (const double*)(w.PA + G); };
+  return run_groups(w.st, rr_part, G, maxiter, ce, group, info, s);
 }
 
 template <class M>
@@ -603,10 +746,21 @@ static int bicg_dispatch(const M& A, int n, int nvec, const float* b, float* x, 
   return WCFD_EINVAL;
 }
 
-int solve_cg(const CsrView& A, const float* b, float* x, double atol, double tol, int maxiter,
-             int check_every, void* workspace, wcfd_solve_info* info, hipStream_t s) {
+template <class M>
+static int cg_dispatch(const M& A, int n, int variant, const float* b, float* x, double atol,
+                       double tol, int maxiter, int check_every, void* ws, wcfd_solve_info* info,
+                       hipStream_t s) {
+  if (variant == WCFD_CG_SINGLE_REDUCTION)
+    return cgs_impl(A, n, b, x, atol, tol, maxiter, check_every, ws, info, s);
+  if (variant == WCFD_CG_STANDARD)
+    return cg_impl(A, n, b, x, atol, tol, maxiter, check_every, ws, info, s);
+  return WCFD_EINVAL;
+}
+
+int solve_cg(const CsrView& A, int variant, const float* b, float* x, double atol, double tol,
+             int maxiter, int check_every, void* workspace, wcfd_solve_info* info, hipStream_t s) {
   const CsrMat M{A.n, A.row_ptr, A.col, A.values, A.diag_pos};
-  return cg_impl(M, A.n, b, x, atol, tol, maxiter, check_every, workspace, info, s);
+  return cg_dispatch(M, A.n, variant, b, x, atol, tol, maxiter, check_every, workspace, info, s);
 }
 
 int solve_bicgstab(const CsrView& A, int nvec, const float* b, float* x, double atol, double tol,
@@ -616,16 +770,16 @@ int solve_bicgstab(const CsrView& A, int nvec, const float* b, float* x, double 
   return bicg_dispatch(M, A.n, nvec, b, x, atol, tol, maxiter, check_every, workspace, info, s);
 }
 
-int solve_cg_ell(const wcfd_mesh& m, const float* values, const float* b, float* x, double atol,
-                 double tol, int maxiter, int check_every, void* workspace, wcfd_solve_info* info,
-                 hipStream_t s) {
+int solve_cg_ell(const wcfd_mesh& m, const float* values, int variant, const float* b, float* x,
+                 double atol, double tol, int maxiter, int check_every, void* workspace,
+                 wcfd_solve_info* info, hipStream_t s) {
   const int n = m.n_cells;
   if (m.slot_dnb) {
     const EllMat<true> M{n, values, m.slot_nbr, m.slot_dnb};
-    return cg_impl(M, n, b, x, atol, tol, maxiter, check_every, workspace, info, s);
+    return cg_dispatch(M, n, variant, b, x, atol, tol, maxiter, check_every, workspace, info, s);
   }
   const EllMat<false> M{n, values, m.slot_nbr, m.slot_dnb};
-  return cg_impl(M, n, b, x, atol, tol, maxiter, check_every, workspace, info, s);
+  return cg_dispatch(M, n, variant, b, x, atol, tol, maxiter, check_every, workspace, info, s);
 }
 
 int solve_bicgstab_ell(const wcfd_mesh& m, const float* values, int nvec, const float* b, float* x,
@@ -713,20 +867,21 @@ size_t wcfd_krylov_workspace_bytes(int32_t n, int nvec) {
 }
 
 int wcfd_cg(int32_t n, const int32_t* row_ptr, const int32_t* col, const float* values,
-            const int32_t* diag_pos, const float* b, float* x, double atol, double tol, int maxiter,
-            int check_every, void* workspace, wcfd_solve_info* info, void* stream) {
+            const int32_t* diag_pos, int variant, const float* b, float* x, double atol, double tol,
+            int maxiter, int check_every, void* workspace, wcfd_solve_info* info, void* stream) {
   if (n <= 0 || !row_ptr || !col || !values || !diag_pos || !b || !x || !workspace)
     return WCFD_EINVAL;
   CsrView A{n, row_ptr, col, values, diag_pos};
-  return solve_cg(A, b, x, atol, tol, maxiter, check_every, workspace, info, (hipStream_t)stream);
+  return solve_cg(A, variant, b, x, atol, tol, maxiter, check_every, workspace, info,
+                  (hipStream_t)stream);
 }
 
-int wcfd_cg_ell(const wcfd_mesh* m, const float* values, const float* b, float* x, double atol,
-                double tol, int maxiter, int check_every, void* workspace, wcfd_solve_info* info,
-                void* stream) {
+int wcfd_cg_ell(const wcfd_mesh* m, const float* values, int variant, const float* b, float* x,
+                double atol, double tol, int maxiter, int check_every, void* workspace,
+                wcfd_solve_info* info, void* stream) {
   if (!m || m->n_cells <= 0 || !m->slot_nbr || !values || !b || !x || !workspace)
     return WCFD_EINVAL;
-  return solve_cg_ell(*m, values, b, x, atol, tol, maxiter, check_every, workspace, info,
+  return solve_cg_ell(*m, values, variant, b, x, atol, tol, maxiter, check_every, workspace, info,
                       (hipStream_t)stream);
 }
 

@@ -87,8 +87,8 @@ int wcfd_simple_iteration(const wcfd_mesh* mp, wcfd_bc bc, const wcfd_simple_par
                                         prm->ref_cell,
                                         prm->ref_value, st->p_values, st->p_rhs, nullptr, s)))
         return e;
-      e = solve_cg_ell(m, st->p_values, st->p_rhs, st->p_x, prm->atol, prm->tol, prm->maxiter,
-                       prm->check_every, w.krylov, &inf->pressure, s);
+      e = solve_cg_ell(m, st->p_values, prm->cg_variant, st->p_rhs, st->p_x, prm->atol, prm->tol,
+                       prm->maxiter, prm->check_every, w.krylov, &inf->pressure, s);
       if (e) return e;
       if ((e = launch_relax_p(m, st->p_x, prm->alpha_p, st->cell_values, s))) return e;
       if ((e = launch_correct_velocity(m, bc, st->cell_values, st->cell_grad, st->face_values,

@@ -34,7 +34,7 @@ def _laplacian_ell(dm):
     return ell.contiguous()
 
 
-def time_solvers(dm, grid, block, iters=60, reps=2):
+def time_solvers(dm, grid, block, iters=60, reps=2, cg_variant=0):
     """Milliseconds of `iters` CG iterations and `iters` 3-vector BiCGSTAB
     iterations at launch shape (grid, block)."""
     lib = _lib.load()
@@ -56,8 +56,8 @@ def time_solvers(dm, grid, block, iters=60, reps=2):
             torch.cuda.synchronize()
             e0.record()
             if nv == 1:
-                code = lib.wcfd_cg_ell(dm.desc_ref, ptr(A), ptr(b), ptr(x), 0.0, 0.0, iters, iters,
-                                       ptr(ws), ctypes.byref(info), stream_ptr())
+                code = lib.wcfd_cg_ell(dm.desc_ref, ptr(A), cg_variant, ptr(b), ptr(x), 0.0, 0.0, iters,
+                                       iters, ptr(ws), ctypes.byref(info), stream_ptr())
             else:
                 code = lib.wcfd_bicgstab_ell(dm.desc_ref, ptr(-A), 3, ptr(b3), ptr(x), 0.0, 0.0, iters,
                                              iters, ptr(ws), ctypes.byref(info), stream_ptr())
@@ -68,11 +68,12 @@ def time_solvers(dm, grid, block, iters=60, reps=2):
     return best
 
 
-def autotune_krylov(dm, grids=GRIDS, blocks=BLOCKS, iters=60, cg_weight=1.0, bicg_weight=0.1):
+def autotune_krylov(dm, grids=GRIDS, blocks=BLOCKS, iters=60, cg_weight=1.0, bicg_weight=0.1,
+                    cg_variant=0):
     """Pick and install the fastest (grid, block) for mesh ``dm``
     (a DeviceMesh).  The weights reflect a SIMPLE iteration, where the
     pressure CG runs far more iterations than the momentum BiCGSTAB."""
-    key = dm.n_cells
+    key = (dm.n_cells, cg_variant)
     if key in _cache:
         g, b = _cache[key]['grid'], _cache[key]['block']
         _lib.check(_lib.load().wcfd_set_krylov_launch(g, b), 'set_krylov_launch')
@@ -82,7 +83,7 @@ def autotune_krylov(dm, grids=GRIDS, blocks=BLOCKS, iters=60, cg_weight=1.0, bic
         for grid in grids:
             if grid * blk > 4 * dm.n_cells and grid > grids[0]:
                 continue            # more threads than rows: no gain
-            cg_ms, bi_ms = time_solvers(dm, grid, blk, iters)
+            cg_ms, bi_ms = time_solvers(dm, grid, blk, iters, cg_variant=cg_variant)
             results.append({'grid': grid, 'block': blk, 'cg_ms': cg_ms, 'bicg_ms': bi_ms,
                             'score': cg_weight * cg_ms + bicg_weight * bi_ms})
     best = min(results, key=lambda r: r['score'])
