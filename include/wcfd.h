@@ -87,6 +87,37 @@ typedef struct wcfd_bc {
   const float* value;   /* [4][Fb] value or normal gradient */
 } wcfd_bc;
 
+/* Batch of independent cases on one mesh descriptor (no counterpart in the
+ * reference, which runs one case per FVM object).  The cases' meshes are
+ * placed side by side in one wcfd_mesh: case k owns the cells
+ * [cell_off[k], cell_end[k]) and a range of internal and of boundary
+ * faces; no face or matrix coefficient couples two cases.  Cells and faces
+ * that belong to no case (free space of a serving arena) have case -1.
+ * Row-parallel kernels (Krylov, continuity) give every case the workgroups
+ * [blk_off[k], blk_off[k+1]) of `block` threads, each owning `chunk`
+ * consecutive rows, so a case's arithmetic -- including every float64 sum --
+ * is bitwise the same alone or beside any other cases.  Kernels skip cases
+ * whose active flag is 0; the SIMPLE step clears the flag of a case whose
+ * continuity residual reaches its stop value. */
+typedef struct wcfd_batch {
+  int32_t n_cases;          /* K */
+  int32_t n_blocks;         /* G = blk_off[K] */
+  int32_t chunk;            /* rows per workgroup */
+  int32_t block;            /* threads per workgroup (64..1024, multiple of 64) */
+  const int32_t* cell_off;  /* [K+1] first cell of case k */
+  const int32_t* cell_end;  /* [K] one past the last cell of case k; NULL: cell_off[k+1] */
+  const int32_t* blk_off;   /* [K+1] */
+  const int32_t* blk_case;  /* [G] case of each workgroup */
+  const int32_t* cell_case; /* [C] case of each cell, -1 if none */
+  const int32_t* face_case; /* [F] case of each face, -1 if none */
+  uint8_t* active;          /* [K] 1 = iterate the case */
+  const float* nu;          /* [K] kinematic viscosity per case */
+  const int32_t* ref_cell;  /* [K] reference-pressure row per case, -1 none */
+  const float* ref_value;   /* [K] */
+  const double* stop;       /* [K] continuity residual at which the case stops, 0 never */
+  int32_t* outer_iters;     /* [K] SIMPLE iterations run per case */
+} wcfd_batch;
+
 /* ---------------------------------------------------------------- errors */
 int wcfd_last_hip_error(void);
 const char* wcfd_error_string(int code); /* [host] static string */
@@ -256,7 +287,8 @@ typedef struct wcfd_simple_params {
   float ref_value;
   int32_t n_inner;     /* NUM_INNER_LOOPS */
   int32_t n_ortho;     /* NUM_ORTHOGONAL_CORRECTORS */
-  double atol, tol;    /* Krylov tolerances */
+  double atol, tol;    /* Krylov tolerances: stop at ||r|| <= max(tol ||b||, rtol0 ||r_0||, atol) */
+  double rtol0;        /* relative to the warm-start residual r_0; 0 = off (the reference) */
   int32_t maxiter, check_every;
   int32_t first_iteration; /* 1: u,v,w gradients may be non-zero on entry */
   int32_t cg_variant;      /* WCFD_CG_* for the pressure equation */
@@ -293,6 +325,34 @@ size_t wcfd_simple_workspace_bytes(int32_t n_cells, int32_t n_faces);
 int wcfd_simple_iteration(const wcfd_mesh* m, wcfd_bc bc, const wcfd_simple_params* prm,
                           const wcfd_simple_state* st, void* workspace,
                           wcfd_simple_info* info, void* stream);
+
+/* ------------------------------------------------------------ batched path
+ * The same operations on a wcfd_batch (declared above).  Every case has its
+ * own Krylov scalars, stopping test and iteration count; `info` arrays are
+ * [host] [K].  Workspaces are sized for (C, K, G). */
+size_t wcfd_batch_krylov_workspace_bytes(int32_t n, int32_t n_cases, int32_t n_blocks);
+int wcfd_batch_cg_ell(const wcfd_mesh* m, const wcfd_batch* b, const float* values, int variant,
+                      const float* rhs, float* x, double atol, double tol, int maxiter,
+                      int check_every, void* workspace, wcfd_solve_info* info, void* stream);
+int wcfd_batch_bicgstab_ell(const wcfd_mesh* m, const wcfd_batch* b, const float* values,
+                            int nvec, const float* rhs, float* x, double atol, double tol,
+                            int maxiter, int check_every, void* workspace,
+                            wcfd_solve_info* info, void* stream);
+
+size_t wcfd_batch_simple_workspace_bytes(int32_t n_cells, int32_t n_faces, int32_t n_cases,
+                                         int32_t n_blocks);
+
+/* One SIMPLE outer iteration of every active case of the batch
+ * (IncompressibleSolver.run, incompressible.py:95-156, per case).  prm's
+ * nu, ref_cell and ref_value are ignored (per case in the batch).
+ * st->resid is [K][2].  After the corrector each active case's continuity
+ * residual is compared with b->stop[k]; cases at or below it get active = 0
+ * (on the device, no host round trip) and b->outer_iters[k] counts the
+ * iterations each case ran.  info is [host] [K] or NULL (no sync at the
+ * end). */
+int wcfd_batch_simple_iteration(const wcfd_mesh* m, const wcfd_batch* b, wcfd_bc bc,
+                                const wcfd_simple_params* prm, const wcfd_simple_state* st,
+                                void* workspace, wcfd_simple_info* info, void* stream);
 
 /* ---------------------------------------------------------------- profiler
  * No counterpart in the reference (it has no timers).  While active, every

@@ -6,7 +6,9 @@ preconditioner, atol = 1e-5 (float32) or 1e-6 (float64), maxiter = 500
 absent here, so this is our statement of those published algorithms:
 
 * stopping threshold  thr = max(tol * ||b||_2, atol) with tol = atol (the
-  reference passes atol only);
+  reference passes atol only).  ``rtol0`` > 0 adds rtol0 * ||r_0||_2 (the
+  warm-start residual) to the max -- not in the reference; the HIP path
+  offers it for runs to a continuity target;
 * the residual checked is the recurrence residual r;
 * the check happens before the first iteration and after every
   ``check_every`` = 10 iterations; at most ``maxiter`` iterations;
@@ -29,17 +31,20 @@ def _safe_div(a, b):
     return a / b if b != 0.0 else 0.0
 
 
-def threshold(b, atol, tol=None):
+def threshold(b, atol, tol=None, r0=None, rtol0=0.0):
     tol = atol if tol is None else tol
-    return max(tol * np.sqrt(_dot(b, b)), atol)
+    t = max(tol * np.sqrt(_dot(b, b)), atol)
+    if rtol0 and r0 is not None:
+        t = max(t, rtol0 * np.sqrt(_dot(r0, r0)))
+    return t
 
 
-def cg(A, b, x, inv_diag, atol=1e-5, tol=None, maxiter=500, check_every=10):
+def cg(A, b, x, inv_diag, atol=1e-5, tol=None, maxiter=500, check_every=10, rtol0=0.0):
     """Jacobi-preconditioned conjugate gradients on CSR ``A``; returns
     (iterations, ||r||)."""
     dt = x.dtype.type
-    thr2 = threshold(b, atol, tol) ** 2
     r = (b - A @ x).astype(x.dtype)
+    thr2 = threshold(b, atol, tol, r, rtol0) ** 2
     z = (r * inv_diag).astype(x.dtype)
     p = z.copy()
     rz = _dot(r, z)
@@ -64,13 +69,13 @@ def cg(A, b, x, inv_diag, atol=1e-5, tol=None, maxiter=500, check_every=10):
             return it, np.sqrt(rr)
 
 
-def bicgstab(A, b, x, inv_diag, atol=1e-5, tol=None, maxiter=500, check_every=10):
+def bicgstab(A, b, x, inv_diag, atol=1e-5, tol=None, maxiter=500, check_every=10, rtol0=0.0):
     """Right-Jacobi-preconditioned BiCGSTAB on CSR ``A`` (all components of
     the interleaved momentum vector share one set of scalars, as in the
     reference's single 3C solve); returns (iterations, ||r||)."""
     dt = x.dtype.type
-    thr2 = threshold(b, atol, tol) ** 2
     r = (b - A @ x).astype(x.dtype)
+    thr2 = threshold(b, atol, tol, r, rtol0) ** 2
     r0 = r.copy()
     p = r.copy()
     v = np.zeros_like(r)
@@ -101,14 +106,14 @@ def bicgstab(A, b, x, inv_diag, atol=1e-5, tol=None, maxiter=500, check_every=10
             return it, np.sqrt(rr)
 
 
-def cg_single_reduction(A, b, x, inv_diag, atol=1e-5, tol=None, maxiter=500, check_every=10):
+def cg_single_reduction(A, b, x, inv_diag, atol=1e-5, tol=None, maxiter=500, check_every=10, rtol0=0.0):
     """Chronopoulos-Gear PCG: the same iterates as ``cg`` in exact arithmetic,
     with both inner products of an iteration taken together (one global
     reduction per iteration).  Restates the variant the HIP path can run as
     one fused kernel per iteration; same stopping rule as ``cg``."""
     dt = x.dtype.type
-    thr2 = threshold(b, atol, tol) ** 2
     r = (b - A @ x).astype(x.dtype)
+    thr2 = threshold(b, atol, tol, r, rtol0) ** 2
     u = (r * inv_diag).astype(x.dtype)
     w = (A @ u).astype(x.dtype)
     gamma = _dot(r, u)

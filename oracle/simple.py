@@ -25,7 +25,7 @@ from . import krylov
 class OracleSolver:
     def __init__(self, fvm, u_relax=0.7, p_relax=0.3, correction=False, steady_state=True,
                  nu=0.01, rho=1.0, atol=None, maxiter=500, check_every=10,
-                 implicit_outflow=False, cg_variant=0):
+                 implicit_outflow=False, cg_variant=0, tol=None, rtol0=0.0):
         self.fvm = fvm
         self.u_relax = u_relax
         self.p_relax = p_relax
@@ -35,6 +35,8 @@ class OracleSolver:
         self.rho = rho
         d = fvm.dtype
         self.atol = atol if atol is not None else (1e-5 if d == np.float32 else 1e-6)
+        self.tol = tol              # None: = atol, as the reference's call
+        self.rtol0 = rtol0
         self.maxiter = maxiter
         self.check_every = check_every
         self.implicit_outflow = implicit_outflow
@@ -123,8 +125,9 @@ class OracleSolver:
         Ap = self.vel_eq.diagonal
         A = self.vel_eq.matrix()
         inv = (1.0 / Ap).astype(d)
-        it_u, res_u = krylov.bicgstab(A, self.vel_eq.rhs, self.HbyA, inv, self.atol,
-                                      maxiter=self.maxiter, check_every=self.check_every)
+        it_u, res_u = krylov.bicgstab(A, self.vel_eq.rhs, self.HbyA, inv, self.atol, self.tol,
+                                      maxiter=self.maxiter, check_every=self.check_every,
+                                      rtol0=self.rtol0)
         info['u_iters'] = it_u
         p_grad = fvm.get_gradient(3).reshape(-1)
         self._rua(Ap)
@@ -145,7 +148,8 @@ class OracleSolver:
                 Pd = self.p_eq.diagonal
                 invp = (1.0 / Pd).astype(d)
                 it_p, res_p = self.cg(self.p_eq.matrix(), self.p_eq.rhs, self.p_eq.x, invp,
-                                        self.atol, maxiter=self.maxiter, check_every=self.check_every)
+                                      self.atol, self.tol, maxiter=self.maxiter,
+                                      check_every=self.check_every, rtol0=self.rtol0)
                 p_iters.append(it_p)
                 p_new = self.p_eq.x
                 if self.p_relax is not None:

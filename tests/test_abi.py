@@ -47,7 +47,7 @@ def test_struct_layouts_match_header(tmp_path):
     out = {k: int(v) for k, v in out.items()}
     S = {'wcfd_mesh': _lib.WcfdMesh, 'wcfd_bc': _lib.WcfdBC, 'wcfd_solve_info': _lib.SolveInfo,
          'wcfd_simple_params': _lib.SimpleParams, 'wcfd_simple_state': _lib.SimpleState,
-         'wcfd_simple_info': _lib.SimpleInfo}
+         'wcfd_simple_info': _lib.SimpleInfo, 'wcfd_batch': _lib.WcfdBatch}
     for name, cls in S.items():
         assert ctypes.sizeof(cls) == out[name], name
     for key, val in out.items():

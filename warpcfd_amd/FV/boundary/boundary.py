@@ -87,10 +87,16 @@ class Boundary:
         t = self.boundary_type
         return bool(np.all(t[:, 0] == t[:, 1]) and np.all(t[:, 0] == t[:, 2]))
 
-    def device_arrays(self, layout, device):
+    def host_arrays(self, layout):
+        """BC type (uint8) and value (float32) as [4][Fb] in device face
+        order."""
         rows = layout.bnd_ref_row
-        t = np.ascontiguousarray(self.boundary_type[rows].T)
+        t = np.ascontiguousarray(self.boundary_type[rows].T).astype(np.uint8)
         v = np.ascontiguousarray(self.boundary_values[rows].T.astype(np.float32))
+        return t, v
+
+    def device_arrays(self, layout, device):
+        t, v = self.host_arrays(layout)
         if t.size == 0:
             t = np.zeros((4, 1), np.uint8)
             v = np.zeros((4, 1), np.float32)

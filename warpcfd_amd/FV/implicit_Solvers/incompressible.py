@@ -54,6 +54,9 @@ class IncompressibleSolver:
         self.max_iter = 500
         self.tol = 1e-5          # equation.py:36-39 (float32)
         self.rtol = None         # None: relative tolerance = atol, as the reference's call
+        # not in the reference: also stop at rtol0 * ||r_0|| (the warm-start
+        # residual); 0 = off
+        self.rtol0 = 0.0
         self.check_every = 10
         # pressure CG: WCFD_CG_STANDARD (the reference's two-reduction form) or
         # WCFD_CG_SINGLE_REDUCTION (same iterates in exact arithmetic, one
@@ -101,6 +104,7 @@ class IncompressibleSolver:
         p.n_ortho = int(self.NUM_ORTHOGONAL_CORRECTORS)
         p.atol = float(self.tol)
         p.tol = float(self.tol if self.rtol is None else self.rtol)
+        p.rtol0 = float(self.rtol0)
         p.maxiter = int(self.max_iter)
         p.check_every = int(self.check_every)
         p.first_iteration = 0 if m._uvw_grad_zero else 1

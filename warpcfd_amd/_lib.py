@@ -45,10 +45,17 @@ class SolveInfo(ctypes.Structure):
         return (self.iterations, self.residual, self.threshold)
 
 
+class WcfdBatch(ctypes.Structure):
+    _fields_ = [('n_cases', c_i32), ('n_blocks', c_i32), ('chunk', c_i32), ('block', c_i32),
+                ('cell_off', c_p), ('cell_end', c_p), ('blk_off', c_p), ('blk_case', c_p), ('cell_case', c_p),
+                ('face_case', c_p), ('active', c_p), ('nu', c_p), ('ref_cell', c_p),
+                ('ref_value', c_p), ('stop', c_p), ('outer_iters', c_p)]
+
+
 class SimpleParams(ctypes.Structure):
     _fields_ = [('nu', c_f), ('density', c_f), ('alpha_u', c_f), ('alpha_p', c_f), ('dt', c_f),
                 ('scheme_flags', c_i32), ('ref_cell', c_i32), ('ref_value', c_f),
-                ('n_inner', c_i32), ('n_ortho', c_i32), ('atol', c_d), ('tol', c_d),
+                ('n_inner', c_i32), ('n_ortho', c_i32), ('atol', c_d), ('tol', c_d), ('rtol0', c_d),
                 ('maxiter', c_i32), ('check_every', c_i32), ('first_iteration', c_i32),
                 ('cg_variant', c_i32)]
 
@@ -64,6 +71,7 @@ class SimpleInfo(ctypes.Structure):
 
 
 _PM = ctypes.POINTER(WcfdMesh)
+_PB = ctypes.POINTER(WcfdBatch)
 _SIGS = {
     'wcfd_last_hip_error': (ctypes.c_int, []),
     'wcfd_error_string': (ctypes.c_char_p, [ctypes.c_int]),
@@ -107,6 +115,17 @@ _SIGS = {
     'wcfd_simple_iteration': (ctypes.c_int, [_PM, WcfdBC, ctypes.POINTER(SimpleParams),
                                              ctypes.POINTER(SimpleState), c_p,
                                              ctypes.POINTER(SimpleInfo), c_p]),
+    'wcfd_batch_krylov_workspace_bytes': (ctypes.c_size_t, [c_i32, c_i32, c_i32]),
+    'wcfd_batch_cg_ell': (ctypes.c_int, [_PM, _PB, c_p, ctypes.c_int, c_p, c_p, c_d, c_d,
+                                         ctypes.c_int, ctypes.c_int, c_p, ctypes.POINTER(SolveInfo),
+                                         c_p]),
+    'wcfd_batch_bicgstab_ell': (ctypes.c_int, [_PM, _PB, c_p, ctypes.c_int, c_p, c_p, c_d, c_d,
+                                               ctypes.c_int, ctypes.c_int, c_p,
+                                               ctypes.POINTER(SolveInfo), c_p]),
+    'wcfd_batch_simple_workspace_bytes': (ctypes.c_size_t, [c_i32, c_i32, c_i32, c_i32]),
+    'wcfd_batch_simple_iteration': (ctypes.c_int, [_PM, _PB, WcfdBC, ctypes.POINTER(SimpleParams),
+                                                   ctypes.POINTER(SimpleState), c_p,
+                                                   ctypes.POINTER(SimpleInfo), c_p]),
 }
 
 _lib = None

@@ -1,0 +1,481 @@
+"""Many independent steady cases on one GPU: a device arena and batched SIMPLE.
+
+No counterpart in the reference, which solves one case per ``FVM`` object
+(incompressible.py:95-156 is the per-case step repeated here for every case).
+
+A ``CaseArena`` holds, at a fixed capacity, the mesh arrays, fields, matrices
+and boundary conditions of many cases side by side in one ``wcfd_mesh``:
+every case takes a contiguous range of cells, of internal faces and of
+boundary faces (first fit), with its cell and face indices offset into the
+arena.  No face couples two cases, so the arena's matrices are block
+diagonal.  One ``wcfd_batch_simple_iteration`` call advances every active
+case by one SIMPLE iteration: face and cell kernels run over the whole arena
+and skip cells and faces whose case is inactive or absent; the Krylov and
+continuity reductions give each case its own workgroups, partials, scalars
+and stopping test.  A case's arithmetic therefore does not depend on which
+other cases share the arena or where it sits in it (its results are bitwise
+the same alone or in a mix).  The device clears a case's active flag when
+its continuity residual reaches the case's stop value; the host only polls
+the flags.
+
+Cases are admitted and released while others run (``admit`` / ``release``),
+which is what the serving loop in ``warpcfd_amd.serve`` builds on.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .FV.device import ptr, require_gpu, stream_ptr
+from .preprocess.layout import DeviceLayout
+
+
+class CaseSpec:
+    """One steady case prepared on the host for an arena.
+
+    ``model`` is a ``warpcfd_amd.FV.FVM`` whose boundary conditions, material
+    and reference pressure are set (it need not be finalized; no device
+    memory of its own is used).  ``stop`` is the continuity residual at which
+    the case is converged, ``max_outer`` a cap on its SIMPLE iterations.
+    """
+
+    def __init__(self, model, stop=1e-6, max_outer=20000, name=None, layout=None):
+        b = model.boundary
+        b.finalize()
+        if not b.uniform_velocity_types():
+            raise NotImplementedError('u, v and w must share the boundary-condition type on every '
+                                      'face')
+        L = layout if layout is not None else DeviceLayout(model.mesh, renumber=model._renumber)
+        if L.slot_dnb is None:
+            raise ValueError('the case band does not fit int16 neighbour offsets')
+        self.layout = L
+        self.bc_type, self.bc_value = b.host_arrays(L)
+        rho = float(model.material.density)
+        self.density = rho
+        self.nu = float(model.material.viscosity) / rho
+        rc = model.reference_pressure_cell_id
+        self.ref_cell = -1 if rc is None else int(L.old2new[rc])
+        self.ref_value = float(model.reference_pressure)
+        self.stop = float(stop)
+        self.max_outer = int(max_outer)
+        self.name = name
+        self.n_cells = L.num_cells
+        self.n_internal = L.n_internal
+        self.n_boundary = L.n_boundary
+
+    def device_bytes(self):
+        """Arena bytes the case occupies (mesh, fields, matrices, BCs)."""
+        C, Fi, Fb = self.n_cells, self.n_internal, self.n_boundary
+        return CaseArena.bytes_for(C, Fi, Fb)
+
+
+class _Ranges:
+    """First-fit allocator of contiguous index ranges in [0, cap)."""
+
+    def __init__(self, cap):
+        self.cap = int(cap)
+        self.free = [(0, self.cap)]          # sorted, disjoint, coalesced
+
+    def alloc(self, n):
+        if n == 0:
+            return 0
+        for i, (a, m) in enumerate(self.free):
+            if m >= n:
+                if m == n:
+                    self.free.pop(i)
+                else:
+                    self.free[i] = (a + n, m - n)
+                return a
+        return None
+
+    def release(self, a, n):
+        if n == 0:
+            return
+        self.free.append((a, n))
+        self.free.sort()
+        out = []
+        for s, m in self.free:
+            if out and out[-1][0] + out[-1][1] == s:
+                out[-1] = (out[-1][0], out[-1][1] + m)
+            else:
+                out.append((s, m))
+        self.free = out
+
+    def largest(self):
+        return max((m for _, m in self.free), default=0)
+
+
+class CaseArena:
+    """Fixed-capacity device arena of up to ``max_cases`` cases.
+
+    Capacities are in cells, internal faces and boundary faces.  ``chunk``
+    rows per workgroup of ``block`` threads define every case's reduction
+    workgroups (keep them fixed to compare results across arenas bitwise).
+    Solver settings are shared by all cases: relaxation factors, Krylov
+    tolerances, density and scheme flags.
+    """
+
+    # bytes per cell / internal face / boundary face held by the arena
+    CELL_BYTES = (4 + 6 * 4 + 6 + 3 * 6 * 4 + 6 * 4 + 6 * 2 + 4 + 4 * 4 + 12 * 4 + 3 * 4 + 4 + 4 +
+                  7 * 4 + 3 * 4 + 7 * 4 + 4)
+    FACE_BYTES = 4 + 4 + 3 * 4 + 4 + 4 + 4 * 4 + 4 + 4
+    INTERNAL_BYTES = 4 + 4 + 3 * 4 + 3 * 4
+    BOUNDARY_BYTES = 4 + 4 * 4
+
+    @classmethod
+    def bytes_for(cls, C, Fi, Fb):
+        # fields and matrices + mesh + the Krylov/workspace vectors (27 floats per cell)
+        return (C * (cls.CELL_BYTES + 27 * 4) + (Fi + Fb) * (cls.FACE_BYTES + 4) +
+                Fi * cls.INTERNAL_BYTES + Fb * cls.BOUNDARY_BYTES)
+
+    def __init__(self, cell_cap, internal_cap, boundary_cap, max_cases, device='cuda',
+                 chunk=256, block=256, density=1.0, alpha_u=0.7, alpha_p=0.3, atol=0.0,
+                 tol=1e-6, rtol0=0.0, maxiter=2000, check_every=10,
+                 cg_variant=_lib.CG_SINGLE_REDUCTION, implicit_outflow=True, correction=False):
+        self.device = require_gpu(device)
+        self.lib = _lib.load()
+        Cc, Fic, Fbc, K = int(cell_cap), int(internal_cap), int(boundary_cap), int(max_cases)
+        if min(Cc, Fic, Fbc, K) <= 0:
+            raise ValueError('capacities must be positive')
+        if block % 64 or not 64 <= block <= 1024 or chunk <= 0:
+            raise ValueError('block must be a multiple of 64 in [64, 1024], chunk > 0')
+        self.Cc, self.Fic, self.Fbc, self.K = Cc, Fic, Fbc, K
+        self.F = Fic + Fbc
+        self.chunk, self.block = int(chunk), int(block)
+        self.Gmax = (Cc + chunk - 1) // chunk + K
+        dev = self.device
+        f32, i32 = torch.float32, torch.int32
+        z = lambda *s, dt=f32: torch.zeros(*s, dtype=dt, device=dev)
+        F = self.F
+        # mesh
+        self.volume = z(Cc)
+        self.slot_face = z(6, Cc, dt=i32)
+        self.slot_side = z(6, Cc, dt=torch.int8)
+        self.slot_sn = z(3, 6, Cc)
+        self.slot_nbr = z(6, Cc, dt=i32)
+        self.slot_dnb = z(6, Cc, dt=torch.int16)
+        self.owner = z(F, dt=i32)
+        self.area = z(F)
+        self.normal = z(3, F)
+        self.dmag = torch.ones(F, dtype=f32, device=dev)
+        self.neigh = z(Fic, dt=i32)
+        self.psi = z(Fic)
+        self.skew = z(3, Fic)
+        self.dist = z(3, Fic)
+        self.bc_type = torch.full((4, Fbc), 2, dtype=torch.uint8, device=dev)
+        self.bc_value = z(4, Fbc)
+        self.cell_case = torch.full((Cc,), -1, dtype=i32, device=dev)
+        self.face_case = torch.full((F,), -1, dtype=i32, device=dev)
+        # state
+        self.cv = z(4, Cc)
+        self.cg = z(12, Cc)
+        self.fv = z(4, F)
+        self.flux = z(F)
+        self.hbya = z(3, Cc)
+        self.p_x = z(Cc)
+        self.rua = z(Cc)
+        self.rua_face = z(F)
+        self.mom_values = z(7, Cc)
+        self.mom_rhs = z(3, Cc)
+        self.p_values = z(7, Cc)
+        self.p_rhs = z(Cc)
+        self.resid = torch.zeros(K, 2, dtype=torch.float64, device=dev)
+        # per case
+        self.cell_off = z(K + 1, dt=i32)
+        self.cell_end = z(K, dt=i32)
+        self.blk_off = z(K + 1, dt=i32)
+        self.blk_case = z(self.Gmax, dt=i32)
+        self.active = z(K, dt=torch.uint8)
+        self.nu = z(K)
+        self.ref_cell = torch.full((K,), -1, dtype=i32, device=dev)
+        self.ref_value = z(K)
+        self.stop = torch.zeros(K, dtype=torch.float64, device=dev)
+        self.outer_iters = z(K, dt=i32)
+        self.workspace = torch.zeros(
+            self.lib.wcfd_batch_simple_workspace_bytes(Cc, F, K, self.Gmax), dtype=torch.uint8,
+            device=dev)
+        # host bookkeeping
+        self.cells = _Ranges(Cc)
+        self.ifaces = _Ranges(Fic)
+        self.bfaces = _Ranges(Fbc)
+        self.slots = [None] * K           # slot -> dict(spec, c0, i0, b0)
+        self.h_cell_off = np.zeros(K + 1, np.int32)
+        self.h_cell_end = np.zeros(K, np.int32)
+        self.h_active = np.zeros(K, np.uint8)
+        self.h_iters = np.zeros(K, np.int32)
+        self.n_blocks = 0
+        # settings
+        self.density = float(density)
+        self.alpha_u, self.alpha_p = alpha_u, alpha_p
+        self.atol, self.tol, self.rtol0 = atol, tol, rtol0
+        self.maxiter, self.check_every = maxiter, check_every
+        self.cg_variant = cg_variant
+        self.scheme_flags = (_lib.SCHEME_IMPLICIT_OUTFLOW if implicit_outflow else 0) | \
+            (_lib.SCHEME_CORRECTED if correction else 0)
+        self._build_descriptors()
+
+    # ----------------------------------------------------------- capacity
+    @classmethod
+    def for_cases(cls, specs, max_cases=None, **kw):
+        """An arena exactly large enough for ``specs`` (all admitted)."""
+        C = sum(s.n_cells for s in specs)
+        Fi = sum(s.n_internal for s in specs)
+        Fb = sum(s.n_boundary for s in specs)
+        kw.setdefault('density', specs[0].density)
+        a = cls(max(C, 1), max(Fi, 1), max(Fb, 1), max_cases or len(specs), **kw)
+        for s in specs:
+            if a.admit(s) is None:
+                raise RuntimeError('arena sizing error')
+        return a
+
+    @classmethod
+    def for_budget(cls, hbm_bytes, max_cases, cells_per_face=(2.9, 0.25), **kw):
+        """An arena whose capacity fits ``hbm_bytes`` of device memory, with
+        internal / boundary face capacities in the given ratio to cells."""
+        ri, rb = cells_per_face
+        per_cell = cls.bytes_for(1, ri, rb)
+        C = int(hbm_bytes // per_cell)
+        return cls(C, int(C * ri) + 1, int(C * rb) + 1, max_cases, **kw)
+
+    def fits(self, spec):
+        return (spec.n_cells <= self.cells.largest() and spec.n_internal <= self.ifaces.largest()
+                and spec.n_boundary <= self.bfaces.largest() and None in self.slots)
+
+    # --------------------------------------------------------- admission
+    def admit(self, spec):
+        """Place ``spec`` in the arena with zero initial fields; returns its
+        slot, or None if no slot or no contiguous space is free."""
+        if abs(spec.density - self.density) > 0:
+            raise ValueError('every case of an arena shares the density')
+        if None not in self.slots:
+            return None
+        C, Fi, Fb = spec.n_cells, spec.n_internal, spec.n_boundary
+        c0 = self.cells.alloc(C)
+        if c0 is None:
+            return None
+        i0 = self.ifaces.alloc(Fi)
+        if i0 is None:
+            self.cells.release(c0, C)
+            return None
+        b0 = self.bfaces.alloc(Fb)
+        if b0 is None:
+            self.cells.release(c0, C)
+            self.ifaces.release(i0, Fi)
+            return None
+        k = self.slots.index(None)
+        self.slots[k] = {'spec': spec, 'c0': c0, 'i0': i0, 'b0': b0}
+        self._upload(k)
+        self.h_cell_off[k] = c0
+        self.h_cell_end[k] = c0 + C
+        self.h_active[k] = 1
+        self.h_iters[k] = 0
+        self._build_descriptors()
+        return k
+
+    def release(self, k):
+        """Free slot ``k`` (its cells and faces become free space)."""
+        sl = self.slots[k]
+        if sl is None:
+            return
+        s = sl['spec']
+        c0, i0, b0 = sl['c0'], sl['i0'], sl['b0']
+        self.cell_case[c0:c0 + s.n_cells] = -1
+        self.face_case[i0:i0 + s.n_internal] = -1
+        self.face_case[self.Fic + b0:self.Fic + b0 + s.n_boundary] = -1
+        self.active[k] = 0
+        self.cells.release(c0, s.n_cells)
+        self.ifaces.release(i0, s.n_internal)
+        self.bfaces.release(b0, s.n_boundary)
+        self.slots[k] = None
+        self.h_active[k] = 0
+        self.h_cell_off[k] = 0
+        self.h_cell_end[k] = 0
+        self._build_descriptors()
+
+    def _upload(self, k):
+        sl = self.slots[k]
+        s, c0, i0, b0 = sl['spec'], sl['c0'], sl['i0'], sl['b0']
+        L = s.layout
+        C, Fi, Fb = s.n_cells, s.n_internal, s.n_boundary
+        dev = self.device
+        t = lambda a, dt=None: torch.from_numpy(np.ascontiguousarray(a if dt is None else
+                                                                     np.asarray(a).astype(dt))).to(dev)
+        cs = slice(c0, c0 + C)
+        fi = slice(i0, i0 + Fi)
+        gb = slice(self.Fic + b0, self.Fic + b0 + Fb)    # boundary faces as arena faces
+        lb = slice(b0, b0 + Fb)                          # columns of the BC arrays
+        # local face f -> arena face: internal f -> i0 + f, boundary Fi + j -> Fic + b0 + j
+        sf = L.slot_face.astype(np.int64)
+        sf = np.where(sf < Fi, sf + i0, sf - Fi + self.Fic + b0).astype(np.int32)
+        self.volume[cs] = t(L.volume)
+        self.slot_face[:, cs] = t(sf)
+        self.slot_side[:, cs] = t(L.slot_side)
+        self.slot_sn[:, :, cs] = t(L.slot_sn)
+        self.slot_nbr[:, cs] = t(L.slot_nbr.astype(np.int64) + c0, np.int32)
+        self.slot_dnb[:, cs] = t(L.slot_dnb)
+        own = L.owner.astype(np.int64) + c0
+        self.owner[fi] = t(own[:Fi], np.int32)
+        self.owner[gb] = t(own[Fi:], np.int32)
+        for dst, src in ((self.area, L.area), (self.dmag, L.dmag)):
+            dst[fi] = t(src[:Fi])
+            dst[gb] = t(src[Fi:])
+        self.normal[:, fi] = t(L.normal[:, :Fi])
+        self.normal[:, gb] = t(L.normal[:, Fi:])
+        if Fi:
+            self.neigh[fi] = t(L.neigh.astype(np.int64) + c0, np.int32)
+            self.psi[fi] = t(L.psi)
+            self.skew[:, fi] = t(L.skew)
+            self.dist[:, fi] = t(L.dist)
+        if Fb:
+            self.bc_type[:, lb] = t(s.bc_type)
+            self.bc_value[:, lb] = t(s.bc_value)
+        # zero state of the case's rows
+        for a in (self.cv, self.cg, self.hbya, self.mom_values, self.mom_rhs, self.p_values):
+            a[:, cs] = 0
+        for a in (self.p_x, self.rua, self.p_rhs):
+            a[cs] = 0
+        for a in (self.fv,):
+            a[:, fi] = 0
+            a[:, gb] = 0
+        for a in (self.flux, self.rua_face):
+            a[fi] = 0
+            a[gb] = 0
+        self.cell_case[cs] = k
+        self.face_case[fi] = k
+        self.face_case[gb] = k
+        self.nu[k] = s.nu
+        self.ref_cell[k] = (c0 + s.ref_cell) if s.ref_cell >= 0 else -1
+        self.ref_value[k] = s.ref_value
+        self.stop[k] = s.stop
+        self.outer_iters[k] = 0
+        self.resid[k] = 0
+        self.active[k] = 1
+
+    def _build_descriptors(self):
+        """Workgroup ranges of the resident cases and the C-ABI structs."""
+        K, ch = self.K, self.chunk
+        nb = np.zeros(K, np.int64)
+        for k, sl in enumerate(self.slots):
+            if sl is not None:
+                nb[k] = (sl['spec'].n_cells + ch - 1) // ch
+        blk_off = np.zeros(K + 1, np.int32)
+        blk_off[1:] = np.cumsum(nb)
+        G = int(blk_off[-1])
+        blk_case = np.repeat(np.arange(K, dtype=np.int32), nb)
+        self.n_blocks = G
+        cell_off = self.h_cell_off.copy()
+        cell_off[K] = self.Cc
+        dev = self.device
+        self.cell_off.copy_(torch.from_numpy(cell_off).to(dev))
+        self.cell_end.copy_(torch.from_numpy(self.h_cell_end).to(dev))
+        self.blk_off.copy_(torch.from_numpy(blk_off).to(dev))
+        if G:
+            self.blk_case[:G] = torch.from_numpy(blk_case).to(dev)
+        m = _lib.WcfdMesh()
+        m.n_cells, m.n_faces, m.n_internal, m.n_boundary = self.Cc, self.F, self.Fic, self.Fbc
+        m.nnz, m.reserved = 0, 0
+        for name in ('owner', 'neigh', 'psi', 'skew', 'area', 'normal', 'dist', 'dmag', 'volume',
+                     'slot_face', 'slot_side', 'slot_sn', 'slot_nbr', 'slot_dnb'):
+            setattr(m, name, getattr(self, name).data_ptr())
+        m.slot_pos = m.diag_pos = m.row_ptr = m.col = None
+        self.desc = m
+        b = _lib.WcfdBatch()
+        b.n_cases, b.n_blocks, b.chunk, b.block = K, G, self.chunk, self.block
+        for name in ('cell_off', 'cell_end', 'blk_off', 'blk_case', 'cell_case', 'face_case',
+                     'active', 'nu', 'ref_cell', 'ref_value', 'stop', 'outer_iters'):
+            setattr(b, name, getattr(self, name).data_ptr())
+        self.batch = b
+        self.bc = _lib.WcfdBC(self.bc_type.data_ptr(), self.bc_value.data_ptr())
+        st = _lib.SimpleState()
+        for name, tt in (('cell_values', self.cv), ('cell_grad', self.cg),
+                         ('face_values', self.fv), ('mass_flux', self.flux),
+                         ('hbya', self.hbya), ('p_x', self.p_x), ('rua', self.rua),
+                         ('rua_face', self.rua_face), ('mom_values', self.mom_values),
+                         ('mom_rhs', self.mom_rhs), ('p_values', self.p_values),
+                         ('p_rhs', self.p_rhs), ('resid', self.resid)):
+            setattr(st, name, tt.data_ptr())
+        self.state = st
+
+    # ----------------------------------------------------------- stepping
+    def params(self):
+        p = _lib.SimpleParams()
+        p.nu = 0.0
+        p.density = self.density
+        p.alpha_u = float(self.alpha_u)
+        p.alpha_p = float(self.alpha_p)
+        p.dt = 0.0
+        p.scheme_flags = self.scheme_flags
+        p.ref_cell = -1
+        p.ref_value = 0.0
+        p.n_inner = p.n_ortho = 1
+        p.atol, p.tol, p.rtol0 = float(self.atol), float(self.tol), float(self.rtol0)
+        p.maxiter, p.check_every = int(self.maxiter), int(self.check_every)
+        p.first_iteration = 0
+        p.cg_variant = int(self.cg_variant)
+        return p
+
+    def step(self, info=False, stream=None):
+        """One SIMPLE iteration of every active case.  With ``info`` returns
+        per-slot (momentum iterations, pressure iterations, continuity)."""
+        if self.n_blocks == 0:
+            return None
+        prm = self.params()
+        arr = (_lib.SimpleInfo * self.K)() if info else None
+        code = self.lib.wcfd_batch_simple_iteration(
+            ctypes.byref(self.desc), ctypes.byref(self.batch), self.bc, ctypes.byref(prm),
+            ctypes.byref(self.state), ptr(self.workspace), arr, stream_ptr(stream))
+        _lib.check(code, 'batch simple iteration')
+        if info:
+            return [(arr[k].momentum.iterations, arr[k].pressure.iterations, arr[k].continuity)
+                    for k in range(self.K)]
+        return None
+
+    def poll(self):
+        """Synchronise and read the active flags and iteration counts; cases
+        past their iteration cap are deactivated here.  Returns the slots
+        that are resident and no longer active."""
+        act = self.active.cpu().numpy()
+        its = self.outer_iters.cpu().numpy()
+        self.h_iters[:] = its
+        capped = []
+        for k, sl in enumerate(self.slots):
+            if sl is not None and act[k] and its[k] >= sl['spec'].max_outer:
+                capped.append(k)
+        for k in capped:
+            self.active[k] = 0
+            act[k] = 0
+        self.h_active[:] = act
+        return [k for k, sl in enumerate(self.slots) if sl is not None and not act[k]]
+
+    def continuity(self, k):
+        r = self.resid[k].cpu().numpy()
+        return float(r[0] / r[1]) if r[1] > 0 else 0.0
+
+    def run(self, max_steps=100000, poll_every=1):
+        """Iterate until no resident case is active; returns the number of
+        batch iterations."""
+        n = 0
+        while n < max_steps:
+            self.step()
+            n += 1
+            if n % poll_every == 0:
+                self.poll()
+                if not any(self.h_active[k] for k, sl in enumerate(self.slots) if sl is not None):
+                    break
+        return n
+
+    # ------------------------------------------------------------ results
+    def cell_values(self, k):
+        """(C, 4) float32 numpy array of case ``k`` in its reference cell order."""
+        sl = self.slots[k]
+        s, c0 = sl['spec'], sl['c0']
+        dev = self.cv[:, c0:c0 + s.n_cells].cpu().numpy()
+        return np.ascontiguousarray(dev[:, s.layout.old2new].T)
+
+    def converged(self, k):
+        sl = self.slots[k]
+        return self.continuity(k) <= sl['spec'].stop
+
+    def iterations(self, k):
+        return int(self.outer_iters[k].item())

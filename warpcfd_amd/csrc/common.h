@@ -98,6 +98,12 @@ struct Carver {
   }
 };
 
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
 // ------------------------------------------------------ XCD-aware mapping
 // MI355X deals workgroups to its 8 XCDs round-robin (workgroup b runs on XCD
 // b % 8) and each XCD has its own L2.  xcd_remap() renumbers workgroups so
@@ -116,32 +122,147 @@ __device__ inline int xcd_item() {
   return (int)(xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
 }
 
-// Rows of a reduction kernel: workgroup (virtual id vb) owns the contiguous
-// chunk [vb*chunk, (vb+1)*chunk) and its threads stride through it.
-struct RowRange {
-  int begin, end;
+// ------------------------------------------------------- segmentation
+// Rows of a reduction kernel.  The rows [0, n) hold one or more independent
+// cases; case k owns rows [cell_off[k], cell_end[k]) and virtual
+// workgroups [blk_off[k], blk_off[k+1]).  Virtual workgroup vb (the XCD
+// remap of blockIdx.x) owns the contiguous chunk of `chunk` rows starting at
+// cell_off[k] + (vb - blk_off[k]) * chunk of its case, and its threads
+// stride through it.  With cell_off == NULL there is one case and
+// workgroup vb owns [vb*chunk, (vb+1)*chunk).  Because a case's chunks,
+// partials and fold order depend only on the case itself (and the chunk and
+// workgroup size), a case gives bitwise the same sums alone or in a batch.
+struct Seg {
+  int n;                    // rows in total
+  int G;                    // virtual workgroups in total
+  int chunk;                // rows per workgroup
+  int n_cases;              // K
+  const int32_t* cell_off;  // [K+1]; NULL: one case owning [0, n)
+  const int32_t* cell_end;  // [K]; NULL: cell_off[k+1]
+  const int32_t* blk_off;   // [K+1]
+  const int32_t* blk_case;  // [G]
+  const uint8_t* active;    // [K]; NULL: every case active
 };
-__device__ inline RowRange my_rows(int n) {
-  const int vb = (int)xcd_remap(blockIdx.x, gridDim.x);
-  const int chunk = (n + (int)gridDim.x - 1) / (int)gridDim.x;
-  const int b = vb * chunk;
-  return {b, b + chunk < n ? b + chunk : n};
+
+struct BlockRows {
+  int vb, k, begin, end, b0, nb;
+};
+
+__device__ inline BlockRows block_rows(const Seg& s) {
+  BlockRows r;
+  r.vb = (int)xcd_remap(blockIdx.x, gridDim.x);
+  if (!s.cell_off) {
+    r.k = 0;
+    r.b0 = 0;
+    r.nb = s.G;
+    r.begin = r.vb * s.chunk;
+    r.end = r.begin + s.chunk < s.n ? r.begin + s.chunk : s.n;
+  } else {
+    r.k = s.blk_case[r.vb];
+    r.b0 = s.blk_off[r.k];
+    r.nb = s.blk_off[r.k + 1] - r.b0;
+    r.begin = s.cell_off[r.k] + (r.vb - r.b0) * s.chunk;
+    const int e = s.cell_end ? s.cell_end[r.k] : s.cell_off[r.k + 1];
+    r.end = r.begin + s.chunk < e ? r.begin + s.chunk : e;
+  }
+  return r;
 }
 
-// ------------------------------------------------------------ reductions
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-  return v;
+// first virtual workgroup and workgroup count of case k
+__device__ inline void case_blocks(const Seg& s, int k, int& b0, int& nb) {
+  b0 = s.blk_off ? s.blk_off[k] : 0;
+  nb = s.blk_off ? s.blk_off[k + 1] - b0 : s.G;
 }
 
-// Block sum of ND values per thread; result valid in thread 0.  Order is fixed
-// by (blockDim, thread index), so the result is deterministic.
+// Single-case segmentation of n rows over at most max_g workgroups of
+// `block` threads.
+inline Seg single_seg(int n, int block, int max_g) {
+  int64_t g = (n + (int64_t)block - 1) / block;
+  if (g > max_g) g = max_g;
+  if (g < 1) g = 1;
+  Seg sg{};
+  sg.n = n;
+  sg.G = (int)g;
+  sg.chunk = (int)((n + g - 1) / g);
+  sg.n_cases = 1;
+  return sg;
+}
+
+// Segmentation of a batch descriptor (rows = cells of the batch mesh).
+inline Seg batch_seg(const wcfd_batch& bt, int n) {
+  Seg sg{};
+  sg.n = n;
+  sg.G = bt.n_blocks;
+  sg.chunk = bt.chunk;
+  sg.n_cases = bt.n_cases;
+  sg.cell_off = bt.cell_off;
+  sg.cell_end = bt.cell_end;
+  sg.blk_off = bt.blk_off;
+  sg.blk_case = bt.blk_case;
+  sg.active = bt.active;
+  return sg;
+}
+
+// Per-cell / per-face case lookups of a batch for the face and cell kernels.
+// With cell_case == NULL there is one case, always active, and the scalar
+// kernel arguments (viscosity, reference row) apply.
+struct BatchDev {
+  const int32_t* cell_case;  // [C] case of each cell, -1 for unused rows
+  const int32_t* face_case;  // [F]
+  const uint8_t* active;     // [K]
+  const float* nu;           // [K] or NULL
+  const int32_t* ref_cell;   // [K] or NULL
+  const float* ref_value;    // [K]
+  __device__ bool cell_skip(int c) const {
+    if (!cell_case) return false;
+    const int k = cell_case[c];
+    return k < 0 || !active[k];
+  }
+  __device__ bool face_skip(int f) const {
+    if (!face_case) return false;
+    const int k = face_case[f];
+    return k < 0 || !active[k];
+  }
+};
+
+inline BatchDev batch_dev(const wcfd_batch* bt) {
+  BatchDev d{};
+  if (bt) {
+    d.cell_case = bt->cell_case;
+    d.face_case = bt->face_case;
+    d.active = bt->active;
+    d.nu = bt->nu;
+    d.ref_cell = bt->ref_cell;
+    d.ref_value = bt->ref_value;
+  }
+  return d;
+}
+
+// Sum of nb partials (the same order in every workgroup); valid in every
+// thread.
+__device__ inline double fold(const double* __restrict__ part, int nb) {
+  __shared__ double lds[kMaxWaves];
+  __shared__ double res;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[b];
+  s = wave_sum(s);
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  if (lane == 0) lds[wid] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += lds[w];
+    res = t;
+  }
+  __syncthreads();
+  return res;   // a following fold writes res only after its own barrier
+}
+
+// Block sum of ND values; thread 0 stores them as partials part[i*G + vb].
 template <int ND>
-__device__ inline void block_sum(double (&v)[ND], double* lds /* [ND][kMaxWaves] */) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wid = threadIdx.x / kWave;
-  const int nw = blockDim.x / kWave;
+__device__ inline void store_partials(double (&v)[ND], double* __restrict__ part, int G, int vb) {
+  __shared__ double lds[ND * kMaxWaves];
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
 #pragma unroll
   for (int i = 0; i < ND; ++i) {
     double s = wave_sum(v[i]);
@@ -151,12 +272,11 @@ __device__ inline void block_sum(double (&v)[ND], double* lds /* [ND][kMaxWaves]
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int i = 0; i < ND; ++i) {
-      double s = 0.0;
-      for (int w = 0; w < nw; ++w) s += lds[i * kMaxWaves + w];
-      v[i] = s;
+      double t = 0.0;
+      for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += lds[i * kMaxWaves + w];
+      part[i * G + vb] = t;
     }
   }
-  __syncthreads();
 }
 
 // ------------------------------------------------------- matrix access

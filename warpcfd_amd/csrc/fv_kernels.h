@@ -4,28 +4,43 @@
 
 namespace wcfd {
 
-int launch_set_bc(const wcfd_mesh& m, wcfd_bc bc, float* fv, int mask, hipStream_t s);
-int launch_face_interp(const wcfd_mesh& m, wcfd_bc bc, const float* cv, const float* cg, float* fv,
-                       int mask, int skew_mask, float* flux, const float* rua, float* rua_f,
-                       int apply_dirichlet, hipStream_t s);
+// Face and cell operators (fv_kernels.hip).  `bd` carries a batch's per-cell
+// and per-face case lookups (all NULL for a single case).
+int launch_set_bc(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc, float* fv, int mask,
+                  hipStream_t s);
+int launch_face_interp(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc, const float* cv,
+                       const float* cg, float* fv, int mask, int skew_mask, float* flux,
+                       const float* rua, float* rua_f, int apply_dirichlet, hipStream_t s);
 int launch_green_gauss(const wcfd_mesh& m, const float* fv, float* cg, int mask, int zero_others,
                        hipStream_t s);
-int launch_assemble_momentum(const wcfd_mesh& m, wcfd_bc bc, const float* cv, const float* fv,
-                             const float* flux, float* cg, int compute_grad, float nu,
-                             float inv_density, float alpha, float dt, int correction,
+int launch_assemble_momentum(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc, const float* cv,
+                             const float* fv, const float* flux, float* cg, int compute_grad,
+                             float nu, float inv_density, float alpha, float dt, int flags,
                              float* values, float* rhs, hipStream_t s);
-int launch_assemble_pressure(const wcfd_mesh& m, wcfd_bc bc, const float* fv, const float* flux,
-                             const float* rua_f, float* cg, int compute_grad_p, int correction,
-                             int ref_cell, float ref_value, float* values, float* rhs,
-                             float* div_out, hipStream_t s);
-int launch_rua_hbya(const wcfd_mesh& m, const float* values, float density, const float* cg,
-                    float* hbya, float* cv, float* rua, hipStream_t s);
-int launch_relax_p(const wcfd_mesh& m, const float* pn, float alpha, float* cv, hipStream_t s);
-int launch_correct_velocity(const wcfd_mesh& m, wcfd_bc bc, float* cv, float* cg, float* fv,
+int launch_assemble_pressure(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc, const float* fv,
+                             const float* flux, const float* rua_f, float* cg, int compute_grad_p,
+                             int correction, int ref_cell, float ref_value, float* values,
+                             float* rhs, float* div_out, hipStream_t s);
+int launch_rua_hbya(const wcfd_mesh& m, const BatchDev& bd, const float* values, float density,
+                    const float* cg, float* hbya, float* cv, float* rua, hipStream_t s);
+int launch_relax_p(const wcfd_mesh& m, const BatchDev& bd, const float* pn, float alpha, float* cv,
+                   hipStream_t s);
+// Corrector tail.  sg segments the cells for the continuity partials (one
+// case, or the batch); resid gets [K][2] (numerator, denominator).  With a
+// batch (`bt`), cases whose residual reaches bt->stop[k] are deactivated and
+// bt->outer_iters[k] is incremented for every active case.
+int launch_correct_velocity(const wcfd_mesh& m, const BatchDev& bd, const Seg& sg, int block,
+                            const wcfd_batch* bt, wcfd_bc bc, float* cv, float* cg, float* fv,
                             const float* flux, const float* rua, const float* rua_f, float* hbya,
                             double* resid, float* fc_scratch, double* partials, hipStream_t s);
 
-// Krylov solvers (krylov.hip)
+// Krylov solvers (krylov.hip).  Stopping threshold
+// max(tol ||b||, rtol0 ||r_0||, atol); the reference's call has tol = atol
+// and rtol0 = 0.
+struct Tol {
+  double atol, tol, rtol0;
+};
+
 struct CsrView {
   int32_t n;
   const int32_t* row_ptr;
@@ -34,17 +49,20 @@ struct CsrView {
   const int32_t* diag_pos;
 };
 
-int solve_cg(const CsrView& A, int variant, const float* b, float* x, double atol, double tol,
-             int maxiter, int check_every, void* workspace, wcfd_solve_info* info, hipStream_t s);
-int solve_bicgstab(const CsrView& A, int nvec, const float* b, float* x, double atol, double tol,
+int solve_cg(const CsrView& A, int variant, const float* b, float* x, const Tol& tl, int maxiter,
+             int check_every, void* workspace, wcfd_solve_info* info, hipStream_t s);
+int solve_bicgstab(const CsrView& A, int nvec, const float* b, float* x, const Tol& tl,
                    int maxiter, int check_every, void* workspace, wcfd_solve_info* info,
                    hipStream_t s);
-int solve_cg_ell(const wcfd_mesh& m, const float* values, int variant, const float* b, float* x,
-                 double atol, double tol, int maxiter, int check_every, void* workspace,
-                 wcfd_solve_info* info, hipStream_t s);
-int solve_bicgstab_ell(const wcfd_mesh& m, const float* values, int nvec, const float* b, float* x,
-                       double atol, double tol, int maxiter, int check_every, void* workspace,
-                       wcfd_solve_info* info, hipStream_t s);
+// `bt` NULL: one case, launch shape from wcfd_set_krylov_launch.  info: [K].
+int solve_cg_ell(const wcfd_mesh& m, const wcfd_batch* bt, const float* values, int variant,
+                 const float* b, float* x, const Tol& tl, int maxiter, int check_every,
+                 void* workspace, wcfd_solve_info* info, hipStream_t s);
+int solve_bicgstab_ell(const wcfd_mesh& m, const wcfd_batch* bt, const float* values, int nvec,
+                       const float* b, float* x, const Tol& tl, int maxiter, int check_every,
+                       void* workspace, wcfd_solve_info* info, hipStream_t s);
 size_t krylov_workspace_bytes(int32_t n, int nvec);
+size_t krylov_workspace_bytes_batch(int32_t n, int nvec, int n_cases, int max_blocks);
+bool batch_ok(const wcfd_mesh* m, const wcfd_batch* bt);
 
 }  // namespace wcfd

@@ -13,11 +13,12 @@ namespace wcfd {
 thread_local int g_last_hip_error = 0;
 
 // ------------------------------------------------------------------ BCs
-__global__ void k_set_bc(int F, int Fi, int Fb, const uint8_t* __restrict__ type,
+__global__ void k_set_bc(BatchDev bd, int F, int Fi, int Fb, const uint8_t* __restrict__ type,
                          const float* __restrict__ value, float* __restrict__ fv, int mask) {
   const int j = xcd_item();
   if (j >= Fb) return;
   int f = Fi + j;
+  if (bd.face_skip(f)) return;
 #pragma unroll
   for (int q = 0; q < 4; ++q)
     if ((mask >> q) & 1)
@@ -27,14 +28,14 @@ __global__ void k_set_bc(int F, int Fi, int Fb, const uint8_t* __restrict__ type
 // --------------------------------------------------- face interpolation
 // interpolation_kernels.py:5-54, interpolation_functions.py:6-14,
 // skewness_functions.py:7-20, model_kernels.py:46-60, solver_kernels.py:22-36.
-__global__ void k_face_interp(wcfd_mesh m, wcfd_bc bc, const float* __restrict__ cv,
+__global__ void k_face_interp(wcfd_mesh m, BatchDev bd, wcfd_bc bc, const float* __restrict__ cv,
                               const float* __restrict__ cg, float* __restrict__ fv,
                               int mask, int skew_mask, float* __restrict__ flux,
                               const float* __restrict__ rua, float* __restrict__ rua_f,
                               int apply_dirichlet) {
   const int f = xcd_item();
   const int C = m.n_cells, F = m.n_faces, Fi = m.n_internal, Fb = m.n_boundary;
-  if (f >= F) return;
+  if (f >= F || bd.face_skip(f)) return;
   float val[4];
   if (f < Fi) {
     const int o = m.owner[f], n = m.neigh[f];
@@ -175,16 +176,18 @@ __device__ inline float lap_factor(const wcfd_mesh& m, int f, int correction) {
 }
 
 // ------------------------------------------------------ momentum assembly
-__global__ void k_assemble_momentum(wcfd_mesh m, wcfd_bc bc, const float* __restrict__ cv,
-                                    const float* __restrict__ fv, const float* __restrict__ flux,
-                                    float* __restrict__ cg, int compute_grad, float nu,
+__global__ void k_assemble_momentum(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
+                                    const float* __restrict__ cv, const float* __restrict__ fv,
+                                    const float* __restrict__ flux, float* __restrict__ cg,
+                                    int compute_grad, float nu_scalar,
                                     float inv_density, float alpha, float dt, int flags,
                                     float* __restrict__ values, float* __restrict__ rhs) {
   const int correction = flags & WCFD_SCHEME_CORRECTED;
   const bool implicit_outflow = (flags & WCFD_SCHEME_IMPLICIT_OUTFLOW) != 0;
   const int c = xcd_item();
   const int C = m.n_cells, F = m.n_faces, Fi = m.n_internal, Fb = m.n_boundary;
-  if (c >= C) return;
+  if (c >= C || bd.cell_skip(c)) return;
+  const float nu = bd.nu ? bd.nu[bd.cell_case[c]] : nu_scalar;
   int fk[6];
   int side[6];
 #pragma unroll
@@ -296,7 +299,8 @@ __global__ void k_assemble_momentum(wcfd_mesh m, wcfd_bc bc, const float* __rest
 }
 
 // ------------------------------------------------------ pressure assembly
-__global__ void k_assemble_pressure(wcfd_mesh m, wcfd_bc bc, const float* __restrict__ fv,
+__global__ void k_assemble_pressure(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
+                                    const float* __restrict__ fv,
                                     const float* __restrict__ flux,
                                     const float* __restrict__ rua_f, float* __restrict__ cg,
                                     int compute_grad_p, int correction, int ref_cell,
@@ -304,7 +308,12 @@ __global__ void k_assemble_pressure(wcfd_mesh m, wcfd_bc bc, const float* __rest
                                     float* __restrict__ rhs, float* __restrict__ div_out) {
   const int c = xcd_item();
   const int C = m.n_cells, F = m.n_faces, Fi = m.n_internal, Fb = m.n_boundary;
-  if (c >= C) return;
+  if (c >= C || bd.cell_skip(c)) return;
+  if (bd.ref_cell) {
+    const int k = bd.cell_case[c];
+    ref_cell = bd.ref_cell[k];
+    ref_value = bd.ref_value[k];
+  }
   int fk[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k) fk[k] = m.slot_face[k * C + c];
@@ -352,12 +361,12 @@ __global__ void k_assemble_pressure(wcfd_mesh m, wcfd_bc bc, const float* __rest
 }
 
 // ------------------------------------------------------------ rUA, HbyA
-__global__ void k_rua_hbya(wcfd_mesh m, const float* __restrict__ values, float density,
-                           const float* __restrict__ cg, float* __restrict__ hbya,
+__global__ void k_rua_hbya(wcfd_mesh m, BatchDev bd, const float* __restrict__ values,
+                           float density, const float* __restrict__ cg, float* __restrict__ hbya,
                            float* __restrict__ cv, float* __restrict__ rua) {
   const int c = xcd_item();
   const int C = m.n_cells;
-  if (c >= C) return;
+  if (c >= C || bd.cell_skip(c)) return;
   const float r = m.volume[c] / (values[c] * density);
   rua[c] = r;
 #pragma unroll
@@ -368,21 +377,22 @@ __global__ void k_rua_hbya(wcfd_mesh m, const float* __restrict__ values, float 
   }
 }
 
-__global__ void k_relax_p(int C, const float* __restrict__ pn, float alpha, float* __restrict__ cv) {
+__global__ void k_relax_p(BatchDev bd, int C, const float* __restrict__ pn, float alpha,
+                          float* __restrict__ cv) {
   const int c = xcd_item();
-  if (c >= C) return;
+  if (c >= C || bd.cell_skip(c)) return;
   float* p = cv + (size_t)3 * C;
   p[c] = alpha > 0.f ? alpha * pn[c] + (1.0f - alpha) * p[c] : pn[c];
 }
 
 // face_interpolation('p') + the corrected conservative flux F_c per face.
-__global__ void k_face_p_fc(wcfd_mesh m, wcfd_bc bc, const float* __restrict__ cv,
+__global__ void k_face_p_fc(wcfd_mesh m, BatchDev bd, wcfd_bc bc, const float* __restrict__ cv,
                             const float* __restrict__ cg, float* __restrict__ fv,
                             const float* __restrict__ flux, const float* __restrict__ rua_f,
                             float* __restrict__ fc) {
   const int f = xcd_item();
   const int C = m.n_cells, F = m.n_faces, Fi = m.n_internal, Fb = m.n_boundary;
-  if (f >= F) return;
+  if (f >= F || bd.face_skip(f)) return;
   const float* p = cv + (size_t)3 * C;
   if (f < Fi) {
     const int o = m.owner[f], n = m.neigh[f];
@@ -411,16 +421,18 @@ __global__ void k_face_p_fc(wcfd_mesh m, wcfd_bc bc, const float* __restrict__ c
 }
 
 // calculate_gradients('p') + HbyA -= rUA grad p + replace_cell_values, plus
-// the float64 continuity partial sums of the corrected flux (grid-stride so
-// the reduction grid stays within kMaxReduceBlocks).
-__global__ void k_grad_p_correct(wcfd_mesh m, const float* __restrict__ fv,
+// the float64 continuity partial sums of the corrected flux: one pair per
+// workgroup of the segmentation (one case, or a batch), folded per case by
+// k_fold_resid.
+__global__ void k_grad_p_correct(wcfd_mesh m, Seg sg, const float* __restrict__ fv,
                                  float* __restrict__ cg, float* __restrict__ cv,
                                  float* __restrict__ hbya, const float* __restrict__ rua,
                                  const float* __restrict__ fc, double* __restrict__ partials) {
   const int C = m.n_cells, F = m.n_faces;
+  const BlockRows br = block_rows(sg);
+  if (sg.active && !sg.active[br.k]) return;   // whole workgroup: one case
   double acc[2] = {0.0, 0.0};
-  const RowRange rr_ = my_rows(C);
-  for (int c = rr_.begin + (int)threadIdx.x; c < rr_.end; c += blockDim.x) {
+  for (int c = br.begin + (int)threadIdx.x; c < br.end; c += blockDim.x) {
     int fk[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) fk[k] = m.slot_face[k * C + c];
@@ -450,45 +462,28 @@ __global__ void k_grad_p_correct(wcfd_mesh m, const float* __restrict__ fv,
       acc[1] += den;
     }
   }
-  if (!fc) return;
-  // per-workgroup partials, folded by k_fold_resid (no cross-workgroup wait)
-  __shared__ double lds[2 * kMaxWaves];
-  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-#pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    const double s = wave_sum(acc[i]);
-    if (lane == 0) lds[i * kMaxWaves + wid] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int i = 0; i < 2; ++i) {
-      double t = 0.0;
-      for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += lds[i * kMaxWaves + w];
-      partials[i * gridDim.x + blockIdx.x] = t;
-    }
-  }
+  if (fc) store_partials<2>(acc, partials, sg.G, br.vb);
 }
 
-// one workgroup: fold the continuity partials in block order
-__global__ void k_fold_resid(const double* __restrict__ partials, int G, double* __restrict__ resid) {
-  __shared__ double lds[2 * kMaxWaves];
-  double acc[2] = {0.0, 0.0};
-  for (int b = threadIdx.x; b < G; b += blockDim.x) {
-    acc[0] += partials[b];
-    acc[1] += partials[G + b];
-  }
-  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-  for (int i = 0; i < 2; ++i) {
-    const double s = wave_sum(acc[i]);
-    if (lane == 0) lds[i * kMaxWaves + wid] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int i = 0; i < 2; ++i) {
-      double t = 0.0;
-      for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += lds[i * kMaxWaves + w];
-      resid[i] = t;
-    }
+// One workgroup per case: fold the case's continuity partials to
+// resid[2k], resid[2k+1].  With `stop` (a batch), count the iteration and
+// deactivate the case once num/den <= stop[k].
+__global__ void k_fold_resid(Seg sg, const double* __restrict__ partials,
+                             double* __restrict__ resid, const double* __restrict__ stop,
+                             int32_t* __restrict__ outer_iters, uint8_t* __restrict__ active) {
+  const int k = blockIdx.x;
+  if (sg.active && !sg.active[k]) return;
+  int b0, nb;
+  case_blocks(sg, k, b0, nb);
+  const double num = fold(partials + b0, nb);
+  const double den = fold(partials + sg.G + b0, nb);
+  if (threadIdx.x != 0) return;
+  resid[2 * k] = num;
+  resid[2 * k + 1] = den;
+  if (outer_iters) outer_iters[k] += 1;
+  if (stop && active) {
+    const double r = den > 0.0 ? num / den : 0.0;
+    if (stop[k] > 0.0 && r <= stop[k]) active[k] = 0;
   }
 }
 
@@ -506,18 +501,19 @@ __global__ void k_ell_to_csr(wcfd_mesh m, const float* __restrict__ ell, float* 
 }
 
 // ========================================================== host launchers
-int launch_set_bc(const wcfd_mesh& m, wcfd_bc bc, float* fv, int mask, hipStream_t s) {
+int launch_set_bc(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc, float* fv, int mask,
+                  hipStream_t s) {
   if (m.n_boundary == 0) return WCFD_OK;
-  WCFD_LAUNCH(K_SET_BC, s, k_set_bc, grid_for(m.n_boundary), m.n_faces, m.n_internal, m.n_boundary, bc.type,
-                                                     bc.value, fv, mask);
+  WCFD_LAUNCH(K_SET_BC, s, k_set_bc, grid_for(m.n_boundary), bd, m.n_faces, m.n_internal,
+              m.n_boundary, bc.type, bc.value, fv, mask);
   return check_launch();
 }
 
-int launch_face_interp(const wcfd_mesh& m, wcfd_bc bc, const float* cv, const float* cg, float* fv,
-                       int mask, int skew_mask, float* flux, const float* rua, float* rua_f,
-                       int apply_dirichlet, hipStream_t s) {
-  WCFD_LAUNCH(K_FACE_INTERP, s, k_face_interp, grid_for(m.n_faces), m, bc, cv, cg, fv, mask, skew_mask, flux,
-                                                       rua, rua_f, apply_dirichlet);
+int launch_face_interp(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc, const float* cv,
+                       const float* cg, float* fv, int mask, int skew_mask, float* flux,
+                       const float* rua, float* rua_f, int apply_dirichlet, hipStream_t s) {
+  WCFD_LAUNCH(K_FACE_INTERP, s, k_face_interp, grid_for(m.n_faces), m, bd, bc, cv, cg, fv, mask,
+              skew_mask, flux, rua, rua_f, apply_dirichlet);
   return check_launch();
 }
 
@@ -527,46 +523,52 @@ int launch_green_gauss(const wcfd_mesh& m, const float* fv, float* cg, int mask,
   return check_launch();
 }
 
-int launch_assemble_momentum(const wcfd_mesh& m, wcfd_bc bc, const float* cv, const float* fv,
-                             const float* flux, float* cg, int compute_grad, float nu,
-                             float inv_density, float alpha, float dt, int flags,
+int launch_assemble_momentum(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc, const float* cv,
+                             const float* fv, const float* flux, float* cg, int compute_grad,
+                             float nu, float inv_density, float alpha, float dt, int flags,
                              float* values, float* rhs, hipStream_t s) {
-  WCFD_LAUNCH(K_ASM_MOM, s, k_assemble_momentum, grid_for(m.n_cells),
-      m, bc, cv, fv, flux, cg, compute_grad, nu, inv_density, alpha, dt, flags, values, rhs);
+  WCFD_LAUNCH(K_ASM_MOM, s, k_assemble_momentum, grid_for(m.n_cells), m, bd, bc, cv, fv, flux, cg,
+              compute_grad, nu, inv_density, alpha, dt, flags, values, rhs);
   return check_launch();
 }
 
-int launch_assemble_pressure(const wcfd_mesh& m, wcfd_bc bc, const float* fv, const float* flux,
-                             const float* rua_f, float* cg, int compute_grad_p, int correction,
-                             int ref_cell, float ref_value, float* values, float* rhs,
-                             float* div_out, hipStream_t s) {
-  WCFD_LAUNCH(K_ASM_P, s, k_assemble_pressure, grid_for(m.n_cells), m, bc, fv, flux, rua_f, cg,
-                                                             compute_grad_p, correction, ref_cell,
-                                                             ref_value, values, rhs, div_out);
+int launch_assemble_pressure(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc, const float* fv,
+                             const float* flux, const float* rua_f, float* cg, int compute_grad_p,
+                             int correction, int ref_cell, float ref_value, float* values,
+                             float* rhs, float* div_out, hipStream_t s) {
+  WCFD_LAUNCH(K_ASM_P, s, k_assemble_pressure, grid_for(m.n_cells), m, bd, bc, fv, flux, rua_f,
+              cg, compute_grad_p, correction, ref_cell, ref_value, values, rhs, div_out);
   return check_launch();
 }
 
-int launch_rua_hbya(const wcfd_mesh& m, const float* values, float density, const float* cg,
-                    float* hbya, float* cv, float* rua, hipStream_t s) {
-  WCFD_LAUNCH(K_RUA_HBYA, s, k_rua_hbya, grid_for(m.n_cells), m, values, density, cg, hbya, cv, rua);
+int launch_rua_hbya(const wcfd_mesh& m, const BatchDev& bd, const float* values, float density,
+                    const float* cg, float* hbya, float* cv, float* rua, hipStream_t s) {
+  WCFD_LAUNCH(K_RUA_HBYA, s, k_rua_hbya, grid_for(m.n_cells), m, bd, values, density, cg, hbya,
+              cv, rua);
   return check_launch();
 }
 
-int launch_relax_p(const wcfd_mesh& m, const float* pn, float alpha, float* cv, hipStream_t s) {
-  WCFD_LAUNCH(K_RELAX_P, s, k_relax_p, grid_for(m.n_cells), m.n_cells, pn, alpha, cv);
+int launch_relax_p(const wcfd_mesh& m, const BatchDev& bd, const float* pn, float alpha, float* cv,
+                   hipStream_t s) {
+  WCFD_LAUNCH(K_RELAX_P, s, k_relax_p, grid_for(m.n_cells), bd, m.n_cells, pn, alpha, cv);
   return check_launch();
 }
 
-int launch_correct_velocity(const wcfd_mesh& m, wcfd_bc bc, float* cv, float* cg, float* fv,
+int launch_correct_velocity(const wcfd_mesh& m, const BatchDev& bd, const Seg& sg, int block,
+                            const wcfd_batch* bt, wcfd_bc bc, float* cv, float* cg, float* fv,
                             const float* flux, const float* rua, const float* rua_f, float* hbya,
                             double* resid, float* fc_scratch, double* partials, hipStream_t s) {
   float* fc = resid ? fc_scratch : nullptr;
-  WCFD_LAUNCH(K_FACE_P_FC, s, k_face_p_fc, grid_for(m.n_faces), m, bc, cv, cg, fv, flux, rua_f, fc);
+  WCFD_LAUNCH(K_FACE_P_FC, s, k_face_p_fc, grid_for(m.n_faces), m, bd, bc, cv, cg, fv, flux,
+              rua_f, fc);
   int e = check_launch();
   if (e) return e;
-  const unsigned G = reduce_grid(m.n_cells);
-  WCFD_LAUNCH(K_GRAD_P_CORR, s, k_grad_p_correct, G, m, fv, cg, cv, hbya, rua, fc, partials);
-  if (resid) WCFD_LAUNCH(K_FOLD, s, k_fold_resid, 1, partials, (int)G, resid);
+  WCFD_LAUNCH_B(K_GRAD_P_CORR, s, k_grad_p_correct, sg.G, block, m, sg, fv, cg, cv, hbya, rua, fc,
+                partials);
+  if (resid)
+    WCFD_LAUNCH(K_FOLD, s, k_fold_resid, sg.n_cases, sg, partials, resid,
+                bt ? bt->stop : nullptr, bt ? bt->outer_iters : nullptr,
+                bt ? bt->active : nullptr);
   return check_launch();
 }
 
@@ -600,7 +602,7 @@ int wcfd_abi_version(void) { return 1; }
 int wcfd_set_boundary_conditions(const wcfd_mesh* m, wcfd_bc bc, float* face_values,
                                  int field_mask, void* stream) {
   if (!mesh_ok(m) || !face_values || !bc.type || !bc.value) return WCFD_EINVAL;
-  return launch_set_bc(*m, bc, face_values, field_mask, (hipStream_t)stream);
+  return launch_set_bc(*m, BatchDev{}, bc, face_values, field_mask, (hipStream_t)stream);
 }
 
 int wcfd_face_interpolation(const wcfd_mesh* m, wcfd_bc bc, const float* cell_values,
@@ -609,7 +611,7 @@ int wcfd_face_interpolation(const wcfd_mesh* m, wcfd_bc bc, const float* cell_va
                             float* rua_face, void* stream) {
   if (!mesh_ok(m) || !cell_values || !cell_grad || !face_values) return WCFD_EINVAL;
   if ((rua_cell == nullptr) != (rua_face == nullptr)) return WCFD_EINVAL;
-  return launch_face_interp(*m, bc, cell_values, cell_grad, face_values, field_mask & 15,
+  return launch_face_interp(*m, BatchDev{}, bc, cell_values, cell_grad, face_values, field_mask & 15,
                             skew_mask & 15, mass_flux, rua_cell, rua_face, 0, (hipStream_t)stream);
 }
 
@@ -638,7 +640,7 @@ int wcfd_assemble_momentum(const wcfd_mesh* m, wcfd_bc bc, const float* cell_val
                            float dt, int scheme_flags, float* values, float* rhs, void* stream) {
   if (!mesh_ok(m) || !cell_values || !face_values || !mass_flux || !cell_grad || !values || !rhs)
     return WCFD_EINVAL;
-  return launch_assemble_momentum(*m, bc, cell_values, face_values, mass_flux, cell_grad,
+  return launch_assemble_momentum(*m, BatchDev{}, bc, cell_values, face_values, mass_flux, cell_grad,
                                   compute_gradients, nu, inv_density, alpha_u, dt, scheme_flags,
                                   values, rhs, (hipStream_t)stream);
 }
@@ -650,7 +652,7 @@ int wcfd_assemble_pressure(const wcfd_mesh* m, wcfd_bc bc, const float* face_val
   if (!mesh_ok(m) || !face_values || !mass_flux || !rua_face || !values || !rhs) return WCFD_EINVAL;
   if (compute_grad_p && !cell_grad) return WCFD_EINVAL;
   if (ref_cell >= m->n_cells) return WCFD_EINVAL;
-  return launch_assemble_pressure(*m, bc, face_values, mass_flux, rua_face, cell_grad,
+  return launch_assemble_pressure(*m, BatchDev{}, bc, face_values, mass_flux, rua_face, cell_grad,
                                   compute_grad_p, correction, ref_cell, ref_value, values, rhs,
                                   div_out, (hipStream_t)stream);
 }
@@ -658,14 +660,14 @@ int wcfd_assemble_pressure(const wcfd_mesh* m, wcfd_bc bc, const float* face_val
 int wcfd_rua_hbya(const wcfd_mesh* m, const float* values, float density, const float* cell_grad,
                   float* hbya, float* cell_values, float* rua, void* stream) {
   if (!mesh_ok(m) || !values || !cell_grad || !hbya || !cell_values || !rua) return WCFD_EINVAL;
-  return launch_rua_hbya(*m, values, density, cell_grad, hbya, cell_values, rua,
+  return launch_rua_hbya(*m, BatchDev{}, values, density, cell_grad, hbya, cell_values, rua,
                          (hipStream_t)stream);
 }
 
 int wcfd_relax_pressure(const wcfd_mesh* m, const float* p_new, float alpha, float* cell_values,
                         void* stream) {
   if (!mesh_ok(m) || !p_new || !cell_values) return WCFD_EINVAL;
-  return launch_relax_p(*m, p_new, alpha, cell_values, (hipStream_t)stream);
+  return launch_relax_p(*m, BatchDev{}, p_new, alpha, cell_values, (hipStream_t)stream);
 }
 
 int wcfd_ell_to_csr(const wcfd_mesh* m, const float* ell_values, float* csr_values, void* stream) {
@@ -692,8 +694,10 @@ int wcfd_correct_velocity(const wcfd_mesh* m, wcfd_bc bc, float* cell_values, fl
   Carver w(workspace);
   double* partials = w.take<double>(2 * kMaxReduceBlocks);
   float* fc = w.take<float>(m->n_faces);
-  return launch_correct_velocity(*m, bc, cell_values, cell_grad, face_values, mass_flux, rua,
-                                 rua_face, hbya, resid, fc, partials, (hipStream_t)stream);
+  const Seg sg = single_seg(m->n_cells, kBlock, kMaxReduceBlocks);
+  return launch_correct_velocity(*m, BatchDev{}, sg, kBlock, nullptr, bc, cell_values, cell_grad,
+                                 face_values, mass_flux, rua, rua_face, hbya, resid, fc, partials,
+                                 (hipStream_t)stream);
 }
 
 }  // extern "C"

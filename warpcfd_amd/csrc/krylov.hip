@@ -1,17 +1,28 @@
-// Krylov solvers on CSR matrices (gfx950): Jacobi-preconditioned CG and
-// right-preconditioned BiCGSTAB, the two solvers the reference takes from
+// Krylov solvers (gfx950): Jacobi-preconditioned CG -- in the standard
+// two-reduction form and in the single-reduction (Chronopoulos-Gear) form --
+// and right-preconditioned BiCGSTAB, the solvers the reference takes from
 // warp.optim.linear (equation.py:195-200; incompressible.py:32).
 //
-// Vectors are float32; every dot product and norm accumulates in float64 and
-// is folded deterministically (fixed block partition, last-block fold), so a
-// solve gives bitwise the same result every run.  Each iteration is a short
-// chain of fused kernels:
-//   CG:        [p = z + beta p (on the fly) ; q = A p ; p.q]  [x, r update ; r.z, r.r]
-//   BiCGSTAB:  [p = r + beta(p - omega v) ; v = A M p ; r0.v]
-//              [s = r - alpha v ; t = A M s ; t.s, t.t, s.s]
-//              [x += alpha M p + omega M s ; r = s - omega t ; r0.r, r.r]
-// The scalars live in device memory; the host reads ||r|| only every
-// check_every iterations (the same check interval as the reference's solver).
+// One launch solves K independent systems (a batch of cases whose matrices
+// sit block-diagonally in one array; K = 1 for a single case).  Every case
+// owns a contiguous range of rows and of workgroups (struct Seg), so each case
+// has its own dot products, scalars, stopping flag and iteration count, and
+// its arithmetic does not depend on which other cases share the launch.
+//
+// Vectors are float32; dot products and norms accumulate in float64.  Every
+// reduction writes one partial per workgroup; the NEXT kernel folds its case's
+// partials in its prologue (all workgroups of the case in the same order), so
+// no kernel waits on another workgroup -- a cross-workgroup handshake costs
+// an agent-scope release per workgroup, measured 5-6x slower on MI355X at
+// 320 k rows -- and every run is bitwise reproducible.  Iteration kernels:
+//   CG:         [p = z + beta p ; q = A p ; p.q]  [x, r update ; r.z, r.r]
+//   CG (1-red): [p, s recurrences, x, r updates, w = A M r ; r.u, w.u, r.r]
+//   BiCGSTAB:   [p = r + beta(p - omega v) ; v = A M p ; r0.v]
+//               [s = r - alpha v ; t = A M s ; t.s, t.t, s.s]
+//               [x += alpha M p + omega M s ; r = s - omega t ; r0.r, r.r]
+// A check kernel (one workgroup per case) after every `check_every`
+// iterations folds ||r||^2 and sets the case's done flag; the iteration
+// kernels of a done case return at entry.
 #include "fv_kernels.h"
 
 #include <math.h>
@@ -20,72 +31,26 @@
 
 namespace wcfd {
 
-// Scalars of one solve.  Quantities that change every iteration sit in
-// two-slot rings indexed by iteration parity: block 0 of a kernel writes the
-// slot of the current iteration while the other blocks of the same kernel
-// read the previous one.
 // Flags the host polls (copied to pinned memory after every check).
 struct KFlags {
-  int32_t done;   // set once converged, non-finite or out of iterations
+  int32_t done;   // converged, non-finite, out of iterations or inactive
   int32_t iters;  // iterations run when done was set
   double rr;      // ||r||^2 at the last check
   double thr2;    // squared stopping threshold
 };
 
+// Scalars of one case.  Values that change every iteration sit in two-slot
+// rings indexed by iteration parity: the case's first workgroup writes the
+// slot of the current iteration while the others read the previous one.
 struct KState {
   KFlags f;
   double rz[2], rho[2], alpha[2], omega[2];
-  double atol, tol, bb, ss;
+  double atol, tol, rtol0, bb, ss;
 };
 
 __device__ inline double safe_div(double a, double b) { return b != 0.0 ? a / b : 0.0; }
 
-// Every reduction writes one float64 partial per workgroup; the NEXT kernel
-// folds the partials in its prologue, every workgroup in the same fixed order,
-// so all of them hold bitwise the same scalar.  No kernel waits on another
-// workgroup: a cross-workgroup handshake inside a kernel costs an agent-scope
-// release (an L2 write-back on every XCD) per workgroup, which measured
-// 5-6x slower than this scheme on MI355X at 320 k rows.
-__device__ inline double fold(const double* __restrict__ part, int G) {
-  __shared__ double lds[kMaxWaves];
-  __shared__ double res;
-  double s = 0.0;
-  for (int b = threadIdx.x; b < G; b += blockDim.x) s += part[b];
-  s = wave_sum(s);
-  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-  if (lane == 0) lds[wid] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += lds[w];
-    res = t;
-  }
-  __syncthreads();
-  return res;
-}
-
-// Block sum of ND values; thread 0 stores them as this block's partials
-// part[i * G + blockIdx.x].
-template <int ND>
-__device__ inline void store_partials(double (&v)[ND], double* __restrict__ part, int G) {
-  __shared__ double lds[ND * kMaxWaves];
-  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-#pragma unroll
-  for (int i = 0; i < ND; ++i) {
-    double s = wave_sum(v[i]);
-    if (lane == 0) lds[i * kMaxWaves + wid] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int i = 0; i < ND; ++i) {
-      double t = 0.0;
-      for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += lds[i * kMaxWaves + w];
-      part[i * G + blockIdx.x] = t;
-    }
-  }
-}
-
+// ------------------------------------------------------------ small kernels
 template <class M>
 __global__ void k_inv_diag(M A, int n, float* __restrict__ inv) {
   const int i = xcd_item();
@@ -109,13 +74,13 @@ __global__ void k_spmv(M A, int n, const float* __restrict__ x, float* __restric
   for (int q = 0; q < NV; ++q) y[(size_t)q * n + i] = acc[q];
 }
 
-__global__ void k_dot(int n, const float* __restrict__ a, const float* __restrict__ b,
+__global__ void k_dot(Seg sg, const float* __restrict__ a, const float* __restrict__ b,
                       double* __restrict__ part) {
+  const BlockRows br = block_rows(sg);
   double acc[1] = {0.0};
-  const RowRange rr_ = my_rows(n);
-  for (int i = rr_.begin + (int)threadIdx.x; i < rr_.end; i += blockDim.x)
+  for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x)
     acc[0] += (double)a[i] * (double)b[i];
-  store_partials<1>(acc, part, gridDim.x);
+  store_partials<1>(acc, part, sg.G, br.vb);
 }
 
 __global__ void k_fold_to(const double* __restrict__ part, int G, double* out) {
@@ -123,15 +88,77 @@ __global__ void k_fold_to(const double* __restrict__ part, int G, double* out) {
   if (threadIdx.x == 0) out[0] = v;
 }
 
+__global__ void k_set_tolerances(KState* st, int n_cases, Tol tl) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_cases) return;
+  st[k].atol = tl.atol;
+  st[k].tol = tl.tol;
+  st[k].rtol0 = tl.rtol0;
+}
+
+// Init fold, one workgroup per case: threshold, first scalars and the done
+// flag.  kind 0 (CG): P = {r.z, r.r, b.b}.  kind 1 (BiCGSTAB):
+// P = {r0.r, r.r, b.b}.  kind 2 (single-reduction CG): P = {r.u, w.u, r.r}
+// and Pb holds b.b in slot 2.
+__global__ void k_init_fold(Seg sg, const double* __restrict__ P, const double* __restrict__ Pb,
+                            KState* st, int kind, int maxiter) {
+  const int k = blockIdx.x;
+  int b0, nb;
+  case_blocks(sg, k, b0, nb);
+  const int G = sg.G;
+  const double a = fold(P + b0, nb);
+  const double x1 = fold(P + G + b0, nb);
+  const double x2 = fold((kind == 2 ? Pb : P) + 2 * G + b0, nb);
+  const double x3 = kind == 2 ? fold(P + 2 * G + b0, nb) : 0.0;
+  if (threadIdx.x != 0) return;
+  const double rr = kind == 2 ? x3 : x1;
+  const double bb = x2;
+  KState* S = st + k;
+  S->bb = bb;
+  const double thr = fmax(fmax(S->tol * sqrt(bb), S->rtol0 * sqrt(rr)), S->atol);
+  S->f.rr = rr;
+  S->f.thr2 = thr * thr;
+  S->f.iters = 0;
+  const bool inactive = sg.active && !sg.active[k];
+  // converged already, non-finite, no iterations allowed, or not iterating
+  S->f.done = (inactive || !(rr > thr * thr) || maxiter <= 0) ? 1 : 0;
+  S->alpha[0] = S->alpha[1] = 0.0;
+  S->omega[0] = S->omega[1] = 0.0;
+  if (kind == 1) {
+    S->rho[0] = a;
+  } else {
+    S->rz[0] = a;
+    if (kind == 2) S->alpha[0] = safe_div(a, x1);
+  }
+}
+
+// one workgroup per case, after every check_every iterations
+__global__ void k_check(Seg sg, const double* __restrict__ part, KState* st, int it_end,
+                        int maxiter) {
+  const int k = blockIdx.x;
+  KState* S = st + k;
+  if (S->f.done) return;
+  int b0, nb;
+  case_blocks(sg, k, b0, nb);
+  const double rr = fold(part + b0, nb);
+  if (threadIdx.x == 0) {
+    S->f.rr = rr;
+    if (!(rr > S->f.thr2) || it_end >= maxiter) {
+      S->f.done = 1;
+      S->f.iters = it_end;
+    }
+  }
+}
+
 // ------------------------------------------------------------------ CG
 // partial slots: PA[3][G] = {r.z, r.r, b.b} (init, k_cg_2), PB[1][G] = {p.q} (k_cg_1)
 template <class M>
-__global__ void k_cg_init(M A, int n, const float* __restrict__ b, const float* __restrict__ x,
+__global__ void k_cg_init(M A, Seg sg, const float* __restrict__ b, const float* __restrict__ x,
                           float* __restrict__ r, const float* __restrict__ inv,
                           double* __restrict__ PA) {
+  const BlockRows br = block_rows(sg);
   double acc[3] = {0.0, 0.0, 0.0};
-  const RowRange rr_ = my_rows(n);
-  for (int i = rr_.begin + (int)threadIdx.x; i < rr_.end; i += blockDim.x) {
+  for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
     float ax = 0.f;
     A.row(i, [&](float a, int c) { ax += a * x[c]; });
     const float ri = b[i] - ax;
@@ -141,61 +168,28 @@ __global__ void k_cg_init(M A, int n, const float* __restrict__ b, const float* 
     acc[1] += (double)ri * (double)ri;
     acc[2] += (double)b[i] * (double)b[i];
   }
-  store_partials<3>(acc, PA, gridDim.x);
-}
-
-// one workgroup: fold the init partials, set the threshold and rz[0] / rho[0]
-__global__ void k_init_fold(const double* __restrict__ PA, int G, KState* st, int is_cg,
-                            int maxiter) {
-  const double a = fold(PA, G), rr = fold(PA + G, G), bb = fold(PA + 2 * G, G);
-  if (threadIdx.x == 0) {
-    if (is_cg) st->rz[0] = a; else st->rho[0] = a;
-    st->bb = bb;
-    const double thr = fmax(st->tol * sqrt(bb), st->atol);
-    st->f.rr = rr;
-    st->f.thr2 = thr * thr;
-    st->f.iters = 0;
-    // converged already, non-finite, or no iterations allowed
-    st->f.done = (!(rr > thr * thr) || maxiter <= 0) ? 1 : 0;
-    st->alpha[0] = st->alpha[1] = 0.0;
-    st->omega[0] = st->omega[1] = 0.0;
-  }
-}
-
-// one workgroup, after every check_every iterations: fold ||r||^2 and set
-// the done flag the iteration kernels test at entry
-__global__ void k_check(const double* __restrict__ part, int G, KState* st, int it_end,
-                        int maxiter) {
-  if (st->f.done) return;
-  const double rr = fold(part, G);
-  if (threadIdx.x == 0) {
-    st->f.rr = rr;
-    if (!(rr > st->f.thr2) || it_end >= maxiter) {
-      st->f.done = 1;
-      st->f.iters = it_end;
-    }
-  }
+  store_partials<3>(acc, PA, sg.G, br.vb);
 }
 
 template <class M>
-__global__ void k_cg_1(M A, int n, int it, const float* __restrict__ r,
+__global__ void k_cg_1(M A, Seg sg, int it, const float* __restrict__ r,
                        const float* __restrict__ inv, const float* __restrict__ p_in,
                        float* __restrict__ p_out, float* __restrict__ q,
                        const double* __restrict__ PA, double* __restrict__ PB, KState* st) {
-  if (st->f.done) return;
-  const int G = gridDim.x;
+  const BlockRows br = block_rows(sg);
+  KState* S = st + br.k;
+  if (S->f.done) return;
   bool first = (it == 0);
   double beta_d = 0.0;
   if (!first) {
-    const double rz = fold(PA, G);                 // r.z of the previous iteration
-    beta_d = safe_div(rz, st->rz[(it - 1) & 1]);
-    if (blockIdx.x == 0 && threadIdx.x == 0) st->rz[it & 1] = rz;
+    const double rz = fold(PA + br.b0, br.nb);   // r.z of the previous iteration
+    beta_d = safe_div(rz, S->rz[(it - 1) & 1]);
+    if (br.vb == br.b0 && threadIdx.x == 0) S->rz[it & 1] = rz;
     first = (beta_d == 0.0);
   }
   const float beta = (float)beta_d;
   double acc[1] = {0.0};
-  const RowRange rr_ = my_rows(n);
-  for (int i = rr_.begin + (int)threadIdx.x; i < rr_.end; i += blockDim.x) {
+  for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
     float qi = 0.f;
     A.row(i, [&](float a, int c) {
       const float z = r[c] * inv[c];
@@ -208,20 +202,20 @@ __global__ void k_cg_1(M A, int n, int it, const float* __restrict__ r,
     q[i] = qi;
     acc[0] += (double)pi * (double)qi;
   }
-  store_partials<1>(acc, PB, G);
+  store_partials<1>(acc, PB, sg.G, br.vb);
 }
 
-__global__ void k_cg_2(int n, int it, float* __restrict__ x, float* __restrict__ r,
+__global__ void k_cg_2(Seg sg, int it, float* __restrict__ x, float* __restrict__ r,
                        const float* __restrict__ inv, const float* __restrict__ p,
                        const float* __restrict__ q, const double* __restrict__ PB,
                        double* __restrict__ PA, KState* st) {
-  if (st->f.done) return;
-  const int G = gridDim.x;
-  const double pq = fold(PB, G);
-  const float alpha = (float)safe_div(st->rz[it & 1], pq);
+  const BlockRows br = block_rows(sg);
+  KState* S = st + br.k;
+  if (S->f.done) return;
+  const double pq = fold(PB + br.b0, br.nb);
+  const float alpha = (float)safe_div(S->rz[it & 1], pq);
   double acc[2] = {0.0, 0.0};
-  const RowRange rr_ = my_rows(n);
-  for (int i = rr_.begin + (int)threadIdx.x; i < rr_.end; i += blockDim.x) {
+  for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
     x[i] = x[i] + alpha * p[i];
     const float ri = r[i] - alpha * q[i];
     r[i] = ri;
@@ -229,7 +223,7 @@ __global__ void k_cg_2(int n, int it, float* __restrict__ x, float* __restrict__
     acc[0] += (double)ri * (double)zi;
     acc[1] += (double)ri * (double)ri;
   }
-  store_partials<2>(acc, PA, G);
+  store_partials<2>(acc, PA, sg.G, br.vb);
 }
 
 // ------------------------------------------- CG with one reduction per iteration
@@ -243,11 +237,11 @@ __global__ void k_cg_2(int n, int it, float* __restrict__ x, float* __restrict__
 // fly.  Partials ping-pong between PA and PC: kernel i reads the buffer
 // kernel i-1 wrote.
 template <class M>
-__global__ void k_cgs_init1(M A, int n, const float* __restrict__ b, const float* __restrict__ x,
+__global__ void k_cgs_init1(M A, Seg sg, const float* __restrict__ b, const float* __restrict__ x,
                             float* __restrict__ r, double* __restrict__ PC) {
+  const BlockRows br = block_rows(sg);
   double acc[3] = {0.0, 0.0, 0.0};
-  const RowRange rr_ = my_rows(n);
-  for (int i = rr_.begin + (int)threadIdx.x; i < rr_.end; i += blockDim.x) {
+  for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
     float ax = 0.f;
     A.row(i, [&](float a, int c) { ax += a * x[c]; });
     const float ri = b[i] - ax;
@@ -255,15 +249,15 @@ __global__ void k_cgs_init1(M A, int n, const float* __restrict__ b, const float
     acc[1] += (double)ri * (double)ri;
     acc[2] += (double)b[i] * (double)b[i];
   }
-  store_partials<3>(acc, PC, gridDim.x);
+  store_partials<3>(acc, PC, sg.G, br.vb);
 }
 
 template <class M>
-__global__ void k_cgs_init2(M A, int n, const float* __restrict__ r, const float* __restrict__ inv,
+__global__ void k_cgs_init2(M A, Seg sg, const float* __restrict__ r, const float* __restrict__ inv,
                             float* __restrict__ w, double* __restrict__ PA) {
+  const BlockRows br = block_rows(sg);
   double acc[3] = {0.0, 0.0, 0.0};
-  const RowRange rr_ = my_rows(n);
-  for (int i = rr_.begin + (int)threadIdx.x; i < rr_.end; i += blockDim.x) {
+  for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
     float wi = 0.f;
     A.row(i, [&](float a, int c) { wi += a * (r[c] * inv[c]); });
     w[i] = wi;
@@ -272,52 +266,38 @@ __global__ void k_cgs_init2(M A, int n, const float* __restrict__ r, const float
     acc[1] += (double)wi * (double)ui;
     acc[2] += (double)r[i] * (double)r[i];
   }
-  store_partials<3>(acc, PA, gridDim.x);
-}
-
-__global__ void k_cgs_init_fold(const double* __restrict__ PA, const double* __restrict__ PC, int G,
-                                KState* st, int maxiter) {
-  const double gamma = fold(PA, G), delta = fold(PA + G, G), rr = fold(PA + 2 * G, G);
-  const double bb = fold(PC + 2 * G, G);
-  if (threadIdx.x == 0) {
-    st->bb = bb;
-    const double thr = fmax(st->tol * sqrt(bb), st->atol);
-    st->f.rr = rr;
-    st->f.thr2 = thr * thr;
-    st->f.iters = 0;
-    st->f.done = (!(rr > thr * thr) || maxiter <= 0) ? 1 : 0;
-    st->rz[0] = gamma;
-    st->alpha[0] = safe_div(gamma, delta);
-  }
+  store_partials<3>(acc, PA, sg.G, br.vb);
 }
 
 template <class M>
-__global__ void k_cgs_iter(M A, int n, int it, float* __restrict__ x, const float* __restrict__ r_in,
-                           float* __restrict__ r_out, const float* __restrict__ w_in,
-                           float* __restrict__ w_out, const float* __restrict__ s_in,
-                           float* __restrict__ s_out, float* __restrict__ p,
-                           const float* __restrict__ inv, const double* __restrict__ P_in,
-                           double* __restrict__ P_out, KState* st) {
-  if (st->f.done) return;
-  const int G = gridDim.x;
+__global__ void k_cgs_iter(M A, Seg sg, int it, float* __restrict__ x,
+                           const float* __restrict__ r_in, float* __restrict__ r_out,
+                           const float* __restrict__ w_in, float* __restrict__ w_out,
+                           const float* __restrict__ s_in, float* __restrict__ s_out,
+                           float* __restrict__ p, const float* __restrict__ inv,
+                           const double* __restrict__ P_in, double* __restrict__ P_out,
+                           KState* st) {
+  const BlockRows br = block_rows(sg);
+  KState* S = st + br.k;
+  if (S->f.done) return;
+  const int G = sg.G;
   const bool first = (it == 0);
   double alpha_d, beta_d = 0.0;
   if (first) {
-    alpha_d = st->alpha[0];
+    alpha_d = S->alpha[0];
   } else {
     const int pv = (it - 1) & 1;
-    const double g = fold(P_in, G), d = fold(P_in + G, G);
-    beta_d = safe_div(g, st->rz[pv]);
-    alpha_d = safe_div(g, d - beta_d * safe_div(g, st->alpha[pv]));
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      st->rz[it & 1] = g;
-      st->alpha[it & 1] = alpha_d;
+    const double g = fold(P_in + br.b0, br.nb), d = fold(P_in + G + br.b0, br.nb);
+    beta_d = safe_div(g, S->rz[pv]);
+    alpha_d = safe_div(g, d - beta_d * safe_div(g, S->alpha[pv]));
+    if (br.vb == br.b0 && threadIdx.x == 0) {
+      S->rz[it & 1] = g;
+      S->alpha[it & 1] = alpha_d;
     }
   }
   const float alpha = (float)alpha_d, beta = (float)beta_d;
   double acc[3] = {0.0, 0.0, 0.0};
-  const RowRange rr_ = my_rows(n);
-  for (int i = rr_.begin + (int)threadIdx.x; i < rr_.end; i += blockDim.x) {
+  for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
     float wn = 0.f;
     A.row(i, [&](float a, int c) {
       const float sc = first ? w_in[c] : w_in[c] + beta * s_in[c];
@@ -338,18 +318,19 @@ __global__ void k_cgs_iter(M A, int n, int it, float* __restrict__ x, const floa
     acc[1] += (double)wn * (double)un;
     acc[2] += (double)rn * (double)rn;
   }
-  store_partials<3>(acc, P_out, G);
+  store_partials<3>(acc, P_out, G, br.vb);
 }
 
 // ------------------------------------------------------------- BiCGSTAB
 // partial slots: PA[3][G] = {r0.r, r.r, b.b} (init, k_bi_3), PB[1][G] = {r0.v}
 // (k_bi_1), PC[3][G] = {t.s, t.t, s.s} (k_bi_2)
 template <class M, int NV>
-__global__ void k_bi_init(M A, int n, const float* __restrict__ b, const float* __restrict__ x,
+__global__ void k_bi_init(M A, Seg sg, const float* __restrict__ b, const float* __restrict__ x,
                           float* __restrict__ r, float* __restrict__ r0, double* __restrict__ PA) {
+  const BlockRows br = block_rows(sg);
+  const int n = sg.n;
   double acc[3] = {0.0, 0.0, 0.0};
-  const RowRange rr_ = my_rows(n);
-  for (int i = rr_.begin + (int)threadIdx.x; i < rr_.end; i += blockDim.x) {
+  for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
     float ax[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) ax[q] = 0.f;
@@ -369,32 +350,33 @@ __global__ void k_bi_init(M A, int n, const float* __restrict__ b, const float* 
       acc[2] += (double)bi * (double)bi;
     }
   }
-  store_partials<3>(acc, PA, gridDim.x);
+  store_partials<3>(acc, PA, sg.G, br.vb);
 }
 
 template <class M, int NV>
-__global__ void k_bi_1(M A, int n, int it, const float* __restrict__ r,
+__global__ void k_bi_1(M A, Seg sg, int it, const float* __restrict__ r,
                        const float* __restrict__ inv, const float* __restrict__ p_in,
                        const float* __restrict__ v_in, float* __restrict__ p_out,
                        float* __restrict__ v_out, const float* __restrict__ r0,
                        const double* __restrict__ PA, double* __restrict__ PB, KState* st) {
-  if (st->f.done) return;
-  const int G = gridDim.x;
+  const BlockRows br = block_rows(sg);
+  KState* S = st + br.k;
+  if (S->f.done) return;
+  const int n = sg.n;
   bool first = (it == 0);
   double beta_d = 0.0, omega_d = 0.0;
   if (!first) {
     const int pv = (it - 1) & 1;
-    const double rho = fold(PA, G);               // r0.r after the previous iteration
-    omega_d = st->omega[pv];
-    beta_d = safe_div(rho, st->rho[pv]) * safe_div(st->alpha[pv], omega_d);
-    if (blockIdx.x == 0 && threadIdx.x == 0) st->rho[it & 1] = rho;
+    const double rho = fold(PA + br.b0, br.nb);   // r0.r after the previous iteration
+    omega_d = S->omega[pv];
+    beta_d = safe_div(rho, S->rho[pv]) * safe_div(S->alpha[pv], omega_d);
+    if (br.vb == br.b0 && threadIdx.x == 0) S->rho[it & 1] = rho;
     first = (beta_d == 0.0);
   }
   const float beta = (float)beta_d;
   const float omega = (float)omega_d;
   double acc[1] = {0.0};
-  const RowRange rr_ = my_rows(n);
-  for (int i = rr_.begin + (int)threadIdx.x; i < rr_.end; i += blockDim.x) {
+  for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
     float v[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) v[q] = 0.f;
@@ -416,23 +398,24 @@ __global__ void k_bi_1(M A, int n, int it, const float* __restrict__ r,
       acc[0] += (double)r0[k] * (double)v[q];
     }
   }
-  store_partials<1>(acc, PB, G);
+  store_partials<1>(acc, PB, sg.G, br.vb);
 }
 
 template <class M, int NV>
-__global__ void k_bi_2(M A, int n, int it, const float* __restrict__ r,
+__global__ void k_bi_2(M A, Seg sg, int it, const float* __restrict__ r,
                        const float* __restrict__ inv, const float* __restrict__ v,
                        float* __restrict__ s, float* __restrict__ t,
                        const double* __restrict__ PB, double* __restrict__ PC, KState* st) {
-  if (st->f.done) return;
-  const int G = gridDim.x;
-  const double r0v = fold(PB, G);
-  const double alpha_d = safe_div(st->rho[it & 1], r0v);
-  if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha[it & 1] = alpha_d;
+  const BlockRows br = block_rows(sg);
+  KState* S = st + br.k;
+  if (S->f.done) return;
+  const int n = sg.n;
+  const double r0v = fold(PB + br.b0, br.nb);
+  const double alpha_d = safe_div(S->rho[it & 1], r0v);
+  if (br.vb == br.b0 && threadIdx.x == 0) S->alpha[it & 1] = alpha_d;
   const float alpha = (float)alpha_d;
   double acc[3] = {0.0, 0.0, 0.0};
-  const RowRange rr_ = my_rows(n);
-  for (int i = rr_.begin + (int)threadIdx.x; i < rr_.end; i += blockDim.x) {
+  for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
     float tv[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) tv[q] = 0.f;
@@ -456,25 +439,30 @@ __global__ void k_bi_2(M A, int n, int it, const float* __restrict__ r,
       acc[2] += (double)si * (double)si;
     }
   }
-  store_partials<3>(acc, PC, G);
+  store_partials<3>(acc, PC, sg.G, br.vb);
 }
 
 template <int NV>
-__global__ void k_bi_3(int n, int it, float* __restrict__ x, float* __restrict__ r,
+__global__ void k_bi_3(Seg sg, int it, float* __restrict__ x, float* __restrict__ r,
                        const float* __restrict__ inv, const float* __restrict__ p,
                        const float* __restrict__ s, const float* __restrict__ t,
                        const float* __restrict__ r0, const double* __restrict__ PC,
                        double* __restrict__ PA, KState* st) {
-  if (st->f.done) return;
-  const int G = gridDim.x;
-  const double ts = fold(PC, G), tt = fold(PC + G, G), ss = fold(PC + 2 * G, G);
-  const double omega_d = ss > st->f.thr2 ? safe_div(ts, tt) : 0.0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) { st->omega[it & 1] = omega_d; st->ss = ss; }
-  const float alpha = (float)st->alpha[it & 1];
+  const BlockRows br = block_rows(sg);
+  KState* S = st + br.k;
+  if (S->f.done) return;
+  const int n = sg.n, G = sg.G;
+  const double ts = fold(PC + br.b0, br.nb), tt = fold(PC + G + br.b0, br.nb);
+  const double ss = fold(PC + 2 * G + br.b0, br.nb);
+  const double omega_d = ss > S->f.thr2 ? safe_div(ts, tt) : 0.0;
+  if (br.vb == br.b0 && threadIdx.x == 0) {
+    S->omega[it & 1] = omega_d;
+    S->ss = ss;
+  }
+  const float alpha = (float)S->alpha[it & 1];
   const float omega = (float)omega_d;
   double acc[2] = {0.0, 0.0};
-  const RowRange rr_ = my_rows(n);
-  for (int i = rr_.begin + (int)threadIdx.x; i < rr_.end; i += blockDim.x) {
+  for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
     const float ii = inv[i];
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
@@ -488,7 +476,7 @@ __global__ void k_bi_3(int n, int it, float* __restrict__ x, float* __restrict__
       acc[1] += (double)rk * (double)rk;
     }
   }
-  store_partials<2>(acc, PA, G);
+  store_partials<2>(acc, PA, G, br.vb);
 }
 
 // ------------------------------------------------------------- host side
@@ -498,16 +486,15 @@ struct KWork {
   KState* st;
 };
 
-// The scalar state and the reduction partials come first, at offsets that do
-// not depend on nvec: CG (1 vector) and BiCGSTAB (3 vectors) may share one
-// workspace.
-static KWork carve_krylov(void* ws, int32_t n, int nvec) {
+// State and partials first, at offsets that depend only on (n_cases,
+// max_blocks): CG (1 vector) and BiCGSTAB (3 vectors) may share a workspace.
+static KWork carve_krylov(void* ws, int32_t n, int nvec, int n_cases, int max_blocks) {
   Carver c(ws);
   KWork w;
-  w.st = c.take<KState>(1);
-  w.PA = c.take<double>(3 * kMaxReduceBlocks);
-  w.PB = c.take<double>(kMaxReduceBlocks);
-  w.PC = c.take<double>(3 * kMaxReduceBlocks);
+  w.st = c.take<KState>(n_cases);
+  w.PA = c.take<double>(3 * (size_t)max_blocks);
+  w.PB = c.take<double>((size_t)max_blocks);
+  w.PC = c.take<double>(3 * (size_t)max_blocks);
   const size_t nv = (size_t)n * nvec;
   w.r = c.take<float>(nv);
   w.r0 = c.take<float>(nv);
@@ -521,24 +508,20 @@ static KWork carve_krylov(void* ws, int32_t n, int nvec) {
   return w;
 }
 
-size_t krylov_workspace_bytes(int32_t n, int nvec) {
+size_t krylov_workspace_bytes_batch(int32_t n, int nvec, int n_cases, int max_blocks) {
   Carver c(nullptr);
-  c.take<KState>(1);
-  c.take<double>(3 * kMaxReduceBlocks);
-  c.take<double>(kMaxReduceBlocks);
-  c.take<double>(3 * kMaxReduceBlocks);
+  c.take<KState>(n_cases);
+  c.take<double>(3 * (size_t)max_blocks);
+  c.take<double>((size_t)max_blocks);
+  c.take<double>(3 * (size_t)max_blocks);
   const size_t nv = (size_t)n * nvec;
   for (int i = 0; i < 8; ++i) c.take<float>(nv);
   c.take<float>(n);
   return c.off;
 }
 
-static int set_tolerances(KState* dev, double atol, double tol, hipStream_t s) {
-  double t[2] = {atol, tol};
-  int e = check_hip(hipMemcpyAsync(&dev->atol, t, sizeof(t), hipMemcpyHostToDevice, s));
-  if (e) return e;
-  // the host array must stay alive until the copy has been consumed
-  return check_hip(hipStreamSynchronize(s));
+size_t krylov_workspace_bytes(int32_t n, int nvec) {
+  return krylov_workspace_bytes_batch(n, nvec, 1, kMaxReduceBlocks);
 }
 
 static void fill_info(wcfd_solve_info* info, const KFlags& h) {
@@ -549,85 +532,109 @@ static void fill_info(wcfd_solve_info* info, const KFlags& h) {
   info->threshold = sqrt(h.thr2);
 }
 
-// Pinned host slots and events for polling the done flag without draining
-// the stream (one set per host thread, created on first use).
+// Pinned host slots and events for polling the done flags without draining
+// the stream (one set per host thread, grown to the largest batch seen).
 constexpr int kSlots = 4;
 struct HostSync {
-  KFlags* slot = nullptr;
+  KFlags* slot = nullptr;   // [kSlots][cap]
+  int cap = 0;
   hipEvent_t ev[kSlots] = {};
   hipEvent_t tail = nullptr;
   bool tail_pending = false;
+  bool ready = false;
 };
 
-static HostSync* host_sync() {
+static HostSync* host_sync(int n_cases) {
   static thread_local HostSync hs;
-  static thread_local bool ready = false;
-  if (!ready) {
-    if (check_hip(hipHostMalloc((void**)&hs.slot, sizeof(KFlags) * kSlots, hipHostMallocDefault)))
-      return nullptr;
+  if (!hs.ready) {
     for (auto& e : hs.ev)
       if (check_hip(hipEventCreateWithFlags(&e, hipEventDisableTiming))) return nullptr;
     if (check_hip(hipEventCreateWithFlags(&hs.tail, hipEventDisableTiming))) return nullptr;
-    ready = true;
+    hs.ready = true;
+  }
+  if (hs.tail_pending) {   // copies of an earlier solve may still target the slots
+    if (check_hip(hipEventSynchronize(hs.tail))) return nullptr;
+    hs.tail_pending = false;
+  }
+  if (hs.cap < n_cases) {
+    if (hs.slot && check_hip(hipHostFree(hs.slot))) return nullptr;
+    hs.slot = nullptr;
+    hs.cap = 0;
+    const int cap = n_cases > 64 ? n_cases : 64;
+    if (check_hip(hipHostMalloc((void**)&hs.slot, sizeof(KFlags) * kSlots * (size_t)cap,
+                                hipHostMallocDefault)))
+      return nullptr;
+    hs.cap = cap;
   }
   return &hs;
 }
 
-// Runs the iteration groups of one solve.  Each group is `ce` iterations
-// followed by k_check; its flags are copied to a pinned slot and an event is
-// recorded.  The host keeps `depth` groups in flight and stops launching as
-// soon as a polled group reports done; groups already queued find the flag
-// set and exit at entry, so the result is the same as checking after every
-// group synchronously (the reference solver's check interval).
+static bool all_done(const KFlags* f, int n) {
+  for (int k = 0; k < n; ++k)
+    if (!f[k].done) return false;
+  return true;
+}
+
+static int finish(const KFlags* f, int n, wcfd_solve_info* info) {
+  bool finite = true;
+  for (int k = 0; k < n; ++k) {
+    if (info) fill_info(info + k, f[k]);
+    if (!isfinite(f[k].rr)) finite = false;
+  }
+  return finite ? WCFD_OK : WCFD_ENAN;
+}
+
+// Copies the K flag records (strided in KState) to a pinned slot.
+static int copy_flags(KFlags* dst, const KState* st, int K, hipStream_t s) {
+  return check_hip(hipMemcpy2DAsync(dst, sizeof(KFlags), &st[0].f, sizeof(KState), sizeof(KFlags),
+                                    K, hipMemcpyDeviceToHost, s));
+}
+
+// Runs the iteration groups of one (batched) solve.  Each group is `ce`
+// iterations followed by the check kernel; the K flag records are copied to a
+// pinned slot and an event is recorded.  The host keeps `depth` groups in
+// flight and stops launching once a polled group reports every case done;
+// groups already queued find the flags set and exit at entry, so the result
+// is the same as checking after every group synchronously (the reference
+// solver's check interval).
 template <class LaunchGroup, class RrPart>
-static int run_groups(KState* st, RrPart&& rr_part, unsigned G, int maxiter, int ce,
+static int run_groups(const Seg& sg, KState* st, RrPart&& rr_part, int maxiter, int ce,
                       LaunchGroup&& launch_group, wcfd_solve_info* info, hipStream_t s) {
-  HostSync* hs = host_sync();
+  const int K = sg.n_cases;
+  HostSync* hs = host_sync(K);
   if (!hs) return WCFD_EHIP;
   int e;
-  if (hs->tail_pending) {   // copies of an earlier solve may still target the slots
-    if ((e = check_hip(hipEventSynchronize(hs->tail)))) return e;
-    hs->tail_pending = false;
-  }
   // flags after init (synchronous: one round trip per solve)
-  if ((e = check_hip(hipMemcpyAsync(&hs->slot[0], &st->f, sizeof(KFlags), hipMemcpyDeviceToHost, s))))
-    return e;
+  if ((e = copy_flags(hs->slot, st, K, s))) return e;
   if ((e = check_hip(hipStreamSynchronize(s)))) return e;
-  KFlags last = hs->slot[0];
-  if (last.done) {
-    fill_info(info, last);
-    return isfinite(last.rr) ? WCFD_OK : WCFD_ENAN;
-  }
+  if (all_done(hs->slot, K)) return finish(hs->slot, K, info);
   const int depth = 2;
   int launched = 0, polled = 0, it = 0;
-  const int cap = maxiter;
+  const KFlags* last = hs->slot;
   while (true) {
-    if (it < cap && launched - polled < depth) {
+    if (it < maxiter && launched - polled < depth) {
       launch_group(it);
       it += ce;
-      WCFD_LAUNCH(K_FOLD, s, k_check, 1, rr_part(it), (int)G, st, it, cap);
-      const int sl = launched % kSlots;
+      WCFD_LAUNCH(K_FOLD, s, k_check, K, sg, rr_part(it), st, it, maxiter);
       if ((e = check_launch())) return e;
-      if ((e = check_hip(hipMemcpyAsync(&hs->slot[sl], &st->f, sizeof(KFlags),
-                                        hipMemcpyDeviceToHost, s))))
-        return e;
+      const int sl = launched % kSlots;
+      if ((e = copy_flags(hs->slot + (size_t)sl * hs->cap, st, K, s))) return e;
       if ((e = check_hip(hipEventRecord(hs->ev[sl], s)))) return e;
       ++launched;
       continue;
     }
-    if (polled == launched) break;   // nothing left (cannot happen: the last check sets done)
+    if (polled == launched) break;   // cannot happen: the last check sets every flag
     const int sl = polled % kSlots;
     if ((e = check_hip(hipEventSynchronize(hs->ev[sl])))) return e;
-    last = hs->slot[sl];
+    last = hs->slot + (size_t)sl * hs->cap;
     ++polled;
-    if (last.done) break;
+    if (all_done(last, K)) break;
   }
   if (launched > polled) {
     if ((e = check_hip(hipEventRecord(hs->tail, s)))) return e;
     hs->tail_pending = true;
   }
-  fill_info(info, last);
-  return isfinite(last.rr) ? WCFD_OK : WCFD_ENAN;
+  return finish(last, K, info);
 }
 
 namespace {
@@ -637,87 +644,84 @@ std::atomic<int> g_krylov_block{kBlock};
 
 KrylovLaunch krylov_launch() { return {g_krylov_grid.load(), g_krylov_block.load()}; }
 
-// workgroups of the reduction kernels: a grid-stride grid of at most the
-// tuned size (every workgroup of the next kernel folds that many partials)
-static unsigned krylov_grid(int64_t n, int block) {
-  int64_t g = (n + block - 1) / block;
-  const int64_t cap = krylov_launch().grid;
-  if (g > cap) g = cap;
-  return static_cast<unsigned>(g < 1 ? 1 : g);
+// Launch of one solve: the segmentation and the workgroup size.
+struct KLaunch {
+  Seg sg;
+  int B;
+};
+
+static int prepare(KState* st, const Seg& sg, const Tol& tl, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_tolerances, dim3(grid_for(sg.n_cases)), dim3(kBlock), 0, s, st,
+                     sg.n_cases, tl);
+  return check_launch();
 }
 
 template <class M>
-static int cg_impl(const M& A, int n, const float* b, float* x, double atol, double tol, int maxiter,
-                   int check_every, void* workspace, wcfd_solve_info* info, hipStream_t s) {
-  KWork w = carve_krylov(workspace, n, 1);
+static int cg_impl(const M& A, const KLaunch& L, const float* b, float* x, const Tol& tl,
+                   int maxiter, int check_every, KWork& w, wcfd_solve_info* info, hipStream_t s) {
+  const Seg& sg = L.sg;
+  const int n = sg.n, B = L.B, G = sg.G, K = sg.n_cases;
   float* q = w.s;
-  int e = set_tolerances(w.st, atol, tol, s);
+  int e = prepare(w.st, sg, tl, s);
   if (e) return e;
-  const int B = krylov_launch().block;
-  const unsigned g = grid_for(n), G = krylov_grid(n, B);
-  WCFD_LAUNCH(K_INV_DIAG, s, k_inv_diag<M>, g, A, n, w.inv);
-  WCFD_LAUNCH_B(K_CG_INIT, s, k_cg_init<M>, G, B, A, n, b, x, w.r, w.inv, w.PA);
-  WCFD_LAUNCH(K_FOLD, s, k_init_fold, 1, w.PA, (int)G, w.st, 1, maxiter);
+  WCFD_LAUNCH(K_INV_DIAG, s, k_inv_diag<M>, grid_for(n), A, n, w.inv);
+  WCFD_LAUNCH_B(K_CG_INIT, s, k_cg_init<M>, G, B, A, sg, b, x, w.r, w.inv, w.PA);
+  WCFD_LAUNCH(K_FOLD, s, k_init_fold, K, sg, w.PA, w.PA, w.st, 0, maxiter);
   if ((e = check_launch())) return e;
   float* pb[2] = {w.p0, w.p1};
   const int ce = check_every > 0 ? check_every : 1;
   auto group = [&](int it0) {
     for (int it = it0; it < it0 + ce; ++it) {
       const int cur = it & 1;
-      WCFD_LAUNCH_B(K_CG_1, s, k_cg_1<M>, G, B, A, n, it, w.r, w.inv, pb[cur], pb[1 - cur], q,
+      WCFD_LAUNCH_B(K_CG_1, s, k_cg_1<M>, G, B, A, sg, it, w.r, w.inv, pb[cur], pb[1 - cur], q,
                     w.PA, w.PB, w.st);
-      WCFD_LAUNCH_B(K_CG_2, s, k_cg_2, G, B, n, it, x, w.r, w.inv, pb[1 - cur], q, w.PB, w.PA,
+      WCFD_LAUNCH_B(K_CG_2, s, k_cg_2, G, B, sg, it, x, w.r, w.inv, pb[1 - cur], q, w.PB, w.PA,
                     w.st);
     }
   };
   auto rr_part = [&](int) { return (const double*)(w.PA + G); };
-  return run_groups(w.st, rr_part, G, maxiter, ce, group, info, s);
+  return run_groups(sg, w.st, rr_part, maxiter, ce, group, info, s);
 }
 
-// Chronopoulos-Gear CG: one fused kernel per iteration.
 template <class M>
-static int cgs_impl(const M& A, int n, const float* b, float* x, double atol, double tol,
-                    int maxiter, int check_every, void* workspace, wcfd_solve_info* info,
+static int cgs_impl(const M& A, const KLaunch& L, const float* b, float* x, const Tol& tl, int maxiter, int check_every, KWork& w, wcfd_solve_info* info,
                     hipStream_t s) {
-  KWork w = carve_krylov(workspace, n, 1);
-  int e = set_tolerances(w.st, atol, tol, s);
+  const Seg& sg = L.sg;
+  const int n = sg.n, B = L.B, G = sg.G, K = sg.n_cases;
+  int e = prepare(w.st, sg, tl, s);
   if (e) return e;
-  const int B = krylov_launch().block;
-  const unsigned g = grid_for(n), G = krylov_grid(n, B);
   float* rb[2] = {w.r, w.r0};
   float* wb[2] = {w.v0, w.v1};
   float* sb[2] = {w.s, w.t};
   double* pb[2] = {w.PA, w.PC};   // iteration i reads pb[i&1], writes pb[(i+1)&1]
-  WCFD_LAUNCH(K_INV_DIAG, s, k_inv_diag<M>, g, A, n, w.inv);
-  WCFD_LAUNCH_B(K_CG_INIT, s, k_cgs_init1<M>, G, B, A, n, b, x, rb[0], w.PC);
-  WCFD_LAUNCH_B(K_CG_INIT, s, k_cgs_init2<M>, G, B, A, n, rb[0], w.inv, wb[0], w.PA);
-  WCFD_LAUNCH(K_FOLD, s, k_cgs_init_fold, 1, w.PA, w.PC, (int)G, w.st, maxiter);
+  WCFD_LAUNCH(K_INV_DIAG, s, k_inv_diag<M>, grid_for(n), A, n, w.inv);
+  WCFD_LAUNCH_B(K_CG_INIT, s, k_cgs_init1<M>, G, B, A, sg, b, x, rb[0], w.PC);
+  WCFD_LAUNCH_B(K_CG_INIT, s, k_cgs_init2<M>, G, B, A, sg, rb[0], w.inv, wb[0], w.PA);
+  WCFD_LAUNCH(K_FOLD, s, k_init_fold, K, sg, w.PA, w.PC, w.st, 2, maxiter);
   if ((e = check_launch())) return e;
   const int ce = check_every > 0 ? check_every : 1;
   auto group = [&](int it0) {
     for (int it = it0; it < it0 + ce; ++it) {
       const int cur = it & 1;
-      WCFD_LAUNCH_B(K_CG_1, s, k_cgs_iter<M>, G, B, A, n, it, x, rb[cur], rb[1 - cur], wb[cur],
+      WCFD_LAUNCH_B(K_CG_1, s, k_cgs_iter<M>, G, B, A, sg, it, x, rb[cur], rb[1 - cur], wb[cur],
                     wb[1 - cur], sb[cur], sb[1 - cur], w.p0, w.inv, pb[cur], pb[1 - cur], w.st);
     }
   };
   // the r.r partials of the last iteration (it_end - 1) are in pb[it_end & 1], slot 2
   auto rr_part = [&](int it_end) { return (const double*)(pb[it_end & 1] + 2 * G); };
-  return run_groups(w.st, rr_part, G, maxiter, ce, group, info, s);
+  return run_groups(sg, w.st, rr_part, maxiter, ce, group, info, s);
 }
 
 template <class M, int NV>
-static int bicg_impl(const M& A, int n, const float* b, float* x, double atol, double tol,
-                     int maxiter, int check_every, void* workspace, wcfd_solve_info* info,
+static int bicg_impl(const M& A, const KLaunch& L, const float* b, float* x, const Tol& tl, int maxiter, int check_every, KWork& w, wcfd_solve_info* info,
                      hipStream_t s) {
-  KWork w = carve_krylov(workspace, n, NV);
-  int e = set_tolerances(w.st, atol, tol, s);
+  const Seg& sg = L.sg;
+  const int n = sg.n, B = L.B, G = sg.G, K = sg.n_cases;
+  int e = prepare(w.st, sg, tl, s);
   if (e) return e;
-  const int B = krylov_launch().block;
-  const unsigned g = grid_for(n), G = krylov_grid(n, B);
-  WCFD_LAUNCH(K_INV_DIAG, s, k_inv_diag<M>, g, A, n, w.inv);
-  WCFD_LAUNCH_B(K_BI_INIT, s, (k_bi_init<M, NV>), G, B, A, n, b, x, w.r, w.r0, w.PA);
-  WCFD_LAUNCH(K_FOLD, s, k_init_fold, 1, w.PA, (int)G, w.st, 0, maxiter);
+  WCFD_LAUNCH(K_INV_DIAG, s, k_inv_diag<M>, grid_for(n), A, n, w.inv);
+  WCFD_LAUNCH_B(K_BI_INIT, s, (k_bi_init<M, NV>), G, B, A, sg, b, x, w.r, w.r0, w.PA);
+  WCFD_LAUNCH(K_FOLD, s, k_init_fold, K, sg, w.PA, w.PA, w.st, 1, maxiter);
   if ((e = check_launch())) return e;
   float* pb[2] = {w.p0, w.p1};
   float* vb[2] = {w.v0, w.v1};
@@ -725,73 +729,103 @@ static int bicg_impl(const M& A, int n, const float* b, float* x, double atol, d
   auto group = [&](int it0) {
     for (int it = it0; it < it0 + ce; ++it) {
       const int cur = it & 1;
-      WCFD_LAUNCH_B(K_BI_1, s, (k_bi_1<M, NV>), G, B, A, n, it, w.r, w.inv, pb[cur], vb[cur],
+      WCFD_LAUNCH_B(K_BI_1, s, (k_bi_1<M, NV>), G, B, A, sg, it, w.r, w.inv, pb[cur], vb[cur],
                     pb[1 - cur], vb[1 - cur], w.r0, w.PA, w.PB, w.st);
-      WCFD_LAUNCH_B(K_BI_2, s, (k_bi_2<M, NV>), G, B, A, n, it, w.r, w.inv, vb[1 - cur], w.s, w.t,
-                    w.PB, w.PC, w.st);
-      WCFD_LAUNCH_B(K_BI_3, s, k_bi_3<NV>, G, B, n, it, x, w.r, w.inv, pb[1 - cur], w.s, w.t, w.r0,
-                    w.PC, w.PA, w.st);
+      WCFD_LAUNCH_B(K_BI_2, s, (k_bi_2<M, NV>), G, B, A, sg, it, w.r, w.inv, vb[1 - cur], w.s,
+                    w.t, w.PB, w.PC, w.st);
+      WCFD_LAUNCH_B(K_BI_3, s, k_bi_3<NV>, G, B, sg, it, x, w.r, w.inv, pb[1 - cur], w.s, w.t,
+                    w.r0, w.PC, w.PA, w.st);
     }
   };
   auto rr_part = [&](int) { return (const double*)(w.PA + G); };
-  return run_groups(w.st, rr_part, G, maxiter, ce, group, info, s);
+  return run_groups(sg, w.st, rr_part, maxiter, ce, group, info, s);
 }
 
 template <class M>
-static int bicg_dispatch(const M& A, int n, int nvec, const float* b, float* x, double atol,
-                         double tol, int maxiter, int check_every, void* ws,
-                         wcfd_solve_info* info, hipStream_t s) {
-  if (nvec == 1) return bicg_impl<M, 1>(A, n, b, x, atol, tol, maxiter, check_every, ws, info, s);
-  if (nvec == 3) return bicg_impl<M, 3>(A, n, b, x, atol, tol, maxiter, check_every, ws, info, s);
-  return WCFD_EINVAL;
-}
-
-template <class M>
-static int cg_dispatch(const M& A, int n, int variant, const float* b, float* x, double atol,
-                       double tol, int maxiter, int check_every, void* ws, wcfd_solve_info* info,
-                       hipStream_t s) {
+static int cg_dispatch(const M& A, const KLaunch& L, int variant, const float* b, float* x,
+                       const Tol& tl, int maxiter, int check_every, KWork& w,
+                       wcfd_solve_info* info, hipStream_t s) {
   if (variant == WCFD_CG_SINGLE_REDUCTION)
-    return cgs_impl(A, n, b, x, atol, tol, maxiter, check_every, ws, info, s);
+    return cgs_impl(A, L, b, x, tl, maxiter, check_every, w, info, s);
   if (variant == WCFD_CG_STANDARD)
-    return cg_impl(A, n, b, x, atol, tol, maxiter, check_every, ws, info, s);
+    return cg_impl(A, L, b, x, tl, maxiter, check_every, w, info, s);
   return WCFD_EINVAL;
 }
 
-int solve_cg(const CsrView& A, int variant, const float* b, float* x, double atol, double tol,
+template <class M>
+static int bicg_dispatch(const M& A, const KLaunch& L, int nvec, const float* b, float* x,
+                         const Tol& tl, int maxiter, int check_every, KWork& w,
+                         wcfd_solve_info* info, hipStream_t s) {
+  if (nvec == 1) return bicg_impl<M, 1>(A, L, b, x, tl, maxiter, check_every, w, info, s);
+  if (nvec == 3) return bicg_impl<M, 3>(A, L, b, x, tl, maxiter, check_every, w, info, s);
+  return WCFD_EINVAL;
+}
+
+// ELL matrix of a mesh, with or without int16 neighbour offsets
+template <class F>
+static int with_ell(const wcfd_mesh& m, const float* values, F&& f) {
+  if (m.slot_dnb) return f(EllMat<true>{m.n_cells, values, m.slot_nbr, m.slot_dnb});
+  return f(EllMat<false>{m.n_cells, values, m.slot_nbr, m.slot_dnb});
+}
+
+static KLaunch single_launch(int n) {
+  const KrylovLaunch kl = krylov_launch();
+  return {single_seg(n, kl.block, kl.grid), kl.block};
+}
+
+static KLaunch launch_of(const wcfd_batch* bt, int n) {
+  if (!bt) return single_launch(n);
+  return {batch_seg(*bt, n), bt->block};
+}
+
+static KWork work_of(void* ws, const wcfd_batch* bt, int n, int nvec) {
+  return bt ? carve_krylov(ws, n, nvec, bt->n_cases, bt->n_blocks)
+            : carve_krylov(ws, n, nvec, 1, kMaxReduceBlocks);
+}
+
+int solve_cg(const CsrView& A, int variant, const float* b, float* x, const Tol& tl,
              int maxiter, int check_every, void* workspace, wcfd_solve_info* info, hipStream_t s) {
   const CsrMat M{A.n, A.row_ptr, A.col, A.values, A.diag_pos};
-  return cg_dispatch(M, A.n, variant, b, x, atol, tol, maxiter, check_every, workspace, info, s);
+  KWork w = work_of(workspace, nullptr, A.n, 1);
+  return cg_dispatch(M, single_launch(A.n), variant, b, x, tl, maxiter, check_every, w,
+                     info, s);
 }
 
-int solve_bicgstab(const CsrView& A, int nvec, const float* b, float* x, double atol, double tol,
+int solve_bicgstab(const CsrView& A, int nvec, const float* b, float* x, const Tol& tl,
                    int maxiter, int check_every, void* workspace, wcfd_solve_info* info,
                    hipStream_t s) {
   const CsrMat M{A.n, A.row_ptr, A.col, A.values, A.diag_pos};
-  return bicg_dispatch(M, A.n, nvec, b, x, atol, tol, maxiter, check_every, workspace, info, s);
+  KWork w = work_of(workspace, nullptr, A.n, nvec);
+  return bicg_dispatch(M, single_launch(A.n), nvec, b, x, tl, maxiter, check_every, w,
+                       info, s);
 }
 
-int solve_cg_ell(const wcfd_mesh& m, const float* values, int variant, const float* b, float* x,
-                 double atol, double tol, int maxiter, int check_every, void* workspace,
-                 wcfd_solve_info* info, hipStream_t s) {
+int solve_cg_ell(const wcfd_mesh& m, const wcfd_batch* bt, const float* values, int variant,
+                 const float* b, float* x, const Tol& tl, int maxiter, int check_every,
+                 void* workspace, wcfd_solve_info* info, hipStream_t s) {
   const int n = m.n_cells;
-  if (m.slot_dnb) {
-    const EllMat<true> M{n, values, m.slot_nbr, m.slot_dnb};
-    return cg_dispatch(M, n, variant, b, x, atol, tol, maxiter, check_every, workspace, info, s);
-  }
-  const EllMat<false> M{n, values, m.slot_nbr, m.slot_dnb};
-  return cg_dispatch(M, n, variant, b, x, atol, tol, maxiter, check_every, workspace, info, s);
+  const KLaunch L = launch_of(bt, n);
+  KWork w = work_of(workspace, bt, n, 1);
+  return with_ell(m, values, [&](auto A) {
+    return cg_dispatch(A, L, variant, b, x, tl, maxiter, check_every, w, info, s);
+  });
 }
 
-int solve_bicgstab_ell(const wcfd_mesh& m, const float* values, int nvec, const float* b, float* x,
-                       double atol, double tol, int maxiter, int check_every, void* workspace,
-                       wcfd_solve_info* info, hipStream_t s) {
+int solve_bicgstab_ell(const wcfd_mesh& m, const wcfd_batch* bt, const float* values, int nvec,
+                       const float* b, float* x, const Tol& tl, int maxiter,
+                       int check_every, void* workspace, wcfd_solve_info* info, hipStream_t s) {
   const int n = m.n_cells;
-  if (m.slot_dnb) {
-    const EllMat<true> M{n, values, m.slot_nbr, m.slot_dnb};
-    return bicg_dispatch(M, n, nvec, b, x, atol, tol, maxiter, check_every, workspace, info, s);
-  }
-  const EllMat<false> M{n, values, m.slot_nbr, m.slot_dnb};
-  return bicg_dispatch(M, n, nvec, b, x, atol, tol, maxiter, check_every, workspace, info, s);
+  const KLaunch L = launch_of(bt, n);
+  KWork w = work_of(workspace, bt, n, nvec);
+  return with_ell(m, values, [&](auto A) {
+    return bicg_dispatch(A, L, nvec, b, x, tl, maxiter, check_every, w, info, s);
+  });
+}
+
+bool batch_ok(const wcfd_mesh* m, const wcfd_batch* bt) {
+  return m && bt && m->n_cells > 0 && bt->n_cases > 0 && bt->n_blocks > 0 && bt->chunk > 0 &&
+         bt->block >= kWave && bt->block <= kMaxWaves * kWave && bt->block % kWave == 0 &&
+         bt->cell_off && bt->blk_off && bt->blk_case && bt->active;
 }
 
 }  // namespace wcfd
@@ -817,19 +851,15 @@ int wcfd_spmv(int32_t n, const int32_t* row_ptr, const int32_t* col, const float
 int wcfd_spmv_ell(const wcfd_mesh* m, const float* values, int nvec, const float* x, float* y,
                   void* stream) {
   if (!m || m->n_cells <= 0 || !m->slot_nbr || !values || !x || !y) return WCFD_EINVAL;
+  if (nvec != 1 && nvec != 3) return WCFD_EINVAL;
   hipStream_t s = (hipStream_t)stream;
   const int n = m->n_cells;
-  if (nvec != 1 && nvec != 3) return WCFD_EINVAL;
-  if (m->slot_dnb) {
-    const EllMat<true> A{n, values, m->slot_nbr, m->slot_dnb};
-    if (nvec == 1) WCFD_LAUNCH(K_SPMV, s, (k_spmv<EllMat<true>, 1>), grid_for(n), A, n, x, y);
-    else WCFD_LAUNCH(K_SPMV, s, (k_spmv<EllMat<true>, 3>), grid_for(n), A, n, x, y);
-  } else {
-    const EllMat<false> A{n, values, m->slot_nbr, m->slot_dnb};
-    if (nvec == 1) WCFD_LAUNCH(K_SPMV, s, (k_spmv<EllMat<false>, 1>), grid_for(n), A, n, x, y);
-    else WCFD_LAUNCH(K_SPMV, s, (k_spmv<EllMat<false>, 3>), grid_for(n), A, n, x, y);
-  }
-  return check_launch();
+  return with_ell(*m, values, [&](auto A) {
+    using M = decltype(A);
+    if (nvec == 1) WCFD_LAUNCH(K_SPMV, s, (k_spmv<M, 1>), grid_for(n), A, n, x, y);
+    else WCFD_LAUNCH(K_SPMV, s, (k_spmv<M, 3>), grid_for(n), A, n, x, y);
+    return check_launch();
+  });
 }
 
 int wcfd_dot_f64(int32_t n, const float* a, const float* b, double* out, void* workspace,
@@ -838,9 +868,9 @@ int wcfd_dot_f64(int32_t n, const float* a, const float* b, double* out, void* w
   Carver c(workspace);
   double* part = c.take<double>(kMaxReduceBlocks);
   hipStream_t s = (hipStream_t)stream;
-  const unsigned G = reduce_grid(n);
-  WCFD_LAUNCH(K_DOT, s, k_dot, G, n, a, b, part);
-  WCFD_LAUNCH(K_FOLD, s, k_fold_to, 1, part, (int)G, out);
+  const Seg sg = single_seg(n, kBlock, kMaxReduceBlocks);
+  WCFD_LAUNCH(K_DOT, s, k_dot, sg.G, sg, a, b, part);
+  WCFD_LAUNCH(K_FOLD, s, k_fold_to, 1, part, sg.G, out);
   return check_launch();
 }
 
@@ -866,32 +896,19 @@ size_t wcfd_krylov_workspace_bytes(int32_t n, int nvec) {
   return krylov_workspace_bytes(n, nvec);
 }
 
+size_t wcfd_batch_krylov_workspace_bytes(int32_t n, int32_t n_cases, int32_t n_blocks) {
+  if (n <= 0 || n_cases <= 0 || n_blocks <= 0) return 0;
+  return krylov_workspace_bytes_batch(n, 3, n_cases, n_blocks);
+}
+
 int wcfd_cg(int32_t n, const int32_t* row_ptr, const int32_t* col, const float* values,
             const int32_t* diag_pos, int variant, const float* b, float* x, double atol, double tol,
             int maxiter, int check_every, void* workspace, wcfd_solve_info* info, void* stream) {
   if (n <= 0 || !row_ptr || !col || !values || !diag_pos || !b || !x || !workspace)
     return WCFD_EINVAL;
   CsrView A{n, row_ptr, col, values, diag_pos};
-  return solve_cg(A, variant, b, x, atol, tol, maxiter, check_every, workspace, info,
+  return solve_cg(A, variant, b, x, Tol{atol, tol, 0.0}, maxiter, check_every, workspace, info,
                   (hipStream_t)stream);
-}
-
-int wcfd_cg_ell(const wcfd_mesh* m, const float* values, int variant, const float* b, float* x,
-                double atol, double tol, int maxiter, int check_every, void* workspace,
-                wcfd_solve_info* info, void* stream) {
-  if (!m || m->n_cells <= 0 || !m->slot_nbr || !values || !b || !x || !workspace)
-    return WCFD_EINVAL;
-  return solve_cg_ell(*m, values, variant, b, x, atol, tol, maxiter, check_every, workspace, info,
-                      (hipStream_t)stream);
-}
-
-int wcfd_bicgstab_ell(const wcfd_mesh* m, const float* values, int nvec, const float* b, float* x,
-                      double atol, double tol, int maxiter, int check_every, void* workspace,
-                      wcfd_solve_info* info, void* stream) {
-  if (!m || m->n_cells <= 0 || !m->slot_nbr || !values || !b || !x || !workspace)
-    return WCFD_EINVAL;
-  return solve_bicgstab_ell(*m, values, nvec, b, x, atol, tol, maxiter, check_every, workspace,
-                            info, (hipStream_t)stream);
 }
 
 int wcfd_bicgstab(int32_t n, const int32_t* row_ptr, const int32_t* col, const float* values,
@@ -901,8 +918,43 @@ int wcfd_bicgstab(int32_t n, const int32_t* row_ptr, const int32_t* col, const f
   if (n <= 0 || !row_ptr || !col || !values || !diag_pos || !b || !x || !workspace)
     return WCFD_EINVAL;
   CsrView A{n, row_ptr, col, values, diag_pos};
-  return solve_bicgstab(A, nvec, b, x, atol, tol, maxiter, check_every, workspace, info,
+  return solve_bicgstab(A, nvec, b, x, Tol{atol, tol, 0.0}, maxiter, check_every, workspace, info,
                         (hipStream_t)stream);
+}
+
+int wcfd_cg_ell(const wcfd_mesh* m, const float* values, int variant, const float* b, float* x,
+                double atol, double tol, int maxiter, int check_every, void* workspace,
+                wcfd_solve_info* info, void* stream) {
+  if (!m || m->n_cells <= 0 || !m->slot_nbr || !values || !b || !x || !workspace)
+    return WCFD_EINVAL;
+  return solve_cg_ell(*m, nullptr, values, variant, b, x, Tol{atol, tol, 0.0}, maxiter, check_every,
+                      workspace, info, (hipStream_t)stream);
+}
+
+int wcfd_bicgstab_ell(const wcfd_mesh* m, const float* values, int nvec, const float* b, float* x,
+                      double atol, double tol, int maxiter, int check_every, void* workspace,
+                      wcfd_solve_info* info, void* stream) {
+  if (!m || m->n_cells <= 0 || !m->slot_nbr || !values || !b || !x || !workspace)
+    return WCFD_EINVAL;
+  return solve_bicgstab_ell(*m, nullptr, values, nvec, b, x, Tol{atol, tol, 0.0}, maxiter, check_every,
+                            workspace, info, (hipStream_t)stream);
+}
+
+int wcfd_batch_cg_ell(const wcfd_mesh* m, const wcfd_batch* bt, const float* values, int variant,
+                      const float* b, float* x, double atol, double tol, int maxiter,
+                      int check_every, void* workspace, wcfd_solve_info* info, void* stream) {
+  if (!batch_ok(m, bt) || !m->slot_nbr || !values || !b || !x || !workspace) return WCFD_EINVAL;
+  return solve_cg_ell(*m, bt, values, variant, b, x, Tol{atol, tol, 0.0}, maxiter, check_every, workspace,
+                      info, (hipStream_t)stream);
+}
+
+int wcfd_batch_bicgstab_ell(const wcfd_mesh* m, const wcfd_batch* bt, const float* values,
+                            int nvec, const float* b, float* x, double atol, double tol,
+                            int maxiter, int check_every, void* workspace, wcfd_solve_info* info,
+                            void* stream) {
+  if (!batch_ok(m, bt) || !m->slot_nbr || !values || !b || !x || !workspace) return WCFD_EINVAL;
+  return solve_bicgstab_ell(*m, bt, values, nvec, b, x, Tol{atol, tol, 0.0}, maxiter, check_every,
+                            workspace, info, (hipStream_t)stream);
 }
 
 }  // extern "C"
