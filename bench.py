@@ -16,7 +16,11 @@ average launch time, HIP events recorded by the library around every launch
 during a profiled pass of the same number of steps) and ``cpu_baseline``
 (the CPU oracle, single thread, on a bounded sample of the same workload).
 
-    python bench.py [--steps K] [--warmup W] [--workload channel|cavity]
+    python bench.py [--steps K] [--warmup W] [--workload channel|cavity|batch64]
+
+``--workload batch64`` solves the 64-cavity batch (one step = the whole
+batch to continuity <= 1e-6) in one arena and, for comparison, the same
+cases one after another through the plain solver; its value is solves/s.
 """
 import argparse
 import json
@@ -125,12 +129,221 @@ def cpu_baseline(nodes, cells, kind, budget_s=20.0, max_iters=6):
                       f'(numpy/scipy restatement of the reference path), 1 thread, {dt:.1f} s'}
 
 
+def bench_batch64(args):
+    """The 64-cavity batch: every case to continuity <= 1e-6, batched in one
+    arena, and the same 64 cases one after another through the plain
+    single-case solver.  One step = one solve of the whole batch."""
+    import numpy as np
+    import torch
+    from warpcfd_amd import _lib
+    from warpcfd_amd.batch import CaseArena, CaseSpec
+    from warpcfd_amd.workloads import batch64_draws, cavity_model
+    from warpcfd_amd.FV.implicit_Solvers import IncompressibleSolver
+
+    stop = 1e-6
+    stop_u, floor_u = args.stop_u, 1e-7
+    pol = dict(atol=0.0, tol=0.0, rtol0=args.rtol0, maxiter=2000, check_every=10,
+               check_every_u=args.check_u, cg_variant=_lib.CG_SINGLE_REDUCTION)
+    draws = batch64_draws(args.seed)[:args.cases]
+    t = time.perf_counter()
+    specs = [CaseSpec(cavity_model(n, re), stop=stop, max_outer=args.max_outer,
+                      name=f'cavity{n}_re{re:.0f}', stop_u=stop_u, floor_u=floor_u)
+             for n, re in draws]
+    prep_s = time.perf_counter() - t
+    cells = sum(s.n_cells for s in specs)
+    print(f'# batch: {len(specs)} cases, {cells} cells, host preprocessing {prep_s:.1f} s',
+          file=sys.stderr, flush=True)
+    kw = dict(chunk=args.chunk, block=args.block, **pol)
+    # warm-up: load every kernel once
+    warm = CaseArena.for_cases(specs[:2], **kw)
+    for _ in range(3):
+        warm.step()
+    torch.cuda.synchronize()
+    del warm
+
+    arena = CaseArena.for_cases(specs, **kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done_at = {}
+    steps = 0
+    while len(done_at) < len(specs) and steps < args.max_outer + 1:
+        arena.step()
+        steps += 1
+        now = time.perf_counter() - t0
+        for k in arena.poll():
+            done_at.setdefault(k, now)
+        if steps % 1000 == 0:
+            print(f'# batched: {steps} iterations, {len(done_at)} cases done, {now:.1f} s',
+                  file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    batched_s = time.perf_counter() - t0
+    iters = [arena.iterations(k) for k in range(len(specs))]
+    n_conv = sum(arena.converged(k) for k in range(len(specs)))
+    lat = np.array([done_at[k] for k in range(len(specs))])
+    print(f'# batched: {batched_s:.2f} s, {steps} batch iterations, {n_conv}/{len(specs)} '
+          f'converged', file=sys.stderr, flush=True)
+
+    roofline = kernels = None
+    if not args.no_profile:
+        prof_arena = CaseArena.for_cases(specs, **kw)
+        with _lib.Profiler(capacity=400000) as prof:
+            for _ in range(20):
+                prof_arena.step()
+            torch.cuda.synchronize()
+        kernels = {k: {'total_ms': round(v[0], 3), 'launches': v[1],
+                       'avg_us': round(1e3 * v[0] / v[1], 3)}
+                   for k, v in sorted(prof.result.items(), key=lambda kv: -kv[1][0])}
+        dominant = max(prof.result.items(), key=lambda kv: kv[1][0])[0]
+        ms, cnt = prof.result[dominant]
+        key = dominant + ('_single' if dominant == 'cg_1' else '')
+        nnz = sum(s.layout.nnz for s in specs)
+        F = sum(s.n_internal + s.n_boundary for s in specs)
+        Fi = sum(s.n_internal for s in specs)
+        byts = algorithmic_bytes(key, cells, F, Fi, F - Fi, nnz)
+        if byts is not None:
+            avg_s = ms / cnt / 1e3
+            ach = byts / avg_s / 1e9
+            roofline = {'bound': 'hbm', 'kernel': dominant, 'achieved': round(ach, 1),
+                        'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(ach / HBM_PEAK_GBS, 4),
+                        'traffic': None, 'bytes_per_launch': byts,
+                        'avg_launch_us': round(avg_s * 1e6, 3),
+                        'note': 'first 20 batch iterations, all 64 cases active'}
+        del prof_arena
+
+    seq = None
+    if not args.no_sequential:
+        seq_t, seq_it, dev, seq_conv = [], [], [], 0
+        for k, (n, re) in enumerate(draws):
+            model = cavity_model(n, re)
+            model.finalize()
+            sv = IncompressibleSolver(model, 0.7, 0.3)
+            sv.tol, sv.rtol, sv.rtol0 = pol['atol'], pol['tol'], pol['rtol0']
+            sv.max_iter, sv.check_every, sv.cg_variant = pol['maxiter'], 10, pol['cg_variant']
+            sv.check_every_u = pol['check_every_u']
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            it, ok, _ = sv.run_to_convergence(stop, stop_u, floor_u, max_outer=args.max_outer)
+            torch.cuda.synchronize()
+            seq_t.append(time.perf_counter() - t)
+            seq_it.append(it)
+            seq_conv += int(ok)
+            print(f'# sequential case {k} (N={n}, Re={re:.0f}): {it} iterations, '
+                  f'{seq_t[-1]:.2f} s (batched: {iters[k]})', file=sys.stderr, flush=True)
+            a = model.cell_values.cpu().numpy().astype(np.float64)
+            b = arena.cell_values(k).astype(np.float64)
+            dev.append(float(np.linalg.norm(a[:, :2] - b[:, :2]) / np.linalg.norm(a[:, :2])))
+            del sv, model
+        tot = float(np.sum(seq_t))
+        seq = {'total_s': round(tot, 3), 'solves_per_s': round(len(draws) / tot, 4),
+               'converged': seq_conv,
+               'outer_iterations': {'sum': int(np.sum(seq_it)), 'max': int(np.max(seq_it))},
+               'max_rel_l2_velocity_vs_batched': float(np.max(dev))}
+        print(f'# sequential: {tot:.2f} s', file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_solve_baseline(draws, stop, pol['rtol0'])
+
+    value = len(specs) / batched_s
+    out = {
+        'metric': 'SIMPLE iter/s + %HBM roofline; served solves/s, p50/p99 at equal residual',
+        'value': round(value, 4),
+        'unit': 'solves/s',
+        'n_gpus': 1,
+        'steps': 1,
+        'warmup': 0,
+        'ms_per_step': round(1e3 * batched_s, 1),
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32',
+        'data': 'synthetic',
+        'config': {'workload': f'{len(specs)} lid-driven cavities in one arena, N log-uniform in '
+                               f'[32,256], Re uniform in [100,1000] (seed {args.seed}), steady '
+                               f'SIMPLE (alpha_u 0.7, alpha_p 0.3) to continuity <= {stop:g} and '
+                               f'estimated velocity error <= {stop_u:g}', 'cases': len(specs),
+                   'cells': cells, 'chunk': args.chunk, 'block': args.block,
+                   'krylov': {'atol': 0.0, 'tol': 0.0, 'rtol0': pol['rtol0'],
+                              'maxiter': pol['maxiter'], 'check_every': 10,
+                              'check_every_momentum': pol['check_every_u'],
+                              'pressure_cg': 'single'},
+                   'stop': {'continuity': stop, 'velocity_error_estimate': stop_u,
+                            'velocity_change_floor': floor_u},
+                   'reductions': 'f64'},
+        'batched': {'total_s': round(batched_s, 3), 'batch_iterations': steps,
+                    'converged': n_conv, 'outer_iterations_max': int(max(iters)),
+                    'time_to_converged_s': {'p50': round(float(np.percentile(lat, 50)), 3),
+                                            'p99': round(float(np.percentile(lat, 99)), 3)}},
+        'sequential': seq,
+        'speedup_vs_sequential': round(seq['total_s'] / batched_s, 2) if seq else None,
+        'host_preprocessing_s': round(prep_s, 1),
+        'roofline': roofline,
+        'cpu_baseline': cpu,
+        'kernels': kernels,
+    }
+    print(json.dumps(out))
+
+
+def cpu_solve_baseline(draws, stop, rtol0, budget_s=25.0):
+    """CPU oracle, one thread: the smallest case of the batch solved to the
+    same continuity target with the same Krylov policy (or as far as the
+    time budget allows), reported as solves/s of that case."""
+    try:
+        from threadpoolctl import threadpool_limits
+        limiter = threadpool_limits(limits=1)
+    except Exception:
+        limiter = None
+    from oracle.mesh import OracleMesh
+    from oracle.fv import OracleFVM
+    from oracle.simple import OracleSolver
+    from warpcfd_amd.preprocess import cavity_mesh
+    n, re = min(draws)
+    nodes, cells = cavity_mesh(n)
+    om = OracleMesh(nodes, cells)
+    om.define_boundary_walls()
+    of = OracleFVM(om)
+    of.set_reference_pressure(0, 0.0)
+    apply_bcs(of.boundary, 'cavity')
+    of.boundary.finalize()
+    s = OracleSolver(of, 0.7, 0.3, nu=1.0 / re, atol=0.0, tol=0.0, rtol0=rtol0, maxiter=2000,
+                     cg_variant=1)
+    t0 = time.perf_counter()
+    it = 0
+    cont = 1.0
+    while time.perf_counter() - t0 < budget_s:
+        cont = s.outer_iteration()['continuity']
+        it += 1
+        if cont <= stop:
+            break
+    dt = time.perf_counter() - t0
+    if limiter is not None and hasattr(limiter, 'unregister'):
+        limiter.unregister()
+    done = cont <= stop
+    return {'value': round((1.0 / dt) if done else 0.0, 5), 'unit': 'solves/s', 'cores': 1,
+            'kind': 'port',
+            'sample': f'smallest batch case (cavity {n}x{n}, Re {re:.0f}) by the oracle, 1 thread: '
+                      f'{it} SIMPLE iterations in {dt:.1f} s, continuity {cont:.2e}'
+                      + ('' if done else ' (budget reached before the target; value 0)')}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=3)
-    ap.add_argument('--workload', default='channel', choices=sorted(WORKLOADS))
+    ap.add_argument('--workload', default='channel', choices=sorted(WORKLOADS) + ['batch64'])
+    ap.add_argument('--seed', type=int, default=0)
+    ap.add_argument('--cases', type=int, default=64)
+    ap.add_argument('--chunk', type=int, default=256, help='batch: rows per workgroup')
+    ap.add_argument('--block', type=int, default=256, help='batch: threads per workgroup')
+    ap.add_argument('--rtol0', type=float, default=0.1,
+                    help='batch: Krylov stop relative to the warm-start residual')
+    ap.add_argument('--max-outer', type=int, default=40000)
+    ap.add_argument('--stop-u', type=float, default=2e-5,
+                    help='batch: stop at this estimated velocity error (with continuity <= 1e-6)')
+    ap.add_argument('--check-u', type=int, default=2,
+                    help='batch: check interval of the momentum solve')
+    ap.add_argument('--no-sequential', action='store_true')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-profile', action='store_true')
     ap.add_argument('--no-tune', action='store_true')
@@ -145,6 +358,8 @@ def main():
               '(cases shard across GPUs with no communication; see the design notes).',
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == 'batch64':
+        return bench_batch64(args)
 
     import numpy as np
     import torch

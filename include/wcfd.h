@@ -97,13 +97,13 @@ typedef struct wcfd_bc {
  * [blk_off[k], blk_off[k+1]) of `block` threads, each owning `chunk`
  * consecutive rows, so a case's arithmetic -- including every float64 sum --
  * is bitwise the same alone or beside any other cases.  Kernels skip cases
- * whose active flag is 0; the SIMPLE step clears the flag of a case whose
- * continuity residual reaches its stop value. */
+ * whose active flag is 0; the SIMPLE step clears the flag of a case once its
+ * convergence monitor (wcfd_monitor_update, below) reports it converged. */
 typedef struct wcfd_batch {
   int32_t n_cases;          /* K */
   int32_t n_blocks;         /* G = blk_off[K] */
   int32_t chunk;            /* rows per workgroup */
-  int32_t block;            /* threads per workgroup (64..1024, multiple of 64) */
+  int32_t block;            /* threads per workgroup (64..512, multiple of 64) */
   const int32_t* cell_off;  /* [K+1] first cell of case k */
   const int32_t* cell_end;  /* [K] one past the last cell of case k; NULL: cell_off[k+1] */
   const int32_t* blk_off;   /* [K+1] */
@@ -114,9 +114,28 @@ typedef struct wcfd_batch {
   const float* nu;          /* [K] kinematic viscosity per case */
   const int32_t* ref_cell;  /* [K] reference-pressure row per case, -1 none */
   const float* ref_value;   /* [K] */
-  const double* stop;       /* [K] continuity residual at which the case stops, 0 never */
+  const double* stop;       /* [K][3] stop_c, stop_u, floor_u of wcfd_monitor_update;
+                               stop_c = 0: never stop */
+  double* monitor;          /* [K][WCFD_MONITOR_LEN] monitor records, zeroed at the start */
   int32_t* outer_iters;     /* [K] SIMPLE iterations run per case */
 } wcfd_batch;
+
+/* Convergence monitor of a steady run (no counterpart in the reference,
+ * whose check_convergence, model.py:278-315 / Convergence.py:7-106, prints
+ * norms on the host).  Called once per outer iteration with the continuity
+ * residual c and the relative velocity change d = ||u^n - u^(n-1)|| /
+ * ||u^n|| (float64 sums).  SIMPLE converges linearly, so the distance to
+ * the fixed point is estimated as E = dbar rho / (1 - rho), dbar the mean of
+ * d over the last 20 iterations and rho = (D_now / D_prev)^(1/20) from the
+ * sums of d over the last two windows of 20 (E = inf while rho >= 1 or
+ * fewer than 40 iterations are known).  Returns 1 when
+ *   c <= stop_c  and  (E <= stop_u  or  dbar <= floor_u),
+ * else 0.  record: [host] WCFD_MONITOR_LEN doubles, zeroed before the first
+ * call; record[1..5] = c, d, dbar, rho, E.  The batched SIMPLE step runs the
+ * same function on the device. */
+#define WCFD_MONITOR_LEN 48
+int wcfd_monitor_update(double* record, double continuity, double velocity_change,
+                        double stop_c, double stop_u, double floor_u);
 
 /* ---------------------------------------------------------------- errors */
 int wcfd_last_hip_error(void);
@@ -205,8 +224,9 @@ int wcfd_relax_pressure(const wcfd_mesh* m, const float* p_new, float alpha,
  * cell_values[u,v,w] = HbyA.  Also the float64 continuity residual
  * sum_c|sum_f +-F_c| / sum_c sum_f |F_c| of the corrected flux
  * F_c = F - rUA_f A_f (p_N - p_O)/|d|, written to resid[0..1]
- * (numerator, denominator) if resid != NULL.  Needs a workspace of
- * wcfd_reduce_workspace_bytes(F) bytes. */
+ * (numerator, denominator) if resid != NULL; resid[2..3] are set to 0
+ * (the SIMPLE step writes ||u^n - u^(n-1)||^2, ||u^n||^2 there).  Needs a
+ * workspace of wcfd_reduce_workspace_bytes(F) bytes. */
 int wcfd_correct_velocity(const wcfd_mesh* m, wcfd_bc bc, float* cell_values,
                           float* cell_grad, float* face_values, const float* mass_flux,
                           const float* rua, const float* rua_face, float* hbya,
@@ -260,7 +280,7 @@ int wcfd_bicgstab(int32_t n, const int32_t* row_ptr, const int32_t* col, const f
 
 /* Launch shape of the Krylov kernels: `grid` workgroups (grid-stride,
  * 1..2048; every workgroup of a kernel folds the previous kernel's `grid`
- * partials) of `block` threads (64..1024, multiple of 64).  Process-wide;
+ * partials) of `block` threads (64..512, multiple of 64).  Process-wide;
  * chosen by the autotuner (warpcfd_amd/tune.py).  No counterpart in the
  * reference (Warp picks its own launch shapes). */
 int wcfd_set_krylov_launch(int grid, int block);
@@ -292,6 +312,7 @@ typedef struct wcfd_simple_params {
   int32_t maxiter, check_every;
   int32_t first_iteration; /* 1: u,v,w gradients may be non-zero on entry */
   int32_t cg_variant;      /* WCFD_CG_* for the pressure equation */
+  int32_t check_every_u;   /* check interval of the momentum solve; 0: check_every */
 } wcfd_simple_params;
 
 /* Device state of one case. */
@@ -308,13 +329,15 @@ typedef struct wcfd_simple_state {
   float* mom_rhs;      /* [3][C] */
   float* p_values;     /* [7][C] ELL */
   float* p_rhs;        /* [C] */
-  double* resid;       /* [2] continuity numerator / denominator */
+  double* resid;       /* [4] (batch: [K][4]) continuity numerator, denominator,
+                          ||u^n - u^(n-1)||^2, ||u^n||^2 */
 } wcfd_simple_state;
 
 typedef struct wcfd_simple_info {
   wcfd_solve_info momentum;
   wcfd_solve_info pressure; /* last corrector */
   double continuity;        /* numerator / denominator */
+  double velocity_change;   /* ||u^n - u^(n-1)|| / ||u^n|| */
 } wcfd_simple_info;
 
 size_t wcfd_simple_workspace_bytes(int32_t n_cells, int32_t n_faces);
@@ -345,11 +368,11 @@ size_t wcfd_batch_simple_workspace_bytes(int32_t n_cells, int32_t n_faces, int32
 /* One SIMPLE outer iteration of every active case of the batch
  * (IncompressibleSolver.run, incompressible.py:95-156, per case).  prm's
  * nu, ref_cell and ref_value are ignored (per case in the batch).
- * st->resid is [K][2].  After the corrector each active case's continuity
- * residual is compared with b->stop[k]; cases at or below it get active = 0
- * (on the device, no host round trip) and b->outer_iters[k] counts the
- * iterations each case ran.  info is [host] [K] or NULL (no sync at the
- * end). */
+ * st->resid is [K][4].  After the corrector each active case's monitor
+ * record is updated (wcfd_monitor_update with b->stop[k]); converged cases
+ * get active = 0 on the device, with no host round trip, and
+ * b->outer_iters[k] counts the iterations each case ran.  info is [host]
+ * [K] or NULL (no sync at the end). */
 int wcfd_batch_simple_iteration(const wcfd_mesh* m, const wcfd_batch* b, wcfd_bc bc,
                                 const wcfd_simple_params* prm, const wcfd_simple_state* st,
                                 void* workspace, wcfd_simple_info* info, void* stream);

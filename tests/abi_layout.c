@@ -11,12 +11,12 @@ int main(void) {
   printf("wcfd_solve_info %zu\n", sizeof(wcfd_solve_info));
   P(wcfd_solve_info, residual);
   printf("wcfd_simple_params %zu\n", sizeof(wcfd_simple_params));
-  P(wcfd_simple_params, atol); P(wcfd_simple_params, rtol0); P(wcfd_simple_params, maxiter); P(wcfd_simple_params, first_iteration);
+  P(wcfd_simple_params, atol); P(wcfd_simple_params, rtol0); P(wcfd_simple_params, maxiter); P(wcfd_simple_params, first_iteration); P(wcfd_simple_params, check_every_u);
   printf("wcfd_simple_state %zu\n", sizeof(wcfd_simple_state));
   P(wcfd_simple_state, resid);
   printf("wcfd_simple_info %zu\n", sizeof(wcfd_simple_info));
-  P(wcfd_simple_info, continuity);
+  P(wcfd_simple_info, continuity); P(wcfd_simple_info, velocity_change);
   printf("wcfd_batch %zu\n", sizeof(wcfd_batch));
-  P(wcfd_batch, cell_off); P(wcfd_batch, active); P(wcfd_batch, stop); P(wcfd_batch, outer_iters);
+  P(wcfd_batch, cell_off); P(wcfd_batch, active); P(wcfd_batch, stop); P(wcfd_batch, monitor); P(wcfd_batch, outer_iters);
   return 0;
 }

@@ -1,0 +1,114 @@
+"""Host logic of the batched path (no GPU): the arena's range allocator,
+the batch workload draws and the oracle's warm-start-relative Krylov stop."""
+import numpy as np
+import scipy.sparse as sp
+
+from oracle import krylov
+from warpcfd_amd.batch import _Ranges
+from warpcfd_amd.workloads import batch64_draws
+
+
+def test_ranges_first_fit_and_coalesce():
+    r = _Ranges(100)
+    a = r.alloc(30)
+    b = r.alloc(30)
+    c = r.alloc(30)
+    assert (a, b, c) == (0, 30, 60)
+    assert r.alloc(20) is None            # only 10 left
+    r.release(b, 30)
+    assert r.alloc(25) == 30              # first fit into the hole
+    r.release(a, 30)
+    r.release(30, 25)
+    assert r.free == [(0, 60), (90, 10)]  # coalesced
+    assert r.largest() == 60
+    r.release(c, 30)
+    assert r.free == [(0, 100)]
+    assert r.alloc(0) == 0
+
+
+def test_batch64_draws_match_the_stated_distribution():
+    d = batch64_draws(0)
+    assert len(d) == 64
+    N = np.array([n for n, _ in d])
+    Re = np.array([r for _, r in d])
+    assert N.min() >= 32 and N.max() <= 256 and Re.min() >= 100 and Re.max() <= 1000
+    # the seed-0 batch: sum of N^2 as documented for the workload
+    assert int((N * N).sum()) == 976073
+    assert d == batch64_draws(0)
+
+
+def _spd(n=120, seed=0):
+    rng = np.random.default_rng(seed)
+    A = sp.random(n, n, density=0.05, random_state=seed, format='csr')
+    A = (A + A.T) * 0.5
+    A = A + sp.eye(n) * (np.abs(A).sum(axis=1).max() + 0.5)
+    return A.tocsr().astype(np.float64), rng.standard_normal(n)
+
+
+def test_rtol0_stops_relative_to_the_warm_start_residual():
+    A, b = _spd()
+    inv = 1.0 / A.diagonal()
+    x0 = np.zeros_like(b)
+    r0 = np.linalg.norm(b - A @ x0)
+    for fn in (krylov.cg, krylov.cg_single_reduction, krylov.bicgstab):
+        x = x0.copy()
+        it, res = fn(A, b, x, inv, atol=0.0, tol=0.0, rtol0=1e-3, maxiter=500, check_every=1)
+        assert res <= 1e-3 * r0 * (1 + 1e-12)
+        assert np.linalg.norm(b - A @ x) <= 2e-3 * r0
+        # a stricter relative stop needs more iterations
+        x2 = x0.copy()
+        it2, _ = fn(A, b, x2, inv, atol=0.0, tol=0.0, rtol0=1e-8, maxiter=500, check_every=1)
+        assert it2 > it
+    # warm start near the solution: the stop is relative to the warm-start
+    # residual, not to ||b|| (a tol on ||b|| of the same size stops at once)
+    xs = sp.linalg.spsolve(A.tocsc(), b)
+    xw = xs + 1e-4 * np.random.default_rng(1).standard_normal(b.size)
+    rw = np.linalg.norm(b - A @ xw)
+    x = xw.copy()
+    it, res = krylov.cg(A, b, x, inv, atol=0.0, tol=0.0, rtol0=1e-3, maxiter=500, check_every=1)
+    assert it > 0 and res <= 1e-3 * rw
+    x = xw.copy()
+    it_b, _ = krylov.cg(A, b, x, inv, atol=0.0, tol=1e-3, maxiter=500, check_every=1)
+    assert it_b == 0
+
+
+def _geometric_run(mon, rho, d0, n, c=1e-7):
+    out = None
+    for i in range(n):
+        out = mon.update(c, d0 * rho ** i)
+    return out
+
+
+def test_monitor_error_estimate_on_a_geometric_sequence():
+    """For d_n = d0 rho^n the exact distance to the limit of the iterates is
+    d_n rho / (1 - rho); the monitor's estimate must reproduce it."""
+    from warpcfd_amd.FV.implicit_Solvers.incompressible import Monitor
+    for rho in (0.9, 0.99, 0.998):
+        mon = Monitor(1e-6, 1e-300, 0.0)
+        _geometric_run(mon, rho, 1e-3, 200)
+        c, d, dbar, r, E = mon.estimate
+        assert abs(r - rho) < 1e-9
+        W = 20
+        dbar_exact = 1e-3 * rho ** 199 * sum(rho ** -j for j in range(W)) / W
+        assert abs(dbar - dbar_exact) <= 1e-12 * dbar_exact + 1e-30
+        assert abs(E - dbar_exact * rho / (1 - rho)) <= 1e-9 * E
+
+
+def test_monitor_rule():
+    from warpcfd_amd.FV.implicit_Solvers.incompressible import Monitor
+    # continuity above the stop: never converged, however small the change
+    mon = Monitor(1e-6, 5e-5, 1e-7)
+    assert not _geometric_run(mon, 0.5, 1e-3, 100, c=2e-6)
+    # fewer than 2 windows: no estimate
+    mon = Monitor(1e-6, 5e-5, 0.0)
+    assert not _geometric_run(mon, 0.5, 1e-3, 39)
+    assert mon.estimate[4] == float('inf')
+    assert mon.update(1e-7, 1e-3 * 0.5 ** 39)      # 40th value: E tiny
+    # non-decaying change above the floor: not converged; at the floor: converged
+    mon = Monitor(1e-6, 5e-5, 1e-7)
+    assert not _geometric_run(mon, 1.0, 1e-6, 100)
+    mon = Monitor(1e-6, 5e-5, 1e-7)
+    assert _geometric_run(mon, 1.0, 5e-8, 100)
+    # slow decay: E = dbar rho/(1-rho) above the target until d is small enough
+    mon = Monitor(1e-6, 5e-5, 0.0)
+    assert not _geometric_run(mon, 0.999, 1e-6, 100)   # E ~ 1e-3

@@ -10,8 +10,10 @@ pytestmark = pytest.mark.gpu
 G = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden')
 
 
-def test_cavity24_converged_vs_fp64_oracle(gpu):
-    """Converged float32 GPU fields vs the float64 oracle: <= 1e-4."""
+def test_cavity24_fixed_point_vs_fp64_oracle(gpu):
+    """700 iterations with tight Krylov tolerances reach the fixed point: the
+    float32 GPU fields agree with the float64 oracle's fixed point to 1e-5
+    (the float32 oracle's own distance is ~5e-7)."""
     from warpcfd_amd.FV.implicit_Solvers import IncompressibleSolver
     from warpcfd_amd.preprocess.grid import cavity_mesh
     z = np.load(os.path.join(G, 'cavity24_converged.npz'))
@@ -19,13 +21,36 @@ def test_cavity24_converged_vs_fp64_oracle(gpu):
     model = device_case(nodes, cells, 'cavity')
     sv = IncompressibleSolver(model, 0.7, 0.3)
     sv.tol = 1e-9
-    for _ in range(400):
-        if sv.outer_iteration()['continuity'] < 1e-7:
-            break
+    for _ in range(700):
+        d = sv.outer_iteration()
+    assert d['continuity'] < 1e-7
     got = model.cell_values.cpu().numpy()
-    for q in (0, 1, 3):
-        assert rel_l2(got[:, q], z['f64'][:, q]) <= 1e-4, q
-        assert rel_l2(got[:, q], z['f32'][:, q]) <= 1e-4, q
+    err = [rel_l2(got[:, q], z['f64'][:, q]) for q in (0, 1, 3)]
+    print('rel-L2 vs float64 fixed point (u, v, p):', err)
+    assert max(err) <= 1e-5
+
+
+@pytest.mark.parametrize('rtol0', [0.0, 0.1])
+def test_cavity24_monitor_stop_vs_fp64_oracle(gpu, rtol0):
+    """Stopped by the convergence monitor (continuity <= 1e-6 and estimated
+    velocity error <= 5e-5): within 1e-4 of the float64 fixed point, for the
+    reference Krylov setting and for the warm-start-relative one."""
+    from warpcfd_amd.FV.implicit_Solvers import IncompressibleSolver
+    from warpcfd_amd.preprocess.grid import cavity_mesh
+    z = np.load(os.path.join(G, 'cavity24_converged.npz'))
+    nodes, cells = cavity_mesh(24)
+    model = device_case(nodes, cells, 'cavity')
+    sv = IncompressibleSolver(model, 0.7, 0.3)
+    if rtol0:
+        sv.tol, sv.rtol, sv.rtol0 = 0.0, 0.0, rtol0
+    else:
+        sv.tol = 1e-9
+    it, done, mon = sv.run_to_convergence(1e-6, 5e-5, 1e-7, max_outer=2000)
+    assert done
+    got = model.cell_values.cpu().numpy()
+    err = [rel_l2(got[:, q], z['f64'][:, q]) for q in (0, 1)]
+    print(f'stopped after {it} iterations, monitor {mon.estimate}, rel-L2 (u, v) {err}')
+    assert max(err) <= 1e-4
 
 
 def test_cavity64_500_iterations_reference_settings(gpu):

@@ -14,6 +14,31 @@ from ... import _lib
 from ..device import ptr, stream_ptr
 
 
+class Monitor:
+    """Host side of the convergence monitor of a steady run: the same rule
+    (``wcfd_monitor_update`` in libwcfd.so) that the batched SIMPLE step runs
+    on the device -- continuity <= stop_c and an a-posteriori estimate of the
+    velocity error from the decay rate of the per-iteration velocity change
+    <= stop_u (or that change at the float32 floor, <= floor_u)."""
+
+    def __init__(self, stop_c=1e-6, stop_u=5e-5, floor_u=1e-7):
+        self.stop = (float(stop_c), float(stop_u), float(floor_u))
+        self.record = (ctypes.c_double * _lib.MONITOR_LEN)()
+        self.lib = _lib.load()
+
+    def update(self, continuity, velocity_change):
+        r = self.lib.wcfd_monitor_update(self.record, float(continuity), float(velocity_change),
+                                         *self.stop)
+        if r < 0:
+            _lib.check(-r, 'monitor')
+        return r == 1
+
+    @property
+    def estimate(self):
+        """(continuity, velocity change, mean change, rate, error estimate)."""
+        return tuple(self.record[i] for i in range(1, 6))
+
+
 class IncompressibleSolver:
     def __init__(self, model, u_relaxation_factor=None, p_relaxation_factor=None, correction=False,
                  steady_state=True, implicit_outflow=False):
@@ -48,7 +73,7 @@ class IncompressibleSolver:
         self.mom_rhs = z(3, C)
         self.p_values = z(7, C)
         self.p_rhs = z(C)
-        self.resid = torch.zeros(2, dtype=torch.float64, device=dev)
+        self.resid = torch.zeros(4, dtype=torch.float64, device=dev)
         self.workspace = torch.zeros(self.lib.wcfd_simple_workspace_bytes(C, F), dtype=torch.uint8,
                                      device=dev)
         self.max_iter = 500
@@ -58,6 +83,9 @@ class IncompressibleSolver:
         # residual); 0 = off
         self.rtol0 = 0.0
         self.check_every = 10
+        # momentum solve check interval; None: check_every (the reference's
+        # single setting)
+        self.check_every_u = None
         # pressure CG: WCFD_CG_STANDARD (the reference's two-reduction form) or
         # WCFD_CG_SINGLE_REDUCTION (same iterates in exact arithmetic, one
         # fused kernel per iteration)
@@ -107,6 +135,7 @@ class IncompressibleSolver:
         p.rtol0 = float(self.rtol0)
         p.maxiter = int(self.max_iter)
         p.check_every = int(self.check_every)
+        p.check_every_u = int(self.check_every_u or 0)
         p.first_iteration = 0 if m._uvw_grad_zero else 1
         p.cg_variant = int(self.cg_variant)
         return p
@@ -123,11 +152,25 @@ class IncompressibleSolver:
         m._uvw_grad_zero = True
         d = {'u_iters': info.momentum.iterations, 'u_residual': info.momentum.residual,
              'p_iters': info.pressure.iterations, 'p_residual': info.pressure.residual,
-             'continuity': info.continuity}
+             'continuity': info.continuity, 'velocity_change': info.velocity_change}
         self.history.append(d)
         if not np.isfinite(info.momentum.residual):
             raise ValueError('momentum residual is not finite')
         return d
+
+    def run_to_convergence(self, stop_c=1e-6, stop_u=5e-5, floor_u=1e-7, max_outer=20000):
+        """Steady run until the convergence monitor (``Monitor``: continuity
+        <= stop_c and estimated velocity error <= stop_u) reports converged;
+        not in the reference, whose run() has no stop test.  Returns
+        (iterations, converged, monitor)."""
+        mon = Monitor(stop_c, stop_u, floor_u)
+        it = 0
+        done = False
+        while it < max_outer and not done:
+            d = self.outer_iteration()
+            it += 1
+            done = mon.update(d['continuity'], d['velocity_change'])
+        return it, done, mon
 
     def run(self, steps_per_check=10, callback=None):
         """incompressible.py:72-172 (no early stop, as the reference)."""

@@ -16,6 +16,7 @@ WCFD_OK, WCFD_EINVAL, WCFD_EHIP, WCFD_ENOTCONV, WCFD_ENAN = 0, 1, 2, 3, 4
 DIRICHLET, NEUMANN = 1, 2
 SCHEME_CORRECTED, SCHEME_IMPLICIT_OUTFLOW = 1, 2
 CG_STANDARD, CG_SINGLE_REDUCTION = 0, 1
+MONITOR_LEN = 48
 
 c_p = ctypes.c_void_p
 c_i32 = ctypes.c_int32
@@ -49,7 +50,7 @@ class WcfdBatch(ctypes.Structure):
     _fields_ = [('n_cases', c_i32), ('n_blocks', c_i32), ('chunk', c_i32), ('block', c_i32),
                 ('cell_off', c_p), ('cell_end', c_p), ('blk_off', c_p), ('blk_case', c_p), ('cell_case', c_p),
                 ('face_case', c_p), ('active', c_p), ('nu', c_p), ('ref_cell', c_p),
-                ('ref_value', c_p), ('stop', c_p), ('outer_iters', c_p)]
+                ('ref_value', c_p), ('stop', c_p), ('monitor', c_p), ('outer_iters', c_p)]
 
 
 class SimpleParams(ctypes.Structure):
@@ -57,7 +58,7 @@ class SimpleParams(ctypes.Structure):
                 ('scheme_flags', c_i32), ('ref_cell', c_i32), ('ref_value', c_f),
                 ('n_inner', c_i32), ('n_ortho', c_i32), ('atol', c_d), ('tol', c_d), ('rtol0', c_d),
                 ('maxiter', c_i32), ('check_every', c_i32), ('first_iteration', c_i32),
-                ('cg_variant', c_i32)]
+                ('cg_variant', c_i32), ('check_every_u', c_i32)]
 
 
 class SimpleState(ctypes.Structure):
@@ -67,7 +68,8 @@ class SimpleState(ctypes.Structure):
 
 
 class SimpleInfo(ctypes.Structure):
-    _fields_ = [('momentum', SolveInfo), ('pressure', SolveInfo), ('continuity', c_d)]
+    _fields_ = [('momentum', SolveInfo), ('pressure', SolveInfo), ('continuity', c_d),
+                ('velocity_change', c_d)]
 
 
 _PM = ctypes.POINTER(WcfdMesh)
@@ -115,6 +117,7 @@ _SIGS = {
     'wcfd_simple_iteration': (ctypes.c_int, [_PM, WcfdBC, ctypes.POINTER(SimpleParams),
                                              ctypes.POINTER(SimpleState), c_p,
                                              ctypes.POINTER(SimpleInfo), c_p]),
+    'wcfd_monitor_update': (ctypes.c_int, [ctypes.POINTER(c_d), c_d, c_d, c_d, c_d, c_d]),
     'wcfd_batch_krylov_workspace_bytes': (ctypes.c_size_t, [c_i32, c_i32, c_i32]),
     'wcfd_batch_cg_ell': (ctypes.c_int, [_PM, _PB, c_p, ctypes.c_int, c_p, c_p, c_d, c_d,
                                          ctypes.c_int, ctypes.c_int, c_p, ctypes.POINTER(SolveInfo),
@@ -146,6 +149,10 @@ def load(path=LIB_PATH):
     global _lib
     if _lib is not None:
         return _lib
+    # torch first: it loads its own HIP runtime; libwcfd.so must bind to that
+    # same runtime (loaded the other way round, HIP calls fail with
+    # hipErrorNoDevice on tensors torch allocated)
+    import torch  # noqa: F401
     if not os.path.exists(path):
         raise WcfdError(f'{LIB_NAME} is not built ({path}); run __graft_entry__.build() first')
     lib = ctypes.CDLL(path)

@@ -36,11 +36,15 @@ class CaseSpec:
 
     ``model`` is a ``warpcfd_amd.FV.FVM`` whose boundary conditions, material
     and reference pressure are set (it need not be finalized; no device
-    memory of its own is used).  ``stop`` is the continuity residual at which
-    the case is converged, ``max_outer`` a cap on its SIMPLE iterations.
+    memory of its own is used).  The case is converged when its continuity
+    residual is <= ``stop`` and the convergence monitor's estimate of the
+    velocity error is <= ``stop_u`` (or the per-iteration velocity change
+    sits at the float32 floor, <= ``floor_u``); ``max_outer`` caps its SIMPLE
+    iterations.
     """
 
-    def __init__(self, model, stop=1e-6, max_outer=20000, name=None, layout=None):
+    def __init__(self, model, stop=1e-6, max_outer=20000, name=None, layout=None, stop_u=5e-5,
+                 floor_u=1e-7):
         b = model.boundary
         b.finalize()
         if not b.uniform_velocity_types():
@@ -58,6 +62,8 @@ class CaseSpec:
         self.ref_cell = -1 if rc is None else int(L.old2new[rc])
         self.ref_value = float(model.reference_pressure)
         self.stop = float(stop)
+        self.stop_u = float(stop_u)
+        self.floor_u = float(floor_u)
         self.max_outer = int(max_outer)
         self.name = name
         self.n_cells = L.num_cells
@@ -131,15 +137,15 @@ class CaseArena:
 
     def __init__(self, cell_cap, internal_cap, boundary_cap, max_cases, device='cuda',
                  chunk=256, block=256, density=1.0, alpha_u=0.7, alpha_p=0.3, atol=0.0,
-                 tol=1e-6, rtol0=0.0, maxiter=2000, check_every=10,
+                 tol=1e-6, rtol0=0.0, maxiter=2000, check_every=10, check_every_u=0,
                  cg_variant=_lib.CG_SINGLE_REDUCTION, implicit_outflow=True, correction=False):
         self.device = require_gpu(device)
         self.lib = _lib.load()
         Cc, Fic, Fbc, K = int(cell_cap), int(internal_cap), int(boundary_cap), int(max_cases)
         if min(Cc, Fic, Fbc, K) <= 0:
             raise ValueError('capacities must be positive')
-        if block % 64 or not 64 <= block <= 1024 or chunk <= 0:
-            raise ValueError('block must be a multiple of 64 in [64, 1024], chunk > 0')
+        if block % 64 or not 64 <= block <= 512 or chunk <= 0:
+            raise ValueError('block must be a multiple of 64 in [64, 512], chunk > 0')
         self.Cc, self.Fic, self.Fbc, self.K = Cc, Fic, Fbc, K
         self.F = Fic + Fbc
         self.chunk, self.block = int(chunk), int(block)
@@ -180,7 +186,7 @@ class CaseArena:
         self.mom_rhs = z(3, Cc)
         self.p_values = z(7, Cc)
         self.p_rhs = z(Cc)
-        self.resid = torch.zeros(K, 2, dtype=torch.float64, device=dev)
+        self.resid = torch.zeros(K, 4, dtype=torch.float64, device=dev)
         # per case
         self.cell_off = z(K + 1, dt=i32)
         self.cell_end = z(K, dt=i32)
@@ -190,7 +196,8 @@ class CaseArena:
         self.nu = z(K)
         self.ref_cell = torch.full((K,), -1, dtype=i32, device=dev)
         self.ref_value = z(K)
-        self.stop = torch.zeros(K, dtype=torch.float64, device=dev)
+        self.stop = torch.zeros(K, 3, dtype=torch.float64, device=dev)
+        self.monitor = torch.zeros(K, _lib.MONITOR_LEN, dtype=torch.float64, device=dev)
         self.outer_iters = z(K, dt=i32)
         self.workspace = torch.zeros(
             self.lib.wcfd_batch_simple_workspace_bytes(Cc, F, K, self.Gmax), dtype=torch.uint8,
@@ -209,7 +216,7 @@ class CaseArena:
         self.density = float(density)
         self.alpha_u, self.alpha_p = alpha_u, alpha_p
         self.atol, self.tol, self.rtol0 = atol, tol, rtol0
-        self.maxiter, self.check_every = maxiter, check_every
+        self.maxiter, self.check_every, self.check_every_u = maxiter, check_every, check_every_u
         self.cg_variant = cg_variant
         self.scheme_flags = (_lib.SCHEME_IMPLICIT_OUTFLOW if implicit_outflow else 0) | \
             (_lib.SCHEME_CORRECTED if correction else 0)
@@ -347,7 +354,8 @@ class CaseArena:
         self.nu[k] = s.nu
         self.ref_cell[k] = (c0 + s.ref_cell) if s.ref_cell >= 0 else -1
         self.ref_value[k] = s.ref_value
-        self.stop[k] = s.stop
+        self.stop[k] = torch.tensor([s.stop, s.stop_u, s.floor_u], dtype=torch.float64)
+        self.monitor[k] = 0
         self.outer_iters[k] = 0
         self.resid[k] = 0
         self.active[k] = 1
@@ -383,7 +391,7 @@ class CaseArena:
         b = _lib.WcfdBatch()
         b.n_cases, b.n_blocks, b.chunk, b.block = K, G, self.chunk, self.block
         for name in ('cell_off', 'cell_end', 'blk_off', 'blk_case', 'cell_case', 'face_case',
-                     'active', 'nu', 'ref_cell', 'ref_value', 'stop', 'outer_iters'):
+                     'active', 'nu', 'ref_cell', 'ref_value', 'stop', 'monitor', 'outer_iters'):
             setattr(b, name, getattr(self, name).data_ptr())
         self.batch = b
         self.bc = _lib.WcfdBC(self.bc_type.data_ptr(), self.bc_value.data_ptr())
@@ -411,6 +419,7 @@ class CaseArena:
         p.n_inner = p.n_ortho = 1
         p.atol, p.tol, p.rtol0 = float(self.atol), float(self.tol), float(self.rtol0)
         p.maxiter, p.check_every = int(self.maxiter), int(self.check_every)
+        p.check_every_u = int(self.check_every_u)
         p.first_iteration = 0
         p.cg_variant = int(self.cg_variant)
         return p
@@ -452,6 +461,11 @@ class CaseArena:
         r = self.resid[k].cpu().numpy()
         return float(r[0] / r[1]) if r[1] > 0 else 0.0
 
+    def monitor_record(self, k):
+        """(continuity, velocity change, mean change, rate, error estimate)
+        of slot ``k`` after its last iteration."""
+        return tuple(float(v) for v in self.monitor[k, 1:6].cpu().numpy())
+
     def run(self, max_steps=100000, poll_every=1):
         """Iterate until no resident case is active; returns the number of
         batch iterations."""
@@ -474,8 +488,10 @@ class CaseArena:
         return np.ascontiguousarray(dev[:, s.layout.old2new].T)
 
     def converged(self, k):
-        sl = self.slots[k]
-        return self.continuity(k) <= sl['spec'].stop
+        """The monitor's verdict for slot ``k`` (the rule the device applied)."""
+        s = self.slots[k]['spec']
+        c, _, dbar, _, E = self.monitor_record(k)
+        return c <= s.stop and (E <= s.stop_u or dbar <= s.floor_u)
 
     def iterations(self, k):
         return int(self.outer_iters[k].item())

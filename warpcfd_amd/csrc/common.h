@@ -13,6 +13,10 @@ constexpr int kWave = 64;             // CDNA wavefront
 constexpr int kMaxWaves = 16;         // LDS sizing: workgroups of up to 1024 threads
 constexpr int kMaxReduceBlocks = 2048;  // upper bound on partials of one reduction
 constexpr int kKrylovGrid = 1024;       // workgroups of the Krylov kernels (grid-stride)
+// Largest workgroup of the row-parallel Krylov / reduction kernels: their
+// __launch_bounds__, which lets a wave hold up to 256 VGPRs (a whole row of
+// gathers in flight) without spilling.
+constexpr int kMaxKrylovBlock = 512;
 
 // per-thread HIP error of the last failing call
 extern thread_local int g_last_hip_error;
@@ -284,6 +288,7 @@ __device__ inline void store_partials(double (&v)[ND], double* __restrict__ part
 //
 // CsrMat: a general CSR matrix (row_ptr, col, values, diagonal position).
 struct CsrMat {
+  static constexpr bool kEll = false;
   int n;
   const int32_t* rp;
   const int32_t* col;
@@ -304,11 +309,23 @@ struct CsrMat {
 // band fits), halving the index bytes.
 template <bool D16>
 struct EllMat {
+  static constexpr bool kEll = true;
   int n;
   const float* val;
   const int32_t* nbr;
   const int16_t* dnb;
   __device__ float diag(int i) const { return val[i]; }
+  // the row as 7 (coefficient, column) pairs in row() order, for kernels
+  // that issue every load of a row before they wait on anything
+  __device__ void load(int i, float (&a)[7], int (&c)[7]) const {
+    a[0] = val[i];
+    c[0] = i;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      c[k + 1] = D16 ? i + (int)dnb[k * n + i] : nbr[k * n + i];
+      a[k + 1] = val[(k + 1) * n + i];
+    }
+  }
   template <class F>
   __device__ void row(int i, F&& f) const {
     f(val[i], i);

@@ -21,18 +21,22 @@ int launch_assemble_pressure(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc,
                              const float* flux, const float* rua_f, float* cg, int compute_grad_p,
                              int correction, int ref_cell, float ref_value, float* values,
                              float* rhs, float* div_out, hipStream_t s);
+// u_prev (may be NULL): receives u of the previous outer iteration
 int launch_rua_hbya(const wcfd_mesh& m, const BatchDev& bd, const float* values, float density,
-                    const float* cg, float* hbya, float* cv, float* rua, hipStream_t s);
+                    const float* cg, float* hbya, float* cv, float* rua, float* u_prev,
+                    hipStream_t s);
 int launch_relax_p(const wcfd_mesh& m, const BatchDev& bd, const float* pn, float alpha, float* cv,
                    hipStream_t s);
-// Corrector tail.  sg segments the cells for the continuity partials (one
-// case, or the batch); resid gets [K][2] (numerator, denominator).  With a
-// batch (`bt`), cases whose residual reaches bt->stop[k] are deactivated and
-// bt->outer_iters[k] is incremented for every active case.
+// Corrector tail.  sg segments the cells for the float64 partials (one case,
+// or the batch); resid gets [K][4] (continuity numerator and denominator,
+// ||u - u_prev||^2, ||u||^2; the last two 0 without u_prev).  With a batch
+// (`bt`), every active case's iteration count and convergence monitor are
+// updated and converged cases are deactivated.
 int launch_correct_velocity(const wcfd_mesh& m, const BatchDev& bd, const Seg& sg, int block,
                             const wcfd_batch* bt, wcfd_bc bc, float* cv, float* cg, float* fv,
                             const float* flux, const float* rua, const float* rua_f, float* hbya,
-                            double* resid, float* fc_scratch, double* partials, hipStream_t s);
+                            const float* u_prev, double* resid, float* fc_scratch,
+                            double* partials, hipStream_t s);
 
 // Krylov solvers (krylov.hip).  Stopping threshold
 // max(tol ||b||, rtol0 ||r_0||, atol); the reference's call has tol = atol

@@ -7,6 +7,10 @@
 // equation_kernels.py:75,79,101); the sums here run over the same terms in
 // local-face order.
 #include "fv_kernels.h"
+#include "monitor.h"
+
+static_assert(wcfd::kMonHead + 2 * wcfd::kMonWindow <= WCFD_MONITOR_LEN,
+              "monitor record larger than WCFD_MONITOR_LEN");
 
 namespace wcfd {
 
@@ -363,12 +367,17 @@ __global__ void k_assemble_pressure(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
 // ------------------------------------------------------------ rUA, HbyA
 __global__ void k_rua_hbya(wcfd_mesh m, BatchDev bd, const float* __restrict__ values,
                            float density, const float* __restrict__ cg, float* __restrict__ hbya,
-                           float* __restrict__ cv, float* __restrict__ rua) {
+                           float* __restrict__ cv, float* __restrict__ rua,
+                           float* __restrict__ u_prev) {
   const int c = xcd_item();
   const int C = m.n_cells;
   if (c >= C || bd.cell_skip(c)) return;
   const float r = m.volume[c] / (values[c] * density);
   rua[c] = r;
+  if (u_prev) {   // u of the previous outer iteration, for the velocity change
+#pragma unroll
+    for (int q = 0; q < 3; ++q) u_prev[q * C + c] = cv[q * C + c];
+  }
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     const float h = hbya[q * C + c] + cg[(9 + q) * C + c] * r;
@@ -427,11 +436,12 @@ __global__ void k_face_p_fc(wcfd_mesh m, BatchDev bd, wcfd_bc bc, const float* _
 __global__ void k_grad_p_correct(wcfd_mesh m, Seg sg, const float* __restrict__ fv,
                                  float* __restrict__ cg, float* __restrict__ cv,
                                  float* __restrict__ hbya, const float* __restrict__ rua,
-                                 const float* __restrict__ fc, double* __restrict__ partials) {
+                                 const float* __restrict__ fc, const float* __restrict__ u_prev,
+                                 double* __restrict__ partials) {
   const int C = m.n_cells, F = m.n_faces;
   const BlockRows br = block_rows(sg);
   if (sg.active && !sg.active[br.k]) return;   // whole workgroup: one case
-  double acc[2] = {0.0, 0.0};
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
   for (int c = br.begin + (int)threadIdx.x; c < br.end; c += blockDim.x) {
     int fk[6];
 #pragma unroll
@@ -449,6 +459,11 @@ __global__ void k_grad_p_correct(wcfd_mesh m, Seg sg, const float* __restrict__ 
       const float h = hbya[q * C + c] - r * g[q];
       hbya[q * C + c] = h;
       cv[q * C + c] = h;
+      if (fc && u_prev) {
+        const double dq = (double)h - (double)u_prev[q * C + c];
+        acc[2] += dq * dq;
+        acc[3] += (double)h * (double)h;
+      }
     }
     if (fc) {
       double num = 0.0, den = 0.0;
@@ -462,28 +477,35 @@ __global__ void k_grad_p_correct(wcfd_mesh m, Seg sg, const float* __restrict__ 
       acc[1] += den;
     }
   }
-  if (fc) store_partials<2>(acc, partials, sg.G, br.vb);
+  if (fc) store_partials<4>(acc, partials, sg.G, br.vb);
 }
 
-// One workgroup per case: fold the case's continuity partials to
-// resid[2k], resid[2k+1].  With `stop` (a batch), count the iteration and
-// deactivate the case once num/den <= stop[k].
+// One workgroup per case: fold the case's partials to resid[4k .. 4k+3]
+// (continuity numerator, denominator, ||du||^2, ||u||^2).  With a batch `bt`,
+// count the iteration and run the convergence monitor; a converged case is
+// deactivated.
 __global__ void k_fold_resid(Seg sg, const double* __restrict__ partials,
                              double* __restrict__ resid, const double* __restrict__ stop,
-                             int32_t* __restrict__ outer_iters, uint8_t* __restrict__ active) {
+                             double* __restrict__ monitor, int32_t* __restrict__ outer_iters,
+                             uint8_t* __restrict__ active) {
   const int k = blockIdx.x;
   if (sg.active && !sg.active[k]) return;
   int b0, nb;
   case_blocks(sg, k, b0, nb);
-  const double num = fold(partials + b0, nb);
-  const double den = fold(partials + sg.G + b0, nb);
+  double v[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) v[i] = fold(partials + (size_t)i * sg.G + b0, nb);
   if (threadIdx.x != 0) return;
-  resid[2 * k] = num;
-  resid[2 * k + 1] = den;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) resid[4 * k + i] = v[i];
   if (outer_iters) outer_iters[k] += 1;
-  if (stop && active) {
-    const double r = den > 0.0 ? num / den : 0.0;
-    if (stop[k] > 0.0 && r <= stop[k]) active[k] = 0;
+  if (stop && monitor && active) {
+    const double c = v[1] > 0.0 ? v[0] / v[1] : 0.0;
+    const double d = v[3] > 0.0 ? sqrt(v[2] / v[3]) : 0.0;
+    const double* st = stop + 3 * k;
+    const bool conv = monitor_update(monitor + (size_t)k * WCFD_MONITOR_LEN, c, d, st[0], st[1],
+                                     st[2]);
+    if (st[0] > 0.0 && conv) active[k] = 0;
   }
 }
 
@@ -542,9 +564,10 @@ int launch_assemble_pressure(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc,
 }
 
 int launch_rua_hbya(const wcfd_mesh& m, const BatchDev& bd, const float* values, float density,
-                    const float* cg, float* hbya, float* cv, float* rua, hipStream_t s) {
+                    const float* cg, float* hbya, float* cv, float* rua, float* u_prev,
+                    hipStream_t s) {
   WCFD_LAUNCH(K_RUA_HBYA, s, k_rua_hbya, grid_for(m.n_cells), m, bd, values, density, cg, hbya,
-              cv, rua);
+              cv, rua, u_prev);
   return check_launch();
 }
 
@@ -557,18 +580,19 @@ int launch_relax_p(const wcfd_mesh& m, const BatchDev& bd, const float* pn, floa
 int launch_correct_velocity(const wcfd_mesh& m, const BatchDev& bd, const Seg& sg, int block,
                             const wcfd_batch* bt, wcfd_bc bc, float* cv, float* cg, float* fv,
                             const float* flux, const float* rua, const float* rua_f, float* hbya,
-                            double* resid, float* fc_scratch, double* partials, hipStream_t s) {
+                            const float* u_prev, double* resid, float* fc_scratch,
+                            double* partials, hipStream_t s) {
   float* fc = resid ? fc_scratch : nullptr;
   WCFD_LAUNCH(K_FACE_P_FC, s, k_face_p_fc, grid_for(m.n_faces), m, bd, bc, cv, cg, fv, flux,
               rua_f, fc);
   int e = check_launch();
   if (e) return e;
   WCFD_LAUNCH_B(K_GRAD_P_CORR, s, k_grad_p_correct, sg.G, block, m, sg, fv, cg, cv, hbya, rua, fc,
-                partials);
+                u_prev, partials);
   if (resid)
     WCFD_LAUNCH(K_FOLD, s, k_fold_resid, sg.n_cases, sg, partials, resid,
-                bt ? bt->stop : nullptr, bt ? bt->outer_iters : nullptr,
-                bt ? bt->active : nullptr);
+                bt ? bt->stop : nullptr, bt ? bt->monitor : nullptr,
+                bt ? bt->outer_iters : nullptr, bt ? bt->active : nullptr);
   return check_launch();
 }
 
@@ -597,7 +621,13 @@ const char* wcfd_error_string(int code) {
   }
 }
 
-int wcfd_abi_version(void) { return 1; }
+int wcfd_abi_version(void) { return 2; }
+
+int wcfd_monitor_update(double* record, double continuity, double velocity_change,
+                        double stop_c, double stop_u, double floor_u) {
+  if (!record) return -WCFD_EINVAL;
+  return monitor_update(record, continuity, velocity_change, stop_c, stop_u, floor_u) ? 1 : 0;
+}
 
 int wcfd_set_boundary_conditions(const wcfd_mesh* m, wcfd_bc bc, float* face_values,
                                  int field_mask, void* stream) {
@@ -660,7 +690,7 @@ int wcfd_assemble_pressure(const wcfd_mesh* m, wcfd_bc bc, const float* face_val
 int wcfd_rua_hbya(const wcfd_mesh* m, const float* values, float density, const float* cell_grad,
                   float* hbya, float* cell_values, float* rua, void* stream) {
   if (!mesh_ok(m) || !values || !cell_grad || !hbya || !cell_values || !rua) return WCFD_EINVAL;
-  return launch_rua_hbya(*m, BatchDev{}, values, density, cell_grad, hbya, cell_values, rua,
+  return launch_rua_hbya(*m, BatchDev{}, values, density, cell_grad, hbya, cell_values, rua, nullptr,
                          (hipStream_t)stream);
 }
 
@@ -679,7 +709,7 @@ int wcfd_ell_to_csr(const wcfd_mesh* m, const float* ell_values, float* csr_valu
 
 size_t wcfd_reduce_workspace_bytes(int32_t n_faces) {
   Carver c(nullptr);
-  c.take<double>(2 * kMaxReduceBlocks);
+  c.take<double>(4 * kMaxReduceBlocks);
   c.take<float>(n_faces > 0 ? n_faces : 1);
   return c.off;
 }
@@ -692,12 +722,12 @@ int wcfd_correct_velocity(const wcfd_mesh* m, wcfd_bc bc, float* cell_values, fl
     return WCFD_EINVAL;
   if (resid && (!workspace || !mass_flux)) return WCFD_EINVAL;
   Carver w(workspace);
-  double* partials = w.take<double>(2 * kMaxReduceBlocks);
+  double* partials = w.take<double>(4 * kMaxReduceBlocks);
   float* fc = w.take<float>(m->n_faces);
   const Seg sg = single_seg(m->n_cells, kBlock, kMaxReduceBlocks);
   return launch_correct_velocity(*m, BatchDev{}, sg, kBlock, nullptr, bc, cell_values, cell_grad,
-                                 face_values, mass_flux, rua, rua_face, hbya, resid, fc, partials,
-                                 (hipStream_t)stream);
+                                 face_values, mass_flux, rua, rua_face, hbya, nullptr, resid, fc,
+                                 partials, (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -74,7 +74,7 @@ __global__ void k_spmv(M A, int n, const float* __restrict__ x, float* __restric
   for (int q = 0; q < NV; ++q) y[(size_t)q * n + i] = acc[q];
 }
 
-__global__ void k_dot(Seg sg, const float* __restrict__ a, const float* __restrict__ b,
+__global__ __launch_bounds__(kMaxKrylovBlock) void k_dot(Seg sg, const float* __restrict__ a, const float* __restrict__ b,
                       double* __restrict__ part) {
   const BlockRows br = block_rows(sg);
   double acc[1] = {0.0};
@@ -153,7 +153,7 @@ __global__ void k_check(Seg sg, const double* __restrict__ part, KState* st, int
 // ------------------------------------------------------------------ CG
 // partial slots: PA[3][G] = {r.z, r.r, b.b} (init, k_cg_2), PB[1][G] = {p.q} (k_cg_1)
 template <class M>
-__global__ void k_cg_init(M A, Seg sg, const float* __restrict__ b, const float* __restrict__ x,
+__global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_init(M A, Seg sg, const float* __restrict__ b, const float* __restrict__ x,
                           float* __restrict__ r, const float* __restrict__ inv,
                           double* __restrict__ PA) {
   const BlockRows br = block_rows(sg);
@@ -172,13 +172,30 @@ __global__ void k_cg_init(M A, Seg sg, const float* __restrict__ b, const float*
 }
 
 template <class M>
-__global__ void k_cg_1(M A, Seg sg, int it, const float* __restrict__ r,
+__global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_1(M A, Seg sg, int it, const float* __restrict__ r,
                        const float* __restrict__ inv, const float* __restrict__ p_in,
                        float* __restrict__ p_out, float* __restrict__ q,
                        const double* __restrict__ PA, double* __restrict__ PB, KState* st) {
   const BlockRows br = block_rows(sg);
   KState* S = st + br.k;
   if (S->f.done) return;
+  // one row per thread on an ELL matrix: issue the row's loads before the fold
+  const bool pf = M::kEll && (br.end - br.begin <= (int)blockDim.x);
+  const int i0 = br.begin + (int)threadIdx.x;
+  const bool has = pf && i0 < br.end;
+  float a[7], rc[7], ic[7], pc[7];
+  if constexpr (M::kEll) {
+    if (has) {
+      int c[7];
+      A.load(i0, a, c);
+#pragma unroll
+      for (int j = 0; j < 7; ++j) {
+        rc[j] = r[c[j]];
+        ic[j] = inv[c[j]];
+        pc[j] = it ? p_in[c[j]] : 0.f;
+      }
+    }
+  }
   bool first = (it == 0);
   double beta_d = 0.0;
   if (!first) {
@@ -189,39 +206,78 @@ __global__ void k_cg_1(M A, Seg sg, int it, const float* __restrict__ r,
   }
   const float beta = (float)beta_d;
   double acc[1] = {0.0};
-  for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
-    float qi = 0.f;
-    A.row(i, [&](float a, int c) {
-      const float z = r[c] * inv[c];
-      const float pc = first ? z : z + beta * p_in[c];
-      qi += a * pc;
-    });
-    const float zi = r[i] * inv[i];
-    const float pi = first ? zi : zi + beta * p_in[i];
-    p_out[i] = pi;
-    q[i] = qi;
-    acc[0] += (double)pi * (double)qi;
+  if (pf) {
+    if (has) {
+      float qi = 0.f;
+#pragma unroll
+      for (int j = 0; j < 7; ++j) {
+        const float z = rc[j] * ic[j];
+        qi += a[j] * (first ? z : z + beta * pc[j]);
+      }
+      const float zi = rc[0] * ic[0];
+      const float pi = first ? zi : zi + beta * pc[0];
+      p_out[i0] = pi;
+      q[i0] = qi;
+      acc[0] += (double)pi * (double)qi;
+    }
+  } else {
+    for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
+      float qi = 0.f;
+      A.row(i, [&](float a, int c) {
+        const float z = r[c] * inv[c];
+        const float pc = first ? z : z + beta * p_in[c];
+        qi += a * pc;
+      });
+      const float zi = r[i] * inv[i];
+      const float pi = first ? zi : zi + beta * p_in[i];
+      p_out[i] = pi;
+      q[i] = qi;
+      acc[0] += (double)pi * (double)qi;
+    }
   }
   store_partials<1>(acc, PB, sg.G, br.vb);
 }
 
-__global__ void k_cg_2(Seg sg, int it, float* __restrict__ x, float* __restrict__ r,
+__global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_2(Seg sg, int it, float* __restrict__ x, float* __restrict__ r,
                        const float* __restrict__ inv, const float* __restrict__ p,
                        const float* __restrict__ q, const double* __restrict__ PB,
                        double* __restrict__ PA, KState* st) {
   const BlockRows br = block_rows(sg);
   KState* S = st + br.k;
   if (S->f.done) return;
+  // one row per thread: load it before the fold
+  const bool pf = br.end - br.begin <= (int)blockDim.x;
+  const int i0 = br.begin + (int)threadIdx.x;
+  const bool has = pf && i0 < br.end;
+  float xv = 0.f, pv = 0.f, rv = 0.f, qv = 0.f, iv = 0.f;
+  if (has) {
+    xv = x[i0];
+    pv = p[i0];
+    rv = r[i0];
+    qv = q[i0];
+    iv = inv[i0];
+  }
   const double pq = fold(PB + br.b0, br.nb);
   const float alpha = (float)safe_div(S->rz[it & 1], pq);
   double acc[2] = {0.0, 0.0};
-  for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
-    x[i] = x[i] + alpha * p[i];
-    const float ri = r[i] - alpha * q[i];
-    r[i] = ri;
-    const float zi = ri * inv[i];
-    acc[0] += (double)ri * (double)zi;
-    acc[1] += (double)ri * (double)ri;
+  if (pf) {
+    if (has) {
+      x[i0] = xv + alpha * pv;
+      const float ri = rv - alpha * qv;
+      r[i0] = ri;
+      const float zi = ri * iv;
+      acc[0] += (double)ri * (double)zi;
+      acc[1] += (double)ri * (double)ri;
+    }
+  } else {
+    for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
+      x[i] = x[i] + alpha * p[i];
+      const float ri = r[i] - alpha * q[i];
+      r[i] = ri;
+      const float zi = ri * inv[i];
+      acc[0] += (double)ri * (double)zi;
+      acc[1] += (double)ri * (double)ri;
+    }
   }
   store_partials<2>(acc, PA, sg.G, br.vb);
 }
@@ -237,7 +293,7 @@ __global__ void k_cg_2(Seg sg, int it, float* __restrict__ x, float* __restrict_
 // fly.  Partials ping-pong between PA and PC: kernel i reads the buffer
 // kernel i-1 wrote.
 template <class M>
-__global__ void k_cgs_init1(M A, Seg sg, const float* __restrict__ b, const float* __restrict__ x,
+__global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_init1(M A, Seg sg, const float* __restrict__ b, const float* __restrict__ x,
                             float* __restrict__ r, double* __restrict__ PC) {
   const BlockRows br = block_rows(sg);
   double acc[3] = {0.0, 0.0, 0.0};
@@ -253,7 +309,7 @@ __global__ void k_cgs_init1(M A, Seg sg, const float* __restrict__ b, const floa
 }
 
 template <class M>
-__global__ void k_cgs_init2(M A, Seg sg, const float* __restrict__ r, const float* __restrict__ inv,
+__global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_init2(M A, Seg sg, const float* __restrict__ r, const float* __restrict__ inv,
                             float* __restrict__ w, double* __restrict__ PA) {
   const BlockRows br = block_rows(sg);
   double acc[3] = {0.0, 0.0, 0.0};
@@ -270,7 +326,7 @@ __global__ void k_cgs_init2(M A, Seg sg, const float* __restrict__ r, const floa
 }
 
 template <class M>
-__global__ void k_cgs_iter(M A, Seg sg, int it, float* __restrict__ x,
+__global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int it, float* __restrict__ x,
                            const float* __restrict__ r_in, float* __restrict__ r_out,
                            const float* __restrict__ w_in, float* __restrict__ w_out,
                            const float* __restrict__ s_in, float* __restrict__ s_out,
@@ -282,6 +338,27 @@ __global__ void k_cgs_iter(M A, Seg sg, int it, float* __restrict__ x,
   if (S->f.done) return;
   const int G = sg.G;
   const bool first = (it == 0);
+  // one row per thread on an ELL matrix: issue the row's loads before the fold
+  const bool pf = M::kEll && (br.end - br.begin <= (int)blockDim.x);
+  const int i0 = br.begin + (int)threadIdx.x;
+  const bool has = pf && i0 < br.end;
+  float a[7], wc[7], sc[7], ic[7], rc[7];
+  float p0 = 0.f, x0 = 0.f;
+  if constexpr (M::kEll) {
+    if (has) {
+      int c[7];
+      A.load(i0, a, c);
+#pragma unroll
+      for (int j = 0; j < 7; ++j) {
+        wc[j] = w_in[c[j]];
+        sc[j] = first ? 0.f : s_in[c[j]];
+        ic[j] = inv[c[j]];
+        rc[j] = r_in[c[j]];
+      }
+      p0 = first ? 0.f : p[i0];
+      x0 = x[i0];
+    }
+  }
   double alpha_d, beta_d = 0.0;
   if (first) {
     alpha_d = S->alpha[0];
@@ -297,26 +374,51 @@ __global__ void k_cgs_iter(M A, Seg sg, int it, float* __restrict__ x,
   }
   const float alpha = (float)alpha_d, beta = (float)beta_d;
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
-    float wn = 0.f;
-    A.row(i, [&](float a, int c) {
-      const float sc = first ? w_in[c] : w_in[c] + beta * s_in[c];
-      wn += a * (inv[c] * (r_in[c] - alpha * sc));
-    });
-    const float ii = inv[i];
-    const float ui = r_in[i] * ii;
-    const float pi = first ? ui : ui + beta * p[i];
-    const float si = first ? w_in[i] : w_in[i] + beta * s_in[i];
-    x[i] = x[i] + alpha * pi;
-    const float rn = r_in[i] - alpha * si;
-    p[i] = pi;
-    s_out[i] = si;
-    r_out[i] = rn;
-    w_out[i] = wn;
-    const float un = rn * ii;
-    acc[0] += (double)rn * (double)un;
-    acc[1] += (double)wn * (double)un;
-    acc[2] += (double)rn * (double)rn;
+  if (pf) {
+    if (has) {
+      float wn = 0.f;
+#pragma unroll
+      for (int j = 0; j < 7; ++j) {
+        const float scj = first ? wc[j] : wc[j] + beta * sc[j];
+        wn += a[j] * (ic[j] * (rc[j] - alpha * scj));
+      }
+      const float ii = ic[0];
+      const float ui = rc[0] * ii;
+      const float pi = first ? ui : ui + beta * p0;
+      const float si = first ? wc[0] : wc[0] + beta * sc[0];
+      x[i0] = x0 + alpha * pi;
+      const float rn = rc[0] - alpha * si;
+      p[i0] = pi;
+      s_out[i0] = si;
+      r_out[i0] = rn;
+      w_out[i0] = wn;
+      const float un = rn * ii;
+      acc[0] += (double)rn * (double)un;
+      acc[1] += (double)wn * (double)un;
+      acc[2] += (double)rn * (double)rn;
+    }
+  } else {
+    for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
+      float wn = 0.f;
+      A.row(i, [&](float a, int c) {
+        const float sc = first ? w_in[c] : w_in[c] + beta * s_in[c];
+        wn += a * (inv[c] * (r_in[c] - alpha * sc));
+      });
+      const float ii = inv[i];
+      const float ui = r_in[i] * ii;
+      const float pi = first ? ui : ui + beta * p[i];
+      const float si = first ? w_in[i] : w_in[i] + beta * s_in[i];
+      x[i] = x[i] + alpha * pi;
+      const float rn = r_in[i] - alpha * si;
+      p[i] = pi;
+      s_out[i] = si;
+      r_out[i] = rn;
+      w_out[i] = wn;
+      const float un = rn * ii;
+      acc[0] += (double)rn * (double)un;
+      acc[1] += (double)wn * (double)un;
+      acc[2] += (double)rn * (double)rn;
+    }
   }
   store_partials<3>(acc, P_out, G, br.vb);
 }
@@ -325,7 +427,7 @@ __global__ void k_cgs_iter(M A, Seg sg, int it, float* __restrict__ x,
 // partial slots: PA[3][G] = {r0.r, r.r, b.b} (init, k_bi_3), PB[1][G] = {r0.v}
 // (k_bi_1), PC[3][G] = {t.s, t.t, s.s} (k_bi_2)
 template <class M, int NV>
-__global__ void k_bi_init(M A, Seg sg, const float* __restrict__ b, const float* __restrict__ x,
+__global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_init(M A, Seg sg, const float* __restrict__ b, const float* __restrict__ x,
                           float* __restrict__ r, float* __restrict__ r0, double* __restrict__ PA) {
   const BlockRows br = block_rows(sg);
   const int n = sg.n;
@@ -354,7 +456,7 @@ __global__ void k_bi_init(M A, Seg sg, const float* __restrict__ b, const float*
 }
 
 template <class M, int NV>
-__global__ void k_bi_1(M A, Seg sg, int it, const float* __restrict__ r,
+__global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_1(M A, Seg sg, int it, const float* __restrict__ r,
                        const float* __restrict__ inv, const float* __restrict__ p_in,
                        const float* __restrict__ v_in, float* __restrict__ p_out,
                        float* __restrict__ v_out, const float* __restrict__ r0,
@@ -363,6 +465,30 @@ __global__ void k_bi_1(M A, Seg sg, int it, const float* __restrict__ r,
   KState* S = st + br.k;
   if (S->f.done) return;
   const int n = sg.n;
+  // one row per thread on an ELL matrix: issue the row's loads before the fold
+  const bool pf = M::kEll && (br.end - br.begin <= (int)blockDim.x);
+  const int i0 = br.begin + (int)threadIdx.x;
+  const bool has = pf && i0 < br.end;
+  float a[7], ic[7], rr[NV][7], pp[NV][7], vv[NV][7], r0i[NV];
+  if constexpr (M::kEll) {
+    if (has) {
+      int c[7];
+      A.load(i0, a, c);
+#pragma unroll
+      for (int j = 0; j < 7; ++j) {
+        ic[j] = inv[c[j]];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+          const size_t k = (size_t)q * n + c[j];
+          rr[q][j] = r[k];
+          pp[q][j] = it ? p_in[k] : 0.f;
+          vv[q][j] = it ? v_in[k] : 0.f;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < NV; ++q) r0i[q] = r0[(size_t)q * n + i0];
+    }
+  }
   bool first = (it == 0);
   double beta_d = 0.0, omega_d = 0.0;
   if (!first) {
@@ -376,33 +502,57 @@ __global__ void k_bi_1(M A, Seg sg, int it, const float* __restrict__ r,
   const float beta = (float)beta_d;
   const float omega = (float)omega_d;
   double acc[1] = {0.0};
-  for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
-    float v[NV];
+  if (pf) {
+    if (has) {
+      float v[NV];
 #pragma unroll
-    for (int q = 0; q < NV; ++q) v[q] = 0.f;
-    A.row(i, [&](float a, int c) {
-      const float ic = inv[c];
+      for (int q = 0; q < NV; ++q) v[q] = 0.f;
+#pragma unroll
+      for (int j = 0; j < 7; ++j) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+          const float pc = first ? rr[q][j] : rr[q][j] + beta * (pp[q][j] - omega * vv[q][j]);
+          v[q] += a[j] * (pc * ic[j]);
+        }
+      }
 #pragma unroll
       for (int q = 0; q < NV; ++q) {
-        const size_t k = (size_t)q * n + c;
-        const float pc = first ? r[k] : r[k] + beta * (p_in[k] - omega * v_in[k]);
-        v[q] += a * (pc * ic);
+        const size_t k = (size_t)q * n + i0;
+        const float pi = first ? rr[q][0] : rr[q][0] + beta * (pp[q][0] - omega * vv[q][0]);
+        p_out[k] = pi;
+        v_out[k] = v[q];
+        acc[0] += (double)r0i[q] * (double)v[q];
       }
-    });
+    }
+  } else {
+    for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
+      float v[NV];
 #pragma unroll
-    for (int q = 0; q < NV; ++q) {
-      const size_t k = (size_t)q * n + i;
-      const float pi = first ? r[k] : r[k] + beta * (p_in[k] - omega * v_in[k]);
-      p_out[k] = pi;
-      v_out[k] = v[q];
-      acc[0] += (double)r0[k] * (double)v[q];
+      for (int q = 0; q < NV; ++q) v[q] = 0.f;
+      A.row(i, [&](float a, int c) {
+        const float ic = inv[c];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+          const size_t k = (size_t)q * n + c;
+          const float pc = first ? r[k] : r[k] + beta * (p_in[k] - omega * v_in[k]);
+          v[q] += a * (pc * ic);
+        }
+      });
+#pragma unroll
+      for (int q = 0; q < NV; ++q) {
+        const size_t k = (size_t)q * n + i;
+        const float pi = first ? r[k] : r[k] + beta * (p_in[k] - omega * v_in[k]);
+        p_out[k] = pi;
+        v_out[k] = v[q];
+        acc[0] += (double)r0[k] * (double)v[q];
+      }
     }
   }
   store_partials<1>(acc, PB, sg.G, br.vb);
 }
 
 template <class M, int NV>
-__global__ void k_bi_2(M A, Seg sg, int it, const float* __restrict__ r,
+__global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_2(M A, Seg sg, int it, const float* __restrict__ r,
                        const float* __restrict__ inv, const float* __restrict__ v,
                        float* __restrict__ s, float* __restrict__ t,
                        const double* __restrict__ PB, double* __restrict__ PC, KState* st) {
@@ -410,40 +560,87 @@ __global__ void k_bi_2(M A, Seg sg, int it, const float* __restrict__ r,
   KState* S = st + br.k;
   if (S->f.done) return;
   const int n = sg.n;
+  // one row per thread on an ELL matrix: issue the row's loads before the fold
+  const bool pf = M::kEll && (br.end - br.begin <= (int)blockDim.x);
+  const int i0 = br.begin + (int)threadIdx.x;
+  const bool has = pf && i0 < br.end;
+  float a[7], ic[7], rr[NV][7], vv[NV][7];
+  if constexpr (M::kEll) {
+    if (has) {
+      int c[7];
+      A.load(i0, a, c);
+#pragma unroll
+      for (int j = 0; j < 7; ++j) {
+        ic[j] = inv[c[j]];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+          const size_t k = (size_t)q * n + c[j];
+          rr[q][j] = r[k];
+          vv[q][j] = v[k];
+        }
+      }
+    }
+  }
   const double r0v = fold(PB + br.b0, br.nb);
   const double alpha_d = safe_div(S->rho[it & 1], r0v);
   if (br.vb == br.b0 && threadIdx.x == 0) S->alpha[it & 1] = alpha_d;
   const float alpha = (float)alpha_d;
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
-    float tv[NV];
+  if (pf) {
+    if (has) {
+      float tv[NV];
 #pragma unroll
-    for (int q = 0; q < NV; ++q) tv[q] = 0.f;
-    A.row(i, [&](float a, int c) {
-      const float ic = inv[c];
+      for (int q = 0; q < NV; ++q) tv[q] = 0.f;
+#pragma unroll
+      for (int j = 0; j < 7; ++j) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+          const float sc = rr[q][j] - alpha * vv[q][j];
+          tv[q] += a[j] * (sc * ic[j]);
+        }
+      }
 #pragma unroll
       for (int q = 0; q < NV; ++q) {
-        const size_t k = (size_t)q * n + c;
-        const float sc = r[k] - alpha * v[k];
-        tv[q] += a * (sc * ic);
+        const size_t k = (size_t)q * n + i0;
+        const float si = rr[q][0] - alpha * vv[q][0];
+        s[k] = si;
+        t[k] = tv[q];
+        acc[0] += (double)tv[q] * (double)si;
+        acc[1] += (double)tv[q] * (double)tv[q];
+        acc[2] += (double)si * (double)si;
       }
-    });
+    }
+  } else {
+    for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
+      float tv[NV];
 #pragma unroll
-    for (int q = 0; q < NV; ++q) {
-      const size_t k = (size_t)q * n + i;
-      const float si = r[k] - alpha * v[k];
-      s[k] = si;
-      t[k] = tv[q];
-      acc[0] += (double)tv[q] * (double)si;
-      acc[1] += (double)tv[q] * (double)tv[q];
-      acc[2] += (double)si * (double)si;
+      for (int q = 0; q < NV; ++q) tv[q] = 0.f;
+      A.row(i, [&](float a, int c) {
+        const float ic = inv[c];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+          const size_t k = (size_t)q * n + c;
+          const float sc = r[k] - alpha * v[k];
+          tv[q] += a * (sc * ic);
+        }
+      });
+#pragma unroll
+      for (int q = 0; q < NV; ++q) {
+        const size_t k = (size_t)q * n + i;
+        const float si = r[k] - alpha * v[k];
+        s[k] = si;
+        t[k] = tv[q];
+        acc[0] += (double)tv[q] * (double)si;
+        acc[1] += (double)tv[q] * (double)tv[q];
+        acc[2] += (double)si * (double)si;
+      }
     }
   }
   store_partials<3>(acc, PC, sg.G, br.vb);
 }
 
 template <int NV>
-__global__ void k_bi_3(Seg sg, int it, float* __restrict__ x, float* __restrict__ r,
+__global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_3(Seg sg, int it, float* __restrict__ x, float* __restrict__ r,
                        const float* __restrict__ inv, const float* __restrict__ p,
                        const float* __restrict__ s, const float* __restrict__ t,
                        const float* __restrict__ r0, const double* __restrict__ PC,
@@ -452,6 +649,23 @@ __global__ void k_bi_3(Seg sg, int it, float* __restrict__ x, float* __restrict_
   KState* S = st + br.k;
   if (S->f.done) return;
   const int n = sg.n, G = sg.G;
+  // one row per thread: load it before the fold
+  const bool pf = br.end - br.begin <= (int)blockDim.x;
+  const int i0 = br.begin + (int)threadIdx.x;
+  const bool has = pf && i0 < br.end;
+  float iv = 0.f, pv[NV], sv[NV], tv[NV], xv[NV], r0v[NV];
+  if (has) {
+    iv = inv[i0];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      const size_t k = (size_t)q * n + i0;
+      pv[q] = p[k];
+      sv[q] = s[k];
+      tv[q] = t[k];
+      xv[q] = x[k];
+      r0v[q] = r0[k];
+    }
+  }
   const double ts = fold(PC + br.b0, br.nb), tt = fold(PC + G + br.b0, br.nb);
   const double ss = fold(PC + 2 * G + br.b0, br.nb);
   const double omega_d = ss > S->f.thr2 ? safe_div(ts, tt) : 0.0;
@@ -462,6 +676,23 @@ __global__ void k_bi_3(Seg sg, int it, float* __restrict__ x, float* __restrict_
   const float alpha = (float)S->alpha[it & 1];
   const float omega = (float)omega_d;
   double acc[2] = {0.0, 0.0};
+  if (pf) {
+    if (has) {
+#pragma unroll
+      for (int q = 0; q < NV; ++q) {
+        const size_t k = (size_t)q * n + i0;
+        const float y = pv[q] * iv;
+        const float z = sv[q] * iv;
+        x[k] = xv[q] + (alpha * y + omega * z);
+        const float rk = sv[q] - omega * tv[q];
+        r[k] = rk;
+        acc[0] += (double)r0v[q] * (double)rk;
+        acc[1] += (double)rk * (double)rk;
+      }
+    }
+    store_partials<2>(acc, PA, G, br.vb);
+    return;
+  }
   for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
     const float ii = inv[i];
 #pragma unroll
@@ -604,15 +835,14 @@ static int run_groups(const Seg& sg, KState* st, RrPart&& rr_part, int maxiter, 
   HostSync* hs = host_sync(K);
   if (!hs) return WCFD_EHIP;
   int e;
-  // flags after init (synchronous: one round trip per solve)
-  if ((e = copy_flags(hs->slot, st, K, s))) return e;
-  if ((e = check_hip(hipStreamSynchronize(s)))) return e;
-  if (all_done(hs->slot, K)) return finish(hs->slot, K, info);
+  // No round trip after init: the first group is launched at once (its
+  // kernels exit at entry for cases the init fold already marked done) and
+  // the first poll reports those cases like any other.
   const int depth = 2;
   int launched = 0, polled = 0, it = 0;
-  const KFlags* last = hs->slot;
+  const KFlags* last = nullptr;
   while (true) {
-    if (it < maxiter && launched - polled < depth) {
+    if ((it < maxiter || launched == 0) && launched - polled < depth) {
       launch_group(it);
       it += ce;
       WCFD_LAUNCH(K_FOLD, s, k_check, K, sg, rr_part(it), st, it, maxiter);
@@ -624,6 +854,7 @@ static int run_groups(const Seg& sg, KState* st, RrPart&& rr_part, int maxiter, 
       continue;
     }
     if (polled == launched) break;   // cannot happen: the last check sets every flag
+    // (maxiter <= 0: the init fold set every flag and the one group is empty)
     const int sl = polled % kSlots;
     if ((e = check_hip(hipEventSynchronize(hs->ev[sl])))) return e;
     last = hs->slot + (size_t)sl * hs->cap;
@@ -824,7 +1055,7 @@ int solve_bicgstab_ell(const wcfd_mesh& m, const wcfd_batch* bt, const float* va
 
 bool batch_ok(const wcfd_mesh* m, const wcfd_batch* bt) {
   return m && bt && m->n_cells > 0 && bt->n_cases > 0 && bt->n_blocks > 0 && bt->chunk > 0 &&
-         bt->block >= kWave && bt->block <= kMaxWaves * kWave && bt->block % kWave == 0 &&
+         bt->block >= kWave && bt->block <= kMaxKrylovBlock && bt->block % kWave == 0 &&
          bt->cell_off && bt->blk_off && bt->blk_case && bt->active;
 }
 
@@ -875,7 +1106,7 @@ int wcfd_dot_f64(int32_t n, const float* a, const float* b, double* out, void* w
 }
 
 int wcfd_set_krylov_launch(int grid, int block) {
-  if (grid < 1 || grid > kMaxReduceBlocks || block < kWave || block > kMaxWaves * kWave ||
+  if (grid < 1 || grid > kMaxReduceBlocks || block < kWave || block > kMaxKrylovBlock ||
       block % kWave != 0)
     return WCFD_EINVAL;
   g_krylov_grid = grid;

@@ -13,6 +13,8 @@
 //     p relaxation
 //     face p + corrected flux F_c ; grad p, HbyA -= rUA grad p, continuity
 //     residual per case (+ the batch's stop test)
+#include <math.h>
+
 #include <vector>
 
 #include "fv_kernels.h"
@@ -23,24 +25,27 @@ struct SimpleWork {
   void* krylov;
   double* partials;
   float* fc;
+  float* u_prev;
 };
-
-static size_t simple_bytes(int32_t C, int32_t F, int K, int G, bool batch) {
-  Carver c(nullptr);
-  c.take<char>(batch ? krylov_workspace_bytes_batch(C, 3, K, G) : krylov_workspace_bytes(C, 3));
-  c.take<double>(2 * (size_t)(batch ? G : kMaxReduceBlocks));
-  c.take<float>(F);
-  return c.off;
-}
 
 static SimpleWork carve_simple(void* ws, int32_t C, int32_t F, int K, int G, bool batch) {
   Carver c(ws);
   SimpleWork w;
   w.krylov = c.take<char>(batch ? krylov_workspace_bytes_batch(C, 3, K, G)
                                 : krylov_workspace_bytes(C, 3));
-  w.partials = c.take<double>(2 * (size_t)(batch ? G : kMaxReduceBlocks));
+  w.partials = c.take<double>(4 * (size_t)(batch ? G : kMaxReduceBlocks));
   w.fc = c.take<float>(F);
+  w.u_prev = c.take<float>(3 * (size_t)C);
   return w;
+}
+
+static size_t simple_bytes(int32_t C, int32_t F, int K, int G, bool batch) {
+  Carver c(nullptr);
+  c.take<char>(batch ? krylov_workspace_bytes_batch(C, 3, K, G) : krylov_workspace_bytes(C, 3));
+  c.take<double>(4 * (size_t)(batch ? G : kMaxReduceBlocks));
+  c.take<float>(F);
+  c.take<float>(3 * (size_t)C);
+  return c.off;
 }
 
 static int simple_iteration(const wcfd_mesh& m, const wcfd_batch* bt, wcfd_bc bc,
@@ -67,11 +72,12 @@ static int simple_iteration(const wcfd_mesh& m, const wcfd_batch* bt, wcfd_bc bc
                                     prm->dt, prm->scheme_flags, st->mom_values, st->mom_rhs, s)))
     return e;
   const Tol tl{prm->atol, prm->tol, prm->rtol0};
-  e = solve_bicgstab_ell(m, bt, st->mom_values, 3, st->mom_rhs, st->hbya, tl, prm->maxiter,
-                         prm->check_every, w.krylov, mom.data(), s);
+  const int ce_u = prm->check_every_u > 0 ? prm->check_every_u : prm->check_every;
+  e = solve_bicgstab_ell(m, bt, st->mom_values, 3, st->mom_rhs, st->hbya, tl, prm->maxiter, ce_u,
+                         w.krylov, mom.data(), s);
   if (e) return e;
   if ((e = launch_rua_hbya(m, bd, st->mom_values, prm->density, st->cell_grad, st->hbya,
-                           st->cell_values, st->rua, s)))
+                           st->cell_values, st->rua, w.u_prev, s)))
     return e;
   const int n_inner = prm->n_inner > 0 ? prm->n_inner : 1;
   const int n_ortho = prm->n_ortho > 0 ? prm->n_ortho : 1;
@@ -94,8 +100,8 @@ static int simple_iteration(const wcfd_mesh& m, const wcfd_batch* bt, wcfd_bc bc
       // the batch's stop test and iteration count run after the last corrector only
       if ((e = launch_correct_velocity(m, bd, sg, sg_block, last ? bt : nullptr, bc,
                                        st->cell_values, st->cell_grad, st->face_values,
-                                       st->mass_flux, st->rua, st->rua_face, st->hbya, st->resid,
-                                       w.fc, w.partials, s)))
+                                       st->mass_flux, st->rua, st->rua_face, st->hbya, w.u_prev,
+                                       st->resid, w.fc, w.partials, s)))
         return e;
     }
   }
@@ -104,22 +110,27 @@ static int simple_iteration(const wcfd_mesh& m, const wcfd_batch* bt, wcfd_bc bc
       info[k].momentum = mom[k];
       info[k].pressure = pres[k];
       info[k].continuity = 0.0;
+      info[k].velocity_change = 0.0;
     }
     if (st->resid) {
-      std::vector<double> r(2 * (size_t)K);
-      if ((e = check_hip(hipMemcpyAsync(r.data(), st->resid, sizeof(double) * 2 * K,
+      std::vector<double> r(4 * (size_t)K);
+      if ((e = check_hip(hipMemcpyAsync(r.data(), st->resid, sizeof(double) * 4 * K,
                                         hipMemcpyDeviceToHost, s))))
         return e;
       if ((e = check_hip(hipStreamSynchronize(s)))) return e;
-      for (int k = 0; k < K; ++k)
-        info[k].continuity = r[2 * k + 1] > 0.0 ? r[2 * k] / r[2 * k + 1] : 0.0;
+      for (int k = 0; k < K; ++k) {
+        const double* q = r.data() + 4 * k;
+        info[k].continuity = q[1] > 0.0 ? q[0] / q[1] : 0.0;
+        info[k].velocity_change = q[3] > 0.0 ? sqrt(q[2] / q[3]) : 0.0;
+      }
     }
   }
   return WCFD_OK;
 }
 
 static bool params_ok(const wcfd_simple_params* prm) {
-  return prm && prm->density != 0.f && prm->check_every >= 0 && prm->rtol0 >= 0.0 &&
+  return prm && prm->density != 0.f && prm->check_every >= 0 && prm->check_every_u >= 0 &&
+         prm->rtol0 >= 0.0 &&
          (prm->cg_variant == WCFD_CG_STANDARD || prm->cg_variant == WCFD_CG_SINGLE_REDUCTION);
 }
 
@@ -158,7 +169,7 @@ int wcfd_batch_simple_iteration(const wcfd_mesh* mp, const wcfd_batch* bt, wcfd_
       !mp->slot_nbr || !st->resid)
     return WCFD_EINVAL;
   if (!bt->cell_case || !bt->face_case || !bt->nu || !bt->ref_cell || !bt->ref_value ||
-      !bt->stop || !bt->outer_iters)
+      !bt->stop || !bt->monitor || !bt->outer_iters)
     return WCFD_EINVAL;
   if (mp->n_faces < mp->n_internal) return WCFD_EINVAL;
   return simple_iteration(*mp, bt, bc, prm, st, workspace, info, (hipStream_t)stream);
