@@ -284,6 +284,253 @@ def bench_batch64(args):
     print(json.dumps(out))
 
 
+def bench_serve(args):
+    """The served stream: Poisson arrivals (seeded, rate 20/s) of cavity
+    (N in [32,512], Re in [100,1000]) and channel (50k-500k cells, Re in
+    [50,400]) requests, each solved to continuity <= 1e-6.  Served with
+    continuous batching on an arena sized by an HBM budget, and for
+    comparison one at a time on the plain solver.  One step = the whole
+    stream."""
+    import math
+    import numpy as np
+    import torch
+    from warpcfd_amd import _lib
+    from warpcfd_amd.batch import CaseArena
+    from warpcfd_amd.serve import (ContinuousBatchServer, Preparer, SequentialServer,
+                                   poisson_stream, summarize)
+    stop, stop_u = 1e-6, (args.stop_u if args.serve_stop_u else math.inf)
+    pol = dict(atol=0.0, tol=0.0, rtol0=args.rtol0, maxiter=2000, check_every=10,
+               check_every_u=args.check_u, cg_variant=_lib.CG_SINGLE_REDUCTION)
+    # preparation workers start before the first GPU call of this process
+    prep = Preparer(workers=args.workers, stop=stop, stop_u=stop_u,
+                    floor_u=1e-7 if args.serve_stop_u else 0.0, max_outer=args.max_outer)
+    prep.warm()
+    # warm-up: load every kernel once
+    from warpcfd_amd.batch import CaseSpec
+    from warpcfd_amd.workloads import cavity_model
+    warm = CaseArena.for_cases([CaseSpec(cavity_model(16, 100.0))], chunk=args.chunk,
+                               block=args.block, **pol)
+    for _ in range(3):
+        warm.step()
+    torch.cuda.synchronize()
+    del warm
+
+    stream = poisson_stream(args.seed, rate=args.rate, n_requests=args.requests)
+    cells = sum(r.cells for r in stream)
+    print(f'# serve: {len(stream)} requests, {cells} cells, arrivals over '
+          f'{stream[-1].arrival:.2f} s', file=sys.stderr, flush=True)
+    srv = ContinuousBatchServer(hbm_bytes=int(args.hbm_gb * (1 << 30)), max_cases=args.slots,
+                                chunk=args.chunk, block=args.block, **pol)
+    def report(r):
+        print(f'#   done {r.describe()}: {r.iterations} iterations, '
+              f'{r.t_done - r.arrival:.2f} s after arrival', file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    done = srv.serve(stream, prep, clock_start=t0, on_done=report)
+    torch.cuda.synchronize()
+    wall = max(r.t_done for r in done)
+    served = summarize(done, wall)
+    print(f'# continuous batching: {served}', file=sys.stderr, flush=True)
+    seq = None
+    if not args.no_sequential:
+        stream2 = poisson_stream(args.seed, rate=args.rate, n_requests=args.requests)
+        t0 = time.perf_counter()
+        done2 = SequentialServer(chunk=args.chunk, block=args.block, **pol).serve(
+            stream2, prep, clock_start=t0, on_done=report)
+        torch.cuda.synchronize()
+        seq = summarize(done2, max(r.t_done for r in done2))
+        print(f'# sequential: {seq}', file=sys.stderr, flush=True)
+    out = {
+        'metric': 'SIMPLE iter/s + %HBM roofline; served solves/s, p50/p99 at equal residual',
+        'value': served['solves_per_s'],
+        'unit': 'solves/s',
+        'n_gpus': 1,
+        'steps': 1,
+        'warmup': 0,
+        'ms_per_step': round(1e3 * wall, 1),
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32',
+        'data': 'synthetic',
+        'config': {'workload': f'served stream: {len(stream)} Poisson arrivals (seed {args.seed}, '
+                               f'{args.rate:g}/s) of lid-driven cavities (N in [32,512], Re in '
+                               f'[100,1000]) and 10x2x2 channels (50k-500k cells, Re in '
+                               f'[50,400]), steady SIMPLE to continuity <= {stop:g}'
+                               + ('' if math.isinf(stop_u) else
+                                  f' and estimated velocity error <= {stop_u:g}'),
+                   'requests': len(stream), 'cells': cells, 'hbm_budget_gb': args.hbm_gb,
+                   'slots': args.slots, 'host_workers': args.workers,
+                   'krylov': {'atol': 0.0, 'tol': 0.0, 'rtol0': args.rtol0, 'maxiter': 2000,
+                              'check_every': 10, 'check_every_momentum': args.check_u,
+                              'pressure_cg': 'single'}, 'reductions': 'f64'},
+        'served': served,
+        'sequential': seq,
+        'speedup_vs_sequential': (round(served['solves_per_s'] / seq['solves_per_s'], 2)
+                                  if seq else None),
+        'latency_note': 'time to converged = arrival to convergence, including host '
+                        'preparation (mesh, layout; worker processes) and queueing; '
+                        'resident time = admission to convergence',
+        'roofline': None,
+        'cpu_baseline': None,
+    }
+    prep.close()
+    print(json.dumps(out))
+
+
+def profile_roofline(run, C, F, Fi, Fb, nnz, cg_single=True, capacity=400000, note=None):
+    """Runs ``run()`` under the library's HIP-event profiler; returns
+    (roofline dict of the dominant kernel, per-kernel table)."""
+    from warpcfd_amd import _lib
+    import torch
+    with _lib.Profiler(capacity=capacity) as prof:
+        run()
+        torch.cuda.synchronize()
+    kernels = {k: {'total_ms': round(v[0], 3), 'launches': v[1], 'avg_us': round(1e3 * v[0] / v[1], 3)}
+               for k, v in sorted(prof.result.items(), key=lambda kv: -kv[1][0])}
+    dominant = max(prof.result.items(), key=lambda kv: kv[1][0])[0]
+    ms, cnt = prof.result[dominant]
+    key = dominant + ('_single' if dominant == 'cg_1' and cg_single else '')
+    byts = algorithmic_bytes(key, C, F, Fi, Fb, nnz)
+    roofline = None
+    if byts is not None:
+        avg_s = ms / cnt / 1e3
+        ach = byts / avg_s / 1e9
+        roofline = {'bound': 'hbm', 'kernel': dominant, 'achieved': round(ach, 1),
+                    'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(ach / HBM_PEAK_GBS, 4),
+                    'traffic': None, 'bytes_per_launch': byts, 'avg_launch_us': round(avg_s * 1e6, 3)}
+        if note:
+            roofline['note'] = note
+    return roofline, kernels
+
+
+def bench_cylinder(args):
+    """Transient PISO past a cylinder: box 20 x 8 x 1 with a voxel-carved
+    cylinder D = 1 at (5, 4), h = 0.05 (1 273 680 hexahedra, interior nodes
+    jittered +-10% h, seed 0), inflow u = 1, outflow p = 0, slip side walls,
+    no-slip cylinder, Re = 100, dt = 0.025 (CFL 0.5), two PISO correctors,
+    float64 pressure-residual refinement.  One step = one time step."""
+    import numpy as np
+    import torch
+    from warpcfd_amd import _lib
+    from warpcfd_amd.FV.implicit_Solvers import IncompressibleSolver
+    from warpcfd_amd.workloads import cylinder_model
+    h = args.cyl_h
+    t = time.perf_counter()
+    model = cylinder_model(h=h, re=100.0, jitter=0.1, seed=0)
+    model.finalize()
+    prep_s = time.perf_counter() - t
+    dt = 0.5 * h / 1.0
+    sv = IncompressibleSolver(model, None, None, steady_state=False, implicit_outflow=True)
+    sv.set_iterations(1, 2, 1)
+    sv.set_time_stepping(args.warmup + args.steps, dt)
+    sv.cg_variant = _lib.CG_SINGLE_REDUCTION if args.cg == 'single' else _lib.CG_STANDARD
+    sv.p_refine = args.p_refine
+    dm = model.dm
+    tuned = None
+    if not args.no_tune:
+        from warpcfd_amd.tune import autotune_krylov
+        tt = autotune_krylov(dm, cg_variant=sv.cg_variant)
+        tuned = {'grid': tt['grid'], 'block': tt['block']}
+    C, F, Fi, Fb, nnz = dm.n_cells, dm.n_faces, dm.n_internal, dm.n_boundary, dm.nnz
+    for _ in range(args.warmup):
+        sv.outer_iteration(dt=dt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sv.outer_iteration(dt=dt)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    hist = sv.history[-args.steps:]
+    roofline = kernels = None
+    if not args.no_profile:
+        def run():
+            for _ in range(min(args.steps, 5)):
+                sv.outer_iteration(dt=dt)
+        roofline, kernels = profile_roofline(run, C, F, Fi, Fb, nnz, args.cg == 'single')
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_cylinder_baseline(C, args.p_refine)
+    out = {
+        'metric': 'SIMPLE iter/s + %HBM roofline; served solves/s, p50/p99 at equal residual',
+        'value': round(args.steps / elapsed, 3),
+        'unit': 'PISO time steps/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': round(1e3 * elapsed / args.steps, 3),
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32',
+        'data': 'synthetic',
+        'config': {'workload': f'transient PISO past a cylinder: box 20x8x1, D=1 at (5,4), '
+                               f'h={h} ({C} hex cells, interior vertices jittered +-10% h, seed 0), '
+                               f'inflow u=1, outflow p=0, slip side walls, no-slip cylinder, '
+                               f'Re=100, dt={dt} (CFL 0.5), 2 PISO correctors, float64 '
+                               f'pressure-residual refinement ({args.p_refine} sweep)',
+                   'cells': C, 'faces': F, 'nnz': nnz, 'reductions': 'f64',
+                   'krylov': {'atol': 1e-5, 'maxiter': 500, 'check_every': 10,
+                              'pressure_cg': args.cg, 'p_refine': args.p_refine}},
+        'roofline': roofline,
+        'cpu_baseline': cpu,
+        'continuity_residual': hist[-1]['continuity'],
+        'krylov_iters_per_step': {'momentum': float(np.mean([h_['u_iters'] for h_ in hist])),
+                                  'pressure_last_corrector': float(np.mean([h_['p_iters'] for h_ in hist]))},
+        'krylov_launch': tuned,
+        'host_preprocessing_s': round(prep_s, 1),
+        'kernels': kernels,
+    }
+    print(json.dumps(out))
+
+
+def cpu_cylinder_baseline(C_full, p_refine, budget_s=30.0):
+    """CPU oracle, one thread, on a bounded sample: the same cylinder
+    geometry at h = 0.1 (1/8 of the cells), PISO steps until the budget;
+    reported per step and scaled by the cell-count ratio to the full mesh
+    (iteration counts held equal: a lower bound on the CPU time of a full
+    step)."""
+    try:
+        from threadpoolctl import threadpool_limits
+        limiter = threadpool_limits(limits=1)
+    except Exception:
+        limiter = None
+    import numpy as np
+    from oracle.mesh import OracleMesh
+    from oracle.fv import OracleFVM
+    from oracle.simple import OracleSolver
+    from warpcfd_amd.preprocess import Mesh, box_and_body_walls, cylinder_mesh
+    from warpcfd_amd.workloads import cylinder_bcs
+    h = 0.1
+    nodes, cells = cylinder_mesh(h, jitter=0.1, seed=0)
+    walls = box_and_body_walls(Mesh(nodes, cells), (20.0, 8.0, 1.0))
+    om = OracleMesh(nodes, cells)
+    om.groups = {k: np.asarray(v) for k, v in walls.items()}
+    of = OracleFVM(om, np.float32)
+    cylinder_bcs(of.boundary)
+    of.boundary.finalize()
+    s = OracleSolver(of, None, None, steady_state=False, nu=0.01, implicit_outflow=True,
+                     p_refine=p_refine)
+    s.set_iterations(1, 2, 1)
+    t0 = time.perf_counter()
+    n = 0
+    while n < 5:
+        s.outer_iteration(0.5 * h)
+        n += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    if limiter is not None and hasattr(limiter, 'unregister'):
+        limiter.unregister()
+    per_step = dt / n
+    scale = C_full / om.num_cells
+    return {'value': round(1.0 / (per_step * scale), 5), 'unit': 'PISO time steps/s', 'cores': 1,
+            'kind': 'port',
+            'sample': f'{n} PISO steps of the same cylinder at h=0.1 ({om.num_cells} cells) by the '
+                      f'oracle, 1 thread, {per_step:.2f} s/step; value scaled by the cell ratio '
+                      f'{scale:.2f} to the full mesh'}
+
+
 def cpu_solve_baseline(draws, stop, rtol0, budget_s=25.0):
     """CPU oracle, one thread: the smallest case of the batch solved to the
     same continuity target with the same Krylov policy (or as far as the
@@ -331,7 +578,11 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=3)
-    ap.add_argument('--workload', default='channel', choices=sorted(WORKLOADS) + ['batch64'])
+    ap.add_argument('--workload', default='channel',
+                    choices=sorted(WORKLOADS) + ['batch64', 'serve', 'cylinder'])
+    ap.add_argument('--cyl-h', type=float, default=0.05, help='cylinder: cell size')
+    ap.add_argument('--p-refine', type=int, default=1,
+                    help='cylinder: float64 pressure refinement sweeps')
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--cases', type=int, default=64)
     ap.add_argument('--chunk', type=int, default=256, help='batch: rows per workgroup')
@@ -344,6 +595,13 @@ def main():
     ap.add_argument('--check-u', type=int, default=2,
                     help='batch: check interval of the momentum solve')
     ap.add_argument('--no-sequential', action='store_true')
+    ap.add_argument('--requests', type=int, default=24, help='serve: requests in the stream')
+    ap.add_argument('--rate', type=float, default=20.0, help='serve: arrivals per second')
+    ap.add_argument('--hbm-gb', type=float, default=32.0, help='serve: arena HBM budget')
+    ap.add_argument('--slots', type=int, default=64, help='serve: case slots')
+    ap.add_argument('--workers', type=int, default=8, help='serve: host preparation threads')
+    ap.add_argument('--serve-stop-u', action='store_true',
+                    help='serve: also require the velocity-error estimate (as batch64)')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-profile', action='store_true')
     ap.add_argument('--no-tune', action='store_true')
@@ -360,6 +618,10 @@ def main():
         sys.exit(2)
     if args.workload == 'batch64':
         return bench_batch64(args)
+    if args.workload == 'serve':
+        return bench_serve(args)
+    if args.workload == 'cylinder':
+        return bench_cylinder(args)
 
     import numpy as np
     import torch

@@ -313,6 +313,9 @@ typedef struct wcfd_simple_params {
   int32_t first_iteration; /* 1: u,v,w gradients may be non-zero on entry */
   int32_t cg_variant;      /* WCFD_CG_* for the pressure equation */
   int32_t check_every_u;   /* check interval of the momentum solve; 0: check_every */
+  int32_t p_refine;        /* 0: the reference's pressure solve; n (<= 8): mixed-precision
+                              iterative refinement, 1 + n warm-started CG sweeps, each from
+                              a residual b - A p formed in float64 */
 } wcfd_simple_params;
 
 /* Device state of one case. */

@@ -39,11 +39,33 @@ def threshold(b, atol, tol=None, r0=None, rtol0=0.0):
     return t
 
 
-def cg(A, b, x, inv_diag, atol=1e-5, tol=None, maxiter=500, check_every=10, rtol0=0.0):
+def _residual(A, b, x, r64):
+    """b - A x in the vector dtype, or formed in float64 and rounded once."""
+    if r64:
+        return (b.astype(np.float64) - A.astype(np.float64) @ x.astype(np.float64)).astype(x.dtype)
+    return (b - A @ x).astype(x.dtype)
+
+
+def refined(solver, A, b, x, inv_diag, refine=0, **kw):
+    """Mixed-precision iterative refinement (the HIP path's ``p_refine``; not
+    in the reference): 1 + refine warm-started sweeps of ``solver``, each
+    from a float64 residual.  Returns (total iterations, last ||r||)."""
+    if refine <= 0:
+        return solver(A, b, x, inv_diag, **kw)
+    total = 0
+    res = None
+    for _ in range(1 + refine):
+        it, res = solver(A, b, x, inv_diag, r64=True, **kw)
+        total += it
+    return total, res
+
+
+def cg(A, b, x, inv_diag, atol=1e-5, tol=None, maxiter=500, check_every=10, rtol0=0.0,
+       r64=False):
     """Jacobi-preconditioned conjugate gradients on CSR ``A``; returns
     (iterations, ||r||)."""
     dt = x.dtype.type
-    r = (b - A @ x).astype(x.dtype)
+    r = _residual(A, b, x, r64)
     thr2 = threshold(b, atol, tol, r, rtol0) ** 2
     z = (r * inv_diag).astype(x.dtype)
     p = z.copy()
@@ -106,13 +128,14 @@ def bicgstab(A, b, x, inv_diag, atol=1e-5, tol=None, maxiter=500, check_every=10
             return it, np.sqrt(rr)
 
 
-def cg_single_reduction(A, b, x, inv_diag, atol=1e-5, tol=None, maxiter=500, check_every=10, rtol0=0.0):
+def cg_single_reduction(A, b, x, inv_diag, atol=1e-5, tol=None, maxiter=500, check_every=10,
+                        rtol0=0.0, r64=False):
     """Chronopoulos-Gear PCG: the same iterates as ``cg`` in exact arithmetic,
     with both inner products of an iteration taken together (one global
     reduction per iteration).  Restates the variant the HIP path can run as
     one fused kernel per iteration; same stopping rule as ``cg``."""
     dt = x.dtype.type
-    r = (b - A @ x).astype(x.dtype)
+    r = _residual(A, b, x, r64)
     thr2 = threshold(b, atol, tol, r, rtol0) ** 2
     u = (r * inv_diag).astype(x.dtype)
     w = (A @ u).astype(x.dtype)

@@ -25,7 +25,7 @@ from . import krylov
 class OracleSolver:
     def __init__(self, fvm, u_relax=0.7, p_relax=0.3, correction=False, steady_state=True,
                  nu=0.01, rho=1.0, atol=None, maxiter=500, check_every=10,
-                 implicit_outflow=False, cg_variant=0, tol=None, rtol0=0.0):
+                 implicit_outflow=False, cg_variant=0, tol=None, rtol0=0.0, p_refine=0):
         self.fvm = fvm
         self.u_relax = u_relax
         self.p_relax = p_relax
@@ -37,6 +37,7 @@ class OracleSolver:
         self.atol = atol if atol is not None else (1e-5 if d == np.float32 else 1e-6)
         self.tol = tol              # None: = atol, as the reference's call
         self.rtol0 = rtol0
+        self.p_refine = p_refine
         self.maxiter = maxiter
         self.check_every = check_every
         self.implicit_outflow = implicit_outflow
@@ -147,9 +148,10 @@ class OracleSolver:
                     self.p_eq.replace_row(fvm.reference_pressure_cell_id, fvm.reference_pressure)
                 Pd = self.p_eq.diagonal
                 invp = (1.0 / Pd).astype(d)
-                it_p, res_p = self.cg(self.p_eq.matrix(), self.p_eq.rhs, self.p_eq.x, invp,
-                                      self.atol, self.tol, maxiter=self.maxiter,
-                                      check_every=self.check_every, rtol0=self.rtol0)
+                it_p, res_p = krylov.refined(self.cg, self.p_eq.matrix(), self.p_eq.rhs,
+                                             self.p_eq.x, invp, self.p_refine, atol=self.atol,
+                                             tol=self.tol, maxiter=self.maxiter,
+                                             check_every=self.check_every, rtol0=self.rtol0)
                 p_iters.append(it_p)
                 p_new = self.p_eq.x
                 if self.p_relax is not None:

@@ -81,3 +81,30 @@ def test_single_reduction_cg_matches_cg():
     it3, res3 = krylov.cg_single_reduction(A32, b.astype(np.float32), x3, inv.astype(np.float32),
                                            atol=1e-4, maxiter=2000)
     assert res3 <= 1e-4 * max(1.0, np.linalg.norm(b))
+
+
+def test_float64_refinement_lowers_the_true_residual():
+    """Mixed-precision iterative refinement (oracle.krylov.refined): the true
+    residual of the float32 solution, measured in float64, is lower after
+    extra sweeps from a float64 residual than with the float32 recurrence
+    alone at the same (unreachable) tolerance."""
+    import scipy.sparse as sp
+    from oracle import krylov
+    n = 400
+    rng = np.random.default_rng(5)
+    main = np.full(n, 2.0) + rng.uniform(0, 1e-3, n)
+    A64 = sp.diags([-np.ones(n - 1), main, -np.ones(n - 1)], [-1, 0, 1], format='csr')
+    A = A64.astype(np.float32)
+    b = rng.standard_normal(n).astype(np.float32)
+    inv = (1.0 / A.diagonal()).astype(np.float32)
+
+    def true_res(x):
+        return np.linalg.norm(b.astype(np.float64) - A.astype(np.float64) @ x.astype(np.float64))
+
+    x0 = np.zeros(n, np.float32)
+    it0, _ = krylov.cg(A, b, x0, inv, atol=1e-12, tol=0.0, maxiter=2000, check_every=10)
+    x2 = np.zeros(n, np.float32)
+    it2, _ = krylov.refined(krylov.cg, A, b, x2, inv, refine=2, atol=1e-12, tol=0.0,
+                            maxiter=2000, check_every=10)
+    assert true_res(x2) < true_res(x0)
+    assert it2 > it0

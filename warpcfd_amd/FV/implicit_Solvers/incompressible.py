@@ -86,6 +86,9 @@ class IncompressibleSolver:
         # momentum solve check interval; None: check_every (the reference's
         # single setting)
         self.check_every_u = None
+        # not in the reference: extra pressure-CG sweeps from a float64
+        # residual (mixed-precision iterative refinement); 0 = off
+        self.p_refine = 0
         # pressure CG: WCFD_CG_STANDARD (the reference's two-reduction form) or
         # WCFD_CG_SINGLE_REDUCTION (same iterates in exact arithmetic, one
         # fused kernel per iteration)
@@ -136,6 +139,7 @@ class IncompressibleSolver:
         p.maxiter = int(self.max_iter)
         p.check_every = int(self.check_every)
         p.check_every_u = int(self.check_every_u or 0)
+        p.p_refine = int(self.p_refine)
         p.first_iteration = 0 if m._uvw_grad_zero else 1
         p.cg_variant = int(self.cg_variant)
         return p

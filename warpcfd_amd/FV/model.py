@@ -56,14 +56,16 @@ class FVM:
         self._uvw_grad_zero = True
 
     # ------------------------------------------------------------ set-up
-    def finalize(self):
+    def finalize(self, layout=None):
         """model.py:121-139: upload the mesh, check and upload the BCs,
-        allocate the field arrays."""
+        allocate the field arrays.  ``layout``: a ``DeviceLayout`` of this
+        mesh built earlier (not in the reference; saves renumbering twice)."""
         if self.finalized:
             return
         dev = require_gpu(self._device)
         self.boundary.finalize()
-        self.layout = DeviceLayout(self.mesh, renumber=self._renumber)
+        self.layout = layout if layout is not None else DeviceLayout(self.mesh,
+                                                                     renumber=self._renumber)
         self.dm = DeviceMesh(self.layout, dev)
         self.bc_type, self.bc_value = self.boundary.device_arrays(self.layout, dev)
         self.bc = _lib.WcfdBC(self.bc_type.data_ptr(), self.bc_value.data_ptr())

@@ -53,8 +53,8 @@ class CaseSpec:
         L = layout if layout is not None else DeviceLayout(model.mesh, renumber=model._renumber)
         if L.slot_dnb is None:
             raise ValueError('the case band does not fit int16 neighbour offsets')
-        self.layout = L
         self.bc_type, self.bc_value = b.host_arrays(L)
+        self.layout = L.arena_view()
         rho = float(model.material.density)
         self.density = rho
         self.nu = float(model.material.viscosity) / rho
@@ -137,7 +137,7 @@ class CaseArena:
 
     def __init__(self, cell_cap, internal_cap, boundary_cap, max_cases, device='cuda',
                  chunk=256, block=256, density=1.0, alpha_u=0.7, alpha_p=0.3, atol=0.0,
-                 tol=1e-6, rtol0=0.0, maxiter=2000, check_every=10, check_every_u=0,
+                 tol=1e-6, rtol0=0.0, maxiter=2000, check_every=10, check_every_u=0, p_refine=0,
                  cg_variant=_lib.CG_SINGLE_REDUCTION, implicit_outflow=True, correction=False):
         self.device = require_gpu(device)
         self.lib = _lib.load()
@@ -217,6 +217,7 @@ class CaseArena:
         self.alpha_u, self.alpha_p = alpha_u, alpha_p
         self.atol, self.tol, self.rtol0 = atol, tol, rtol0
         self.maxiter, self.check_every, self.check_every_u = maxiter, check_every, check_every_u
+        self.p_refine = p_refine
         self.cg_variant = cg_variant
         self.scheme_flags = (_lib.SCHEME_IMPLICIT_OUTFLOW if implicit_outflow else 0) | \
             (_lib.SCHEME_CORRECTED if correction else 0)
@@ -237,9 +238,11 @@ class CaseArena:
         return a
 
     @classmethod
-    def for_budget(cls, hbm_bytes, max_cases, cells_per_face=(2.9, 0.25), **kw):
+    def for_budget(cls, hbm_bytes, max_cases, cells_per_face=(3.0, 2.1), **kw):
         """An arena whose capacity fits ``hbm_bytes`` of device memory, with
-        internal / boundary face capacities in the given ratio to cells."""
+        internal / boundary face capacities in the given ratio to cells
+        (3-D hex meshes have about 3 internal faces per cell; one-cell-thick
+        2-D meshes about 2 boundary faces per cell)."""
         ri, rb = cells_per_face
         per_cell = cls.bytes_for(1, ri, rb)
         C = int(hbm_bytes // per_cell)
@@ -420,6 +423,7 @@ class CaseArena:
         p.atol, p.tol, p.rtol0 = float(self.atol), float(self.tol), float(self.rtol0)
         p.maxiter, p.check_every = int(self.maxiter), int(self.check_every)
         p.check_every_u = int(self.check_every_u)
+        p.p_refine = int(self.p_refine)
         p.first_iteration = 0
         p.cg_variant = int(self.cg_variant)
         return p

@@ -40,9 +40,11 @@ int launch_correct_velocity(const wcfd_mesh& m, const BatchDev& bd, const Seg& s
 
 // Krylov solvers (krylov.hip).  Stopping threshold
 // max(tol ||b||, rtol0 ||r_0||, atol); the reference's call has tol = atol
-// and rtol0 = 0.
+// and rtol0 = 0.  refine > 0 (CG only): that many extra warm-started sweeps,
+// every sweep starting from a float64 residual.
 struct Tol {
   double atol, tol, rtol0;
+  int refine;
 };
 
 struct CsrView {

@@ -28,6 +28,7 @@
 #include <math.h>
 
 #include <atomic>
+#include <vector>
 
 namespace wcfd {
 
@@ -160,16 +161,31 @@ __global__ void k_check(Seg sg, const double* __restrict__ part, KState* st, int
 
 // ------------------------------------------------------------------ CG
 // partial slots: PA[3][G] = {r.z, r.r, b.b} (init, k_cg_2), PB[1][G] = {p.q} (k_cg_1)
-template <class M>
+// Initial residual r = b - A x.  With R64 the product and the difference are
+// accumulated in float64 and rounded once (the residual of mixed-precision
+// iterative refinement); otherwise in float32, as the reference's solver.
+template <bool R64, class M>
+__device__ inline float residual_row(const M& A, int i, const float* __restrict__ b,
+                                     const float* __restrict__ x) {
+  if constexpr (R64) {
+    double ax = 0.0;
+    A.row(i, [&](float a, int c) { ax += (double)a * (double)x[c]; });
+    return (float)((double)b[i] - ax);
+  } else {
+    float ax = 0.f;
+    A.row(i, [&](float a, int c) { ax += a * x[c]; });
+    return b[i] - ax;
+  }
+}
+
+template <class M, bool R64>
 __global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_init(M A, Seg sg, const float* __restrict__ b, const float* __restrict__ x,
                           float* __restrict__ r, const float* __restrict__ inv,
                           double* __restrict__ PA) {
   const BlockRows br = block_rows(sg);
   double acc[3] = {0.0, 0.0, 0.0};
   for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
-    float ax = 0.f;
-    A.row(i, [&](float a, int c) { ax += a * x[c]; });
-    const float ri = b[i] - ax;
+    const float ri = residual_row<R64>(A, i, b, x);
     r[i] = ri;
     const float zi = ri * inv[i];
     acc[0] += (double)ri * (double)zi;
@@ -300,15 +316,13 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_2(Seg sg, int it, float*
 // i-1); the neighbour values of r_{i+1} = r_i - alpha_i s_i are formed on the
 // fly.  Partials ping-pong between PA and PC: kernel i reads the buffer
 // kernel i-1 wrote.
-template <class M>
+template <class M, bool R64>
 __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_init1(M A, Seg sg, const float* __restrict__ b, const float* __restrict__ x,
                             float* __restrict__ r, double* __restrict__ PC) {
   const BlockRows br = block_rows(sg);
   double acc[3] = {0.0, 0.0, 0.0};
   for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
-    float ax = 0.f;
-    A.row(i, [&](float a, int c) { ax += a * x[c]; });
-    const float ri = b[i] - ax;
+    const float ri = residual_row<R64>(A, i, b, x);
     r[i] = ri;
     acc[1] += (double)ri * (double)ri;
     acc[2] += (double)b[i] * (double)b[i];
@@ -893,7 +907,7 @@ static int prepare(KState* st, const Seg& sg, const Tol& tl, hipStream_t s) {
 }
 
 template <class M>
-static int cg_impl(const M& A, const KLaunch& L, const float* b, float* x, const Tol& tl,
+static int cg_impl(const M& A, const KLaunch& L, const float* b, float* x, const Tol& tl, bool r64,
                    int maxiter, int check_every, KWork& w, wcfd_solve_info* info, hipStream_t s) {
   const Seg& sg = L.sg;
   const int n = sg.n, B = L.B, G = sg.G, K = sg.n_cases;
@@ -901,7 +915,10 @@ static int cg_impl(const M& A, const KLaunch& L, const float* b, float* x, const
   int e = prepare(w.st, sg, tl, s);
   if (e) return e;
   WCFD_LAUNCH(K_INV_DIAG, s, k_inv_diag<M>, grid_for(n), A, n, w.inv);
-  WCFD_LAUNCH_B(K_CG_INIT, s, k_cg_init<M>, G, B, A, sg, b, x, w.r, w.inv, w.PA);
+  if (r64)
+    WCFD_LAUNCH_B(K_CG_INIT, s, (k_cg_init<M, true>), G, B, A, sg, b, x, w.r, w.inv, w.PA);
+  else
+    WCFD_LAUNCH_B(K_CG_INIT, s, (k_cg_init<M, false>), G, B, A, sg, b, x, w.r, w.inv, w.PA);
   WCFD_LAUNCH(K_FOLD, s, k_init_fold, K, sg, w.PA, w.PA, w.st, 0, maxiter);
   if ((e = check_launch())) return e;
   float* pb[2] = {w.p0, w.p1};
@@ -920,7 +937,7 @@ static int cg_impl(const M& A, const KLaunch& L, const float* b, float* x, const
 }
 
 template <class M>
-static int cgs_impl(const M& A, const KLaunch& L, const float* b, float* x, const Tol& tl, int maxiter, int check_every, KWork& w, wcfd_solve_info* info,
+static int cgs_impl(const M& A, const KLaunch& L, const float* b, float* x, const Tol& tl, bool r64, int maxiter, int check_every, KWork& w, wcfd_solve_info* info,
                     hipStream_t s) {
   const Seg& sg = L.sg;
   const int n = sg.n, B = L.B, G = sg.G, K = sg.n_cases;
@@ -931,7 +948,10 @@ static int cgs_impl(const M& A, const KLaunch& L, const float* b, float* x, cons
   float* sb[2] = {w.s, w.t};
   double* pb[2] = {w.PA, w.PC};   // iteration i reads pb[i&1], writes pb[(i+1)&1]
   WCFD_LAUNCH(K_INV_DIAG, s, k_inv_diag<M>, grid_for(n), A, n, w.inv);
-  WCFD_LAUNCH_B(K_CG_INIT, s, k_cgs_init1<M>, G, B, A, sg, b, x, rb[0], w.PC);
+  if (r64)
+    WCFD_LAUNCH_B(K_CG_INIT, s, (k_cgs_init1<M, true>), G, B, A, sg, b, x, rb[0], w.PC);
+  else
+    WCFD_LAUNCH_B(K_CG_INIT, s, (k_cgs_init1<M, false>), G, B, A, sg, b, x, rb[0], w.PC);
   WCFD_LAUNCH_B(K_CG_INIT, s, k_cgs_init2<M>, G, B, A, sg, rb[0], w.inv, wb[0], w.PA);
   WCFD_LAUNCH(K_FOLD, s, k_init_fold, K, sg, w.PA, w.PC, w.st, 2, maxiter);
   if ((e = check_launch())) return e;
@@ -981,11 +1001,29 @@ template <class M>
 static int cg_dispatch(const M& A, const KLaunch& L, int variant, const float* b, float* x,
                        const Tol& tl, int maxiter, int check_every, KWork& w,
                        wcfd_solve_info* info, hipStream_t s) {
-  if (variant == WCFD_CG_SINGLE_REDUCTION)
-    return cgs_impl(A, L, b, x, tl, maxiter, check_every, w, info, s);
-  if (variant == WCFD_CG_STANDARD)
-    return cg_impl(A, L, b, x, tl, maxiter, check_every, w, info, s);
-  return WCFD_EINVAL;
+  if (variant != WCFD_CG_SINGLE_REDUCTION && variant != WCFD_CG_STANDARD) return WCFD_EINVAL;
+  // Mixed-precision iterative refinement: with tl.refine = n > 0 the solve
+  // runs 1 + n warm-started sweeps, each from a residual b - A x formed in
+  // float64; iteration counts add up over the sweeps.
+  const int sweeps = 1 + (tl.refine > 0 ? tl.refine : 0);
+  const bool r64 = tl.refine > 0;
+  const int K = L.sg.n_cases;
+  std::vector<wcfd_solve_info> tmp((info && sweeps > 1) ? K : 0);
+  for (int sw = 0; sw < sweeps; ++sw) {
+    wcfd_solve_info* out = (info && sweeps > 1) ? tmp.data() : info;
+    const int e = variant == WCFD_CG_SINGLE_REDUCTION
+                      ? cgs_impl(A, L, b, x, tl, r64, maxiter, check_every, w, out, s)
+                      : cg_impl(A, L, b, x, tl, r64, maxiter, check_every, w, out, s);
+    if (e) return e;
+    if (info && sweeps > 1) {
+      for (int k = 0; k < K; ++k) {
+        const int prev = sw ? info[k].iterations : 0;
+        info[k] = tmp[k];
+        info[k].iterations += prev;
+      }
+    }
+  }
+  return WCFD_OK;
 }
 
 template <class M>
@@ -1143,7 +1181,7 @@ int wcfd_cg(int32_t n, const int32_t* row_ptr, const int32_t* col, const float* 
   if (n <= 0 || !row_ptr || !col || !values || !diag_pos || !b || !x || !workspace)
     return WCFD_EINVAL;
   CsrView A{n, row_ptr, col, values, diag_pos};
-  return solve_cg(A, variant, b, x, Tol{atol, tol, 0.0}, maxiter, check_every, workspace, info,
+  return solve_cg(A, variant, b, x, Tol{atol, tol, 0.0, 0}, maxiter, check_every, workspace, info,
                   (hipStream_t)stream);
 }
 
@@ -1154,7 +1192,7 @@ int wcfd_bicgstab(int32_t n, const int32_t* row_ptr, const int32_t* col, const f
   if (n <= 0 || !row_ptr || !col || !values || !diag_pos || !b || !x || !workspace)
     return WCFD_EINVAL;
   CsrView A{n, row_ptr, col, values, diag_pos};
-  return solve_bicgstab(A, nvec, b, x, Tol{atol, tol, 0.0}, maxiter, check_every, workspace, info,
+  return solve_bicgstab(A, nvec, b, x, Tol{atol, tol, 0.0, 0}, maxiter, check_every, workspace, info,
                         (hipStream_t)stream);
 }
 
@@ -1163,7 +1201,7 @@ int wcfd_cg_ell(const wcfd_mesh* m, const float* values, int variant, const floa
                 wcfd_solve_info* info, void* stream) {
   if (!m || m->n_cells <= 0 || !m->slot_nbr || !values || !b || !x || !workspace)
     return WCFD_EINVAL;
-  return solve_cg_ell(*m, nullptr, values, variant, b, x, Tol{atol, tol, 0.0}, maxiter, check_every,
+  return solve_cg_ell(*m, nullptr, values, variant, b, x, Tol{atol, tol, 0.0, 0}, maxiter, check_every,
                       workspace, info, (hipStream_t)stream);
 }
 
@@ -1172,7 +1210,7 @@ int wcfd_bicgstab_ell(const wcfd_mesh* m, const float* values, int nvec, const f
                       wcfd_solve_info* info, void* stream) {
   if (!m || m->n_cells <= 0 || !m->slot_nbr || !values || !b || !x || !workspace)
     return WCFD_EINVAL;
-  return solve_bicgstab_ell(*m, nullptr, values, nvec, b, x, Tol{atol, tol, 0.0}, maxiter, check_every,
+  return solve_bicgstab_ell(*m, nullptr, values, nvec, b, x, Tol{atol, tol, 0.0, 0}, maxiter, check_every,
                             workspace, info, (hipStream_t)stream);
 }
 
@@ -1180,7 +1218,7 @@ int wcfd_batch_cg_ell(const wcfd_mesh* m, const wcfd_batch* bt, const float* val
                       const float* b, float* x, double atol, double tol, int maxiter,
                       int check_every, void* workspace, wcfd_solve_info* info, void* stream) {
   if (!batch_ok(m, bt) || !m->slot_nbr || !values || !b || !x || !workspace) return WCFD_EINVAL;
-  return solve_cg_ell(*m, bt, values, variant, b, x, Tol{atol, tol, 0.0}, maxiter, check_every, workspace,
+  return solve_cg_ell(*m, bt, values, variant, b, x, Tol{atol, tol, 0.0, 0}, maxiter, check_every, workspace,
                       info, (hipStream_t)stream);
 }
 
@@ -1189,7 +1227,7 @@ int wcfd_batch_bicgstab_ell(const wcfd_mesh* m, const wcfd_batch* bt, const floa
                             int maxiter, int check_every, void* workspace, wcfd_solve_info* info,
                             void* stream) {
   if (!batch_ok(m, bt) || !m->slot_nbr || !values || !b || !x || !workspace) return WCFD_EINVAL;
-  return solve_bicgstab_ell(*m, bt, values, nvec, b, x, Tol{atol, tol, 0.0}, maxiter, check_every,
+  return solve_bicgstab_ell(*m, bt, values, nvec, b, x, Tol{atol, tol, 0.0, 0}, maxiter, check_every,
                             workspace, info, (hipStream_t)stream);
 }
 

@@ -71,7 +71,8 @@ static int simple_iteration(const wcfd_mesh& m, const wcfd_batch* bt, wcfd_bc bc
                                     st->cell_grad, 1, prm->nu, 1.0f / prm->density, prm->alpha_u,
                                     prm->dt, prm->scheme_flags, st->mom_values, st->mom_rhs, s)))
     return e;
-  const Tol tl{prm->atol, prm->tol, prm->rtol0};
+  const Tol tl{prm->atol, prm->tol, prm->rtol0, 0};
+  const Tol tp{prm->atol, prm->tol, prm->rtol0, prm->p_refine};
   const int ce_u = prm->check_every_u > 0 ? prm->check_every_u : prm->check_every;
   e = solve_bicgstab_ell(m, bt, st->mom_values, 3, st->mom_rhs, st->hbya, tl, prm->maxiter, ce_u,
                          w.krylov, mom.data(), s);
@@ -93,7 +94,7 @@ static int simple_iteration(const wcfd_mesh& m, const wcfd_batch* bt, wcfd_bc bc
                                         prm->ref_cell, prm->ref_value, st->p_values, st->p_rhs,
                                         nullptr, s)))
         return e;
-      e = solve_cg_ell(m, bt, st->p_values, prm->cg_variant, st->p_rhs, st->p_x, tl,
+      e = solve_cg_ell(m, bt, st->p_values, prm->cg_variant, st->p_rhs, st->p_x, tp,
                        prm->maxiter, prm->check_every, w.krylov, pres.data(), s);
       if (e) return e;
       if ((e = launch_relax_p(m, bd, st->p_x, prm->alpha_p, st->cell_values, s))) return e;
@@ -130,7 +131,7 @@ static int simple_iteration(const wcfd_mesh& m, const wcfd_batch* bt, wcfd_bc bc
 
 static bool params_ok(const wcfd_simple_params* prm) {
   return prm && prm->density != 0.f && prm->check_every >= 0 && prm->check_every_u >= 0 &&
-         prm->rtol0 >= 0.0 &&
+         prm->rtol0 >= 0.0 && prm->p_refine >= 0 && prm->p_refine <= 8 &&
          (prm->cg_variant == WCFD_CG_STANDARD || prm->cg_variant == WCFD_CG_SINGLE_REDUCTION);
 }
 

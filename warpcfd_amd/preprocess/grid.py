@@ -99,3 +99,27 @@ def define_boundary_walls(mesh):
         walls[name] = b[np.abs(n @ np.array(e, dtype=n.dtype) - 1) < 1e-3]
     mesh.add_groups(walls, 'face')
     return walls
+
+
+def box_and_body_walls(mesh, lengths, origin=(0.0, 0.0, 0.0), name='cylinder', tol=1e-6):
+    """Boundary groups of a box with a carved body: faces whose centroid lies
+    on a box plane go to '-X', '+X', '-Y', '+Y', '-Z', '+Z'; all other
+    boundary faces (the carved surface) to ``name``.  Not in the reference:
+    define_boundary_walls (grid.py:165-194) classifies by normal, which puts
+    the axis-aligned faces of a voxel-carved body into the box groups."""
+    b = mesh.boundary_face_ids
+    c = np.asarray(mesh.face_properties.centroid, np.float64)[b]
+    lo = np.asarray(origin, np.float64)
+    hi = lo + np.asarray(lengths, np.float64)
+    walls = {}
+    on_box = np.zeros(b.size, bool)
+    for ax, (n_lo, n_hi) in enumerate((('-X', '+X'), ('-Y', '+Y'), ('-Z', '+Z'))):
+        at_lo = np.abs(c[:, ax] - lo[ax]) < tol * max(1.0, abs(hi[ax]))
+        at_hi = np.abs(c[:, ax] - hi[ax]) < tol * max(1.0, abs(hi[ax]))
+        walls[n_lo] = b[at_lo & ~on_box]
+        on_box |= at_lo
+        walls[n_hi] = b[at_hi & ~on_box]
+        on_box |= at_hi
+    walls[name] = b[~on_box]
+    mesh.add_groups(walls, 'face')
+    return walls

@@ -145,6 +145,17 @@ class DeviceLayout:
         self.face_k0 = k0[fo].astype(np.int8)
         self.face_k1 = k1[fi_sorted].astype(np.int8)
 
+    ARENA_FIELDS = ('volume', 'slot_face', 'slot_side', 'slot_sn', 'slot_nbr', 'slot_dnb',
+                    'owner', 'neigh', 'psi', 'skew', 'area', 'normal', 'dist', 'dmag', 'old2new',
+                    'num_cells', 'num_faces', 'n_internal', 'n_boundary', 'nnz', 'band')
+
+    def arena_view(self):
+        """The arrays a case needs inside a batch arena (no CSR pattern, no
+        face maps): about half the bytes, for shipping prepared cases
+        between processes."""
+        import types
+        return types.SimpleNamespace(**{k: getattr(self, k) for k in self.ARENA_FIELDS})
+
     def ell_to_csr(self, ell):
         """Host conversion of ELL values [7][C] (diagonal, 6 slots) to values
         on (row_ptr, col)."""
