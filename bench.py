@@ -143,7 +143,8 @@ def bench_batch64(args):
     stop = 1e-6
     stop_u, floor_u = args.stop_u, 1e-7
     pol = dict(atol=0.0, tol=0.0, rtol0=args.rtol0, maxiter=2000, check_every=10,
-               check_every_u=args.check_u, cg_variant=_lib.CG_SINGLE_REDUCTION)
+               check_every_u=args.check_u, maxiter_u=args.maxiter_u,
+               cg_variant=_lib.CG_SINGLE_REDUCTION)
     draws = batch64_draws(args.seed)[:args.cases]
     t = time.perf_counter()
     specs = [CaseSpec(cavity_model(n, re), stop=stop, max_outer=args.max_outer,
@@ -166,12 +167,28 @@ def bench_batch64(args):
     t0 = time.perf_counter()
     done_at = {}
     steps = 0
+    cur = None
+    n_capture = 0
     while len(done_at) < len(specs) and steps < args.max_outer + 1:
-        arena.step()
+        if args.graph:
+            # HIP-graph replay with a Krylov launch budget measured every 100 iterations
+            stale = cur is not None and arena._graph['blocks'] != arena.n_blocks
+            if cur is None or steps % 100 == 0 or stale:
+                need = arena.budget_for(arena.step(info=True), 1.5)
+                if (stale or cur is None or need[0] > cur[0] or need[1] > cur[1]
+                        or 2 * need[0] < cur[0] or 2 * need[1] < cur[1]):
+                    cur = need
+                    arena.capture(cur)
+                    n_capture += 1
+            else:
+                arena.replay()
+        else:
+            arena.step()
         steps += 1
         now = time.perf_counter() - t0
-        for k in arena.poll():
-            done_at.setdefault(k, now)
+        if steps % args.poll_every == 0 or args.graph:
+            for k in arena.poll():
+                done_at.setdefault(k, now)
         if steps % 1000 == 0:
             print(f'# batched: {steps} iterations, {len(done_at)} cases done, {now:.1f} s',
                   file=sys.stderr, flush=True)
@@ -220,6 +237,7 @@ def bench_batch64(args):
             sv.tol, sv.rtol, sv.rtol0 = pol['atol'], pol['tol'], pol['rtol0']
             sv.max_iter, sv.check_every, sv.cg_variant = pol['maxiter'], 10, pol['cg_variant']
             sv.check_every_u = pol['check_every_u']
+            sv.max_iter_u = pol['maxiter_u']
             torch.cuda.synchronize()
             t = time.perf_counter()
             it, ok, _ = sv.run_to_convergence(stop, stop_u, floor_u, max_outer=args.max_outer)
@@ -242,7 +260,7 @@ def bench_batch64(args):
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_solve_baseline(draws, stop, pol['rtol0'])
+        cpu = cpu_solve_baseline(draws, stop, pol['rtol0'], stop_u)
 
     value = len(specs) / batched_s
     out = {
@@ -266,11 +284,14 @@ def bench_batch64(args):
                    'krylov': {'atol': 0.0, 'tol': 0.0, 'rtol0': pol['rtol0'],
                               'maxiter': pol['maxiter'], 'check_every': 10,
                               'check_every_momentum': pol['check_every_u'],
+                              'maxiter_momentum': pol['maxiter_u'] or pol['maxiter'],
                               'pressure_cg': 'single'},
                    'stop': {'continuity': stop, 'velocity_error_estimate': stop_u,
                             'velocity_change_floor': floor_u},
                    'reductions': 'f64'},
         'batched': {'total_s': round(batched_s, 3), 'batch_iterations': steps,
+                    'hip_graph': ({'captures': n_capture, 'last_budget': cur, 'recheck_every': 100}
+                                  if args.graph else None),
                     'converged': n_conv, 'outer_iterations_max': int(max(iters)),
                     'time_to_converged_s': {'p50': round(float(np.percentile(lat, 50)), 3),
                                             'p99': round(float(np.percentile(lat, 99)), 3)}},
@@ -531,10 +552,11 @@ def cpu_cylinder_baseline(C_full, p_refine, budget_s=30.0):
                       f'{scale:.2f} to the full mesh'}
 
 
-def cpu_solve_baseline(draws, stop, rtol0, budget_s=25.0):
+def cpu_solve_baseline(draws, stop, rtol0, stop_u, budget_s=25.0):
     """CPU oracle, one thread: the smallest case of the batch solved to the
     same continuity target with the same Krylov policy (or as far as the
     time budget allows), reported as solves/s of that case."""
+    import numpy as np
     try:
         from threadpoolctl import threadpool_limits
         limiter = threadpool_limits(limits=1)
@@ -554,23 +576,31 @@ def cpu_solve_baseline(draws, stop, rtol0, budget_s=25.0):
     of.boundary.finalize()
     s = OracleSolver(of, 0.7, 0.3, nu=1.0 / re, atol=0.0, tol=0.0, rtol0=rtol0, maxiter=2000,
                      cg_variant=1)
+    from warpcfd_amd.FV.implicit_Solvers.incompressible import Monitor
+    mon = Monitor(stop, stop_u, 1e-7)      # the same stop rule as the GPU runs
     t0 = time.perf_counter()
     it = 0
     cont = 1.0
+    done = False
+    prev = of.cell_values[:, :3].astype(np.float64).copy()
     while time.perf_counter() - t0 < budget_s:
         cont = s.outer_iteration()['continuity']
         it += 1
-        if cont <= stop:
+        cur = of.cell_values[:, :3].astype(np.float64)
+        d = float(np.linalg.norm(cur - prev) / max(np.linalg.norm(cur), 1e-300))
+        prev = cur.copy()
+        if mon.update(cont, d):
+            done = True
             break
     dt = time.perf_counter() - t0
     if limiter is not None and hasattr(limiter, 'unregister'):
         limiter.unregister()
-    done = cont <= stop
     return {'value': round((1.0 / dt) if done else 0.0, 5), 'unit': 'solves/s', 'cores': 1,
             'kind': 'port',
             'sample': f'smallest batch case (cavity {n}x{n}, Re {re:.0f}) by the oracle, 1 thread: '
-                      f'{it} SIMPLE iterations in {dt:.1f} s, continuity {cont:.2e}'
-                      + ('' if done else ' (budget reached before the target; value 0)')}
+                      f'{it} SIMPLE iterations in {dt:.1f} s, continuity {cont:.2e}, stopped by '
+                      f'the same convergence monitor as the GPU runs'
+                      + ('' if done else ' (budget reached before convergence; value 0)')}
 
 
 def main():
@@ -592,9 +622,16 @@ def main():
     ap.add_argument('--max-outer', type=int, default=40000)
     ap.add_argument('--stop-u', type=float, default=2e-5,
                     help='batch: stop at this estimated velocity error (with continuity <= 1e-6)')
+    ap.add_argument('--poll-every', type=int, default=4,
+                    help='batch: read the active flags every this many iterations')
+    ap.add_argument('--maxiter-u', type=int, default=0,
+                    help='batch: momentum-solve iteration cap (0: maxiter)')
     ap.add_argument('--check-u', type=int, default=2,
                     help='batch: check interval of the momentum solve')
     ap.add_argument('--no-sequential', action='store_true')
+    ap.add_argument('--graph', action='store_true',
+                    help='batch: replay a HIP-graph-captured iteration (Krylov launch budget '
+                         'measured every 100 iterations)')
     ap.add_argument('--requests', type=int, default=24, help='serve: requests in the stream')
     ap.add_argument('--rate', type=float, default=20.0, help='serve: arrivals per second')
     ap.add_argument('--hbm-gb', type=float, default=32.0, help='serve: arena HBM budget')

@@ -316,6 +316,12 @@ typedef struct wcfd_simple_params {
   int32_t p_refine;        /* 0: the reference's pressure solve; n (<= 8): mixed-precision
                               iterative refinement, 1 + n warm-started CG sweeps, each from
                               a residual b - A p formed in float64 */
+  int32_t graph_groups_u;  /* 0: the Krylov solves poll the host (adaptive).  > 0 (both):
+                              fixed launch budget for stream capture -- the momentum and */
+  int32_t graph_groups_p;  /* pressure solves launch exactly this many check groups, with no
+                              event or host read (maxiter clipped to groups * check_every;
+                              cases done earlier exit at entry).  Pass info = NULL. */
+  int32_t maxiter_u;       /* momentum-solve iteration cap; 0: maxiter (the reference) */
 } wcfd_simple_params;
 
 /* Device state of one case. */

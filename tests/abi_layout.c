@@ -11,7 +11,7 @@ int main(void) {
   printf("wcfd_solve_info %zu\n", sizeof(wcfd_solve_info));
   P(wcfd_solve_info, residual);
   printf("wcfd_simple_params %zu\n", sizeof(wcfd_simple_params));
-  P(wcfd_simple_params, atol); P(wcfd_simple_params, rtol0); P(wcfd_simple_params, maxiter); P(wcfd_simple_params, first_iteration); P(wcfd_simple_params, check_every_u); P(wcfd_simple_params, p_refine);
+  P(wcfd_simple_params, atol); P(wcfd_simple_params, rtol0); P(wcfd_simple_params, maxiter); P(wcfd_simple_params, first_iteration); P(wcfd_simple_params, check_every_u); P(wcfd_simple_params, p_refine); P(wcfd_simple_params, graph_groups_p); P(wcfd_simple_params, maxiter_u);
   printf("wcfd_simple_state %zu\n", sizeof(wcfd_simple_state));
   P(wcfd_simple_state, resid);
   printf("wcfd_simple_info %zu\n", sizeof(wcfd_simple_info));

@@ -89,6 +89,9 @@ class IncompressibleSolver:
         # not in the reference: extra pressure-CG sweeps from a float64
         # residual (mixed-precision iterative refinement); 0 = off
         self.p_refine = 0
+        # not in the reference: iteration cap of the momentum solve (None:
+        # max_iter, as the reference)
+        self.max_iter_u = None
         # pressure CG: WCFD_CG_STANDARD (the reference's two-reduction form) or
         # WCFD_CG_SINGLE_REDUCTION (same iterates in exact arithmetic, one
         # fused kernel per iteration)
@@ -140,6 +143,7 @@ class IncompressibleSolver:
         p.check_every = int(self.check_every)
         p.check_every_u = int(self.check_every_u or 0)
         p.p_refine = int(self.p_refine)
+        p.maxiter_u = int(self.max_iter_u or 0)
         p.first_iteration = 0 if m._uvw_grad_zero else 1
         p.cg_variant = int(self.cg_variant)
         return p

@@ -58,7 +58,8 @@ class SimpleParams(ctypes.Structure):
                 ('scheme_flags', c_i32), ('ref_cell', c_i32), ('ref_value', c_f),
                 ('n_inner', c_i32), ('n_ortho', c_i32), ('atol', c_d), ('tol', c_d), ('rtol0', c_d),
                 ('maxiter', c_i32), ('check_every', c_i32), ('first_iteration', c_i32),
-                ('cg_variant', c_i32), ('check_every_u', c_i32), ('p_refine', c_i32)]
+                ('cg_variant', c_i32), ('check_every_u', c_i32), ('p_refine', c_i32),
+                ('graph_groups_u', c_i32), ('graph_groups_p', c_i32), ('maxiter_u', c_i32)]
 
 
 class SimpleState(ctypes.Structure):

@@ -22,6 +22,7 @@ Cases are admitted and released while others run (``admit`` / ``release``),
 which is what the serving loop in ``warpcfd_amd.serve`` builds on.
 """
 import ctypes
+import math
 
 import numpy as np
 import torch
@@ -138,6 +139,7 @@ class CaseArena:
     def __init__(self, cell_cap, internal_cap, boundary_cap, max_cases, device='cuda',
                  chunk=256, block=256, density=1.0, alpha_u=0.7, alpha_p=0.3, atol=0.0,
                  tol=1e-6, rtol0=0.0, maxiter=2000, check_every=10, check_every_u=0, p_refine=0,
+                 maxiter_u=0,
                  cg_variant=_lib.CG_SINGLE_REDUCTION, implicit_outflow=True, correction=False):
         self.device = require_gpu(device)
         self.lib = _lib.load()
@@ -210,7 +212,7 @@ class CaseArena:
         self.h_cell_off = np.zeros(K + 1, np.int32)
         self.h_cell_end = np.zeros(K, np.int32)
         self.h_active = np.zeros(K, np.uint8)
-        self.h_iters = np.zeros(K, np.int32)
+        self.h_steps = np.zeros(K, np.int64)   # iterations issued while the host saw the case active
         self.n_blocks = 0
         # settings
         self.density = float(density)
@@ -218,6 +220,7 @@ class CaseArena:
         self.atol, self.tol, self.rtol0 = atol, tol, rtol0
         self.maxiter, self.check_every, self.check_every_u = maxiter, check_every, check_every_u
         self.p_refine = p_refine
+        self.maxiter_u = maxiter_u
         self.cg_variant = cg_variant
         self.scheme_flags = (_lib.SCHEME_IMPLICIT_OUTFLOW if implicit_outflow else 0) | \
             (_lib.SCHEME_CORRECTED if correction else 0)
@@ -279,7 +282,7 @@ class CaseArena:
         self.h_cell_off[k] = c0
         self.h_cell_end[k] = c0 + C
         self.h_active[k] = 1
-        self.h_iters[k] = 0
+        self.h_steps[k] = 0
         self._build_descriptors()
         return k
 
@@ -364,11 +367,15 @@ class CaseArena:
         self.active[k] = 1
 
     def _build_descriptors(self):
-        """Workgroup ranges of the resident cases and the C-ABI structs."""
+        """Workgroup ranges of the resident, active cases and the C-ABI
+        structs.  Cases that have finished get no workgroups, so the
+        row-parallel kernels launch only the workgroups of cases still
+        iterating; a case's own partition (and so its arithmetic) does not
+        change when others drop out."""
         K, ch = self.K, self.chunk
         nb = np.zeros(K, np.int64)
         for k, sl in enumerate(self.slots):
-            if sl is not None:
+            if sl is not None and self.h_active[k]:
                 nb[k] = (sl['spec'].n_cells + ch - 1) // ch
         blk_off = np.zeros(K + 1, np.int32)
         blk_off[1:] = np.cumsum(nb)
@@ -424,41 +431,90 @@ class CaseArena:
         p.maxiter, p.check_every = int(self.maxiter), int(self.check_every)
         p.check_every_u = int(self.check_every_u)
         p.p_refine = int(self.p_refine)
+        p.maxiter_u = int(self.maxiter_u)
         p.first_iteration = 0
         p.cg_variant = int(self.cg_variant)
         return p
 
-    def step(self, info=False, stream=None):
+    def step(self, info=False, stream=None, budget=None):
         """One SIMPLE iteration of every active case.  With ``info`` returns
-        per-slot (momentum iterations, pressure iterations, continuity)."""
+        per-slot (momentum iterations, pressure iterations, continuity).
+        ``budget`` = (groups_u, groups_p) runs the Krylov solves with that
+        fixed launch budget instead of polling the host (the mode a captured
+        graph replays)."""
         if self.n_blocks == 0:
             return None
         prm = self.params()
+        if budget is not None:
+            prm.graph_groups_u, prm.graph_groups_p = (int(b) for b in budget)
+            info = False
         arr = (_lib.SimpleInfo * self.K)() if info else None
         code = self.lib.wcfd_batch_simple_iteration(
             ctypes.byref(self.desc), ctypes.byref(self.batch), self.bc, ctypes.byref(prm),
             ctypes.byref(self.state), ptr(self.workspace), arr, stream_ptr(stream))
         _lib.check(code, 'batch simple iteration')
+        self.h_steps += self.h_active.astype(np.int64)
         if info:
             return [(arr[k].momentum.iterations, arr[k].pressure.iterations, arr[k].continuity)
                     for k in range(self.K)]
         return None
 
+    # ---------------------------------------------------------- HIP graph
+    def capture(self, budget):
+        """Capture one batched SIMPLE iteration with the fixed Krylov launch
+        budget ``(groups_u, groups_p)`` as a HIP graph (torch's stream
+        capture; the library launches only kernels in this mode, with no
+        memset, copy or host read).  Kernels read the batch descriptors
+        through pointers, so the graph stays valid while cases finish; it
+        must be captured again after an admission or release changes the
+        number of workgroups.  Call ``step(budget=...)`` once before the
+        first capture, so every kernel is loaded outside the capture."""
+        side = torch.cuda.Stream(device=self.device)
+        side.wait_stream(torch.cuda.current_stream())
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=side):
+            self.step(stream=side, budget=budget)
+        torch.cuda.current_stream().wait_stream(side)
+        self._graph = {'graph': g, 'blocks': self.n_blocks, 'budget': tuple(budget)}
+        return g
+
+    def replay(self):
+        """One batched SIMPLE iteration by replaying the captured graph."""
+        gr = getattr(self, '_graph', None)
+        if gr is None or gr['blocks'] != self.n_blocks:
+            raise RuntimeError('no graph for the current arena contents: call capture() again')
+        gr['graph'].replay()
+        self.h_steps += self.h_active.astype(np.int64)
+
+    def budget_for(self, res, margin=1.5):
+        """Launch budget (groups_u, groups_p) covering ``margin`` times the
+        most Krylov iterations any resident case needed in ``res`` (the
+        return value of ``step(info=True)``)."""
+        cu = self.check_every_u or self.check_every
+        cp = self.check_every
+        act = [k for k, sl in enumerate(self.slots) if sl is not None]
+        mu = max([1] + [res[k][0] for k in act])
+        mp = max([1] + [res[k][1] for k in act])
+        return (max(1, math.ceil(margin * mu / cu)), max(1, math.ceil(margin * mp / cp)))
+
     def poll(self):
-        """Synchronise and read the active flags and iteration counts; cases
-        past their iteration cap are deactivated here.  Returns the slots
-        that are resident and no longer active."""
+        """Synchronise and read the active flags (one small copy); cases past
+        their iteration cap are deactivated here.  Returns the slots that
+        are resident and no longer active.  The device stops cases by
+        itself, so polling less often only delays this bookkeeping."""
         act = self.active.cpu().numpy()
-        its = self.outer_iters.cpu().numpy()
-        self.h_iters[:] = its
         capped = []
         for k, sl in enumerate(self.slots):
-            if sl is not None and act[k] and its[k] >= sl['spec'].max_outer:
+            # an active case has run exactly h_steps[k] iterations
+            if sl is not None and act[k] and self.h_steps[k] >= sl['spec'].max_outer:
                 capped.append(k)
         for k in capped:
             self.active[k] = 0
             act[k] = 0
+        finished = bool(np.any((self.h_active != 0) & (act == 0)))
         self.h_active[:] = act
+        if finished:
+            self._build_descriptors()      # drop the finished cases' workgroups
         return [k for k, sl in enumerate(self.slots) if sl is not None and not act[k]]
 
     def continuity(self, k):
@@ -470,12 +526,42 @@ class CaseArena:
         of slot ``k`` after its last iteration."""
         return tuple(float(v) for v in self.monitor[k, 1:6].cpu().numpy())
 
-    def run(self, max_steps=100000, poll_every=1):
+    def run(self, max_steps=100000, poll_every=1, budget=None, graph=False, recheck=100,
+            margin=1.5):
         """Iterate until no resident case is active; returns the number of
-        batch iterations."""
+        batch iterations.
+
+        ``budget`` = (groups_u, groups_p): every iteration after the first
+        replays a HIP graph captured with that fixed Krylov launch budget.
+        ``graph=True``: the budget follows the solves -- every ``recheck``
+        iterations one polled iteration measures the Krylov iterations the
+        cases need, and the graph is captured again when ``margin`` times
+        that no longer fits the budget or needs less than half of it.  A
+        solve that needs more than the budget stops at it (like maxiter)."""
         n = 0
+        cur = None
         while n < max_steps:
-            self.step()
+            if budget is not None:
+                if n == 0:
+                    self.step(budget=budget)
+                    self.capture(budget)
+                elif self._graph['blocks'] != self.n_blocks:
+                    self.capture(budget)           # workgroups changed: capture again
+                    self.replay()
+                else:
+                    self.replay()
+            elif graph:
+                stale = cur is not None and self._graph['blocks'] != self.n_blocks
+                if cur is None or n % recheck == 0 or stale:
+                    need = self.budget_for(self.step(info=True), margin)
+                    if (stale or cur is None or need[0] > cur[0] or need[1] > cur[1]
+                            or 2 * need[0] < cur[0] or 2 * need[1] < cur[1]):
+                        cur = need
+                        self.capture(cur)
+                else:
+                    self.replay()
+            else:
+                self.step()
             n += 1
             if n % poll_every == 0:
                 self.poll()

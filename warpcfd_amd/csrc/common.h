@@ -262,6 +262,36 @@ __device__ inline double fold(const double* __restrict__ part, int nb) {
   return res;   // a following fold writes res only after its own barrier
 }
 
+// ND folds in one pass: out[i] = sum of p[i][0 .. nb) (the order of fold());
+// one pair of barriers for all of them.
+template <int ND>
+__device__ inline void fold_n(const double* const (&p)[ND], int nb, double (&out)[ND]) {
+  __shared__ double lds[ND * kMaxWaves];
+  __shared__ double res[ND];
+  double s[ND];
+#pragma unroll
+  for (int i = 0; i < ND; ++i) s[i] = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+#pragma unroll
+    for (int i = 0; i < ND; ++i) s[i] += p[i][b];
+  }
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int i = 0; i < ND; ++i) {
+    s[i] = wave_sum(s[i]);
+    if (lane == 0) lds[i * kMaxWaves + wid] = s[i];
+  }
+  __syncthreads();
+  if (threadIdx.x < ND) {
+    double t = 0.0;
+    for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += lds[threadIdx.x * kMaxWaves + w];
+    res[threadIdx.x] = t;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < ND; ++i) out[i] = res[i];
+}
+
 // Block sum of ND values; thread 0 stores them as partials part[i*G + vb].
 template <int ND>
 __device__ inline void store_partials(double (&v)[ND], double* __restrict__ part, int G, int vb) {

@@ -41,10 +41,13 @@ int launch_correct_velocity(const wcfd_mesh& m, const BatchDev& bd, const Seg& s
 // Krylov solvers (krylov.hip).  Stopping threshold
 // max(tol ||b||, rtol0 ||r_0||, atol); the reference's call has tol = atol
 // and rtol0 = 0.  refine > 0 (CG only): that many extra warm-started sweeps,
-// every sweep starting from a float64 residual.
+// every sweep starting from a float64 residual.  fixed_groups > 0: launch
+// exactly that many check groups without polling (stream capture), maxiter
+// clipped to fixed_groups * check_every.
 struct Tol {
   double atol, tol, rtol0;
   int refine;
+  int fixed_groups;
 };
 
 struct CsrView {

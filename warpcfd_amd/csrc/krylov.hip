@@ -153,7 +153,7 @@ __global__ void k_check(Seg sg, const double* __restrict__ part, KState* st, int
       }
     }
   }
-  if (threadIdx.x == 0) {
+  if (out && threadIdx.x == 0) {
     out[k] = S->f;
     __threadfence_system();   // visible to the host once the kernel has completed
   }
@@ -386,7 +386,10 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int i
     alpha_d = S->alpha[0];
   } else {
     const int pv = (it - 1) & 1;
-    const double g = fold(P_in + br.b0, br.nb), d = fold(P_in + G + br.b0, br.nb);
+    const double* pp[2] = {P_in + br.b0, P_in + G + br.b0};
+    double gd[2];
+    fold_n<2>(pp, br.nb, gd);
+    const double g = gd[0], d = gd[1];
     beta_d = safe_div(g, S->rz[pv]);
     alpha_d = safe_div(g, d - beta_d * safe_div(g, S->alpha[pv]));
     if (br.vb == br.b0 && threadIdx.x == 0) {
@@ -688,8 +691,10 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_3(Seg sg, int it, float*
       r0v[q] = r0[k];
     }
   }
-  const double ts = fold(PC + br.b0, br.nb), tt = fold(PC + G + br.b0, br.nb);
-  const double ss = fold(PC + 2 * G + br.b0, br.nb);
+  const double* pp[3] = {PC + br.b0, PC + G + br.b0, PC + 2 * G + br.b0};
+  double tv3[3];
+  fold_n<3>(pp, br.nb, tv3);
+  const double ts = tv3[0], tt = tv3[1], ss = tv3[2];
   const double omega_d = ss > S->f.thr2 ? safe_div(ts, tt) : 0.0;
   if (br.vb == br.b0 && threadIdx.x == 0) {
     S->omega[it & 1] = omega_d;
@@ -849,8 +854,21 @@ static int finish(const KFlags* f, int n, wcfd_solve_info* info) {
 // solver's check interval).
 template <class LaunchGroup, class RrPart>
 static int run_groups(const Seg& sg, KState* st, RrPart&& rr_part, int maxiter, int ce,
-                      LaunchGroup&& launch_group, wcfd_solve_info* info, hipStream_t s) {
+                      int fixed_groups, LaunchGroup&& launch_group, wcfd_solve_info* info,
+                      hipStream_t s) {
   const int K = sg.n_cases;
+  if (fixed_groups > 0) {
+    // Fixed launch budget (stream capture): exactly fixed_groups groups, no
+    // event, no host read; the check kernels still stop every case at the
+    // same iteration as the polled loop, and maxiter was clipped to the
+    // budget by the caller.  info is left untouched.
+    for (int g = 0, it = 0; g < fixed_groups; ++g) {
+      launch_group(it);
+      it += ce;
+      WCFD_LAUNCH(K_FOLD, s, k_check, K, sg, rr_part(it), st, it, maxiter, (KFlags*)nullptr);
+    }
+    return check_launch();
+  }
   HostSync* hs = host_sync(K);
   if (!hs) return WCFD_EHIP;
   int e;
@@ -907,8 +925,9 @@ static int prepare(KState* st, const Seg& sg, const Tol& tl, hipStream_t s) {
 }
 
 template <class M>
-static int cg_impl(const M& A, const KLaunch& L, const float* b, float* x, const Tol& tl, bool r64,
-                   int maxiter, int check_every, KWork& w, wcfd_solve_info* info, hipStream_t s) {
+static int cg_impl(const M& A, const KLaunch& L, const float* b, float* x, const Tol& tl,
+                   bool r64, int maxiter, int check_every, KWork& w, wcfd_solve_info* info,
+                   hipStream_t s) {
   const Seg& sg = L.sg;
   const int n = sg.n, B = L.B, G = sg.G, K = sg.n_cases;
   float* q = w.s;
@@ -933,11 +952,12 @@ static int cg_impl(const M& A, const KLaunch& L, const float* b, float* x, const
     }
   };
   auto rr_part = [&](int) { return (const double*)(w.PA + G); };
-  return run_groups(sg, w.st, rr_part, maxiter, ce, group, info, s);
+  return run_groups(sg, w.st, rr_part, maxiter, ce, tl.fixed_groups, group, info, s);
 }
 
 template <class M>
-static int cgs_impl(const M& A, const KLaunch& L, const float* b, float* x, const Tol& tl, bool r64, int maxiter, int check_every, KWork& w, wcfd_solve_info* info,
+static int cgs_impl(const M& A, const KLaunch& L, const float* b, float* x, const Tol& tl,
+                    bool r64, int maxiter, int check_every, KWork& w, wcfd_solve_info* info,
                     hipStream_t s) {
   const Seg& sg = L.sg;
   const int n = sg.n, B = L.B, G = sg.G, K = sg.n_cases;
@@ -965,11 +985,12 @@ static int cgs_impl(const M& A, const KLaunch& L, const float* b, float* x, cons
   };
   // the r.r partials of the last iteration (it_end - 1) are in pb[it_end & 1], slot 2
   auto rr_part = [&](int it_end) { return (const double*)(pb[it_end & 1] + 2 * G); };
-  return run_groups(sg, w.st, rr_part, maxiter, ce, group, info, s);
+  return run_groups(sg, w.st, rr_part, maxiter, ce, tl.fixed_groups, group, info, s);
 }
 
 template <class M, int NV>
-static int bicg_impl(const M& A, const KLaunch& L, const float* b, float* x, const Tol& tl, int maxiter, int check_every, KWork& w, wcfd_solve_info* info,
+static int bicg_impl(const M& A, const KLaunch& L, const float* b, float* x, const Tol& tl,
+                     int maxiter, int check_every, KWork& w, wcfd_solve_info* info,
                      hipStream_t s) {
   const Seg& sg = L.sg;
   const int n = sg.n, B = L.B, G = sg.G, K = sg.n_cases;
@@ -994,7 +1015,14 @@ static int bicg_impl(const M& A, const KLaunch& L, const float* b, float* x, con
     }
   };
   auto rr_part = [&](int) { return (const double*)(w.PA + G); };
-  return run_groups(sg, w.st, rr_part, maxiter, ce, group, info, s);
+  return run_groups(sg, w.st, rr_part, maxiter, ce, tl.fixed_groups, group, info, s);
+}
+
+// maxiter under a fixed launch budget: the budget's iterations at most
+static int budget_maxiter(const Tol& tl, int maxiter, int check_every) {
+  if (tl.fixed_groups <= 0) return maxiter;
+  const long cap = (long)tl.fixed_groups * (check_every > 0 ? check_every : 1);
+  return maxiter < cap ? maxiter : (int)cap;
 }
 
 template <class M>
@@ -1002,6 +1030,7 @@ static int cg_dispatch(const M& A, const KLaunch& L, int variant, const float* b
                        const Tol& tl, int maxiter, int check_every, KWork& w,
                        wcfd_solve_info* info, hipStream_t s) {
   if (variant != WCFD_CG_SINGLE_REDUCTION && variant != WCFD_CG_STANDARD) return WCFD_EINVAL;
+  maxiter = budget_maxiter(tl, maxiter, check_every);
   // Mixed-precision iterative refinement: with tl.refine = n > 0 the solve
   // runs 1 + n warm-started sweeps, each from a residual b - A x formed in
   // float64; iteration counts add up over the sweeps.
@@ -1030,6 +1059,7 @@ template <class M>
 static int bicg_dispatch(const M& A, const KLaunch& L, int nvec, const float* b, float* x,
                          const Tol& tl, int maxiter, int check_every, KWork& w,
                          wcfd_solve_info* info, hipStream_t s) {
+  maxiter = budget_maxiter(tl, maxiter, check_every);
   if (nvec == 1) return bicg_impl<M, 1>(A, L, b, x, tl, maxiter, check_every, w, info, s);
   if (nvec == 3) return bicg_impl<M, 3>(A, L, b, x, tl, maxiter, check_every, w, info, s);
   return WCFD_EINVAL;
@@ -1181,7 +1211,7 @@ int wcfd_cg(int32_t n, const int32_t* row_ptr, const int32_t* col, const float* 
   if (n <= 0 || !row_ptr || !col || !values || !diag_pos || !b || !x || !workspace)
     return WCFD_EINVAL;
   CsrView A{n, row_ptr, col, values, diag_pos};
-  return solve_cg(A, variant, b, x, Tol{atol, tol, 0.0, 0}, maxiter, check_every, workspace, info,
+  return solve_cg(A, variant, b, x, Tol{atol, tol, 0.0, 0, 0}, maxiter, check_every, workspace, info,
                   (hipStream_t)stream);
 }
 
@@ -1192,7 +1222,7 @@ int wcfd_bicgstab(int32_t n, const int32_t* row_ptr, const int32_t* col, const f
   if (n <= 0 || !row_ptr || !col || !values || !diag_pos || !b || !x || !workspace)
     return WCFD_EINVAL;
   CsrView A{n, row_ptr, col, values, diag_pos};
-  return solve_bicgstab(A, nvec, b, x, Tol{atol, tol, 0.0, 0}, maxiter, check_every, workspace, info,
+  return solve_bicgstab(A, nvec, b, x, Tol{atol, tol, 0.0, 0, 0}, maxiter, check_every, workspace, info,
                         (hipStream_t)stream);
 }
 
@@ -1201,7 +1231,7 @@ int wcfd_cg_ell(const wcfd_mesh* m, const float* values, int variant, const floa
                 wcfd_solve_info* info, void* stream) {
   if (!m || m->n_cells <= 0 || !m->slot_nbr || !values || !b || !x || !workspace)
     return WCFD_EINVAL;
-  return solve_cg_ell(*m, nullptr, values, variant, b, x, Tol{atol, tol, 0.0, 0}, maxiter, check_every,
+  return solve_cg_ell(*m, nullptr, values, variant, b, x, Tol{atol, tol, 0.0, 0, 0}, maxiter, check_every,
                       workspace, info, (hipStream_t)stream);
 }
 
@@ -1210,7 +1240,7 @@ int wcfd_bicgstab_ell(const wcfd_mesh* m, const float* values, int nvec, const f
                       wcfd_solve_info* info, void* stream) {
   if (!m || m->n_cells <= 0 || !m->slot_nbr || !values || !b || !x || !workspace)
     return WCFD_EINVAL;
-  return solve_bicgstab_ell(*m, nullptr, values, nvec, b, x, Tol{atol, tol, 0.0, 0}, maxiter, check_every,
+  return solve_bicgstab_ell(*m, nullptr, values, nvec, b, x, Tol{atol, tol, 0.0, 0, 0}, maxiter, check_every,
                             workspace, info, (hipStream_t)stream);
 }
 
@@ -1218,7 +1248,7 @@ int wcfd_batch_cg_ell(const wcfd_mesh* m, const wcfd_batch* bt, const float* val
                       const float* b, float* x, double atol, double tol, int maxiter,
                       int check_every, void* workspace, wcfd_solve_info* info, void* stream) {
   if (!batch_ok(m, bt) || !m->slot_nbr || !values || !b || !x || !workspace) return WCFD_EINVAL;
-  return solve_cg_ell(*m, bt, values, variant, b, x, Tol{atol, tol, 0.0, 0}, maxiter, check_every, workspace,
+  return solve_cg_ell(*m, bt, values, variant, b, x, Tol{atol, tol, 0.0, 0, 0}, maxiter, check_every, workspace,
                       info, (hipStream_t)stream);
 }
 
@@ -1227,7 +1257,7 @@ int wcfd_batch_bicgstab_ell(const wcfd_mesh* m, const wcfd_batch* bt, const floa
                             int maxiter, int check_every, void* workspace, wcfd_solve_info* info,
                             void* stream) {
   if (!batch_ok(m, bt) || !m->slot_nbr || !values || !b || !x || !workspace) return WCFD_EINVAL;
-  return solve_bicgstab_ell(*m, bt, values, nvec, b, x, Tol{atol, tol, 0.0, 0}, maxiter, check_every,
+  return solve_bicgstab_ell(*m, bt, values, nvec, b, x, Tol{atol, tol, 0.0, 0, 0}, maxiter, check_every,
                             workspace, info, (hipStream_t)stream);
 }
 

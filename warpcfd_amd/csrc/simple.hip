@@ -71,10 +71,11 @@ static int simple_iteration(const wcfd_mesh& m, const wcfd_batch* bt, wcfd_bc bc
                                     st->cell_grad, 1, prm->nu, 1.0f / prm->density, prm->alpha_u,
                                     prm->dt, prm->scheme_flags, st->mom_values, st->mom_rhs, s)))
     return e;
-  const Tol tl{prm->atol, prm->tol, prm->rtol0, 0};
-  const Tol tp{prm->atol, prm->tol, prm->rtol0, prm->p_refine};
+  const Tol tl{prm->atol, prm->tol, prm->rtol0, 0, prm->graph_groups_u};
+  const Tol tp{prm->atol, prm->tol, prm->rtol0, prm->p_refine, prm->graph_groups_p};
   const int ce_u = prm->check_every_u > 0 ? prm->check_every_u : prm->check_every;
-  e = solve_bicgstab_ell(m, bt, st->mom_values, 3, st->mom_rhs, st->hbya, tl, prm->maxiter, ce_u,
+  const int mi_u = prm->maxiter_u > 0 ? prm->maxiter_u : prm->maxiter;
+  e = solve_bicgstab_ell(m, bt, st->mom_values, 3, st->mom_rhs, st->hbya, tl, mi_u, ce_u,
                          w.krylov, mom.data(), s);
   if (e) return e;
   if ((e = launch_rua_hbya(m, bd, st->mom_values, prm->density, st->cell_grad, st->hbya,
@@ -132,6 +133,8 @@ static int simple_iteration(const wcfd_mesh& m, const wcfd_batch* bt, wcfd_bc bc
 static bool params_ok(const wcfd_simple_params* prm) {
   return prm && prm->density != 0.f && prm->check_every >= 0 && prm->check_every_u >= 0 &&
          prm->rtol0 >= 0.0 && prm->p_refine >= 0 && prm->p_refine <= 8 &&
+         prm->graph_groups_u >= 0 && prm->graph_groups_p >= 0 && prm->maxiter_u >= 0 &&
+         (prm->graph_groups_u > 0) == (prm->graph_groups_p > 0) &&
          (prm->cg_variant == WCFD_CG_STANDARD || prm->cg_variant == WCFD_CG_SINGLE_REDUCTION);
 }
 
