@@ -43,10 +43,10 @@ WORKLOADS = {
 }
 
 
-def build_case(workload):
+def build_case(workload, dims=(200, 40, 40)):
     from warpcfd_amd.preprocess import Mesh, define_boundary_walls, channel_mesh, cavity_mesh
     if workload == 'channel':
-        nodes, cells = channel_mesh(200, 40, 40, (10.0, 2.0, 2.0), jitter=0.1, seed=0)
+        nodes, cells = channel_mesh(*dims, (10.0, 2.0, 2.0), jitter=0.1, seed=0)
     else:
         nodes, cells = cavity_mesh(64)
     return nodes, cells
@@ -63,6 +63,19 @@ def apply_bcs(b, kind):
             b.no_slip_wall(w)
         b.velocity_BC('+Y', u=1.0, v=0.0, w=0.0)
         b.slip_wall('+Z'); b.slip_wall('-Z')
+
+
+def calibration_copy(mb):
+    """One elementwise kernel of known traffic (reads and writes ``mb`` MiB)
+    for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings."""
+    import torch
+    n = int(mb * (1 << 20) // 4)
+    x = torch.ones(n, dtype=torch.float32, device='cuda')
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    del x, y
 
 
 def algorithmic_bytes(kernel, C, F, Fi, Fb, nnz):
@@ -155,6 +168,13 @@ def bench_batch64(args):
     print(f'# batch: {len(specs)} cases, {cells} cells, host preprocessing {prep_s:.1f} s',
           file=sys.stderr, flush=True)
     kw = dict(chunk=args.chunk, block=args.block, **pol)
+    tuned = None
+    if not args.no_tune:
+        # the arena's launch shape and CG form, timed on the wide phase of this batch
+        from warpcfd_amd.tune import autotune_arena
+        tuned = autotune_arena(specs, dict(kw))
+        kw.update(cg_variant=tuned['cg_variant'], chunk=tuned['chunk'], block=tuned['block'])
+        print(f'# autotuned arena: {kw}', file=sys.stderr, flush=True)
     # warm-up: load every kernel once
     warm = CaseArena.for_cases(specs[:2], **kw)
     for _ in range(3):
@@ -235,7 +255,7 @@ def bench_batch64(args):
             model.finalize()
             sv = IncompressibleSolver(model, 0.7, 0.3)
             sv.tol, sv.rtol, sv.rtol0 = pol['atol'], pol['tol'], pol['rtol0']
-            sv.max_iter, sv.check_every, sv.cg_variant = pol['maxiter'], 10, pol['cg_variant']
+            sv.max_iter, sv.check_every, sv.cg_variant = pol['maxiter'], 10, kw['cg_variant']
             sv.check_every_u = pol['check_every_u']
             sv.max_iter_u = pol['maxiter_u']
             torch.cuda.synchronize()
@@ -280,15 +300,16 @@ def bench_batch64(args):
                                f'[32,256], Re uniform in [100,1000] (seed {args.seed}), steady '
                                f'SIMPLE (alpha_u 0.7, alpha_p 0.3) to continuity <= {stop:g} and '
                                f'estimated velocity error <= {stop_u:g}', 'cases': len(specs),
-                   'cells': cells, 'chunk': args.chunk, 'block': args.block,
+                   'cells': cells, 'chunk': kw['chunk'], 'block': kw['block'],
                    'krylov': {'atol': 0.0, 'tol': 0.0, 'rtol0': pol['rtol0'],
                               'maxiter': pol['maxiter'], 'check_every': 10,
                               'check_every_momentum': pol['check_every_u'],
                               'maxiter_momentum': pol['maxiter_u'] or pol['maxiter'],
-                              'pressure_cg': 'single'},
+                              'pressure_cg': 'single' if kw['cg_variant'] else 'standard'},
                    'stop': {'continuity': stop, 'velocity_error_estimate': stop_u,
                             'velocity_change_floor': floor_u},
                    'reductions': 'f64'},
+        'arena_tuning': tuned,
         'batched': {'total_s': round(batched_s, 3), 'batch_iterations': steps,
                     'hip_graph': ({'captures': n_capture, 'last_budget': cur, 'recheck_every': 100}
                                   if args.graph else None),
@@ -611,6 +632,8 @@ def main():
     ap.add_argument('--workload', default='channel',
                     choices=sorted(WORKLOADS) + ['batch64', 'serve', 'cylinder'])
     ap.add_argument('--cyl-h', type=float, default=0.05, help='cylinder: cell size')
+    ap.add_argument('--channel-dims', default='200x40x40',
+                    help='channel: hexahedra along x, y, z (the 10x2x2 box)')
     ap.add_argument('--p-refine', type=int, default=1,
                     help='cylinder: float64 pressure refinement sweeps')
     ap.add_argument('--seed', type=int, default=0)
@@ -629,6 +652,13 @@ def main():
     ap.add_argument('--check-u', type=int, default=2,
                     help='batch: check interval of the momentum solve')
     ap.add_argument('--no-sequential', action='store_true')
+    ap.add_argument('--calibration-copy-mb', type=float, default=0.0,
+                    help='run one elementwise copy of this size (rocprofv3 counter calibration)')
+    ap.add_argument('--traffic-from', default=None,
+                    help='directory with fetch/ and write/ rocprofv3 --pmc runs of this '
+                         'workload: fills roofline.traffic')
+    ap.add_argument('--traffic-copy-mb', type=float, default=512.0,
+                    help='size of the calibration copy in those runs')
     ap.add_argument('--graph', action='store_true',
                     help='batch: replay a HIP-graph-captured iteration (Krylov launch budget '
                          'measured every 100 iterations)')
@@ -667,8 +697,13 @@ def main():
     from warpcfd_amd.FV import FVM
     from warpcfd_amd.FV.implicit_Solvers import IncompressibleSolver
 
-    wl = WORKLOADS[args.workload]
-    nodes, cells = build_case(args.workload)
+    wl = dict(WORKLOADS[args.workload])
+    dims = tuple(int(v) for v in args.channel_dims.lower().split('x'))
+    if args.workload == 'channel' and dims != (200, 40, 40):
+        C = dims[0] * dims[1] * dims[2]
+        wl['desc'] = wl['desc'].replace('200x40x40 hex (320000 cells)', f'{args.channel_dims} hex '
+                                        f'({C} cells)')
+    nodes, cells = build_case(args.workload, dims)
     m = Mesh(nodes, cells)
     define_boundary_walls(m)
     model = FVM(m)
@@ -723,6 +758,17 @@ def main():
                         'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(ach / HBM_PEAK_GBS, 4),
                         'traffic': None, 'bytes_per_launch': byts,
                         'avg_launch_us': round(avg_s * 1e6, 3)}
+
+    if args.calibration_copy_mb:
+        calibration_copy(args.calibration_copy_mb)
+    if roofline is not None and args.traffic_from:
+        from warpcfd_amd.pmc import traffic
+        sym = {'cg_1': 'k_cgs_iter' if args.cg == 'single' else 'k_cg_1', 'cg_2': 'k_cg_2',
+               'bicgstab_1': 'k_bi_1', 'bicgstab_2': 'k_bi_2', 'bicgstab_3': 'k_bi_3'}
+        t = traffic(args.traffic_from, sym.get(roofline['kernel'], roofline['kernel']),
+                    args.traffic_copy_mb)
+        roofline['traffic'] = t['bytes_per_launch']
+        roofline['traffic_detail'] = t
 
     cpu = None
     if not args.no_cpu_baseline:

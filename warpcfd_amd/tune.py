@@ -91,3 +91,40 @@ def autotune_krylov(dm, grids=GRIDS, blocks=BLOCKS, iters=60, cg_weight=1.0, bic
     out = dict(best, iters=iters, candidates=results)
     _cache[key] = out
     return out
+
+
+ARENA_CANDIDATES = (
+    # (pressure CG form, rows per workgroup, threads per workgroup)
+    (1, 256, 256), (1, 512, 512), (1, 128, 128), (1, 512, 256),
+    (0, 256, 256), (0, 512, 512),
+)
+
+
+def autotune_arena(specs, arena_kw, candidates=ARENA_CANDIDATES, warm=2, steps=4):
+    """Pick the launch shape of a batch arena: for every candidate
+    (pressure CG variant, chunk, block) build the arena for ``specs``, run
+    ``warm`` untimed and ``steps`` timed SIMPLE iterations (the wide phase,
+    every case active) and return the fastest as a dict of arena keywords
+    plus the timings.  The shape is fixed for the whole run afterwards, so
+    results stay independent of the mix."""
+    import time
+    import torch
+    from .batch import CaseArena
+    results = []
+    for variant, chunk, block in candidates:
+        kw = dict(arena_kw, cg_variant=variant, chunk=chunk, block=block)
+        a = CaseArena.for_cases(specs, **kw)
+        for _ in range(warm):
+            a.step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            a.step()
+        torch.cuda.synchronize()
+        results.append({'cg_variant': variant, 'chunk': chunk, 'block': block,
+                        'ms_per_iteration': 1e3 * (time.perf_counter() - t0) / steps})
+        del a
+        torch.cuda.empty_cache()
+    best = min(results, key=lambda r: r['ms_per_iteration'])
+    return {'cg_variant': best['cg_variant'], 'chunk': best['chunk'], 'block': best['block'],
+            'candidates': results}
