@@ -473,7 +473,7 @@ def bench_cylinder(args):
     if not args.no_tune:
         from warpcfd_amd.tune import autotune_krylov
         tt = autotune_krylov(dm, cg_variant=sv.cg_variant)
-        tuned = {'grid': tt['grid'], 'block': tt['block']}
+        tuned = {'grid': tt['grid'], 'block': tt['block'], 'fold': tt['fold']}
     C, F, Fi, Fb, nnz = dm.n_cells, dm.n_faces, dm.n_internal, dm.n_boundary, dm.nnz
     for _ in range(args.warmup):
         sv.outer_iteration(dt=dt)
@@ -676,7 +676,8 @@ def main():
                     help='pressure CG form: standard (two reductions per iteration) or single '
                          '(Chronopoulos-Gear, one fused kernel per iteration)')
     ap.add_argument('--krylov-launch', default=None,
-                    help='GRIDxBLOCK: fixed Krylov launch shape instead of autotuning')
+                    help='GRIDxBLOCK[xFOLD]: fixed Krylov launch shape (grid 0: one row per '
+                         'thread; FOLD: inline-fold limit, default 2048) instead of autotuning')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py runs on one GPU only: this build has no multi-GPU path '
@@ -717,13 +718,17 @@ def main():
     dm = model.dm
     tuned = None
     if args.krylov_launch:
-        g, b = (int(v) for v in args.krylov_launch.lower().split('x'))
-        _lib.check(_lib.load().wcfd_set_krylov_launch(g, b), 'set_krylov_launch')
-        tuned = {'grid': g, 'block': b, 'source': 'command line'}
+        parts = [int(v) for v in args.krylov_launch.lower().split('x')]
+        g, b = parts[:2]
+        fold = parts[2] if len(parts) > 2 else 2048
+        from warpcfd_amd.tune import install
+        install(g, b, fold)
+        tuned = {'grid': g, 'block': b, 'fold': fold, 'source': 'command line'}
     elif not args.no_tune:
         from warpcfd_amd.tune import autotune_krylov
         t = autotune_krylov(dm, cg_variant=solver.cg_variant)
-        tuned = {'grid': t['grid'], 'block': t['block'], 'cg_ms_per_60_it': round(t['cg_ms'], 3),
+        tuned = {'grid': t['grid'], 'block': t['block'], 'fold': t['fold'],
+                 'cg_ms_per_60_it': round(t['cg_ms'], 3),
                  'candidates': len(t['candidates'])}
     C, F, Fi, Fb, nnz = dm.n_cells, dm.n_faces, dm.n_internal, dm.n_boundary, dm.nnz
 

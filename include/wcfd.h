@@ -118,6 +118,7 @@ typedef struct wcfd_batch {
                                stop_c = 0: never stop */
   double* monitor;          /* [K][WCFD_MONITOR_LEN] monitor records, zeroed at the start */
   int32_t* outer_iters;     /* [K] SIMPLE iterations run per case */
+  int32_t max_case_blocks;  /* largest blk_off[k+1] - blk_off[k]; 0: taken as n_blocks */
 } wcfd_batch;
 
 /* Convergence monitor of a steady run (no counterpart in the reference,
@@ -279,12 +280,20 @@ int wcfd_bicgstab(int32_t n, const int32_t* row_ptr, const int32_t* col, const f
                   wcfd_solve_info* info, void* stream);
 
 /* Launch shape of the Krylov kernels: `grid` workgroups (grid-stride,
- * 1..2048; every workgroup of a kernel folds the previous kernel's `grid`
- * partials) of `block` threads (64..512, multiple of 64).  Process-wide;
+ * 1..65536; 0 = one row per thread, i.e. ceil(n / block) workgroups) of
+ * `block` threads (64..512, multiple of 64).  Every workgroup writes one
+ * float64 partial per dot product.  When a case has at most `max_inline`
+ * workgroups (wcfd_set_krylov_fold; default 2048), every workgroup of the
+ * next kernel folds all of them in its prologue; above that, a fold kernel
+ * (one workgroup per case) reduces them to one value first, so the fold
+ * work grows linearly rather than quadratically with the grid.  Both forms
+ * sum in the same order and give bitwise the same results.  Process-wide;
  * chosen by the autotuner (warpcfd_amd/tune.py).  No counterpart in the
  * reference (Warp picks its own launch shapes). */
 int wcfd_set_krylov_launch(int grid, int block);
 int wcfd_get_krylov_launch(int* grid, int* block); /* [host] outputs */
+int wcfd_set_krylov_fold(int max_inline);           /* max_inline >= 0 */
+int wcfd_get_krylov_fold(void);
 
 /* wcfd_cg / wcfd_bicgstab on the ELL matrix `values` [7][C] of mesh m. */
 int wcfd_cg_ell(const wcfd_mesh* m, const float* values, int variant, const float* b, float* x,

@@ -19,7 +19,7 @@ def test_library_exports_every_header_symbol():
     missing = [s for s in syms if not hasattr(lib, s)]
     assert not missing
     assert set(_lib._SIGS) == set(syms)
-    assert lib.wcfd_abi_version() == 2
+    assert lib.wcfd_abi_version() == 3
     assert lib.wcfd_error_string(_lib.WCFD_EINVAL) == b'invalid argument'
     assert lib.wcfd_kernel_count() > 10
     assert lib.wcfd_krylov_workspace_bytes(1000, 3) > 8 * 3 * 1000 * 4

@@ -50,7 +50,8 @@ class WcfdBatch(ctypes.Structure):
     _fields_ = [('n_cases', c_i32), ('n_blocks', c_i32), ('chunk', c_i32), ('block', c_i32),
                 ('cell_off', c_p), ('cell_end', c_p), ('blk_off', c_p), ('blk_case', c_p), ('cell_case', c_p),
                 ('face_case', c_p), ('active', c_p), ('nu', c_p), ('ref_cell', c_p),
-                ('ref_value', c_p), ('stop', c_p), ('monitor', c_p), ('outer_iters', c_p)]
+                ('ref_value', c_p), ('stop', c_p), ('monitor', c_p), ('outer_iters', c_p),
+                ('max_case_blocks', c_i32)]
 
 
 class SimpleParams(ctypes.Structure):
@@ -110,6 +111,8 @@ _SIGS = {
                                      c_p]),
     'wcfd_set_krylov_launch': (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     'wcfd_get_krylov_launch': (ctypes.c_int, [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+    'wcfd_set_krylov_fold': (ctypes.c_int, [ctypes.c_int]),
+    'wcfd_get_krylov_fold': (ctypes.c_int, []),
     'wcfd_simple_workspace_bytes': (ctypes.c_size_t, [c_i32, c_i32]),
     'wcfd_kernel_count': (ctypes.c_int, []),
     'wcfd_kernel_name': (ctypes.c_char_p, [ctypes.c_int]),

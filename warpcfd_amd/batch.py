@@ -214,6 +214,7 @@ class CaseArena:
         self.h_active = np.zeros(K, np.uint8)
         self.h_steps = np.zeros(K, np.int64)   # iterations issued while the host saw the case active
         self.n_blocks = 0
+        self.max_case_blocks = 0
         # settings
         self.density = float(density)
         self.alpha_u, self.alpha_p = alpha_u, alpha_p
@@ -382,6 +383,7 @@ class CaseArena:
         G = int(blk_off[-1])
         blk_case = np.repeat(np.arange(K, dtype=np.int32), nb)
         self.n_blocks = G
+        self.max_case_blocks = int(nb.max()) if K else 0
         cell_off = self.h_cell_off.copy()
         cell_off[K] = self.Cc
         dev = self.device
@@ -400,6 +402,7 @@ class CaseArena:
         self.desc = m
         b = _lib.WcfdBatch()
         b.n_cases, b.n_blocks, b.chunk, b.block = K, G, self.chunk, self.block
+        b.max_case_blocks = self.max_case_blocks
         for name in ('cell_off', 'cell_end', 'blk_off', 'blk_case', 'cell_case', 'face_case',
                      'active', 'nu', 'ref_cell', 'ref_value', 'stop', 'monitor', 'outer_iters'):
             setattr(b, name, getattr(self, name).data_ptr())

@@ -13,6 +13,8 @@ constexpr int kWave = 64;             // CDNA wavefront
 constexpr int kMaxWaves = 16;         // LDS sizing: workgroups of up to 1024 threads
 constexpr int kMaxReduceBlocks = 2048;  // upper bound on partials of one reduction
 constexpr int kKrylovGrid = 1024;       // workgroups of the Krylov kernels (grid-stride)
+constexpr int kMaxKrylovBlocks = 65536; // upper bound on the Krylov kernels' workgroups
+constexpr int kMaxInlineFold = 2048;    // default: largest per-case fold done inline
 // Largest workgroup of the row-parallel Krylov / reduction kernels: their
 // __launch_bounds__, which lets a wave hold up to 256 VGPRs (a whole row of
 // gathers in flight) without spilling.
@@ -108,6 +110,38 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
+// ------------------------------------------------ uniform base + offset
+// base (uniform across the wave: a kernel argument plus uniform terms) plus
+// a 32-bit byte offset per lane.  The compiler forms such an address as a
+// scalar base plus one offset VGPR (global_load ... vOff, s[base]), so the
+// loads of several arrays at the same row share one offset register instead
+// of a 64-bit address pair each.  Offsets must stay below 2^32 bytes: the
+// ELL entry points reject meshes with 7 n floats beyond that.
+template <class T>
+__device__ __forceinline__ T ld_off(const T* base, uint32_t byte_off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+template <class T>
+__device__ __forceinline__ void st_off(T* base, uint32_t byte_off, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off) = v;
+}
+
+// Raw buffer view of a float array for gathers: the descriptor sits in
+// scalar registers and each load takes one 32-bit byte offset per lane
+// (buffer_load_dword vOff, s[desc]), so the gathers of several vectors at
+// the same neighbour share one offset register.  Offsets at or beyond
+// `bytes` read 0.  Word 3 is the gfx9 raw dword-buffer format.
+constexpr int kBufWord3 = 0x00020000;
+struct FBuf {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ FBuf(const float* p, uint32_t bytes)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), (short)0, (int)bytes,
+                                            kBufWord3)) {}
+  __device__ __forceinline__ float operator()(uint32_t off) const {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+  }
+};
+
 // ------------------------------------------------------ XCD-aware mapping
 // MI355X deals workgroups to its 8 XCDs round-robin (workgroup b runs on XCD
 // b % 8) and each XCD has its own L2.  xcd_remap() renumbers workgroups so
@@ -146,10 +180,15 @@ struct Seg {
   const int32_t* blk_off;   // [K+1]
   const int32_t* blk_case;  // [G]
   const uint8_t* active;    // [K]; NULL: every case active
+  // 1: the partials the Krylov iteration kernels write are reduced to one
+  // value per case (at the case's first slot) by a fold kernel before the
+  // next kernel reads them; 0: every reader folds all of its case's partials
+  int prefold;
 };
 
 struct BlockRows {
   int vb, k, begin, end, b0, nb;
+  int nf;   // partials a reader of iteration partials folds: nb, or 1 when prefolded
 };
 
 __device__ inline BlockRows block_rows(const Seg& s) {
@@ -169,6 +208,7 @@ __device__ inline BlockRows block_rows(const Seg& s) {
     const int e = s.cell_end ? s.cell_end[r.k] : s.cell_off[r.k + 1];
     r.end = r.begin + s.chunk < e ? r.begin + s.chunk : e;
   }
+  r.nf = s.prefold ? 1 : r.nb;
   return r;
 }
 
@@ -242,14 +282,50 @@ inline BatchDev batch_dev(const wcfd_batch* bt) {
   return d;
 }
 
+// Thread t's share of a fold: s = ((0 + p[t]) + p[t+B]) + p[t+2B] ...,
+// with the loads of U terms issued before their additions (one memory round
+// trip per U terms instead of per term); the order of the additions, and so
+// the sum, does not depend on U.
+template <int ND, int U>
+__device__ __forceinline__ void fold_thread(const double* const (&p)[ND], int nb,
+                                            double (&s)[ND]) {
+  const int step = blockDim.x;
+  int b = threadIdx.x;
+  for (; b + (U - 1) * step < nb; b += U * step) {
+    double v[U][ND];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int i = 0; i < ND; ++i) v[u][i] = p[i][b + u * step];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int i = 0; i < ND; ++i) s[i] += v[u][i];
+  }
+  for (; b < nb; b += step) {
+#pragma unroll
+    for (int i = 0; i < ND; ++i) s[i] += p[i][b];
+  }
+}
+
+// Loads in flight per thread in the folds inside iteration kernels, where
+// they add to the registers held by the prefetched row (build-time choice;
+// the prefold kernel uses 8).
+#ifndef WCFD_FOLD_UNROLL
+#define WCFD_FOLD_UNROLL 2
+#endif
+constexpr int kFoldUnroll = WCFD_FOLD_UNROLL;
+
 // Sum of nb partials (the same order in every workgroup); valid in every
 // thread.
+template <int U = kFoldUnroll>
 __device__ inline double fold(const double* __restrict__ part, int nb) {
   __shared__ double lds[kMaxWaves];
   __shared__ double res;
-  double s = 0.0;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[b];
-  s = wave_sum(s);
+  const double* const pp[1] = {part};
+  double sv[1] = {0.0};
+  fold_thread<1, U>(pp, nb, sv);
+  double s = wave_sum(sv[0]);
   const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
   if (lane == 0) lds[wid] = s;
   __syncthreads();
@@ -264,17 +340,14 @@ __device__ inline double fold(const double* __restrict__ part, int nb) {
 
 // ND folds in one pass: out[i] = sum of p[i][0 .. nb) (the order of fold());
 // one pair of barriers for all of them.
-template <int ND>
+template <int ND, int U = kFoldUnroll>
 __device__ inline void fold_n(const double* const (&p)[ND], int nb, double (&out)[ND]) {
   __shared__ double lds[ND * kMaxWaves];
   __shared__ double res[ND];
   double s[ND];
 #pragma unroll
   for (int i = 0; i < ND; ++i) s[i] = 0.0;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-#pragma unroll
-    for (int i = 0; i < ND; ++i) s[i] += p[i][b];
-  }
+  fold_thread<ND, U>(p, nb, s);
   const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
 #pragma unroll
   for (int i = 0; i < ND; ++i) {
@@ -354,6 +427,21 @@ struct EllMat {
     for (int k = 0; k < 6; ++k) {
       c[k + 1] = D16 ? i + (int)dnb[k * n + i] : nbr[k * n + i];
       a[k + 1] = val[(k + 1) * n + i];
+    }
+  }
+  // The same row with the columns as byte offsets of float vectors
+  // (o[j] = 4 c[j]); every load is a uniform base plus a 32-bit offset.
+  __device__ void load_off(int i, float (&a)[7], uint32_t (&o)[7]) const {
+    const uint32_t oi = (uint32_t)i * 4u;
+    a[0] = ld_off(val, oi);
+    o[0] = oi;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const size_t col0 = (size_t)k * (size_t)n;
+      const int cj = D16 ? i + (int)ld_off(dnb + col0, (uint32_t)i * 2u)
+                         : ld_off(nbr + col0, oi);
+      o[k + 1] = (uint32_t)cj * 4u;
+      a[k + 1] = ld_off(val + col0 + (size_t)n, oi);
     }
   }
   template <class F>

@@ -621,7 +621,7 @@ const char* wcfd_error_string(int code) {
   }
 }
 
-int wcfd_abi_version(void) { return 2; }
+int wcfd_abi_version(void) { return 3; }
 
 int wcfd_monitor_update(double* record, double continuity, double velocity_change,
                         double stop_c, double stop_u, double floor_u) {

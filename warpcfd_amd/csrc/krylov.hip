@@ -14,7 +14,12 @@
 // partials in its prologue (all workgroups of the case in the same order), so
 // no kernel waits on another workgroup -- a cross-workgroup handshake costs
 // an agent-scope release per workgroup, measured 5-6x slower on MI355X at
-// 320 k rows -- and every run is bitwise reproducible.  Iteration kernels:
+// 320 k rows -- and every run is bitwise reproducible.  Folding in every
+// workgroup costs G^2 reads per kernel, so above a set number of workgroups
+// per case (Seg::prefold) a fold kernel with one workgroup per case first
+// reduces the partials to one value, in the same order and workgroup size,
+// and the readers take that value: the sums are bitwise the same either way.
+// Iteration kernels:
 //   CG:         [p = z + beta p ; q = A p ; p.q]  [x, r update ; r.z, r.r]
 //   CG (1-red): [p, s recurrences, x, r updates, w = A M r ; r.u, w.u, r.r]
 //   BiCGSTAB:   [p = r + beta(p - omega v) ; v = A M p ; r0.v]
@@ -144,7 +149,7 @@ __global__ void k_check(Seg sg, const double* __restrict__ part, KState* st, int
   if (!S->f.done) {
     int b0, nb;
     case_blocks(sg, k, b0, nb);
-    const double rr = fold(part + b0, nb);
+    const double rr = fold(part + b0, sg.prefold ? 1 : nb);
     if (threadIdx.x == 0) {
       S->f.rr = rr;
       if (!(rr > S->f.thr2) || it_end >= maxiter) {
@@ -156,6 +161,30 @@ __global__ void k_check(Seg sg, const double* __restrict__ part, KState* st, int
   if (out && threadIdx.x == 0) {
     out[k] = S->f;
     __threadfence_system();   // visible to the host once the kernel has completed
+  }
+}
+
+// Prefold, one workgroup per case (launched with the iteration kernels'
+// workgroup size): the ND partial rows of a case are folded in the order of
+// fold() and the sums replace the case's first partial of each row.  Every
+// thread's reads finish before the first barrier inside fold_n, so thread
+// 0's in-place store cannot race them.  Done cases keep their partials.
+template <int ND>
+__global__ __launch_bounds__(kMaxKrylovBlock) void k_prefold(Seg sg, double* __restrict__ P,
+                                                            const KState* st) {
+  const int k = blockIdx.x;
+  if (st[k].f.done) return;
+  int b0, nb;
+  case_blocks(sg, k, b0, nb);
+  const int G = sg.G;
+  const double* pp[ND];
+#pragma unroll
+  for (int i = 0; i < ND; ++i) pp[i] = P + (size_t)i * G + b0;
+  double v[ND];
+  fold_n<ND, 8>(pp, nb, v);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < ND; ++i) P[(size_t)i * G + b0] = v[i];
   }
 }
 
@@ -195,7 +224,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_init(M A, Seg sg, const 
   store_partials<3>(acc, PA, sg.G, br.vb);
 }
 
-template <class M>
+template <class M, bool PF>
 __global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_1(M A, Seg sg, int it, const float* __restrict__ r,
                        const float* __restrict__ inv, const float* __restrict__ p_in,
                        float* __restrict__ p_out, float* __restrict__ q,
@@ -204,26 +233,28 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_1(M A, Seg sg, int it, c
   KState* S = st + br.k;
   if (S->f.done) return;
   // one row per thread on an ELL matrix: issue the row's loads before the fold
-  const bool pf = M::kEll && (br.end - br.begin <= (int)blockDim.x);
+  constexpr bool pf = PF && M::kEll;
   const int i0 = br.begin + (int)threadIdx.x;
   const bool has = pf && i0 < br.end;
   float a[7], rc[7], ic[7], pc[7];
   if constexpr (M::kEll) {
     if (has) {
-      int c[7];
-      A.load(i0, a, c);
+      uint32_t o[7];
+      A.load_off(i0, a, o);
+      const uint32_t nb4 = (uint32_t)A.n * 4u;
+      const FBuf R(r, nb4), I(inv, nb4), P(p_in, nb4);
 #pragma unroll
       for (int j = 0; j < 7; ++j) {
-        rc[j] = r[c[j]];
-        ic[j] = inv[c[j]];
-        pc[j] = it ? p_in[c[j]] : 0.f;
+        rc[j] = R(o[j]);
+        ic[j] = I(o[j]);
+        pc[j] = it ? P(o[j]) : 0.f;
       }
     }
   }
   bool first = (it == 0);
   double beta_d = 0.0;
   if (!first) {
-    const double rz = fold(PA + br.b0, br.nb);   // r.z of the previous iteration
+    const double rz = fold(PA + br.b0, br.nf);   // r.z of the previous iteration
     beta_d = safe_div(rz, S->rz[(it - 1) & 1]);
     if (br.vb == br.b0 && threadIdx.x == 0) S->rz[it & 1] = rz;
     first = (beta_d == 0.0);
@@ -240,8 +271,9 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_1(M A, Seg sg, int it, c
       }
       const float zi = rc[0] * ic[0];
       const float pi = first ? zi : zi + beta * pc[0];
-      p_out[i0] = pi;
-      q[i0] = qi;
+      const uint32_t o0 = (uint32_t)i0 * 4u;
+      st_off(p_out, o0, pi);
+      st_off(q, o0, qi);
       acc[0] += (double)pi * (double)qi;
     }
   } else {
@@ -262,6 +294,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_1(M A, Seg sg, int it, c
   store_partials<1>(acc, PB, sg.G, br.vb);
 }
 
+template <bool PF>
 __global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_2(Seg sg, int it, float* __restrict__ x, float* __restrict__ r,
                        const float* __restrict__ inv, const float* __restrict__ p,
                        const float* __restrict__ q, const double* __restrict__ PB,
@@ -270,7 +303,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_2(Seg sg, int it, float*
   KState* S = st + br.k;
   if (S->f.done) return;
   // one row per thread: load it before the fold
-  const bool pf = br.end - br.begin <= (int)blockDim.x;
+  constexpr bool pf = PF;
   const int i0 = br.begin + (int)threadIdx.x;
   const bool has = pf && i0 < br.end;
   float xv = 0.f, pv = 0.f, rv = 0.f, qv = 0.f, iv = 0.f;
@@ -281,7 +314,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_2(Seg sg, int it, float*
     qv = q[i0];
     iv = inv[i0];
   }
-  const double pq = fold(PB + br.b0, br.nb);
+  const double pq = fold(PB + br.b0, br.nf);
   const float alpha = (float)safe_div(S->rz[it & 1], pq);
   double acc[2] = {0.0, 0.0};
   if (pf) {
@@ -347,7 +380,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_init2(M A, Seg sg, cons
   store_partials<3>(acc, PA, sg.G, br.vb);
 }
 
-template <class M>
+template <class M, bool PF>
 __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int it, float* __restrict__ x,
                            const float* __restrict__ r_in, float* __restrict__ r_out,
                            const float* __restrict__ w_in, float* __restrict__ w_out,
@@ -361,24 +394,27 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int i
   const int G = sg.G;
   const bool first = (it == 0);
   // one row per thread on an ELL matrix: issue the row's loads before the fold
-  const bool pf = M::kEll && (br.end - br.begin <= (int)blockDim.x);
+  constexpr bool pf = PF && M::kEll;
   const int i0 = br.begin + (int)threadIdx.x;
   const bool has = pf && i0 < br.end;
   float a[7], wc[7], sc[7], ic[7], rc[7];
   float p0 = 0.f, x0 = 0.f;
+  const uint32_t o0 = (uint32_t)i0 * 4u;
   if constexpr (M::kEll) {
     if (has) {
-      int c[7];
-      A.load(i0, a, c);
+      uint32_t o[7];
+      A.load_off(i0, a, o);
+      const uint32_t nb4 = (uint32_t)A.n * 4u;
+      const FBuf W(w_in, nb4), Sv(s_in, nb4), I(inv, nb4), R(r_in, nb4);
 #pragma unroll
       for (int j = 0; j < 7; ++j) {
-        wc[j] = w_in[c[j]];
-        sc[j] = first ? 0.f : s_in[c[j]];
-        ic[j] = inv[c[j]];
-        rc[j] = r_in[c[j]];
+        wc[j] = W(o[j]);
+        sc[j] = first ? 0.f : Sv(o[j]);
+        ic[j] = I(o[j]);
+        rc[j] = R(o[j]);
       }
-      p0 = first ? 0.f : p[i0];
-      x0 = x[i0];
+      p0 = first ? 0.f : ld_off((const float*)p, o0);
+      x0 = ld_off((const float*)x, o0);
     }
   }
   double alpha_d, beta_d = 0.0;
@@ -388,7 +424,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int i
     const int pv = (it - 1) & 1;
     const double* pp[2] = {P_in + br.b0, P_in + G + br.b0};
     double gd[2];
-    fold_n<2>(pp, br.nb, gd);
+    fold_n<2>(pp, br.nf, gd);
     const double g = gd[0], d = gd[1];
     beta_d = safe_div(g, S->rz[pv]);
     alpha_d = safe_div(g, d - beta_d * safe_div(g, S->alpha[pv]));
@@ -411,12 +447,12 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int i
       const float ui = rc[0] * ii;
       const float pi = first ? ui : ui + beta * p0;
       const float si = first ? wc[0] : wc[0] + beta * sc[0];
-      x[i0] = x0 + alpha * pi;
+      st_off(x, o0, x0 + alpha * pi);
       const float rn = rc[0] - alpha * si;
-      p[i0] = pi;
-      s_out[i0] = si;
-      r_out[i0] = rn;
-      w_out[i0] = wn;
+      st_off(p, o0, pi);
+      st_off(s_out, o0, si);
+      st_off(r_out, o0, rn);
+      st_off(w_out, o0, wn);
       const float un = rn * ii;
       acc[0] += (double)rn * (double)un;
       acc[1] += (double)wn * (double)un;
@@ -480,7 +516,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_init(M A, Seg sg, const 
   store_partials<3>(acc, PA, sg.G, br.vb);
 }
 
-template <class M, int NV>
+template <class M, int NV, bool PF>
 __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_1(M A, Seg sg, int it, const float* __restrict__ r,
                        const float* __restrict__ inv, const float* __restrict__ p_in,
                        const float* __restrict__ v_in, float* __restrict__ p_out,
@@ -491,34 +527,36 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_1(M A, Seg sg, int it, c
   if (S->f.done) return;
   const int n = sg.n;
   // one row per thread on an ELL matrix: issue the row's loads before the fold
-  const bool pf = M::kEll && (br.end - br.begin <= (int)blockDim.x);
+  constexpr bool pf = PF && M::kEll;
   const int i0 = br.begin + (int)threadIdx.x;
   const bool has = pf && i0 < br.end;
   float a[7], ic[7], rr[NV][7], pp[NV][7], vv[NV][7], r0i[NV];
   if constexpr (M::kEll) {
     if (has) {
-      int c[7];
-      A.load(i0, a, c);
+      uint32_t o[7];
+      A.load_off(i0, a, o);
+      const uint32_t nb4 = (uint32_t)n * 4u * NV;
+      const FBuf I(inv, (uint32_t)n * 4u), R(r, nb4), P(p_in, nb4), V(v_in, nb4);
 #pragma unroll
       for (int j = 0; j < 7; ++j) {
-        ic[j] = inv[c[j]];
+        ic[j] = I(o[j]);
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
-          const size_t k = (size_t)q * n + c[j];
-          rr[q][j] = r[k];
-          pp[q][j] = it ? p_in[k] : 0.f;
-          vv[q][j] = it ? v_in[k] : 0.f;
+          const uint32_t oq = o[j] + (uint32_t)q * (uint32_t)n * 4u;
+          rr[q][j] = R(oq);
+          pp[q][j] = it ? P(oq) : 0.f;
+          vv[q][j] = it ? V(oq) : 0.f;
         }
       }
 #pragma unroll
-      for (int q = 0; q < NV; ++q) r0i[q] = r0[(size_t)q * n + i0];
+      for (int q = 0; q < NV; ++q) r0i[q] = ld_off(r0 + (size_t)q * n, o[0]);
     }
   }
   bool first = (it == 0);
   double beta_d = 0.0, omega_d = 0.0;
   if (!first) {
     const int pv = (it - 1) & 1;
-    const double rho = fold(PA + br.b0, br.nb);   // r0.r after the previous iteration
+    const double rho = fold(PA + br.b0, br.nf);   // r0.r after the previous iteration
     omega_d = S->omega[pv];
     beta_d = safe_div(rho, S->rho[pv]) * safe_div(S->alpha[pv], omega_d);
     if (br.vb == br.b0 && threadIdx.x == 0) S->rho[it & 1] = rho;
@@ -540,12 +578,13 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_1(M A, Seg sg, int it, c
           v[q] += a[j] * (pc * ic[j]);
         }
       }
+      const uint32_t o0 = (uint32_t)i0 * 4u;
 #pragma unroll
       for (int q = 0; q < NV; ++q) {
-        const size_t k = (size_t)q * n + i0;
+        const size_t qn = (size_t)q * n;
         const float pi = first ? rr[q][0] : rr[q][0] + beta * (pp[q][0] - omega * vv[q][0]);
-        p_out[k] = pi;
-        v_out[k] = v[q];
+        st_off(p_out + qn, o0, pi);
+        st_off(v_out + qn, o0, v[q]);
         acc[0] += (double)r0i[q] * (double)v[q];
       }
     }
@@ -576,7 +615,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_1(M A, Seg sg, int it, c
   store_partials<1>(acc, PB, sg.G, br.vb);
 }
 
-template <class M, int NV>
+template <class M, int NV, bool PF>
 __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_2(M A, Seg sg, int it, const float* __restrict__ r,
                        const float* __restrict__ inv, const float* __restrict__ v,
                        float* __restrict__ s, float* __restrict__ t,
@@ -586,27 +625,29 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_2(M A, Seg sg, int it, c
   if (S->f.done) return;
   const int n = sg.n;
   // one row per thread on an ELL matrix: issue the row's loads before the fold
-  const bool pf = M::kEll && (br.end - br.begin <= (int)blockDim.x);
+  constexpr bool pf = PF && M::kEll;
   const int i0 = br.begin + (int)threadIdx.x;
   const bool has = pf && i0 < br.end;
   float a[7], ic[7], rr[NV][7], vv[NV][7];
   if constexpr (M::kEll) {
     if (has) {
-      int c[7];
-      A.load(i0, a, c);
+      uint32_t o[7];
+      A.load_off(i0, a, o);
+      const uint32_t nb4 = (uint32_t)n * 4u * NV;
+      const FBuf I(inv, (uint32_t)n * 4u), R(r, nb4), V(v, nb4);
 #pragma unroll
       for (int j = 0; j < 7; ++j) {
-        ic[j] = inv[c[j]];
+        ic[j] = I(o[j]);
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
-          const size_t k = (size_t)q * n + c[j];
-          rr[q][j] = r[k];
-          vv[q][j] = v[k];
+          const uint32_t oq = o[j] + (uint32_t)q * (uint32_t)n * 4u;
+          rr[q][j] = R(oq);
+          vv[q][j] = V(oq);
         }
       }
     }
   }
-  const double r0v = fold(PB + br.b0, br.nb);
+  const double r0v = fold(PB + br.b0, br.nf);
   const double alpha_d = safe_div(S->rho[it & 1], r0v);
   if (br.vb == br.b0 && threadIdx.x == 0) S->alpha[it & 1] = alpha_d;
   const float alpha = (float)alpha_d;
@@ -624,12 +665,13 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_2(M A, Seg sg, int it, c
           tv[q] += a[j] * (sc * ic[j]);
         }
       }
+      const uint32_t o0 = (uint32_t)i0 * 4u;
 #pragma unroll
       for (int q = 0; q < NV; ++q) {
-        const size_t k = (size_t)q * n + i0;
+        const size_t qn = (size_t)q * n;
         const float si = rr[q][0] - alpha * vv[q][0];
-        s[k] = si;
-        t[k] = tv[q];
+        st_off(s + qn, o0, si);
+        st_off(t + qn, o0, tv[q]);
         acc[0] += (double)tv[q] * (double)si;
         acc[1] += (double)tv[q] * (double)tv[q];
         acc[2] += (double)si * (double)si;
@@ -664,7 +706,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_2(M A, Seg sg, int it, c
   store_partials<3>(acc, PC, sg.G, br.vb);
 }
 
-template <int NV>
+template <int NV, bool PF>
 __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_3(Seg sg, int it, float* __restrict__ x, float* __restrict__ r,
                        const float* __restrict__ inv, const float* __restrict__ p,
                        const float* __restrict__ s, const float* __restrict__ t,
@@ -675,7 +717,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_3(Seg sg, int it, float*
   if (S->f.done) return;
   const int n = sg.n, G = sg.G;
   // one row per thread: load it before the fold
-  const bool pf = br.end - br.begin <= (int)blockDim.x;
+  constexpr bool pf = PF;
   const int i0 = br.begin + (int)threadIdx.x;
   const bool has = pf && i0 < br.end;
   float iv = 0.f, pv[NV], sv[NV], tv[NV], xv[NV], r0v[NV];
@@ -693,7 +735,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_3(Seg sg, int it, float*
   }
   const double* pp[3] = {PC + br.b0, PC + G + br.b0, PC + 2 * G + br.b0};
   double tv3[3];
-  fold_n<3>(pp, br.nb, tv3);
+  fold_n<3>(pp, br.nf, tv3);
   const double ts = tv3[0], tt = tv3[1], ss = tv3[2];
   const double omega_d = ss > S->f.thr2 ? safe_div(ts, tt) : 0.0;
   if (br.vb == br.b0 && threadIdx.x == 0) {
@@ -778,8 +820,15 @@ size_t krylov_workspace_bytes_batch(int32_t n, int nvec, int n_cases, int max_bl
   return c.off;
 }
 
+// Most workgroups a single-case launch over n rows can have (one row per
+// thread at the smallest workgroup): the partial rows of its workspace.
+static int single_max_blocks(int32_t n) {
+  const int64_t g = ((int64_t)n + kWave - 1) / kWave;
+  return (int)(g < kMaxKrylovBlocks ? (g < 1 ? 1 : g) : kMaxKrylovBlocks);
+}
+
 size_t krylov_workspace_bytes(int32_t n, int nvec) {
-  return krylov_workspace_bytes_batch(n, nvec, 1, kMaxReduceBlocks);
+  return krylov_workspace_bytes_batch(n, nvec, 1, single_max_blocks(n));
 }
 
 static void fill_info(wcfd_solve_info* info, const KFlags& h) {
@@ -852,8 +901,10 @@ static int finish(const KFlags* f, int n, wcfd_solve_info* info) {
 // groups already queued find the flags set and exit at entry, so the result
 // is the same as checking after every group synchronously (the reference
 // solver's check interval).
+// The check kernels run with the iteration kernels' workgroup size B, so
+// that their fold of ||r||^2 has the order of a prefold.
 template <class LaunchGroup, class RrPart>
-static int run_groups(const Seg& sg, KState* st, RrPart&& rr_part, int maxiter, int ce,
+static int run_groups(const Seg& sg, int B, KState* st, RrPart&& rr_part, int maxiter, int ce,
                       int fixed_groups, LaunchGroup&& launch_group, wcfd_solve_info* info,
                       hipStream_t s) {
   const int K = sg.n_cases;
@@ -865,7 +916,8 @@ static int run_groups(const Seg& sg, KState* st, RrPart&& rr_part, int maxiter, 
     for (int g = 0, it = 0; g < fixed_groups; ++g) {
       launch_group(it);
       it += ce;
-      WCFD_LAUNCH(K_FOLD, s, k_check, K, sg, rr_part(it), st, it, maxiter, (KFlags*)nullptr);
+      WCFD_LAUNCH_B(K_FOLD, s, k_check, K, B, sg, rr_part(it), st, it, maxiter,
+                    (KFlags*)nullptr);
     }
     return check_launch();
   }
@@ -883,8 +935,8 @@ static int run_groups(const Seg& sg, KState* st, RrPart&& rr_part, int maxiter, 
       launch_group(it);
       it += ce;
       const int sl = launched % kSlots;
-      WCFD_LAUNCH(K_FOLD, s, k_check, K, sg, rr_part(it), st, it, maxiter,
-                  hs->dslot + (size_t)sl * hs->cap);
+      WCFD_LAUNCH_B(K_FOLD, s, k_check, K, B, sg, rr_part(it), st, it, maxiter,
+                    hs->dslot + (size_t)sl * hs->cap);
       if ((e = check_launch())) return e;
       if ((e = check_hip(hipEventRecord(hs->ev[sl], s)))) return e;
       ++launched;
@@ -908,6 +960,7 @@ static int run_groups(const Seg& sg, KState* st, RrPart&& rr_part, int maxiter, 
 namespace {
 std::atomic<int> g_krylov_grid{kKrylovGrid};
 std::atomic<int> g_krylov_block{kBlock};
+std::atomic<int> g_krylov_fold{kMaxInlineFold};
 }  // namespace
 
 KrylovLaunch krylov_launch() { return {g_krylov_grid.load(), g_krylov_block.load()}; }
@@ -917,6 +970,23 @@ struct KLaunch {
   Seg sg;
   int B;
 };
+
+// One row per thread (every workgroup's chunk fits its threads): the
+// iteration kernels' prefetching form, compiled without the grid-stride
+// loop, whose 64-bit gather addresses would otherwise set the register
+// count (and so the occupancy) of the whole kernel.  Both forms evaluate a
+// row with the same expression, so the choice does not change results.
+template <class M>
+static bool row_per_thread(const KLaunch& L) {
+  return M::kEll && L.sg.chunk <= L.B;
+}
+
+// Reduces the ND partial rows at P to one value per case (Seg::prefold).
+template <int ND>
+static void prefold(const KLaunch& L, double* P, KState* st, hipStream_t s) {
+  if (L.sg.prefold)
+    WCFD_LAUNCH_B(K_FOLD, s, k_prefold<ND>, L.sg.n_cases, L.B, L.sg, P, st);
+}
 
 static int prepare(KState* st, const Seg& sg, const Tol& tl, hipStream_t s) {
   hipLaunchKernelGGL(k_set_tolerances, dim3(grid_for(sg.n_cases)), dim3(kBlock), 0, s, st,
@@ -930,6 +1000,7 @@ static int cg_impl(const M& A, const KLaunch& L, const float* b, float* x, const
                    hipStream_t s) {
   const Seg& sg = L.sg;
   const int n = sg.n, B = L.B, G = sg.G, K = sg.n_cases;
+  const bool pf = row_per_thread<M>(L);
   float* q = w.s;
   int e = prepare(w.st, sg, tl, s);
   if (e) return e;
@@ -945,14 +1016,24 @@ static int cg_impl(const M& A, const KLaunch& L, const float* b, float* x, const
   auto group = [&](int it0) {
     for (int it = it0; it < it0 + ce; ++it) {
       const int cur = it & 1;
-      WCFD_LAUNCH_B(K_CG_1, s, k_cg_1<M>, G, B, A, sg, it, w.r, w.inv, pb[cur], pb[1 - cur], q,
-                    w.PA, w.PB, w.st);
-      WCFD_LAUNCH_B(K_CG_2, s, k_cg_2, G, B, sg, it, x, w.r, w.inv, pb[1 - cur], q, w.PB, w.PA,
-                    w.st);
+      if (pf)
+        WCFD_LAUNCH_B(K_CG_1, s, (k_cg_1<M, true>), G, B, A, sg, it, w.r, w.inv, pb[cur],
+                      pb[1 - cur], q, w.PA, w.PB, w.st);
+      else
+        WCFD_LAUNCH_B(K_CG_1, s, (k_cg_1<M, false>), G, B, A, sg, it, w.r, w.inv, pb[cur],
+                      pb[1 - cur], q, w.PA, w.PB, w.st);
+      prefold<1>(L, w.PB, w.st, s);
+      if (pf)
+        WCFD_LAUNCH_B(K_CG_2, s, k_cg_2<true>, G, B, sg, it, x, w.r, w.inv, pb[1 - cur], q, w.PB,
+                      w.PA, w.st);
+      else
+        WCFD_LAUNCH_B(K_CG_2, s, k_cg_2<false>, G, B, sg, it, x, w.r, w.inv, pb[1 - cur], q, w.PB,
+                      w.PA, w.st);
+      prefold<2>(L, w.PA, w.st, s);
     }
   };
   auto rr_part = [&](int) { return (const double*)(w.PA + G); };
-  return run_groups(sg, w.st, rr_part, maxiter, ce, tl.fixed_groups, group, info, s);
+  return run_groups(sg, B, w.st, rr_part, maxiter, ce, tl.fixed_groups, group, info, s);
 }
 
 template <class M>
@@ -961,6 +1042,7 @@ static int cgs_impl(const M& A, const KLaunch& L, const float* b, float* x, cons
                     hipStream_t s) {
   const Seg& sg = L.sg;
   const int n = sg.n, B = L.B, G = sg.G, K = sg.n_cases;
+  const bool pf = row_per_thread<M>(L);
   int e = prepare(w.st, sg, tl, s);
   if (e) return e;
   float* rb[2] = {w.r, w.r0};
@@ -979,13 +1061,20 @@ static int cgs_impl(const M& A, const KLaunch& L, const float* b, float* x, cons
   auto group = [&](int it0) {
     for (int it = it0; it < it0 + ce; ++it) {
       const int cur = it & 1;
-      WCFD_LAUNCH_B(K_CG_1, s, k_cgs_iter<M>, G, B, A, sg, it, x, rb[cur], rb[1 - cur], wb[cur],
-                    wb[1 - cur], sb[cur], sb[1 - cur], w.p0, w.inv, pb[cur], pb[1 - cur], w.st);
+      if (pf)
+        WCFD_LAUNCH_B(K_CG_1, s, (k_cgs_iter<M, true>), G, B, A, sg, it, x, rb[cur], rb[1 - cur],
+                      wb[cur], wb[1 - cur], sb[cur], sb[1 - cur], w.p0, w.inv, pb[cur],
+                      pb[1 - cur], w.st);
+      else
+        WCFD_LAUNCH_B(K_CG_1, s, (k_cgs_iter<M, false>), G, B, A, sg, it, x, rb[cur], rb[1 - cur],
+                      wb[cur], wb[1 - cur], sb[cur], sb[1 - cur], w.p0, w.inv, pb[cur],
+                      pb[1 - cur], w.st);
+      prefold<3>(L, pb[1 - cur], w.st, s);
     }
   };
   // the r.r partials of the last iteration (it_end - 1) are in pb[it_end & 1], slot 2
   auto rr_part = [&](int it_end) { return (const double*)(pb[it_end & 1] + 2 * G); };
-  return run_groups(sg, w.st, rr_part, maxiter, ce, tl.fixed_groups, group, info, s);
+  return run_groups(sg, B, w.st, rr_part, maxiter, ce, tl.fixed_groups, group, info, s);
 }
 
 template <class M, int NV>
@@ -994,6 +1083,7 @@ static int bicg_impl(const M& A, const KLaunch& L, const float* b, float* x, con
                      hipStream_t s) {
   const Seg& sg = L.sg;
   const int n = sg.n, B = L.B, G = sg.G, K = sg.n_cases;
+  const bool pf = row_per_thread<M>(L);
   int e = prepare(w.st, sg, tl, s);
   if (e) return e;
   WCFD_LAUNCH(K_INV_DIAG, s, k_inv_diag<M>, grid_for(n), A, n, w.inv);
@@ -1006,16 +1096,31 @@ static int bicg_impl(const M& A, const KLaunch& L, const float* b, float* x, con
   auto group = [&](int it0) {
     for (int it = it0; it < it0 + ce; ++it) {
       const int cur = it & 1;
-      WCFD_LAUNCH_B(K_BI_1, s, (k_bi_1<M, NV>), G, B, A, sg, it, w.r, w.inv, pb[cur], vb[cur],
-                    pb[1 - cur], vb[1 - cur], w.r0, w.PA, w.PB, w.st);
-      WCFD_LAUNCH_B(K_BI_2, s, (k_bi_2<M, NV>), G, B, A, sg, it, w.r, w.inv, vb[1 - cur], w.s,
-                    w.t, w.PB, w.PC, w.st);
-      WCFD_LAUNCH_B(K_BI_3, s, k_bi_3<NV>, G, B, sg, it, x, w.r, w.inv, pb[1 - cur], w.s, w.t,
-                    w.r0, w.PC, w.PA, w.st);
+      if (pf)
+        WCFD_LAUNCH_B(K_BI_1, s, (k_bi_1<M, NV, true>), G, B, A, sg, it, w.r, w.inv, pb[cur],
+                      vb[cur], pb[1 - cur], vb[1 - cur], w.r0, w.PA, w.PB, w.st);
+      else
+        WCFD_LAUNCH_B(K_BI_1, s, (k_bi_1<M, NV, false>), G, B, A, sg, it, w.r, w.inv, pb[cur],
+                      vb[cur], pb[1 - cur], vb[1 - cur], w.r0, w.PA, w.PB, w.st);
+      prefold<1>(L, w.PB, w.st, s);
+      if (pf)
+        WCFD_LAUNCH_B(K_BI_2, s, (k_bi_2<M, NV, true>), G, B, A, sg, it, w.r, w.inv, vb[1 - cur],
+                      w.s, w.t, w.PB, w.PC, w.st);
+      else
+        WCFD_LAUNCH_B(K_BI_2, s, (k_bi_2<M, NV, false>), G, B, A, sg, it, w.r, w.inv, vb[1 - cur],
+                      w.s, w.t, w.PB, w.PC, w.st);
+      prefold<3>(L, w.PC, w.st, s);
+      if (pf)
+        WCFD_LAUNCH_B(K_BI_3, s, (k_bi_3<NV, true>), G, B, sg, it, x, w.r, w.inv, pb[1 - cur], w.s,
+                      w.t, w.r0, w.PC, w.PA, w.st);
+      else
+        WCFD_LAUNCH_B(K_BI_3, s, (k_bi_3<NV, false>), G, B, sg, it, x, w.r, w.inv, pb[1 - cur],
+                      w.s, w.t, w.r0, w.PC, w.PA, w.st);
+      prefold<2>(L, w.PA, w.st, s);
     }
   };
   auto rr_part = [&](int) { return (const double*)(w.PA + G); };
-  return run_groups(sg, w.st, rr_part, maxiter, ce, tl.fixed_groups, group, info, s);
+  return run_groups(sg, B, w.st, rr_part, maxiter, ce, tl.fixed_groups, group, info, s);
 }
 
 // maxiter under a fixed launch budget: the budget's iterations at most
@@ -1072,19 +1177,25 @@ static int with_ell(const wcfd_mesh& m, const float* values, F&& f) {
   return f(EllMat<false>{m.n_cells, values, m.slot_nbr, m.slot_dnb});
 }
 
+// grid 0: one row per thread (single_seg caps the grid at ceil(n / block))
 static KLaunch single_launch(int n) {
   const KrylovLaunch kl = krylov_launch();
-  return {single_seg(n, kl.block, kl.grid), kl.block};
+  Seg sg = single_seg(n, kl.block, kl.grid > 0 ? kl.grid : kMaxKrylovBlocks);
+  sg.prefold = sg.G > g_krylov_fold.load() ? 1 : 0;
+  return {sg, kl.block};
 }
 
 static KLaunch launch_of(const wcfd_batch* bt, int n) {
   if (!bt) return single_launch(n);
-  return {batch_seg(*bt, n), bt->block};
+  Seg sg = batch_seg(*bt, n);
+  const int widest = bt->max_case_blocks > 0 ? bt->max_case_blocks : bt->n_blocks;
+  sg.prefold = widest > g_krylov_fold.load() ? 1 : 0;
+  return {sg, bt->block};
 }
 
 static KWork work_of(void* ws, const wcfd_batch* bt, int n, int nvec) {
   return bt ? carve_krylov(ws, n, nvec, bt->n_cases, bt->n_blocks)
-            : carve_krylov(ws, n, nvec, 1, kMaxReduceBlocks);
+            : carve_krylov(ws, n, nvec, 1, single_max_blocks(n));
 }
 
 int solve_cg(const CsrView& A, int variant, const float* b, float* x, const Tol& tl,
@@ -1179,13 +1290,21 @@ int wcfd_dot_f64(int32_t n, const float* a, const float* b, double* out, void* w
 }
 
 int wcfd_set_krylov_launch(int grid, int block) {
-  if (grid < 1 || grid > kMaxReduceBlocks || block < kWave || block > kMaxKrylovBlock ||
+  if (grid < 0 || grid > kMaxKrylovBlocks || block < kWave || block > kMaxKrylovBlock ||
       block % kWave != 0)
     return WCFD_EINVAL;
   g_krylov_grid = grid;
   g_krylov_block = block;
   return WCFD_OK;
 }
+
+int wcfd_set_krylov_fold(int max_inline) {
+  if (max_inline < 0) return WCFD_EINVAL;
+  g_krylov_fold = max_inline;
+  return WCFD_OK;
+}
+
+int wcfd_get_krylov_fold(void) { return g_krylov_fold.load(); }
 
 int wcfd_get_krylov_launch(int* grid, int* block) {
   if (!grid || !block) return WCFD_EINVAL;

@@ -1,12 +1,14 @@
-"""Autotuner for the Krylov launch shape (workgroups x threads).
+"""Autotuner for the Krylov launch shape (workgroups x threads, fold mode).
 
 The CG and BiCGSTAB kernels are grid-stride loops whose workgroups each write
 one float64 partial per reduction; the next kernel folds all of them in its
-prologue.  More workgroups hide memory latency better but make every fold
+prologue, or, above ``wcfd_set_krylov_fold``'s limit, a fold kernel reduces
+them to one value first.  More workgroups hide memory latency better (one row
+per thread lets a row's loads issue before the fold) but make inline folds
 longer, so the best shape depends on the mesh size.  ``autotune_krylov``
 times fixed-iteration solves on a pressure-like matrix built on the mesh's
-own pattern for every candidate shape, installs the fastest with
-``wcfd_set_krylov_launch`` and caches it by cell count.
+own pattern for every candidate, installs the fastest and caches it by cell
+count.  Grid 0 means one row per thread.
 """
 import ctypes
 
@@ -15,8 +17,11 @@ import torch
 from . import _lib
 from .FV.device import ptr, stream_ptr
 
-GRIDS = (128, 256, 512, 1024, 2048)
+GRIDS = (128, 256, 512, 1024, 2048, 0)
 BLOCKS = (128, 256, 512)
+# the library's default inline-fold limit, and "always prefold"
+FOLDS = (2048, 0)
+PREFOLD_GRIDS = (1024, 2048, 0)    # grids on which "always prefold" is tried
 
 _cache = {}
 
@@ -34,11 +39,17 @@ def _laplacian_ell(dm):
     return ell.contiguous()
 
 
-def time_solvers(dm, grid, block, iters=60, reps=2, cg_variant=0):
-    """Milliseconds of `iters` CG iterations and `iters` 3-vector BiCGSTAB
-    iterations at launch shape (grid, block)."""
+def install(grid, block, fold=FOLDS[0]):
     lib = _lib.load()
     _lib.check(lib.wcfd_set_krylov_launch(grid, block), 'set_krylov_launch')
+    _lib.check(lib.wcfd_set_krylov_fold(fold), 'set_krylov_fold')
+
+
+def time_solvers(dm, grid, block, iters=60, reps=2, cg_variant=0, fold=FOLDS[0]):
+    """Milliseconds of `iters` CG iterations and `iters` 3-vector BiCGSTAB
+    iterations at launch shape (grid, block) and inline-fold limit ``fold``."""
+    lib = _lib.load()
+    install(grid, block, fold)
     C = dm.n_cells
     dev = dm.device
     A = _laplacian_ell(dm)
@@ -70,24 +81,28 @@ def time_solvers(dm, grid, block, iters=60, reps=2, cg_variant=0):
 
 def autotune_krylov(dm, grids=GRIDS, blocks=BLOCKS, iters=60, cg_weight=1.0, bicg_weight=0.1,
                     cg_variant=0):
-    """Pick and install the fastest (grid, block) for mesh ``dm``
+    """Pick and install the fastest (grid, block, fold) for mesh ``dm``
     (a DeviceMesh).  The weights reflect a SIMPLE iteration, where the
     pressure CG runs far more iterations than the momentum BiCGSTAB."""
     key = (dm.n_cells, cg_variant)
     if key in _cache:
-        g, b = _cache[key]['grid'], _cache[key]['block']
-        _lib.check(_lib.load().wcfd_set_krylov_launch(g, b), 'set_krylov_launch')
-        return _cache[key]
+        c = _cache[key]
+        install(c['grid'], c['block'], c['fold'])
+        return c
     results = []
     for blk in blocks:
         for grid in grids:
             if grid * blk > 4 * dm.n_cells and grid > grids[0]:
                 continue            # more threads than rows: no gain
-            cg_ms, bi_ms = time_solvers(dm, grid, blk, iters, cg_variant=cg_variant)
-            results.append({'grid': grid, 'block': blk, 'cg_ms': cg_ms, 'bicg_ms': bi_ms,
-                            'score': cg_weight * cg_ms + bicg_weight * bi_ms})
+            for fold in FOLDS:
+                if fold != FOLDS[0] and grid not in PREFOLD_GRIDS:
+                    continue
+                cg_ms, bi_ms = time_solvers(dm, grid, blk, iters, cg_variant=cg_variant,
+                                            fold=fold)
+                results.append({'grid': grid, 'block': blk, 'fold': fold, 'cg_ms': cg_ms,
+                                'bicg_ms': bi_ms, 'score': cg_weight * cg_ms + bicg_weight * bi_ms})
     best = min(results, key=lambda r: r['score'])
-    _lib.check(_lib.load().wcfd_set_krylov_launch(best['grid'], best['block']), 'set_krylov_launch')
+    install(best['grid'], best['block'], best['fold'])
     out = dict(best, iters=iters, candidates=results)
     _cache[key] = out
     return out
