@@ -310,7 +310,7 @@ __device__ __forceinline__ void fold_thread(const double* const (&p)[ND], int nb
 
 // Loads in flight per thread in the folds inside iteration kernels, where
 // they add to the registers held by the prefetched row (build-time choice;
-// the prefold kernel uses 8).
+// the prefold kernel uses 16).
 #ifndef WCFD_FOLD_UNROLL
 #define WCFD_FOLD_UNROLL 2
 #endif
