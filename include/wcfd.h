@@ -119,6 +119,14 @@ typedef struct wcfd_batch {
   double* monitor;          /* [K][WCFD_MONITOR_LEN] monitor records, zeroed at the start */
   int32_t* outer_iters;     /* [K] SIMPLE iterations run per case */
   int32_t max_case_blocks;  /* largest blk_off[k+1] - blk_off[k]; 0: taken as n_blocks */
+  /* Compacted face/cell launches: [2][n] workgroup item ranges (starts, then
+   * ends; at most 256 items each) covering only the cases to iterate --
+   * cells, all faces (internal then boundary numbering), boundary faces
+   * (0-based).  NULL: the face and cell kernels cover the whole arena. */
+  int32_t n_cell_blk, n_face_blk, n_bface_blk;
+  const int32_t* cell_blk;
+  const int32_t* face_blk;
+  const int32_t* bface_blk;
 } wcfd_batch;
 
 /* Convergence monitor of a steady run (no counterpart in the reference,

@@ -17,6 +17,6 @@ int main(void) {
   printf("wcfd_simple_info %zu\n", sizeof(wcfd_simple_info));
   P(wcfd_simple_info, continuity); P(wcfd_simple_info, velocity_change);
   printf("wcfd_batch %zu\n", sizeof(wcfd_batch));
-  P(wcfd_batch, cell_off); P(wcfd_batch, active); P(wcfd_batch, stop); P(wcfd_batch, monitor); P(wcfd_batch, outer_iters); P(wcfd_batch, max_case_blocks);
+  P(wcfd_batch, cell_off); P(wcfd_batch, active); P(wcfd_batch, stop); P(wcfd_batch, monitor); P(wcfd_batch, outer_iters); P(wcfd_batch, max_case_blocks); P(wcfd_batch, n_bface_blk); P(wcfd_batch, cell_blk); P(wcfd_batch, bface_blk);
   return 0;
 }

@@ -4,7 +4,7 @@ import numpy as np
 import scipy.sparse as sp
 
 from oracle import krylov
-from warpcfd_amd.batch import _Ranges
+from warpcfd_amd.batch import _Ranges, item_blocks
 from warpcfd_amd.workloads import batch64_draws
 
 
@@ -24,6 +24,24 @@ def test_ranges_first_fit_and_coalesce():
     r.release(c, 30)
     assert r.free == [(0, 100)]
     assert r.alloc(0) == 0
+
+
+def test_item_blocks_cover_each_range_exactly_once():
+    """Compacted launch ranges: every item of every range in exactly one
+    workgroup, none outside, at most `per` per workgroup, and ranges never
+    share a workgroup."""
+    assert item_blocks([]).shape == (2, 0)
+    assert item_blocks([(5, 5)]).shape == (2, 0)
+    ranges = [(0, 256), (300, 301), (1000, 1600), (2000, 2513)]
+    blk = item_blocks(ranges, per=256)
+    st, en = blk
+    assert blk.dtype == np.int32 and np.all(en > st) and np.all(en - st <= 256)
+    covered = np.concatenate([np.arange(a, b) for a, b in zip(st, en)])
+    want = np.concatenate([np.arange(a, b) for a, b in ranges])
+    assert np.array_equal(np.sort(covered), want)
+    assert blk.shape[1] == 1 + 1 + 3 + 3
+    for a, b in zip(st, en):
+        assert any(lo <= a and b <= hi for lo, hi in ranges)
 
 
 def test_batch64_draws_match_the_stated_distribution():

@@ -51,7 +51,8 @@ class WcfdBatch(ctypes.Structure):
                 ('cell_off', c_p), ('cell_end', c_p), ('blk_off', c_p), ('blk_case', c_p), ('cell_case', c_p),
                 ('face_case', c_p), ('active', c_p), ('nu', c_p), ('ref_cell', c_p),
                 ('ref_value', c_p), ('stop', c_p), ('monitor', c_p), ('outer_iters', c_p),
-                ('max_case_blocks', c_i32)]
+                ('max_case_blocks', c_i32), ('n_cell_blk', c_i32), ('n_face_blk', c_i32),
+                ('n_bface_blk', c_i32), ('cell_blk', c_p), ('face_blk', c_p), ('bface_blk', c_p)]
 
 
 class SimpleParams(ctypes.Structure):

@@ -77,6 +77,26 @@ class CaseSpec:
         return CaseArena.bytes_for(C, Fi, Fb)
 
 
+ITEMS_PER_BLOCK = 256   # items per workgroup of the face and cell kernels
+
+
+def item_blocks(ranges, per=ITEMS_PER_BLOCK):
+    """Workgroup item ranges of a compacted launch: every [a, b) of
+    ``ranges`` cut into pieces of at most ``per`` items, as a (2, G) int32
+    array of starts and ends."""
+    ranges = [(int(a), int(b)) for a, b in ranges if b > a]
+    if not ranges:
+        return np.zeros((2, 0), np.int32)
+    a = np.array([r[0] for r in ranges], np.int64)
+    b = np.array([r[1] for r in ranges], np.int64)
+    n = (b - a + per - 1) // per
+    first = np.repeat(np.cumsum(n) - n, n)
+    j = np.arange(int(n.sum()), dtype=np.int64) - first
+    st = np.repeat(a, n) + per * j
+    en = np.minimum(st + per, np.repeat(b, n))
+    return np.stack([st, en]).astype(np.int32)
+
+
 class _Ranges:
     """First-fit allocator of contiguous index ranges in [0, cap)."""
 
@@ -152,6 +172,10 @@ class CaseArena:
         self.F = Fic + Fbc
         self.chunk, self.block = int(chunk), int(block)
         self.Gmax = (Cc + chunk - 1) // chunk + K
+        per = ITEMS_PER_BLOCK
+        self.item_cap = {'cells': (Cc + per - 1) // per + K,
+                         'faces': (Fic + per - 1) // per + (Fbc + per - 1) // per + 2 * K,
+                         'bfaces': (Fbc + per - 1) // per + K}
         dev = self.device
         f32, i32 = torch.float32, torch.int32
         z = lambda *s, dt=f32: torch.zeros(*s, dtype=dt, device=dev)
@@ -201,6 +225,9 @@ class CaseArena:
         self.stop = torch.zeros(K, 3, dtype=torch.float64, device=dev)
         self.monitor = torch.zeros(K, _lib.MONITOR_LEN, dtype=torch.float64, device=dev)
         self.outer_iters = z(K, dt=i32)
+        # compacted face / cell launch ranges ([2][n] starts, ends)
+        self.items = {name: z(2 * cap, dt=i32) for name, cap in self.item_cap.items()}
+        self.n_items = {name: 0 for name in self.item_cap}
         self.workspace = torch.zeros(
             self.lib.wcfd_batch_simple_workspace_bytes(Cc, F, K, self.Gmax), dtype=torch.uint8,
             device=dev)
@@ -392,6 +419,22 @@ class CaseArena:
         self.blk_off.copy_(torch.from_numpy(blk_off).to(dev))
         if G:
             self.blk_case[:G] = torch.from_numpy(blk_case).to(dev)
+        # face and cell kernels: the active cases' cells, internal faces
+        # (numbered first) and boundary faces (after all internal faces)
+        live = [sl for k, sl in enumerate(self.slots) if sl is not None and self.h_active[k]]
+        cr = [(sl['c0'], sl['c0'] + sl['spec'].n_cells) for sl in live]
+        ir = [(sl['i0'], sl['i0'] + sl['spec'].n_internal) for sl in live]
+        br = [(sl['b0'], sl['b0'] + sl['spec'].n_boundary) for sl in live]
+        lists = {'cells': item_blocks(cr), 'bfaces': item_blocks(br),
+                 'faces': item_blocks(ir + [(self.Fic + a, self.Fic + b) for a, b in br])}
+        for name, blk in lists.items():
+            n = blk.shape[1]
+            assert n <= self.item_cap[name]
+            self.n_items[name] = n
+            if n:
+                both = np.zeros(2 * n, np.int32)
+                both[:n], both[n:] = blk[0], blk[1]
+                self.items[name][:2 * n] = torch.from_numpy(both).to(dev)
         m = _lib.WcfdMesh()
         m.n_cells, m.n_faces, m.n_internal, m.n_boundary = self.Cc, self.F, self.Fic, self.Fbc
         m.nnz, m.reserved = 0, 0
@@ -403,6 +446,10 @@ class CaseArena:
         b = _lib.WcfdBatch()
         b.n_cases, b.n_blocks, b.chunk, b.block = K, G, self.chunk, self.block
         b.max_case_blocks = self.max_case_blocks
+        b.n_cell_blk, b.n_face_blk, b.n_bface_blk = (self.n_items[k] for k in
+                                                     ('cells', 'faces', 'bfaces'))
+        b.cell_blk, b.face_blk, b.bface_blk = (self.items[k].data_ptr() for k in
+                                               ('cells', 'faces', 'bfaces'))
         for name in ('cell_off', 'cell_end', 'blk_off', 'blk_case', 'cell_case', 'face_case',
                      'active', 'nu', 'ref_cell', 'ref_value', 'stop', 'monitor', 'outer_iters'):
             setattr(b, name, getattr(self, name).data_ptr())
@@ -438,6 +485,13 @@ class CaseArena:
         p.first_iteration = 0
         p.cg_variant = int(self.cg_variant)
         return p
+
+    def launch_key(self):
+        """Grid sizes of every launch of a batched iteration; a captured graph
+        stays valid while they are unchanged (the ranges themselves are read
+        from device memory at run time)."""
+        return (self.n_blocks, self.max_case_blocks, self.n_items['cells'],
+                self.n_items['faces'], self.n_items['bfaces'])
 
     def step(self, info=False, stream=None, budget=None):
         """One SIMPLE iteration of every active case.  With ``info`` returns
@@ -478,13 +532,13 @@ class CaseArena:
         with torch.cuda.graph(g, stream=side):
             self.step(stream=side, budget=budget)
         torch.cuda.current_stream().wait_stream(side)
-        self._graph = {'graph': g, 'blocks': self.n_blocks, 'budget': tuple(budget)}
+        self._graph = {'graph': g, 'blocks': self.launch_key(), 'budget': tuple(budget)}
         return g
 
     def replay(self):
         """One batched SIMPLE iteration by replaying the captured graph."""
         gr = getattr(self, '_graph', None)
-        if gr is None or gr['blocks'] != self.n_blocks:
+        if gr is None or gr['blocks'] != self.launch_key():
             raise RuntimeError('no graph for the current arena contents: call capture() again')
         gr['graph'].replay()
         self.h_steps += self.h_active.astype(np.int64)
@@ -548,13 +602,13 @@ class CaseArena:
                 if n == 0:
                     self.step(budget=budget)
                     self.capture(budget)
-                elif self._graph['blocks'] != self.n_blocks:
+                elif self._graph['blocks'] != self.launch_key():
                     self.capture(budget)           # workgroups changed: capture again
                     self.replay()
                 else:
                     self.replay()
             elif graph:
-                stale = cur is not None and self._graph['blocks'] != self.n_blocks
+                stale = cur is not None and self._graph['blocks'] != self.launch_key()
                 if cur is None or n % recheck == 0 or stale:
                     need = self.budget_for(self.step(info=True), margin)
                     if (stale or cur is None or need[0] > cur[0] or need[1] > cur[1]

@@ -11,8 +11,8 @@ int launch_set_bc(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc, float* fv,
 int launch_face_interp(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc, const float* cv,
                        const float* cg, float* fv, int mask, int skew_mask, float* flux,
                        const float* rua, float* rua_f, int apply_dirichlet, hipStream_t s);
-int launch_green_gauss(const wcfd_mesh& m, const float* fv, float* cg, int mask, int zero_others,
-                       hipStream_t s);
+int launch_green_gauss(const wcfd_mesh& m, const BatchDev& bd, const float* fv, float* cg,
+                       int mask, int zero_others, hipStream_t s);
 int launch_assemble_momentum(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc, const float* cv,
                              const float* fv, const float* flux, float* cg, int compute_grad,
                              float nu, float inv_density, float alpha, float dt, int flags,

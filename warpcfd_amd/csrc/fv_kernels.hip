@@ -19,7 +19,7 @@ thread_local int g_last_hip_error = 0;
 // ------------------------------------------------------------------ BCs
 __global__ void k_set_bc(BatchDev bd, int F, int Fi, int Fb, const uint8_t* __restrict__ type,
                          const float* __restrict__ value, float* __restrict__ fv, int mask) {
-  const int j = xcd_item();
+  const int j = item_index(bd.bfaces);
   if (j >= Fb) return;
   int f = Fi + j;
   if (bd.face_skip(f)) return;
@@ -37,7 +37,7 @@ __global__ void k_face_interp(wcfd_mesh m, BatchDev bd, wcfd_bc bc, const float*
                               int mask, int skew_mask, float* __restrict__ flux,
                               const float* __restrict__ rua, float* __restrict__ rua_f,
                               int apply_dirichlet) {
-  const int f = xcd_item();
+  const int f = item_index(bd.faces);
   const int C = m.n_cells, F = m.n_faces, Fi = m.n_internal, Fb = m.n_boundary;
   if (f >= F || bd.face_skip(f)) return;
   float val[4];
@@ -109,9 +109,9 @@ __device__ inline void gg_field(const wcfd_mesh& m, int c, const int (&fk)[6],
   g[0] = a0; g[1] = a1; g[2] = a2;
 }
 
-__global__ void k_green_gauss(wcfd_mesh m, const float* __restrict__ fv, float* __restrict__ cg,
-                              int mask, int zero_others) {
-  const int c = xcd_item();
+__global__ void k_green_gauss(wcfd_mesh m, Items items, const float* __restrict__ fv,
+                              float* __restrict__ cg, int mask, int zero_others) {
+  const int c = item_index(items);
   const int C = m.n_cells, F = m.n_faces;
   if (c >= C) return;
   int fk[6];
@@ -188,7 +188,7 @@ __global__ void k_assemble_momentum(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
                                     float* __restrict__ values, float* __restrict__ rhs) {
   const int correction = flags & WCFD_SCHEME_CORRECTED;
   const bool implicit_outflow = (flags & WCFD_SCHEME_IMPLICIT_OUTFLOW) != 0;
-  const int c = xcd_item();
+  const int c = item_index(bd.cells);
   const int C = m.n_cells, F = m.n_faces, Fi = m.n_internal, Fb = m.n_boundary;
   if (c >= C || bd.cell_skip(c)) return;
   const float nu = bd.nu ? bd.nu[bd.cell_case[c]] : nu_scalar;
@@ -310,7 +310,7 @@ __global__ void k_assemble_pressure(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
                                     int compute_grad_p, int correction, int ref_cell,
                                     float ref_value, float* __restrict__ values,
                                     float* __restrict__ rhs, float* __restrict__ div_out) {
-  const int c = xcd_item();
+  const int c = item_index(bd.cells);
   const int C = m.n_cells, F = m.n_faces, Fi = m.n_internal, Fb = m.n_boundary;
   if (c >= C || bd.cell_skip(c)) return;
   if (bd.ref_cell) {
@@ -369,7 +369,7 @@ __global__ void k_rua_hbya(wcfd_mesh m, BatchDev bd, const float* __restrict__ v
                            float density, const float* __restrict__ cg, float* __restrict__ hbya,
                            float* __restrict__ cv, float* __restrict__ rua,
                            float* __restrict__ u_prev) {
-  const int c = xcd_item();
+  const int c = item_index(bd.cells);
   const int C = m.n_cells;
   if (c >= C || bd.cell_skip(c)) return;
   const float r = m.volume[c] / (values[c] * density);
@@ -388,7 +388,7 @@ __global__ void k_rua_hbya(wcfd_mesh m, BatchDev bd, const float* __restrict__ v
 
 __global__ void k_relax_p(BatchDev bd, int C, const float* __restrict__ pn, float alpha,
                           float* __restrict__ cv) {
-  const int c = xcd_item();
+  const int c = item_index(bd.cells);
   if (c >= C || bd.cell_skip(c)) return;
   float* p = cv + (size_t)3 * C;
   p[c] = alpha > 0.f ? alpha * pn[c] + (1.0f - alpha) * p[c] : pn[c];
@@ -399,7 +399,7 @@ __global__ void k_face_p_fc(wcfd_mesh m, BatchDev bd, wcfd_bc bc, const float* _
                             const float* __restrict__ cg, float* __restrict__ fv,
                             const float* __restrict__ flux, const float* __restrict__ rua_f,
                             float* __restrict__ fc) {
-  const int f = xcd_item();
+  const int f = item_index(bd.faces);
   const int C = m.n_cells, F = m.n_faces, Fi = m.n_internal, Fb = m.n_boundary;
   if (f >= F || bd.face_skip(f)) return;
   const float* p = cv + (size_t)3 * C;
@@ -525,23 +525,28 @@ __global__ void k_ell_to_csr(wcfd_mesh m, const float* __restrict__ ell, float* 
 // ========================================================== host launchers
 int launch_set_bc(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc, float* fv, int mask,
                   hipStream_t s) {
-  if (m.n_boundary == 0) return WCFD_OK;
-  WCFD_LAUNCH(K_SET_BC, s, k_set_bc, grid_for(m.n_boundary), bd, m.n_faces, m.n_internal,
-              m.n_boundary, bc.type, bc.value, fv, mask);
+  const unsigned g = items_grid(bd.bfaces, m.n_boundary);
+  if (g == 0) return WCFD_OK;
+  WCFD_LAUNCH(K_SET_BC, s, k_set_bc, g, bd, m.n_faces, m.n_internal, m.n_boundary, bc.type,
+              bc.value, fv, mask);
   return check_launch();
 }
 
 int launch_face_interp(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc, const float* cv,
                        const float* cg, float* fv, int mask, int skew_mask, float* flux,
                        const float* rua, float* rua_f, int apply_dirichlet, hipStream_t s) {
-  WCFD_LAUNCH(K_FACE_INTERP, s, k_face_interp, grid_for(m.n_faces), m, bd, bc, cv, cg, fv, mask,
-              skew_mask, flux, rua, rua_f, apply_dirichlet);
+  const unsigned g = items_grid(bd.faces, m.n_faces);
+  if (g == 0) return WCFD_OK;
+  WCFD_LAUNCH(K_FACE_INTERP, s, k_face_interp, g, m, bd, bc, cv, cg, fv, mask, skew_mask, flux,
+              rua, rua_f, apply_dirichlet);
   return check_launch();
 }
 
-int launch_green_gauss(const wcfd_mesh& m, const float* fv, float* cg, int mask, int zero_others,
-                       hipStream_t s) {
-  WCFD_LAUNCH(K_GREEN_GAUSS, s, k_green_gauss, grid_for(m.n_cells), m, fv, cg, mask, zero_others);
+int launch_green_gauss(const wcfd_mesh& m, const BatchDev& bd, const float* fv, float* cg,
+                       int mask, int zero_others, hipStream_t s) {
+  const unsigned g = items_grid(bd.cells, m.n_cells);
+  if (g == 0) return WCFD_OK;
+  WCFD_LAUNCH(K_GREEN_GAUSS, s, k_green_gauss, g, m, bd.cells, fv, cg, mask, zero_others);
   return check_launch();
 }
 
@@ -549,8 +554,10 @@ int launch_assemble_momentum(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc,
                              const float* fv, const float* flux, float* cg, int compute_grad,
                              float nu, float inv_density, float alpha, float dt, int flags,
                              float* values, float* rhs, hipStream_t s) {
-  WCFD_LAUNCH(K_ASM_MOM, s, k_assemble_momentum, grid_for(m.n_cells), m, bd, bc, cv, fv, flux, cg,
-              compute_grad, nu, inv_density, alpha, dt, flags, values, rhs);
+  const unsigned g = items_grid(bd.cells, m.n_cells);
+  if (g == 0) return WCFD_OK;
+  WCFD_LAUNCH(K_ASM_MOM, s, k_assemble_momentum, g, m, bd, bc, cv, fv, flux, cg, compute_grad, nu,
+              inv_density, alpha, dt, flags, values, rhs);
   return check_launch();
 }
 
@@ -558,22 +565,27 @@ int launch_assemble_pressure(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc,
                              const float* flux, const float* rua_f, float* cg, int compute_grad_p,
                              int correction, int ref_cell, float ref_value, float* values,
                              float* rhs, float* div_out, hipStream_t s) {
-  WCFD_LAUNCH(K_ASM_P, s, k_assemble_pressure, grid_for(m.n_cells), m, bd, bc, fv, flux, rua_f,
-              cg, compute_grad_p, correction, ref_cell, ref_value, values, rhs, div_out);
+  const unsigned g = items_grid(bd.cells, m.n_cells);
+  if (g == 0) return WCFD_OK;
+  WCFD_LAUNCH(K_ASM_P, s, k_assemble_pressure, g, m, bd, bc, fv, flux, rua_f, cg, compute_grad_p,
+              correction, ref_cell, ref_value, values, rhs, div_out);
   return check_launch();
 }
 
 int launch_rua_hbya(const wcfd_mesh& m, const BatchDev& bd, const float* values, float density,
                     const float* cg, float* hbya, float* cv, float* rua, float* u_prev,
                     hipStream_t s) {
-  WCFD_LAUNCH(K_RUA_HBYA, s, k_rua_hbya, grid_for(m.n_cells), m, bd, values, density, cg, hbya,
-              cv, rua, u_prev);
+  const unsigned g = items_grid(bd.cells, m.n_cells);
+  if (g == 0) return WCFD_OK;
+  WCFD_LAUNCH(K_RUA_HBYA, s, k_rua_hbya, g, m, bd, values, density, cg, hbya, cv, rua, u_prev);
   return check_launch();
 }
 
 int launch_relax_p(const wcfd_mesh& m, const BatchDev& bd, const float* pn, float alpha, float* cv,
                    hipStream_t s) {
-  WCFD_LAUNCH(K_RELAX_P, s, k_relax_p, grid_for(m.n_cells), bd, m.n_cells, pn, alpha, cv);
+  const unsigned g = items_grid(bd.cells, m.n_cells);
+  if (g == 0) return WCFD_OK;
+  WCFD_LAUNCH(K_RELAX_P, s, k_relax_p, g, bd, m.n_cells, pn, alpha, cv);
   return check_launch();
 }
 
@@ -583,8 +595,9 @@ int launch_correct_velocity(const wcfd_mesh& m, const BatchDev& bd, const Seg& s
                             const float* u_prev, double* resid, float* fc_scratch,
                             double* partials, hipStream_t s) {
   float* fc = resid ? fc_scratch : nullptr;
-  WCFD_LAUNCH(K_FACE_P_FC, s, k_face_p_fc, grid_for(m.n_faces), m, bd, bc, cv, cg, fv, flux,
-              rua_f, fc);
+  const unsigned g = items_grid(bd.faces, m.n_faces);
+  if (g > 0)
+    WCFD_LAUNCH(K_FACE_P_FC, s, k_face_p_fc, g, m, bd, bc, cv, cg, fv, flux, rua_f, fc);
   int e = check_launch();
   if (e) return e;
   WCFD_LAUNCH_B(K_GRAD_P_CORR, s, k_grad_p_correct, sg.G, block, m, sg, fv, cg, cv, hbya, rua, fc,
@@ -648,7 +661,7 @@ int wcfd_face_interpolation(const wcfd_mesh* m, wcfd_bc bc, const float* cell_va
 int wcfd_green_gauss(const wcfd_mesh* m, const float* face_values, float* cell_grad, int field_mask,
                      int zero_others, void* stream) {
   if (!mesh_ok(m) || !face_values || !cell_grad) return WCFD_EINVAL;
-  return launch_green_gauss(*m, face_values, cell_grad, field_mask & 15, zero_others,
+  return launch_green_gauss(*m, BatchDev{}, face_values, cell_grad, field_mask & 15, zero_others,
                             (hipStream_t)stream);
 }
 

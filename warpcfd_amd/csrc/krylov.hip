@@ -58,8 +58,8 @@ __device__ inline double safe_div(double a, double b) { return b != 0.0 ? a / b 
 
 // ------------------------------------------------------------ small kernels
 template <class M>
-__global__ void k_inv_diag(M A, int n, float* __restrict__ inv) {
-  const int i = xcd_item();
+__global__ void k_inv_diag(M A, Items items, int n, float* __restrict__ inv) {
+  const int i = item_index(items);
   if (i >= n) return;
   const float d = A.diag(i);
   inv[i] = d != 0.0f ? 1.0f / d : 1.0f;
@@ -965,10 +965,12 @@ std::atomic<int> g_krylov_fold{kMaxInlineFold};
 
 KrylovLaunch krylov_launch() { return {g_krylov_grid.load(), g_krylov_block.load()}; }
 
-// Launch of one solve: the segmentation and the workgroup size.
+// Launch of one solve: the segmentation, the workgroup size and the
+// compacted row ranges of the batch's cell launches (for k_inv_diag).
 struct KLaunch {
   Seg sg;
   int B;
+  Items rows;
 };
 
 // One row per thread (every workgroup's chunk fits its threads): the
@@ -1004,7 +1006,8 @@ static int cg_impl(const M& A, const KLaunch& L, const float* b, float* x, const
   float* q = w.s;
   int e = prepare(w.st, sg, tl, s);
   if (e) return e;
-  WCFD_LAUNCH(K_INV_DIAG, s, k_inv_diag<M>, grid_for(n), A, n, w.inv);
+  if (const unsigned gi = items_grid(L.rows, n))
+    WCFD_LAUNCH(K_INV_DIAG, s, k_inv_diag<M>, gi, A, L.rows, n, w.inv);
   if (r64)
     WCFD_LAUNCH_B(K_CG_INIT, s, (k_cg_init<M, true>), G, B, A, sg, b, x, w.r, w.inv, w.PA);
   else
@@ -1049,7 +1052,8 @@ static int cgs_impl(const M& A, const KLaunch& L, const float* b, float* x, cons
   float* wb[2] = {w.v0, w.v1};
   float* sb[2] = {w.s, w.t};
   double* pb[2] = {w.PA, w.PC};   // iteration i reads pb[i&1], writes pb[(i+1)&1]
-  WCFD_LAUNCH(K_INV_DIAG, s, k_inv_diag<M>, grid_for(n), A, n, w.inv);
+  if (const unsigned gi = items_grid(L.rows, n))
+    WCFD_LAUNCH(K_INV_DIAG, s, k_inv_diag<M>, gi, A, L.rows, n, w.inv);
   if (r64)
     WCFD_LAUNCH_B(K_CG_INIT, s, (k_cgs_init1<M, true>), G, B, A, sg, b, x, rb[0], w.PC);
   else
@@ -1086,7 +1090,8 @@ static int bicg_impl(const M& A, const KLaunch& L, const float* b, float* x, con
   const bool pf = row_per_thread<M>(L);
   int e = prepare(w.st, sg, tl, s);
   if (e) return e;
-  WCFD_LAUNCH(K_INV_DIAG, s, k_inv_diag<M>, grid_for(n), A, n, w.inv);
+  if (const unsigned gi = items_grid(L.rows, n))
+    WCFD_LAUNCH(K_INV_DIAG, s, k_inv_diag<M>, gi, A, L.rows, n, w.inv);
   WCFD_LAUNCH_B(K_BI_INIT, s, (k_bi_init<M, NV>), G, B, A, sg, b, x, w.r, w.r0, w.PA);
   WCFD_LAUNCH(K_FOLD, s, k_init_fold, K, sg, w.PA, w.PA, w.st, 1, maxiter);
   if ((e = check_launch())) return e;
@@ -1182,7 +1187,7 @@ static KLaunch single_launch(int n) {
   const KrylovLaunch kl = krylov_launch();
   Seg sg = single_seg(n, kl.block, kl.grid > 0 ? kl.grid : kMaxKrylovBlocks);
   sg.prefold = sg.G > g_krylov_fold.load() ? 1 : 0;
-  return {sg, kl.block};
+  return {sg, kl.block, Items{}};
 }
 
 static KLaunch launch_of(const wcfd_batch* bt, int n) {
@@ -1190,7 +1195,7 @@ static KLaunch launch_of(const wcfd_batch* bt, int n) {
   Seg sg = batch_seg(*bt, n);
   const int widest = bt->max_case_blocks > 0 ? bt->max_case_blocks : bt->n_blocks;
   sg.prefold = widest > g_krylov_fold.load() ? 1 : 0;
-  return {sg, bt->block};
+  return {sg, bt->block, Items{bt->cell_blk, bt->n_cell_blk}};
 }
 
 static KWork work_of(void* ws, const wcfd_batch* bt, int n, int nvec) {
