@@ -103,6 +103,16 @@ def algorithmic_bytes(kernel, C, F, Fi, Fb, nnz):
                              + 4 * F + 24 * C + 4 * C + 20 * Fb + 16 * C,
         'assemble_pressure': 24 * C + 6 * C + 72 * C + 4 * C + 4 * F + 12 * C + 4 * nnz + 4 * C
                              + 8 * F + 24 * C + 4 * C + 8 * Fb,
+        # internal: owner, neigh, psi, skew, fv(p) out, flux, rua_f, area, dmag, fc out;
+        # boundary: owner, bc type + value, dmag, fv out, flux, rua_f, area, fc out;
+        # p and grad p per cell
+        'face_p_fc': 48 * Fi + 33 * Fb + 16 * C,
+        # slot faces, sides, normals, volume; p face values and corrected fluxes
+        # (each face once); gradients out; rUA; HbyA in/out; u out; previous u
+        'grad_p_correct': 24 * C + 6 * C + 72 * C + 4 * C + 8 * F + 48 * C + 4 * C + 24 * C
+                          + 12 * C + 12 * C,
+        # volume, diagonal, rUA out, u in / previous u out, HbyA in/out, grad p, u out
+        'rua_hbya': 84 * C,
     }
     return table.get(kernel)
 
@@ -227,24 +237,13 @@ def bench_batch64(args):
             for _ in range(20):
                 prof_arena.step()
             torch.cuda.synchronize()
-        kernels = {k: {'total_ms': round(v[0], 3), 'launches': v[1],
-                       'avg_us': round(1e3 * v[0] / v[1], 3)}
-                   for k, v in sorted(prof.result.items(), key=lambda kv: -kv[1][0])}
-        dominant = max(prof.result.items(), key=lambda kv: kv[1][0])[0]
-        ms, cnt = prof.result[dominant]
-        key = dominant + ('_single' if dominant == 'cg_1' else '')
         nnz = sum(s.layout.nnz for s in specs)
         F = sum(s.n_internal + s.n_boundary for s in specs)
         Fi = sum(s.n_internal for s in specs)
-        byts = algorithmic_bytes(key, cells, F, Fi, F - Fi, nnz)
-        if byts is not None:
-            avg_s = ms / cnt / 1e3
-            ach = byts / avg_s / 1e9
-            roofline = {'bound': 'hbm', 'kernel': dominant, 'achieved': round(ach, 1),
-                        'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(ach / HBM_PEAK_GBS, 4),
-                        'traffic': None, 'bytes_per_launch': byts,
-                        'avg_launch_us': round(avg_s * 1e6, 3),
-                        'note': 'first 20 batch iterations, all 64 cases active'}
+        single = prof_arena.cg_variant == _lib.CG_SINGLE_REDUCTION
+        roofline, kernels = rooflines_from(
+            prof.result, cells, F, Fi, F - Fi, nnz, single,
+            note='first 20 batch iterations, all 64 cases active')
         del prof_arena
 
     seq = None
@@ -420,6 +419,31 @@ def bench_serve(args):
     print(json.dumps(out))
 
 
+def rooflines_from(result, C, F, Fi, Fb, nnz, cg_single=True, note=None):
+    """(roofline of the dominant kernel, per-kernel table with each kernel's own
+    roofline) from the library profiler's {kernel: (total ms, launches)}."""
+    kernels = {k: {'total_ms': round(v[0], 3), 'launches': v[1], 'avg_us': round(1e3 * v[0] / v[1], 3)}
+               for k, v in sorted(result.items(), key=lambda kv: -kv[1][0])}
+    dominant = max(result.items(), key=lambda kv: kv[1][0])[0]
+    roofline = None
+    for k, v in kernels.items():
+        kb = algorithmic_bytes(k + ('_single' if k == 'cg_1' and cg_single else ''),
+                               C, F, Fi, Fb, nnz)
+        if kb is None or v['avg_us'] <= 0:
+            continue
+        gbs = kb / (v['avg_us'] * 1e-6) / 1e9
+        v['bytes_per_launch'] = kb
+        v['gbs'] = round(gbs, 1)
+        v['frac'] = round(gbs / HBM_PEAK_GBS, 4)
+        if k == dominant:
+            roofline = {'bound': 'hbm', 'kernel': k, 'achieved': round(gbs, 1),
+                        'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(gbs / HBM_PEAK_GBS, 4),
+                        'traffic': None, 'bytes_per_launch': kb, 'avg_launch_us': v['avg_us']}
+            if note:
+                roofline['note'] = note
+    return roofline, kernels
+
+
 def profile_roofline(run, C, F, Fi, Fb, nnz, cg_single=True, capacity=400000, note=None):
     """Runs ``run()`` under the library's HIP-event profiler; returns
     (roofline dict of the dominant kernel, per-kernel table)."""
@@ -428,22 +452,7 @@ def profile_roofline(run, C, F, Fi, Fb, nnz, cg_single=True, capacity=400000, no
     with _lib.Profiler(capacity=capacity) as prof:
         run()
         torch.cuda.synchronize()
-    kernels = {k: {'total_ms': round(v[0], 3), 'launches': v[1], 'avg_us': round(1e3 * v[0] / v[1], 3)}
-               for k, v in sorted(prof.result.items(), key=lambda kv: -kv[1][0])}
-    dominant = max(prof.result.items(), key=lambda kv: kv[1][0])[0]
-    ms, cnt = prof.result[dominant]
-    key = dominant + ('_single' if dominant == 'cg_1' and cg_single else '')
-    byts = algorithmic_bytes(key, C, F, Fi, Fb, nnz)
-    roofline = None
-    if byts is not None:
-        avg_s = ms / cnt / 1e3
-        ach = byts / avg_s / 1e9
-        roofline = {'bound': 'hbm', 'kernel': dominant, 'achieved': round(ach, 1),
-                    'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(ach / HBM_PEAK_GBS, 4),
-                    'traffic': None, 'bytes_per_launch': byts, 'avg_launch_us': round(avg_s * 1e6, 3)}
-        if note:
-            roofline['note'] = note
-    return roofline, kernels
+    return rooflines_from(prof.result, C, F, Fi, Fb, nnz, cg_single, note)
 
 
 def bench_cylinder(args):
@@ -750,19 +759,7 @@ def main():
             for _ in range(args.steps):
                 solver.outer_iteration()
             torch.cuda.synchronize()
-        kernels = {k: {'total_ms': round(v[0], 3), 'launches': v[1], 'avg_us': round(1e3 * v[0] / v[1], 3)}
-                   for k, v in sorted(prof.result.items(), key=lambda kv: -kv[1][0])}
-        dominant = max(prof.result.items(), key=lambda kv: kv[1][0])[0]
-        ms, cnt = prof.result[dominant]
-        key = dominant + ('_single' if dominant == 'cg_1' and args.cg == 'single' else '')
-        byts = algorithmic_bytes(key, C, F, Fi, Fb, nnz)
-        if byts is not None:
-            avg_s = ms / cnt / 1e3
-            ach = byts / avg_s / 1e9
-            roofline = {'bound': 'hbm', 'kernel': dominant, 'achieved': round(ach, 1),
-                        'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(ach / HBM_PEAK_GBS, 4),
-                        'traffic': None, 'bytes_per_launch': byts,
-                        'avg_launch_us': round(avg_s * 1e6, 3)}
+        roofline, kernels = rooflines_from(prof.result, C, F, Fi, Fb, nnz, args.cg == 'single')
 
     if args.calibration_copy_mb:
         calibration_copy(args.calibration_copy_mb)
