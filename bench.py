@@ -419,6 +419,11 @@ def bench_serve(args):
     print(json.dumps(out))
 
 
+KERNELS_NOTE = ('avg_us and gbs of a Krylov kernel average over every launch, including the '
+                'launches that exit at entry once their solve has converged inside a check '
+                'group (up to check_every - 1 per solve)')
+
+
 def rooflines_from(result, C, F, Fi, Fb, nnz, cg_single=True, note=None):
     """(roofline of the dominant kernel, per-kernel table with each kernel's own
     roofline) from the library profiler's {kernel: (total ms, launches)}."""
@@ -713,6 +718,8 @@ def main():
         C = dims[0] * dims[1] * dims[2]
         wl['desc'] = wl['desc'].replace('200x40x40 hex (320000 cells)', f'{args.channel_dims} hex '
                                         f'({C} cells)')
+    t_prep = time.perf_counter()
+    print(f'# building {args.workload} mesh', file=sys.stderr, flush=True)
     nodes, cells = build_case(args.workload, dims)
     m = Mesh(nodes, cells)
     define_boundary_walls(m)
@@ -722,6 +729,9 @@ def main():
         model.set_reference_pressure(0, 0.0)
     apply_bcs(model.boundary, wl['kind'])
     model.finalize()
+    prep_s = time.perf_counter() - t_prep
+    print(f'# {model.layout.num_cells} cells prepared in {prep_s:.1f} s', file=sys.stderr,
+          flush=True)
     solver = IncompressibleSolver(model, 0.7, 0.3, implicit_outflow=(wl['kind'] == 'channel'))
     solver.cg_variant = _lib.CG_SINGLE_REDUCTION if args.cg == 'single' else _lib.CG_STANDARD
     dm = model.dm
@@ -771,6 +781,11 @@ def main():
                     args.traffic_copy_mb)
         roofline['traffic'] = t['bytes_per_launch']
         roofline['traffic_detail'] = t
+        # the same counters for every kernel with a byte count
+        for k, v in (kernels or {}).items():
+            if 'bytes_per_launch' in v:
+                tk = traffic(args.traffic_from, sym.get(k, 'k_' + k), args.traffic_copy_mb)
+                v['traffic'] = tk['bytes_per_launch']
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -798,7 +813,9 @@ def main():
         'krylov_iters_per_step': {'momentum': float(np.mean([h['u_iters'] for h in hist])),
                                   'pressure': float(np.mean([h['p_iters'] for h in hist]))},
         'krylov_launch': tuned,
+        'host_preprocessing_s': round(prep_s, 1),
         'kernels': kernels,
+        'kernels_note': KERNELS_NOTE,
     }
     print(json.dumps(out))
 
