@@ -212,6 +212,73 @@ int wcfd_assemble_pressure(const wcfd_mesh* m, wcfd_bc bc, const float* face_val
                            int compute_grad_p, int correction, int ref_cell, float ref_value,
                            float* values, float* rhs, float* div_out, void* stream);
 
+/* ------------------------------------------------- composable terms
+ * The reference's term-by-term API (FV/terms/*.py), for callers that build
+ * their own equations; the SIMPLE step uses the fused assembly above.
+ *
+ * Face-based term weights are the reference's Term.weights (C, 6, O, 3) in
+ * the layout w[O][3][6][C]: output o, part (0 owner-cell coefficient, 1
+ * neighbour coefficient, 2 explicit), local face k, cell c (device order).
+ * Cell-based weights: w[O][2][C] (implicit diagonal, explicit).  `outputs`:
+ * [host] n_outputs (1..4) global field indices (0..3). */
+#define WCFD_CONV_UPWIND 0        /* convectionScheme.py:28-52 (tie -> neighbour) */
+#define WCFD_CONV_CENTRAL 1       /* convectionScheme.py:3-26 */
+#define WCFD_CONV_UPWIND_LINEAR 2 /* convectionScheme.py:54-88 */
+
+/* ConvectionTerm.calculate_weights (convection.py:33-88): scheme weights
+ * times the face's mass flux, plus the skew correction of internal faces.
+ * face_c2f: [2][3][Fi] per internal face, x_f - x_owner and the owner's
+ * centroid-to-face vector of the neighbour's local face index (the vector
+ * upwindLinear reads when the neighbour is upwind, convectionScheme.py:82);
+ * required for WCFD_CONV_UPWIND_LINEAR only.  flags: WCFD_SCHEME_IMPLICIT_
+ * OUTFLOW. */
+int wcfd_convection_weights(const wcfd_mesh* m, wcfd_bc bc, const float* cell_grad,
+                            const float* face_values, const float* mass_flux,
+                            const float* face_c2f, int scheme, int flags, const int32_t* outputs,
+                            int n_outputs, float* weights, void* stream);
+
+/* DiffusionTerm.calculate_weights (diffusion.py:40-49, 52-197; laplacianScheme.py
+ * central_difference / central_difference_corrected).  viscosity_faces: [F]
+ * device face order, or NULL for the scalar `viscosity`.  bc_mode 0: the
+ * boundary conditions; 1: the von_neumann= (and dirchlet=, which the
+ * reference builds with the same kernel) option, explicit nu * bc_value * A
+ * on every boundary face, one output only. */
+int wcfd_diffusion_weights(const wcfd_mesh* m, wcfd_bc bc, const float* cell_grad,
+                           const float* face_values, const float* viscosity_faces,
+                           float viscosity, int correction, int bc_mode, float bc_value,
+                           const int32_t* outputs, int n_outputs, float* weights, void* stream);
+
+/* DdtTerm.calculate_weights, backwards_Euler (ddt.py:18-33): (V/dt, -phi V/dt)
+ * per output, indexed by local output. */
+int wcfd_ddt_weights(const wcfd_mesh* m, const float* cell_values, float dt,
+                     const int32_t* outputs, int n_outputs, float* weights, void* stream);
+
+/* GradTerm.calculate_weights (grad.py:17-28): w[d][c] = (grad phi_field)_c[d] V_c,
+ * an explicit term (component-major [3][C]). */
+int wcfd_grad_weights(const wcfd_mesh* m, const float* cell_grad, int field, float* weights,
+                      void* stream);
+
+/* Equation of n_outputs outputs: values[O][7][C] (one ELL plane per output:
+ * the reference's BSR rows O*c + o), rhs[O][C].  Equation.form_system
+ * (equation.py:99-124) with calculate_BSR_values_kernel and
+ * calculate_RHS_values_kernel (equation_kernels.py:52-101) for a face-based
+ * term, add_Implicit_cell_based_term_kernel (104-122) for a cell-based one,
+ * rhs += scale arr for an explicit term or add_RHS (equation.py:169-170). */
+int wcfd_equation_add_face_term(const wcfd_mesh* m, const float* weights, int n_outputs,
+                                float scale, float* values, float* rhs, void* stream);
+int wcfd_equation_add_cell_term(const wcfd_mesh* m, const float* weights, int n_outputs,
+                                float scale, float* values, float* rhs, void* stream);
+int wcfd_equation_add_rhs(int32_t n, const float* arr, float scale, float* rhs, void* stream);
+/* Equation.relax (equation.py:173-181; implicit_relaxation_kernel,
+ * equation_kernels.py:128-152) with the outputs' current cell values. */
+int wcfd_equation_relax(const wcfd_mesh* m, const float* cell_values, const int32_t* outputs,
+                        int n_outputs, float alpha, float* values, float* rhs, void* stream);
+/* Equation.replace_row (equation.py:184-193): identity row of (output, cell)
+ * with right-hand side `value` (the reference reads it from a length-1 array
+ * at index row_id, T7). */
+int wcfd_equation_replace_row(const wcfd_mesh* m, int n_outputs, int output, int32_t cell,
+                              float value, float* values, float* rhs, void* stream);
+
 /* ELL values [7][C] -> CSR values [nnz] on (row_ptr, col). */
 int wcfd_ell_to_csr(const wcfd_mesh* m, const float* ell_values, float* csr_values, void* stream);
 

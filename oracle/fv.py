@@ -177,8 +177,14 @@ class OracleFVM:
 
 
 # ------------------------------------------------------------------ terms
-def convection_weights(fvm, outputs=(0, 1, 2), implicit_outflow=False):
-    """convection.py:44-88 with convectionScheme.upwind (28-52).
+def convection_weights(fvm, outputs=(0, 1, 2), implicit_outflow=False, scheme='upwind'):
+    """convection.py:44-88 with convectionScheme.py: 'upwind' (28-52),
+    'central difference' (3-26: owner psi, neighbour 1 - psi) or
+    'upwindLinear' (54-88: upwind weights plus the explicit
+    grad(phi)_upwind . r of the global field, r = the owner's
+    centroid-to-face vector of local face j[0] when the owner is upwind and,
+    as the reference reads it, of local face j[1] -- the neighbour's index --
+    when the neighbour is).
 
     Internal explicit (skew) weights are computed like the reference
     (convection.py:76-78, local output index) but never reach b (T2).
@@ -200,9 +206,17 @@ def convection_weights(fvm, outputs=(0, 1, 2), implicit_outflow=False):
     k0, k1 = m.cell_face_index[fi, 0], m.cell_face_index[fi, 1]
     Fi = F[fi]
     up = Fi > 0                                         # T6: tie -> neighbour
-    w0 = np.where(up, dt.type(1), dt.type(0))
-    w1 = np.where(up, dt.type(0), dt.type(1))
     psi = fvm.psi[fi][:, None]; psi2 = fvm.psi2[fi][:, None]
+    if scheme == 'central difference':
+        w0, w1 = psi[:, 0].copy(), psi2[:, 0].copy()
+    elif scheme in ('upwind', 'upwindLinear'):
+        w0 = np.where(up, dt.type(1), dt.type(0))
+        w1 = np.where(up, dt.type(0), dt.type(1))
+    else:
+        raise ValueError(scheme)
+    if scheme == 'upwindLinear':
+        upc = np.where(up, own, nb)
+        r_up = np.where(up[:, None], fvm.c2f[own, k0], fvm.c2f[own, k1])
     rk = fvm.face_centroid[fi] - (fvm.centroids[own] * psi + fvm.centroids[nb] * psi2)
     bnd = fvm.boundary
     c2f = fvm.c2f[ob, kb]
@@ -217,6 +231,10 @@ def convection_weights(fvm, outputs=(0, 1, 2), implicit_outflow=False):
             w[ob[out], kb[out], lo, 2] = F[b][out] * (dmag_b[out] * fvm.face_gradients[b[out], go])
         gi = fvm.cell_gradients[own, lo] * psi + psi2 * fvm.cell_gradients[nb, lo]
         skew = (gi * rk).sum(axis=1, dtype=dt)
+        if scheme == 'upwindLinear':
+            gu = fvm.cell_gradients[upc, go]
+            lin = gu[:, 0] * r_up[:, 0] + gu[:, 1] * r_up[:, 1] + gu[:, 2] * r_up[:, 2]
+            skew = lin + skew
         w[own, k0, lo, 0] = w0 * Fi
         w[own, k0, lo, 1] = w1 * Fi
         w[own, k0, lo, 2] = skew * Fi

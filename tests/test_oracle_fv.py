@@ -130,6 +130,70 @@ def test_upwind_rows_closed_form():
     np.testing.assert_allclose(eq.rhs[0::3], [1.0, 0.0, 0.0], atol=1e-12)
 
 
+def _uniform_flow(n=(6, 5, 4), u=(0.7, -0.4, 0.25)):
+    """Uniform grid, uniform velocity u on every face (fluxes of both signs)."""
+    nodes, cells = structured_hex(n, (1.2, 1.0, 0.8))
+    f = _fvm(nodes, cells)
+    u = np.asarray(u, np.float64)
+    f.cell_values[:, :3] = u
+    bnd = f.boundary
+    bnd.values[:, :3] = u; bnd.types[:, :3] = 1; bnd.types[:, 3] = 2
+    f.set_boundary_conditions(); f.face_interpolation(); f.calculate_mass_flux()
+    return f, u
+
+
+def test_central_convection_of_quadratic_field_closed_form():
+    """Central weights (psi, 1 - psi) on a uniform grid in uniform flow: for
+    phi = x^2 the face averages are x_f^2 + h^2/4, and the opposite faces'
+    h^2/4 cancel, so an interior row of the assembled operator is exactly
+    V u_x 2 x_c.  Upwind misses it by V u_x h, so the closed form tells the
+    schemes apart."""
+    f, u = _uniform_flow()
+    x = f.centroids[:, 0]
+    phi = x * x
+    eq = OracleEquation(f, [0])
+    eq.form_system([('face', convection_weights(f, outputs=(0,), scheme='central difference'),
+                     1.0)])
+    interior = np.all(f.m.neighbors >= 0, axis=1)
+    assert interior.sum() > 10
+    exact = f.volumes * u[0] * 2.0 * x
+    np.testing.assert_allclose((eq.matrix() @ phi)[interior], exact[interior], rtol=1e-10,
+                               atol=1e-12)
+    eq.form_system([('face', convection_weights(f, outputs=(0,)), 1.0)])
+    h = 1.2 / 6
+    np.testing.assert_allclose((eq.matrix() @ phi)[interior],
+                               (exact - f.volumes * u[0] * h)[interior], rtol=1e-10, atol=1e-12)
+
+
+def test_upwind_linear_reconstructs_from_the_upwind_cell():
+    """upwindLinear: on every internal face, w0 phi_O + w1 phi_N + w2 (the
+    explicit part; the skew term vanishes on a uniform grid) equals
+    F (phi_up + grad(phi)_up . (x_f - x_up)), up = the upwind cell, for
+    fluxes of both signs.  phi = x^2 with its exact gradient 2x at each
+    centroid, so a gradient taken from the wrong cell fails."""
+    f, u = _uniform_flow()
+    x = f.centroids
+    f.cell_values[:, 0] = x[:, 0] ** 2
+    f.cell_gradients[:, 0] = 0.0
+    f.cell_gradients[:, 0, 0] = 2.0 * x[:, 0]
+    w = convection_weights(f, outputs=(0,), scheme='upwindLinear')
+    m = f.m
+    fi = m.internal_face_ids
+    own, nb = m.adjacent_cells[fi, 0], m.adjacent_cells[fi, 1]
+    k0, k1 = m.cell_face_index[fi, 0], m.cell_face_index[fi, 1]
+    F = f.mass_fluxes[fi]
+    assert (F > 0).any() and (F < 0).any()
+    upc = np.where(F > 0, own, nb)
+    xf = f.face_centroid[fi]
+    expect = F * (x[upc, 0] ** 2 + 2.0 * x[upc, 0] * (xf[:, 0] - x[upc, 0]))
+    phi = f.cell_values[:, 0]
+    got_own = w[own, k0, 0, 0] * phi[own] + w[own, k0, 0, 1] * phi[nb] + w[own, k0, 0, 2]
+    np.testing.assert_allclose(got_own, expect, rtol=1e-12, atol=1e-12)
+    # the neighbour's row carries the same face with the flux negated
+    got_nb = w[nb, k1, 0, 0] * phi[nb] + w[nb, k1, 0, 1] * phi[own] + w[nb, k1, 0, 2]
+    np.testing.assert_allclose(got_nb, -expect, rtol=1e-12, atol=1e-12)
+
+
 def test_upwind_tie_goes_to_neighbour():
     """T6: F == 0 puts the implicit weight on the neighbour column."""
     f = _line(2)

@@ -16,6 +16,7 @@ WCFD_OK, WCFD_EINVAL, WCFD_EHIP, WCFD_ENOTCONV, WCFD_ENAN = 0, 1, 2, 3, 4
 DIRICHLET, NEUMANN = 1, 2
 SCHEME_CORRECTED, SCHEME_IMPLICIT_OUTFLOW = 1, 2
 CG_STANDARD, CG_SINGLE_REDUCTION = 0, 1
+CONV_UPWIND, CONV_CENTRAL, CONV_UPWIND_LINEAR = 0, 1, 2
 MONITOR_LEN = 48
 
 c_p = ctypes.c_void_p
@@ -76,6 +77,7 @@ class SimpleInfo(ctypes.Structure):
 
 
 _PM = ctypes.POINTER(WcfdMesh)
+_PI32 = ctypes.POINTER(ctypes.c_int32)
 _PB = ctypes.POINTER(WcfdBatch)
 _SIGS = {
     'wcfd_last_hip_error': (ctypes.c_int, []),
@@ -92,6 +94,18 @@ _SIGS = {
     'wcfd_assemble_pressure': (ctypes.c_int, [_PM, WcfdBC, c_p, c_p, c_p, c_p, ctypes.c_int,
                                               ctypes.c_int, ctypes.c_int, c_f, c_p, c_p, c_p, c_p]),
     'wcfd_ell_to_csr': (ctypes.c_int, [_PM, c_p, c_p, c_p]),
+    'wcfd_convection_weights': (ctypes.c_int, [_PM, WcfdBC, c_p, c_p, c_p, c_p, ctypes.c_int,
+                                               ctypes.c_int, _PI32, ctypes.c_int, c_p, c_p]),
+    'wcfd_diffusion_weights': (ctypes.c_int, [_PM, WcfdBC, c_p, c_p, c_p, c_f, ctypes.c_int,
+                                              ctypes.c_int, c_f, _PI32, ctypes.c_int, c_p, c_p]),
+    'wcfd_ddt_weights': (ctypes.c_int, [_PM, c_p, c_f, _PI32, ctypes.c_int, c_p, c_p]),
+    'wcfd_grad_weights': (ctypes.c_int, [_PM, c_p, ctypes.c_int, c_p, c_p]),
+    'wcfd_equation_add_face_term': (ctypes.c_int, [_PM, c_p, ctypes.c_int, c_f, c_p, c_p, c_p]),
+    'wcfd_equation_add_cell_term': (ctypes.c_int, [_PM, c_p, ctypes.c_int, c_f, c_p, c_p, c_p]),
+    'wcfd_equation_add_rhs': (ctypes.c_int, [ctypes.c_int32, c_p, c_f, c_p, c_p]),
+    'wcfd_equation_relax': (ctypes.c_int, [_PM, c_p, _PI32, ctypes.c_int, c_f, c_p, c_p, c_p]),
+    'wcfd_equation_replace_row': (ctypes.c_int, [_PM, ctypes.c_int, ctypes.c_int, ctypes.c_int32,
+                                                 c_f, c_p, c_p, c_p]),
     'wcfd_spmv_ell': (ctypes.c_int, [_PM, c_p, ctypes.c_int, c_p, c_p, c_p]),
     'wcfd_cg_ell': (ctypes.c_int, [_PM, c_p, ctypes.c_int, c_p, c_p, c_d, c_d, ctypes.c_int,
                                    ctypes.c_int, c_p, ctypes.POINTER(SolveInfo), c_p]),

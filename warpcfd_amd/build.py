@@ -8,7 +8,8 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SOURCES = ['csrc/fv_kernels.hip', 'csrc/krylov.hip', 'csrc/simple.hip', 'csrc/profile.cpp']
+SOURCES = ['csrc/fv_kernels.hip', 'csrc/krylov.hip', 'csrc/simple.hip', 'csrc/terms.hip',
+           'csrc/profile.cpp']
 OUT = os.path.join(HERE, 'libwcfd.so')
 ARCH = os.environ.get('WCFD_OFFLOAD_ARCH', 'gfx950')
 

@@ -41,7 +41,8 @@ extern thread_local int g_last_hip_error;
 enum KernelId {
   K_SET_BC, K_FACE_INTERP, K_GREEN_GAUSS, K_MASS_FLUX, K_DIVERGENCE, K_ASM_MOM, K_ASM_P,
   K_RUA_HBYA, K_RELAX_P, K_FACE_P_FC, K_GRAD_P_CORR, K_INV_DIAG, K_SPMV, K_DOT, K_CG_INIT,
-  K_CG_1, K_CG_2, K_BI_INIT, K_BI_1, K_BI_2, K_BI_3, K_FOLD, K_ELL_TO_CSR, K_COUNT
+  K_CG_1, K_CG_2, K_BI_INIT, K_BI_1, K_BI_2, K_BI_3, K_FOLD, K_ELL_TO_CSR, K_TERM_WEIGHTS,
+  K_EQUATION, K_COUNT
 };
 bool prof_active();
 void prof_record(int id, hipStream_t s, bool start);

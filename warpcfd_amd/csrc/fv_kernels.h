@@ -4,6 +4,24 @@
 
 namespace wcfd {
 
+// Laplacian factor of internal face f: 1/|d| (laplacianScheme.py:14-37), or
+// with `correction` the over-relaxed |n1|/|d|, n1 = d / (n_f . d)
+// (laplacianScheme.py:39-66).
+__device__ inline float lap_factor(const wcfd_mesh& m, int f, int correction) {
+  const float dm = m.dmag[f];
+  if (!correction) return 1.0f / dm;
+  const int Fi = m.n_internal, F = m.n_faces;
+  const float d0 = m.dist[f], d1 = m.dist[Fi + f], d2 = m.dist[2 * Fi + f];
+  const float n0 = m.normal[f], n1 = m.normal[F + f], n2 = m.normal[2 * F + f];
+  const float inv_nd = 1.0f / (n0 * d0 + n1 * d1 + n2 * d2);
+  const float e0 = d0 * inv_nd, e1 = d1 * inv_nd, e2 = d2 * inv_nd;
+  const float n1m = sqrtf(e0 * e0 + e1 * e1 + e2 * e2);
+  return 1.0f / dm * n1m;
+}
+
+// a mesh descriptor with every array the face and cell kernels read
+bool mesh_ok(const wcfd_mesh* m);
+
 // Face and cell operators (fv_kernels.hip).  `bd` carries a batch's per-cell
 // and per-face case lookups (all NULL for a single case).
 int launch_set_bc(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc, float* fv, int mask,

@@ -167,18 +167,6 @@ __global__ void k_divergence(wcfd_mesh m, const float* __restrict__ flux, float*
 // Laplacian face coefficient factors (laplacianScheme.py:14-66): returns
 // (1/|d|) or, with the over-relaxed correction, |n1|/|d| where
 // n1 = d / (n_f . d).
-__device__ inline float lap_factor(const wcfd_mesh& m, int f, int correction) {
-  const float dm = m.dmag[f];
-  if (!correction) return 1.0f / dm;
-  const int Fi = m.n_internal, F = m.n_faces;
-  const float d0 = m.dist[f], d1 = m.dist[Fi + f], d2 = m.dist[2 * Fi + f];
-  const float n0 = m.normal[f], n1 = m.normal[F + f], n2 = m.normal[2 * F + f];
-  const float inv_nd = 1.0f / (n0 * d0 + n1 * d1 + n2 * d2);
-  const float e0 = d0 * inv_nd, e1 = d1 * inv_nd, e2 = d2 * inv_nd;
-  const float n1m = sqrtf(e0 * e0 + e1 * e1 + e2 * e2);
-  return 1.0f / dm * n1m;
-}
-
 // ------------------------------------------------------ momentum assembly
 __global__ void k_assemble_momentum(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
                                     const float* __restrict__ cv, const float* __restrict__ fv,
@@ -614,7 +602,7 @@ int launch_correct_velocity(const wcfd_mesh& m, const BatchDev& bd, const Seg& s
 // ============================================================== C ABI
 using namespace wcfd;
 
-static bool mesh_ok(const wcfd_mesh* m) {
+bool wcfd::mesh_ok(const wcfd_mesh* m) {
   return m && m->n_cells > 0 && m->n_faces >= m->n_internal && m->n_internal >= 0 &&
          m->n_boundary == m->n_faces - m->n_internal && m->owner && m->slot_face && m->volume;
 }
