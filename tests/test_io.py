@@ -1,0 +1,191 @@
+"""Mesh files and result export (warpcfd_amd.preprocess.io, warpcfd_amd.postprocess).
+
+* hand-written gmsh files (format 2.2 and 4.1) of two unit hexahedra with
+  physical groups: nodes, cells in VTK order and groups read as written;
+* round trips through write_msh / read_msh and write_vtu / read_vtu (ascii,
+  inline binary, zlib, appended raw) on a jittered box: identical points,
+  cells, groups and cell data;
+* a mesh read from a file has the same faces, owners and CSR pattern as
+  the mesh built from the arrays;
+* line sampling on a uniform grid returns the containing cells' values
+  (closed form: cell index from floor(x / h)).
+"""
+import struct
+
+import numpy as np
+import pytest
+
+from warpcfd_amd.postprocess import locate_cells, sample_line
+from warpcfd_amd.preprocess import Mesh, define_boundary_walls, structured_hex
+from warpcfd_amd.preprocess.io import (MeshData, apply_groups, face_groups_of, read_msh,
+                                       read_vtu, write_msh, write_vtu)
+
+# two unit cubes along x; gmsh hexahedron node order = VTK's
+NODES = [(0, 0, 0), (1, 0, 0), (2, 0, 0), (0, 1, 0), (1, 1, 0), (2, 1, 0),
+         (0, 0, 1), (1, 0, 1), (2, 0, 1), (0, 1, 1), (1, 1, 1), (2, 1, 1)]
+HEX = [[0, 1, 4, 3, 6, 7, 10, 9], [1, 2, 5, 4, 7, 8, 11, 10]]
+
+MSH2 = """$MeshFormat
+2.2 0 8
+$EndMeshFormat
+$PhysicalNames
+3
+2 1 "inlet"
+2 2 "outlet"
+3 3 "fluid"
+$EndPhysicalNames
+$Nodes
+12
+""" + ''.join(f'{i + 11} {x} {y} {z}\n' for i, (x, y, z) in enumerate(NODES)) + """$EndNodes
+$Elements
+4
+1 3 2 1 7 11 14 20 17
+2 3 2 2 8 13 16 22 19
+3 5 2 3 1 11 12 15 14 17 18 21 20
+4 5 2 3 1 12 13 16 15 18 19 22 21
+$EndElements
+"""
+
+MSH4 = """$MeshFormat
+4.1 0 8
+$EndMeshFormat
+$PhysicalNames
+3
+2 1 "inlet"
+2 2 "outlet"
+3 3 "fluid"
+$EndPhysicalNames
+$Entities
+0 0 2 1
+7 0 0 0 0 1 1 1 1 0
+8 2 0 0 2 1 1 1 2 0
+1 0 0 0 2 1 1 1 3 0
+$EndEntities
+$Nodes
+1 12 11 22
+3 1 0 12
+""" + ''.join(f'{i + 11}\n' for i in range(12)) + ''.join(f'{x} {y} {z}\n' for x, y, z in NODES) + """$EndNodes
+$Elements
+3 4 1 4
+2 7 3 1
+1 11 14 20 17
+2 8 3 1
+2 13 16 22 19
+3 1 5 2
+3 11 12 15 14 17 18 21 20
+4 12 13 16 15 18 19 22 21
+$EndElements
+"""
+
+
+def _check_two_cubes(d):
+    assert isinstance(d, MeshData)
+    np.testing.assert_array_equal(d.points, np.array(NODES, float))
+    np.testing.assert_array_equal(d.hexes, np.array(HEX))
+    assert set(d.groups) == {'inlet', 'outlet', 'fluid'}
+    assert d.groups['fluid'][0] == 3 and list(d.groups['fluid'][1]) == [0, 1]
+    m = Mesh(d)
+    apply_groups(m, d)
+    fc = m.face_properties.centroid
+    (fi,) = m.groups['inlet'].ids
+    (fo,) = m.groups['outlet'].ids
+    assert np.allclose(fc[fi], [0.0, 0.5, 0.5]) and np.allclose(fc[fo], [2.0, 0.5, 0.5])
+    assert m.groups['fluid'].type == 'cell'
+
+
+@pytest.mark.parametrize('text', [MSH2, MSH4], ids=['msh2.2', 'msh4.1'])
+def test_read_handwritten_msh(tmp_path, text):
+    p = tmp_path / 'two.msh'
+    p.write_text(text)
+    _check_two_cubes(read_msh(str(p)))
+
+
+def _box():
+    nodes, cells = structured_hex((5, 4, 3), (1.0, 0.8, 0.6), jitter=0.1, seed=2)
+    m = Mesh(nodes, cells)
+    define_boundary_walls(m)
+    return nodes, cells, m
+
+
+@pytest.mark.parametrize('version', [2.2, 4.1])
+def test_msh_round_trip(tmp_path, version):
+    nodes, cells, m = _box()
+    p = tmp_path / 'box.msh'
+    write_msh(str(p), nodes, cells, face_groups=face_groups_of(m), version=version,
+              cell_groups={'left': np.arange(0, 30), 'right': np.arange(30, len(cells))})
+    d = read_msh(str(p))
+    np.testing.assert_array_equal(d.points, np.asarray(nodes, np.float64))
+    np.testing.assert_array_equal(d.hexes, cells)
+    m2 = Mesh(d)
+    apply_groups(m2, d)
+    for name, g in m.groups.items():
+        assert np.array_equal(np.sort(m2.groups[name].ids), np.sort(g.ids)), name
+    assert np.array_equal(m2.groups['left'].ids, np.arange(30))
+    # same mesh: faces, owners, local indices and the CSR pattern
+    fa, fb = m.face_properties, m2.face_properties
+    np.testing.assert_array_equal(fa.unique_faces, fb.unique_faces)
+    np.testing.assert_array_equal(fa.adjacent_cells, fb.adjacent_cells)
+    for a, b in zip(m.csr_pattern(3), m2.csr_pattern(3)):
+        np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize('binary,compress', [(False, False), (True, False), (True, True)])
+def test_vtu_round_trip(tmp_path, binary, compress):
+    nodes, cells, m = _box()
+    rng = np.random.default_rng(3)
+    data = {'p': rng.standard_normal(len(cells)).astype(np.float32),
+            'velocity': rng.standard_normal((len(cells), 3)),
+            'id': np.arange(len(cells), dtype=np.int32)}
+    p = tmp_path / 'box.vtu'
+    write_vtu(str(p), nodes, cells, data, binary=binary, compress=compress)
+    d = read_vtu(str(p))
+    np.testing.assert_array_equal(d.points, np.asarray(nodes, np.float64))
+    np.testing.assert_array_equal(d.hexes, cells)
+    for k, v in data.items():
+        assert d.cell_data[k].dtype == v.dtype
+        np.testing.assert_array_equal(d.cell_data[k], v)
+
+
+def test_vtu_appended_raw(tmp_path):
+    """A VTK-style file with raw appended data (UInt32 headers)."""
+    pts = np.array(NODES, np.float32).reshape(-1)
+    conn = np.array(HEX, np.int32).reshape(-1)
+    offs = np.array([8, 16], np.int32)
+    types = np.array([12, 12], np.uint8)
+    blobs, offsets, pos = [], [], 0
+    for a in (pts, conn, offs, types):
+        b = struct.pack('<I', a.nbytes) + a.tobytes()
+        offsets.append(pos)
+        pos += len(b)
+        blobs.append(b)
+    head = ('<?xml version="1.0"?>\n<VTKFile type="UnstructuredGrid" version="0.1" '
+            'byte_order="LittleEndian" header_type="UInt32">\n<UnstructuredGrid>\n'
+            '<Piece NumberOfPoints="12" NumberOfCells="2">\n<Points>\n'
+            f'<DataArray type="Float32" NumberOfComponents="3" format="appended" offset="{offsets[0]}"/>\n'
+            '</Points>\n<Cells>\n'
+            f'<DataArray type="Int32" Name="connectivity" format="appended" offset="{offsets[1]}"/>\n'
+            f'<DataArray type="Int32" Name="offsets" format="appended" offset="{offsets[2]}"/>\n'
+            f'<DataArray type="UInt8" Name="types" format="appended" offset="{offsets[3]}"/>\n'
+            '</Cells>\n</Piece>\n</UnstructuredGrid>\n<AppendedData encoding="raw">\n_')
+    p = tmp_path / 'app.vtu'
+    p.write_bytes(head.encode() + b''.join(blobs) + b'\n</AppendedData>\n</VTKFile>\n')
+    d = read_vtu(str(p))
+    np.testing.assert_array_equal(d.points, np.array(NODES, np.float64))
+    np.testing.assert_array_equal(d.hexes, np.array(HEX))
+
+
+def test_sample_line_uniform_grid():
+    n = 8
+    nodes, cells = structured_hex((n, n, 1), (1.0, 1.0, 0.1))
+    m = Mesh(nodes, cells)
+    vals = np.arange(n * n, dtype=np.float64)           # cells are x-fastest: id = i + n j
+    x = (np.arange(n) + 0.5) / n
+    pts, got, ids = sample_line(m, vals, (x[0], 0.3, 0.05), (x[-1], 0.3, 0.05), n)
+    j = int(np.floor(0.3 * n))
+    np.testing.assert_array_equal(ids, np.arange(n) + n * j)
+    np.testing.assert_array_equal(got, vals[np.arange(n) + n * j])
+    # a point on the face shared by two cells takes the lower id; outside -> NaN
+    ids2 = locate_cells(m, [[0.5, 0.3, 0.05], [1.5, 0.3, 0.05]])
+    assert ids2[0] == n * j + n // 2 - 1 and ids2[1] == -1
+    _, got2, _ = sample_line(m, {'v': vals}, (0.5, 0.3, 0.05), (1.5, 0.3, 0.05), 2)
+    assert got2['v'][0] == vals[n * j + n // 2 - 1] and np.isnan(got2['v'][1])
