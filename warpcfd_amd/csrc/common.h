@@ -95,6 +95,13 @@ inline unsigned grid_for(int64_t n, int block = kBlock) {
   return static_cast<unsigned>(g < 1 ? 1 : g);
 }
 
+// Workgroups of a one-row-per-thread reduction over n cells of kBlock rows,
+// at least kMaxReduceBlocks (the partial rows a workspace holds).
+inline int64_t cell_reduce_blocks(int64_t n) {
+  const int64_t g = (n + kBlock - 1) / kBlock;
+  return g > kMaxReduceBlocks ? g : kMaxReduceBlocks;
+}
+
 inline unsigned reduce_grid(int64_t n, int block = kBlock) {
   int64_t g = (n + block - 1) / block;
   if (g > kMaxReduceBlocks) g = kMaxReduceBlocks;
@@ -140,6 +147,11 @@ __device__ __forceinline__ void st_off(T* base, uint32_t byte_off, T v) {
 #else
   *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off) = v;
 #endif
+}
+// an ordinary store at a uniform base plus a 32-bit byte offset
+template <class T>
+__device__ __forceinline__ void st_plain(T* base, uint32_t byte_off, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off) = v;
 }
 // matrix stream (read once per iteration)
 template <class T>

@@ -19,6 +19,22 @@ __device__ inline float lap_factor(const wcfd_mesh& m, int f, int correction) {
   return 1.0f / dm * n1m;
 }
 
+// The same factor with the face arrays read through buffer descriptors at
+// byte offset f4 = 4 f (dmag, and dist / normal planes when corrected).
+__device__ inline float lap_factor_b(const wcfd_mesh& m, const FBuf& dmag, uint32_t f4,
+                                     int correction) {
+  const float dm = dmag(f4);
+  if (!correction) return 1.0f / dm;
+  const uint32_t Fi4 = (uint32_t)m.n_internal * 4u, F4 = (uint32_t)m.n_faces * 4u;
+  const FBuf dist(m.dist, 3u * Fi4), nrm(m.normal, 3u * F4);
+  const float d0 = dist(f4), d1 = dist(Fi4 + f4), d2 = dist(2u * Fi4 + f4);
+  const float n0 = nrm(f4), n1 = nrm(F4 + f4), n2 = nrm(2u * F4 + f4);
+  const float inv_nd = 1.0f / (n0 * d0 + n1 * d1 + n2 * d2);
+  const float e0 = d0 * inv_nd, e1 = d1 * inv_nd, e2 = d2 * inv_nd;
+  const float n1m = sqrtf(e0 * e0 + e1 * e1 + e2 * e2);
+  return 1.0f / dm * n1m;
+}
+
 // a mesh descriptor with every array the face and cell kernels read
 bool mesh_ok(const wcfd_mesh* m);
 

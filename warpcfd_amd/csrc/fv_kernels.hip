@@ -32,6 +32,15 @@ __global__ void k_set_bc(BatchDev bd, int F, int Fi, int Fb, const uint8_t* __re
 // --------------------------------------------------- face interpolation
 // interpolation_kernels.py:5-54, interpolation_functions.py:6-14,
 // skewness_functions.py:7-20, model_kernels.py:46-60, solver_kernels.py:22-36.
+// k_face_p_fc gathers its two cells' values through buffer descriptors at
+// the cells' byte offsets (one offset register per cell, shared by every
+// cell array) and streams the face arrays at the face's byte offset; for
+// k_face_interp that form measured 3% slower at 8.6 M cells, so it keeps
+// plain indexing.
+__device__ inline FBuf cell_plane(const wcfd_mesh& m, const float* a, int q) {
+  return FBuf(a + (size_t)q * m.n_cells, (uint32_t)m.n_cells * 4u);
+}
+
 __global__ void k_face_interp(wcfd_mesh m, BatchDev bd, wcfd_bc bc, const float* __restrict__ cv,
                               const float* __restrict__ cg, float* __restrict__ fv,
                               int mask, int skew_mask, float* __restrict__ flux,
@@ -94,43 +103,57 @@ __global__ void k_face_interp(wcfd_mesh m, BatchDev bd, wcfd_bc bc, const float*
 }
 
 // ------------------------------------------------------------ Green-Gauss
+// Cell kernels read their cell's arrays at a uniform base plus the cell's
+// byte offset c4 = 4 c, and the faces' arrays through buffer descriptors at
+// f4 = 4 f, so the six faces of a cell need six offset registers however
+// many face arrays are read (rather than a 64-bit address per array and
+// face).
+
+// the six faces of cell c as byte offsets
+__device__ inline void slot_faces(const wcfd_mesh& m, uint32_t c4, uint32_t (&f4)[6]) {
+  const size_t C = m.n_cells;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) f4[k] = (uint32_t)ld_off(m.slot_face + k * C, c4) * 4u;
+}
+
 // model_kernels.py:72-90: grad_c = sum_k phi_f A_f n_{c,k} / V_c.
-__device__ inline void gg_field(const wcfd_mesh& m, int c, const int (&fk)[6],
-                                const float* __restrict__ phi_f, float V, float (&g)[3]) {
-  const int C = m.n_cells;
+__device__ inline void gg_field(const wcfd_mesh& m, uint32_t c4, const uint32_t (&f4)[6],
+                                const FBuf& phi_f, float V, float (&g)[3]) {
+  const size_t C = m.n_cells;
   float a0 = 0.f, a1 = 0.f, a2 = 0.f;
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
-    const float phi = phi_f[fk[k]];
-    a0 = a0 + (phi * m.slot_sn[(0 * 6 + k) * C + c]) / V;
-    a1 = a1 + (phi * m.slot_sn[(1 * 6 + k) * C + c]) / V;
-    a2 = a2 + (phi * m.slot_sn[(2 * 6 + k) * C + c]) / V;
+    const float phi = phi_f(f4[k]);
+    a0 = a0 + (phi * ld_off(m.slot_sn + (0 * 6 + k) * C, c4)) / V;
+    a1 = a1 + (phi * ld_off(m.slot_sn + (1 * 6 + k) * C, c4)) / V;
+    a2 = a2 + (phi * ld_off(m.slot_sn + (2 * 6 + k) * C, c4)) / V;
   }
   g[0] = a0; g[1] = a1; g[2] = a2;
+}
+
+__device__ inline FBuf face_plane(const wcfd_mesh& m, const float* a, int q) {
+  return FBuf(a + (size_t)q * m.n_faces, (uint32_t)m.n_faces * 4u);
 }
 
 __global__ void k_green_gauss(wcfd_mesh m, Items items, const float* __restrict__ fv,
                               float* __restrict__ cg, int mask, int zero_others) {
   const int c = item_index(items);
-  const int C = m.n_cells, F = m.n_faces;
+  const int C = m.n_cells;
   if (c >= C) return;
-  int fk[6];
-#pragma unroll
-  for (int k = 0; k < 6; ++k) fk[k] = m.slot_face[k * C + c];
-  const float V = m.volume[c];
+  const uint32_t c4 = (uint32_t)c * 4u;
+  uint32_t f4[6];
+  slot_faces(m, c4, f4);
+  const float V = ld_off(m.volume, c4);
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
+    float g[3] = {0.f, 0.f, 0.f};
     if ((mask >> q) & 1) {
-      float g[3];
-      gg_field(m, c, fk, fv + (size_t)q * F, V, g);
-      cg[(3 * q + 0) * C + c] = g[0];
-      cg[(3 * q + 1) * C + c] = g[1];
-      cg[(3 * q + 2) * C + c] = g[2];
-    } else if (zero_others) {
-      cg[(3 * q + 0) * C + c] = 0.f;
-      cg[(3 * q + 1) * C + c] = 0.f;
-      cg[(3 * q + 2) * C + c] = 0.f;
+      gg_field(m, c4, f4, face_plane(m, fv, q), V, g);
+    } else if (!zero_others) {
+      continue;
     }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) st_plain(cg + (size_t)(3 * q + d) * C, c4, g[d]);
   }
 }
 
@@ -142,14 +165,14 @@ __global__ void k_mass_flux(wcfd_mesh m, const float* __restrict__ fv, float* __
             m.area[f];
 }
 
-__device__ inline float cell_div(const wcfd_mesh& m, int c, const int (&fk)[6],
-                                 const float* __restrict__ flux) {
-  const int C = m.n_cells;
+__device__ inline float cell_div(const wcfd_mesh& m, uint32_t c4, const uint32_t (&f4)[6],
+                                 const FBuf& flux) {
+  const size_t C = m.n_cells;
   float acc = 0.f;
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
-    const float fl = flux[fk[k]];
-    acc = acc + (m.slot_side[k * C + c] == 0 ? fl : -fl);
+    const float fl = flux(f4[k]);
+    acc = acc + (ld_off(m.slot_side + k * C, c4 >> 2) == 0 ? fl : -fl);
   }
   return acc;
 }
@@ -158,15 +181,12 @@ __global__ void k_divergence(wcfd_mesh m, const float* __restrict__ flux, float*
   const int c = xcd_item();
   const int C = m.n_cells;
   if (c >= C) return;
-  int fk[6];
-#pragma unroll
-  for (int k = 0; k < 6; ++k) fk[k] = m.slot_face[k * C + c];
-  div[c] = cell_div(m, c, fk, flux);
+  const uint32_t c4 = (uint32_t)c * 4u;
+  uint32_t f4[6];
+  slot_faces(m, c4, f4);
+  st_plain(div, c4, cell_div(m, c4, f4, FBuf(flux, (uint32_t)m.n_faces * 4u)));
 }
 
-// Laplacian face coefficient factors (laplacianScheme.py:14-66): returns
-// (1/|d|) or, with the over-relaxed correction, |n1|/|d| where
-// n1 = d / (n_f . d).
 // ------------------------------------------------------ momentum assembly
 __global__ void k_assemble_momentum(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
                                     const float* __restrict__ cv, const float* __restrict__ fv,
@@ -177,31 +197,38 @@ __global__ void k_assemble_momentum(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
   const int correction = flags & WCFD_SCHEME_CORRECTED;
   const bool implicit_outflow = (flags & WCFD_SCHEME_IMPLICIT_OUTFLOW) != 0;
   const int c = item_index(bd.cells);
-  const int C = m.n_cells, F = m.n_faces, Fi = m.n_internal, Fb = m.n_boundary;
+  const int C = m.n_cells, Fi = m.n_internal, Fb = m.n_boundary;
   if (c >= C || bd.cell_skip(c)) return;
   const float nu = bd.nu ? bd.nu[bd.cell_case[c]] : nu_scalar;
-  int fk[6];
+  const uint32_t c4 = (uint32_t)c * 4u, Fi4 = (uint32_t)Fi * 4u;
+  const size_t Cz = C;
+  uint32_t f4[6];
+  slot_faces(m, c4, f4);
   int side[6];
 #pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    fk[k] = m.slot_face[k * C + c];
-    side[k] = m.slot_side[k * C + c];
-  }
-  const float V = m.volume[c];
+  for (int k = 0; k < 6; ++k) side[k] = ld_off(m.slot_side + k * Cz, (uint32_t)c);
+  const FBuf Fl_(flux, (uint32_t)m.n_faces * 4u);
+  const FBuf area(m.area, (uint32_t)m.n_faces * 4u), dmag(m.dmag, (uint32_t)m.n_faces * 4u);
+  const float V = ld_off(m.volume, c4);
   float gp[3];
   if (compute_grad) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       float g[3];
-      gg_field(m, c, fk, fv + (size_t)q * F, V, g);
-      cg[(3 * q + 0) * C + c] = g[0];
-      cg[(3 * q + 1) * C + c] = g[1];
-      cg[(3 * q + 2) * C + c] = g[2];
+      gg_field(m, c4, f4, face_plane(m, fv, q), V, g);
+      st_plain(cg + (3 * q + 0) * Cz, c4, g[0]);
+      st_plain(cg + (3 * q + 1) * Cz, c4, g[1]);
+      st_plain(cg + (3 * q + 2) * Cz, c4, g[2]);
       if (q == 3) { gp[0] = g[0]; gp[1] = g[1]; gp[2] = g[2]; }
     }
   } else {
-    gp[0] = cg[9 * C + c]; gp[1] = cg[10 * C + c]; gp[2] = cg[11 * C + c];
+    gp[0] = ld_off(cg + 9 * Cz, c4);
+    gp[1] = ld_off(cg + 10 * Cz, c4);
+    gp[2] = ld_off(cg + 11 * Cz, c4);
   }
+  float u[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) u[q] = ld_off(cv + q * Cz, c4);
   float diag = 0.f;
   float b[3] = {0.f, 0.f, 0.f};
   float off[6];
@@ -210,7 +237,7 @@ __global__ void k_assemble_momentum(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
     const float vdt = V / dt;
     diag = diag + vdt;
 #pragma unroll
-    for (int q = 0; q < 3; ++q) b[q] = b[q] + (-(-cv[q * C + c] * vdt));
+    for (int q = 0; q < 3; ++q) b[q] = b[q] + (-(-u[q] * vdt));
   }
   // ConvectionTerm (scale +1), upwind, tie -> neighbour (T6).  Boundary
   // faces go to b with the lagged face value (convection.py:63-67); with
@@ -219,10 +246,9 @@ __global__ void k_assemble_momentum(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
   unsigned outflow = 0u;
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
-    const int f = fk[k];
-    const float Fl = flux[f];
+    const float Fl = Fl_(f4[k]);
     off[k] = 0.f;
-    if (f < Fi) {
+    if (f4[k] < Fi4) {
       const bool up = Fl > 0.f;
       if (side[k] == 0) {
         diag = diag + (up ? Fl : 0.f);
@@ -231,44 +257,44 @@ __global__ void k_assemble_momentum(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
         diag = diag + (up ? 0.f : -Fl);
         off[k] = up ? -Fl : 0.f;
       }
-    } else if (implicit_outflow && Fl > 0.f && bc.type[f - Fi] == WCFD_NEUMANN) {
+    } else if (implicit_outflow && Fl > 0.f && bc.type[(f4[k] - Fi4) >> 2] == WCFD_NEUMANN) {
       diag = diag + Fl;
       outflow |= 1u << k;
     }
   }
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
-    const int f = fk[k];
-    if (f >= Fi) {
-      const float Fl = flux[f];
+    if (f4[k] >= Fi4) {
+      const float Fl = Fl_(f4[k]);
       if ((outflow >> k) & 1u) {
-        const int j = f - Fi;
+        const int j = (int)((f4[k] - Fi4) >> 2);
+        const float dm = dmag(f4[k]);
 #pragma unroll
-        for (int q = 0; q < 3; ++q) b[q] = b[q] + (-(Fl * (m.dmag[f] * bc.value[q * Fb + j])));
+        for (int q = 0; q < 3; ++q) b[q] = b[q] + (-(Fl * (dm * bc.value[q * Fb + j])));
       } else {
 #pragma unroll
-        for (int q = 0; q < 3; ++q) b[q] = b[q] + (-(Fl * fv[q * F + f]));
+        for (int q = 0; q < 3; ++q) b[q] = b[q] + (-(Fl * face_plane(m, fv, q)(f4[k])));
       }
     }
   }
   // DiffusionTerm (scale -1)
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
-    const int f = fk[k];
-    if (f < Fi) {
-      const float w = lap_factor(m, f, correction) * (nu * m.area[f]);
+    if (f4[k] < Fi4) {
+      const float w = lap_factor_b(m, dmag, f4[k], correction) * (nu * area(f4[k]));
       diag = diag + w;
       off[k] = off[k] + (-w);
     } else {
-      const int j = f - Fi;
+      const int j = (int)((f4[k] - Fi4) >> 2);
       if (bc.type[j] == WCFD_NEUMANN) {
+        const float A = area(f4[k]);
 #pragma unroll
-        for (int q = 0; q < 3; ++q) b[q] = b[q] + (nu * bc.value[q * Fb + j]) * m.area[f];
+        for (int q = 0; q < 3; ++q) b[q] = b[q] + (nu * bc.value[q * Fb + j]) * A;
       } else {
-        const float w = (nu * m.area[f]) / m.dmag[f];
+        const float w = (nu * area(f4[k])) / dmag(f4[k]);
         diag = diag + w;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) b[q] = b[q] + w * fv[q * F + f];
+        for (int q = 0; q < 3; ++q) b[q] = b[q] + w * face_plane(m, fv, q)(f4[k]);
       }
     }
   }
@@ -279,15 +305,15 @@ __global__ void k_assemble_momentum(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
   if (alpha > 0.f) {
     const float r = (1.0f - alpha) / alpha;
 #pragma unroll
-    for (int q = 0; q < 3; ++q) b[q] = b[q] + r * diag * cv[q * C + c];
+    for (int q = 0; q < 3; ++q) b[q] = b[q] + r * diag * u[q];
     diag = diag / alpha;
   }
   // slot-major (ELL) matrix: diagonal, then the 6 slot coefficients
-  values[c] = diag;
+  st_plain(values, c4, diag);
 #pragma unroll
-  for (int k = 0; k < 6; ++k) values[(k + 1) * C + c] = off[k];
+  for (int k = 0; k < 6; ++k) st_plain(values + (k + 1) * Cz, c4, off[k]);
 #pragma unroll
-  for (int q = 0; q < 3; ++q) rhs[q * C + c] = b[q];
+  for (int q = 0; q < 3; ++q) st_plain(rhs + q * Cz, c4, b[q]);
 }
 
 // ------------------------------------------------------ pressure assembly
@@ -299,43 +325,45 @@ __global__ void k_assemble_pressure(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
                                     float ref_value, float* __restrict__ values,
                                     float* __restrict__ rhs, float* __restrict__ div_out) {
   const int c = item_index(bd.cells);
-  const int C = m.n_cells, F = m.n_faces, Fi = m.n_internal, Fb = m.n_boundary;
+  const int C = m.n_cells, Fi = m.n_internal, Fb = m.n_boundary;
   if (c >= C || bd.cell_skip(c)) return;
   if (bd.ref_cell) {
     const int k = bd.cell_case[c];
     ref_cell = bd.ref_cell[k];
     ref_value = bd.ref_value[k];
   }
-  int fk[6];
-#pragma unroll
-  for (int k = 0; k < 6; ++k) fk[k] = m.slot_face[k * C + c];
+  const uint32_t c4 = (uint32_t)c * 4u, Fi4 = (uint32_t)Fi * 4u, F4 = (uint32_t)m.n_faces * 4u;
+  const size_t Cz = C;
+  uint32_t f4[6];
+  slot_faces(m, c4, f4);
+  const FBuf pf = face_plane(m, fv, 3);
   if (compute_grad_p) {
     float g[3];
-    gg_field(m, c, fk, fv + (size_t)3 * F, m.volume[c], g);
-    cg[9 * C + c] = g[0];
-    cg[10 * C + c] = g[1];
-    cg[11 * C + c] = g[2];
+    gg_field(m, c4, f4, pf, ld_off(m.volume, c4), g);
+    st_plain(cg + 9 * Cz, c4, g[0]);
+    st_plain(cg + 10 * Cz, c4, g[1]);
+    st_plain(cg + 11 * Cz, c4, g[2]);
   }
-  const float div = cell_div(m, c, fk, flux);
-  if (div_out) div_out[c] = div;
+  const float div = cell_div(m, c4, f4, FBuf(flux, F4));
+  if (div_out) st_plain(div_out, c4, div);
+  const FBuf ruaf(rua_f, F4), area(m.area, F4), dmag(m.dmag, F4);
   float diag = 0.f, b = 0.f;
   float off[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
-    const int f = fk[k];
     off[k] = 0.f;
-    if (f < Fi) {
-      const float w = lap_factor(m, f, correction) * (rua_f[f] * m.area[f]);
+    if (f4[k] < Fi4) {
+      const float w = lap_factor_b(m, dmag, f4[k], correction) * (ruaf(f4[k]) * area(f4[k]));
       diag = diag + (-w);
       off[k] = w;
     } else {
-      const int j = f - Fi;
+      const int j = (int)((f4[k] - Fi4) >> 2);
       if (bc.type[3 * Fb + j] == WCFD_NEUMANN) {
-        b = b + (-((rua_f[f] * bc.value[3 * Fb + j]) * m.area[f]));
+        b = b + (-((ruaf(f4[k]) * bc.value[3 * Fb + j]) * area(f4[k])));
       } else {
-        const float w = (rua_f[f] * m.area[f]) / m.dmag[f];
+        const float w = (ruaf(f4[k]) * area(f4[k])) / dmag(f4[k]);
         diag = diag + (-w);
-        b = b + (-(w * fv[3 * F + f]));
+        b = b + (-(w * pf(f4[k])));
       }
     }
   }
@@ -346,10 +374,10 @@ __global__ void k_assemble_pressure(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
     for (int k = 0; k < 6; ++k) off[k] = 0.f;
     b = ref_value;
   }
-  values[c] = diag;
+  st_plain(values, c4, diag);
 #pragma unroll
-  for (int k = 0; k < 6; ++k) values[(k + 1) * C + c] = off[k];
-  rhs[c] = b;
+  for (int k = 0; k < 6; ++k) st_plain(values + (k + 1) * Cz, c4, off[k]);
+  st_plain(rhs, c4, b);
 }
 
 // ------------------------------------------------------------ rUA, HbyA
@@ -388,31 +416,39 @@ __global__ void k_face_p_fc(wcfd_mesh m, BatchDev bd, wcfd_bc bc, const float* _
                             const float* __restrict__ flux, const float* __restrict__ rua_f,
                             float* __restrict__ fc) {
   const int f = item_index(bd.faces);
-  const int C = m.n_cells, F = m.n_faces, Fi = m.n_internal, Fb = m.n_boundary;
+  const int F = m.n_faces, Fi = m.n_internal, Fb = m.n_boundary;
   if (f >= F || bd.face_skip(f)) return;
-  const float* p = cv + (size_t)3 * C;
+  const uint32_t f4 = (uint32_t)f * 4u;
+  const size_t Fz = F, Fiz = Fi;
+  const FBuf p = cell_plane(m, cv, 3);
   if (f < Fi) {
-    const int o = m.owner[f], n = m.neigh[f];
-    const float psi = m.psi[f];
+    const uint32_t o4 = (uint32_t)ld_off(m.owner, f4) * 4u, n4 = (uint32_t)ld_off(m.neigh, f4) * 4u;
+    const float psi = ld_off(m.psi, f4);
     const float psi2 = 1.0f - psi;
-    const float* g = cg + (size_t)9 * C;
-    float v = p[o] * psi + psi2 * p[n];
-    float g0 = g[o] * psi + psi2 * g[n];
-    float g1 = g[C + o] * psi + psi2 * g[C + n];
-    float g2 = g[2 * C + o] * psi + psi2 * g[2 * C + n];
-    v = v + (g0 * m.skew[f] + g1 * m.skew[Fi + f] + g2 * m.skew[2 * Fi + f]);
-    fv[3 * F + f] = v;
-    if (fc) fc[f] = flux[f] - (rua_f[f] * m.area[f] / m.dmag[f]) * (p[n] - p[o]);
+    const FBuf g0b = cell_plane(m, cg, 9), g1b = cell_plane(m, cg, 10), g2b = cell_plane(m, cg, 11);
+    const float po = p(o4), pn = p(n4);
+    float v = po * psi + psi2 * pn;
+    float g0 = g0b(o4) * psi + psi2 * g0b(n4);
+    float g1 = g1b(o4) * psi + psi2 * g1b(n4);
+    float g2 = g2b(o4) * psi + psi2 * g2b(n4);
+    v = v + (g0 * ld_off(m.skew, f4) + g1 * ld_off(m.skew + Fiz, f4) +
+             g2 * ld_off(m.skew + 2 * Fiz, f4));
+    st_plain(fv + 3 * Fz, f4, v);
+    if (fc)
+      st_plain(fc, f4, ld_off(flux, f4) - (ld_off(rua_f, f4) * ld_off(m.area, f4) /
+                                           ld_off(m.dmag, f4)) * (pn - po));
   } else {
     const int j = f - Fi;
-    const int P = m.owner[f];
+    const uint32_t p4 = (uint32_t)ld_off(m.owner, f4) * 4u;
     const uint8_t t = bc.type[3 * Fb + j];
-    if (t == WCFD_NEUMANN) fv[3 * F + f] = m.dmag[f] * bc.value[3 * Fb + j] + p[P];
+    if (t == WCFD_NEUMANN)
+      st_plain(fv + 3 * Fz, f4, ld_off(m.dmag, f4) * bc.value[3 * Fb + j] + p(p4));
     if (fc) {
-      float F_ = flux[f];
+      float F_ = ld_off(flux, f4);
       if (t == WCFD_DIRICHLET)
-        F_ = F_ - (rua_f[f] * m.area[f] / m.dmag[f]) * (bc.value[3 * Fb + j] - p[P]);
-      fc[f] = F_;
+        F_ = F_ - (ld_off(rua_f, f4) * ld_off(m.area, f4) / ld_off(m.dmag, f4)) *
+                      (bc.value[3 * Fb + j] - p(p4));
+      st_plain(fc, f4, F_);
     }
   }
 }
@@ -426,29 +462,31 @@ __global__ void k_grad_p_correct(wcfd_mesh m, Seg sg, const float* __restrict__ 
                                  float* __restrict__ hbya, const float* __restrict__ rua,
                                  const float* __restrict__ fc, const float* __restrict__ u_prev,
                                  double* __restrict__ partials) {
-  const int C = m.n_cells, F = m.n_faces;
+  const size_t Cz = m.n_cells;
   const BlockRows br = block_rows(sg);
   if (sg.active && !sg.active[br.k]) return;   // whole workgroup: one case
+  const FBuf pf = face_plane(m, fv, 3);
+  const FBuf fcb(fc, (uint32_t)m.n_faces * 4u);
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   for (int c = br.begin + (int)threadIdx.x; c < br.end; c += blockDim.x) {
-    int fk[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) fk[k] = m.slot_face[k * C + c];
+    const uint32_t c4 = (uint32_t)c * 4u;
+    uint32_t f4[6];
+    slot_faces(m, c4, f4);
     float g[3];
-    gg_field(m, c, fk, fv + (size_t)3 * F, m.volume[c], g);
+    gg_field(m, c4, f4, pf, ld_off(m.volume, c4), g);
 #pragma unroll
-    for (int i = 0; i < 9; ++i) cg[i * C + c] = 0.f;
-    cg[9 * C + c] = g[0];
-    cg[10 * C + c] = g[1];
-    cg[11 * C + c] = g[2];
-    const float r = rua[c];
+    for (int i = 0; i < 9; ++i) st_plain(cg + i * Cz, c4, 0.f);
+    st_plain(cg + 9 * Cz, c4, g[0]);
+    st_plain(cg + 10 * Cz, c4, g[1]);
+    st_plain(cg + 11 * Cz, c4, g[2]);
+    const float r = ld_off(rua, c4);
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      const float h = hbya[q * C + c] - r * g[q];
-      hbya[q * C + c] = h;
-      cv[q * C + c] = h;
+      const float h = ld_off(hbya + q * Cz, c4) - r * g[q];
+      st_plain(hbya + q * Cz, c4, h);
+      st_plain(cv + q * Cz, c4, h);
       if (fc && u_prev) {
-        const double dq = (double)h - (double)u_prev[q * C + c];
+        const double dq = (double)h - (double)ld_off(u_prev + q * Cz, c4);
         acc[2] += dq * dq;
         acc[3] += (double)h * (double)h;
       }
@@ -457,8 +495,8 @@ __global__ void k_grad_p_correct(wcfd_mesh m, Seg sg, const float* __restrict__ 
       double num = 0.0, den = 0.0;
 #pragma unroll
       for (int k = 0; k < 6; ++k) {
-        const double v = (double)fc[fk[k]];
-        num += m.slot_side[k * C + c] == 0 ? v : -v;
+        const double v = (double)fcb(f4[k]);
+        num += ld_off(m.slot_side + k * Cz, (uint32_t)c) == 0 ? v : -v;
         den += fabs(v);
       }
       acc[0] += fabs(num);
@@ -480,9 +518,11 @@ __global__ void k_fold_resid(Seg sg, const double* __restrict__ partials,
   if (sg.active && !sg.active[k]) return;
   int b0, nb;
   case_blocks(sg, k, b0, nb);
-  double v[4];
+  const double* pp[4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) v[i] = fold(partials + (size_t)i * sg.G + b0, nb);
+  for (int i = 0; i < 4; ++i) pp[i] = partials + (size_t)i * sg.G + b0;
+  double v[4];
+  fold_n<4, 8>(pp, nb, v);
   if (threadIdx.x != 0) return;
 #pragma unroll
   for (int i = 0; i < 4; ++i) resid[4 * k + i] = v[i];
@@ -708,9 +748,11 @@ int wcfd_ell_to_csr(const wcfd_mesh* m, const float* ell_values, float* csr_valu
   return check_launch();
 }
 
+// Partial rows for n_faces: a hex mesh has at least 3 faces per cell, so
+// cell_reduce_blocks(n_faces) covers one row per thread over its cells.
 size_t wcfd_reduce_workspace_bytes(int32_t n_faces) {
   Carver c(nullptr);
-  c.take<double>(4 * kMaxReduceBlocks);
+  c.take<double>(4 * (size_t)cell_reduce_blocks(n_faces));
   c.take<float>(n_faces > 0 ? n_faces : 1);
   return c.off;
 }
@@ -722,10 +764,11 @@ int wcfd_correct_velocity(const wcfd_mesh* m, wcfd_bc bc, float* cell_values, fl
   if (!mesh_ok(m) || !cell_values || !cell_grad || !face_values || !rua || !hbya || !rua_face)
     return WCFD_EINVAL;
   if (resid && (!workspace || !mass_flux)) return WCFD_EINVAL;
+  if (cell_reduce_blocks(m->n_cells) > cell_reduce_blocks(m->n_faces)) return WCFD_EINVAL;
   Carver w(workspace);
-  double* partials = w.take<double>(4 * kMaxReduceBlocks);
+  double* partials = w.take<double>(4 * (size_t)cell_reduce_blocks(m->n_faces));
   float* fc = w.take<float>(m->n_faces);
-  const Seg sg = single_seg(m->n_cells, kBlock, kMaxReduceBlocks);
+  const Seg sg = single_seg(m->n_cells, kBlock, (int)cell_reduce_blocks(m->n_cells));
   return launch_correct_velocity(*m, BatchDev{}, sg, kBlock, nullptr, bc, cell_values, cell_grad,
                                  face_values, mass_flux, rua, rua_face, hbya, nullptr, resid, fc,
                                  partials, (hipStream_t)stream);

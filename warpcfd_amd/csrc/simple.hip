@@ -33,7 +33,7 @@ static SimpleWork carve_simple(void* ws, int32_t C, int32_t F, int K, int G, boo
   SimpleWork w;
   w.krylov = c.take<char>(batch ? krylov_workspace_bytes_batch(C, 3, K, G)
                                 : krylov_workspace_bytes(C, 3));
-  w.partials = c.take<double>(4 * (size_t)(batch ? G : kMaxReduceBlocks));
+  w.partials = c.take<double>(4 * (size_t)(batch ? G : cell_reduce_blocks(C)));
   w.fc = c.take<float>(F);
   w.u_prev = c.take<float>(3 * (size_t)C);
   return w;
@@ -42,7 +42,7 @@ static SimpleWork carve_simple(void* ws, int32_t C, int32_t F, int K, int G, boo
 static size_t simple_bytes(int32_t C, int32_t F, int K, int G, bool batch) {
   Carver c(nullptr);
   c.take<char>(batch ? krylov_workspace_bytes_batch(C, 3, K, G) : krylov_workspace_bytes(C, 3));
-  c.take<double>(4 * (size_t)(batch ? G : kMaxReduceBlocks));
+  c.take<double>(4 * (size_t)(batch ? G : cell_reduce_blocks(C)));
   c.take<float>(F);
   c.take<float>(3 * (size_t)C);
   return c.off;
@@ -56,7 +56,8 @@ static int simple_iteration(const wcfd_mesh& m, const wcfd_batch* bt, wcfd_bc bc
   SimpleWork w = carve_simple(workspace, m.n_cells, m.n_faces, K, Gb, bt != nullptr);
   const BatchDev bd = batch_dev(bt);
   // continuity partials: the batch's segmentation, or one case over kBlock
-  const Seg sg = bt ? batch_seg(*bt, m.n_cells) : single_seg(m.n_cells, kBlock, kMaxReduceBlocks);
+  const Seg sg = bt ? batch_seg(*bt, m.n_cells)
+                    : single_seg(m.n_cells, kBlock, (int)cell_reduce_blocks(m.n_cells));
   const int sg_block = bt ? bt->block : kBlock;
   std::vector<wcfd_solve_info> mom(K), pres(K);
   int e;
