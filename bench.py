@@ -424,6 +424,16 @@ KERNELS_NOTE = ('avg_us and gbs of a Krylov kernel average over every launch, in
                 'group (up to check_every - 1 per solve)')
 
 
+def install_launch(spec):
+    """GRIDxBLOCK[xFOLD] from the command line: install and describe it."""
+    from warpcfd_amd.tune import install
+    parts = [int(v) for v in spec.lower().split('x')]
+    g, b = parts[:2]
+    fold = parts[2] if len(parts) > 2 else 2048
+    install(g, b, fold)
+    return {'grid': g, 'block': b, 'fold': fold, 'source': 'command line'}
+
+
 def rooflines_from(result, C, F, Fi, Fb, nnz, cg_single=True, note=None):
     """(roofline of the dominant kernel, per-kernel table with each kernel's own
     roofline) from the library profiler's {kernel: (total ms, launches)}."""
@@ -484,7 +494,9 @@ def bench_cylinder(args):
     sv.p_refine = args.p_refine
     dm = model.dm
     tuned = None
-    if not args.no_tune:
+    if args.krylov_launch:
+        tuned = install_launch(args.krylov_launch)
+    elif not args.no_tune:
         from warpcfd_amd.tune import autotune_krylov
         tt = autotune_krylov(dm, cg_variant=sv.cg_variant)
         tuned = {'grid': tt['grid'], 'block': tt['block'], 'fold': tt['fold']}
@@ -737,12 +749,7 @@ def main():
     dm = model.dm
     tuned = None
     if args.krylov_launch:
-        parts = [int(v) for v in args.krylov_launch.lower().split('x')]
-        g, b = parts[:2]
-        fold = parts[2] if len(parts) > 2 else 2048
-        from warpcfd_amd.tune import install
-        install(g, b, fold)
-        tuned = {'grid': g, 'block': b, 'fold': fold, 'source': 'command line'}
+        tuned = install_launch(args.krylov_launch)
     elif not args.no_tune:
         from warpcfd_amd.tune import autotune_krylov
         t = autotune_krylov(dm, cg_variant=solver.cg_variant)

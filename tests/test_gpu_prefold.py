@@ -61,7 +61,9 @@ def _solve(lib, model, ell, negell, b1, b3, grid, block, fold):
     return out
 
 
-@pytest.mark.parametrize('grid,block', [(0, 64), (0, 256), (24, 128)])
+# (0, b): one row per thread; (38, 128): chunks of 253 rows, two per thread;
+# (24, 128): the grid-stride form
+@pytest.mark.parametrize('grid,block', [(0, 64), (0, 256), (38, 128), (24, 128)])
 def test_prefold_bitwise_equals_inline(restore_launch, grid, block):
     import torch
     lib = restore_launch

@@ -484,6 +484,97 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int i
   store_partials<3>(acc, P_out, G, br.vb);
 }
 
+// Two rows per thread (chunk <= 2 B): rows tid and tid + B of the chunk,
+// the grid-stride order, with all their loads issued before the fold; the
+// same sums as the grid-stride form.  Kept apart from k_cgs_iter so that
+// the one-row form keeps its register count (and occupancy).
+template <class M>
+__global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter2(M A, Seg sg, int it, float* __restrict__ x,
+                           const float* __restrict__ r_in, float* __restrict__ r_out,
+                           const float* __restrict__ w_in, float* __restrict__ w_out,
+                           const float* __restrict__ s_in, float* __restrict__ s_out,
+                           float* __restrict__ p, const float* __restrict__ inv,
+                           const double* __restrict__ P_in, double* __restrict__ P_out,
+                           KState* st) {
+  const BlockRows br = block_rows(sg);
+  KState* S = st + br.k;
+  if (S->f.done) return;
+  const int G = sg.G;
+  const bool first = (it == 0);
+  static_assert(M::kEll, "the two-row form reads ELL rows");
+  constexpr int NR = 2;
+  float a[NR][7], wc[NR][7], sc[NR][7], ic[NR][7], rc[NR][7];
+  float p0[NR], x0[NR];
+  bool has[NR];
+  uint32_t o0[NR];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    const int i = br.begin + (int)threadIdx.x + q * (int)blockDim.x;
+    has[q] = i < br.end;
+    o0[q] = (uint32_t)i * 4u;
+    p0[q] = 0.f;
+    x0[q] = 0.f;
+    if (has[q]) {
+      const uint32_t nb4 = (uint32_t)A.n * 4u;
+      const FBuf W(w_in, nb4), Sv(s_in, nb4), I(inv, nb4), Rv(r_in, nb4);
+      uint32_t o[7];
+      A.load_off(i, a[q], o);
+#pragma unroll
+      for (int j = 0; j < 7; ++j) {
+        wc[q][j] = W(o[j]);
+        sc[q][j] = first ? 0.f : Sv(o[j]);
+        ic[q][j] = I(o[j]);
+        rc[q][j] = Rv(o[j]);
+      }
+      p0[q] = first ? 0.f : ld_off((const float*)p, o0[q]);
+      x0[q] = ld_off((const float*)x, o0[q]);
+    }
+  }
+  double alpha_d, beta_d = 0.0;
+  if (first) {
+    alpha_d = S->alpha[0];
+  } else {
+    const int pv = (it - 1) & 1;
+    const double* pp[2] = {P_in + br.b0, P_in + G + br.b0};
+    double gd[2];
+    fold_n<2>(pp, br.nf, gd);
+    const double g = gd[0], d = gd[1];
+    beta_d = safe_div(g, S->rz[pv]);
+    alpha_d = safe_div(g, d - beta_d * safe_div(g, S->alpha[pv]));
+    if (br.vb == br.b0 && threadIdx.x == 0) {
+      S->rz[it & 1] = g;
+      S->alpha[it & 1] = alpha_d;
+    }
+  }
+  const float alpha = (float)alpha_d, beta = (float)beta_d;
+  double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    if (!has[q]) continue;
+    float wn = 0.f;
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+      const float scj = first ? wc[q][j] : wc[q][j] + beta * sc[q][j];
+      wn += a[q][j] * (ic[q][j] * (rc[q][j] - alpha * scj));
+    }
+    const float ii = ic[q][0];
+    const float ui = rc[q][0] * ii;
+    const float pi = first ? ui : ui + beta * p0[q];
+    const float si = first ? wc[q][0] : wc[q][0] + beta * sc[q][0];
+    st_off(x, o0[q], x0[q] + alpha * pi);
+    const float rn = rc[q][0] - alpha * si;
+    st_off(p, o0[q], pi);
+    st_off(s_out, o0[q], si);
+    st_off(r_out, o0[q], rn);
+    st_off(w_out, o0[q], wn);
+    const float un = rn * ii;
+    acc[0] += (double)rn * (double)un;
+    acc[1] += (double)wn * (double)un;
+    acc[2] += (double)rn * (double)rn;
+  }
+  store_partials<3>(acc, P_out, G, br.vb);
+}
+
 // ------------------------------------------------------------- BiCGSTAB
 // partial slots: PA[3][G] = {r0.r, r.r, b.b} (init, k_bi_3), PB[1][G] = {r0.v}
 // (k_bi_1), PC[3][G] = {t.s, t.t, s.s} (k_bi_2)
@@ -983,6 +1074,16 @@ static bool row_per_thread(const KLaunch& L) {
   return M::kEll && L.sg.chunk <= L.B;
 }
 
+// rows per thread of the prefetching single-reduction CG kernel: 1 or 2
+// when every workgroup's chunk fits, else 0 (grid-stride)
+template <class M>
+static int rows_per_thread(const KLaunch& L) {
+  if (!M::kEll) return 0;
+  if (L.sg.chunk <= L.B) return 1;
+  if (L.sg.chunk <= 2 * L.B) return 2;
+  return 0;
+}
+
 // Reduces the ND partial rows at P to one value per case (Seg::prefold).
 template <int ND>
 static void prefold(const KLaunch& L, double* P, KState* st, hipStream_t s) {
@@ -1045,7 +1146,7 @@ static int cgs_impl(const M& A, const KLaunch& L, const float* b, float* x, cons
                     hipStream_t s) {
   const Seg& sg = L.sg;
   const int n = sg.n, B = L.B, G = sg.G, K = sg.n_cases;
-  const bool pf = row_per_thread<M>(L);
+  const int rows = rows_per_thread<M>(L);
   int e = prepare(w.st, sg, tl, s);
   if (e) return e;
   float* rb[2] = {w.r, w.r0};
@@ -1065,14 +1166,20 @@ static int cgs_impl(const M& A, const KLaunch& L, const float* b, float* x, cons
   auto group = [&](int it0) {
     for (int it = it0; it < it0 + ce; ++it) {
       const int cur = it & 1;
-      if (pf)
-        WCFD_LAUNCH_B(K_CG_1, s, (k_cgs_iter<M, true>), G, B, A, sg, it, x, rb[cur], rb[1 - cur],
-                      wb[cur], wb[1 - cur], sb[cur], sb[1 - cur], w.p0, w.inv, pb[cur],
-                      pb[1 - cur], w.st);
-      else
-        WCFD_LAUNCH_B(K_CG_1, s, (k_cgs_iter<M, false>), G, B, A, sg, it, x, rb[cur], rb[1 - cur],
-                      wb[cur], wb[1 - cur], sb[cur], sb[1 - cur], w.p0, w.inv, pb[cur],
-                      pb[1 - cur], w.st);
+#define WCFD_CGS_ITER(KERNEL)                                                             \
+  WCFD_LAUNCH_B(K_CG_1, s, KERNEL, G, B, A, sg, it, x, rb[cur], rb[1 - cur], wb[cur],       \
+                wb[1 - cur], sb[cur], sb[1 - cur], w.p0, w.inv, pb[cur], pb[1 - cur], w.st)
+      if (rows == 1) {
+        WCFD_CGS_ITER((k_cgs_iter<M, true>));
+      } else if constexpr (M::kEll) {
+        if (rows == 2)
+          WCFD_CGS_ITER(k_cgs_iter2<M>);
+        else
+          WCFD_CGS_ITER((k_cgs_iter<M, false>));
+      } else {
+        WCFD_CGS_ITER((k_cgs_iter<M, false>));
+      }
+#undef WCFD_CGS_ITER
       prefold<3>(L, pb[1 - cur], w.st, s);
     }
   };

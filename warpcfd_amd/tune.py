@@ -8,7 +8,9 @@ per thread lets a row's loads issue before the fold) but make inline folds
 longer, so the best shape depends on the mesh size.  ``autotune_krylov``
 times fixed-iteration solves on a pressure-like matrix built on the mesh's
 own pattern for every candidate, installs the fastest and caches it by cell
-count.  Grid 0 means one row per thread.
+count.  Grid 0 means one row per thread; for each block size the grid
+ceil(C / 2 block) is tried too, where the CG kernel takes two rows per
+thread (rows t and t + block of the workgroup's chunk).
 """
 import ctypes
 
@@ -19,9 +21,12 @@ from .FV.device import ptr, stream_ptr
 
 GRIDS = (128, 256, 512, 1024, 2048, 0)
 BLOCKS = (128, 256, 512)
-# the library's default inline-fold limit, and "always prefold"
-FOLDS = (2048, 0)
+# the library's default inline-fold limit, "always prefold", and higher
+# limits that keep the fold inline for one row per thread on mid-size meshes
+FOLDS = (2048, 0, 4096, 8192)
 PREFOLD_GRIDS = (1024, 2048, 0)    # grids on which "always prefold" is tried
+WIDE_FOLD_GRIDS = (0,)             # grids on which the higher limits are tried
+MAX_GRID = 65536                   # the library's workgroup limit for a Krylov launch
 
 _cache = {}
 
@@ -90,12 +95,20 @@ def autotune_krylov(dm, grids=GRIDS, blocks=BLOCKS, iters=60, cg_weight=1.0, bic
         install(c['grid'], c['block'], c['fold'])
         return c
     results = []
+    C = dm.n_cells
     for blk in blocks:
-        for grid in grids:
-            if grid * blk > 4 * dm.n_cells and grid > grids[0]:
+        two_rows = -(-C // (2 * blk))
+        shapes = list(grids)
+        if C > blk and two_rows <= MAX_GRID and two_rows not in shapes:
+            shapes.append(two_rows)
+        for grid in shapes:
+            if grid * blk > 4 * C and grid > grids[0]:
                 continue            # more threads than rows: no gain
+            wide = grid in WIDE_FOLD_GRIDS or grid == two_rows
             for fold in FOLDS:
-                if fold != FOLDS[0] and grid not in PREFOLD_GRIDS:
+                if fold == 0 and grid not in PREFOLD_GRIDS and grid != two_rows:
+                    continue
+                if fold > FOLDS[0] and not wide:
                     continue
                 cg_ms, bi_ms = time_solvers(dm, grid, blk, iters, cg_variant=cg_variant,
                                             fold=fold)
