@@ -2,7 +2,8 @@
 
 Every array keeps the reference layout: cell_values (C,4), cell_gradients
 (C,4,3), face_values (F,4), face_gradients (F,4), mass_fluxes (F), term
-weights (C,6,O,3).  Loops that the reference runs serially (Warp-CPU launches
+weights (C,Fc,O,3) with Fc faces per cell (6 hexahedron, 5 wedge, 4
+tetrahedron).  Loops that the reference runs serially (Warp-CPU launches
 in tid order, atomics accumulated in that order) are vectorised over the
 outer index and kept sequential over the inner one, so each accumulation
 happens in the reference's order.
@@ -137,7 +138,7 @@ class OracleFVM:
         m = self.m
         for o in outputs:
             acc = np.zeros((m.num_cells, 3), self.dtype)
-            for k in range(6):
+            for k in range(m.faces_per_cell):
                 f = m.cell_faces[:, k]
                 s = self.face_values[f, o] * self.face_area[f]
                 acc = acc + (s[:, None] * self.normal[:, k]) / self.volumes[:, None]
@@ -155,7 +156,7 @@ class OracleFVM:
         """model.py:269-275; model_kernels.py:27-34."""
         m = self.m
         acc = np.zeros(m.num_cells, self.dtype)
-        for k in range(6):
+        for k in range(m.faces_per_cell):
             fl = self.mass_fluxes[m.cell_faces[:, k]]
             acc = acc + np.where(m.face_side[:, k] == 0, fl, -fl)
         return acc
@@ -197,7 +198,7 @@ def convection_weights(fvm, outputs=(0, 1, 2), implicit_outflow=False, scheme='u
     m = fvm.m
     dt = fvm.dtype
     O = len(outputs)
-    w = np.zeros((m.num_cells, 6, O, 3), dt)
+    w = np.zeros((m.num_cells, m.faces_per_cell, O, 3), dt)
     F = fvm.mass_fluxes
     b = m.boundary_face_ids
     ob, kb = m.adjacent_cells[b, 0], m.cell_face_index[b, 0]
@@ -251,7 +252,7 @@ def diffusion_weights(fvm, viscosity, outputs, correction=False):
     m = fvm.m
     dt = fvm.dtype
     O = len(outputs)
-    w = np.zeros((m.num_cells, 6, O, 3), dt)
+    w = np.zeros((m.num_cells, m.faces_per_cell, O, 3), dt)
     nu_arr = np.atleast_1d(np.asarray(viscosity, dt))
     nu_f = (lambda f: nu_arr[0]) if nu_arr.shape[0] == 1 else (lambda f: nu_arr[f])
     bnd = fvm.boundary
@@ -337,11 +338,11 @@ class OracleEquation:
         rr = self.rows
         dpos = np.nonzero(rr == self.columns)[0]
         self.diag_pos[rr[dpos]] = dpos
-        self.slot_pos = -np.ones((C, 6, self.O), np.int64)
+        self.slot_pos = -np.ones((C, m.faces_per_cell, self.O), np.int64)
         # rows ascending, columns sorted inside a row -> (row, col) keys sorted
         keys = self.rows.astype(np.int64) * self.n + self.columns
         for o in range(self.O):
-            for k in range(6):
+            for k in range(m.faces_per_cell):
                 nbr = m.neighbors[:, k]
                 cc = np.nonzero(nbr >= 0)[0]
                 row = self.O * cc + o
@@ -356,7 +357,7 @@ class OracleEquation:
 
     def add_face_term(self, w, scale):
         """calculate_BSR_values_kernel (equation_kernels.py:52-79) and
-        calculate_RHS_values_kernel (84-101): diagonal = scale * sum over the 6
+        calculate_RHS_values_kernel (84-101): diagonal = scale * sum over the
         local faces of w[...,0] (boundary slots included); off-diagonal =
         scale * w[...,1]; b -= scale * w[...,2] for boundary faces only (T2)."""
         d = self.fvm.dtype
@@ -366,10 +367,10 @@ class OracleEquation:
         for o in range(self.O):
             dp = self.diag_pos[self.O * np.arange(C) + o]
             acc = self.values[dp]
-            for k in range(6):
+            for k in range(m.faces_per_cell):
                 acc = acc + s * w[:, k, o, 0]
             self.values[dp] = acc
-            for k in range(6):
+            for k in range(m.faces_per_cell):
                 sp = self.slot_pos[:, k, o]
                 ok = sp >= 0
                 self.values[sp[ok]] = self.values[sp[ok]] + s * w[ok, k, o, 1]

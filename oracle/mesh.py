@@ -5,8 +5,14 @@ calculate_face_normal_and_area 161-183, get_neighbors 186-205,
 get_mesh_properties 220-302), FV/mesh_structs.py:init_structs 141-251 (psi,
 face area) and preprocess/grid.py:define_boundary_walls 165-194.
 
-Only hexahedra are restated: the reference's struct selection rejects its own
-tet and wedge meshes (mesh_structs.py:112-132).
+Hexahedra, tetrahedra and wedges follow get_faces' three branches (the
+local face tables of mesh.py:69-91; a wedge's two triangles form the first
+face set and its three quadrilaterals the second, mesh.py:101-104).  The
+reference's own struct selection rejects its tet and wedge meshes
+(mesh_structs.py:112-132 asks a tetrahedron for 3 faces and a wedge for
+3-node faces); the arrays restated here are those the rest of the reference
+computes for them, which is what a corrected selection (4 faces of 3 nodes;
+5 faces, padded to 4 nodes) would hand its kernels.
 """
 import numpy as np
 
@@ -17,6 +23,26 @@ HEX_FACES = np.array([[3, 2, 1, 0],
                       [1, 2, 6, 5],
                       [3, 7, 6, 2],
                       [3, 0, 4, 7]], dtype=np.int32)
+# VTK tetrahedron, mesh.py:78-83.
+TET_FACES = np.array([[2, 1, 0], [0, 1, 3], [3, 2, 0], [1, 2, 3]], dtype=np.int32)
+# VTK wedge, mesh.py:85-91 (-1 pads the two triangles).
+WEDGE_FACES = np.array([[0, 2, 1, -1], [3, 4, 5, -1], [0, 3, 5, 2], [1, 2, 5, 4],
+                        [0, 1, 4, 3]], dtype=np.int32)
+
+#: nodes per cell -> (reference cell type name, VTK type, face sets in local order)
+CELL_TYPES = {
+    8: ('HEXAHEDRON', 12, [HEX_FACES]),
+    4: ('TETRA', 10, [TET_FACES]),
+    6: ('WEDGE', 13, [WEDGE_FACES[0:2, :-1], WEDGE_FACES[2:, :]]),
+}
+
+
+def face_sets(cells):
+    """The face sets of get_faces (mesh.py:101-104) for a (C, n) cell array."""
+    n = np.asarray(cells).shape[-1]
+    if n not in CELL_TYPES:
+        raise ValueError('only tetrahedra (4 nodes), wedges (6) and hexahedra (8) are supported')
+    return CELL_TYPES[n][2]
 
 
 def fan_area_normal(face_coords):
@@ -40,25 +66,31 @@ def fan_area_normal(face_coords):
     return area, normal
 
 
-def hex_volume(nodes, cells):
-    """Cell volume by the divergence theorem over the fan triangles of the 6
-    faces (V = 1/3 sum_tri (x_c . n_tri A_tri)), evaluated in float64.
-    Stand-in for VTK compute_cell_sizes (mesh.py:33), which is absent."""
-    x = nodes.astype(np.float64)[cells]               # (C,8,3)
-    fc = x[:, HEX_FACES]                               # (C,6,4,3)
-    v0 = fc[:, :, 0, :]
+def cell_volume(nodes, cells):
+    """Cell volume by the divergence theorem over the fan triangles of the
+    cell's faces (V = 1/3 sum_tri (x_c . n_tri A_tri)), evaluated in float64.
+    Stand-in for VTK compute_cell_sizes (mesh.py:33), which is absent; exact
+    for a tetrahedron and for cells with planar faces."""
+    x = nodes.astype(np.float64)[cells]               # (C,n,3)
     vol = np.zeros(cells.shape[0])
-    for i in (1, 2):
-        a = fc[:, :, i, :] - v0
-        b = fc[:, :, i + 1, :] - v0
-        cr = np.cross(a, b)                             # 2 A n for the triangle
-        tri_c = (v0 + fc[:, :, i, :] + fc[:, :, i + 1, :]) / 3.0
-        vol += (cr * tri_c).sum(axis=(1, 2)) / 6.0
+    for table in face_sets(cells):
+        fc = x[:, table]                               # (C,Fs,Ns,3)
+        v0 = fc[:, :, 0, :]
+        for i in range(1, table.shape[1] - 1):
+            a = fc[:, :, i, :] - v0
+            b = fc[:, :, i + 1, :] - v0
+            cr = np.cross(a, b)                         # 2 A n for the triangle
+            tri_c = (v0 + fc[:, :, i, :] + fc[:, :, i + 1, :]) / 3.0
+            vol += (cr * tri_c).sum(axis=(1, 2)) / 6.0
     return np.abs(vol)
 
 
+def hex_volume(nodes, cells):
+    return cell_volume(nodes, cells)
+
+
 class OracleMesh:
-    """Reference-order mesh arrays (hexahedra only).
+    """Reference-order mesh arrays (hexahedra, tetrahedra or wedges).
 
     Attribute names follow cells_data / faces_data (mesh.py:306-510) and the
     Cell/Face structs (mesh_structs.py:5-64).
@@ -72,37 +104,55 @@ class OracleMesh:
         self.cells = np.asarray(cells, dtype=np.int32)
         C = self.cells.shape[0]
         self.num_cells = C
-        # mesh.py:32 (VTK cell_centers) -> vertex mean for a hexahedron.
+        self.cell_type = CELL_TYPES[self.cells.shape[1]][0] if self.cells.shape[1] in CELL_TYPES \
+            else None
+        sets = face_sets(self.cells)
+        # mesh.py:32 (VTK cell_centers: the parametric centre, which is the
+        # vertex mean for a hexahedron, a tetrahedron and a wedge).
         self.centroids = self.nodes.astype(np.float64)[self.cells].mean(axis=1).astype(gd)
         # mesh.py:33 (VTK compute_cell_sizes, abs) -> divergence theorem.
-        self.volumes = hex_volume(self.nodes, self.cells).astype(gd)
+        self.volumes = cell_volume(self.nodes, self.cells).astype(gd)
 
-        # ---- get_faces, mesh.py:99-159 (hex branch: one face set) ----
-        faces = self.cells[:, HEX_FACES]                        # (C,6,4)
-        faces_sorted = np.sort(faces, axis=2)
-        faces_reshaped = faces_sorted.reshape(-1, 4)
-        unique_faces, inv = np.unique(faces_reshaped, axis=0, return_inverse=True)
-        cell_face_ids = inv.reshape(C, 6).astype(np.int32)
-        area, normal = fan_area_normal(self.nodes[faces])        # (C,6), (C,6,3)
-        face_centroid = self.nodes[unique_faces].mean(axis=1)    # mesh.py:128
-        to_face = face_centroid[cell_face_ids] - self.centroids[:, np.newaxis, :]  # mesh.py:140
+        # ---- get_faces, mesh.py:99-159: one face set, or two for a wedge ----
+        N = max(t.shape[1] for t in sets)
+        parts, shift = [], 0
+        for table in sets:
+            faces = self.cells[:, table]                          # (C,Fs,Ns)
+            ns = table.shape[1]
+            faces_reshaped = np.sort(faces, axis=2).reshape(-1, ns)
+            uf, inv = np.unique(faces_reshaped, axis=0, return_inverse=True)
+            cfi = inv.reshape(C, table.shape[0]).astype(np.int32)
+            area, normal = fan_area_normal(self.nodes[faces])      # (C,Fs), (C,Fs,3)
+            fcen = self.nodes[uf].mean(axis=1)                     # mesh.py:128
+            to_face = fcen[cfi] - self.centroids[:, np.newaxis, :]  # mesh.py:140
+            if ns < N:                                             # mesh.py:131-135
+                uf = np.hstack([uf, -np.ones((uf.shape[0], N - ns), uf.dtype)])
+            parts.append((uf, cfi + shift, area, normal, np.full(len(uf), ns, np.int32), fcen,
+                          to_face))
+            shift += len(uf)
+        unique_faces = np.concatenate([p[0] for p in parts], axis=0)
+        cell_face_ids = np.concatenate([p[1] for p in parts], axis=1).astype(np.int32)
 
         self.unique_faces = unique_faces.astype(np.int32)
         self.cell_faces = cell_face_ids
-        self.area = area.astype(gd)
-        self.normal = normal.astype(gd)
-        self.face_centroid = face_centroid.astype(gd)
-        self.c2f = to_face.astype(gd)
+        self.area = np.concatenate([p[2] for p in parts], axis=1).astype(gd)
+        self.normal = np.concatenate([p[3] for p in parts], axis=1).astype(gd)
+        self.face_num_nodes = np.concatenate([p[4] for p in parts])
+        self.face_centroid = np.concatenate([p[5] for p in parts], axis=0).astype(gd)
+        self.c2f = np.concatenate([p[6] for p in parts], axis=1).astype(gd)
         K = unique_faces.shape[0]
         self.num_faces = K
+        Fc = cell_face_ids.shape[1]
+        self.faces_per_cell = Fc
+        self.nodes_per_face = N
 
         # ---- get_mesh_properties, mesh.py:220-302 ----
         self.adjacent_cells = -np.ones((K, 2), np.int32)
         self.cell_face_index = -np.ones((K, 2), np.int32)
         self.distance = -np.ones((K, 3), gd)
         self.is_boundary = np.ones(K, np.uint8)
-        self.neighbors = -np.ones((C, 6), np.int32)
-        self.face_side = np.zeros((C, 6), np.int32)
+        self.neighbors = -np.ones((C, Fc), np.int32)
+        self.face_side = np.zeros((C, Fc), np.int32)
         self.nnz_per_cell = np.ones(C, np.int32)
 
         if pair_loop:
@@ -146,7 +196,7 @@ class OracleMesh:
         K = self.num_faces
         face_cells = [[] for _ in range(K)]
         for c in range(self.num_cells):
-            for k in range(6):
+            for k in range(self.faces_per_cell):
                 face_cells[cf[c, k]].append(c)
         pairs = sorted({tuple(sorted(v)) for v in face_cells if len(v) == 2})
         for i, j in pairs:
@@ -178,8 +228,9 @@ class OracleMesh:
         internal = np.nonzero(counts == 2)[0]
         s0 = order[start[internal]]
         s1 = order[start[internal] + 1]
-        c0, k0 = s0 // 6, s0 % 6
-        c1, k1 = s1 // 6, s1 % 6
+        Fc = self.faces_per_cell
+        c0, k0 = s0 // Fc, s0 % Fc
+        c1, k1 = s1 // Fc, s1 % Fc
         swap = c0 > c1
         ci = np.where(swap, c1, c0); ki = np.where(swap, k1, k0)
         cj = np.where(swap, c0, c1); kj = np.where(swap, k0, k1)
@@ -230,7 +281,7 @@ class OracleMesh:
             tot = offsets + nnz_pc * o
             rows[tot] = O * np.arange(C) + o
             cols[tot] = rows[tot]
-            for k in range(6):
+            for k in range(self.faces_per_cell):
                 f = self.cell_faces[:, k]
                 internal = self.is_boundary[f] == 0
                 cc = np.nonzero(internal)[0]

@@ -97,7 +97,7 @@ class OracleSolver:
         Fc[b[dirp]] = F[b[dirp]] - coef_b[dirp] * (bnd.values[dirp, 3].astype(np.float64) - p[ob[dirp]])
         num = np.zeros(m.num_cells)
         den = np.zeros(m.num_cells)
-        for k in range(6):
+        for k in range(m.faces_per_cell):
             f = m.cell_faces[:, k]
             s = np.where(m.face_side[:, k] == 0, 1.0, -1.0)
             num += s * Fc[f]

@@ -11,6 +11,13 @@ def apply_bcs(boundary, kind):
             boundary.no_slip_wall(w)
         boundary.velocity_BC('+Y', u=1.0, v=0.0, w=0.0)
         boundary.slip_wall('+Z'); boundary.slip_wall('-Z')
+    elif kind == 'closed_cavity':
+        # every wall no-slip: on cells without z-symmetry (tetrahedra) the
+        # slip walls above let mass through, since slip_wall gives w a zero
+        # normal gradient too (boundary.py:101-105)
+        for w in ('-X', '+X', '-Y', '-Z', '+Z'):
+            boundary.no_slip_wall(w)
+        boundary.velocity_BC('+Y', u=1.0, v=0.0, w=0.0)
     elif kind == 'channel':
         boundary.velocity_BC('-X', u=1.0, v=0.0, w=0.0)
         boundary.pressure_BC('+X', p=0.0)

@@ -81,4 +81,8 @@ class DeviceMesh:
         return t.index_select(-1, self.face_old2new)
 
     def faces_to_device(self, t):
+        """(..., K) reference order -> (..., F) device order; the null face
+        of a tetrahedron or wedge mesh gets 0."""
+        if self.layout.null_face >= 0:
+            t = torch.cat([t, t.new_zeros(t.shape[:-1] + (1,))], dim=-1)
         return t.index_select(-1, self.face_new2old)

@@ -62,7 +62,7 @@ class IncompressibleSolver:
         self.u_relaxation_factor = u_relaxation_factor
         self.material = model.material
         self.lib = _lib.load()
-        C, F, nnz = model.num_cells, model.num_faces, model.dm.nnz
+        C, F, nnz = model.num_cells, model.dm.n_faces, model.dm.nnz
         dev = model._cv.device
         z = lambda *s: torch.zeros(*s, dtype=torch.float32, device=dev)
         self.HbyA = z(3, C)

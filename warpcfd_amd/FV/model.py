@@ -44,9 +44,9 @@ class FVM:
         self.num_nodes = mesh.nodes.shape[0]
         self.num_cells = mesh.cells.shape[0]
         self.num_faces = self.face_properties.num_faces
-        self.nodes_per_cell = 8
-        self.faces_per_cell = 6
-        self.nodes_per_face = 4
+        self.nodes_per_cell = mesh.cells.shape[1]
+        self.faces_per_cell = self.cell_properties.faces_per_cell
+        self.nodes_per_face = self.cell_properties.nodes_per_face
         self.reference_pressure_cell_id = None
         self.reference_pressure = 0.0
         self.finalized = False
@@ -69,7 +69,7 @@ class FVM:
         self.dm = DeviceMesh(self.layout, dev)
         self.bc_type, self.bc_value = self.boundary.device_arrays(self.layout, dev)
         self.bc = _lib.WcfdBC(self.bc_type.data_ptr(), self.bc_value.data_ptr())
-        C, F = self.num_cells, self.num_faces
+        C, F = self.num_cells, self.dm.n_faces        # device faces (with a null face)
         z = lambda *s: torch.zeros(*s, dtype=torch.float32, device=dev)
         self._cv = z(4, C)
         self._cg = z(12, C)

@@ -26,10 +26,12 @@ def face_c2f(fv):
         L = fv.layout
         fp, cp = fv.mesh.face_properties, fv.mesh.cell_properties
         fi_ref = L.face_new2old[:L.n_internal]
-        own = fp.adjacent_cells[fi_ref, 0]
-        k0, k1 = fp.cell_face_index[fi_ref, 0], fp.cell_face_index[fi_ref, 1]
+        real = fi_ref < fp.num_faces                 # not the null face
+        own = fp.adjacent_cells[fi_ref[real], 0]
+        k0, k1 = fp.cell_face_index[fi_ref[real], 0], fp.cell_face_index[fi_ref[real], 1]
         c2f = cp.cell_centroid_to_face_centroid
-        a = np.stack([c2f[own, k0].T, c2f[own, k1].T]).astype(np.float32)
+        a = np.zeros((2, 3, L.n_internal), np.float32)
+        a[:, :, real] = np.stack([c2f[own, k0].T, c2f[own, k1].T])
         if a.size == 0:
             a = np.zeros((2, 3, 1), np.float32)
         fv._face_c2f = torch.from_numpy(np.ascontiguousarray(a)).to(fv._cv.device)

@@ -111,7 +111,8 @@ class Term:
             return dm.cells_to_reference(w).T.reshape(-1).contiguous()
         if self.cell_based:                              # (O, 2, C) -> (C, O, 2)
             return dm.cells_to_reference(w).permute(2, 0, 1).contiguous()
-        return dm.cells_to_reference(w).permute(3, 2, 0, 1).contiguous()   # (C, 6, O, 3)
+        w = dm.cells_to_reference(w).permute(3, 2, 0, 1)                  # (C, 6, O, 3)
+        return w[:, :self._fv.faces_per_cell].contiguous()   # the mesh's faces per cell
 
     def _outputs(self):
         """(array, pointer) of the global field indices; keep the array alive

@@ -1,11 +1,60 @@
-"""Synthetic hexahedral mesh generators and boundary wall tagging.
+"""Synthetic mesh generators and boundary wall tagging.
 
 The reference meshes with gmsh (preprocess/grid.py:8-159), which is absent;
 these generators produce the same kind of input -- raw ``(nodes, cells)``
-arrays of VTK hexahedra -- for the cavity, channel and cylinder workloads.
-``define_boundary_walls`` follows preprocess/grid.py:165-194.
+arrays of VTK hexahedra, tetrahedra or wedges -- for the cavity, channel and
+cylinder workloads.  ``define_boundary_walls`` follows
+preprocess/grid.py:165-194.
 """
 import numpy as np
+
+# a lattice cube in VTK hexahedron order: node k at these (x, y, z) offsets
+_CUBE = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                  [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], np.float64)
+# Kuhn subdivision: one tetrahedron per order of the axes along the path
+# from node 0 to node 6; conforming between lattice neighbours
+_KUHN = [[0, 1, 2, 6], [0, 1, 5, 6], [0, 3, 2, 6], [0, 3, 7, 6], [0, 4, 5, 6], [0, 4, 7, 6]]
+# two prisms on the (0, 2) diagonal of the bottom and top faces
+_PRISMS = [[0, 1, 2, 4, 5, 6], [0, 2, 3, 4, 6, 7]]
+
+
+def _oriented(split, swap):
+    """Orders each sub-cell so that its first three nodes (p1 - p0) x (p2 - p0)
+    point at the rest of the cell -- the orientation for which the local
+    face tables (preprocess/mesh.py:69-91) give outward normals."""
+    out = []
+    for t in split:
+        t = list(t)
+        p = _CUBE[t]
+        n = np.cross(p[1] - p[0], p[2] - p[0])
+        if np.dot(n, p[len(t) // 2 if len(t) == 6 else 3] - p[0]) < 0:
+            for a, b in swap:
+                t[a], t[b] = t[b], t[a]
+        out.append(t)
+    return np.array(out, np.int64)
+
+
+def split_hexes(cells, cell_type):
+    """Split lattice hexahedra (VTK order, as ``structured_hex`` makes them)
+    into ``'tetra'`` (6 per hexahedron) or ``'wedge'`` (2 per hexahedron)
+    cells, the sub-cells of hexahedron i at rows i*k .. i*k + k - 1."""
+    cells = np.asarray(cells)
+    if cell_type == 'tetra':
+        split = _oriented(_KUHN, [(1, 2)])
+    elif cell_type == 'wedge':
+        split = _oriented(_PRISMS, [(1, 2), (4, 5)])
+    else:
+        raise ValueError("cell_type must be 'tetra' or 'wedge'")
+    return cells[:, split].reshape(-1, split.shape[1]).astype(np.int32)
+
+
+def structured_cells(counts, lengths=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0), jitter=0.0,
+                     seed=0, cell_type='hexahedron'):
+    """``structured_hex``'s box as hexahedra, tetrahedra or wedges."""
+    nodes, cells = structured_hex(counts, lengths, origin, jitter, seed)
+    if cell_type == 'hexahedron':
+        return nodes, cells
+    return nodes, split_hexes(cells, cell_type)
 
 
 def structured_hex(counts, lengths=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0),

@@ -1,27 +1,55 @@
 """Host mesh topology and geometry in the reference's order.
 
 Mirrors ``warp_cfd.preprocess.Mesh`` (preprocess/mesh.py:17-302): face ids in
-lexicographic order of sorted node tuples (mesh.py:115-121), owner = lower
-cell id (mesh.py:205,280), fan-triangulated face areas and normals per
-cell-local face (mesh.py:161-183), face centroid = node mean (mesh.py:128),
-cell->face-centroid vectors (mesh.py:140), neighbour / face_side / nnz /
-face_offset_index tables (mesh.py:259-300).  The Python pair loop of
-mesh.py:263-281 is replaced by a sort over face ids, which gives identical
-integer tables.
+lexicographic order of sorted node tuples (mesh.py:115-121), per face set
+(a wedge's triangles first, then its quadrilaterals, mesh.py:101-104,
+142-157), owner = lower cell id (mesh.py:205,280), fan-triangulated face
+areas and normals per cell-local face (mesh.py:161-183), face centroid =
+node mean (mesh.py:128), cell->face-centroid vectors (mesh.py:140),
+neighbour / face_side / nnz / face_offset_index tables (mesh.py:259-300).
+The Python pair loop of mesh.py:263-281 is replaced by a sort over face
+ids, which gives identical integer tables.
 
-pyvista is not available, so the mesh is built from raw arrays:
-``Mesh(nodes, cells)`` with ``cells`` (C,8) VTK hexahedra.  An object with
-``.points`` and ``.cells_dict`` (a pyvista grid) is accepted as well.
+Cells are hexahedra (8 nodes, 6 faces), tetrahedra (4 nodes, 4 faces) or
+wedges (6 nodes, 5 faces), in VTK node order, one type per mesh
+(mesh.py:20).  pyvista is not available, so the mesh is built from raw
+arrays: ``Mesh(nodes, cells)``.  An object with ``.points`` and
+``.cells_dict`` (a pyvista grid) is accepted as well.
 """
 import numpy as np
 
-VTK_HEXAHEDRON = 12
+VTK_TETRA, VTK_HEXAHEDRON, VTK_WEDGE = 10, 12, 13
 
 #: local faces of a VTK hexahedron, outward by the right-hand rule (mesh.py:69-76)
 HEX_FACE_NODES = np.array([[3, 2, 1, 0], [4, 5, 6, 7], [0, 1, 5, 4],
                            [1, 2, 6, 5], [3, 7, 6, 2], [3, 0, 4, 7]], dtype=np.int32)
+#: tetrahedron (mesh.py:78-83)
+TET_FACE_NODES = np.array([[2, 1, 0], [0, 1, 3], [3, 2, 0], [1, 2, 3]], dtype=np.int32)
+#: wedge: two triangles, then three quadrilaterals (mesh.py:85-91)
+WEDGE_TRI_NODES = np.array([[0, 2, 1], [3, 4, 5]], dtype=np.int32)
+WEDGE_QUAD_NODES = np.array([[0, 3, 5, 2], [1, 2, 5, 4], [0, 1, 4, 3]], dtype=np.int32)
+
+#: VTK type -> (name, nodes per cell, face sets in local face order)
+CELL_TYPES = {
+    VTK_HEXAHEDRON: ('HEXAHEDRON', 8, (HEX_FACE_NODES,)),
+    VTK_TETRA: ('TETRA', 4, (TET_FACE_NODES,)),
+    VTK_WEDGE: ('WEDGE', 6, (WEDGE_TRI_NODES, WEDGE_QUAD_NODES)),
+}
+_BY_NODES = {v[1]: k for k, v in CELL_TYPES.items()}
 
 
+def cell_type_of(cells):
+    """VTK type of a (C, n) cell array: n = 8, 4 or 6."""
+    n = np.asarray(cells).shape[-1]
+    if n not in _BY_NODES:
+        raise ValueError('only Tetra and Hexahedron and Wedge elements are supported '
+                         '(cells of 4, 8 or 6 nodes)')
+    return _BY_NODES[n]
+
+
+def local_faces(cell_type):
+    """Local faces of a cell type in face order, each a node list."""
+    return [list(r) for t in CELL_TYPES[cell_type][2] for r in t]
 class group:
     """preprocess/mesh.py:8-15."""
 
@@ -43,15 +71,17 @@ def _fan(coords):
     return area, sum(parts) / area[..., None]
 
 
-def _hex_volume(x):
-    """Divergence theorem over the fan triangles of the 6 faces, float64.
-    (Stand-in for VTK compute_cell_sizes, mesh.py:33.)"""
-    fc = x[:, HEX_FACE_NODES]
-    v0 = fc[:, :, 0]
+def _volume(x, sets):
+    """Divergence theorem over the fan triangles of the faces, float64.
+    (Stand-in for VTK compute_cell_sizes, mesh.py:33; exact for tetrahedra
+    and planar faces.)"""
     vol = np.zeros(x.shape[0])
-    for i in (1, 2):
-        cr = np.cross(fc[:, :, i] - v0, fc[:, :, i + 1] - v0)
-        vol += np.einsum('cfk,cfk->c', cr, v0 + fc[:, :, i] + fc[:, :, i + 1]) / 18.0
+    for t in sets:
+        fc = x[:, t]
+        v0 = fc[:, :, 0]
+        for i in range(1, t.shape[1] - 1):
+            cr = np.cross(fc[:, :, i] - v0, fc[:, :, i + 1] - v0)
+            vol += np.einsum('cfk,cfk->c', cr, v0 + fc[:, :, i] + fc[:, :, i + 1]) / 18.0
     return np.abs(vol)
 
 
@@ -63,22 +93,25 @@ class Mesh:
             if len(cd) != 1:
                 raise AssertionError('Meshes can only contain a single cell type for now, Got multiple')
             ctype, cells = next(iter(cd.items()))
-            if int(ctype) != VTK_HEXAHEDRON:
-                raise ValueError('only hexahedral meshes are supported')
+            if int(ctype) not in CELL_TYPES:
+                raise AssertionError('only Tetra and Hexahedron and Wedge elements are supported')
             nodes = pts
         else:
             nodes = mesh
         self.float_dtype = np.float32
         self.int_dtype = np.int32
         self.dtype = dtype
-        self.cellType = ('HEXAHEDRON', VTK_HEXAHEDRON)
         self.nodes = np.asarray(nodes, dtype=np.float32)            # mesh.py:26 (T13)
         self.cells = np.asarray(cells, dtype=np.int32)               # mesh.py:31
-        if self.cells.ndim != 2 or self.cells.shape[1] != 8:
-            raise ValueError('cells must be a (C,8) array of VTK hexahedra')
+        if self.cells.ndim != 2:
+            raise ValueError('cells must be a (C, n) array of VTK cells')
+        ct = cell_type_of(self.cells)
+        self.cellType = (CELL_TYPES[ct][0], ct)
+        self._sets = CELL_TYPES[ct][2]
         x = self.nodes.astype(np.float64)[self.cells]
-        self.cell_centroids = x.mean(axis=1).astype(np.float32)      # VTK cell centre (hex)
-        self.cell_volumes = _hex_volume(x).astype(np.float32)
+        # VTK cell centre = the parametric centre = the vertex mean for all three types
+        self.cell_centroids = x.mean(axis=1).astype(np.float32)
+        self.cell_volumes = _volume(x, self._sets).astype(np.float32)
         self.groups = {}
         self.gridType = 'Unstructured'
         self.face_properties, self.cell_properties = self.get_mesh_properties()
@@ -106,16 +139,36 @@ class Mesh:
         b = fp.boundary_face_ids
         return cp.normal[fp.adjacent_cells[b, 0], fp.cell_face_index[b, 0]]
 
+    @property
+    def faces_per_cell(self):
+        return sum(t.shape[0] for t in self._sets)
+
     def get_mesh_properties(self):
         C = self.cells.shape[0]
-        local = self.cells[:, HEX_FACE_NODES]                        # (C,6,4)
-        keyed = np.sort(local, axis=2).reshape(-1, 4)
-        uniq, inv = np.unique(keyed, axis=0, return_inverse=True)
-        cell_faces = inv.reshape(C, 6).astype(np.int32)
-        area, normal = _fan(self.nodes[local])
-        fcen = self.nodes[uniq].mean(axis=1)
-        c2f = fcen[cell_faces] - self.cell_centroids[:, None, :]
+        N = max(t.shape[1] for t in self._sets)
+        parts, shift = [], 0
+        for t in self._sets:                                          # mesh.py:108-148
+            local = self.cells[:, t]                                  # (C,Fs,Ns)
+            keyed = np.sort(local, axis=2).reshape(-1, t.shape[1])
+            uq, inv = np.unique(keyed, axis=0, return_inverse=True)
+            cf = inv.reshape(C, t.shape[0]).astype(np.int32)
+            a, n = _fan(self.nodes[local])
+            fc = self.nodes[uq].mean(axis=1)
+            c2f_s = fc[cf] - self.cell_centroids[:, None, :]
+            nn = np.full(len(uq), t.shape[1], np.int32)
+            if t.shape[1] < N:                                        # tri faces padded with -1
+                uq = np.hstack([uq, -np.ones((len(uq), N - t.shape[1]), uq.dtype)])
+            parts.append((uq, cf + shift, a, n, fc, c2f_s, nn))
+            shift += len(uq)
+        uniq = np.concatenate([p[0] for p in parts], axis=0)
+        cell_faces = np.concatenate([p[1] for p in parts], axis=1).astype(np.int32)
+        area = np.concatenate([p[2] for p in parts], axis=1)
+        normal = np.concatenate([p[3] for p in parts], axis=1)
+        fcen = np.concatenate([p[4] for p in parts], axis=0)
+        c2f = np.concatenate([p[5] for p in parts], axis=1)
+        num_nodes = np.concatenate([p[6] for p in parts])
         K = uniq.shape[0]
+        Fc = cell_faces.shape[1]
 
         # pair the two (cell, local face) slots of every shared face
         flat = cell_faces.ravel()
@@ -126,12 +179,12 @@ class Mesh:
         first = np.concatenate([[0], np.cumsum(counts)[:-1]])
         fi = np.nonzero(counts == 2)[0].astype(np.int32)
         a, b = order[first[fi]], order[first[fi] + 1]      # slot a has the lower cell id
-        ca, ka, cb, kb = a // 6, a % 6, b // 6, b % 6
+        ca, ka, cb, kb = a // Fc, a % Fc, b // Fc, b % Fc
 
-        neighbors = np.full((C, 6), -1, np.int32)
+        neighbors = np.full((C, Fc), -1, np.int32)
         neighbors[ca, ka] = cb
         neighbors[cb, kb] = ca
-        face_side = np.zeros((C, 6), np.int32)
+        face_side = np.zeros((C, Fc), np.int32)
         face_side[cb, kb] = 1
         adjacent = np.full((K, 2), -1, np.int32)
         cfi = np.full((K, 2), -1, np.int32)
@@ -141,8 +194,8 @@ class Mesh:
         is_b[fi] = 0
         bf = np.nonzero(counts == 1)[0]
         sb = order[first[bf]]
-        adjacent[bf, 0] = sb // 6
-        cfi[bf, 0] = sb % 6
+        adjacent[bf, 0] = sb // Fc
+        cfi[bf, 0] = sb % Fc
         dist = np.full((K, 3), -1.0, np.float32)
         dist[fi] = self.cell_centroids[cb] - self.cell_centroids[ca]
         mask = neighbors != -1
@@ -151,7 +204,7 @@ class Mesh:
         fp.num_faces = K
         fp.unique_faces = uniq.astype(np.int32)
         fp.centroid = fcen.astype(np.float32)
-        fp.num_nodes = np.full(K, 4, np.int32)
+        fp.num_nodes = num_nodes
         fp.adjacent_cells = adjacent
         fp.cell_face_index = cfi
         fp.distance = dist
@@ -164,8 +217,8 @@ class Mesh:
 
         cp = _Props()
         cp.num_cells = C
-        cp.faces_per_cell = 6
-        cp.nodes_per_face = 4
+        cp.faces_per_cell = Fc
+        cp.nodes_per_face = N
         cp.dimension = 3
         cp.centroids = self.cell_centroids
         cp.volumes = self.cell_volumes
@@ -207,7 +260,7 @@ class Mesh:
         valid = cols_cell != np.iinfo(np.int32).max
         counts = valid.sum(axis=1)
         rows_cols = [np.where(valid, O * cols_cell + k, -1) for k in range(O)]
-        stacked = np.stack(rows_cols, axis=1).reshape(C * O, 7)
+        stacked = np.stack(rows_cols, axis=1).reshape(C * O, cols_cell.shape[1])
         row_ptr = np.zeros(C * O + 1, np.int64)
         row_ptr[1:] = np.cumsum(np.repeat(counts, O))
         cols = stacked[stacked >= 0]
