@@ -189,3 +189,114 @@ def test_sample_line_uniform_grid():
     assert ids2[0] == n * j + n // 2 - 1 and ids2[1] == -1
     _, got2, _ = sample_line(m, {'v': vals}, (0.5, 0.3, 0.05), (1.5, 0.3, 0.05), 2)
     assert got2['v'][0] == vals[n * j + n // 2 - 1] and np.isnan(got2['v'][1])
+
+
+# one prism (gmsh type 6) with a triangle (2) and a quadrilateral (3) group
+MSH2_WEDGE = """$MeshFormat
+2.2 0 8
+$EndMeshFormat
+$PhysicalNames
+2
+2 1 "bottom"
+2 2 "side"
+$EndPhysicalNames
+$Nodes
+6
+1 0 0 0
+2 1 0 0
+3 0 1 0
+4 0 0 1
+5 1 0 1
+6 0 1 1
+$EndNodes
+$Elements
+3
+1 2 2 1 1 1 3 2
+2 3 2 2 2 1 2 5 4
+3 6 2 3 3 1 2 3 4 5 6
+$EndElements
+"""
+
+
+def test_read_handwritten_wedge_msh(tmp_path):
+    from warpcfd_amd.preprocess.io import VTK_WEDGE
+    p = tmp_path / 'wedge.msh'
+    p.write_text(MSH2_WEDGE)
+    d = read_msh(str(p))
+    assert d.cell_type == VTK_WEDGE and d.cells.tolist() == [[0, 1, 2, 3, 4, 5]]
+    assert d.groups['bottom'][1].tolist() == [[0, 2, 1, -1]]
+    assert d.groups['side'][1].tolist() == [[0, 1, 4, 3]]
+    m = Mesh(d)
+    apply_groups(m, d)
+    n = m.cell_properties.normal[0]
+    (fb,) = m.groups['bottom'].ids
+    (fs,) = m.groups['side'].ids
+    # the face ids of the bottom triangle and the y = 0 quadrilateral
+    k = list(m.cell_properties.faces[0])
+    assert np.allclose(n[k.index(fb)], [0, 0, -1]) and np.allclose(n[k.index(fs)], [0, -1, 0])
+
+
+def test_vtu_tetra_handwritten(tmp_path):
+    """VTK type 10, ascii: one tetrahedron."""
+    from warpcfd_amd.preprocess.io import VTK_TETRA
+    xml = ('<?xml version="1.0"?>\n<VTKFile type="UnstructuredGrid" version="0.1">\n'
+           '<UnstructuredGrid><Piece NumberOfPoints="4" NumberOfCells="1"><Points>'
+           '<DataArray type="Float32" NumberOfComponents="3" format="ascii">'
+           '0 0 0 1 0 0 0 1 0 0 0 1</DataArray></Points><Cells>'
+           '<DataArray type="Int32" Name="connectivity" format="ascii">0 1 2 3</DataArray>'
+           '<DataArray type="Int32" Name="offsets" format="ascii">4</DataArray>'
+           '<DataArray type="UInt8" Name="types" format="ascii">10</DataArray>'
+           '</Cells></Piece></UnstructuredGrid></VTKFile>\n')
+    p = tmp_path / 'tet.vtu'
+    p.write_text(xml)
+    d = read_vtu(str(p))
+    assert d.cell_type == VTK_TETRA and d.cells.tolist() == [[0, 1, 2, 3]]
+    assert abs(float(Mesh(d).cell_volumes[0]) - 1 / 6) < 1e-7
+
+
+def _split_box(cell_type):
+    from warpcfd_amd.preprocess.grid import structured_cells
+    nodes, cells = structured_cells((4, 3, 2), (1.0, 0.8, 0.6), jitter=0.1, seed=2,
+                                    cell_type=cell_type)
+    m = Mesh(nodes, cells)
+    define_boundary_walls(m)
+    return nodes, cells, m
+
+
+@pytest.mark.parametrize('cell_type', ['tetra', 'wedge'])
+@pytest.mark.parametrize('version', [2.2, 4.1])
+def test_msh_round_trip_tet_wedge(tmp_path, cell_type, version):
+    nodes, cells, m = _split_box(cell_type)
+    p = tmp_path / 'box.msh'
+    write_msh(str(p), nodes, cells, face_groups=face_groups_of(m), version=version,
+              cell_groups={'half': np.arange(0, len(cells) // 2)})
+    d = read_msh(str(p))
+    np.testing.assert_array_equal(d.cells, cells)
+    m2 = Mesh(d)
+    apply_groups(m2, d)
+    for name, g in m.groups.items():
+        assert np.array_equal(np.sort(m2.groups[name].ids), np.sort(g.ids)), name
+    fa, fb = m.face_properties, m2.face_properties
+    np.testing.assert_array_equal(fa.unique_faces, fb.unique_faces)
+    np.testing.assert_array_equal(fa.adjacent_cells, fb.adjacent_cells)
+
+
+@pytest.mark.parametrize('cell_type', ['tetra', 'wedge'])
+def test_vtu_round_trip_tet_wedge(tmp_path, cell_type):
+    nodes, cells, m = _split_box(cell_type)
+    p = tmp_path / 'box.vtu'
+    v = np.arange(len(cells), dtype=np.float32)
+    write_vtu(str(p), nodes, cells, {'v': v})
+    d = read_vtu(str(p))
+    np.testing.assert_array_equal(d.cells, cells)
+    np.testing.assert_array_equal(d.cell_data['v'], v)
+    np.testing.assert_array_equal(Mesh(d).cell_properties.neighbors, m.cell_properties.neighbors)
+
+
+def test_mixed_cell_types_rejected(tmp_path):
+    text = MSH2_WEDGE.replace('$Elements\n3\n', '$Elements\n4\n').replace(
+        '$EndElements', '4 4 2 3 3 1 2 3 4\n$EndElements')
+    p = tmp_path / 'mixed.msh'
+    p.write_text(text)
+    with pytest.raises(ValueError, match='single cell type'):
+        read_msh(str(p))

@@ -4,15 +4,17 @@ not).
 
 The reference builds meshes with gmsh and pyvista (preprocess/grid.py:8-159,
 mesh.py:17-43), which are absent here.  ``read_msh`` and ``read_vtu``
-return a ``MeshData`` -- points, hexahedra in file order, named groups --
-that ``Mesh`` accepts as it accepts a pyvista grid; node ids become 0-based
-positions in file order, as ``create_2D_grid`` makes them
-(grid.py:153-154).  ``apply_groups`` turns physical groups into the mesh's
-groups: boundary quadrilaterals into face groups (matched by their node
-sets), hexahedra into cell groups.  The writers produce files that gmsh,
-meshio and ParaView read, and ``write_vtu`` is also the result export (cell
-fields beside the mesh, as the reference's examples attach them to the
-pyvista grid, examples/LDC/LDC.py:61-65).
+return a ``MeshData`` -- points, the cells of one type (hexahedra,
+tetrahedra or wedges, the types the reference's Mesh takes, mesh.py:20,37)
+in file order, named groups -- that ``Mesh`` accepts as it accepts a pyvista
+grid; node ids become 0-based positions in file order, as
+``create_2D_grid`` makes them (grid.py:153-154).  ``apply_groups`` turns
+physical groups into the mesh's groups: boundary triangles and
+quadrilaterals into face groups (matched by their node sets), volume
+elements into cell groups.  The writers produce files that gmsh, meshio and
+ParaView read, and ``write_vtu`` is also the result export (cell fields
+beside the mesh, as the reference's examples attach them to the pyvista
+grid, examples/LDC/LDC.py:61-65).
 """
 import base64
 import re
@@ -22,24 +24,56 @@ import xml.etree.ElementTree as ET
 
 import numpy as np
 
-VTK_HEXAHEDRON = 12
-GMSH_HEX, GMSH_QUAD = 5, 3
+VTK_TETRA, VTK_HEXAHEDRON, VTK_WEDGE = 10, 12, 13
+GMSH_TRI, GMSH_QUAD, GMSH_TET, GMSH_HEX, GMSH_PRISM = 2, 3, 4, 5, 6
+# gmsh and VTK list the nodes of these three volume elements in the same order
+_GMSH_TO_VTK = {GMSH_TET: VTK_TETRA, GMSH_HEX: VTK_HEXAHEDRON, GMSH_PRISM: VTK_WEDGE}
+_VTK_TO_GMSH = {v: k for k, v in _GMSH_TO_VTK.items()}
+_NODES = {VTK_TETRA: 4, VTK_HEXAHEDRON: 8, VTK_WEDGE: 6}
+_TYPE_OF_NODES = {v: k for k, v in _NODES.items()}
+_FACE_TYPES = (GMSH_TRI, GMSH_QUAD)
+
+
+def _vtk_type(cells):
+    n = np.asarray(cells).shape[-1]
+    if n not in _TYPE_OF_NODES:
+        raise ValueError('cells must be tetrahedra (4 nodes), wedges (6) or hexahedra (8)')
+    return _TYPE_OF_NODES[n]
+
+
+def _pad4(faces):
+    """Face node lists as (Q, 4), triangles padded with -1 (the reference
+    pads a wedge's triangles so, mesh.py:131-135)."""
+    f = np.asarray(faces, np.int64)
+    f = f.reshape(-1, f.shape[-1]) if f.size else f.reshape(0, 4)
+    if f.shape[1] < 4:
+        f = np.hstack([f, -np.ones((f.shape[0], 4 - f.shape[1]), np.int64)])
+    return f
 
 
 class MeshData:
-    """Points (N, 3), hexahedra (C, 8) int32 (VTK node order), cell data
-    {name: (C, ...)} and groups {name: (dim, array)}: dim 3 -> hexahedron
-    indices, dim 2 -> quadrilateral node ids (Q, 4)."""
+    """Points (N, 3), cells (C, n) int32 of one VTK type (10 tetrahedron,
+    12 hexahedron, 13 wedge; VTK node order), cell data {name: (C, ...)}
+    and groups {name: (dim, array)}: dim 3 -> cell indices, dim 2 -> face
+    node ids (Q, 4), triangles padded with -1."""
 
-    def __init__(self, points, hexes, cell_data=None, groups=None):
+    def __init__(self, points, cells, cell_data=None, groups=None, cell_type=None):
         self.points = np.asarray(points, np.float64)
-        self.hexes = np.asarray(hexes, np.int32).reshape(-1, 8)
+        cells = np.asarray(cells, np.int32)
+        self.cell_type = int(cell_type) if cell_type is not None else _vtk_type(cells)
+        self.cells = cells.reshape(-1, _NODES[self.cell_type])
         self.cell_data = dict(cell_data or {})
         self.groups = dict(groups or {})
 
     @property
+    def hexes(self):
+        if self.cell_type != VTK_HEXAHEDRON:
+            raise AttributeError('the mesh holds no hexahedra')
+        return self.cells
+
+    @property
     def cells_dict(self):
-        return {VTK_HEXAHEDRON: self.hexes}
+        return {self.cell_type: self.cells}
 
 
 # ------------------------------------------------------------------ gmsh
@@ -55,9 +89,10 @@ def _ints(s):
 
 
 def read_msh(path):
-    """gmsh .msh, format 2.2 or 4.1, ASCII.  Hexahedra (type 5) become the
-    cells; physical groups of quadrilaterals (type 3) and hexahedra become
-    named groups (the physical name, else 'physical_<tag>')."""
+    """gmsh .msh, format 2.2 or 4.1, ASCII.  The volume elements -- all
+    hexahedra (type 5), tetrahedra (4) or prisms (6) -- become the cells;
+    physical groups of triangles (2), quadrilaterals (3) and volume elements
+    become named groups (the physical name, else 'physical_<tag>')."""
     with open(path, 'rb') as f:
         raw = f.read()
     text = raw.decode('latin-1')
@@ -91,27 +126,29 @@ def _finish(tags, coords, blocks, names):
         if np.any(p >= st.size) or np.any(st[np.minimum(p, st.size - 1)] != t):
             raise ValueError('an element references a node that is not defined')
         return order[p]
-    hexes, hex_phys, quads = [], [], {}
+    vols, vol_phys, faces = {}, {}, {}
     for etype, phys, nodes in blocks:
-        if etype == GMSH_HEX:
-            hexes.append(index(nodes))
-            hex_phys.append(np.full(len(nodes), -1 if phys is None else phys, np.int64)
-                            if np.isscalar(phys) or phys is None else phys)
-        elif etype == GMSH_QUAD:
-            ph = (np.full(len(nodes), -1 if phys is None else phys, np.int64)
-                  if np.isscalar(phys) or phys is None else phys)
+        ph = (np.full(len(nodes), -1 if phys is None else phys, np.int64)
+              if np.isscalar(phys) or phys is None else phys)
+        if etype in _GMSH_TO_VTK:
+            vols.setdefault(etype, []).append(index(nodes))
+            vol_phys.setdefault(etype, []).append(ph)
+        elif etype in _FACE_TYPES:
             for t in np.unique(ph[ph > 0]):
-                quads.setdefault(int(t), []).append(index(nodes[ph == t]))
-    if not hexes:
-        raise ValueError('no hexahedra (gmsh element type 5) in the file')
-    hexes = np.concatenate(hexes).astype(np.int32)
-    hp = np.concatenate(hex_phys)
+                faces.setdefault(int(t), []).append(_pad4(index(nodes[ph == t])))
+    if not vols:
+        raise ValueError('no volume elements (gmsh types 4, 5, 6) in the file')
+    if len(vols) > 1:
+        raise ValueError('Meshes can only contain a single cell type for now, Got multiple')
+    (etype, parts), = vols.items()
+    cells = np.concatenate(parts).astype(np.int32)
+    hp = np.concatenate(vol_phys[etype])
     groups = {}
-    for tag in sorted(quads):
-        groups[names.get((2, tag), f'physical_{tag}')] = (2, np.concatenate(quads[tag]).astype(np.int32))
+    for tag in sorted(faces):
+        groups[names.get((2, tag), f'physical_{tag}')] = (2, np.concatenate(faces[tag]).astype(np.int32))
     for tag in np.unique(hp[hp > 0]).tolist():
         groups[names.get((3, tag), f'physical_{tag}')] = (3, np.nonzero(hp == tag)[0].astype(np.int32))
-    return MeshData(coords, hexes, groups=groups)
+    return MeshData(coords, cells, groups=groups, cell_type=_GMSH_TO_VTK[etype])
 
 
 def _read_msh2(sec, names):
@@ -194,19 +231,27 @@ def _read_msh4(sec, names):
     return _finish(tags, coords, blocks, names)
 
 
+def _face_rows(q):
+    """(gmsh type, node lists) pairs of a (Q, 4) face array: triangles
+    (padded with -1) and quadrilaterals."""
+    tri = q[:, 3] < 0
+    return [(t, q[m][:, :n]) for t, m, n in ((GMSH_TRI, tri, 3), (GMSH_QUAD, ~tri, 4)) if m.any()]
+
+
 def write_msh(path, nodes, cells, face_groups=None, version=2.2, cell_groups=None):
-    """ASCII .msh of hexahedra, with physical groups: ``face_groups``
-    {name: (Q, 4) node ids of boundary quadrilaterals} and ``cell_groups``
-    {name: hexahedron indices}.  Node ids are written 1-based in array
-    order."""
+    """ASCII .msh of hexahedra, tetrahedra or wedges (prisms), with physical
+    groups: ``face_groups`` {name: (Q, 3 or 4) node ids of boundary faces,
+    triangles padded with -1} and ``cell_groups`` {name: cell indices}.
+    Node ids are written 1-based in array order."""
     nodes = np.asarray(nodes, np.float64)
     cells = np.asarray(cells, np.int64)
+    gtype = _VTK_TO_GMSH[_vtk_type(cells)]
     face_groups = face_groups or {}
     cell_groups = cell_groups or {}
     phys = []                                   # (dim, tag, name, element node arrays)
     tag = 1
     for name, q in face_groups.items():
-        phys.append((2, tag, name, np.asarray(q, np.int64).reshape(-1, 4)))
+        phys.append((2, tag, name, _pad4(q)))
         tag += 1
     hex_tag = np.zeros(len(cells), np.int64)
     for name, ids in cell_groups.items():
@@ -234,12 +279,13 @@ def write_msh(path, nodes, cells, face_groups=None, version=2.2, cell_groups=Non
         for dim, t, name, q in phys:
             if q is None:
                 continue
-            for e in q:
-                rows.append(f'{eid} {GMSH_QUAD} 2 {t} {t} ' + ' '.join(str(v + 1) for v in e))
-                eid += 1
+            for ft, fq in _face_rows(q):
+                for e in fq:
+                    rows.append(f'{eid} {ft} 2 {t} {t} ' + ' '.join(str(v + 1) for v in e))
+                    eid += 1
         for i, e in enumerate(cells):
             t = int(hex_tag[i])
-            rows.append(f'{eid} {GMSH_HEX} 2 {t} {max(t, 1)} ' + ' '.join(str(v + 1) for v in e))
+            rows.append(f'{eid} {gtype} 2 {t} {max(t, 1)} ' + ' '.join(str(v + 1) for v in e))
             eid += 1
         w(f'$Elements\n{len(rows)}\n' + '\n'.join(rows) + '\n$EndElements\n')
     else:
@@ -265,59 +311,67 @@ def write_msh(path, nodes, cells, face_groups=None, version=2.2, cell_groups=Non
         blocks = []
         eid = 1
         for i, (t, q) in enumerate(surf):
-            rows = []
-            for e in q:
-                rows.append(f'{eid} ' + ' '.join(str(v + 1) for v in e))
-                eid += 1
-            blocks.append(f'2 {i + 1} {GMSH_QUAD} {len(rows)}\n' + '\n'.join(rows))
+            for ft, fq in _face_rows(q):          # one block per element type
+                rows = []
+                for e in fq:
+                    rows.append(f'{eid} ' + ' '.join(str(v + 1) for v in e))
+                    eid += 1
+                blocks.append(f'2 {i + 1} {ft} {len(rows)}\n' + '\n'.join(rows))
         for et, t in vol_entities:
             sel = np.nonzero(hex_tag == t)[0]
             rows = []
             for c in sel:
                 rows.append(f'{eid} ' + ' '.join(str(v + 1) for v in cells[c]))
                 eid += 1
-            blocks.append(f'3 {et} {GMSH_HEX} {len(rows)}\n' + '\n'.join(rows))
+            blocks.append(f'3 {et} {gtype} {len(rows)}\n' + '\n'.join(rows))
         total = eid - 1
         w(f'$Elements\n{len(blocks)} {total} 1 {total}\n' + '\n'.join(blocks) + '\n$EndElements\n')
     with open(path, 'w') as f:
         f.write(''.join(out))
 
 
-def boundary_quads(mesh, face_ids):
+def boundary_faces(mesh, face_ids):
     """Node ids (Q, 4) of faces, in the owner cell's local face node order
-    (a closed cycle, as a gmsh or VTK quadrilateral needs)."""
-    from .mesh import HEX_FACE_NODES
+    (a closed cycle, as a gmsh or VTK polygon needs), triangles padded with
+    -1."""
+    from .mesh import local_faces
     fp = mesh.face_properties
     f = np.asarray(face_ids, np.int64)
     own, k0 = fp.adjacent_cells[f, 0], fp.cell_face_index[f, 0]
-    return mesh.cells[own[:, None], np.asarray(HEX_FACE_NODES)[k0]].astype(np.int32)
+    table = _pad4([r + [-1] * (4 - len(r)) for r in local_faces(mesh.cellType[1])])
+    loc = table[k0]
+    out = np.where(loc >= 0, mesh.cells[own[:, None], np.maximum(loc, 0)], -1)
+    return out.astype(np.int32)
+
+
+boundary_quads = boundary_faces
 
 
 def face_groups_of(mesh):
-    """{name: boundary quadrilaterals} of every face group of ``mesh``."""
-    return {name: boundary_quads(mesh, g.ids) for name, g in mesh.groups.items()
+    """{name: boundary faces} of every face group of ``mesh``."""
+    return {name: boundary_faces(mesh, g.ids) for name, g in mesh.groups.items()
             if g.type == 'face'}
 
 
 def apply_groups(mesh, data):
     """Add the physical groups of ``data`` (a MeshData) to ``mesh`` (a Mesh
-    built from it): quadrilateral groups as face groups (each quadrilateral
-    must be a face of the mesh), hexahedron groups as cell groups."""
+    built from it): face groups (each triangle or quadrilateral must be a
+    face of the mesh) and cell groups."""
     fp = mesh.face_properties
-    keys = np.sort(fp.unique_faces, axis=1)
+    keys = np.sort(_pad4(fp.unique_faces), axis=1)
     view = np.ascontiguousarray(keys).view([('', keys.dtype)] * 4).ravel()
     order = np.argsort(view)
     for name, (dim, ids) in data.groups.items():
         if dim == 3:
             mesh.add_group(name, ids, 'cell')
             continue
-        q = np.sort(np.asarray(ids, keys.dtype).reshape(-1, 4), axis=1)
+        q = np.sort(_pad4(ids).astype(keys.dtype), axis=1)
         qv = np.ascontiguousarray(q).view([('', keys.dtype)] * 4).ravel()
         p = np.searchsorted(view[order], qv)
         ok = (p < view.size)
         ok[ok] = view[order][p[ok]] == qv[ok]
         if not np.all(ok):
-            raise ValueError(f'group {name!r}: {int((~ok).sum())} quadrilaterals are no mesh face')
+            raise ValueError(f'group {name!r}: {int((~ok).sum())} faces are no mesh face')
         mesh.add_group(name, order[p].astype(np.int32), 'face')
     return mesh
 
@@ -373,8 +427,9 @@ def _decode_raw(buf, off, dtype, header, compressed):
 
 
 def read_vtu(path):
-    """VTK XML UnstructuredGrid of hexahedra (cell type 12): points, cells,
-    and every CellData array."""
+    """VTK XML UnstructuredGrid of one cell type -- hexahedra (12),
+    tetrahedra (10) or wedges (13): points, cells, and every CellData
+    array."""
     with open(path, 'rb') as f:
         raw = f.read()
     appended = None
@@ -419,16 +474,22 @@ def read_vtu(path):
     conn = read(arrays['connectivity']).astype(np.int64)
     offs = read(arrays['offsets']).astype(np.int64)
     types = read(arrays['types']).astype(np.int64)
-    if not np.all(types == VTK_HEXAHEDRON):
-        raise ValueError('only hexahedral cells (VTK type 12) are supported')
-    if not np.array_equal(offs, 8 * np.arange(1, len(types) + 1)):
-        raise ValueError('inconsistent hexahedron offsets')
+    kinds = np.unique(types)
+    if kinds.size > 1:
+        raise ValueError('Meshes can only contain a single cell type for now, Got multiple')
+    if kinds.size == 0 or int(kinds[0]) not in _NODES:
+        raise ValueError('only tetrahedra (VTK type 10), hexahedra (12) and wedges (13) '
+                         'are supported')
+    ctype = int(kinds[0])
+    nn = _NODES[ctype]
+    if not np.array_equal(offs, nn * np.arange(1, len(types) + 1)):
+        raise ValueError('inconsistent cell offsets')
     cell_data = {}
     cd = piece.find('CellData')
     if cd is not None:
         for da in cd.findall('DataArray'):
             cell_data[da.get('Name')] = read(da)
-    return MeshData(pts, conn.reshape(-1, 8), cell_data=cell_data)
+    return MeshData(pts, conn.reshape(-1, nn), cell_data=cell_data, cell_type=ctype)
 
 
 def _encode_binary(a, compress):
@@ -443,11 +504,14 @@ def _encode_binary(a, compress):
 
 
 def write_vtu(path, nodes, cells, cell_data=None, binary=True, compress=True):
-    """VTK XML UnstructuredGrid of hexahedra with CellData arrays (one value
-    or one vector per cell).  binary: inline base64 (UInt64 headers),
-    zlib-compressed when ``compress``; else ascii."""
+    """VTK XML UnstructuredGrid of hexahedra, tetrahedra or wedges (by the
+    nodes per cell) with CellData arrays (one value or one vector per
+    cell).  binary: inline base64 (UInt64 headers), zlib-compressed when
+    ``compress``; else ascii."""
     nodes = np.asarray(nodes, np.float64).reshape(-1, 3)
-    cells = np.asarray(cells, np.int64).reshape(-1, 8)
+    cells = np.asarray(cells, np.int64)
+    ctype = _vtk_type(cells)
+    nn = _NODES[ctype]
     C = len(cells)
     cell_data = cell_data or {}
     fmt = 'binary' if binary else 'ascii'
@@ -468,8 +532,8 @@ def write_vtu(path, nodes, cells, cell_data=None, binary=True, compress=True):
            f'<Piece NumberOfPoints="{len(nodes)}" NumberOfCells="{C}">\n']
     out.append('<Points>\n' + da('Points', nodes, 3) + '</Points>\n')
     out.append('<Cells>\n' + da('connectivity', cells.reshape(-1))
-               + da('offsets', 8 * np.arange(1, C + 1, dtype=np.int64))
-               + da('types', np.full(C, VTK_HEXAHEDRON, np.uint8)) + '</Cells>\n')
+               + da('offsets', nn * np.arange(1, C + 1, dtype=np.int64))
+               + da('types', np.full(C, ctype, np.uint8)) + '</Cells>\n')
     if cell_data:
         out.append('<CellData>\n')
         for name, a in cell_data.items():
