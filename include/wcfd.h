@@ -213,7 +213,7 @@ int wcfd_assemble_pressure(const wcfd_mesh* m, wcfd_bc bc, const float* face_val
                            float* values, float* rhs, float* div_out, void* stream);
 
 /* ------------------------------------------------- composable terms
- * The reference's term-by-term API (FV/terms/*.py), for callers that build
+ * The reference's term-by-term API (the FV/terms package), for callers that build
  * their own equations; the SIMPLE step uses the fused assembly above.
  *
  * Face-based term weights are the reference's Term.weights (C, 6, O, 3) in

@@ -41,65 +41,164 @@ __device__ inline FBuf cell_plane(const wcfd_mesh& m, const float* a, int q) {
   return FBuf(a + (size_t)q * m.n_cells, (uint32_t)m.n_cells * 4u);
 }
 
+// R internal faces of one thread: every index and face-array load of the R
+// faces first, then per field the gathers of all R faces before their
+// arithmetic, so R faces' loads are in flight together.  Each face gets the
+// arithmetic of the one-face form, in the same order.
+template <int R>
+__device__ inline void face_interp_internal(const wcfd_mesh& m, const int (&f)[R],
+                                            const bool (&on)[R], const float* __restrict__ cv,
+                                            const float* __restrict__ cg, float* __restrict__ fv,
+                                            int mask, int skew_mask, float* __restrict__ flux,
+                                            const float* __restrict__ rua,
+                                            float* __restrict__ rua_f) {
+  const int C = m.n_cells, F = m.n_faces, Fi = m.n_internal;
+  int o[R], n[R];
+  float psi[R], s0[R], s1[R], s2[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int g = on[r] ? f[r] : 0;
+    o[r] = m.owner[g];
+    n[r] = m.neigh[g];
+    psi[r] = m.psi[g];
+    s0[r] = m.skew[g];
+    s1[r] = m.skew[Fi + g];
+    s2[r] = m.skew[2 * Fi + g];
+  }
+  float val[R][4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if ((mask >> q) & 1) {
+      float co[R], cn[R], go[R][3], gn[R][3];
+      const bool sk = (skew_mask >> q) & 1;
+      const float* g = cg + (size_t)(3 * q) * C;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        co[r] = cv[q * C + o[r]];
+        cn[r] = cv[q * C + n[r]];
+        if (sk) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            go[r][d] = g[d * C + o[r]];
+            gn[r][d] = g[d * C + n[r]];
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const float psi2 = 1.0f - psi[r];
+        float v = co[r] * psi[r] + psi2 * cn[r];
+        if (sk) {
+          const float g0 = go[r][0] * psi[r] + psi2 * gn[r][0];
+          const float g1 = go[r][1] * psi[r] + psi2 * gn[r][1];
+          const float g2 = go[r][2] * psi[r] + psi2 * gn[r][2];
+          v = v + (g0 * s0[r] + g1 * s1[r] + g2 * s2[r]);
+        }
+        if (on[r]) fv[q * F + f[r]] = v;
+        val[r][q] = v;
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        val[r][q] = (q < 3 && flux && on[r]) ? fv[q * F + f[r]] : 0.0f;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (!on[r]) continue;
+    if (rua_f) rua_f[f[r]] = psi[r] * rua[o[r]] + (1.0f - psi[r]) * rua[n[r]];
+    if (flux) {
+      const int g = f[r];
+      const float n0 = m.normal[g], n1 = m.normal[F + g], n2 = m.normal[2 * F + g];
+      flux[g] = (val[r][0] * n0 + val[r][1] * n1 + val[r][2] * n2) * m.area[g];
+    }
+  }
+}
+
+__device__ inline void face_interp_boundary(const wcfd_mesh& m, wcfd_bc bc, int f,
+                                            const float* __restrict__ cv, float* __restrict__ fv,
+                                            int mask, float* __restrict__ flux,
+                                            const float* __restrict__ rua,
+                                            float* __restrict__ rua_f, int apply_dirichlet) {
+  const int C = m.n_cells, F = m.n_faces, Fi = m.n_internal, Fb = m.n_boundary;
+  float val[4];
+  const int j = f - Fi;
+  const int P = m.owner[f];
+  const float d = m.dmag[f];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const bool want = ((mask >> q) & 1) || (q < 3 && flux);
+    if (!want) continue;
+    float v;
+    const uint8_t t = bc.type[q * Fb + j];
+    if (((mask >> q) & 1) && t == WCFD_NEUMANN) {
+      v = d * bc.value[q * Fb + j] + cv[q * C + P];
+      fv[q * F + f] = v;
+    } else if (((mask >> q) & 1) && apply_dirichlet && t == WCFD_DIRICHLET) {
+      v = bc.value[q * Fb + j];
+      fv[q * F + f] = v;
+    } else {
+      v = fv[q * F + f];
+    }
+    val[q] = v;
+  }
+  if (rua_f) rua_f[f] = rua[P];
+  if (flux) {
+    const float n0 = m.normal[f], n1 = m.normal[F + f], n2 = m.normal[2 * F + f];
+    flux[f] = (val[0] * n0 + val[1] * n1 + val[2] * n2) * m.area[f];
+  }
+}
+
 __global__ void k_face_interp(wcfd_mesh m, BatchDev bd, wcfd_bc bc, const float* __restrict__ cv,
                               const float* __restrict__ cg, float* __restrict__ fv,
                               int mask, int skew_mask, float* __restrict__ flux,
                               const float* __restrict__ rua, float* __restrict__ rua_f,
                               int apply_dirichlet) {
   const int f = item_index(bd.faces);
-  const int C = m.n_cells, F = m.n_faces, Fi = m.n_internal, Fb = m.n_boundary;
-  if (f >= F || bd.face_skip(f)) return;
-  float val[4];
-  if (f < Fi) {
-    const int o = m.owner[f], n = m.neigh[f];
-    const float psi = m.psi[f];
-    const float psi2 = 1.0f - psi;
-    const float s0 = m.skew[f], s1 = m.skew[Fi + f], s2 = m.skew[2 * Fi + f];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      if ((mask >> q) & 1) {
-        float v = cv[q * C + o] * psi + psi2 * cv[q * C + n];
-        if ((skew_mask >> q) & 1) {
-          const float* g = cg + (size_t)(3 * q) * C;
-          float g0 = g[o] * psi + psi2 * g[n];
-          float g1 = g[C + o] * psi + psi2 * g[C + n];
-          float g2 = g[2 * C + o] * psi + psi2 * g[2 * C + n];
-          v = v + (g0 * s0 + g1 * s1 + g2 * s2);
-        }
-        fv[q * F + f] = v;
-        val[q] = v;
-      } else {
-        val[q] = (q < 3 && flux) ? fv[q * F + f] : 0.0f;
-      }
-    }
-    if (rua_f) rua_f[f] = psi * rua[o] + psi2 * rua[n];
+  if (f >= m.n_faces || bd.face_skip(f)) return;
+  if (f < m.n_internal) {
+    const int ff[1] = {f};
+    const bool on[1] = {true};
+    face_interp_internal<1>(m, ff, on, cv, cg, fv, mask, skew_mask, flux, rua, rua_f);
   } else {
-    const int j = f - Fi;
-    const int P = m.owner[f];
-    const float d = m.dmag[f];
+    face_interp_boundary(m, bc, f, cv, fv, mask, flux, rua, rua_f, apply_dirichlet);
+  }
+}
+
+// The items of a two-per-thread launch (blockDim = kBlock / 2 threads over
+// the kBlock items of k_face_interp's workgroup): items t and t + blockDim.
+__device__ inline void item_pair(const Items& it, int (&f)[2]) {
+  const int vb = (int)xcd_remap(blockIdx.x, gridDim.x);
+  const int B = (int)blockDim.x, t = (int)threadIdx.x;
+  if (!it.blk) {
+    f[0] = vb * 2 * B + t;
+    f[1] = f[0] + B;
+    return;
+  }
+  const int b = it.blk[vb] + t, e = it.blk[it.G + vb];
+  f[0] = b < e ? b : 0x7fffffff;
+  f[1] = b + B < e ? b + B : 0x7fffffff;
+}
+
+__global__ __launch_bounds__(kBlock / 2) void k_face_interp2(
+    wcfd_mesh m, BatchDev bd, wcfd_bc bc, const float* __restrict__ cv,
+    const float* __restrict__ cg, float* __restrict__ fv, int mask, int skew_mask,
+    float* __restrict__ flux, const float* __restrict__ rua, float* __restrict__ rua_f,
+    int apply_dirichlet) {
+  int f[2];
+  item_pair(bd.faces, f);
+  bool live[2], in[2];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const bool want = ((mask >> q) & 1) || (q < 3 && flux);
-      if (!want) continue;
-      float v;
-      const uint8_t t = bc.type[q * Fb + j];
-      if (((mask >> q) & 1) && t == WCFD_NEUMANN) {
-        v = d * bc.value[q * Fb + j] + cv[q * C + P];
-        fv[q * F + f] = v;
-      } else if (((mask >> q) & 1) && apply_dirichlet && t == WCFD_DIRICHLET) {
-        v = bc.value[q * Fb + j];
-        fv[q * F + f] = v;
-      } else {
-        v = fv[q * F + f];
-      }
-      val[q] = v;
-    }
-    if (rua_f) rua_f[f] = rua[P];
+  for (int r = 0; r < 2; ++r) {
+    live[r] = f[r] < m.n_faces && !bd.face_skip(f[r]);
+    in[r] = live[r] && f[r] < m.n_internal;
   }
-  if (flux) {
-    const float n0 = m.normal[f], n1 = m.normal[F + f], n2 = m.normal[2 * F + f];
-    flux[f] = (val[0] * n0 + val[1] * n1 + val[2] * n2) * m.area[f];
-  }
+  if (in[0] || in[1])
+    face_interp_internal<2>(m, f, in, cv, cg, fv, mask, skew_mask, flux, rua, rua_f);
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+    if (live[r] && !in[r])
+      face_interp_boundary(m, bc, f[r], cv, fv, mask, flux, rua, rua_f, apply_dirichlet);
 }
 
 // ------------------------------------------------------------ Green-Gauss
@@ -565,8 +664,12 @@ int launch_face_interp(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc, const
                        const float* rua, float* rua_f, int apply_dirichlet, hipStream_t s) {
   const unsigned g = items_grid(bd.faces, m.n_faces);
   if (g == 0) return WCFD_OK;
-  WCFD_LAUNCH(K_FACE_INTERP, s, k_face_interp, g, m, bd, bc, cv, cg, fv, mask, skew_mask, flux,
-              rua, rua_f, apply_dirichlet);
+  if constexpr (WCFD_FACE_PAIRS != 0)
+    WCFD_LAUNCH_B(K_FACE_INTERP, s, k_face_interp2, g, kBlock / 2, m, bd, bc, cv, cg, fv, mask,
+                  skew_mask, flux, rua, rua_f, apply_dirichlet);
+  else
+    WCFD_LAUNCH(K_FACE_INTERP, s, k_face_interp, g, m, bd, bc, cv, cg, fv, mask, skew_mask, flux,
+                rua, rua_f, apply_dirichlet);
   return check_launch();
 }
 
