@@ -24,6 +24,7 @@ cases one after another through the plain solver; its value is solves/s.
 """
 import argparse
 import json
+import re
 import os
 import sys
 import time
@@ -43,12 +44,17 @@ WORKLOADS = {
 }
 
 
-def build_case(workload, dims=(200, 40, 40)):
-    from warpcfd_amd.preprocess import Mesh, define_boundary_walls, channel_mesh, cavity_mesh
+def build_case(workload, dims=(200, 40, 40), cell_type='hexahedron'):
+    """The workload's nodes and cells; ``cell_type`` 'tetra' or 'wedge'
+    splits every hexahedron into 6 Kuhn tetrahedra or 2 prisms."""
+    from warpcfd_amd.preprocess import channel_mesh, cavity_mesh
+    from warpcfd_amd.preprocess.grid import split_hexes
     if workload == 'channel':
         nodes, cells = channel_mesh(*dims, (10.0, 2.0, 2.0), jitter=0.1, seed=0)
     else:
         nodes, cells = cavity_mesh(64)
+    if cell_type != 'hexahedron':
+        cells = split_hexes(cells, cell_type)
     return nodes, cells
 
 
@@ -78,22 +84,27 @@ def calibration_copy(mb):
     del x, y
 
 
-def algorithmic_bytes(kernel, C, F, Fi, Fb, nnz):
+def algorithmic_bytes(kernel, C, F, Fi, Fb, nnz, slots=6):
     """Compulsory bytes of one launch: every input element read once, every
-    output element written once (float32 / int32 = 4 B)."""
+    output element written once (float32 / int32 = 4 B).  ``slots``: matrix
+    slots the Krylov kernels read per row (6 hexahedra, 5 wedges, 4
+    tetrahedra); a row is the diagonal and the slots' coefficients (4 B
+    each) plus the slots' int16 neighbour offsets.  The face and cell
+    kernels read all six slot tables whatever the cell type."""
     rp = 4 * (C + 1)
+    mat = 4 * (slots + 1) + 2 * slots          # 40 B for six slots
     table = {
         # slot-major (ELL) matrix: diagonal + 6 coefficients (28 B) + int16
         # neighbour offsets (12 B) per row; q = A (z + beta p) gathered on the
         # fly; reads r, inv, p_in (12 B), writes p_out, q (8 B)
-        'cg_1': 40 * C + 12 * C + 8 * C,
+        'cg_1': mat * C + 12 * C + 8 * C,
         # x += a p, r -= a q: reads x p r q inv, writes x r
         'cg_2': 28 * C,
         # fused single-reduction CG iteration: ELL matrix 40 B, gathers of
         # r, w, s, inv (16 B), own p, x (8 B), writes p, s, x, r, w (20 B)
-        'cg_1_single': 40 * C + 16 * C + 8 * C + 20 * C,
-        'bicgstab_1': 40 * C + 4 * C + 36 * C + 24 * C + 12 * C,
-        'bicgstab_2': 40 * C + 4 * C + 24 * C + 24 * C,
+        'cg_1_single': mat * C + 16 * C + 8 * C + 20 * C,
+        'bicgstab_1': mat * C + 4 * C + 36 * C + 24 * C + 12 * C,
+        'bicgstab_2': mat * C + 4 * C + 24 * C + 24 * C,
         'bicgstab_3': 64 * C + 24 * C,
         # owner/neigh, psi, skew, normal, area, dmag, bc; cell values + grads; face values + flux
         'face_interp': 4 * F + 4 * Fi + 4 * Fi + 12 * Fi + 12 * F + 4 * F + 4 * Fb + 20 * Fb
@@ -434,7 +445,7 @@ def install_launch(spec):
     return {'grid': g, 'block': b, 'fold': fold, 'source': 'command line'}
 
 
-def rooflines_from(result, C, F, Fi, Fb, nnz, cg_single=True, note=None):
+def rooflines_from(result, C, F, Fi, Fb, nnz, cg_single=True, note=None, slots=6):
     """(roofline of the dominant kernel, per-kernel table with each kernel's own
     roofline) from the library profiler's {kernel: (total ms, launches)}."""
     kernels = {k: {'total_ms': round(v[0], 3), 'launches': v[1], 'avg_us': round(1e3 * v[0] / v[1], 3)}
@@ -443,7 +454,7 @@ def rooflines_from(result, C, F, Fi, Fb, nnz, cg_single=True, note=None):
     roofline = None
     for k, v in kernels.items():
         kb = algorithmic_bytes(k + ('_single' if k == 'cg_1' and cg_single else ''),
-                               C, F, Fi, Fb, nnz)
+                               C, F, Fi, Fb, nnz, slots)
         if kb is None or v['avg_us'] <= 0:
             continue
         gbs = kb / (v['avg_us'] * 1e-6) / 1e9
@@ -658,6 +669,8 @@ def main():
     ap.add_argument('--workload', default='channel',
                     choices=sorted(WORKLOADS) + ['batch64', 'serve', 'cylinder'])
     ap.add_argument('--cyl-h', type=float, default=0.05, help='cylinder: cell size')
+    ap.add_argument('--cell-type', default='hexahedron', choices=['hexahedron', 'tetra', 'wedge'],
+                    help='channel / cavity: split the hexahedra into tetrahedra or wedges')
     ap.add_argument('--channel-dims', default='200x40x40',
                     help='channel: hexahedra along x, y, z (the 10x2x2 box)')
     ap.add_argument('--p-refine', type=int, default=1,
@@ -730,9 +743,17 @@ def main():
         C = dims[0] * dims[1] * dims[2]
         wl['desc'] = wl['desc'].replace('200x40x40 hex (320000 cells)', f'{args.channel_dims} hex '
                                         f'({C} cells)')
+    if args.cell_type != 'hexahedron':
+        per = {'tetra': 6, 'wedge': 2}[args.cell_type]
+        name = {'tetra': 'Kuhn tetrahedra', 'wedge': 'prisms'}[args.cell_type]
+        wl['desc'] = re.sub(r' hex \((\d+) cells\)',
+                            lambda mo: f' hex, each split into {per} {name} '
+                                       f'({per * int(mo.group(1))} cells; kernels read the '
+                                       f'six-slot layout, unused slots on the null face)',
+                            wl['desc'])
     t_prep = time.perf_counter()
     print(f'# building {args.workload} mesh', file=sys.stderr, flush=True)
-    nodes, cells = build_case(args.workload, dims)
+    nodes, cells = build_case(args.workload, dims, args.cell_type)
     m = Mesh(nodes, cells)
     define_boundary_walls(m)
     model = FVM(m)
@@ -776,7 +797,8 @@ def main():
             for _ in range(args.steps):
                 solver.outer_iteration()
             torch.cuda.synchronize()
-        roofline, kernels = rooflines_from(prof.result, C, F, Fi, Fb, nnz, args.cg == 'single')
+        roofline, kernels = rooflines_from(prof.result, C, F, Fi, Fb, nnz, args.cg == 'single',
+                                           slots=model.layout.faces_per_cell)
 
     if args.calibration_copy_mb:
         calibration_copy(args.calibration_copy_mb)

@@ -6,7 +6,7 @@
 #define P(T, F) printf(#T "." #F " %zu\n", offsetof(T, F))
 int main(void) {
   printf("wcfd_mesh %zu\n", sizeof(wcfd_mesh));
-  P(wcfd_mesh, owner); P(wcfd_mesh, col); P(wcfd_mesh, slot_sn);
+  P(wcfd_mesh, n_slots); P(wcfd_mesh, owner); P(wcfd_mesh, col); P(wcfd_mesh, slot_sn);
   printf("wcfd_bc %zu\n", sizeof(wcfd_bc));
   printf("wcfd_solve_info %zu\n", sizeof(wcfd_solve_info));
   P(wcfd_solve_info, residual);

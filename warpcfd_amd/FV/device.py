@@ -60,7 +60,7 @@ class DeviceMesh:
         self.n_internal, self.n_boundary, self.nnz = L.n_internal, L.n_boundary, L.nnz
         d = _lib.WcfdMesh()
         d.n_cells, d.n_faces, d.n_internal = L.num_cells, L.num_faces, L.n_internal
-        d.n_boundary, d.nnz, d.reserved = L.n_boundary, L.nnz, 0
+        d.n_boundary, d.nnz, d.n_slots = L.n_boundary, L.nnz, L.faces_per_cell
         for name in ('owner', 'neigh', 'psi', 'skew', 'area', 'normal', 'dist', 'dmag', 'volume',
                      'slot_face', 'slot_side', 'slot_sn', 'slot_pos', 'diag_pos', 'row_ptr', 'col',
                      'slot_nbr'):

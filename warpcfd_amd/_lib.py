@@ -27,7 +27,7 @@ c_d = ctypes.c_double
 
 class WcfdMesh(ctypes.Structure):
     _fields_ = [('n_cells', c_i32), ('n_faces', c_i32), ('n_internal', c_i32),
-                ('n_boundary', c_i32), ('nnz', c_i32), ('reserved', c_i32),
+                ('n_boundary', c_i32), ('nnz', c_i32), ('n_slots', c_i32),
                 ('owner', c_p), ('neigh', c_p), ('psi', c_p), ('skew', c_p), ('area', c_p),
                 ('normal', c_p), ('dist', c_p), ('dmag', c_p), ('volume', c_p),
                 ('slot_face', c_p), ('slot_side', c_p), ('slot_sn', c_p), ('slot_pos', c_p),

@@ -437,7 +437,7 @@ class CaseArena:
                 self.items[name][:2 * n] = torch.from_numpy(both).to(dev)
         m = _lib.WcfdMesh()
         m.n_cells, m.n_faces, m.n_internal, m.n_boundary = self.Cc, self.F, self.Fic, self.Fbc
-        m.nnz, m.reserved = 0, 0
+        m.nnz, m.n_slots = 0, 0           # 0: six slots, right for every cell type
         for name in ('owner', 'neigh', 'psi', 'skew', 'area', 'normal', 'dist', 'dmag', 'volume',
                      'slot_face', 'slot_side', 'slot_sn', 'slot_nbr', 'slot_dnb'):
             setattr(m, name, getattr(self, name).data_ptr())

@@ -462,6 +462,7 @@ __device__ inline void store_partials(double (&v)[ND], double* __restrict__ part
 // CsrMat: a general CSR matrix (row_ptr, col, values, diagonal position).
 struct CsrMat {
   static constexpr bool kEll = false;
+  static constexpr int kRow = 7;      // register arrays of the prefetching forms (unused)
   int n;
   const int32_t* rp;
   const int32_t* col;
@@ -479,10 +480,14 @@ struct CsrMat {
 // slot k (0 at boundary slots, whose neighbour index is the row itself).
 // Every load is coalesced across a wavefront and no row pointer is read.
 // With D16 the neighbour is stored as an int16 offset from the row (the RCM
-// band fits), halving the index bytes.
-template <bool D16>
+// band fits), halving the index bytes.  NS slots are read (6 hexahedra, 5
+// wedges, 4 tetrahedra); the planes of slots NS..5 hold zeros and are
+// skipped.
+template <bool D16, int NS = 6>
 struct EllMat {
   static constexpr bool kEll = true;
+  static constexpr int kSlots = NS;
+  static constexpr int kRow = NS + 1;   // diagonal + slots
   int n;
   const float* val;
   const int32_t* nbr;
@@ -490,23 +495,23 @@ struct EllMat {
   __device__ float diag(int i) const { return val[i]; }
   // the row as 7 (coefficient, column) pairs in row() order, for kernels
   // that issue every load of a row before they wait on anything
-  __device__ void load(int i, float (&a)[7], int (&c)[7]) const {
+  __device__ void load(int i, float (&a)[kRow], int (&c)[kRow]) const {
     a[0] = val[i];
     c[0] = i;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
+    for (int k = 0; k < NS; ++k) {
       c[k + 1] = D16 ? i + (int)dnb[k * n + i] : nbr[k * n + i];
       a[k + 1] = val[(k + 1) * n + i];
     }
   }
   // The same row with the columns as byte offsets of float vectors
   // (o[j] = 4 c[j]); every load is a uniform base plus a 32-bit offset.
-  __device__ void load_off(int i, float (&a)[7], uint32_t (&o)[7]) const {
+  __device__ void load_off(int i, float (&a)[kRow], uint32_t (&o)[kRow]) const {
     const uint32_t oi = (uint32_t)i * 4u;
     a[0] = ld_mat(val, oi);
     o[0] = oi;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
+    for (int k = 0; k < NS; ++k) {
       const size_t col0 = (size_t)k * (size_t)n;
       const int cj = D16 ? i + (int)ld_mat(dnb + col0, (uint32_t)i * 2u)
                          : ld_mat(nbr + col0, oi);
@@ -518,7 +523,7 @@ struct EllMat {
   __device__ void row(int i, F&& f) const {
     f(val[i], i);
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
+    for (int k = 0; k < NS; ++k) {
       const int c = D16 ? i + (int)dnb[k * n + i] : nbr[k * n + i];
       f(val[(k + 1) * n + i], c);
     }

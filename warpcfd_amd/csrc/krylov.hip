@@ -236,15 +236,15 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_1(M A, Seg sg, int it, c
   constexpr bool pf = PF && M::kEll;
   const int i0 = br.begin + (int)threadIdx.x;
   const bool has = pf && i0 < br.end;
-  float a[7], rc[7], ic[7], pc[7];
+  float a[M::kRow], rc[M::kRow], ic[M::kRow], pc[M::kRow];
   if constexpr (M::kEll) {
     if (has) {
-      uint32_t o[7];
+      uint32_t o[M::kRow];
       A.load_off(i0, a, o);
       const uint32_t nb4 = (uint32_t)A.n * 4u;
       const FBuf R(r, nb4), I(inv, nb4), P(p_in, nb4);
 #pragma unroll
-      for (int j = 0; j < 7; ++j) {
+      for (int j = 0; j < M::kRow; ++j) {
         rc[j] = R(o[j]);
         ic[j] = I(o[j]);
         pc[j] = it ? P(o[j]) : 0.f;
@@ -265,7 +265,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_1(M A, Seg sg, int it, c
     if (has) {
       float qi = 0.f;
 #pragma unroll
-      for (int j = 0; j < 7; ++j) {
+      for (int j = 0; j < M::kRow; ++j) {
         const float z = rc[j] * ic[j];
         qi += a[j] * (first ? z : z + beta * pc[j]);
       }
@@ -397,17 +397,17 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int i
   constexpr bool pf = PF && M::kEll;
   const int i0 = br.begin + (int)threadIdx.x;
   const bool has = pf && i0 < br.end;
-  float a[7], wc[7], sc[7], ic[7], rc[7];
+  float a[M::kRow], wc[M::kRow], sc[M::kRow], ic[M::kRow], rc[M::kRow];
   float p0 = 0.f, x0 = 0.f;
   const uint32_t o0 = (uint32_t)i0 * 4u;
   if constexpr (M::kEll) {
     if (has) {
-      uint32_t o[7];
+      uint32_t o[M::kRow];
       A.load_off(i0, a, o);
       const uint32_t nb4 = (uint32_t)A.n * 4u;
       const FBuf W(w_in, nb4), Sv(s_in, nb4), I(inv, nb4), R(r_in, nb4);
 #pragma unroll
-      for (int j = 0; j < 7; ++j) {
+      for (int j = 0; j < M::kRow; ++j) {
         wc[j] = W(o[j]);
         sc[j] = first ? 0.f : Sv(o[j]);
         ic[j] = I(o[j]);
@@ -439,7 +439,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int i
     if (has) {
       float wn = 0.f;
 #pragma unroll
-      for (int j = 0; j < 7; ++j) {
+      for (int j = 0; j < M::kRow; ++j) {
         const float scj = first ? wc[j] : wc[j] + beta * sc[j];
         wn += a[j] * (ic[j] * (rc[j] - alpha * scj));
       }
@@ -503,7 +503,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter2(M A, Seg sg, int 
   const bool first = (it == 0);
   static_assert(M::kEll, "the two-row form reads ELL rows");
   constexpr int NR = 2;
-  float a[NR][7], wc[NR][7], sc[NR][7], ic[NR][7], rc[NR][7];
+  float a[NR][M::kRow], wc[NR][M::kRow], sc[NR][M::kRow], ic[NR][M::kRow], rc[NR][M::kRow];
   float p0[NR], x0[NR];
   bool has[NR];
   uint32_t o0[NR];
@@ -517,10 +517,10 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter2(M A, Seg sg, int 
     if (has[q]) {
       const uint32_t nb4 = (uint32_t)A.n * 4u;
       const FBuf W(w_in, nb4), Sv(s_in, nb4), I(inv, nb4), Rv(r_in, nb4);
-      uint32_t o[7];
+      uint32_t o[M::kRow];
       A.load_off(i, a[q], o);
 #pragma unroll
-      for (int j = 0; j < 7; ++j) {
+      for (int j = 0; j < M::kRow; ++j) {
         wc[q][j] = W(o[j]);
         sc[q][j] = first ? 0.f : Sv(o[j]);
         ic[q][j] = I(o[j]);
@@ -553,7 +553,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter2(M A, Seg sg, int 
     if (!has[q]) continue;
     float wn = 0.f;
 #pragma unroll
-    for (int j = 0; j < 7; ++j) {
+    for (int j = 0; j < M::kRow; ++j) {
       const float scj = first ? wc[q][j] : wc[q][j] + beta * sc[q][j];
       wn += a[q][j] * (ic[q][j] * (rc[q][j] - alpha * scj));
     }
@@ -621,15 +621,15 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_1(M A, Seg sg, int it, c
   constexpr bool pf = PF && M::kEll;
   const int i0 = br.begin + (int)threadIdx.x;
   const bool has = pf && i0 < br.end;
-  float a[7], ic[7], rr[NV][7], pp[NV][7], vv[NV][7], r0i[NV];
+  float a[M::kRow], ic[M::kRow], rr[NV][M::kRow], pp[NV][M::kRow], vv[NV][M::kRow], r0i[NV];
   if constexpr (M::kEll) {
     if (has) {
-      uint32_t o[7];
+      uint32_t o[M::kRow];
       A.load_off(i0, a, o);
       const uint32_t nb4 = (uint32_t)n * 4u * NV;
       const FBuf I(inv, (uint32_t)n * 4u), R(r, nb4), P(p_in, nb4), V(v_in, nb4);
 #pragma unroll
-      for (int j = 0; j < 7; ++j) {
+      for (int j = 0; j < M::kRow; ++j) {
         ic[j] = I(o[j]);
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
@@ -662,7 +662,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_1(M A, Seg sg, int it, c
 #pragma unroll
       for (int q = 0; q < NV; ++q) v[q] = 0.f;
 #pragma unroll
-      for (int j = 0; j < 7; ++j) {
+      for (int j = 0; j < M::kRow; ++j) {
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
           const float pc = first ? rr[q][j] : rr[q][j] + beta * (pp[q][j] - omega * vv[q][j]);
@@ -719,15 +719,15 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_2(M A, Seg sg, int it, c
   constexpr bool pf = PF && M::kEll;
   const int i0 = br.begin + (int)threadIdx.x;
   const bool has = pf && i0 < br.end;
-  float a[7], ic[7], rr[NV][7], vv[NV][7];
+  float a[M::kRow], ic[M::kRow], rr[NV][M::kRow], vv[NV][M::kRow];
   if constexpr (M::kEll) {
     if (has) {
-      uint32_t o[7];
+      uint32_t o[M::kRow];
       A.load_off(i0, a, o);
       const uint32_t nb4 = (uint32_t)n * 4u * NV;
       const FBuf I(inv, (uint32_t)n * 4u), R(r, nb4), V(v, nb4);
 #pragma unroll
-      for (int j = 0; j < 7; ++j) {
+      for (int j = 0; j < M::kRow; ++j) {
         ic[j] = I(o[j]);
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
@@ -749,7 +749,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_2(M A, Seg sg, int it, c
 #pragma unroll
       for (int q = 0; q < NV; ++q) tv[q] = 0.f;
 #pragma unroll
-      for (int j = 0; j < 7; ++j) {
+      for (int j = 0; j < M::kRow; ++j) {
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
           const float sc = rr[q][j] - alpha * vv[q][j];
@@ -1285,7 +1285,13 @@ static int bicg_dispatch(const M& A, const KLaunch& L, int nvec, const float* b,
 // ELL matrix of a mesh, with or without int16 neighbour offsets
 template <class F>
 static int with_ell(const wcfd_mesh& m, const float* values, F&& f) {
-  if (m.slot_dnb) return f(EllMat<true>{m.n_cells, values, m.slot_nbr, m.slot_dnb});
+  // tetrahedra and wedges read 4 or 5 slots (int16 offsets only; the int32
+  // form reads all six, the unused ones zero)
+  if (m.slot_dnb) {
+    if (m.n_slots == 4) return f(EllMat<true, 4>{m.n_cells, values, m.slot_nbr, m.slot_dnb});
+    if (m.n_slots == 5) return f(EllMat<true, 5>{m.n_cells, values, m.slot_nbr, m.slot_dnb});
+    return f(EllMat<true>{m.n_cells, values, m.slot_nbr, m.slot_dnb});
+  }
   return f(EllMat<false>{m.n_cells, values, m.slot_nbr, m.slot_dnb});
 }
 
