@@ -23,11 +23,6 @@ constexpr int kMaxInlineFold = 2048;    // default: largest per-case fold done i
 #ifndef WCFD_NT_STORE
 #define WCFD_NT_STORE 1
 #endif
-// face interpolation with two faces per thread (k_face_interp2) rather than
-// one (k_face_interp); both give the same values bit for bit
-#ifndef WCFD_FACE_PAIRS
-#define WCFD_FACE_PAIRS 1
-#endif
 #ifndef WCFD_NT_MATRIX
 #define WCFD_NT_MATRIX 0
 #endif

@@ -41,78 +41,76 @@ __device__ inline FBuf cell_plane(const wcfd_mesh& m, const float* a, int q) {
   return FBuf(a + (size_t)q * m.n_cells, (uint32_t)m.n_cells * 4u);
 }
 
-// R internal faces of one thread: every index and face-array load of the R
-// faces first, then per field the gathers of all R faces before their
-// arithmetic, so R faces' loads are in flight together.  Each face gets the
-// arithmetic of the one-face form, in the same order.
-template <int R>
-__device__ inline void face_interp_internal(const wcfd_mesh& m, const int (&f)[R],
-                                            const bool (&on)[R], const float* __restrict__ cv,
+// An internal face in three phases: every load (indices, then the cell
+// values and gradients of every field, the face normal and area), then the
+// arithmetic, then every store.  With no store between two fields, all of
+// the face's gathers are in flight together rather than one field's at a
+// time (2-3 % faster at 4.4-8.6 M cells; two faces per thread this way
+// needs 104 VGPRs and lost 25 %).
+__device__ inline void face_interp_internal(const wcfd_mesh& m, int f,
+                                            const float* __restrict__ cv,
                                             const float* __restrict__ cg, float* __restrict__ fv,
                                             int mask, int skew_mask, float* __restrict__ flux,
                                             const float* __restrict__ rua,
                                             float* __restrict__ rua_f) {
   const int C = m.n_cells, F = m.n_faces, Fi = m.n_internal;
-  int o[R], n[R];
-  float psi[R], s0[R], s1[R], s2[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int g = on[r] ? f[r] : 0;
-    o[r] = m.owner[g];
-    n[r] = m.neigh[g];
-    psi[r] = m.psi[g];
-    s0[r] = m.skew[g];
-    s1[r] = m.skew[Fi + g];
-    s2[r] = m.skew[2 * Fi + g];
-  }
-  float val[R][4];
+  const int o = m.owner[f], n = m.neigh[f];
+  const float psi = m.psi[f];
+  const float s0 = m.skew[f], s1 = m.skew[Fi + f], s2 = m.skew[2 * Fi + f];
+  // loads of every field
+  float co[4], cn[4], go[4][3], gn[4][3], old[3];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     if ((mask >> q) & 1) {
-      float co[R], cn[R], go[R][3], gn[R][3];
-      const bool sk = (skew_mask >> q) & 1;
-      const float* g = cg + (size_t)(3 * q) * C;
+      const float* gq = cg + (size_t)(3 * q) * C;
+      co[q] = cv[q * C + o];
+      cn[q] = cv[q * C + n];
+      if ((skew_mask >> q) & 1) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        co[r] = cv[q * C + o[r]];
-        cn[r] = cv[q * C + n[r]];
-        if (sk) {
-#pragma unroll
-          for (int d = 0; d < 3; ++d) {
-            go[r][d] = g[d * C + o[r]];
-            gn[r][d] = g[d * C + n[r]];
-          }
+        for (int d = 0; d < 3; ++d) {
+          go[q][d] = gq[d * C + o];
+          gn[q][d] = gq[d * C + n];
         }
       }
+    } else if (q < 3 && flux) {
+      old[q] = fv[q * F + f];
+    }
+  }
+  float nr[3], ar = 0.f, ro = 0.f, rn = 0.f;
+  if (flux) {
+    nr[0] = m.normal[f];
+    nr[1] = m.normal[F + f];
+    nr[2] = m.normal[2 * F + f];
+    ar = m.area[f];
+  }
+  if (rua_f) {
+    ro = rua[o];
+    rn = rua[n];
+  }
+  // arithmetic
+  const float psi2 = 1.0f - psi;
+  float val[4];
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const float psi2 = 1.0f - psi[r];
-        float v = co[r] * psi[r] + psi2 * cn[r];
-        if (sk) {
-          const float g0 = go[r][0] * psi[r] + psi2 * gn[r][0];
-          const float g1 = go[r][1] * psi[r] + psi2 * gn[r][1];
-          const float g2 = go[r][2] * psi[r] + psi2 * gn[r][2];
-          v = v + (g0 * s0[r] + g1 * s1[r] + g2 * s2[r]);
-        }
-        if (on[r]) fv[q * F + f[r]] = v;
-        val[r][q] = v;
+  for (int q = 0; q < 4; ++q) {
+    if ((mask >> q) & 1) {
+      float v = co[q] * psi + psi2 * cn[q];
+      if ((skew_mask >> q) & 1) {
+        const float g0 = go[q][0] * psi + psi2 * gn[q][0];
+        const float g1 = go[q][1] * psi + psi2 * gn[q][1];
+        const float g2 = go[q][2] * psi + psi2 * gn[q][2];
+        v = v + (g0 * s0 + g1 * s1 + g2 * s2);
       }
+      val[q] = v;
     } else {
-#pragma unroll
-      for (int r = 0; r < R; ++r)
-        val[r][q] = (q < 3 && flux && on[r]) ? fv[q * F + f[r]] : 0.0f;
+      val[q] = (q < 3 && flux) ? old[q] : 0.0f;
     }
   }
+  // stores
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    if (!on[r]) continue;
-    if (rua_f) rua_f[f[r]] = psi[r] * rua[o[r]] + (1.0f - psi[r]) * rua[n[r]];
-    if (flux) {
-      const int g = f[r];
-      const float n0 = m.normal[g], n1 = m.normal[F + g], n2 = m.normal[2 * F + g];
-      flux[g] = (val[r][0] * n0 + val[r][1] * n1 + val[r][2] * n2) * m.area[g];
-    }
-  }
+  for (int q = 0; q < 4; ++q)
+    if ((mask >> q) & 1) fv[q * F + f] = val[q];
+  if (rua_f) rua_f[f] = psi * ro + psi2 * rn;
+  if (flux) flux[f] = (val[0] * nr[0] + val[1] * nr[1] + val[2] * nr[2]) * ar;
 }
 
 __device__ inline void face_interp_boundary(const wcfd_mesh& m, wcfd_bc bc, int f,
@@ -157,48 +155,10 @@ __global__ void k_face_interp(wcfd_mesh m, BatchDev bd, wcfd_bc bc, const float*
   const int f = item_index(bd.faces);
   if (f >= m.n_faces || bd.face_skip(f)) return;
   if (f < m.n_internal) {
-    const int ff[1] = {f};
-    const bool on[1] = {true};
-    face_interp_internal<1>(m, ff, on, cv, cg, fv, mask, skew_mask, flux, rua, rua_f);
+    face_interp_internal(m, f, cv, cg, fv, mask, skew_mask, flux, rua, rua_f);
   } else {
     face_interp_boundary(m, bc, f, cv, fv, mask, flux, rua, rua_f, apply_dirichlet);
   }
-}
-
-// The items of a two-per-thread launch (blockDim = kBlock / 2 threads over
-// the kBlock items of k_face_interp's workgroup): items t and t + blockDim.
-__device__ inline void item_pair(const Items& it, int (&f)[2]) {
-  const int vb = (int)xcd_remap(blockIdx.x, gridDim.x);
-  const int B = (int)blockDim.x, t = (int)threadIdx.x;
-  if (!it.blk) {
-    f[0] = vb * 2 * B + t;
-    f[1] = f[0] + B;
-    return;
-  }
-  const int b = it.blk[vb] + t, e = it.blk[it.G + vb];
-  f[0] = b < e ? b : 0x7fffffff;
-  f[1] = b + B < e ? b + B : 0x7fffffff;
-}
-
-__global__ __launch_bounds__(kBlock / 2) void k_face_interp2(
-    wcfd_mesh m, BatchDev bd, wcfd_bc bc, const float* __restrict__ cv,
-    const float* __restrict__ cg, float* __restrict__ fv, int mask, int skew_mask,
-    float* __restrict__ flux, const float* __restrict__ rua, float* __restrict__ rua_f,
-    int apply_dirichlet) {
-  int f[2];
-  item_pair(bd.faces, f);
-  bool live[2], in[2];
-#pragma unroll
-  for (int r = 0; r < 2; ++r) {
-    live[r] = f[r] < m.n_faces && !bd.face_skip(f[r]);
-    in[r] = live[r] && f[r] < m.n_internal;
-  }
-  if (in[0] || in[1])
-    face_interp_internal<2>(m, f, in, cv, cg, fv, mask, skew_mask, flux, rua, rua_f);
-#pragma unroll
-  for (int r = 0; r < 2; ++r)
-    if (live[r] && !in[r])
-      face_interp_boundary(m, bc, f[r], cv, fv, mask, flux, rua, rua_f, apply_dirichlet);
 }
 
 // ------------------------------------------------------------ Green-Gauss
@@ -309,6 +269,8 @@ __global__ void k_assemble_momentum(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
   const FBuf Fl_(flux, (uint32_t)m.n_faces * 4u);
   const FBuf area(m.area, (uint32_t)m.n_faces * 4u), dmag(m.dmag, (uint32_t)m.n_faces * 4u);
   const float V = ld_off(m.volume, c4);
+  // (storing the gradients at the end, after every load, measured 1.5 %
+  // slower here, unlike the face and corrector kernels)
   float gp[3];
   if (compute_grad) {
 #pragma unroll
@@ -571,35 +533,56 @@ __global__ void k_grad_p_correct(wcfd_mesh m, Seg sg, const float* __restrict__ 
     const uint32_t c4 = (uint32_t)c * 4u;
     uint32_t f4[6];
     slot_faces(m, c4, f4);
+    // every load (the cell's gradient sums, rUA, HbyA, the previous
+    // velocity, the corrected face fluxes) before the first store
     float g[3];
     gg_field(m, c4, f4, pf, ld_off(m.volume, c4), g);
-#pragma unroll
-    for (int i = 0; i < 9; ++i) st_plain(cg + i * Cz, c4, 0.f);
-    st_plain(cg + 9 * Cz, c4, g[0]);
-    st_plain(cg + 10 * Cz, c4, g[1]);
-    st_plain(cg + 11 * Cz, c4, g[2]);
     const float r = ld_off(rua, c4);
+    float hb[3], up[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      const float h = ld_off(hbya + q * Cz, c4) - r * g[q];
-      st_plain(hbya + q * Cz, c4, h);
-      st_plain(cv + q * Cz, c4, h);
+      hb[q] = ld_off(hbya + q * Cz, c4);
+      up[q] = (fc && u_prev) ? ld_off(u_prev + q * Cz, c4) : 0.f;
+    }
+    float fk[6];
+    int sd[6];
+    if (fc) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        fk[k] = fcb(f4[k]);
+        sd[k] = ld_off(m.slot_side + k * Cz, (uint32_t)c);
+      }
+    }
+    float h[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      h[q] = hb[q] - r * g[q];
       if (fc && u_prev) {
-        const double dq = (double)h - (double)ld_off(u_prev + q * Cz, c4);
+        const double dq = (double)h[q] - (double)up[q];
         acc[2] += dq * dq;
-        acc[3] += (double)h * (double)h;
+        acc[3] += (double)h[q] * (double)h[q];
       }
     }
     if (fc) {
       double num = 0.0, den = 0.0;
 #pragma unroll
       for (int k = 0; k < 6; ++k) {
-        const double v = (double)fcb(f4[k]);
-        num += ld_off(m.slot_side + k * Cz, (uint32_t)c) == 0 ? v : -v;
+        const double v = (double)fk[k];
+        num += sd[k] == 0 ? v : -v;
         den += fabs(v);
       }
       acc[0] += fabs(num);
       acc[1] += den;
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) st_plain(cg + i * Cz, c4, 0.f);
+    st_plain(cg + 9 * Cz, c4, g[0]);
+    st_plain(cg + 10 * Cz, c4, g[1]);
+    st_plain(cg + 11 * Cz, c4, g[2]);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      st_plain(hbya + q * Cz, c4, h[q]);
+      st_plain(cv + q * Cz, c4, h[q]);
     }
   }
   if (fc) store_partials<4>(acc, partials, sg.G, br.vb);
@@ -664,12 +647,8 @@ int launch_face_interp(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc, const
                        const float* rua, float* rua_f, int apply_dirichlet, hipStream_t s) {
   const unsigned g = items_grid(bd.faces, m.n_faces);
   if (g == 0) return WCFD_OK;
-  if constexpr (WCFD_FACE_PAIRS != 0)
-    WCFD_LAUNCH_B(K_FACE_INTERP, s, k_face_interp2, g, kBlock / 2, m, bd, bc, cv, cg, fv, mask,
-                  skew_mask, flux, rua, rua_f, apply_dirichlet);
-  else
-    WCFD_LAUNCH(K_FACE_INTERP, s, k_face_interp, g, m, bd, bc, cv, cg, fv, mask, skew_mask, flux,
-                rua, rua_f, apply_dirichlet);
+  WCFD_LAUNCH(K_FACE_INTERP, s, k_face_interp, g, m, bd, bc, cv, cg, fv, mask, skew_mask, flux,
+              rua, rua_f, apply_dirichlet);
   return check_launch();
 }
 
