@@ -128,13 +128,17 @@ ARENA_CANDIDATES = (
 )
 
 
-def autotune_arena(specs, arena_kw, candidates=ARENA_CANDIDATES, warm=2, steps=4):
+def autotune_arena(specs, arena_kw, candidates=ARENA_CANDIDATES, warm=2, steps=4, tie=0.02):
     """Pick the launch shape of a batch arena: for every candidate
     (pressure CG variant, chunk, block) build the arena for ``specs``, run
     ``warm`` untimed and ``steps`` timed SIMPLE iterations (the wide phase,
     every case active) and return the fastest as a dict of arena keywords
-    plus the timings.  The shape is fixed for the whole run afterwards, so
-    results stay independent of the mix."""
+    plus the timings.  Candidates within ``tie`` (relative) of the fastest
+    count as equal and the first of them in ``candidates`` order is taken,
+    so timing noise does not switch shapes between runs (a shape fixes the
+    reduction order, hence the last bits of the results).  The shape is
+    fixed for the whole run afterwards, so results stay independent of the
+    mix."""
     import time
     import torch
     from .batch import CaseArena
@@ -153,6 +157,7 @@ def autotune_arena(specs, arena_kw, candidates=ARENA_CANDIDATES, warm=2, steps=4
                         'ms_per_iteration': 1e3 * (time.perf_counter() - t0) / steps})
         del a
         torch.cuda.empty_cache()
-    best = min(results, key=lambda r: r['ms_per_iteration'])
+    fastest = min(r['ms_per_iteration'] for r in results)
+    best = next(r for r in results if r['ms_per_iteration'] <= (1.0 + tie) * fastest)
     return {'cg_variant': best['cg_variant'], 'chunk': best['chunk'], 'block': best['block'],
             'candidates': results}
