@@ -130,3 +130,12 @@ def test_monitor_rule():
     # slow decay: E = dbar rho/(1-rho) above the target until d is small enough
     mon = Monitor(1e-6, 5e-5, 0.0)
     assert not _geometric_run(mon, 0.999, 1e-6, 100)   # E ~ 1e-3
+
+
+def test_arena_tuner_ties_take_the_first_candidate():
+    from warpcfd_amd.tune import pick_with_ties
+    r = [{'n': 0, 't': 6.364}, {'n': 1, 't': 6.618}, {'n': 2, 't': 6.353}, {'n': 3, 't': 7.2}]
+    assert pick_with_ties(r, 't', 0.02)['n'] == 0        # 0.2 % slower than the fastest: a tie
+    assert pick_with_ties(r, 't', 0.0)['n'] == 2         # no tolerance: the fastest
+    r[0]['t'] = 6.6
+    assert pick_with_ties(r, 't', 0.02)['n'] == 2        # 3.9 % slower: not a tie

@@ -157,7 +157,13 @@ def autotune_arena(specs, arena_kw, candidates=ARENA_CANDIDATES, warm=2, steps=4
                         'ms_per_iteration': 1e3 * (time.perf_counter() - t0) / steps})
         del a
         torch.cuda.empty_cache()
-    fastest = min(r['ms_per_iteration'] for r in results)
-    best = next(r for r in results if r['ms_per_iteration'] <= (1.0 + tie) * fastest)
+    best = pick_with_ties(results, 'ms_per_iteration', tie)
     return {'cg_variant': best['cg_variant'], 'chunk': best['chunk'], 'block': best['block'],
             'candidates': results}
+
+
+def pick_with_ties(results, key, tie):
+    """The first entry of ``results`` whose ``key`` is within ``tie``
+    (relative) of the smallest."""
+    fastest = min(r[key] for r in results)
+    return next(r for r in results if r[key] <= (1.0 + tie) * fastest)
