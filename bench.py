@@ -12,8 +12,9 @@ the timed region.
 
 Prints ONE JSON line.  Besides the contract fields it carries
 ``roofline`` (the dominant kernel's algorithmic bytes per launch over its
-average launch time, HIP events recorded by the library around every launch
-during a profiled pass of the same number of steps) and ``cpu_baseline``
+average launch time, from HIP events the runtime stamps with each kernel's
+begin and end during a profiled pass of the same number of steps) and
+``cpu_baseline``
 (the CPU oracle, single thread, on a bounded sample of the same workload).
 
     python bench.py [--steps K] [--warmup W] [--workload channel|cavity|batch64]

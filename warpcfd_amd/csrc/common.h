@@ -1,5 +1,6 @@
 // Shared helpers for the wcfd HIP kernels (gfx950, wave64).
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 #include <stdint.h>
@@ -35,9 +36,11 @@ constexpr int kMaxKrylovBlock = 512;
 extern thread_local int g_last_hip_error;
 
 // --------------------------------------------------------- opt-in profiler
-// Every launch site is bracketed by a ProfScope.  While wcfd_profile_begin()
-// is active each launch records a start and a stop hipEvent on its own
-// stream; wcfd_profile_end() folds them into per-kernel totals.
+// While wcfd_profile_begin() is active every launch takes a start and a stop
+// hipEvent that the runtime stamps with the kernel's own begin and end
+// (hipExtLaunchKernelGGL), so a kernel's time excludes the launch gap before
+// it; wcfd_profile_end() folds them into per-kernel totals.  Otherwise
+// kernels launch with <<<>>> and no event.
 enum KernelId {
   K_SET_BC, K_FACE_INTERP, K_GREEN_GAUSS, K_MASS_FLUX, K_DIVERGENCE, K_ASM_MOM, K_ASM_P,
   K_RUA_HBYA, K_RELAX_P, K_FACE_P_FC, K_GRAD_P_CORR, K_INV_DIAG, K_SPMV, K_DOT, K_CG_INIT,
@@ -45,32 +48,23 @@ enum KernelId {
   K_EQUATION, K_COUNT
 };
 bool prof_active();
-void prof_record(int id, hipStream_t s, bool start);
-struct ProfScope {
-  int id;
-  hipStream_t s;
-  bool on;
-  ProfScope(int id_, hipStream_t s_) : id(id_), s(s_), on(prof_active()) {
-    if (on) prof_record(id, s, true);
-  }
-  ~ProfScope() {
-    if (on) prof_record(id, s, false);
-  }
-};
+// a start/stop event pair for one launch of kernel `id` (false: not
+// profiling, or the event pool is used up)
+bool prof_events(int id, hipEvent_t* start, hipEvent_t* stop);
 
-// launch KERNEL<<<GRID, kBlock, 0, S>>>(args...) inside a profiler scope
-#define WCFD_LAUNCH(ID, S, KERNEL, GRID, ...)          \
-  do {                                                \
-    ::wcfd::ProfScope wcfd_ps_((ID), (S));            \
-    KERNEL<<<(GRID), kBlock, 0, (S)>>>(__VA_ARGS__);  \
+// launch KERNEL<<<GRID, BLOCK, 0, S>>>(args...), timed when profiling
+#define WCFD_LAUNCH_B(ID, S, KERNEL, GRID, BLOCK, ...)                                   \
+  do {                                                                                  \
+    hipEvent_t wcfd_e0_ = nullptr, wcfd_e1_ = nullptr;                                  \
+    if (::wcfd::prof_active() && ::wcfd::prof_events((ID), &wcfd_e0_, &wcfd_e1_))       \
+      hipExtLaunchKernelGGL((KERNEL), dim3(GRID), dim3(BLOCK), 0, (S), wcfd_e0_, wcfd_e1_, \
+                            0u, __VA_ARGS__);                                           \
+    else                                                                                \
+      KERNEL<<<(GRID), (BLOCK), 0, (S)>>>(__VA_ARGS__);                                 \
   } while (0)
 
-// the same with a run-time workgroup size
-#define WCFD_LAUNCH_B(ID, S, KERNEL, GRID, BLOCK, ...)   \
-  do {                                                  \
-    ::wcfd::ProfScope wcfd_ps_((ID), (S));              \
-    KERNEL<<<(GRID), (BLOCK), 0, (S)>>>(__VA_ARGS__);   \
-  } while (0)
+// the same with kBlock threads per workgroup
+#define WCFD_LAUNCH(ID, S, KERNEL, GRID, ...) WCFD_LAUNCH_B(ID, S, KERNEL, GRID, kBlock, __VA_ARGS__)
 
 // Launch shape of the Krylov kernels (workgroups, threads per workgroup),
 // set by the autotuner through wcfd_set_krylov_launch(); process-wide.

@@ -1,5 +1,7 @@
 // Opt-in per-launch HIP-event profiler (the only process-wide state of the
 // library; guarded by a mutex, off unless wcfd_profile_begin() is called).
+// The events of a launch are stamped by the runtime with the kernel's own
+// begin and end (hipExtLaunchKernelGGL in WCFD_LAUNCH_B).
 #include <hip/hip_runtime.h>
 
 #include <atomic>
@@ -20,7 +22,6 @@ std::atomic<bool> g_on{false};
 std::vector<hipEvent_t> g_pool;
 std::vector<Rec> g_recs;
 size_t g_next = 0;
-int g_open = -1;  // index of the record waiting for its stop event
 const char* kNames[K_COUNT] = {
     "set_bc", "face_interp", "green_gauss", "mass_flux", "divergence", "assemble_momentum",
     "assemble_pressure", "rua_hbya", "relax_p", "face_p_fc", "grad_p_correct", "inv_diag", "spmv",
@@ -29,20 +30,15 @@ const char* kNames[K_COUNT] = {
 
 bool prof_active() { return g_on.load(std::memory_order_relaxed); }
 
-void prof_record(int id, hipStream_t s, bool start) {
+bool prof_events(int id, hipEvent_t* start, hipEvent_t* stop) {
   std::lock_guard<std::mutex> lk(g_mu);
-  if (!g_on) return;
-  if (start) {
-    if (g_next + 2 > g_pool.size()) { g_open = -1; return; }
-    Rec r{id, g_pool[g_next], g_pool[g_next + 1]};
-    g_next += 2;
-    (void)hipEventRecord(r.a, s);
-    g_recs.push_back(r);
-    g_open = static_cast<int>(g_recs.size()) - 1;
-  } else if (g_open >= 0) {
-    (void)hipEventRecord(g_recs[g_open].b, s);
-    g_open = -1;
-  }
+  if (!g_on || g_next + 2 > g_pool.size()) return false;
+  Rec r{id, g_pool[g_next], g_pool[g_next + 1]};
+  g_next += 2;
+  g_recs.push_back(r);
+  *start = r.a;
+  *stop = r.b;
+  return true;
 }
 
 }  // namespace wcfd
@@ -62,7 +58,6 @@ int wcfd_profile_begin(int capacity) {
   g_pool.clear();
   g_recs.clear();
   g_next = 0;
-  g_open = -1;
   g_pool.resize(2 * static_cast<size_t>(capacity));
   for (auto& e : g_pool) {
     int rc = check_hip(hipEventCreate(&e));
