@@ -24,6 +24,14 @@ constexpr int kMaxInlineFold = 2048;    // default: largest per-case fold done i
 #ifndef WCFD_NT_STORE
 #define WCFD_NT_STORE 1
 #endif
+// face interpolation with the owners' data staged in LDS (k_face_interp_lds)
+// or gathered (k_face_interp, the default); the two agree bit for bit.  The
+// LDS form measured 18-22 % slower at 0.32-8.6 M cells: its barrier makes
+// the workgroup wait for its slowest load, and the L1 already serves the
+// owners' reuse.
+#ifndef WCFD_FACE_LDS
+#define WCFD_FACE_LDS 0
+#endif
 #ifndef WCFD_NT_MATRIX
 #define WCFD_NT_MATRIX 0
 #endif

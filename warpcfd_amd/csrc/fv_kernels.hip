@@ -161,6 +161,143 @@ __global__ void k_face_interp(wcfd_mesh m, BatchDev bd, wcfd_bc bc, const float*
   }
 }
 
+// Owner data of a workgroup's internal faces staged in LDS.  Internal faces
+// are numbered by owner (per case in a batch, and the compacted item ranges
+// never span two cases), so the owners of one workgroup's faces are one cell
+// range [o_lo, o_hi].  The workgroup loads those cells' values, skewed
+// gradients and rUA once, coalesced, into LDS while each thread's neighbour
+// gathers are in flight; after the barrier a face reads its owner's data
+// from LDS.  A range longer than kStage cells, or an owner outside it (an
+// uncompacted launch over several cases), falls back to global loads, so the
+// kernel is correct for any face order.  The arithmetic is face_interp_
+// internal's, so the two forms agree bit for bit.
+constexpr int kStage = kBlock;
+
+__device__ inline void item_range(const Items& it, int n, int& a, int& b) {
+  const int vb = (int)xcd_remap(blockIdx.x, gridDim.x);
+  if (it.blk) {
+    a = it.blk[vb];
+    b = it.blk[it.G + vb];
+  } else {
+    a = vb * (int)blockDim.x;
+    b = min(a + (int)blockDim.x, n);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_face_interp_lds(
+    wcfd_mesh m, BatchDev bd, wcfd_bc bc, const float* __restrict__ cv,
+    const float* __restrict__ cg, float* __restrict__ fv, int mask, int skew_mask,
+    float* __restrict__ flux, const float* __restrict__ rua, float* __restrict__ rua_f,
+    int apply_dirichlet) {
+  __shared__ float s_cv[4][kStage];
+  __shared__ float s_cg[4][3][kStage];
+  __shared__ float s_rua[kStage];
+  const int C = m.n_cells, F = m.n_faces, Fi = m.n_internal;
+  int a, b;
+  item_range(bd.faces, F, a, b);
+  const int f = a + (int)threadIdx.x;
+  const bool live = f < b && f < F && !bd.face_skip(f);
+  const int ie = min(b, Fi);              // internal faces of the workgroup: [a, ie)
+  // uniform: the staged owner range
+  int o_lo = 0, R = 0;
+  if (a < ie) {
+    o_lo = m.owner[a];
+    R = m.owner[ie - 1] - o_lo + 1;
+    if (R < 1 || R > kStage) R = 0;       // not staged: global loads
+  }
+  const bool in = live && f < Fi;
+  // this face's indices and its neighbour's data
+  int o = 0, n = 0;
+  float psi = 0.f, s0 = 0.f, s1 = 0.f, s2 = 0.f;
+  float cn[4], gn[4][3], old[3], nr[3], ar = 0.f, rn = 0.f;
+  if (in) {
+    o = m.owner[f];
+    n = m.neigh[f];
+    psi = m.psi[f];
+    s0 = m.skew[f];
+    s1 = m.skew[Fi + f];
+    s2 = m.skew[2 * Fi + f];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if ((mask >> q) & 1) {
+        cn[q] = cv[q * C + n];
+        if ((skew_mask >> q) & 1) {
+          const float* gq = cg + (size_t)(3 * q) * C;
+#pragma unroll
+          for (int d = 0; d < 3; ++d) gn[q][d] = gq[d * C + n];
+        }
+      } else if (q < 3 && flux) {
+        old[q] = fv[q * F + f];
+      }
+    }
+    if (flux) {
+      nr[0] = m.normal[f];
+      nr[1] = m.normal[F + f];
+      nr[2] = m.normal[2 * F + f];
+      ar = m.area[f];
+    }
+    if (rua_f) rn = rua[n];
+  }
+  // stage the owners' data
+  for (int i = (int)threadIdx.x; i < R; i += (int)blockDim.x) {
+    const int c = o_lo + i;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if ((mask >> q) & 1) {
+        s_cv[q][i] = cv[q * C + c];
+        if ((skew_mask >> q) & 1) {
+          const float* gq = cg + (size_t)(3 * q) * C;
+#pragma unroll
+          for (int d = 0; d < 3; ++d) s_cg[q][d][i] = gq[d * C + c];
+        }
+      }
+    }
+    if (rua_f) s_rua[i] = rua[c];
+  }
+  __syncthreads();
+  if (in) {
+    const int j = o - o_lo;
+    const bool st = j >= 0 && j < R;
+    float co[4], go[4][3], ro = 0.f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if ((mask >> q) & 1) {
+        co[q] = st ? s_cv[q][j] : cv[q * C + o];
+        if ((skew_mask >> q) & 1) {
+          const float* gq = cg + (size_t)(3 * q) * C;
+#pragma unroll
+          for (int d = 0; d < 3; ++d) go[q][d] = st ? s_cg[q][d][j] : gq[d * C + o];
+        }
+      }
+    }
+    if (rua_f) ro = st ? s_rua[j] : rua[o];
+    const float psi2 = 1.0f - psi;
+    float val[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if ((mask >> q) & 1) {
+        float v = co[q] * psi + psi2 * cn[q];
+        if ((skew_mask >> q) & 1) {
+          const float g0 = go[q][0] * psi + psi2 * gn[q][0];
+          const float g1 = go[q][1] * psi + psi2 * gn[q][1];
+          const float g2 = go[q][2] * psi + psi2 * gn[q][2];
+          v = v + (g0 * s0 + g1 * s1 + g2 * s2);
+        }
+        val[q] = v;
+      } else {
+        val[q] = (q < 3 && flux) ? old[q] : 0.0f;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if ((mask >> q) & 1) fv[q * F + f] = val[q];
+    if (rua_f) rua_f[f] = psi * ro + psi2 * rn;
+    if (flux) flux[f] = (val[0] * nr[0] + val[1] * nr[1] + val[2] * nr[2]) * ar;
+  } else if (live) {
+    face_interp_boundary(m, bc, f, cv, fv, mask, flux, rua, rua_f, apply_dirichlet);
+  }
+}
+
 // ------------------------------------------------------------ Green-Gauss
 // Cell kernels read their cell's arrays at a uniform base plus the cell's
 // byte offset c4 = 4 c, and the faces' arrays through buffer descriptors at
@@ -647,8 +784,13 @@ int launch_face_interp(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc, const
                        const float* rua, float* rua_f, int apply_dirichlet, hipStream_t s) {
   const unsigned g = items_grid(bd.faces, m.n_faces);
   if (g == 0) return WCFD_OK;
+#if WCFD_FACE_LDS
+  WCFD_LAUNCH(K_FACE_INTERP, s, k_face_interp_lds, g, m, bd, bc, cv, cg, fv, mask, skew_mask,
+              flux, rua, rua_f, apply_dirichlet);
+#else
   WCFD_LAUNCH(K_FACE_INTERP, s, k_face_interp, g, m, bd, bc, cv, cg, fv, mask, skew_mask, flux,
               rua, rua_f, apply_dirichlet);
+#endif
   return check_launch();
 }
 
