@@ -10,7 +10,9 @@ nothing anywhere:
 * composable term weights (the reference's (C, faces per cell, O, 3)
   layout) and the Equation built from them;
 * SIMPLE iterations on a channel, and a closed cavity converged to a
-  continuity residual below 1e-7 (u, v, p within 1e-4).
+  continuity residual below 1e-7 (u, v, p within 1e-4);
+* the Krylov kernels' slot count (4 or 5) against a padded six-slot read;
+* hexahedral, tetrahedral and wedge cases in one batch arena.
 Tolerances: rel-L2 <= 1e-5 per kernel output (float32), 1e-4 for fields.
 """
 import numpy as np
@@ -230,3 +232,50 @@ def test_krylov_reads_only_used_slots(gpu, cell_type):
         assert ia == ib and np.array_equal(xa, xb), key
     xs = spla.spsolve(A.astype(np.float64).tocsc(), b.cpu().numpy().astype(np.float64))
     assert rel_l2(out[(fc, 1)][0], xs) <= 1e-5
+
+
+def _closed_cavity_spec(cell_type, n=12, re=100.0):
+    from cases import apply_bcs
+    from warpcfd_amd.FV import FVM
+    from warpcfd_amd.batch import CaseSpec
+    from warpcfd_amd.preprocess import Mesh, cavity_mesh, define_boundary_walls
+    from warpcfd_amd.preprocess.grid import split_hexes
+    nodes, cells = cavity_mesh(n)
+    if cell_type != 'hexahedron':
+        cells = split_hexes(cells, cell_type)
+    m = Mesh(nodes, cells)
+    define_boundary_walls(m)
+    model = FVM(m)
+    model.material.create_incompressible_newtonian_fluid(viscosity=1.0 / re, density=1.0)
+    model.set_reference_pressure(0, 0.0)
+    apply_bcs(model.boundary, 'closed_cavity')
+    return CaseSpec(model, stop=1e-6, max_outer=3000, name=f'{cell_type}{n}'), nodes, cells
+
+
+def test_batch_of_mixed_cell_types(gpu):
+    """A hexahedral, a tetrahedral and a wedge cavity in one arena (six
+    slots for all; the null faces of the tetrahedra and wedges are ordinary
+    internal faces of their cases): every case stops by the monitor, the
+    tetrahedral case is bitwise the same alone and in the mix, and each
+    case's velocity is within 1e-4 of the FP64 oracle's fixed point."""
+    from oracle.simple import OracleSolver
+    from test_gpu_batch import KW, _run
+    from warpcfd_amd.batch import CaseArena
+    built = [_closed_cavity_spec(ct) for ct in ('hexahedron', 'tetra', 'wedge')]
+    specs = [b[0] for b in built]
+    mix = CaseArena.for_cases(specs, **KW)
+    _run(mix)
+    alone = CaseArena.for_cases([_closed_cavity_spec('tetra')[0]], **KW)
+    _run(alone)
+    assert alone.iterations(0) == mix.iterations(1)
+    assert np.array_equal(alone.cell_values(0), mix.cell_values(1))
+    for k, (_, nodes, cells) in enumerate(built):
+        assert mix.converged(k), (k, mix.continuity(k))
+        of = oracle_case(np.asarray(nodes, np.float32).astype(np.float64), cells, 'closed_cavity',
+                         dtype=np.float64, ref_cell=0)
+        s = OracleSolver(of, 0.7, 0.3, nu=0.01, atol=1e-12)
+        for _ in range(600):
+            if s.outer_iteration()['continuity'] < 1e-10:
+                break
+        got = mix.cell_values(k)
+        assert rel_l2(got[:, :3], of.cell_values[:, :3]) <= 1e-4, k
