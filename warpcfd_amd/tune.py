@@ -8,9 +8,10 @@ per thread lets a row's loads issue before the fold) but make inline folds
 longer, so the best shape depends on the mesh size.  ``autotune_krylov``
 times fixed-iteration solves on a pressure-like matrix built on the mesh's
 own pattern for every candidate, installs the fastest and caches it by cell
-count.  Grid 0 means one row per thread; for each block size the grid
-ceil(C / 2 block) is tried too, where the CG kernel takes two rows per
-thread (rows t and t + block of the workgroup's chunk).
+count.  Grid 0 means one row per thread; for each block size the grids
+that give 2, 1.5 and 1.25 rows per thread (chunk <= 2 block) are tried
+too, where the single-reduction CG kernel takes two rows per thread (rows t
+and t + block of the workgroup's chunk) and folds fewer partials.
 """
 import ctypes
 
@@ -27,6 +28,7 @@ FOLDS = (2048, 0, 4096, 8192)
 PREFOLD_GRIDS = (1024, 2048, 0)    # grids on which "always prefold" is tried
 WIDE_FOLD_GRIDS = (0,)             # grids on which the higher limits are tried
 MAX_GRID = 65536                   # the library's workgroup limit for a Krylov launch
+TWO_ROW_FILL = (2.0, 1.5, 1.25)    # rows per thread of the extra two-row grids
 
 _cache = {}
 
@@ -97,16 +99,18 @@ def autotune_krylov(dm, grids=GRIDS, blocks=BLOCKS, iters=60, cg_weight=1.0, bic
     results = []
     C = dm.n_cells
     for blk in blocks:
-        two_rows = -(-C // (2 * blk))
-        shapes = list(grids)
-        if C > blk and two_rows <= MAX_GRID and two_rows not in shapes:
-            shapes.append(two_rows)
+        two_rows = []
+        for fill in TWO_ROW_FILL:
+            g = -(-C // int(fill * blk))
+            if C > blk and g <= MAX_GRID and g not in grids and g not in two_rows:
+                two_rows.append(g)
+        shapes = list(grids) + two_rows
         for grid in shapes:
             if grid * blk > 4 * C and grid > grids[0]:
                 continue            # more threads than rows: no gain
-            wide = grid in WIDE_FOLD_GRIDS or grid == two_rows
+            wide = grid in WIDE_FOLD_GRIDS or grid in two_rows
             for fold in FOLDS:
-                if fold == 0 and grid not in PREFOLD_GRIDS and grid != two_rows:
+                if fold == 0 and grid not in PREFOLD_GRIDS and grid not in two_rows:
                     continue
                 if fold > FOLDS[0] and not wide:
                     continue
