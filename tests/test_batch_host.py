@@ -139,3 +139,25 @@ def test_arena_tuner_ties_take_the_first_candidate():
     assert pick_with_ties(r, 't', 0.0)['n'] == 2         # no tolerance: the fastest
     r[0]['t'] = 6.6
     assert pick_with_ties(r, 't', 0.02)['n'] == 2        # 3.9 % slower: not a tie
+
+
+def test_krylov_tuner_candidates():
+    """Every fixed grid and the 2 / 1.5 / 1.25 rows-per-thread grids per block
+    are tried; two-row grids fit 2 block rows per workgroup; each shape is
+    listed once; grids stay within the library's limit."""
+    from warpcfd_amd.tune import MAX_GRID, krylov_candidates
+    C = 320000
+    cand = krylov_candidates(C)
+    assert len(cand) == len(set(cand))
+    shapes = {(g, b) for g, b, _ in cand}
+    for b in (128, 256, 512):
+        assert (0, b) in shapes
+        for fill in (2.0, 1.5, 1.25):
+            g = -(-C // int(fill * b))
+            assert (g, b) in shapes
+            chunk = -(-C // g)
+            assert b < chunk <= 2 * b          # the two-rows-per-thread kernel
+    assert all(g <= MAX_GRID for g, _, _ in cand)
+    assert (500, 512, 4096) in cand and (500, 512, 0) in cand
+    # a case smaller than a workgroup: no two-row grids, no grids far beyond its rows
+    assert {g for g, _, _ in krylov_candidates(100)} <= {0, 128}

@@ -86,6 +86,31 @@ def time_solvers(dm, grid, block, iters=60, reps=2, cg_variant=0, fold=FOLDS[0])
     return best
 
 
+def krylov_candidates(C, grids=GRIDS, blocks=BLOCKS):
+    """(grid, block, inline-fold limit) shapes tried for a case of C rows:
+    the fixed grids, and per block the grids that give TWO_ROW_FILL rows per
+    thread; prefolding on PREFOLD_GRIDS and the two-row grids, the higher
+    inline limits on WIDE_FOLD_GRIDS and the two-row grids."""
+    out = []
+    for blk in blocks:
+        two_rows = []
+        for fill in TWO_ROW_FILL:
+            g = -(-C // int(fill * blk))
+            if C > blk and g <= MAX_GRID and g not in grids and g not in two_rows:
+                two_rows.append(g)
+        for grid in list(grids) + two_rows:
+            if grid * blk > 4 * C and grid > grids[0]:
+                continue            # more threads than rows: no gain
+            wide = grid in WIDE_FOLD_GRIDS or grid in two_rows
+            for fold in FOLDS:
+                if fold == 0 and grid not in PREFOLD_GRIDS and grid not in two_rows:
+                    continue
+                if fold > FOLDS[0] and not wide:
+                    continue
+                out.append((grid, blk, fold))
+    return out
+
+
 def autotune_krylov(dm, grids=GRIDS, blocks=BLOCKS, iters=60, cg_weight=1.0, bicg_weight=0.1,
                     cg_variant=0):
     """Pick and install the fastest (grid, block, fold) for mesh ``dm``
@@ -97,27 +122,10 @@ def autotune_krylov(dm, grids=GRIDS, blocks=BLOCKS, iters=60, cg_weight=1.0, bic
         install(c['grid'], c['block'], c['fold'])
         return c
     results = []
-    C = dm.n_cells
-    for blk in blocks:
-        two_rows = []
-        for fill in TWO_ROW_FILL:
-            g = -(-C // int(fill * blk))
-            if C > blk and g <= MAX_GRID and g not in grids and g not in two_rows:
-                two_rows.append(g)
-        shapes = list(grids) + two_rows
-        for grid in shapes:
-            if grid * blk > 4 * C and grid > grids[0]:
-                continue            # more threads than rows: no gain
-            wide = grid in WIDE_FOLD_GRIDS or grid in two_rows
-            for fold in FOLDS:
-                if fold == 0 and grid not in PREFOLD_GRIDS and grid not in two_rows:
-                    continue
-                if fold > FOLDS[0] and not wide:
-                    continue
-                cg_ms, bi_ms = time_solvers(dm, grid, blk, iters, cg_variant=cg_variant,
-                                            fold=fold)
-                results.append({'grid': grid, 'block': blk, 'fold': fold, 'cg_ms': cg_ms,
-                                'bicg_ms': bi_ms, 'score': cg_weight * cg_ms + bicg_weight * bi_ms})
+    for grid, blk, fold in krylov_candidates(dm.n_cells, grids, blocks):
+        cg_ms, bi_ms = time_solvers(dm, grid, blk, iters, cg_variant=cg_variant, fold=fold)
+        results.append({'grid': grid, 'block': blk, 'fold': fold, 'cg_ms': cg_ms,
+                        'bicg_ms': bi_ms, 'score': cg_weight * cg_ms + bicg_weight * bi_ms})
     best = min(results, key=lambda r: r['score'])
     install(best['grid'], best['block'], best['fold'])
     out = dict(best, iters=iters, candidates=results)
