@@ -181,7 +181,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_prefold(Seg sg, double* __r
 #pragma unroll
   for (int i = 0; i < ND; ++i) pp[i] = P + (size_t)i * G + b0;
   double v[ND];
-  fold_n<ND, 16>(pp, nb, v);
+  fold_n<ND, WCFD_PREFOLD_UNROLL>(pp, nb, v);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int i = 0; i < ND; ++i) P[(size_t)i * G + b0] = v[i];
