@@ -383,8 +383,8 @@ __device__ __forceinline__ void fold_thread(const double* const (&p)[ND], int nb
 #endif
 constexpr int kFoldUnroll = WCFD_FOLD_UNROLL;
 // the same in the prefold kernel, which runs alone (one workgroup per case):
-// 4 to 32 measured within 4 % of each other at 4.4 and 8.6 M cells, where
-// the kernel is bound by its one CU's read rate, and 24 or 32 slower
+// 4, 8 and 16 measured within 4 % of each other at 4.4 and 8.6 M cells,
+// where the kernel is bound by its one CU's read rate; 24 and 32 were slower
 #ifndef WCFD_PREFOLD_UNROLL
 #define WCFD_PREFOLD_UNROLL 16
 #endif
