@@ -300,3 +300,17 @@ def test_mixed_cell_types_rejected(tmp_path):
     p.write_text(text)
     with pytest.raises(ValueError, match='single cell type'):
         read_msh(str(p))
+
+
+@pytest.mark.parametrize('cell_type', ['tetra', 'wedge'])
+def test_locate_cells_tet_wedge(cell_type):
+    """Every cell's centroid lies in that cell (convex cells, outward face
+    normals), and a point outside the box is in none."""
+    from warpcfd_amd.preprocess.grid import structured_cells
+    nodes, cells = structured_cells((3, 3, 2), (1.0, 1.0, 0.5), jitter=0.1, seed=4,
+                                    cell_type=cell_type)
+    m = Mesh(nodes, cells)
+    cen = np.asarray(m.cell_properties.centroids, np.float64)
+    ids = locate_cells(m, np.vstack([cen, [[2.0, 0.5, 0.25]]]))
+    np.testing.assert_array_equal(ids[:-1], np.arange(len(cells)))
+    assert ids[-1] == -1
