@@ -34,6 +34,18 @@ def test_invalid_arguments_are_rejected_without_gpu_work():
                        None) == _lib.WCFD_EINVAL
     m = _lib.WcfdMesh()
     assert lib.wcfd_green_gauss(ctypes.byref(m), None, None, 15, 1, None) == _lib.WCFD_EINVAL
+    # a well-formed descriptor except for its slot count (0, 4, 5 or 6 only): the check
+    # comes before any pointer is read, so placeholder addresses are never touched
+    fake = 4096
+    m = _lib.WcfdMesh(n_cells=8, n_faces=12, n_internal=10, n_boundary=2, nnz=28, n_slots=7)
+    for f in ('owner', 'slot_face', 'volume', 'slot_nbr'):
+        setattr(m, f, fake)
+    assert lib.wcfd_green_gauss(ctypes.byref(m), fake, fake, 15, 1, None) == _lib.WCFD_EINVAL
+    assert lib.wcfd_cg_ell(ctypes.byref(m), fake, 0, fake, fake, 1e-5, 1e-5, 10, 10, fake, None,
+                           None) == _lib.WCFD_EINVAL
+    assert lib.wcfd_bicgstab_ell(ctypes.byref(m), fake, 1, fake, fake, 1e-5, 1e-5, 10, 10, fake,
+                                 None, None) == _lib.WCFD_EINVAL
+    assert lib.wcfd_spmv_ell(ctypes.byref(m), fake, 1, fake, fake, None) == _lib.WCFD_EINVAL
 
 
 @pytest.mark.skipif(shutil.which('gcc') is None, reason='gcc missing')

@@ -40,6 +40,11 @@ constexpr int kMaxInlineFold = 2048;    // default: largest per-case fold done i
 // gathers in flight) without spilling.
 constexpr int kMaxKrylovBlock = 512;
 
+// the mesh descriptor's slot count: 0 (six), 4, 5 or 6
+inline bool slots_ok(const wcfd_mesh* m) {
+  return m->n_slots == 0 || (m->n_slots >= 4 && m->n_slots <= 6);
+}
+
 // per-thread HIP error of the last failing call
 extern thread_local int g_last_hip_error;
 

@@ -868,7 +868,8 @@ using namespace wcfd;
 
 bool wcfd::mesh_ok(const wcfd_mesh* m) {
   return m && m->n_cells > 0 && m->n_faces >= m->n_internal && m->n_internal >= 0 &&
-         m->n_boundary == m->n_faces - m->n_internal && m->owner && m->slot_face && m->volume;
+         m->n_boundary == m->n_faces - m->n_internal && m->owner && m->slot_face && m->volume &&
+         slots_ok(m);
 }
 
 extern "C" {

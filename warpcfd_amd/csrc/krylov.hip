@@ -1383,7 +1383,8 @@ int wcfd_spmv(int32_t n, const int32_t* row_ptr, const int32_t* col, const float
 
 int wcfd_spmv_ell(const wcfd_mesh* m, const float* values, int nvec, const float* x, float* y,
                   void* stream) {
-  if (!m || m->n_cells <= 0 || !m->slot_nbr || !values || !x || !y) return WCFD_EINVAL;
+  if (!m || m->n_cells <= 0 || !m->slot_nbr || !slots_ok(m) || !values || !x || !y)
+    return WCFD_EINVAL;
   if (nvec != 1 && nvec != 3) return WCFD_EINVAL;
   hipStream_t s = (hipStream_t)stream;
   const int n = m->n_cells;
@@ -1466,7 +1467,8 @@ int wcfd_bicgstab(int32_t n, const int32_t* row_ptr, const int32_t* col, const f
 int wcfd_cg_ell(const wcfd_mesh* m, const float* values, int variant, const float* b, float* x,
                 double atol, double tol, int maxiter, int check_every, void* workspace,
                 wcfd_solve_info* info, void* stream) {
-  if (!m || m->n_cells <= 0 || !m->slot_nbr || !values || !b || !x || !workspace)
+  if (!m || m->n_cells <= 0 || !m->slot_nbr || !slots_ok(m) || !values || !b || !x ||
+      !workspace)
     return WCFD_EINVAL;
   return solve_cg_ell(*m, nullptr, values, variant, b, x, Tol{atol, tol, 0.0, 0, 0}, maxiter, check_every,
                       workspace, info, (hipStream_t)stream);
@@ -1475,7 +1477,8 @@ int wcfd_cg_ell(const wcfd_mesh* m, const float* values, int variant, const floa
 int wcfd_bicgstab_ell(const wcfd_mesh* m, const float* values, int nvec, const float* b, float* x,
                       double atol, double tol, int maxiter, int check_every, void* workspace,
                       wcfd_solve_info* info, void* stream) {
-  if (!m || m->n_cells <= 0 || !m->slot_nbr || !values || !b || !x || !workspace)
+  if (!m || m->n_cells <= 0 || !m->slot_nbr || !slots_ok(m) || !values || !b || !x ||
+      !workspace)
     return WCFD_EINVAL;
   return solve_bicgstab_ell(*m, nullptr, values, nvec, b, x, Tol{atol, tol, 0.0, 0, 0}, maxiter, check_every,
                             workspace, info, (hipStream_t)stream);
@@ -1484,7 +1487,8 @@ int wcfd_bicgstab_ell(const wcfd_mesh* m, const float* values, int nvec, const f
 int wcfd_batch_cg_ell(const wcfd_mesh* m, const wcfd_batch* bt, const float* values, int variant,
                       const float* b, float* x, double atol, double tol, int maxiter,
                       int check_every, void* workspace, wcfd_solve_info* info, void* stream) {
-  if (!batch_ok(m, bt) || !m->slot_nbr || !values || !b || !x || !workspace) return WCFD_EINVAL;
+  if (!batch_ok(m, bt) || !m->slot_nbr || !slots_ok(m) || !values || !b || !x || !workspace)
+    return WCFD_EINVAL;
   return solve_cg_ell(*m, bt, values, variant, b, x, Tol{atol, tol, 0.0, 0, 0}, maxiter, check_every, workspace,
                       info, (hipStream_t)stream);
 }
@@ -1493,7 +1497,8 @@ int wcfd_batch_bicgstab_ell(const wcfd_mesh* m, const wcfd_batch* bt, const floa
                             int nvec, const float* b, float* x, double atol, double tol,
                             int maxiter, int check_every, void* workspace, wcfd_solve_info* info,
                             void* stream) {
-  if (!batch_ok(m, bt) || !m->slot_nbr || !values || !b || !x || !workspace) return WCFD_EINVAL;
+  if (!batch_ok(m, bt) || !m->slot_nbr || !slots_ok(m) || !values || !b || !x || !workspace)
+    return WCFD_EINVAL;
   return solve_bicgstab_ell(*m, bt, values, nvec, b, x, Tol{atol, tol, 0.0, 0, 0}, maxiter, check_every,
                             workspace, info, (hipStream_t)stream);
 }

@@ -159,7 +159,8 @@ size_t wcfd_batch_simple_workspace_bytes(int32_t n_cells, int32_t n_faces, int32
 int wcfd_simple_iteration(const wcfd_mesh* mp, wcfd_bc bc, const wcfd_simple_params* prm,
                           const wcfd_simple_state* st, void* workspace, wcfd_simple_info* info,
                           void* stream) {
-  if (!mp || !params_ok(prm) || !st || !workspace || !bc.type || !bc.value || !mp->slot_nbr)
+  if (!mp || !params_ok(prm) || !st || !workspace || !bc.type || !bc.value || !mp->slot_nbr ||
+      !slots_ok(mp))
     return WCFD_EINVAL;
   const wcfd_mesh& m = *mp;
   if (m.n_cells <= 0 || m.n_faces < m.n_internal) return WCFD_EINVAL;
@@ -171,7 +172,7 @@ int wcfd_batch_simple_iteration(const wcfd_mesh* mp, const wcfd_batch* bt, wcfd_
                                 const wcfd_simple_params* prm, const wcfd_simple_state* st,
                                 void* workspace, wcfd_simple_info* info, void* stream) {
   if (!batch_ok(mp, bt) || !params_ok(prm) || !st || !workspace || !bc.type || !bc.value ||
-      !mp->slot_nbr || !st->resid)
+      !mp->slot_nbr || !slots_ok(mp) || !st->resid)
     return WCFD_EINVAL;
   if (!bt->cell_case || !bt->face_case || !bt->nu || !bt->ref_cell || !bt->ref_value ||
       !bt->stop || !bt->monitor || !bt->outer_iters)
