@@ -56,7 +56,8 @@ typedef struct wcfd_mesh {
   int32_t nnz;          /* nnz of the C x C CSR pattern */
   int32_t n_slots;      /* slots per cell in use: 6 (hexahedra; 0 also means 6), 5 (wedges),
                            4 (tetrahedra).  Slots k >= n_slots hold the null face and zero
-                           matrix coefficients; the Krylov kernels skip them. */
+                           matrix coefficients; the Krylov kernels skip them.  Any other
+                           value makes every entry point return WCFD_EINVAL. */
   const int32_t* owner;     /* [F]    owner cell (the reference's owner) */
   const int32_t* neigh;     /* [Fi]   neighbour cell */
   const float* psi;         /* [Fi]   |x_f - x_O| / |x_O - x_N| (mesh_structs.py:206) */
