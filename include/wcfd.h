@@ -313,6 +313,25 @@ int wcfd_correct_velocity(const wcfd_mesh* m, wcfd_bc bc, float* cell_values,
 
 size_t wcfd_reduce_workspace_bytes(int32_t n_faces);
 
+/* Convergence report of an outer iteration: the norms FVM.check_convergence
+ * (model.py:278-315) hands to Convergence/Criteria (Convergence.py:7-106),
+ * computed on the device in float64 and written to out[0..8] (device):
+ *   out[0..2] ||A u_k - b_k||_2, out[3..5] ||b_k||_2, k = x, y, z, for the
+ *             momentum matrix `values` [7][C] (ELL, shared by the three
+ *             components), u and b [3][C] (momentum_x/y/z);
+ *   out[6]    max_k,c |c_k| with c_k = rua * corr_k (float32, as
+ *             get_gradient('p', coeff = rUA)), or corr_k when rua == NULL;
+ *             corr [3][C] (velocity_correction);
+ *   out[7]    max |p|, p [C] (pressure_correction: the pressure solve result);
+ *   out[8]    sum_c |div F|_c of the face fluxes `flux` [F] (continuity and
+ *             local_continuity).
+ * A NULL group (values, corr, p, flux) leaves its outputs 0.  Deterministic
+ * (fixed reduction order).  workspace: wcfd_convergence_workspace_bytes(). */
+int wcfd_convergence_report(const wcfd_mesh* m, const float* values, const float* u,
+                            const float* b, const float* rua, const float* corr, const float* p,
+                            const float* flux, void* workspace, double* out, void* stream);
+size_t wcfd_convergence_workspace_bytes(void);
+
 /* ------------------------------------------------------------------ Krylov */
 
 /* y[q] = A x[q] for q < nvec, CSR (row_ptr, col, values) of size n. */

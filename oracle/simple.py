@@ -13,6 +13,13 @@ of the corrected conservative flux F_c = F_HbyA - rUA_f A_f (p_N - p_O)/|d|
 evaluated with the relaxed pressure.  The reference itself reports the L1
 divergence of the pre-correction flux (model.py:286; Convergence.py:55), which
 does not tend to zero; ours is the stop test the batched/served solvers use.
+
+``convergence_report`` / ``convergence_verdicts`` restate check_convergence
+(model.py:278-315) and the criteria of Convergence.py:7-106, pinned by
+closed forms (identity and scaled-identity systems with distinct per-component
+values, known maxima and sums) and by the reference's choice of flux: the
+reported divergence equals the L1 norm of div_u taken inside the iteration,
+before the correction.
 """
 import numpy as np
 import scipy.sparse as sp
@@ -165,9 +172,18 @@ class OracleSolver:
                 vel_corr = fvm.get_gradient(3, self.rUA).reshape(-1)
                 self.HbyA -= vel_corr
                 fvm.replace_cell_values([0, 1, 2], self.HbyA)
+        # what incompressible.py:163 hands to check_convergence
+        self.vel_corr = vel_corr.copy()
+        self.p_new = p_new.copy()
         info['p_iters'] = p_iters
         self.history.append(info)
         return info
+
+    def convergence_report(self):
+        """The norms of model.check_convergence (model.py:278-315) after the
+        last outer iteration; see ``convergence_report``."""
+        return convergence_report(self.vel_eq.matrix(), self.HbyA, self.vel_eq.rhs,
+                                  self.fvm.divFlux(), self.vel_corr, self.p_new)
 
     def run(self):
         """incompressible.py:91-172 without printing (no early stop, T10)."""
@@ -191,3 +207,39 @@ def lid_driven_cavity(n, dtype=np.float32, nu=0.01, dz=0.1, mesh_builder=None):
     b.slip_wall('+Z'); b.slip_wall('-Z')
     b.finalize()
     return fvm
+
+
+def convergence_report(A, u, b, div_u, vel_corr, p_new):
+    """model.py:278-315: the norms check_convergence hands to the criteria
+    of Convergence.py, for the momentum matrix A (3C x 3C, rows 3c+k), the
+    velocity u and right-hand side b (3C, interleaved), the divergence of the
+    mass fluxes div_u (C), the velocity correction (3C) and the pressure
+    solve's result (C).  In float64 (the reference forms A u with Warp's
+    float32 bsr_mv and takes numpy norms of float32 arrays):
+    r[k] = ||(A u - b)_k||_2, b[k] = ||b_k||_2, vel_corr = ||c||_inf,
+    p = ||p_new||_inf, div = ||div_u||_1."""
+    u64 = np.asarray(u, np.float64)
+    b64 = np.asarray(b, np.float64).reshape(-1, 3)
+    ax = np.asarray(A.astype(np.float64) @ u64).reshape(-1, 3)
+    res = ax - b64
+    return {'r': [float(np.linalg.norm(res[:, k])) for k in range(3)],
+            'b': [float(np.linalg.norm(b64[:, k])) for k in range(3)],
+            'vel_corr': float(np.abs(np.asarray(vel_corr, np.float64)).max()),
+            'p': float(np.abs(np.asarray(p_new, np.float64)).max()),
+            'div': float(np.abs(np.asarray(div_u, np.float64)).sum())}
+
+
+def convergence_verdicts(rep):
+    """Convergence.py:51-84 on a report: (residual, converged) per criterion
+    and has_converged over continuity, momentum_x/y/z, pressure_correction
+    and velocity_correction (not local_continuity)."""
+    out = {}
+    for k, name in enumerate(('momentum_x', 'momentum_y', 'momentum_z')):
+        r = rep['r'][k] / max(rep['b'][k], 1e-6)
+        out[name] = (r, r < 1e-3)
+    for name, key in (('continuity', 'div'), ('local_continuity', 'div'),
+                      ('velocity_correction', 'vel_corr'), ('pressure_correction', 'p')):
+        out[name] = (rep[key], rep[key] < 1e-4)
+    crit = ('continuity', 'momentum_x', 'momentum_y', 'momentum_z', 'pressure_correction',
+            'velocity_correction')
+    return out, all(out[c][1] for c in crit)

@@ -46,6 +46,17 @@ def test_invalid_arguments_are_rejected_without_gpu_work():
     assert lib.wcfd_bicgstab_ell(ctypes.byref(m), fake, 1, fake, fake, 1e-5, 1e-5, 10, 10, fake,
                                  None, None) == _lib.WCFD_EINVAL
     assert lib.wcfd_spmv_ell(ctypes.byref(m), fake, 1, fake, fake, None) == _lib.WCFD_EINVAL
+    assert lib.wcfd_convergence_report(ctypes.byref(m), None, None, None, None, None, fake, None,
+                                       fake, fake, None) == _lib.WCFD_EINVAL
+    m.n_slots = 6
+    # no output; a momentum group without its vectors; rUA without a correction
+    assert lib.wcfd_convergence_report(ctypes.byref(m), None, None, None, None, None, fake, None,
+                                       fake, None, None) == _lib.WCFD_EINVAL
+    assert lib.wcfd_convergence_report(ctypes.byref(m), fake, None, fake, None, None, None, None,
+                                       fake, fake, None) == _lib.WCFD_EINVAL
+    assert lib.wcfd_convergence_report(ctypes.byref(m), None, None, None, fake, None, None, None,
+                                       fake, fake, None) == _lib.WCFD_EINVAL
+    assert lib.wcfd_convergence_workspace_bytes() == 9 * 1024 * 8
 
 
 @pytest.mark.skipif(shutil.which('gcc') is None, reason='gcc missing')

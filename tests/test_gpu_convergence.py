@@ -1,0 +1,98 @@
+"""The device convergence report (wcfd_convergence_report, behind
+FVM.check_convergence / IncompressibleSolver.check_convergence) against the
+oracle's restatement of check_convergence (model.py:278-315):
+* on the device's own state after five SIMPLE iterations -- the oracle
+  function applied to the same matrix, right-hand side, velocity, rUA grad p,
+  pressure and fluxes, converted to reference order: every norm within 1e-9
+  relative (float64 sums in another order);
+* against an independent oracle run of the same case (reference Krylov
+  settings, so the runs differ by the CG stops): every norm within 1e-4
+  (measured 1e-6 to 2e-6), the same verdicts;
+on a jittered hexahedral channel and a tetrahedral closed cavity.  Also the
+reference's run(): a check every steps_per_check steps and after the last,
+logged in model.Convergence."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from cases import device_case, jittered_box, oracle_case, rel_l2
+from test_gpu_parity import _dev_matrix_ref
+
+pytestmark = pytest.mark.gpu
+
+
+def _cases():
+    from warpcfd_amd.preprocess.grid import cavity_mesh, split_hexes
+    nodes, cells = jittered_box((12, 6, 5), 0.1, 13, lengths=(2.0, 1.0, 0.8))
+    yield 'hex channel', nodes, cells, 'channel', None, True
+    nodes, hexes = cavity_mesh(8)
+    yield 'tetra cavity', nodes, split_hexes(hexes, 'tetra'), 'closed_cavity', 0, False
+
+
+def _close(a, b, tol):
+    return abs(a - b) <= tol * max(abs(b), 1e-30)
+
+
+@pytest.mark.parametrize('case', list(_cases()), ids=lambda c: c[0])
+def test_report_vs_oracle(gpu, case):
+    from oracle.simple import OracleSolver, convergence_report, convergence_verdicts
+    from warpcfd_amd.FV.implicit_Solvers import IncompressibleSolver
+    name, nodes, cells, kind, ref_cell, outflow = case
+    model = device_case(nodes, cells, kind, ref_cell=ref_cell)
+    sv = IncompressibleSolver(model, 0.7, 0.3, implicit_outflow=outflow)
+    of = oracle_case(nodes, cells, kind, ref_cell=ref_cell)
+    os_ = OracleSolver(of, 0.7, 0.3, nu=0.01, implicit_outflow=outflow)
+    for _ in range(5):
+        sv.outer_iteration()
+        os_.outer_iteration()
+    got = model.convergence_norms(sv.mom_values, None, sv.mom_rhs, (sv.rUA, model._cg[9:12]),
+                                  sv.p_x)
+    # the oracle function on the device's own state, in reference order
+    dm = model.dm
+    A = sp.kron(_dev_matrix_ref(model, sv.mom_values), sp.identity(3), format='csr')
+    u = model.cell_values.cpu().numpy()[:, :3].reshape(-1)
+    b = dm.cells_to_reference(sv.mom_rhs).cpu().numpy().T.reshape(-1)
+    rua = dm.cells_to_reference(sv.rUA).cpu().numpy()
+    grad_p = model.cell_gradients.cpu().numpy()[:, 3]
+    corr = (rua[:, None] * grad_p).astype(np.float32).reshape(-1)
+    p = dm.cells_to_reference(sv.p_x).cpu().numpy()
+    of_same = oracle_case(nodes, cells, kind, ref_cell=ref_cell)
+    of_same.mass_fluxes[:] = model.mass_fluxes.cpu().numpy()
+    same = convergence_report(A, u, b, of_same.divFlux(), corr, p)
+    for k in range(3):
+        assert _close(got['r'][k], same['r'][k], 1e-9), (k, got, same)
+        assert _close(got['b'][k], same['b'][k], 1e-9), (k, got, same)
+    for key in ('vel_corr', 'p', 'div'):
+        assert _close(got[key], same[key], 1e-9), (key, got, same)
+    # an independent oracle run of the same case
+    ref = os_.convergence_report()
+    print(f'{name}: device {got}\n{name}: oracle {ref}')
+    for k in range(3):
+        if ref['b'][k] > 1e-6:
+            assert _close(got['r'][k], ref['r'][k], 1e-4), (k, got, ref)
+            assert _close(got['b'][k], ref['b'][k], 1e-4), (k, got, ref)
+    for key in ('vel_corr', 'p', 'div'):
+        assert _close(got[key], ref[key], 1e-4), (key, got, ref)
+    verdict_ref, conv_ref = convergence_verdicts(ref)
+    assert sv.check_convergence(verbose=False) == conv_ref
+    logged = model.Convergence.log_list[-1]
+    for n, (_, ok) in verdict_ref.items():
+        assert logged[n][1] == ok, n
+    assert rel_l2(model.cell_values.cpu().numpy(), of.cell_values) <= 1e-3
+
+
+def test_run_checks_like_the_reference(gpu):
+    from warpcfd_amd.FV.implicit_Solvers import IncompressibleSolver
+    from warpcfd_amd.preprocess.grid import cavity_mesh
+    nodes, cells = cavity_mesh(12)
+    model = device_case(nodes, cells, 'cavity')
+    sv = IncompressibleSolver(model, 0.7, 0.3)
+    sv.set_iterations(25, 1, 1)
+    sv.run(steps_per_check=10)
+    # steps 0, 10, 20 and the final check
+    assert len(model.Convergence.log_list) == 4
+    last = model.Convergence.log_list[-1]
+    assert set(last) == {'momentum_x', 'momentum_y', 'momentum_z', 'continuity',
+                         'local_continuity', 'velocity_correction', 'pressure_correction'}
+    assert last['continuity'][0] == last['local_continuity'][0]
+    assert all(np.isfinite(v[0]) for v in last.values())

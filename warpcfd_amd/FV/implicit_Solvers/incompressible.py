@@ -180,13 +180,32 @@ class IncompressibleSolver:
             done = mon.update(d['continuity'], d['velocity_change'])
         return it, done, mon
 
-    def run(self, steps_per_check=10, callback=None):
-        """incompressible.py:72-172 (no early stop, as the reference)."""
+    def check_convergence(self, log=True, verbose=True):
+        """The check of incompressible.py:158-163 and 169-172,
+        model.check_convergence(vel_equation.matrix, HbyA, vel_equation.rhs,
+        div_u, vel_correction, p_new), on the device state of the last outer
+        iteration: the relaxed momentum matrix and right-hand side, the
+        corrected velocity, rUA grad p and the pressure solve's result."""
+        m = self.model
+        return m.check_convergence(self.mom_values, None, self.mom_rhs, None,
+                                   (self.rUA, m._cg[9:12]), self.p_x, log=log, verbose=verbose)
+
+    def run(self, steps_per_check=10, callback=None, check=True):
+        """incompressible.py:72-172 (no early stop, as the reference): every
+        steps_per_check steps, and once more after the last, the reference's
+        convergence check (printed and logged in model.Convergence) unless
+        ``check`` is False; ``callback(steps, diagnostics)`` (not in the
+        reference) on the same steps."""
         n_t = self.MAX_TIME_STEPS
         for nt in range(n_t):
             for it in range(self.NUM_OUTER_LOOPS):
                 d = self.outer_iteration()
                 steps = it if self.steady_state else nt
-                if callback is not None and steps % steps_per_check == 0:
-                    callback(steps, d)
+                if steps % steps_per_check == 0:
+                    if callback is not None:
+                        callback(steps, d)
+                    if check:
+                        self.check_convergence()
+        if check:
+            self.check_convergence()
         return self.history

@@ -15,6 +15,7 @@ from .. import _lib
 from ..material import Material
 from ..preprocess.layout import DeviceLayout
 from .boundary import Boundary
+from .convergence import Convergence
 from .device import DeviceMesh, ptr, require_gpu, stream_ptr
 from .field import Field
 
@@ -54,6 +55,8 @@ class FVM:
         self._renumber = renumber
         # u,v,w gradients are known to be zero (after calculate_gradients('p'))
         self._uvw_grad_zero = True
+        self.Convergence = Convergence()
+        self._conv_ws = None
 
     # ------------------------------------------------------------ set-up
     def finalize(self, layout=None):
@@ -212,6 +215,90 @@ class FVM:
             arr.copy_(out)
             return arr
         return out
+
+    def convergence_norms(self, vel_matrix=None, velocity=None, b=None, velocity_correction=None,
+                          p_correction=None):
+        """The norms of check_convergence from one device report
+        (``wcfd_convergence_report``, float64), as a dict of floats: ``r``
+        and ``b`` (3 each: ||A u_k - b_k||_2, ||b_k||_2), ``vel_corr``
+        (max |c|), ``p`` (max |p|), ``div`` (sum |div F| of the current mass
+        fluxes).  Arguments are device-order tensors: ``vel_matrix`` the
+        momentum ELL values [7][C], ``velocity`` and ``b`` [3][C] (velocity
+        None: the current u, v, w), ``velocity_correction`` [3][C] or a pair
+        (rUA [C], grad p [3][C]), ``p_correction`` [C].  None skips a group
+        (its norms are NaN)."""
+        self._need()
+        dev = self._cv.device
+        if self._conv_ws is None:
+            n = self.lib.wcfd_convergence_workspace_bytes()
+            self._conv_ws = torch.zeros(n, dtype=torch.uint8, device=dev)
+            self._conv_out = torch.zeros(9, dtype=torch.float64, device=dev)
+        C = self.num_cells
+
+        def vec(t, rows):
+            if t is None:
+                return None
+            if tuple(t.shape) != ((rows, C) if rows > 1 else (C,)) or not t.is_contiguous() \
+                    or t.dtype != torch.float32 or t.device != dev:
+                raise ValueError(f'expected a contiguous float32 device tensor of shape '
+                                 f'{(rows, C) if rows > 1 else (C,)}, got {tuple(t.shape)}')
+            return t
+
+        vals = vec(vel_matrix, 7)
+        if vals is not None:
+            u = vec(self._cv[:3] if velocity is None else velocity, 3)
+            bb = vec(b, 3)
+        else:
+            u = bb = None
+        rua = corr = None
+        if isinstance(velocity_correction, (tuple, list)):
+            rua, corr = vec(velocity_correction[0], 1), vec(velocity_correction[1], 3)
+        else:
+            corr = vec(velocity_correction, 3)
+        p = vec(p_correction, 1)
+        _lib.check(self.lib.wcfd_convergence_report(
+            self.dm.desc_ref, ptr(vals), ptr(u), ptr(bb), ptr(rua), ptr(corr), ptr(p),
+            ptr(self._flux), ptr(self._conv_ws), ptr(self._conv_out), self._s()),
+            'convergence report')
+        o = self._conv_out.cpu().numpy()
+        nan = float('nan')
+        return {'r': [float(x) for x in o[0:3]] if vals is not None else [nan] * 3,
+                'b': [float(x) for x in o[3:6]] if vals is not None else [nan] * 3,
+                'vel_corr': float(o[6]) if corr is not None else nan,
+                'p': float(o[7]) if p is not None else nan,
+                'div': float(o[8])}
+
+    def check_convergence(self, vel_matrix, velocity, b, div_u=None, velocity_correction=None,
+                          p_correction=None, log=True, verbose=True):
+        """model.py:278-315: momentum_x/y/z, continuity, local_continuity,
+        velocity_correction and pressure_correction of ``self.Convergence``
+        from one device report (see ``convergence_norms`` for the
+        arguments).  As the reference, continuity is taken from the current
+        mass fluxes (``div_u`` is only a buffer there, recomputed by
+        divFlux), a missing correction reports NaN and leaves its criterion's
+        verdict as it was, the dict is printed and logged, and a non-finite
+        momentum_x residual raises ValueError.  Returns has_converged()."""
+        n = self.convergence_norms(vel_matrix, velocity, b, velocity_correction, p_correction)
+        cv = self.Convergence
+        nan = float('nan')
+        conv = {
+            'momentum_x': cv.momentum_x.check_value(n['r'][0], n['b'][0]),
+            'momentum_y': cv.momentum_y.check_value(n['r'][1], n['b'][1]),
+            'momentum_z': cv.momentum_z.check_value(n['r'][2], n['b'][2]),
+            'continuity': cv.continuity.check_value(n['div']),
+            'local_continuity': cv.local_continuity.check_value(n['div']),
+            'velocity_correction': (cv.velocity_correction.check_value(n['vel_corr'])
+                                    if velocity_correction is not None else nan),
+            'pressure_correction': (cv.pressure_correction.check_value(n['p'])
+                                    if p_correction is not None else nan),
+        }
+        if verbose:
+            print(conv)
+        if log:
+            cv.log(conv)
+        if np.isnan(conv['momentum_x'][0]):
+            raise ValueError('momentum_x residual is not finite')
+        return cv.has_converged()
 
     def get_gradient(self, output_index, coeff=1.):
         """model.py:242-254: coeff * grad(phi), (C,3) in reference order."""

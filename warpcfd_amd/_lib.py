@@ -116,6 +116,9 @@ _SIGS = {
     'wcfd_correct_velocity': (ctypes.c_int, [_PM, WcfdBC, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p,
                                              c_p, c_p]),
     'wcfd_reduce_workspace_bytes': (ctypes.c_size_t, [c_i32]),
+    'wcfd_convergence_report': (ctypes.c_int, [_PM, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p,
+                                               c_p]),
+    'wcfd_convergence_workspace_bytes': (ctypes.c_size_t, []),
     'wcfd_spmv': (ctypes.c_int, [c_i32, c_p, c_p, c_p, ctypes.c_int, c_p, c_p, c_p]),
     'wcfd_dot_f64': (ctypes.c_int, [c_i32, c_p, c_p, c_p, c_p, c_p]),
     'wcfd_krylov_workspace_bytes': (ctypes.c_size_t, [c_i32, ctypes.c_int]),

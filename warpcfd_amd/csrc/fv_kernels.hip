@@ -1000,3 +1000,152 @@ int wcfd_correct_velocity(const wcfd_mesh* m, wcfd_bc bc, float* cell_values, fl
 }
 
 }  // extern "C"
+
+// ======================================================= convergence report
+// model.py:278-315 with Convergence.py:7-106: the norms check_convergence
+// reports after an outer iteration, on the device and in float64.  One pass
+// over the cells writes per-workgroup partials; one workgroup folds them in a
+// fixed order, so the report is deterministic.
+namespace wcfd {
+namespace {
+
+constexpr int kConvN = 9;           // partial rows, see k_conv_cells
+constexpr int kConvMaxBlocks = 1024;
+
+// maximum that keeps a NaN (as numpy's norm(x, inf) does)
+__device__ inline double max_nan(double a, double b) { return (a > b || a != a) ? a : b; }
+
+__device__ inline double wave_max(double v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v = max_nan(v, __shfl_xor(v, off, kWave));
+  return v;
+}
+
+// Block reduction of the report rows: 0-5 and 8 are sums, 6 and 7 maxima.
+__device__ inline bool conv_is_max(int j) { return j == 6 || j == 7; }
+
+__device__ inline void conv_block(double (&v)[kConvN], double* __restrict__ out, int stride,
+                                  int idx) {
+  __shared__ double lds[kConvN * kMaxWaves];
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int j = 0; j < kConvN; ++j) {
+    const double s = conv_is_max(j) ? wave_max(v[j]) : wave_sum(v[j]);
+    if (lane == 0) lds[j * kMaxWaves + wid] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int j = 0; j < kConvN; ++j) {
+      double t = 0.0;
+      for (int w = 0; w < (int)(blockDim.x / kWave); ++w)
+        t = conv_is_max(j) ? max_nan(t, lds[j * kMaxWaves + w]) : t + lds[j * kMaxWaves + w];
+      out[j * stride + idx] = t;
+    }
+  }
+}
+
+// Rows of the partials, per workgroup over a grid-stride range of cells:
+// 0-2 sum (A u_k - b_k)^2 and 3-5 sum b_k^2 (k = x, y, z; A u in float64 from
+// the float32 coefficients and values), 6 max |c_k| with c_k = rua * corr_k
+// in float32 as get_gradient('p', coeff = rUA) forms it (or corr_k when rua
+// is NULL), 7 max |p|, 8 sum |div F| with div F summed over the cell's faces
+// in float32 in local-face order, as divFlux_kernel (model_kernels.py:27-34).
+__global__ __launch_bounds__(kBlock) void k_conv_cells(
+    wcfd_mesh m, const float* __restrict__ val, const float* __restrict__ u,
+    const float* __restrict__ b, const float* __restrict__ rua, const float* __restrict__ corr,
+    const float* __restrict__ p, const float* __restrict__ flux, double* __restrict__ part) {
+  const int C = m.n_cells;
+  double acc[kConvN];
+#pragma unroll
+  for (int j = 0; j < kConvN; ++j) acc[j] = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < C; i += gridDim.x * blockDim.x) {
+    if (val) {
+      double r[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) r[k] = (double)val[i] * (double)u[(size_t)k * C + i];
+      for (int s = 0; s < 6; ++s) {
+        const double a = (double)val[(size_t)(s + 1) * C + i];
+        const int c = m.slot_nbr[(size_t)s * C + i];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) r[k] += a * (double)u[(size_t)k * C + c];
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double bk = (double)b[(size_t)k * C + i];
+        r[k] -= bk;
+        acc[k] += r[k] * r[k];
+        acc[3 + k] += bk * bk;
+      }
+    }
+    if (corr) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const float c = rua ? rua[i] * corr[(size_t)k * C + i] : corr[(size_t)k * C + i];
+        acc[6] = max_nan(acc[6], fabs((double)c));
+      }
+    }
+    if (p) acc[7] = max_nan(acc[7], fabs((double)p[i]));
+    if (flux) {
+      const int ns = m.n_slots ? m.n_slots : 6;
+      float d = 0.f;
+      for (int s = 0; s < ns; ++s) {
+        const float fl = flux[m.slot_face[(size_t)s * C + i]];
+        d = m.slot_side[(size_t)s * C + i] == 0 ? d + fl : d - fl;
+      }
+      acc[8] += fabs((double)d);
+    }
+  }
+  conv_block(acc, part, (int)gridDim.x, (int)blockIdx.x);
+}
+
+// One workgroup: out[0-2] ||A u_k - b_k||_2, out[3-5] ||b_k||_2, out[6] max |c|,
+// out[7] max |p|, out[8] sum |div F|.
+__global__ __launch_bounds__(kBlock) void k_conv_fold(const double* __restrict__ part, int G,
+                                                     double* __restrict__ out) {
+  double acc[kConvN];
+#pragma unroll
+  for (int j = 0; j < kConvN; ++j) {
+    double t = 0.0;
+    for (int g = (int)threadIdx.x; g < G; g += blockDim.x)
+      t = conv_is_max(j) ? max_nan(t, part[(size_t)j * G + g]) : t + part[(size_t)j * G + g];
+    acc[j] = t;
+  }
+  __shared__ double res[kConvN];
+  conv_block(acc, res, 1, 0);
+  __syncthreads();
+  if (threadIdx.x < kConvN) {
+    const int j = (int)threadIdx.x;
+    out[j] = j < 6 ? sqrt(res[j]) : res[j];
+  }
+}
+
+int conv_grid(int C) {
+  const int64_t g = ((int64_t)C + kBlock - 1) / kBlock;
+  return (int)(g < kConvMaxBlocks ? g : kConvMaxBlocks);
+}
+
+}  // namespace
+}  // namespace wcfd
+
+extern "C" {
+
+size_t wcfd_convergence_workspace_bytes(void) {
+  return sizeof(double) * (size_t)kConvN * (size_t)kConvMaxBlocks;
+}
+
+int wcfd_convergence_report(const wcfd_mesh* m, const float* values, const float* u,
+                            const float* b, const float* rua, const float* corr, const float* p,
+                            const float* flux, void* workspace, double* out, void* stream) {
+  if (!mesh_ok(m) || !workspace || !out) return WCFD_EINVAL;
+  if (values && (!u || !b || !m->slot_nbr)) return WCFD_EINVAL;
+  if (flux && !m->slot_side) return WCFD_EINVAL;
+  if (rua && !corr) return WCFD_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  const int G = conv_grid(m->n_cells);
+  double* part = (double*)workspace;
+  WCFD_LAUNCH(K_CONVERGENCE, s, k_conv_cells, G, *m, values, u, b, rua, corr, p, flux, part);
+  WCFD_LAUNCH(K_CONVERGENCE, s, k_conv_fold, 1, part, G, out);
+  return check_launch();
+}
+
+}  // extern "C"
