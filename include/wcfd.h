@@ -332,6 +332,17 @@ int wcfd_convergence_report(const wcfd_mesh* m, const float* values, const float
                             const float* flux, void* workspace, double* out, void* stream);
 size_t wcfd_convergence_workspace_bytes(void);
 
+/* The same report for every case of a batch arena (the per-case form of
+ * model.py:278-315): out[9k .. 9k+8] (device) for case k, from the case's
+ * own workgroups (bt's cell and workgroup ranges; finished cases, which have
+ * none, report 0), so a case's report does not depend on the other cases.
+ * workspace: wcfd_batch_convergence_workspace_bytes(bt->n_blocks). */
+int wcfd_batch_convergence_report(const wcfd_mesh* m, const wcfd_batch* bt, const float* values,
+                                  const float* u, const float* b, const float* rua,
+                                  const float* corr, const float* p, const float* flux,
+                                  void* workspace, double* out, void* stream);
+size_t wcfd_batch_convergence_workspace_bytes(int32_t n_blocks);
+
 /* ------------------------------------------------------------------ Krylov */
 
 /* y[q] = A x[q] for q < nvec, CSR (row_ptr, col, values) of size n. */

@@ -10,7 +10,8 @@ oracle's restatement of check_convergence (model.py:278-315):
   (measured 1e-6 to 2e-6), the same verdicts;
 on a jittered hexahedral channel and a tetrahedral closed cavity.  Also the
 reference's run(): a check every steps_per_check steps and after the last,
-logged in model.Convergence."""
+logged in model.Convergence; and the per-case report of a batch arena
+(wcfd_batch_convergence_report)."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -96,3 +97,61 @@ def test_run_checks_like_the_reference(gpu):
                          'local_continuity', 'velocity_correction', 'pressure_correction'}
     assert last['continuity'][0] == last['local_continuity'][0]
     assert all(np.isfinite(v[0]) for v in last.values())
+
+
+def _case_inputs(arena, k):
+    """Case k's momentum matrix (3C x 3C), u, b, rUA grad p, p and
+    sum |div F| from the arena's buffers, in the case's reference order."""
+    sl = arena.slots[k]
+    spec, c0 = sl['spec'], sl['c0']
+    L, C = spec.layout, spec.n_cells
+    cols = slice(c0, c0 + C)
+    # the ELL rows as COO (boundary slots add their zero coefficient to the
+    # diagonal), renumbered to the reference order
+    vals = arena.mom_values[:, cols].cpu().numpy()
+    nbr = arena.slot_nbr[:, cols].cpu().numpy().astype(np.int64) - c0
+    n2o = np.argsort(L.old2new)
+    rows = np.tile(np.arange(C), 7)
+    cix = np.concatenate([np.arange(C)] + [nbr[s] for s in range(6)])
+    A = sp.csr_matrix((vals.reshape(-1), (n2o[rows], n2o[cix])), shape=(C, C))
+    ref = lambda t: t[..., L.old2new]
+    u = arena.cell_values(k)[:, :3].reshape(-1)
+    b = ref(arena.mom_rhs[:, cols].cpu().numpy()).T.reshape(-1)
+    rua = ref(arena.rua[cols].cpu().numpy())
+    gp = ref(arena.cg[9:12, cols].cpu().numpy()).T
+    corr = (rua[:, None] * gp).astype(np.float32).reshape(-1)
+    p = ref(arena.p_x[cols].cpu().numpy())
+    # divergence over the arena's slot tables (local-face order, float32)
+    flux = arena.flux.cpu().numpy()
+    sf = arena.slot_face[:, cols].cpu().numpy()
+    side = arena.slot_side[:, cols].cpu().numpy()
+    div = np.zeros(C, np.float32)
+    for s in range(6):
+        f = flux[sf[s]]
+        div = np.where(side[s] == 0, div + f, div - f)
+    return sp.kron(A, sp.identity(3), format='csr'), u, b, div, corr, p
+
+
+def test_batch_report_per_case(gpu):
+    """Per-case reports of a batch arena: a case's report is bitwise the same
+    alone and third in a mix (other offsets and neighbours), and its norms
+    equal the oracle function on the case's own arena state within 1e-9."""
+    from oracle.simple import convergence_report
+    from test_gpu_batch import KW, cavity_spec, channel_spec
+    from warpcfd_amd.batch import CaseArena
+    x = cavity_spec(16, 400.0)
+    alone = CaseArena.for_cases([x], **KW)
+    mix = CaseArena.for_cases([cavity_spec(12, 800.0), channel_spec(), x], **KW)
+    for _ in range(6):
+        alone.step()
+        mix.step()
+    ra, rm = alone.convergence_report(), mix.convergence_report()
+    assert ra[0] == rm[2]
+    for k in range(3):
+        got = rm[k]
+        same = convergence_report(*_case_inputs(mix, k))
+        for j in range(3):
+            assert _close(got['r'][j], same['r'][j], 1e-9), (k, j, got, same)
+            assert _close(got['b'][j], same['b'][j], 1e-9), (k, j, got, same)
+        for key in ('vel_corr', 'p', 'div'):
+            assert _close(got[key], same[key], 1e-9), (k, key, got, same)

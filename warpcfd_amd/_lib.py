@@ -119,6 +119,9 @@ _SIGS = {
     'wcfd_convergence_report': (ctypes.c_int, [_PM, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p,
                                                c_p]),
     'wcfd_convergence_workspace_bytes': (ctypes.c_size_t, []),
+    'wcfd_batch_convergence_report': (ctypes.c_int, [_PM, ctypes.POINTER(WcfdBatch), c_p, c_p, c_p,
+                                                     c_p, c_p, c_p, c_p, c_p, c_p, c_p]),
+    'wcfd_batch_convergence_workspace_bytes': (ctypes.c_size_t, [c_i32]),
     'wcfd_spmv': (ctypes.c_int, [c_i32, c_p, c_p, c_p, ctypes.c_int, c_p, c_p, c_p]),
     'wcfd_dot_f64': (ctypes.c_int, [c_i32, c_p, c_p, c_p, c_p, c_p]),
     'wcfd_krylov_workspace_bytes': (ctypes.c_size_t, [c_i32, ctypes.c_int]),

@@ -574,6 +574,35 @@ class CaseArena:
             self._build_descriptors()      # drop the finished cases' workgroups
         return [k for k, sl in enumerate(self.slots) if sl is not None and not act[k]]
 
+    def convergence_report(self, stream=None):
+        """Per-case norms of check_convergence (model.py:278-315) from one
+        device pass (``wcfd_batch_convergence_report``, float64): for each
+        slot None (empty or finished) or a dict as ``FVM.convergence_norms``
+        returns -- ``r``, ``b`` (||A u_k - b_k||_2, ||b_k||_2), ``vel_corr``
+        (max |rUA grad p|), ``p`` (max |p|, the pressure solve's result),
+        ``div`` (sum |div F| of the case's mass fluxes).  A case's report
+        depends only on the case."""
+        if self.n_blocks == 0:
+            return [None] * self.K
+        need = self.lib.wcfd_batch_convergence_workspace_bytes(self.n_blocks)
+        if getattr(self, '_conv_ws', None) is None or self._conv_ws.numel() < need:
+            self._conv_ws = torch.zeros(need, dtype=torch.uint8, device=self.device)
+            self._conv_out = torch.zeros(9 * self.K, dtype=torch.float64, device=self.device)
+        code = self.lib.wcfd_batch_convergence_report(
+            ctypes.byref(self.desc), ctypes.byref(self.batch), ptr(self.mom_values), ptr(self.cv),
+            ptr(self.mom_rhs), ptr(self.rua), ptr(self.cg[9:12]), ptr(self.p_x), ptr(self.flux),
+            ptr(self._conv_ws), ptr(self._conv_out), stream_ptr(stream))
+        _lib.check(code, 'batch convergence report')
+        o = self._conv_out.cpu().numpy().reshape(self.K, 9)
+        rep = []
+        for k, sl in enumerate(self.slots):
+            if sl is None or not self.h_active[k]:
+                rep.append(None)
+                continue
+            rep.append({'r': [float(x) for x in o[k, 0:3]], 'b': [float(x) for x in o[k, 3:6]],
+                        'vel_corr': float(o[k, 6]), 'p': float(o[k, 7]), 'div': float(o[k, 8])})
+        return rep
+
     def continuity(self, k):
         r = self.resid[k].cpu().numpy()
         return float(r[0] / r[1]) if r[1] > 0 else 0.0

@@ -1050,15 +1050,18 @@ __device__ inline void conv_block(double (&v)[kConvN], double* __restrict__ out,
 // in float32 as get_gradient('p', coeff = rUA) forms it (or corr_k when rua
 // is NULL), 7 max |p|, 8 sum |div F| with div F summed over the cell's faces
 // in float32 in local-face order, as divFlux_kernel (model_kernels.py:27-34).
+// Rows are segmented per case as in the Krylov kernels (Seg): workgroup vb
+// covers its chunk of one case's cells, so each case gets its own partials.
 __global__ __launch_bounds__(kBlock) void k_conv_cells(
-    wcfd_mesh m, const float* __restrict__ val, const float* __restrict__ u,
+    wcfd_mesh m, Seg sg, const float* __restrict__ val, const float* __restrict__ u,
     const float* __restrict__ b, const float* __restrict__ rua, const float* __restrict__ corr,
     const float* __restrict__ p, const float* __restrict__ flux, double* __restrict__ part) {
   const int C = m.n_cells;
+  const BlockRows br = block_rows(sg);
   double acc[kConvN];
 #pragma unroll
   for (int j = 0; j < kConvN; ++j) acc[j] = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < C; i += gridDim.x * blockDim.x) {
+  for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
     if (val) {
       double r[3];
 #pragma unroll
@@ -1095,19 +1098,26 @@ __global__ __launch_bounds__(kBlock) void k_conv_cells(
       acc[8] += fabs((double)d);
     }
   }
-  conv_block(acc, part, (int)gridDim.x, (int)blockIdx.x);
+  conv_block(acc, part, sg.G, br.vb);
 }
 
-// One workgroup: out[0-2] ||A u_k - b_k||_2, out[3-5] ||b_k||_2, out[6] max |c|,
-// out[7] max |p|, out[8] sum |div F|.
-__global__ __launch_bounds__(kBlock) void k_conv_fold(const double* __restrict__ part, int G,
+// One workgroup per case k: out[9k + 0-2] ||A u_k - b_k||_2, out[9k + 3-5]
+// ||b_k||_2, out[9k + 6] max |c|, out[9k + 7] max |p|, out[9k + 8] sum |div F|
+// (all 0 for a case without workgroups).
+__global__ __launch_bounds__(kBlock) void k_conv_fold(Seg sg, const double* __restrict__ part,
                                                      double* __restrict__ out) {
+  const int k = blockIdx.x;
+  int b0, nb;
+  case_blocks(sg, k, b0, nb);
+  const int G = sg.G;
   double acc[kConvN];
 #pragma unroll
   for (int j = 0; j < kConvN; ++j) {
     double t = 0.0;
-    for (int g = (int)threadIdx.x; g < G; g += blockDim.x)
-      t = conv_is_max(j) ? max_nan(t, part[(size_t)j * G + g]) : t + part[(size_t)j * G + g];
+    for (int g = (int)threadIdx.x; g < nb; g += blockDim.x) {
+      const double v = part[(size_t)j * G + b0 + g];
+      t = conv_is_max(j) ? max_nan(t, v) : t + v;
+    }
     acc[j] = t;
   }
   __shared__ double res[kConvN];
@@ -1115,13 +1125,28 @@ __global__ __launch_bounds__(kBlock) void k_conv_fold(const double* __restrict__
   __syncthreads();
   if (threadIdx.x < kConvN) {
     const int j = (int)threadIdx.x;
-    out[j] = j < 6 ? sqrt(res[j]) : res[j];
+    out[(size_t)kConvN * k + j] = j < 6 ? sqrt(res[j]) : res[j];
   }
 }
 
-int conv_grid(int C) {
-  const int64_t g = ((int64_t)C + kBlock - 1) / kBlock;
-  return (int)(g < kConvMaxBlocks ? g : kConvMaxBlocks);
+int launch_conv(const wcfd_mesh& m, const Seg& sg, const float* values, const float* u,
+                const float* b, const float* rua, const float* corr, const float* p,
+                const float* flux, double* part, double* out, hipStream_t s) {
+  if (sg.G > 0)
+    WCFD_LAUNCH(K_CONVERGENCE, s, k_conv_cells, sg.G, m, sg, values, u, b, rua, corr, p, flux,
+                part);
+  WCFD_LAUNCH(K_CONVERGENCE, s, k_conv_fold, sg.n_cases, sg, part, out);
+  return check_launch();
+}
+
+bool conv_args_ok(const wcfd_mesh* m, const float* values, const float* u, const float* b,
+                  const float* rua, const float* corr, const float* flux, const void* workspace,
+                  const double* out) {
+  if (!mesh_ok(m) || !workspace || !out) return false;
+  if (values && (!u || !b || !m->slot_nbr)) return false;
+  if (flux && !m->slot_side) return false;
+  if (rua && !corr) return false;
+  return true;
 }
 
 }  // namespace
@@ -1136,16 +1161,25 @@ size_t wcfd_convergence_workspace_bytes(void) {
 int wcfd_convergence_report(const wcfd_mesh* m, const float* values, const float* u,
                             const float* b, const float* rua, const float* corr, const float* p,
                             const float* flux, void* workspace, double* out, void* stream) {
-  if (!mesh_ok(m) || !workspace || !out) return WCFD_EINVAL;
-  if (values && (!u || !b || !m->slot_nbr)) return WCFD_EINVAL;
-  if (flux && !m->slot_side) return WCFD_EINVAL;
-  if (rua && !corr) return WCFD_EINVAL;
-  hipStream_t s = (hipStream_t)stream;
-  const int G = conv_grid(m->n_cells);
-  double* part = (double*)workspace;
-  WCFD_LAUNCH(K_CONVERGENCE, s, k_conv_cells, G, *m, values, u, b, rua, corr, p, flux, part);
-  WCFD_LAUNCH(K_CONVERGENCE, s, k_conv_fold, 1, part, G, out);
-  return check_launch();
+  if (!conv_args_ok(m, values, u, b, rua, corr, flux, workspace, out)) return WCFD_EINVAL;
+  const Seg sg = single_seg(m->n_cells, kBlock, kConvMaxBlocks);
+  return launch_conv(*m, sg, values, u, b, rua, corr, p, flux, (double*)workspace, out,
+                     (hipStream_t)stream);
+}
+
+size_t wcfd_batch_convergence_workspace_bytes(int32_t n_blocks) {
+  return sizeof(double) * (size_t)kConvN * (size_t)(n_blocks > 0 ? n_blocks : 1);
+}
+
+int wcfd_batch_convergence_report(const wcfd_mesh* m, const wcfd_batch* bt, const float* values,
+                                  const float* u, const float* b, const float* rua,
+                                  const float* corr, const float* p, const float* flux,
+                                  void* workspace, double* out, void* stream) {
+  if (!conv_args_ok(m, values, u, b, rua, corr, flux, workspace, out)) return WCFD_EINVAL;
+  if (!batch_ok(m, bt)) return WCFD_EINVAL;
+  const Seg sg = batch_seg(*bt, m->n_cells);
+  return launch_conv(*m, sg, values, u, b, rua, corr, p, flux, (double*)workspace, out,
+                     (hipStream_t)stream);
 }
 
 }  // extern "C"
