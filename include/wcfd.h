@@ -31,6 +31,7 @@ extern "C" {
 #define WCFD_EHIP 2
 #define WCFD_ENOTCONV 3
 #define WCFD_ENAN 4
+#define WCFD_ESCRATCH 5 /* input larger than a no-allocation entry point's static scratch */
 
 #define WCFD_DIRICHLET 1
 #define WCFD_NEUMANN 2
@@ -342,6 +343,23 @@ int wcfd_batch_convergence_report(const wcfd_mesh* m, const wcfd_batch* bt, cons
                                   const float* corr, const float* p, const float* flux,
                                   void* workspace, double* out, void* stream);
 size_t wcfd_batch_convergence_workspace_bytes(int32_t n_blocks);
+
+/* wcfd_batch_convergence_report with no workspace argument and no allocation
+ * of any kind.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory -- code under HIP graph capture, serving loops and
+ * harnesses that forbid hipMalloc.  The float64 partials live in a static
+ * device array of WCFD_NOALLOC_MAX_BLOCKS x 9 doubles (1.18 MB) that each GPU
+ * the library runs on holds until the process ends; its address is looked up
+ * on the first call on each GPU (the caller's current device).  Returns
+ * WCFD_ESCRATCH when bt->n_blocks > WCFD_NOALLOC_MAX_BLOCKS.  Results are
+ * bitwise those of wcfd_batch_convergence_report (same kernels, same order).
+ * Every call on one GPU shares the array: calls that overlap on one GPU (other
+ * streams or host threads) race and must be serialised by the caller. */
+#define WCFD_NOALLOC_MAX_BLOCKS 16384
+int wcfd_batch_convergence_report_noalloc(const wcfd_mesh* m, const wcfd_batch* bt,
+                                          const float* values, const float* u, const float* b,
+                                          const float* rua, const float* corr, const float* p,
+                                          const float* flux, double* out, void* stream);
 
 /* ------------------------------------------------------------------ Krylov */
 

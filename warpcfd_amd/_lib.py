@@ -12,7 +12,8 @@ LIB_NAME = 'libwcfd.so'
 LIB_PATH = os.path.join(_HERE, LIB_NAME)
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), 'include', 'wcfd.h')
 
-WCFD_OK, WCFD_EINVAL, WCFD_EHIP, WCFD_ENOTCONV, WCFD_ENAN = 0, 1, 2, 3, 4
+WCFD_OK, WCFD_EINVAL, WCFD_EHIP, WCFD_ENOTCONV, WCFD_ENAN, WCFD_ESCRATCH = 0, 1, 2, 3, 4, 5
+NOALLOC_MAX_BLOCKS = 16384
 DIRICHLET, NEUMANN = 1, 2
 SCHEME_CORRECTED, SCHEME_IMPLICIT_OUTFLOW = 1, 2
 CG_STANDARD, CG_SINGLE_REDUCTION = 0, 1
@@ -122,6 +123,9 @@ _SIGS = {
     'wcfd_batch_convergence_report': (ctypes.c_int, [_PM, ctypes.POINTER(WcfdBatch), c_p, c_p, c_p,
                                                      c_p, c_p, c_p, c_p, c_p, c_p, c_p]),
     'wcfd_batch_convergence_workspace_bytes': (ctypes.c_size_t, [c_i32]),
+    'wcfd_batch_convergence_report_noalloc': (ctypes.c_int, [_PM, ctypes.POINTER(WcfdBatch), c_p,
+                                                             c_p, c_p, c_p, c_p, c_p, c_p, c_p,
+                                                             c_p]),
     'wcfd_spmv': (ctypes.c_int, [c_i32, c_p, c_p, c_p, ctypes.c_int, c_p, c_p, c_p]),
     'wcfd_dot_f64': (ctypes.c_int, [c_i32, c_p, c_p, c_p, c_p, c_p]),
     'wcfd_krylov_workspace_bytes': (ctypes.c_size_t, [c_i32, ctypes.c_int]),

@@ -9,6 +9,8 @@
 #include "fv_kernels.h"
 #include "monitor.h"
 
+#include <mutex>
+
 static_assert(wcfd::kMonHead + 2 * wcfd::kMonWindow <= WCFD_MONITOR_LEN,
               "monitor record larger than WCFD_MONITOR_LEN");
 
@@ -883,6 +885,9 @@ const char* wcfd_error_string(int code) {
     case WCFD_EHIP: return "HIP runtime error";
     case WCFD_ENOTCONV: return "solver did not converge";
     case WCFD_ENAN: return "non-finite residual";
+    case WCFD_ESCRATCH:
+      return "input exceeds the static scratch of the no-allocation entry point "
+             "(WCFD_NOALLOC_MAX_BLOCKS workgroups)";
     default: return "unknown error";
   }
 }
@@ -1180,6 +1185,59 @@ int wcfd_batch_convergence_report(const wcfd_mesh* m, const wcfd_batch* bt, cons
   const Seg sg = batch_seg(*bt, m->n_cells);
   return launch_conv(*m, sg, values, u, b, rua, corr, p, flux, (double*)workspace, out,
                      (hipStream_t)stream);
+}
+
+}  // extern "C"
+
+// ============================================ report without an allocation
+// wcfd_batch_convergence_report_noalloc keeps its partials in this array: one
+// copy per GPU, from the first call on that GPU until the process ends.  Host
+// code never passes the array's name to a kernel (on the host it names a
+// stand-in, not device memory); it passes the device address that
+// hipGetSymbolAddress returns for the current GPU.  k_conv_cells writes every
+// partial that k_conv_fold then reads, so nothing needs clearing between calls.
+static __device__ __align__(256) double scratch[wcfd::kConvN * WCFD_NOALLOC_MAX_BLOCKS];
+
+namespace wcfd {
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current GPU, looked up once per GPU.
+int scratch_address(double** out) {
+  int dev = -1;
+  if (const int e = check_hip(hipGetDevice(&dev))) return e;
+  if (dev < 0 || dev >= kMaxDevices) return WCFD_EINVAL;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    if (const int e = check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)))) return e;
+    g_scratch_dev[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return WCFD_OK;
+}
+
+}  // namespace
+}  // namespace wcfd
+
+extern "C" {
+
+int wcfd_batch_convergence_report_noalloc(const wcfd_mesh* m, const wcfd_batch* bt,
+                                          const float* values, const float* u, const float* b,
+                                          const float* rua, const float* corr, const float* p,
+                                          const float* flux, double* out, void* stream) {
+  // the workspace check of conv_args_ok is satisfied by a placeholder: the
+  // partials go to the static array
+  if (!conv_args_ok(m, values, u, b, rua, corr, flux, out, out)) return WCFD_EINVAL;
+  if (!batch_ok(m, bt)) return WCFD_EINVAL;
+  if (bt->n_blocks > WCFD_NOALLOC_MAX_BLOCKS) return WCFD_ESCRATCH;
+  double* part = nullptr;
+  if (const int e = scratch_address(&part)) return e;
+  const Seg sg = batch_seg(*bt, m->n_cells);
+  return launch_conv(*m, sg, values, u, b, rua, corr, p, flux, part, out, (hipStream_t)stream);
 }
 
 }  // extern "C"
