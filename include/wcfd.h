@@ -4,17 +4,30 @@
  *
  * Conventions
  *  - Every pointer is a device pointer unless marked [host].  The caller owns
- *    all memory; the library allocates nothing.  Scratch space comes from a
- *    caller-provided workspace whose size wcfd_*_workspace_bytes() returns.
+ *    all inputs, outputs and workspaces, and the library never allocates or
+ *    frees device memory: scratch space comes from a caller-provided
+ *    workspace whose size wcfd_*_workspace_bytes() returns (the one
+ *    exception, wcfd_batch_convergence_report_noalloc, uses a static device
+ *    array instead).  Host-side, the Krylov solves that poll for their stop
+ *    (wcfd_cg*, wcfd_bicgstab*, wcfd_simple_iteration and the batched forms
+ *    with graph_groups 0) keep one small pinned host buffer (hipHostMalloc,
+ *    4 x max(64, K) flag records) and five hipEvents per host thread,
+ *    created on that thread's first polled solve, grown for a larger K, and
+ *    kept until the process ends; the fixed-budget form (graph_groups > 0)
+ *    uses neither.
  *  - Arrays are structure-of-arrays in the device layout: cells renumbered
  *    (reverse Cuthill-McKee), internal faces first then boundary faces.
  *    "[4][C]" means 4 contiguous rows of C floats (u, v, w, p).
  *  - Every function returns WCFD_OK or an error code; on WCFD_EHIP the HIP
  *    error is kept per thread and readable with wcfd_last_hip_error().
  *  - `stream` is a hipStream_t passed as void* (0 = the null stream).
- *  - Functions keep no global mutable state (the opt-in profiler and the
- *    Krylov launch shape below aside); calls on different streams with
- *    different workspaces may run concurrently.
+ *  - Functions keep no global mutable state, except the opt-in profiler and
+ *    the Krylov launch shape and fold limit (wcfd_set_krylov_launch / _fold),
+ *    which are process-wide: two callers that tune for different meshes must
+ *    not interleave their set and solve calls.  The fold limit never changes
+ *    results; the launch shape sets the order of the float64 partial sums.
+ *    Calls on different streams with different workspaces may run
+ *    concurrently.
  *
  * Each entry point cites the reference function it replaces
  * (paths relative to the JohnCSu/WarpCFD repository root).

@@ -155,3 +155,124 @@ def test_batch_report_per_case(gpu):
             assert _close(got['b'][j], same['b'][j], 1e-9), (k, j, got, same)
         for key in ('vel_corr', 'p', 'div'):
             assert _close(got[key], same[key], 1e-9), (k, key, got, same)
+
+
+def test_check_convergence_takes_the_reference_call(gpu):
+    """model.check_convergence(vel_equation.matrix, HbyA, vel_equation.rhs,
+    div_u, vel_correction.flatten(), p_new) -- incompressible.py:163, with the
+    reference-order objects of the composable API (a scipy matrix on rows
+    3c + k, interleaved (3C,) vectors, a (C,) pressure) -- gives the oracle
+    report of those same arrays; so do the Equation object itself and the
+    (C, 3) forms.  log=True sets model.res = |p_new| as the reference does."""
+    import torch
+    from oracle.simple import convergence_report, convergence_verdicts
+    from warpcfd_amd.FV.implicit_Solvers import IncompressibleSolver
+    from warpcfd_amd.FV.terms import ConvectionTerm, DiffusionTerm, Equation, GradTerm
+    nodes, cells = jittered_box((8, 6, 4), 0.1, 21, lengths=(1.0, 0.8, 0.5))
+    model = device_case(nodes, cells, 'cavity')
+    sv = IncompressibleSolver(model, 0.7, 0.3)
+    for _ in range(4):
+        sv.outer_iteration()
+    # the momentum system of the current state, term by term (incompressible.py:95-112)
+    model.set_boundary_conditions()
+    model.face_interpolation()
+    model.calculate_gradients()
+    model.calculate_mass_flux()
+    uvw = ['u', 'v', 'w']
+    conv, diff, grad = ConvectionTerm(model, uvw, 'upwind'), DiffusionTerm(model, uvw), \
+        GradTerm(model, 'p')
+    conv(model)
+    diff(model, viscosity=0.01)
+    grad(model)
+    vel_equation = Equation(model, fields=uvw)
+    vel_equation.form_system([conv, -diff], explicit_terms=-1.0 * grad, fvm=model)
+    vel_equation.relax(0.7, model)
+    C = model.num_cells
+    HbyA = model.cell_values[:, :3].reshape(-1).contiguous()          # interleaved (3C,)
+    rua_ref = model.dm.cells_to_reference(sv.rUA)
+    vel_correction = model.get_gradient('p', coeff=rua_ref)             # (C, 3)
+    p_new = model.dm.cells_to_reference(sv.p_x).contiguous()            # (C,)
+    div_u = torch.zeros(C, dtype=torch.float32, device=HbyA.device)
+    A = vel_equation.matrix
+    assert A.shape == (3 * C, 3 * C)
+    expect = convergence_report(A, HbyA.cpu().numpy(), vel_equation.rhs.cpu().numpy(),
+                                model.divFlux().cpu().numpy(),
+                                vel_correction.cpu().numpy().reshape(-1), p_new.cpu().numpy())
+    forms = [(A, HbyA, vel_equation.rhs, vel_correction.flatten()),
+             (vel_equation, HbyA.reshape(C, 3), vel_equation.rhs.reshape(C, 3).cpu().numpy(),
+              vel_correction)]
+    for mat, u, b, corr in forms:
+        got = model.convergence_norms(mat, u, b, corr, p_new)
+        for k in range(3):
+            assert _close(got['r'][k], expect['r'][k], 1e-9), (k, got, expect)
+            assert _close(got['b'][k], expect['b'][k], 1e-9), (k, got, expect)
+        for key in ('vel_corr', 'p', 'div'):
+            assert _close(got[key], expect[key], 1e-9), (key, got, expect)
+    verdicts, conv_ok = convergence_verdicts(expect)
+    ok = model.check_convergence(A, HbyA, vel_equation.rhs, div_u, vel_correction.flatten(), p_new,
+                                 verbose=False)
+    assert ok == conv_ok
+    logged = model.Convergence.log_list[-1]
+    for name, (_, v) in verdicts.items():
+        assert logged[name][1] == v, name
+    np.testing.assert_array_equal(model.res, np.abs(p_new.cpu().numpy()))
+
+
+def test_true_verdicts_at_the_fixed_point(gpu):
+    """From the float32 fixed point of the 24 x 24 cavity (committed fixture),
+    three more SIMPLE iterations on the device and in the oracle: the device
+    report's verdicts equal the oracle's criterion by criterion, and the
+    momentum criteria pass (True) on both."""
+    import os
+    from oracle.simple import OracleSolver, convergence_verdicts
+    from warpcfd_amd.FV.implicit_Solvers import IncompressibleSolver
+    from warpcfd_amd.preprocess.grid import cavity_mesh
+    fix = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden',
+                               'cavity24_converged.npz'))
+    nodes, cells = cavity_mesh(24)
+    model = device_case(nodes, cells, 'cavity')
+    model.set_initial_conditions(fix['f32'])
+    sv = IncompressibleSolver(model, 0.7, 0.3)
+    of = oracle_case(nodes, cells, 'cavity')
+    of.cell_values[:] = fix['f32']
+    os_ = OracleSolver(of, 0.7, 0.3, nu=0.01)
+    for _ in range(3):
+        sv.outer_iteration()
+        os_.outer_iteration()
+    verdicts, conv_ok = convergence_verdicts(os_.convergence_report())
+    ok = sv.check_convergence(verbose=False)
+    logged = model.Convergence.log_list[-1]
+    print({n: (logged[n], v) for n, v in verdicts.items()})
+    for name, (_, v) in verdicts.items():
+        assert logged[name][1] == v, (name, logged[name], verdicts[name])
+    for name in ('momentum_x', 'momentum_y'):
+        assert verdicts[name][1] and logged[name][1], name
+    assert ok == conv_ok
+
+
+def test_batch_report_of_a_finished_case(gpu):
+    """A case that the device monitor has stopped (poll() reports it
+    finished, its workgroups dropped from the stepping launches) is still
+    reported from its final state until it is released: the norms equal the
+    oracle function on that state within 1e-9."""
+    from oracle.simple import convergence_report
+    from test_gpu_batch import KW, cavity_spec
+    from warpcfd_amd.batch import CaseArena
+    quick = cavity_spec(8, 100.0, max_outer=20)         # stopped by its iteration cap
+    slow = cavity_spec(12, 400.0)
+    arena = CaseArena.for_cases([quick, slow], **KW)
+    done = []
+    for _ in range(40):
+        arena.step()
+        done += arena.poll()
+        if 0 in done:
+            break
+    assert 0 in done and 1 not in done
+    rep = arena.convergence_report()
+    assert rep[0] is not None and rep[1] is not None
+    same = convergence_report(*_case_inputs(arena, 0))
+    for j in range(3):
+        assert _close(rep[0]['r'][j], same['r'][j], 1e-9), (j, rep[0], same)
+        assert _close(rep[0]['b'][j], same['b'][j], 1e-9), (j, rep[0], same)
+    for key in ('vel_corr', 'p', 'div'):
+        assert _close(rep[0][key], same[key], 1e-9), (key, rep[0], same)

@@ -28,9 +28,10 @@ class Boundary:
         if self.finalized:
             raise RuntimeError('boundary conditions are fixed once the model is finalized')
         face_ids = self.get_face_id_array(face_ids)
-        assert np.all(np.isin(face_ids, self.boundary_ids)), \
-            'One or more of the provided face id is not an external face valid for applying boundary conditions'
-        assert output_vars, 'You must specify some sort of output variable'
+        inside = ~np.isin(face_ids, self.boundary_ids)
+        assert not inside.any(), \
+            f'faces {face_ids[inside][:10].tolist()} are not on the mesh boundary'
+        assert output_vars, 'no field given: name at least one of u, v, w, p as a keyword'
         rows = np.where(np.isin(self.boundary_ids, face_ids))[0]
         for name, value in output_vars.items():
             j = self.fields[name].index
@@ -39,14 +40,16 @@ class Boundary:
 
     def get_face_id_array(self, face_ids):
         if isinstance(face_ids, str):
-            assert face_ids in self.groups, 'Specified group name does not exist'
+            assert face_ids in self.groups, \
+                f'no face group {face_ids!r}; the mesh has {sorted(self.groups)}'
             g = self.groups[face_ids]
             return np.asarray(getattr(g, 'ids', g))
         if isinstance(face_ids, (int, np.integer, list, tuple)):
             return np.atleast_1d(np.asarray(face_ids, np.int32))
         if isinstance(face_ids, np.ndarray):
             return face_ids
-        raise ValueError(f'face_ids can be type string,int,list,tuple, or np.ndarray got {type(face_ids)} instead')
+        raise ValueError('face_ids must be a group name, an int, a list or tuple of ints or an '
+                         f'integer array, not {type(face_ids).__name__}')
 
     def gradient_BC(self, face_ids, **output_vars):
         self.set_value(face_ids, boundary_type='vonNeumann', **output_vars)
@@ -76,9 +79,10 @@ class Boundary:
         """boundary.py:108-121 checks; the arrays stay on the host."""
         if np.any(self.boundary_type < 1):
             bad = self.boundary_ids[(self.boundary_type < 1).any(axis=1)]
-            raise ValueError(f'The following faces have not been assigned a boundary condition: {bad}')
+            raise ValueError(f'{len(bad)} boundary faces lack a condition for some field, '
+                             f'e.g. faces {bad[:10].tolist()}')
         if np.any(self.boundary_type > 3):
-            raise ValueError('Only boundary types 1 and 2 are allowed')
+            raise ValueError('a boundary type is neither 1 (Dirichlet) nor 2 (Neumann)')
         self.finalized = True
 
     def uniform_velocity_types(self):

@@ -16,6 +16,20 @@ def require_gpu(device):
     return dev
 
 
+class HostReadable(torch.Tensor):
+    """A device tensor whose ``numpy()`` first copies to the host, as the
+    reference's warp arrays do: its scripts read results with
+    ``model.cell_values.numpy()`` (examples/LDC/LDC.py:54-55).  Everything
+    else is an ordinary ``torch.Tensor``."""
+
+    def numpy(self, *, force=False):
+        return self.detach().cpu().as_subclass(torch.Tensor).numpy()
+
+
+def host_readable(t):
+    return t.as_subclass(HostReadable)
+
+
 def stream_ptr(stream=None):
     s = stream if stream is not None else torch.cuda.current_stream()
     return ctypes.c_void_p(s.cuda_stream)

@@ -188,7 +188,8 @@ class IncompressibleSolver:
         corrected velocity, rUA grad p and the pressure solve's result."""
         m = self.model
         return m.check_convergence(self.mom_values, None, self.mom_rhs, None,
-                                   (self.rUA, m._cg[9:12]), self.p_x, log=log, verbose=verbose)
+                                   (self.rUA, m._cg[9:12]), m.dm.cells_to_reference(self.p_x),
+                                   log=log, verbose=verbose)
 
     def run(self, steps_per_check=10, callback=None, check=True):
         """incompressible.py:72-172 (no early stop, as the reference): every

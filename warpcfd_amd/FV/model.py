@@ -16,7 +16,7 @@ from ..material import Material
 from ..preprocess.layout import DeviceLayout
 from .boundary import Boundary
 from .convergence import Convergence
-from .device import DeviceMesh, ptr, require_gpu, stream_ptr
+from .device import DeviceMesh, host_readable, ptr, require_gpu, stream_ptr
 from .field import Field
 
 
@@ -139,23 +139,24 @@ class FVM:
     def cell_values(self):
         """(C,4) float32 tensor in reference cell order (a copy)."""
         self._need()
-        return self.dm.cells_to_reference(self._cv).T.contiguous()
+        return host_readable(self.dm.cells_to_reference(self._cv).T.contiguous())
 
     @property
     def cell_gradients(self):
         """(C,4,3) in reference cell order (a copy)."""
         self._need()
-        return self.dm.cells_to_reference(self._cg).T.reshape(self.num_cells, 4, 3).contiguous()
+        return host_readable(
+            self.dm.cells_to_reference(self._cg).T.reshape(self.num_cells, 4, 3).contiguous())
 
     @property
     def face_values(self):
         self._need()
-        return self.dm.faces_to_reference(self._fv).T.contiguous()
+        return host_readable(self.dm.faces_to_reference(self._fv).T.contiguous())
 
     @property
     def mass_fluxes(self):
         self._need()
-        return self.dm.faces_to_reference(self._flux).contiguous()
+        return host_readable(self.dm.faces_to_reference(self._flux).contiguous())
 
     @property
     def face_gradients(self):
@@ -216,46 +217,98 @@ class FVM:
             return arr
         return out
 
+    # ------------------------------------------ check_convergence arguments
+    def _ell_of(self, matrix):
+        """Momentum matrix as device ELL values [7][C]: a (7, C) device
+        tensor as is; an ``Equation`` (its shared plane); or a scipy matrix
+        on the reference's rows (C x C, or 3C x 3C interleaved as
+        ``Equation.matrix`` returns it, whose three component blocks must be
+        equal -- the matrix the reference's bsr_mv applies)."""
+        from .terms.equation import Equation
+        C = self.num_cells
+        if torch.is_tensor(matrix):
+            return matrix
+        if isinstance(matrix, Equation):
+            if not matrix._shared_matrix():
+                raise NotImplementedError('check_convergence: the velocity equation\'s three '
+                                          'outputs must share one matrix')
+            return matrix.values[0].contiguous()
+        import scipy.sparse as sp
+        if not sp.issparse(matrix):
+            raise TypeError(f'vel_matrix: expected an Equation, its .matrix or a device tensor, '
+                            f'got {type(matrix).__name__}')
+        A = matrix.tocsr()
+        n = A.shape[0]
+        if n == 3 * C:
+            Au = A[0::3, 0::3].tocsr()
+            # rows 3c + k hold the u block for every k, and nothing couples components
+            if (A - sp.kron(Au, sp.identity(3), format='csr')).count_nonzero():
+                raise NotImplementedError('check_convergence: the u, v and w blocks of the '
+                                          'velocity matrix must be equal and uncoupled')
+            A = Au
+        elif n != C:
+            raise ValueError(f'vel_matrix has {n} rows; expected {C} or {3 * C}')
+        L = self.layout
+        old = L.new2old
+        vals = np.zeros((7, C), np.float32)
+        vals[0] = np.asarray(A[old, old]).ravel()
+        for k in range(6):
+            nb = L.slot_nbr[k]
+            inner = nb != np.arange(C)
+            vals[k + 1, inner] = np.asarray(A[old[inner], old[nb[inner]]]).ravel()
+        return torch.from_numpy(vals).to(self._cv.device)
+
+    def _cells_of(self, t, rows):
+        """A cell field as a contiguous float32 device tensor in device
+        order: (rows, C) or (C,) device-order tensors as they are; the
+        reference's forms, (C * rows,) interleaved or (C, rows), reordered."""
+        if t is None:
+            return None
+        C, dev = self.num_cells, self._cv.device
+        t = torch.as_tensor(np.asarray(t) if not torch.is_tensor(t) else t).to(dev)
+        shape = tuple(t.shape)
+        if rows == 1 and shape == (C,):
+            return t.to(torch.float32).contiguous()
+        if rows > 1 and shape == (rows, C):
+            return t.to(torch.float32).contiguous()
+        if rows > 1 and shape in ((C * rows,), (C, rows)):
+            ref = t.to(torch.float32).reshape(C, rows).T.contiguous()
+            return self.dm.cells_to_device(ref).contiguous()
+        raise ValueError(f'expected {(rows, C) if rows > 1 else (C,)} in device order or '
+                         f'{(C * rows,)} / {(C, rows)} in reference order, got {shape}')
+
     def convergence_norms(self, vel_matrix=None, velocity=None, b=None, velocity_correction=None,
                           p_correction=None):
         """The norms of check_convergence from one device report
         (``wcfd_convergence_report``, float64), as a dict of floats: ``r``
         and ``b`` (3 each: ||A u_k - b_k||_2, ||b_k||_2), ``vel_corr``
         (max |c|), ``p`` (max |p|), ``div`` (sum |div F| of the current mass
-        fluxes).  Arguments are device-order tensors: ``vel_matrix`` the
-        momentum ELL values [7][C], ``velocity`` and ``b`` [3][C] (velocity
-        None: the current u, v, w), ``velocity_correction`` [3][C] or a pair
-        (rUA [C], grad p [3][C]), ``p_correction`` [C].  None skips a group
-        (its norms are NaN)."""
+        fluxes).  ``vel_matrix``: see ``_ell_of``.  ``velocity`` (None: the
+        current u, v, w), ``b`` and ``velocity_correction``: [3][C] device
+        order, or the reference's interleaved (3C,) / (C, 3) forms;
+        ``velocity_correction`` may also be a pair (rUA [C], grad p [3][C])
+        in device order.  ``p_correction``: (C,), either order (only its
+        maximum is used).  None skips a group (its norms are NaN)."""
         self._need()
         dev = self._cv.device
         if self._conv_ws is None:
             n = self.lib.wcfd_convergence_workspace_bytes()
             self._conv_ws = torch.zeros(n, dtype=torch.uint8, device=dev)
             self._conv_out = torch.zeros(9, dtype=torch.float64, device=dev)
-        C = self.num_cells
-
-        def vec(t, rows):
-            if t is None:
-                return None
-            if tuple(t.shape) != ((rows, C) if rows > 1 else (C,)) or not t.is_contiguous() \
-                    or t.dtype != torch.float32 or t.device != dev:
-                raise ValueError(f'expected a contiguous float32 device tensor of shape '
-                                 f'{(rows, C) if rows > 1 else (C,)}, got {tuple(t.shape)}')
-            return t
-
-        vals = vec(vel_matrix, 7)
+        vals = self._ell_of(vel_matrix) if vel_matrix is not None else None
         if vals is not None:
-            u = vec(self._cv[:3] if velocity is None else velocity, 3)
-            bb = vec(b, 3)
+            vals = self._cells_of(vals, 7)
+            u = self._cells_of(self._cv[:3] if velocity is None else velocity, 3)
+            bb = self._cells_of(b, 3)
         else:
             u = bb = None
         rua = corr = None
         if isinstance(velocity_correction, (tuple, list)):
-            rua, corr = vec(velocity_correction[0], 1), vec(velocity_correction[1], 3)
+            rua, corr = self._cells_of(velocity_correction[0], 1), \
+                self._cells_of(velocity_correction[1], 3)
         else:
-            corr = vec(velocity_correction, 3)
-        p = vec(p_correction, 1)
+            corr = self._cells_of(velocity_correction, 3)
+        p = self._cells_of(p_correction, 1)
         _lib.check(self.lib.wcfd_convergence_report(
             self.dm.desc_ref, ptr(vals), ptr(u), ptr(bb), ptr(rua), ptr(corr), ptr(p),
             ptr(self._flux), ptr(self._conv_ws), ptr(self._conv_out), self._s()),
@@ -272,12 +325,16 @@ class FVM:
                           p_correction=None, log=True, verbose=True):
         """model.py:278-315: momentum_x/y/z, continuity, local_continuity,
         velocity_correction and pressure_correction of ``self.Convergence``
-        from one device report (see ``convergence_norms`` for the
-        arguments).  As the reference, continuity is taken from the current
-        mass fluxes (``div_u`` is only a buffer there, recomputed by
-        divFlux), a missing correction reports NaN and leaves its criterion's
-        verdict as it was, the dict is printed and logged, and a non-finite
-        momentum_x residual raises ValueError.  Returns has_converged()."""
+        from one device report.  Takes the reference's call as it stands
+        (incompressible.py:163: ``vel_equation.matrix, HbyA,
+        vel_equation.rhs, div_u, vel_correction.flatten(), p_new``) as well as
+        device-order buffers; see ``convergence_norms``.  As the reference,
+        continuity is taken from the current mass fluxes (``div_u`` is only a
+        buffer there, recomputed by divFlux), a missing correction reports NaN
+        and leaves its criterion's verdict as it was, the dict is printed and,
+        with ``log``, logged and ``self.res`` set to |p_correction| (a host
+        array), and a non-finite momentum_x residual raises ValueError.
+        Returns has_converged()."""
         n = self.convergence_norms(vel_matrix, velocity, b, velocity_correction, p_correction)
         cv = self.Convergence
         nan = float('nan')
@@ -296,6 +353,10 @@ class FVM:
             print(conv)
         if log:
             cv.log(conv)
+            if p_correction is not None:
+                pc = p_correction.detach().cpu().numpy() if torch.is_tensor(p_correction) \
+                    else np.asarray(p_correction)
+                self.res = np.abs(pc)
         if np.isnan(conv['momentum_x'][0]):
             raise ValueError('momentum_x residual is not finite')
         return cv.has_converged()

@@ -24,7 +24,7 @@ import numpy as np
 import torch
 
 from ... import _lib
-from ..device import ptr, stream_ptr
+from ..device import host_readable, ptr, stream_ptr
 from ..field import Field
 from .terms import Term, int32_array
 
@@ -77,7 +77,7 @@ class Equation:
     # --------------------------------------------------- reference layout
     def _to_ref(self, t):
         """(O, C) device -> (C * O,) interleaved, reference cell order."""
-        return self.fvm.dm.cells_to_reference(t).T.reshape(-1).contiguous()
+        return host_readable(self.fvm.dm.cells_to_reference(t).T.reshape(-1).contiguous())
 
     def _to_dev(self, v):
         """(C * O,) interleaved, reference cell order -> (O, C) device."""

@@ -574,29 +574,64 @@ class CaseArena:
             self._build_descriptors()      # drop the finished cases' workgroups
         return [k for k, sl in enumerate(self.slots) if sl is not None and not act[k]]
 
+    def _report_batch(self):
+        """A copy of the batch descriptor whose workgroup ranges cover every
+        resident case, finished ones included (the stepping descriptor drops
+        those), so a case can be reported after poll() has stopped it.  A
+        case gets the same ceil(C / chunk) workgroups in both descriptors, so
+        its report does not depend on which descriptor it came from."""
+        K, ch = self.K, self.chunk
+        nb = np.zeros(K, np.int64)
+        for k, sl in enumerate(self.slots):
+            if sl is not None:
+                nb[k] = (sl['spec'].n_cells + ch - 1) // ch
+        blk_off = np.zeros(K + 1, np.int32)
+        blk_off[1:] = np.cumsum(nb)
+        G = int(blk_off[-1])
+        dev = self.device
+        if getattr(self, '_rep_blk_off', None) is None:
+            self._rep_blk_off = torch.zeros(K + 1, dtype=torch.int32, device=dev)
+            self._rep_blk_case = torch.zeros(self.Gmax, dtype=torch.int32, device=dev)
+        self._rep_blk_off.copy_(torch.from_numpy(blk_off).to(dev))
+        if G:
+            self._rep_blk_case[:G] = torch.from_numpy(
+                np.repeat(np.arange(K, dtype=np.int32), nb)).to(dev)
+        b = _lib.WcfdBatch.from_buffer_copy(self.batch)
+        b.n_blocks = G
+        b.max_case_blocks = int(nb.max()) if K else 0
+        b.blk_off, b.blk_case = self._rep_blk_off.data_ptr(), self._rep_blk_case.data_ptr()
+        return b
+
     def convergence_report(self, stream=None):
         """Per-case norms of check_convergence (model.py:278-315) from one
         device pass (``wcfd_batch_convergence_report``, float64): for each
-        slot None (empty or finished) or a dict as ``FVM.convergence_norms``
-        returns -- ``r``, ``b`` (||A u_k - b_k||_2, ||b_k||_2), ``vel_corr``
-        (max |rUA grad p|), ``p`` (max |p|, the pressure solve's result),
-        ``div`` (sum |div F| of the case's mass fluxes).  A case's report
-        depends only on the case."""
-        if self.n_blocks == 0:
+        slot None (empty) or a dict as ``FVM.convergence_norms`` returns --
+        ``r``, ``b`` (||A u_k - b_k||_2, ||b_k||_2), ``vel_corr`` (max |rUA
+        grad p|), ``p`` (max |p|, the pressure solve's result), ``div`` (sum
+        |div F| of the case's mass fluxes).  Every resident case is reported,
+        including one that has finished but is not yet released (its final
+        state).  A case's report depends only on the case."""
+        bt = self._report_batch()
+        if bt.n_blocks == 0:
             return [None] * self.K
-        need = self.lib.wcfd_batch_convergence_workspace_bytes(self.n_blocks)
+        need = self.lib.wcfd_batch_convergence_workspace_bytes(bt.n_blocks)
         if getattr(self, '_conv_ws', None) is None or self._conv_ws.numel() < need:
             self._conv_ws = torch.zeros(need, dtype=torch.uint8, device=self.device)
             self._conv_out = torch.zeros(9 * self.K, dtype=torch.float64, device=self.device)
+        if stream is not None:
+            stream.wait_stream(torch.cuda.current_stream())   # buffers written there
         code = self.lib.wcfd_batch_convergence_report(
-            ctypes.byref(self.desc), ctypes.byref(self.batch), ptr(self.mom_values), ptr(self.cv),
+            ctypes.byref(self.desc), ctypes.byref(bt), ptr(self.mom_values), ptr(self.cv),
             ptr(self.mom_rhs), ptr(self.rua), ptr(self.cg[9:12]), ptr(self.p_x), ptr(self.flux),
             ptr(self._conv_ws), ptr(self._conv_out), stream_ptr(stream))
         _lib.check(code, 'batch convergence report')
+        if stream is not None:
+            # the copy below runs on the current stream: order it after the report
+            torch.cuda.current_stream().wait_stream(stream)
         o = self._conv_out.cpu().numpy().reshape(self.K, 9)
         rep = []
         for k, sl in enumerate(self.slots):
-            if sl is None or not self.h_active[k]:
+            if sl is None:
                 rep.append(None)
                 continue
             rep.append({'r': [float(x) for x in o[k, 0:3]], 'b': [float(x) for x in o[k, 3:6]],

@@ -98,6 +98,48 @@ def structured_hex(counts, lengths=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0),
     return nodes, cells.astype(np.int32)
 
 
+class UnstructuredGrid:
+    """The two members of a pyvista ``UnstructuredGrid`` that ``Mesh`` reads:
+    ``points`` (N, 3) and ``cells_dict`` {VTK cell type: (C, n) node ids}."""
+
+    def __init__(self, points, cells, cell_type):
+        self.points = np.asarray(points, np.float64)
+        self.cells_dict = {int(cell_type): np.asarray(cells, np.int32)}
+
+    @property
+    def n_cells(self):
+        return next(iter(self.cells_dict.values())).shape[0]
+
+
+_ELEMENTS = {'hex': (12, None), 'wedge': (13, 'wedge'), 'tet': (10, 'tetra')}
+
+
+def create_2D_grid(origin, nx, ny, dx, dy, dz=0.1, *, element_type='hex',
+                   unstructured_wedge=False, display_mesh=False, save=None,
+                   gmsh_version2=False):
+    """preprocess/grid.py:8-159 without gmsh: an nx x ny x 1 box of size
+    dx x dy x dz at ``origin`` as hexahedra, or each hexahedron split into 2
+    wedges or 6 tetrahedra, returned as an ``UnstructuredGrid`` for ``Mesh``.
+    ``save`` writes ``{save}.msh`` (format 2.2 if ``gmsh_version2``, else
+    4.1, gmsh's default).  ``display_mesh`` opened gmsh's viewer and is
+    ignored.  Cell and node numbering is lexicographic (x fastest) rather than
+    gmsh's; ``unstructured_wedge`` needs gmsh's mesher and is refused -- read
+    such a mesh from its ``.msh`` file with ``read_msh`` instead."""
+    if element_type not in _ELEMENTS:
+        raise ValueError(f'element_type must be one of {sorted(_ELEMENTS)}, got {element_type!r}')
+    if unstructured_wedge:
+        raise NotImplementedError('unstructured wedge meshing needs gmsh; mesh with gmsh and '
+                                  'load the .msh file with warp_cfd.preprocess.read_msh')
+    vtk, split = _ELEMENTS[element_type]
+    nodes, cells = structured_hex((nx, ny, 1), (dx, dy, dz), origin)
+    if split:
+        cells = split_hexes(cells, split)
+    if save is not None:
+        from .io import write_msh
+        write_msh(f'{save}.msh', nodes, cells, version=2.2 if gmsh_version2 else 4.1)
+    return UnstructuredGrid(nodes, cells, vtk)
+
+
 def cavity_mesh(n, dz=0.1):
     """Unit-square n x n x 1 cavity, dz thick (examples/LDC/LDC.py:24-29)."""
     return structured_hex((n, n, 1), (1.0, 1.0, dz))
