@@ -11,10 +11,13 @@ container; reads the reference's data files from /root/reference).
 * cavity24_converged.npz -- oracle float32 and float64, 24x24 cavity run to
                         the fixed point (700 iterations; float64 continuity
                         ~1e-12, Krylov atol 1e-13).
+* cavity64_converged.npz -- oracle float64, the 64x64 cavity (LDC.py set-up)
+                        run to continuity <= 1e-10 (Krylov atol 1e-13): the
+                        fixed point the 500-iteration runs approach.
 * mesh_3x3x2_jitter.npz -- connectivity / CSR tables of a jittered 3x3x2 hex
                         mesh from the brute-force pair loop.
 
-    python tests/golden/make_golden.py [--skip-long | --cavity24]
+    python tests/golden/make_golden.py [--skip-long | --cavity24 | --cavity64]
 """
 import json
 import os
@@ -48,9 +51,12 @@ def cavity(n, dtype, atol, iters, stop=None):
     of = oracle_case(nodes, cells, 'cavity', dtype=dtype)
     s = OracleSolver(of, 0.7, 0.3, nu=0.01, atol=atol)
     hist = []
-    for _ in range(iters):
+    for i in range(iters):
         info = s.outer_iteration()
         hist.append(info['continuity'])
+        if i % 200 == 0:
+            print(f'cavity{n} {np.dtype(dtype).name}: iteration {i}, continuity {hist[-1]:.3e}',
+                  flush=True)
         if stop is not None and info['continuity'] < stop:
             break
     return of.cell_values.copy(), np.array(hist)
@@ -74,12 +80,22 @@ def mesh_tables():
 def main():
     if '--cavity24' in sys.argv:
         return cavity24()
+    if '--cavity64' in sys.argv:
+        return cavity64()
     ghia()
     mesh_tables()
     if '--skip-long' not in sys.argv:
         cv, hist = cavity(64, np.float32, None, 500)
         np.savez_compressed(os.path.join(HERE, 'cavity64_ref500.npz'), cell_values=cv, continuity=hist)
         cavity24()
+        cavity64()
+
+
+def cavity64():
+    """64x64 Re 100 cavity, float64 oracle to continuity <= 1e-10."""
+    c64, h64 = cavity(64, np.float64, 1e-13, 20000, stop=1e-10)
+    np.savez_compressed(os.path.join(HERE, 'cavity64_converged.npz'), f64=c64,
+                        iters64=len(h64), cont64=h64[-1], history=h64[::10])
 
 
 def cavity24():

@@ -78,3 +78,27 @@ def test_cavity64_500_iterations_reference_settings(gpu):
     _, ug, vg = _centrelines(got, 64)
     _, ur, vr = _centrelines(ref, 64)
     assert abs(ug.min() - ur.min()) < 2e-3 and abs(vg.max() - vr.max()) < 2e-3
+
+
+def test_cavity64_fixed_point_vs_fp64_oracle(gpu):
+    """The 64 x 64 cavity (the LDC.py set-up) run to its fixed point with tight
+    Krylov tolerances: u, v and p within 1e-4 of the float64 oracle run to
+    continuity 1e-10 (committed fixture cavity64_converged.npz)."""
+    from warpcfd_amd.FV.implicit_Solvers import IncompressibleSolver
+    from warpcfd_amd.preprocess.grid import cavity_mesh
+    z = np.load(os.path.join(G, 'cavity64_converged.npz'))
+    nodes, cells = cavity_mesh(64)
+    model = device_case(nodes, cells, 'cavity')
+    sv = IncompressibleSolver(model, 0.7, 0.3)
+    sv.tol = 1e-9
+    sv.max_iter = 2000
+    for it in range(4000):
+        d = sv.outer_iteration()
+        if it >= int(z['iters64']) and d['continuity'] < 1e-7:
+            break
+    got = model.cell_values.cpu().numpy()
+    err = [rel_l2(got[:, q], z['f64'][:, q]) for q in (0, 1, 3)]
+    print(f'cavity64: {it + 1} iterations, continuity {d["continuity"]:.2e}, '
+          f'rel-L2 vs float64 fixed point (u, v, p): {err}')
+    assert d['continuity'] < 1e-6
+    assert max(err) <= 1e-4
