@@ -77,7 +77,7 @@ def test_piso_cylinder_tight_krylov_vs_oracle(gpu, p_refine):
     step, 3e-5 to 4e-5 after, against the oracle's 4.4e-4 and 2e-5 to
     9e-5); without it the device and the float32 oracle are two independent
     samples of the float32 floor (1.2e-4 against 5e-5 at step 1), held to
-    twice that floor.
+    twice that floor."""
     from oracle.fv import OracleFVM
     from oracle.mesh import OracleMesh
     from oracle.simple import OracleSolver
