@@ -136,6 +136,22 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
+// wave_sum of ND values at once: each butterfly step issues the exchanges of
+// all ND values before adding, so their LDS-crossbar latencies overlap (one
+// wait per step instead of one per value and step).  Every value is summed in
+// wave_sum's order, so the results are bitwise the same.
+template <int ND>
+__device__ inline void wave_sum_n(double (&v)[ND]) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    double o[ND];
+#pragma unroll
+    for (int i = 0; i < ND; ++i) o[i] = __shfl_xor(v[i], off, kWave);
+#pragma unroll
+    for (int i = 0; i < ND; ++i) v[i] += o[i];
+  }
+}
+
 // ------------------------------------------------ uniform base + offset
 // base (uniform across the wave: a kernel argument plus uniform terms) plus
 // a 32-bit byte offset per lane.  The compiler forms such an address as a
@@ -427,11 +443,10 @@ __device__ inline void fold_n(const double* const (&p)[ND], int nb, double (&out
   for (int i = 0; i < ND; ++i) s[i] = 0.0;
   fold_thread<ND, U>(p, nb, s);
   const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  wave_sum_n<ND>(s);
 #pragma unroll
-  for (int i = 0; i < ND; ++i) {
-    s[i] = wave_sum(s[i]);
+  for (int i = 0; i < ND; ++i)
     if (lane == 0) lds[i * kMaxWaves + wid] = s[i];
-  }
   __syncthreads();
   if (threadIdx.x < ND) {
     double t = 0.0;
@@ -443,16 +458,71 @@ __device__ inline void fold_n(const double* const (&p)[ND], int nb, double (&out
   for (int i = 0; i < ND; ++i) out[i] = res[i];
 }
 
+// A fold_n whose loads start early.  Inside an iteration kernel the fold of
+// the previous kernel's partials and the row's own gathers are independent,
+// but vector loads retire in issue order: partials loaded after the row's
+// gathers can only be summed once every gather has landed.  load() issues a
+// thread's first E terms per sum (all of them when nb <= E * blockDim) before
+// the caller issues the row's loads; finish() adds them, and any further
+// terms, in fold_thread's order and reduces the block as fold_n does, so the
+// result is bitwise fold_n's.  The wave and block reduction then overlaps the
+// gathers still in flight.
+template <int ND, int E = kFoldUnroll>
+struct EarlyFold {
+  double v[E][ND];
+  __device__ __forceinline__ void load(const double* const (&p)[ND], int nb) {
+    const int step = blockDim.x;
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const int b = (int)threadIdx.x + u * step;
+#pragma unroll
+      for (int i = 0; i < ND; ++i) v[u][i] = b < nb ? p[i][b] : 0.0;
+    }
+  }
+  __device__ __forceinline__ void finish(const double* const (&p)[ND], int nb,
+                                         double (&out)[ND]) {
+    __shared__ double lds[ND * kMaxWaves];
+    __shared__ double res[ND];
+    const int step = blockDim.x;
+    double s[ND];
+#pragma unroll
+    for (int i = 0; i < ND; ++i) s[i] = 0.0;
+#pragma unroll
+    for (int u = 0; u < E; ++u)
+      if ((int)threadIdx.x + u * step < nb) {
+#pragma unroll
+        for (int i = 0; i < ND; ++i) s[i] += v[u][i];
+      }
+    for (int b = (int)threadIdx.x + E * step; b < nb; b += step) {
+#pragma unroll
+      for (int i = 0; i < ND; ++i) s[i] += p[i][b];
+    }
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    wave_sum_n<ND>(s);
+#pragma unroll
+    for (int i = 0; i < ND; ++i)
+      if (lane == 0) lds[i * kMaxWaves + wid] = s[i];
+    __syncthreads();
+    if (threadIdx.x < ND) {
+      double t = 0.0;
+      for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += lds[threadIdx.x * kMaxWaves + w];
+      res[threadIdx.x] = t;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < ND; ++i) out[i] = res[i];
+  }
+};
+
 // Block sum of ND values; thread 0 stores them as partials part[i*G + vb].
 template <int ND>
 __device__ inline void store_partials(double (&v)[ND], double* __restrict__ part, int G, int vb) {
   __shared__ double lds[ND * kMaxWaves];
   const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  wave_sum_n<ND>(v);
 #pragma unroll
-  for (int i = 0; i < ND; ++i) {
-    double s = wave_sum(v[i]);
-    if (lane == 0) lds[i * kMaxWaves + wid] = s;
-  }
+  for (int i = 0; i < ND; ++i)
+    if (lane == 0) lds[i * kMaxWaves + wid] = v[i];
   __syncthreads();
   if (threadIdx.x == 0) {
 #pragma unroll

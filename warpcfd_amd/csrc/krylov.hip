@@ -234,6 +234,10 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_1(M A, Seg sg, int it, c
   if (S->f.done) return;
   // one row per thread on an ELL matrix: issue the row's loads before the fold
   constexpr bool pf = PF && M::kEll;
+  const double* const fp[1] = {PA + br.b0};
+  EarlyFold<1> ef;
+  if constexpr (pf)
+    if (it) ef.load(fp, br.nf);
   const int i0 = br.begin + (int)threadIdx.x;
   const bool has = pf && i0 < br.end;
   float a[M::kRow], rc[M::kRow], ic[M::kRow], pc[M::kRow];
@@ -254,7 +258,14 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_1(M A, Seg sg, int it, c
   bool first = (it == 0);
   double beta_d = 0.0;
   if (!first) {
-    const double rz = fold(PA + br.b0, br.nf);   // r.z of the previous iteration
+    double rz;                                   // r.z of the previous iteration
+    if constexpr (pf) {
+      double o1[1];
+      ef.finish(fp, br.nf, o1);
+      rz = o1[0];
+    } else {
+      rz = fold(PA + br.b0, br.nf);
+    }
     beta_d = safe_div(rz, S->rz[(it - 1) & 1]);
     if (br.vb == br.b0 && threadIdx.x == 0) S->rz[it & 1] = rz;
     first = (beta_d == 0.0);
@@ -304,6 +315,9 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_2(Seg sg, int it, float*
   if (S->f.done) return;
   // one row per thread: load it before the fold
   constexpr bool pf = PF;
+  const double* const fp[1] = {PB + br.b0};
+  EarlyFold<1> ef;
+  if constexpr (pf) ef.load(fp, br.nf);
   const int i0 = br.begin + (int)threadIdx.x;
   const bool has = pf && i0 < br.end;
   float xv = 0.f, pv = 0.f, rv = 0.f, qv = 0.f, iv = 0.f;
@@ -314,7 +328,14 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_2(Seg sg, int it, float*
     qv = q[i0];
     iv = inv[i0];
   }
-  const double pq = fold(PB + br.b0, br.nf);
+  double pq;
+  if constexpr (pf) {
+    double o1[1];
+    ef.finish(fp, br.nf, o1);
+    pq = o1[0];
+  } else {
+    pq = fold(PB + br.b0, br.nf);
+  }
   const float alpha = (float)safe_div(S->rz[it & 1], pq);
   double acc[2] = {0.0, 0.0};
   if (pf) {
@@ -380,7 +401,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_init2(M A, Seg sg, cons
   store_partials<3>(acc, PA, sg.G, br.vb);
 }
 
-template <class M, bool PF>
+template <class M, bool PF, bool FIRST>
 __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int it, float* __restrict__ x,
                            const float* __restrict__ r_in, float* __restrict__ r_out,
                            const float* __restrict__ w_in, float* __restrict__ w_out,
@@ -392,9 +413,18 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int i
   KState* S = st + br.k;
   if (S->f.done) return;
   const int G = sg.G;
-  const bool first = (it == 0);
-  // one row per thread on an ELL matrix: issue the row's loads before the fold
+  // one row per thread on an ELL matrix: issue the row's loads before the
+  // fold, and the fold's own loads (and the case's scalars) before those
   constexpr bool pf = PF && M::kEll;
+  const int pv = (it - 1) & 1;
+  const double* pp[2] = {P_in + br.b0, P_in + G + br.b0};
+  EarlyFold<2> ef;
+  double rz_prev = 0.0, alpha_prev = 0.0;
+  if constexpr (!FIRST) {
+    if constexpr (pf) ef.load(pp, br.nf);
+    rz_prev = S->rz[pv];
+    alpha_prev = S->alpha[pv];
+  }
   const int i0 = br.begin + (int)threadIdx.x;
   const bool has = pf && i0 < br.end;
   float a[M::kRow], wc[M::kRow], sc[M::kRow], ic[M::kRow], rc[M::kRow];
@@ -409,25 +439,26 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int i
 #pragma unroll
       for (int j = 0; j < M::kRow; ++j) {
         wc[j] = W(o[j]);
-        sc[j] = first ? 0.f : Sv(o[j]);
+        sc[j] = FIRST ? 0.f : Sv(o[j]);
         ic[j] = I(o[j]);
         rc[j] = R(o[j]);
       }
-      p0 = first ? 0.f : ld_off((const float*)p, o0);
+      p0 = FIRST ? 0.f : ld_off((const float*)p, o0);
       x0 = ld_off((const float*)x, o0);
     }
   }
   double alpha_d, beta_d = 0.0;
-  if (first) {
+  if constexpr (FIRST) {
     alpha_d = S->alpha[0];
   } else {
-    const int pv = (it - 1) & 1;
-    const double* pp[2] = {P_in + br.b0, P_in + G + br.b0};
     double gd[2];
-    fold_n<2>(pp, br.nf, gd);
+    if constexpr (pf)
+      ef.finish(pp, br.nf, gd);
+    else
+      fold_n<2>(pp, br.nf, gd);
     const double g = gd[0], d = gd[1];
-    beta_d = safe_div(g, S->rz[pv]);
-    alpha_d = safe_div(g, d - beta_d * safe_div(g, S->alpha[pv]));
+    beta_d = safe_div(g, rz_prev);
+    alpha_d = safe_div(g, d - beta_d * safe_div(g, alpha_prev));
     if (br.vb == br.b0 && threadIdx.x == 0) {
       S->rz[it & 1] = g;
       S->alpha[it & 1] = alpha_d;
@@ -440,13 +471,13 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int i
       float wn = 0.f;
 #pragma unroll
       for (int j = 0; j < M::kRow; ++j) {
-        const float scj = first ? wc[j] : wc[j] + beta * sc[j];
+        const float scj = FIRST ? wc[j] : wc[j] + beta * sc[j];
         wn += a[j] * (ic[j] * (rc[j] - alpha * scj));
       }
       const float ii = ic[0];
       const float ui = rc[0] * ii;
-      const float pi = first ? ui : ui + beta * p0;
-      const float si = first ? wc[0] : wc[0] + beta * sc[0];
+      const float pi = FIRST ? ui : ui + beta * p0;
+      const float si = FIRST ? wc[0] : wc[0] + beta * sc[0];
       st_off(x, o0, x0 + alpha * pi);
       const float rn = rc[0] - alpha * si;
       st_off(p, o0, pi);
@@ -462,13 +493,13 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int i
     for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
       float wn = 0.f;
       A.row(i, [&](float a, int c) {
-        const float sc = first ? w_in[c] : w_in[c] + beta * s_in[c];
+        const float sc = FIRST ? w_in[c] : w_in[c] + beta * s_in[c];
         wn += a * (inv[c] * (r_in[c] - alpha * sc));
       });
       const float ii = inv[i];
       const float ui = r_in[i] * ii;
-      const float pi = first ? ui : ui + beta * p[i];
-      const float si = first ? w_in[i] : w_in[i] + beta * s_in[i];
+      const float pi = FIRST ? ui : ui + beta * p[i];
+      const float si = FIRST ? w_in[i] : w_in[i] + beta * s_in[i];
       x[i] = x[i] + alpha * pi;
       const float rn = r_in[i] - alpha * si;
       p[i] = pi;
@@ -487,8 +518,11 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int i
 // Two rows per thread (chunk <= 2 B): rows tid and tid + B of the chunk,
 // the grid-stride order, with all their loads issued before the fold; the
 // same sums as the grid-stride form.  Kept apart from k_cgs_iter so that
-// the one-row form keeps its register count (and occupancy).
-template <class M>
+// the one-row form keeps its register count (and occupancy).  The fold's
+// partials and the case's scalars are loaded before the rows (EarlyFold), so
+// the fold completes while the gathers are in flight.  FIRST (iteration 0:
+// no fold, s = p = 0) is a separate instance, without the branches.
+template <class M, bool FIRST>
 __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter2(M A, Seg sg, int it, float* __restrict__ x,
                            const float* __restrict__ r_in, float* __restrict__ r_out,
                            const float* __restrict__ w_in, float* __restrict__ w_out,
@@ -500,9 +534,17 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter2(M A, Seg sg, int 
   KState* S = st + br.k;
   if (S->f.done) return;
   const int G = sg.G;
-  const bool first = (it == 0);
   static_assert(M::kEll, "the two-row form reads ELL rows");
   constexpr int NR = 2;
+  const int pv = (it - 1) & 1;
+  const double* pp[2] = {P_in + br.b0, P_in + G + br.b0};
+  EarlyFold<2> ef;
+  double rz_prev = 0.0, alpha_prev = 0.0;
+  if constexpr (!FIRST) {
+    ef.load(pp, br.nf);
+    rz_prev = S->rz[pv];
+    alpha_prev = S->alpha[pv];
+  }
   float a[NR][M::kRow], wc[NR][M::kRow], sc[NR][M::kRow], ic[NR][M::kRow], rc[NR][M::kRow];
   float p0[NR], x0[NR];
   bool has[NR];
@@ -522,25 +564,23 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter2(M A, Seg sg, int 
 #pragma unroll
       for (int j = 0; j < M::kRow; ++j) {
         wc[q][j] = W(o[j]);
-        sc[q][j] = first ? 0.f : Sv(o[j]);
+        sc[q][j] = FIRST ? 0.f : Sv(o[j]);
         ic[q][j] = I(o[j]);
         rc[q][j] = Rv(o[j]);
       }
-      p0[q] = first ? 0.f : ld_off((const float*)p, o0[q]);
+      p0[q] = FIRST ? 0.f : ld_off((const float*)p, o0[q]);
       x0[q] = ld_off((const float*)x, o0[q]);
     }
   }
   double alpha_d, beta_d = 0.0;
-  if (first) {
+  if constexpr (FIRST) {
     alpha_d = S->alpha[0];
   } else {
-    const int pv = (it - 1) & 1;
-    const double* pp[2] = {P_in + br.b0, P_in + G + br.b0};
     double gd[2];
-    fold_n<2>(pp, br.nf, gd);
+    ef.finish(pp, br.nf, gd);
     const double g = gd[0], d = gd[1];
-    beta_d = safe_div(g, S->rz[pv]);
-    alpha_d = safe_div(g, d - beta_d * safe_div(g, S->alpha[pv]));
+    beta_d = safe_div(g, rz_prev);
+    alpha_d = safe_div(g, d - beta_d * safe_div(g, alpha_prev));
     if (br.vb == br.b0 && threadIdx.x == 0) {
       S->rz[it & 1] = g;
       S->alpha[it & 1] = alpha_d;
@@ -554,13 +594,13 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter2(M A, Seg sg, int 
     float wn = 0.f;
 #pragma unroll
     for (int j = 0; j < M::kRow; ++j) {
-      const float scj = first ? wc[q][j] : wc[q][j] + beta * sc[q][j];
+      const float scj = FIRST ? wc[q][j] : wc[q][j] + beta * sc[q][j];
       wn += a[q][j] * (ic[q][j] * (rc[q][j] - alpha * scj));
     }
     const float ii = ic[q][0];
     const float ui = rc[q][0] * ii;
-    const float pi = first ? ui : ui + beta * p0[q];
-    const float si = first ? wc[q][0] : wc[q][0] + beta * sc[q][0];
+    const float pi = FIRST ? ui : ui + beta * p0[q];
+    const float si = FIRST ? wc[q][0] : wc[q][0] + beta * sc[q][0];
     st_off(x, o0[q], x0[q] + alpha * pi);
     const float rn = rc[q][0] - alpha * si;
     st_off(p, o0[q], pi);
@@ -619,6 +659,10 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_1(M A, Seg sg, int it, c
   const int n = sg.n;
   // one row per thread on an ELL matrix: issue the row's loads before the fold
   constexpr bool pf = PF && M::kEll;
+  const double* const fp[1] = {PA + br.b0};
+  EarlyFold<1> ef;
+  if constexpr (pf)
+    if (it) ef.load(fp, br.nf);
   const int i0 = br.begin + (int)threadIdx.x;
   const bool has = pf && i0 < br.end;
   float a[M::kRow], ic[M::kRow], rr[NV][M::kRow], pp[NV][M::kRow], vv[NV][M::kRow], r0i[NV];
@@ -647,7 +691,14 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_1(M A, Seg sg, int it, c
   double beta_d = 0.0, omega_d = 0.0;
   if (!first) {
     const int pv = (it - 1) & 1;
-    const double rho = fold(PA + br.b0, br.nf);   // r0.r after the previous iteration
+    double rho;                                   // r0.r after the previous iteration
+    if constexpr (pf) {
+      double o1[1];
+      ef.finish(fp, br.nf, o1);
+      rho = o1[0];
+    } else {
+      rho = fold(PA + br.b0, br.nf);
+    }
     omega_d = S->omega[pv];
     beta_d = safe_div(rho, S->rho[pv]) * safe_div(S->alpha[pv], omega_d);
     if (br.vb == br.b0 && threadIdx.x == 0) S->rho[it & 1] = rho;
@@ -717,6 +768,9 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_2(M A, Seg sg, int it, c
   const int n = sg.n;
   // one row per thread on an ELL matrix: issue the row's loads before the fold
   constexpr bool pf = PF && M::kEll;
+  const double* const fp[1] = {PB + br.b0};
+  EarlyFold<1> ef;
+  if constexpr (pf) ef.load(fp, br.nf);
   const int i0 = br.begin + (int)threadIdx.x;
   const bool has = pf && i0 < br.end;
   float a[M::kRow], ic[M::kRow], rr[NV][M::kRow], vv[NV][M::kRow];
@@ -738,7 +792,14 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_2(M A, Seg sg, int it, c
       }
     }
   }
-  const double r0v = fold(PB + br.b0, br.nf);
+  double r0v;
+  if constexpr (pf) {
+    double o1[1];
+    ef.finish(fp, br.nf, o1);
+    r0v = o1[0];
+  } else {
+    r0v = fold(PB + br.b0, br.nf);
+  }
   const double alpha_d = safe_div(S->rho[it & 1], r0v);
   if (br.vb == br.b0 && threadIdx.x == 0) S->alpha[it & 1] = alpha_d;
   const float alpha = (float)alpha_d;
@@ -809,6 +870,9 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_3(Seg sg, int it, float*
   const int n = sg.n, G = sg.G;
   // one row per thread: load it before the fold
   constexpr bool pf = PF;
+  const double* pp[3] = {PC + br.b0, PC + G + br.b0, PC + 2 * G + br.b0};
+  EarlyFold<3> ef;
+  if constexpr (pf) ef.load(pp, br.nf);
   const int i0 = br.begin + (int)threadIdx.x;
   const bool has = pf && i0 < br.end;
   float iv = 0.f, pv[NV], sv[NV], tv[NV], xv[NV], r0v[NV];
@@ -824,9 +888,11 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_3(Seg sg, int it, float*
       r0v[q] = r0[k];
     }
   }
-  const double* pp[3] = {PC + br.b0, PC + G + br.b0, PC + 2 * G + br.b0};
   double tv3[3];
-  fold_n<3>(pp, br.nf, tv3);
+  if constexpr (pf)
+    ef.finish(pp, br.nf, tv3);
+  else
+    fold_n<3>(pp, br.nf, tv3);
   const double ts = tv3[0], tt = tv3[1], ss = tv3[2];
   const double omega_d = ss > S->f.thr2 ? safe_div(ts, tt) : 0.0;
   if (br.vb == br.b0 && threadIdx.x == 0) {
@@ -1169,15 +1235,25 @@ static int cgs_impl(const M& A, const KLaunch& L, const float* b, float* x, cons
 #define WCFD_CGS_ITER(KERNEL)                                                             \
   WCFD_LAUNCH_B(K_CG_1, s, KERNEL, G, B, A, sg, it, x, rb[cur], rb[1 - cur], wb[cur],       \
                 wb[1 - cur], sb[cur], sb[1 - cur], w.p0, w.inv, pb[cur], pb[1 - cur], w.st)
+      const bool first = it == 0;
       if (rows == 1) {
-        WCFD_CGS_ITER((k_cgs_iter<M, true>));
-      } else if constexpr (M::kEll) {
-        if (rows == 2)
-          WCFD_CGS_ITER(k_cgs_iter2<M>);
+        if (first)
+          WCFD_CGS_ITER((k_cgs_iter<M, true, true>));
         else
-          WCFD_CGS_ITER((k_cgs_iter<M, false>));
+          WCFD_CGS_ITER((k_cgs_iter<M, true, false>));
+      } else if constexpr (M::kEll) {
+        if (rows == 2 && first)
+          WCFD_CGS_ITER((k_cgs_iter2<M, true>));
+        else if (rows == 2)
+          WCFD_CGS_ITER((k_cgs_iter2<M, false>));
+        else if (first)
+          WCFD_CGS_ITER((k_cgs_iter<M, false, true>));
+        else
+          WCFD_CGS_ITER((k_cgs_iter<M, false, false>));
+      } else if (first) {
+        WCFD_CGS_ITER((k_cgs_iter<M, false, true>));
       } else {
-        WCFD_CGS_ITER((k_cgs_iter<M, false>));
+        WCFD_CGS_ITER((k_cgs_iter<M, false, false>));
       }
 #undef WCFD_CGS_ITER
       prefold<3>(L, pb[1 - cur], w.st, s);
