@@ -129,14 +129,25 @@ def algorithmic_bytes(kernel, C, F, Fi, Fb, nnz, slots=6):
     return table.get(kernel)
 
 
-def cpu_baseline(nodes, cells, kind, budget_s=20.0, max_iters=6):
-    """CPU oracle (numpy restatement of the reference path), one thread."""
+def host_info():
+    """The host the CPU baselines ran on: usable cores and CPU model."""
     try:
-        from threadpoolctl import threadpool_limits
-        limiter = threadpool_limits(limits=1)
-    except Exception:
-        limiter = None
-    sys.path.insert(0, os.path.join(ROOT, 'tests'))
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    model = 'unknown'
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    model = line.split(':', 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {'nproc': cores, 'cpu_model': model}
+
+
+def _oracle_solver(nodes, cells, kind, cg_variant=1):
     from oracle.mesh import OracleMesh
     from oracle.fv import OracleFVM
     from oracle.simple import OracleSolver
@@ -147,8 +158,23 @@ def cpu_baseline(nodes, cells, kind, budget_s=20.0, max_iters=6):
         of.set_reference_pressure(0, 0.0)
     apply_bcs(of.boundary, kind)
     of.boundary.finalize()
-    s = OracleSolver(of, 0.7, 0.3, nu=0.01, implicit_outflow=(kind == 'channel'))
-    s.outer_iteration()                     # warm-up (untimed)
+    return OracleSolver(of, 0.7, 0.3, nu=0.01, implicit_outflow=(kind == 'channel'),
+                        cg_variant=cg_variant)
+
+
+def _cpu_run(nodes, cells, kind, warmup, budget_s, max_iters, cg_variant=1):
+    """The oracle on one thread: ``warmup`` untimed SIMPLE iterations (the GPU
+    run's warm-up), then timed iterations until ``max_iters`` or the budget.
+    Returns (iterations, seconds, momentum and pressure Krylov iterations per
+    timed step)."""
+    try:
+        from threadpoolctl import threadpool_limits
+        threadpool_limits(limits=1)
+    except Exception:
+        pass
+    s = _oracle_solver(nodes, cells, kind, cg_variant)
+    for _ in range(warmup):
+        s.outer_iteration()
     t0 = time.perf_counter()
     n = 0
     while n < max_iters:
@@ -157,17 +183,62 @@ def cpu_baseline(nodes, cells, kind, budget_s=20.0, max_iters=6):
         if time.perf_counter() - t0 > budget_s:
             break
     dt = time.perf_counter() - t0
-    if limiter is not None:
-        limiter.unregister() if hasattr(limiter, 'unregister') else None
-    return {'value': n / dt, 'unit': 'iter/s', 'cores': 1, 'kind': 'port',
-            'sample': f'{n} SIMPLE iterations (after 1 untimed) of the same workload, oracle '
-                      f'(numpy/scipy restatement of the reference path), 1 thread, {dt:.1f} s'}
+    hist = s.history[-n:]
+    u_it = sum(h['u_iters'] for h in hist) / n
+    p_it = sum(sum(h['p_iters']) for h in hist) / n
+    return n, dt, u_it, p_it
+
+
+def cpu_baseline(nodes, cells, kind, warmup=3, budget_s=15.0, max_iters=20, gpu_window=None,
+                 all_cores=True, cg_variant=1, all_cores_budget_s=10.0):
+    """The CPU oracle (numpy/scipy restatement of the reference path) on the
+    same workload: one thread over the GPU run's iteration window (the same
+    untimed warm-up, then as many of its timed iterations as the budget
+    allows), and -- the reference's Warp-CPU path and this restatement both run
+    one case on one thread -- all host cores as independent copies of the case
+    in one process each, whose summed iteration rate is the host's
+    throughput."""
+    n, dt, u_it, p_it = _cpu_run(nodes, cells, kind, warmup, budget_s, max_iters, cg_variant)
+    host = host_info()
+    out = {'value': round(n / dt, 5), 'unit': 'iter/s', 'cores': 1, 'kind': 'port',
+           'sample': f'SIMPLE iterations {warmup + 1}-{warmup + n} (after {warmup} untimed, as the '
+                     f'GPU run) of the same workload by the oracle (numpy/scipy restatement of the '
+                     f'reference path), 1 thread, {dt:.1f} s',
+           'krylov_iters_per_step': {'momentum': round(u_it, 1), 'pressure': round(p_it, 1)},
+           'host': host}
+    if gpu_window:
+        out['gpu_window'] = gpu_window
+    if all_cores and host['nproc'] > 1:
+        import concurrent.futures as cf
+        import multiprocessing as mp
+        P = min(host['nproc'], 16)
+        t0 = time.perf_counter()
+        with cf.ProcessPoolExecutor(max_workers=P, mp_context=mp.get_context('spawn')) as ex:
+            futs = [ex.submit(_cpu_run, nodes, cells, kind, warmup, all_cores_budget_s,
+                              max_iters, cg_variant) for _ in range(P)]
+            res = [f.result() for f in futs]
+        wall = time.perf_counter() - t0
+        rate = sum(r[0] / r[1] for r in res)
+        out['all_cores'] = {
+            'value': round(rate, 5), 'unit': 'iter/s', 'cores': P, 'kind': 'port',
+            'sample': f'{P} independent copies of the same case, one process and one thread each, '
+                      f'{warmup} untimed then up to {max_iters} timed SIMPLE iterations or '
+                      f'{all_cores_budget_s:g} s each ({wall:.1f} s wall); value = the sum of '
+                      f'their rates',
+            'per_process_iter_s': [round(r[0] / r[1], 5) for r in res]}
+    return out
 
 
 def bench_batch64(args):
     """The 64-cavity batch: every case to continuity <= 1e-6, batched in one
     arena, and the same 64 cases one after another through the plain
     single-case solver.  One step = one solve of the whole batch."""
+    print(json.dumps(run_batch64(args, sequential=not args.no_sequential,
+                                 profile=not args.no_profile, cpu=not args.no_cpu_baseline)))
+
+
+def run_batch64(args, sequential=True, profile=True, cpu=True):
+    """bench_batch64's measurement; returns the result dict."""
     import numpy as np
     import torch
     from warpcfd_amd import _lib
@@ -243,7 +314,7 @@ def bench_batch64(args):
           f'converged', file=sys.stderr, flush=True)
 
     roofline = kernels = None
-    if not args.no_profile:
+    if profile:
         prof_arena = CaseArena.for_cases(specs, **kw)
         with _lib.Profiler(capacity=400000) as prof:
             for _ in range(20):
@@ -255,11 +326,11 @@ def bench_batch64(args):
         single = prof_arena.cg_variant == _lib.CG_SINGLE_REDUCTION
         roofline, kernels = rooflines_from(
             prof.result, cells, F, Fi, F - Fi, nnz, single,
-            note='first 20 batch iterations, all 64 cases active')
+            note='first 20 batch iterations, all 64 cases active', skipped=prof.skipped)
         del prof_arena
 
     seq = None
-    if not args.no_sequential:
+    if sequential:
         seq_t, seq_it, dev, seq_conv = [], [], [], 0
         for k, (n, re) in enumerate(draws):
             model = cavity_model(n, re)
@@ -289,9 +360,9 @@ def bench_batch64(args):
                'max_rel_l2_velocity_vs_batched': float(np.max(dev))}
         print(f'# sequential: {tot:.2f} s', file=sys.stderr, flush=True)
 
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_solve_baseline(draws, stop, pol['rtol0'], stop_u)
+    cpu_b = None
+    if cpu:
+        cpu_b = cpu_solve_baseline(draws, stop, pol['rtol0'], stop_u)
 
     value = len(specs) / batched_s
     out = {
@@ -331,19 +402,50 @@ def bench_batch64(args):
         'speedup_vs_sequential': round(seq['total_s'] / batched_s, 2) if seq else None,
         'host_preprocessing_s': round(prep_s, 1),
         'roofline': roofline,
-        'cpu_baseline': cpu,
+        'cpu_baseline': cpu_b,
         'kernels': kernels,
     }
-    print(json.dumps(out))
+    return out
 
 
-def bench_serve(args):
-    """The served stream: Poisson arrivals (seeded, rate 20/s) of cavity
-    (N in [32,512], Re in [100,1000]) and channel (50k-500k cells, Re in
-    [50,400]) requests, each solved to continuity <= 1e-6.  Served with
-    continuous batching on an arena sized by an HBM budget, and for
-    comparison one at a time on the plain solver.  One step = the whole
-    stream."""
+SERVE_MIX = {'cavity_n': (32, 512), 'channel_cells': (50_000, 500_000)}
+SERVE_SAMPLE_MIX = {'cavity_n': (32, 96), 'channel_cells': (50_000, 150_000)}
+
+
+def _long_run_errors(sample, pol, chunk, block):
+    """Velocity error of served results against long runs of the same cases:
+    every sampled case runs again, all of them together in one arena, for a
+    fixed 4 x its served iterations (at least served + 3000, no stop), and
+    the served u, v, w are compared with that longer solve (rel-L2)."""
+    import copy
+    import numpy as np
+    from warpcfd_amd.batch import CaseArena
+    specs = []
+    for r in sample:
+        sp = copy.copy(r.spec)
+        sp.stop, sp.stop_u, sp.floor_u = 0.0, 0.0, 0.0     # never stop: run to max_outer
+        sp.max_outer = max(4 * r.iterations, r.iterations + 3000)
+        specs.append(sp)
+    arena = CaseArena.for_cases(specs, chunk=chunk, block=block, **pol)
+    arena.run(max_steps=max(sp.max_outer for sp in specs) + 1)
+    out = []
+    for k, r in enumerate(sample):
+        a = np.asarray(r.fields[:, :3], np.float64)
+        b = arena.cell_values(k)[:, :3].astype(np.float64)
+        out.append({'request': r.describe(), 'served_iterations': int(r.iterations),
+                    'long_run_iterations': int(specs[k].max_outer),
+                    'velocity_rel_l2': float(np.linalg.norm(a - b) / np.linalg.norm(b))})
+    return out
+
+
+def run_serve(args, n_requests, mix=SERVE_MIX, sequential=True, error_sample=0, label=''):
+    """Serves a Poisson stream (seeded, args.rate per second) of cavity and
+    channel requests drawn from ``mix``, each to continuity <= 1e-6 and a
+    convergence-monitor velocity-error estimate <= args.stop_u (the batch
+    stop; ``--serve-continuity-only`` drops the second test), with continuous
+    batching on an arena sized by an HBM budget and, if ``sequential``, one
+    at a time through the same kernels; ``error_sample`` served results are
+    checked against long runs.  Returns the result dict."""
     import math
     import numpy as np
     import torch
@@ -351,48 +453,90 @@ def bench_serve(args):
     from warpcfd_amd.batch import CaseArena
     from warpcfd_amd.serve import (ContinuousBatchServer, Preparer, SequentialServer,
                                    poisson_stream, summarize)
-    stop, stop_u = 1e-6, (args.stop_u if args.serve_stop_u else math.inf)
+    stop = 1e-6
+    stop_u = math.inf if args.serve_continuity_only else args.stop_u
+    floor_u = 0.0 if args.serve_continuity_only else 1e-7
     pol = dict(atol=0.0, tol=0.0, rtol0=args.rtol0, maxiter=2000, check_every=10,
                check_every_u=args.check_u, cg_variant=_lib.CG_SINGLE_REDUCTION)
     # preparation workers start before the first GPU call of this process
-    prep = Preparer(workers=args.workers, stop=stop, stop_u=stop_u,
-                    floor_u=1e-7 if args.serve_stop_u else 0.0, max_outer=args.max_outer)
+    prep = Preparer(workers=args.workers, stop=stop, stop_u=stop_u, floor_u=floor_u,
+                    max_outer=args.max_outer)
     prep.warm()
-    # warm-up: load every kernel once
-    from warpcfd_amd.batch import CaseSpec
-    from warpcfd_amd.workloads import cavity_model
-    warm = CaseArena.for_cases([CaseSpec(cavity_model(16, 100.0))], chunk=args.chunk,
-                               block=args.block, **pol)
-    for _ in range(3):
-        warm.step()
-    torch.cuda.synchronize()
-    del warm
-
-    stream = poisson_stream(args.seed, rate=args.rate, n_requests=args.requests)
-    cells = sum(r.cells for r in stream)
-    print(f'# serve: {len(stream)} requests, {cells} cells, arrivals over '
-          f'{stream[-1].arrival:.2f} s', file=sys.stderr, flush=True)
-    srv = ContinuousBatchServer(hbm_bytes=int(args.hbm_gb * (1 << 30)), max_cases=args.slots,
-                                chunk=args.chunk, block=args.block, **pol)
-    def report(r):
-        print(f'#   done {r.describe()}: {r.iterations} iterations, '
-              f'{r.t_done - r.arrival:.2f} s after arrival', file=sys.stderr, flush=True)
-
-    t0 = time.perf_counter()
-    done = srv.serve(stream, prep, clock_start=t0, on_done=report)
-    torch.cuda.synchronize()
-    wall = max(r.t_done for r in done)
-    served = summarize(done, wall)
-    print(f'# continuous batching: {served}', file=sys.stderr, flush=True)
-    seq = None
-    if not args.no_sequential:
-        stream2 = poisson_stream(args.seed, rate=args.rate, n_requests=args.requests)
-        t0 = time.perf_counter()
-        done2 = SequentialServer(chunk=args.chunk, block=args.block, **pol).serve(
-            stream2, prep, clock_start=t0, on_done=report)
+    try:
+        from warpcfd_amd.batch import CaseSpec
+        from warpcfd_amd.workloads import cavity_model
+        warm = CaseArena.for_cases([CaseSpec(cavity_model(16, 100.0))], chunk=args.chunk,
+                                   block=args.block, **pol)
+        for _ in range(3):
+            warm.step()
         torch.cuda.synchronize()
-        seq = summarize(done2, max(r.t_done for r in done2))
-        print(f'# sequential: {seq}', file=sys.stderr, flush=True)
+        del warm
+        stream = poisson_stream(args.seed, rate=args.rate, n_requests=n_requests, **mix)
+        cells = sum(r.cells for r in stream)
+        print(f'# serve{label}: {len(stream)} requests, {cells} cells, arrivals over '
+              f'{stream[-1].arrival:.2f} s', file=sys.stderr, flush=True)
+        srv = ContinuousBatchServer(hbm_bytes=int(args.hbm_gb * (1 << 30)), max_cases=args.slots,
+                                    chunk=args.chunk, block=args.block,
+                                    keep_fields=error_sample > 0, **pol)
+
+        def report(r):
+            print(f'#   done {r.describe()}: {r.iterations} iterations, '
+                  f'{r.t_done - r.arrival:.2f} s after arrival', file=sys.stderr, flush=True)
+
+        t0 = time.perf_counter()
+        done = srv.serve(stream, prep, clock_start=t0, on_done=report)
+        torch.cuda.synchronize()
+        served = summarize(done, max(r.t_done for r in done))
+        print(f'# continuous batching{label}: {served}', file=sys.stderr, flush=True)
+        seq = None
+        if sequential:
+            stream2 = poisson_stream(args.seed, rate=args.rate, n_requests=n_requests, **mix)
+            t0 = time.perf_counter()
+            done2 = SequentialServer(chunk=args.chunk, block=args.block, **pol).serve(
+                stream2, prep, clock_start=t0, on_done=report)
+            torch.cuda.synchronize()
+            seq = summarize(done2, max(r.t_done for r in done2))
+            print(f'# one at a time{label}: {seq}', file=sys.stderr, flush=True)
+        errors = None
+        if error_sample:
+            ok = [r for r in done if r.error is None and r.converged and r.cells <= 60_000]
+            rng = np.random.default_rng(args.seed + 1)
+            pick = sorted(rng.choice(len(ok), size=min(error_sample, len(ok)), replace=False))
+            errors = _long_run_errors([ok[i] for i in pick], pol, args.chunk, args.block)
+            print(f'# velocity error vs long runs{label}: {errors}', file=sys.stderr, flush=True)
+    finally:
+        prep.close()
+    stop_desc = f'continuity <= {stop:g}' + ('' if math.isinf(stop_u) else
+                                             f' and estimated velocity error <= {stop_u:g}')
+    return {
+        'workload': f'served stream: {len(stream)} Poisson arrivals (seed {args.seed}, '
+                    f'{args.rate:g}/s) of lid-driven cavities (N in [{mix["cavity_n"][0]},'
+                    f'{mix["cavity_n"][1]}], Re in [100,1000]) and 10x2x2 channels '
+                    f'({mix["channel_cells"][0] // 1000}k-{mix["channel_cells"][1] // 1000}k cells, '
+                    f'Re in [50,400]), steady SIMPLE to {stop_desc}',
+        'requests': len(stream), 'cells': cells, 'hbm_budget_gb': args.hbm_gb,
+        'slots': args.slots, 'host_workers': args.workers,
+        'krylov': {'atol': 0.0, 'tol': 0.0, 'rtol0': args.rtol0, 'maxiter': 2000,
+                   'check_every': 10, 'check_every_momentum': args.check_u,
+                   'pressure_cg': 'single'},
+        'served': served,
+        'one_at_a_time': seq,
+        'speedup_vs_one_at_a_time': (round(served['solves_per_s'] / seq['solves_per_s'], 2)
+                                     if seq else None),
+        'velocity_error_vs_long_runs': errors,
+        'latency_note': 'time to converged = arrival to convergence, including host '
+                        'preparation (mesh, layout; worker processes) and queueing; '
+                        'resident time = admission to convergence',
+    }
+
+
+def bench_serve(args):
+    """The served stream of the configured mix (cavities N in [32,512],
+    channels 50k-500k cells), ``--requests`` Poisson arrivals.  One step =
+    the whole stream."""
+    res = run_serve(args, args.requests, SERVE_MIX, sequential=not args.no_sequential,
+                    error_sample=args.error_sample)
+    served = res['served']
     out = {
         'metric': 'SIMPLE iter/s + %HBM roofline; served solves/s, p50/p99 at equal residual',
         'value': served['solves_per_s'],
@@ -400,40 +544,24 @@ def bench_serve(args):
         'n_gpus': 1,
         'steps': 1,
         'warmup': 0,
-        'ms_per_step': round(1e3 * wall, 1),
+        'ms_per_step': round(1e3 * served['wall_s'], 1),
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
         'dtype': 'f32',
         'data': 'synthetic',
-        'config': {'workload': f'served stream: {len(stream)} Poisson arrivals (seed {args.seed}, '
-                               f'{args.rate:g}/s) of lid-driven cavities (N in [32,512], Re in '
-                               f'[100,1000]) and 10x2x2 channels (50k-500k cells, Re in '
-                               f'[50,400]), steady SIMPLE to continuity <= {stop:g}'
-                               + ('' if math.isinf(stop_u) else
-                                  f' and estimated velocity error <= {stop_u:g}'),
-                   'requests': len(stream), 'cells': cells, 'hbm_budget_gb': args.hbm_gb,
-                   'slots': args.slots, 'host_workers': args.workers,
-                   'krylov': {'atol': 0.0, 'tol': 0.0, 'rtol0': args.rtol0, 'maxiter': 2000,
-                              'check_every': 10, 'check_every_momentum': args.check_u,
-                              'pressure_cg': 'single'}, 'reductions': 'f64'},
-        'served': served,
-        'sequential': seq,
-        'speedup_vs_sequential': (round(served['solves_per_s'] / seq['solves_per_s'], 2)
-                                  if seq else None),
-        'latency_note': 'time to converged = arrival to convergence, including host '
-                        'preparation (mesh, layout; worker processes) and queueing; '
-                        'resident time = admission to convergence',
+        'config': {'workload': res.pop('workload'), 'reductions': 'f64'},
         'roofline': None,
         'cpu_baseline': None,
     }
-    prep.close()
+    out.update(res)
     print(json.dumps(out))
 
 
-KERNELS_NOTE = ('avg_us and gbs of a Krylov kernel average over every launch, including the '
-                'launches that exit at entry once their solve has converged inside a check '
-                'group (up to check_every - 1 per solve)')
+KERNELS_NOTE = ('avg_us and gbs average over the launches that did work: a Krylov solve keeps '
+                'launching while the host has not yet seen its stop, and those launches exit at '
+                'entry; the library profiler retires them when the solve returns and they are '
+                'counted apart as exit_at_entry')
 
 
 def install_launch(spec):
@@ -446,11 +574,15 @@ def install_launch(spec):
     return {'grid': g, 'block': b, 'fold': fold, 'source': 'command line'}
 
 
-def rooflines_from(result, C, F, Fi, Fb, nnz, cg_single=True, note=None, slots=6):
+def rooflines_from(result, C, F, Fi, Fb, nnz, cg_single=True, note=None, slots=6, skipped=None):
     """(roofline of the dominant kernel, per-kernel table with each kernel's own
-    roofline) from the library profiler's {kernel: (total ms, launches)}."""
+    roofline) from the library profiler's {kernel: (total ms, launches)} over
+    the launches that did work; ``skipped`` {kernel: launches that exited at
+    entry} is reported beside them."""
     kernels = {k: {'total_ms': round(v[0], 3), 'launches': v[1], 'avg_us': round(1e3 * v[0] / v[1], 3)}
                for k, v in sorted(result.items(), key=lambda kv: -kv[1][0])}
+    for k, n in (skipped or {}).items():
+        kernels.setdefault(k, {'total_ms': 0.0, 'launches': 0, 'avg_us': 0.0})['exit_at_entry'] = n
     dominant = max(result.items(), key=lambda kv: kv[1][0])[0]
     roofline = None
     for k, v in kernels.items():
@@ -465,7 +597,9 @@ def rooflines_from(result, C, F, Fi, Fb, nnz, cg_single=True, note=None, slots=6
         if k == dominant:
             roofline = {'bound': 'hbm', 'kernel': k, 'achieved': round(gbs, 1),
                         'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(gbs / HBM_PEAK_GBS, 4),
-                        'traffic': None, 'bytes_per_launch': kb, 'avg_launch_us': v['avg_us']}
+                        'traffic': None, 'bytes_per_launch': kb, 'avg_launch_us': v['avg_us'],
+                        'launches': v['launches'],
+                        'exit_at_entry_launches_excluded': v.get('exit_at_entry', 0)}
             if note:
                 roofline['note'] = note
     return roofline, kernels
@@ -479,7 +613,7 @@ def profile_roofline(run, C, F, Fi, Fb, nnz, cg_single=True, capacity=400000, no
     with _lib.Profiler(capacity=capacity) as prof:
         run()
         torch.cuda.synchronize()
-    return rooflines_from(prof.result, C, F, Fi, Fb, nnz, cg_single, note)
+    return rooflines_from(prof.result, C, F, Fi, Fb, nnz, cg_single, note, skipped=prof.skipped)
 
 
 def bench_cylinder(args):
@@ -702,13 +836,18 @@ def main():
     ap.add_argument('--graph', action='store_true',
                     help='batch: replay a HIP-graph-captured iteration (Krylov launch budget '
                          'measured every 100 iterations)')
-    ap.add_argument('--requests', type=int, default=24, help='serve: requests in the stream')
+    ap.add_argument('--requests', type=int, default=200, help='serve: requests in the stream')
+    ap.add_argument('--error-sample', type=int, default=6,
+                    help='serve: served results checked against long runs')
     ap.add_argument('--rate', type=float, default=20.0, help='serve: arrivals per second')
     ap.add_argument('--hbm-gb', type=float, default=32.0, help='serve: arena HBM budget')
     ap.add_argument('--slots', type=int, default=64, help='serve: case slots')
     ap.add_argument('--workers', type=int, default=8, help='serve: host preparation threads')
-    ap.add_argument('--serve-stop-u', action='store_true',
-                    help='serve: also require the velocity-error estimate (as batch64)')
+    ap.add_argument('--serve-continuity-only', action='store_true',
+                    help='serve: stop at continuity <= 1e-6 alone (without the velocity-error '
+                         'estimate of the batch stop)')
+    ap.add_argument('--no-extras', action='store_true',
+                    help='channel: skip the batch64 and served-sample sub-benchmarks')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-profile', action='store_true')
     ap.add_argument('--no-tune', action='store_true')
@@ -799,7 +938,8 @@ def main():
                 solver.outer_iteration()
             torch.cuda.synchronize()
         roofline, kernels = rooflines_from(prof.result, C, F, Fi, Fb, nnz, args.cg == 'single',
-                                           slots=model.layout.faces_per_cell)
+                                           slots=model.layout.faces_per_cell,
+                                           skipped=prof.skipped)
 
     if args.calibration_copy_mb:
         calibration_copy(args.calibration_copy_mb)
@@ -819,8 +959,17 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(nodes, cells, wl['kind'])
+        cpu = cpu_baseline(nodes, cells, wl['kind'], warmup=args.warmup,
+                           cg_variant=1 if args.cg == 'single' else 0,
+                           gpu_window={'iterations': f'{args.warmup + 1}-{args.warmup + args.steps}',
+                                       'krylov_iters_per_step': {
+                                           'momentum': float(np.mean([h['u_iters'] for h in hist])),
+                                           'pressure': float(np.mean([h['p_iters'] for h in hist]))}})
 
+    extras = {}
+    if not args.no_extras and args.workload == 'channel' and dims == (200, 40, 40) \
+            and args.cell_type == 'hexahedron':
+        extras = channel_extras(args)
     out = {
         'metric': 'SIMPLE iter/s + %HBM roofline; served solves/s, p50/p99 at equal residual',
         'value': round(args.steps / elapsed, 3),
@@ -847,7 +996,40 @@ def main():
         'kernels': kernels,
         'kernels_note': KERNELS_NOTE,
     }
+    out.update(extras)
     print(json.dumps(out))
+
+
+def channel_extras(args):
+    """The other half of the metric, measured in the default run beside the
+    channel line: the 64-cavity batch solved batched (the configured batch,
+    timed as ``--workload batch64`` times it, without its one-after-another
+    leg), and a served sample -- 24 Poisson requests of a reduced mix
+    (cavities N in [32,96], channels 50k-150k cells) to the batch stop,
+    continuous batching against one at a time on the same sample, with a
+    velocity-error check of 4 results against long runs.  The configured
+    served stream (N up to 512, 200 requests) runs for minutes and is
+    measured by ``--workload serve``."""
+    out = {}
+    t = time.perf_counter()
+    try:
+        b = run_batch64(args, sequential=False, profile=False, cpu=False)
+        out['batch64'] = {'workload': b['config']['workload'], 'cases': b['config']['cases'],
+                          'cells': b['config']['cells'], 'solves_per_s': b['value'],
+                          'batched': b['batched'], 'arena_tuning': {
+                              k: b['arena_tuning'][k] for k in ('cg_variant', 'chunk', 'block')}
+                          if b['arena_tuning'] else None}
+    except Exception as exc:                # the channel line is printed either way
+        out['batch64'] = {'error': repr(exc)}
+    print(f'# batch64 sub-benchmark: {time.perf_counter() - t:.1f} s', file=sys.stderr, flush=True)
+    t = time.perf_counter()
+    try:
+        out['served_sample'] = run_serve(args, 24, SERVE_SAMPLE_MIX, sequential=True,
+                                         error_sample=4, label=' sample')
+    except Exception as exc:
+        out['served_sample'] = {'error': repr(exc)}
+    print(f'# served sample: {time.perf_counter() - t:.1f} s', file=sys.stderr, flush=True)
+    return out
 
 
 if __name__ == '__main__':

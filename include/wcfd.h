@@ -544,6 +544,12 @@ int wcfd_kernel_count(void);
 const char* wcfd_kernel_name(int id); /* [host] static string */
 int wcfd_profile_begin(int capacity);
 int wcfd_profile_end(double* total_ms, int32_t* counts, int n_ids); /* [host] arrays */
+/* Krylov solves keep launching while the host has not yet seen their stop;
+ * those launches exit at entry.  wcfd_profile_end counts only launches that
+ * did an iteration's work (a polled solve retires, when it returns, its
+ * launches at or beyond the iteration count it ran); this returns, per
+ * kernel id, how many launches of the last profiled region were left out. */
+int wcfd_profile_skipped(int32_t* counts, int n_ids); /* [host] array */
 
 #ifdef __cplusplus
 }

@@ -143,6 +143,7 @@ _SIGS = {
     'wcfd_kernel_name': (ctypes.c_char_p, [ctypes.c_int]),
     'wcfd_profile_begin': (ctypes.c_int, [ctypes.c_int]),
     'wcfd_profile_end': (ctypes.c_int, [ctypes.POINTER(c_d), ctypes.POINTER(c_i32), ctypes.c_int]),
+    'wcfd_profile_skipped': (ctypes.c_int, [ctypes.POINTER(c_i32), ctypes.c_int]),
     'wcfd_simple_iteration': (ctypes.c_int, [_PM, WcfdBC, ctypes.POINTER(SimpleParams),
                                              ctypes.POINTER(SimpleState), c_p,
                                              ctypes.POINTER(SimpleInfo), c_p]),
@@ -210,6 +211,7 @@ class Profiler:
     def __init__(self, capacity=65536):
         self.capacity = capacity
         self.result = None
+        self.skipped = {}
 
     def __enter__(self):
         check(load().wcfd_profile_begin(self.capacity), 'profile_begin')
@@ -221,5 +223,9 @@ class Profiler:
         ms = (c_d * n)()
         cnt = (c_i32 * n)()
         check(lib.wcfd_profile_end(ms, cnt, n), 'profile_end')
+        skipped = (c_i32 * n)()
+        check(lib.wcfd_profile_skipped(skipped, n), 'profile_skipped')
         self.result = {lib.wcfd_kernel_name(i).decode(): (ms[i], cnt[i]) for i in range(n) if cnt[i]}
+        # launches that exited at entry (their solve had already stopped), left out above
+        self.skipped = {lib.wcfd_kernel_name(i).decode(): skipped[i] for i in range(n) if skipped[i]}
         return False

@@ -64,6 +64,13 @@ bool prof_active();
 // a start/stop event pair for one launch of kernel `id` (false: not
 // profiling, or the event pool is used up)
 bool prof_events(int id, hipEvent_t* start, hipEvent_t* stop);
+// Krylov iteration index attached to the launches that follow on this host
+// thread (-1: not an iteration launch)
+void prof_set_iter(int it);
+// position in the launch records, and retirement of the records from there
+// on whose iteration index is >= iters_run (launches that exited at entry)
+size_t prof_mark();
+void prof_retire(size_t from, int iters_run);
 
 // launch KERNEL<<<GRID, BLOCK, 0, S>>>(args...), timed when profiling
 #define WCFD_LAUNCH_B(ID, S, KERNEL, GRID, BLOCK, ...)                                   \

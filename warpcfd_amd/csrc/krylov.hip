@@ -1080,6 +1080,7 @@ static int run_groups(const Seg& sg, int B, KState* st, RrPart&& rr_part, int ma
   }
   HostSync* hs = host_sync(K);
   if (!hs) return WCFD_EHIP;
+  const size_t prof_from = prof_mark();
   int e;
   // No round trip after init: the first group is launched at once (its
   // kernels exit at entry for cases the init fold already marked done) and
@@ -1110,6 +1111,11 @@ static int run_groups(const Seg& sg, int B, KState* st, RrPart&& rr_part, int ma
   if (launched > polled) {
     if ((e = check_hip(hipEventRecord(hs->tail, s)))) return e;
     hs->tail_pending = true;
+  }
+  if (last) {
+    int ran = 0;
+    for (int k = 0; k < K; ++k) ran = last[k].iters > ran ? last[k].iters : ran;
+    prof_retire(prof_from, ran);
   }
   return finish(last, K, info);
 }
@@ -1185,6 +1191,7 @@ static int cg_impl(const M& A, const KLaunch& L, const float* b, float* x, const
   const int ce = check_every > 0 ? check_every : 1;
   auto group = [&](int it0) {
     for (int it = it0; it < it0 + ce; ++it) {
+      prof_set_iter(it);
       const int cur = it & 1;
       if (pf)
         WCFD_LAUNCH_B(K_CG_1, s, (k_cg_1<M, true>), G, B, A, sg, it, w.r, w.inv, pb[cur],
@@ -1201,6 +1208,7 @@ static int cg_impl(const M& A, const KLaunch& L, const float* b, float* x, const
                       w.PA, w.st);
       prefold<2>(L, w.PA, w.st, s);
     }
+    prof_set_iter(-1);
   };
   auto rr_part = [&](int) { return (const double*)(w.PA + G); };
   return run_groups(sg, B, w.st, rr_part, maxiter, ce, tl.fixed_groups, group, info, s);
@@ -1231,6 +1239,7 @@ static int cgs_impl(const M& A, const KLaunch& L, const float* b, float* x, cons
   const int ce = check_every > 0 ? check_every : 1;
   auto group = [&](int it0) {
     for (int it = it0; it < it0 + ce; ++it) {
+      prof_set_iter(it);
       const int cur = it & 1;
 #define WCFD_CGS_ITER(KERNEL)                                                             \
   WCFD_LAUNCH_B(K_CG_1, s, KERNEL, G, B, A, sg, it, x, rb[cur], rb[1 - cur], wb[cur],       \
@@ -1258,6 +1267,7 @@ static int cgs_impl(const M& A, const KLaunch& L, const float* b, float* x, cons
 #undef WCFD_CGS_ITER
       prefold<3>(L, pb[1 - cur], w.st, s);
     }
+    prof_set_iter(-1);
   };
   // the r.r partials of the last iteration (it_end - 1) are in pb[it_end & 1], slot 2
   auto rr_part = [&](int it_end) { return (const double*)(pb[it_end & 1] + 2 * G); };
@@ -1283,6 +1293,7 @@ static int bicg_impl(const M& A, const KLaunch& L, const float* b, float* x, con
   const int ce = check_every > 0 ? check_every : 1;
   auto group = [&](int it0) {
     for (int it = it0; it < it0 + ce; ++it) {
+      prof_set_iter(it);
       const int cur = it & 1;
       if (pf)
         WCFD_LAUNCH_B(K_BI_1, s, (k_bi_1<M, NV, true>), G, B, A, sg, it, w.r, w.inv, pb[cur],
@@ -1306,6 +1317,7 @@ static int bicg_impl(const M& A, const KLaunch& L, const float* b, float* x, con
                       w.s, w.t, w.r0, w.PC, w.PA, w.st);
       prefold<2>(L, w.PA, w.st, s);
     }
+    prof_set_iter(-1);
   };
   auto rr_part = [&](int) { return (const double*)(w.PA + G); };
   return run_groups(sg, B, w.st, rr_part, maxiter, ce, tl.fixed_groups, group, info, s);

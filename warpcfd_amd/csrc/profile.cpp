@@ -2,6 +2,12 @@
 // library; guarded by a mutex, off unless wcfd_profile_begin() is called).
 // The events of a launch are stamped by the runtime with the kernel's own
 // begin and end (hipExtLaunchKernelGGL in WCFD_LAUNCH_B).
+//
+// Krylov solves keep launching while the host has not yet seen their stop
+// (two check groups in flight), and those launches exit at entry.  Each
+// launch of a Krylov iteration carries its iteration index; when the solve
+// returns, launches at or beyond the iteration count it ran are retired, so
+// the totals cover only launches that did an iteration's work.
 #include <hip/hip_runtime.h>
 
 #include <atomic>
@@ -15,8 +21,12 @@ namespace wcfd {
 namespace {
 struct Rec {
   int id;
+  int iter;     // Krylov iteration of the launch, -1 for any other launch
+  bool dead;    // exited at entry: its solve had stopped before this iteration
   hipEvent_t a, b;
 };
+thread_local int t_iter = -1;
+int32_t g_skipped[K_COUNT] = {};
 std::mutex g_mu;
 std::atomic<bool> g_on{false};
 std::vector<hipEvent_t> g_pool;
@@ -34,12 +44,27 @@ bool prof_active() { return g_on.load(std::memory_order_relaxed); }
 bool prof_events(int id, hipEvent_t* start, hipEvent_t* stop) {
   std::lock_guard<std::mutex> lk(g_mu);
   if (!g_on || g_next + 2 > g_pool.size()) return false;
-  Rec r{id, g_pool[g_next], g_pool[g_next + 1]};
+  Rec r{id, t_iter, false, g_pool[g_next], g_pool[g_next + 1]};
   g_next += 2;
   g_recs.push_back(r);
   *start = r.a;
   *stop = r.b;
   return true;
+}
+
+void prof_set_iter(int it) { t_iter = it; }
+
+size_t prof_mark() {
+  if (!prof_active()) return 0;
+  std::lock_guard<std::mutex> lk(g_mu);
+  return g_recs.size();
+}
+
+void prof_retire(size_t from, int iters_run) {
+  if (!prof_active()) return;
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (size_t i = from; i < g_recs.size(); ++i)
+    if (g_recs[i].iter >= iters_run) g_recs[i].dead = true;
 }
 
 }  // namespace wcfd
@@ -73,8 +98,12 @@ int wcfd_profile_end(double* total_ms, int32_t* counts, int n_ids) {
   std::lock_guard<std::mutex> lk(g_mu);
   g_on = false;
   if (n_ids < K_COUNT || !total_ms || !counts) return WCFD_EINVAL;
-  for (int i = 0; i < K_COUNT; ++i) { total_ms[i] = 0.0; counts[i] = 0; }
+  for (int i = 0; i < K_COUNT; ++i) { total_ms[i] = 0.0; counts[i] = 0; g_skipped[i] = 0; }
   for (auto& r : g_recs) {
+    if (r.dead) {
+      g_skipped[r.id] += 1;
+      continue;
+    }
     int rc = check_hip(hipEventSynchronize(r.b));
     if (rc) return rc;
     float ms = 0.f;
@@ -87,6 +116,13 @@ int wcfd_profile_end(double* total_ms, int32_t* counts, int n_ids) {
   g_pool.clear();
   g_recs.clear();
   g_next = 0;
+  return WCFD_OK;
+}
+
+int wcfd_profile_skipped(int32_t* counts, int n_ids) {
+  if (n_ids < K_COUNT || !counts) return WCFD_EINVAL;
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (int i = 0; i < K_COUNT; ++i) counts[i] = g_skipped[i];
   return WCFD_OK;
 }
 
