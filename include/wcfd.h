@@ -39,6 +39,18 @@
 extern "C" {
 #endif
 
+/* Field type of the library build: libwcfd.so computes in float (the
+ * reference's default float_dtype=wp.float32, FV/model.py:55); libwcfd64.so is
+ * the same source built with -DWCFD_REAL=double (float_dtype=wp.float64).
+ * Every field, geometry and matrix array is wcfd_real; scalar parameters
+ * (viscosity, relaxation, dt, scales) are float in both builds; reductions
+ * are double in both.  wcfd_real_bytes() tells a caller which build it
+ * loaded. */
+#ifndef WCFD_REAL
+#define WCFD_REAL float
+#endif
+typedef WCFD_REAL wcfd_real;
+
 #define WCFD_OK 0
 #define WCFD_EINVAL 1
 #define WCFD_EHIP 2
@@ -74,16 +86,16 @@ typedef struct wcfd_mesh {
                            value makes every entry point return WCFD_EINVAL. */
   const int32_t* owner;     /* [F]    owner cell (the reference's owner) */
   const int32_t* neigh;     /* [Fi]   neighbour cell */
-  const float* psi;         /* [Fi]   |x_f - x_O| / |x_O - x_N| (mesh_structs.py:206) */
-  const float* skew;        /* [3][Fi] x_f - (psi x_O + (1-psi) x_N) */
-  const float* area;        /* [F]    owner-side face area (mesh_structs.py:200) */
-  const float* normal;      /* [3][F] owner-side normal */
-  const float* dist;        /* [3][Fi] x_N - x_O */
-  const float* dmag;        /* [F]    |x_N - x_O| (internal), |x_f - x_P| (boundary) */
-  const float* volume;      /* [C] */
+  const wcfd_real* psi;         /* [Fi]   |x_f - x_O| / |x_O - x_N| (mesh_structs.py:206) */
+  const wcfd_real* skew;        /* [3][Fi] x_f - (psi x_O + (1-psi) x_N) */
+  const wcfd_real* area;        /* [F]    owner-side face area (mesh_structs.py:200) */
+  const wcfd_real* normal;      /* [3][F] owner-side normal */
+  const wcfd_real* dist;        /* [3][Fi] x_N - x_O */
+  const wcfd_real* dmag;        /* [F]    |x_N - x_O| (internal), |x_f - x_P| (boundary) */
+  const wcfd_real* volume;      /* [C] */
   const int32_t* slot_face; /* [6][C] face of local face k of cell c */
   const int8_t* slot_side;  /* [6][C] 0 if c owns that face, 1 otherwise */
-  const float* slot_sn;     /* [3][6][C] A_f * n_{c,k} (the cell's own normal) */
+  const wcfd_real* slot_sn;     /* [3][6][C] A_f * n_{c,k} (the cell's own normal) */
   const int32_t* slot_pos;  /* [6][C] CSR index of (c, neighbour) or -1 */
   const int32_t* diag_pos;  /* [C]    CSR index of (c, c) */
   const int32_t* row_ptr;   /* [C+1] */
@@ -101,7 +113,7 @@ typedef struct wcfd_mesh {
 /* Boundary conditions per boundary face (warp_cfd/FV/boundary/boundary.py). */
 typedef struct wcfd_bc {
   const uint8_t* type;  /* [4][Fb] 1 Dirichlet, 2 Neumann */
-  const float* value;   /* [4][Fb] value or normal gradient */
+  const wcfd_real* value;   /* [4][Fb] value or normal gradient */
 } wcfd_bc;
 
 /* Batch of independent cases on one mesh descriptor (no counterpart in the
@@ -128,9 +140,9 @@ typedef struct wcfd_batch {
   const int32_t* cell_case; /* [C] case of each cell, -1 if none */
   const int32_t* face_case; /* [F] case of each face, -1 if none */
   uint8_t* active;          /* [K] 1 = iterate the case */
-  const float* nu;          /* [K] kinematic viscosity per case */
+  const wcfd_real* nu;          /* [K] kinematic viscosity per case */
   const int32_t* ref_cell;  /* [K] reference-pressure row per case, -1 none */
-  const float* ref_value;   /* [K] */
+  const wcfd_real* ref_value;   /* [K] */
   const double* stop;       /* [K][3] stop_c, stop_u, floor_u of wcfd_monitor_update;
                                stop_c = 0: never stop */
   double* monitor;          /* [K][WCFD_MONITOR_LEN] monitor records, zeroed at the start */
@@ -167,13 +179,14 @@ int wcfd_monitor_update(double* record, double continuity, double velocity_chang
 int wcfd_last_hip_error(void);
 const char* wcfd_error_string(int code); /* [host] static string */
 int wcfd_abi_version(void);
+int wcfd_real_bytes(void);  /* sizeof(wcfd_real): 4 (libwcfd.so) or 8 (libwcfd64.so) */
 
 /* ------------------------------------------------------------ face/cell ops */
 
 /* FVM.set_boundary_conditions (warp_cfd/FV/model.py:206-208,
  * FV/boundary/conditions.py:6-19): Dirichlet face values for the fields in
  * field_mask (bit q = field q). */
-int wcfd_set_boundary_conditions(const wcfd_mesh* m, wcfd_bc bc, float* face_values,
+int wcfd_set_boundary_conditions(const wcfd_mesh* m, wcfd_bc bc, wcfd_real* face_values,
                                  int field_mask, void* stream);
 
 /* FVM.face_interpolation (model.py:210-229; interpolation_kernels.py:5-54;
@@ -184,23 +197,23 @@ int wcfd_set_boundary_conditions(const wcfd_mesh* m, wcfd_bc bc, float* face_val
  * face flux (u_f . n_O) A_f is written too (model_kernels.py:46-60); if
  * rua_cell != NULL the face value of rUA (solver_kernels.py:22-36) is
  * written to rua_face. */
-int wcfd_face_interpolation(const wcfd_mesh* m, wcfd_bc bc, const float* cell_values,
-                            const float* cell_grad, float* face_values, int field_mask,
-                            int skew_mask, float* mass_flux, const float* rua_cell,
-                            float* rua_face, void* stream);
+int wcfd_face_interpolation(const wcfd_mesh* m, wcfd_bc bc, const wcfd_real* cell_values,
+                            const wcfd_real* cell_grad, wcfd_real* face_values, int field_mask,
+                            int skew_mask, wcfd_real* mass_flux, const wcfd_real* rua_cell,
+                            wcfd_real* rua_face, void* stream);
 
 /* FVM.calculate_gradients (model.py:232-239; model_kernels.py:72-90):
  * Green-Gauss gradient of the fields in field_mask; when zero_others != 0 the
  * gradients of the other fields are zeroed, as the reference's zero_() does. */
-int wcfd_green_gauss(const wcfd_mesh* m, const float* face_values, float* cell_grad,
+int wcfd_green_gauss(const wcfd_mesh* m, const wcfd_real* face_values, wcfd_real* cell_grad,
                      int field_mask, int zero_others, void* stream);
 
 /* FVM.calculate_mass_flux (model.py:264-266; model_kernels.py:46-60). */
-int wcfd_mass_flux(const wcfd_mesh* m, const float* face_values, float* mass_flux,
+int wcfd_mass_flux(const wcfd_mesh* m, const wcfd_real* face_values, wcfd_real* mass_flux,
                    void* stream);
 
 /* FVM.divFlux (model.py:269-275; model_kernels.py:27-34). */
-int wcfd_divergence(const wcfd_mesh* m, const float* mass_flux, float* div, void* stream);
+int wcfd_divergence(const wcfd_mesh* m, const wcfd_real* mass_flux, wcfd_real* div, void* stream);
 
 /* Momentum system, fused: ConvectionTerm upwind (convection.py:44-88,
  * convectionScheme.py:28-52) + DiffusionTerm (diffusion.py:113-197,
@@ -212,11 +225,11 @@ int wcfd_divergence(const wcfd_mesh* m, const float* mass_flux, float* div, void
  * If compute_gradients != 0 the Green-Gauss gradients of all 4 fields are
  * computed first (model_kernels.py:72-90) into cell_grad and used for the p
  * term.  scheme_flags: WCFD_SCHEME_* bits.  values: [7][C] ELL. */
-int wcfd_assemble_momentum(const wcfd_mesh* m, wcfd_bc bc, const float* cell_values,
-                           const float* face_values, const float* mass_flux,
-                           float* cell_grad, int compute_gradients, float nu,
+int wcfd_assemble_momentum(const wcfd_mesh* m, wcfd_bc bc, const wcfd_real* cell_values,
+                           const wcfd_real* face_values, const wcfd_real* mass_flux,
+                           wcfd_real* cell_grad, int compute_gradients, float nu,
                            float inv_density, float alpha_u, float dt, int scheme_flags,
-                           float* values, float* rhs, void* stream);
+                           wcfd_real* values, wcfd_real* rhs, void* stream);
 
 /* Pressure equation, fused: divFlux (model_kernels.py:27-34) + DiffusionTerm
  * with per-face viscosity rUA_f (diffusion.py:40-49) -> form_system + add_RHS
@@ -224,10 +237,10 @@ int wcfd_assemble_momentum(const wcfd_mesh* m, wcfd_bc bc, const float* cell_val
  * when ref_cell >= 0.  If compute_grad_p != 0 the Green-Gauss gradient of p
  * is computed into cell_grad (other gradients untouched).  div_out may be
  * NULL.  values: [7][C] ELL. */
-int wcfd_assemble_pressure(const wcfd_mesh* m, wcfd_bc bc, const float* face_values,
-                           const float* mass_flux, const float* rua_face, float* cell_grad,
+int wcfd_assemble_pressure(const wcfd_mesh* m, wcfd_bc bc, const wcfd_real* face_values,
+                           const wcfd_real* mass_flux, const wcfd_real* rua_face, wcfd_real* cell_grad,
                            int compute_grad_p, int correction, int ref_cell, float ref_value,
-                           float* values, float* rhs, float* div_out, void* stream);
+                           wcfd_real* values, wcfd_real* rhs, wcfd_real* div_out, void* stream);
 
 /* ------------------------------------------------- composable terms
  * The reference's term-by-term API (the FV/terms package), for callers that build
@@ -249,10 +262,10 @@ int wcfd_assemble_pressure(const wcfd_mesh* m, wcfd_bc bc, const float* face_val
  * upwindLinear reads when the neighbour is upwind, convectionScheme.py:82);
  * required for WCFD_CONV_UPWIND_LINEAR only.  flags: WCFD_SCHEME_IMPLICIT_
  * OUTFLOW. */
-int wcfd_convection_weights(const wcfd_mesh* m, wcfd_bc bc, const float* cell_grad,
-                            const float* face_values, const float* mass_flux,
-                            const float* face_c2f, int scheme, int flags, const int32_t* outputs,
-                            int n_outputs, float* weights, void* stream);
+int wcfd_convection_weights(const wcfd_mesh* m, wcfd_bc bc, const wcfd_real* cell_grad,
+                            const wcfd_real* face_values, const wcfd_real* mass_flux,
+                            const wcfd_real* face_c2f, int scheme, int flags, const int32_t* outputs,
+                            int n_outputs, wcfd_real* weights, void* stream);
 
 /* DiffusionTerm.calculate_weights (diffusion.py:40-49, 52-197; laplacianScheme.py
  * central_difference / central_difference_corrected).  viscosity_faces: [F]
@@ -260,19 +273,19 @@ int wcfd_convection_weights(const wcfd_mesh* m, wcfd_bc bc, const float* cell_gr
  * boundary conditions; 1: the von_neumann= (and dirchlet=, which the
  * reference builds with the same kernel) option, explicit nu * bc_value * A
  * on every boundary face, one output only. */
-int wcfd_diffusion_weights(const wcfd_mesh* m, wcfd_bc bc, const float* cell_grad,
-                           const float* face_values, const float* viscosity_faces,
+int wcfd_diffusion_weights(const wcfd_mesh* m, wcfd_bc bc, const wcfd_real* cell_grad,
+                           const wcfd_real* face_values, const wcfd_real* viscosity_faces,
                            float viscosity, int correction, int bc_mode, float bc_value,
-                           const int32_t* outputs, int n_outputs, float* weights, void* stream);
+                           const int32_t* outputs, int n_outputs, wcfd_real* weights, void* stream);
 
 /* DdtTerm.calculate_weights, backwards_Euler (ddt.py:18-33): (V/dt, -phi V/dt)
  * per output, indexed by local output. */
-int wcfd_ddt_weights(const wcfd_mesh* m, const float* cell_values, float dt,
-                     const int32_t* outputs, int n_outputs, float* weights, void* stream);
+int wcfd_ddt_weights(const wcfd_mesh* m, const wcfd_real* cell_values, float dt,
+                     const int32_t* outputs, int n_outputs, wcfd_real* weights, void* stream);
 
 /* GradTerm.calculate_weights (grad.py:17-28): w[d][c] = (grad phi_field)_c[d] V_c,
  * an explicit term (component-major [3][C]). */
-int wcfd_grad_weights(const wcfd_mesh* m, const float* cell_grad, int field, float* weights,
+int wcfd_grad_weights(const wcfd_mesh* m, const wcfd_real* cell_grad, int field, wcfd_real* weights,
                       void* stream);
 
 /* Equation of n_outputs outputs: values[O][7][C] (one ELL plane per output:
@@ -281,36 +294,36 @@ int wcfd_grad_weights(const wcfd_mesh* m, const float* cell_grad, int field, flo
  * calculate_RHS_values_kernel (equation_kernels.py:52-101) for a face-based
  * term, add_Implicit_cell_based_term_kernel (104-122) for a cell-based one,
  * rhs += scale arr for an explicit term or add_RHS (equation.py:169-170). */
-int wcfd_equation_add_face_term(const wcfd_mesh* m, const float* weights, int n_outputs,
-                                float scale, float* values, float* rhs, void* stream);
-int wcfd_equation_add_cell_term(const wcfd_mesh* m, const float* weights, int n_outputs,
-                                float scale, float* values, float* rhs, void* stream);
-int wcfd_equation_add_rhs(int32_t n, const float* arr, float scale, float* rhs, void* stream);
+int wcfd_equation_add_face_term(const wcfd_mesh* m, const wcfd_real* weights, int n_outputs,
+                                float scale, wcfd_real* values, wcfd_real* rhs, void* stream);
+int wcfd_equation_add_cell_term(const wcfd_mesh* m, const wcfd_real* weights, int n_outputs,
+                                float scale, wcfd_real* values, wcfd_real* rhs, void* stream);
+int wcfd_equation_add_rhs(int32_t n, const wcfd_real* arr, float scale, wcfd_real* rhs, void* stream);
 /* Equation.relax (equation.py:173-181; implicit_relaxation_kernel,
  * equation_kernels.py:128-152) with the outputs' current cell values. */
-int wcfd_equation_relax(const wcfd_mesh* m, const float* cell_values, const int32_t* outputs,
-                        int n_outputs, float alpha, float* values, float* rhs, void* stream);
+int wcfd_equation_relax(const wcfd_mesh* m, const wcfd_real* cell_values, const int32_t* outputs,
+                        int n_outputs, float alpha, wcfd_real* values, wcfd_real* rhs, void* stream);
 /* Equation.replace_row (equation.py:184-193): identity row of (output, cell)
  * with right-hand side `value` (the reference reads it from a length-1 array
  * at index row_id, T7). */
 int wcfd_equation_replace_row(const wcfd_mesh* m, int n_outputs, int output, int32_t cell,
-                              float value, float* values, float* rhs, void* stream);
+                              float value, wcfd_real* values, wcfd_real* rhs, void* stream);
 
 /* ELL values [7][C] -> CSR values [nnz] on (row_ptr, col). */
-int wcfd_ell_to_csr(const wcfd_mesh* m, const float* ell_values, float* csr_values, void* stream);
+int wcfd_ell_to_csr(const wcfd_mesh* m, const wcfd_real* ell_values, wcfd_real* csr_values, void* stream);
 
 /* calculate_rUA + get_HbyA + replace_cell_values (solver_kernels.py:7-16,
  * 39-43; model_kernels.py:104-109): rUA = V / (a_PP rho), HbyA += rUA grad p,
  * cell_values[u,v,w] = HbyA.  HbyA is [3][C]; values is the momentum ELL
  * matrix (its diagonal a_PP). */
-int wcfd_rua_hbya(const wcfd_mesh* m, const float* values, float density,
-                  const float* cell_grad, float* hbya, float* cell_values, float* rua,
+int wcfd_rua_hbya(const wcfd_mesh* m, const wcfd_real* values, float density,
+                  const wcfd_real* cell_grad, wcfd_real* hbya, wcfd_real* cell_values, wcfd_real* rua,
                   void* stream);
 
 /* FVM.relax (model.py:256-262,381-384): p = alpha p_new + (1-alpha) p;
  * alpha <= 0 means replace (model.py:147). */
-int wcfd_relax_pressure(const wcfd_mesh* m, const float* p_new, float alpha,
-                        float* cell_values, void* stream);
+int wcfd_relax_pressure(const wcfd_mesh* m, const wcfd_real* p_new, float alpha,
+                        wcfd_real* cell_values, void* stream);
 
 /* Corrector tail (incompressible.py:150-156): face_interpolation('p'),
  * calculate_gradients('p') (zeroes u,v,w gradients, T3), HbyA -= rUA grad p,
@@ -320,9 +333,9 @@ int wcfd_relax_pressure(const wcfd_mesh* m, const float* p_new, float alpha,
  * (numerator, denominator) if resid != NULL; resid[2..3] are set to 0
  * (the SIMPLE step writes ||u^n - u^(n-1)||^2, ||u^n||^2 there).  Needs a
  * workspace of wcfd_reduce_workspace_bytes(F) bytes. */
-int wcfd_correct_velocity(const wcfd_mesh* m, wcfd_bc bc, float* cell_values,
-                          float* cell_grad, float* face_values, const float* mass_flux,
-                          const float* rua, const float* rua_face, float* hbya,
+int wcfd_correct_velocity(const wcfd_mesh* m, wcfd_bc bc, wcfd_real* cell_values,
+                          wcfd_real* cell_grad, wcfd_real* face_values, const wcfd_real* mass_flux,
+                          const wcfd_real* rua, const wcfd_real* rua_face, wcfd_real* hbya,
                           double* resid, void* workspace, void* stream);
 
 size_t wcfd_reduce_workspace_bytes(int32_t n_faces);
@@ -341,9 +354,9 @@ size_t wcfd_reduce_workspace_bytes(int32_t n_faces);
  *             local_continuity).
  * A NULL group (values, corr, p, flux) leaves its outputs 0.  Deterministic
  * (fixed reduction order).  workspace: wcfd_convergence_workspace_bytes(). */
-int wcfd_convergence_report(const wcfd_mesh* m, const float* values, const float* u,
-                            const float* b, const float* rua, const float* corr, const float* p,
-                            const float* flux, void* workspace, double* out, void* stream);
+int wcfd_convergence_report(const wcfd_mesh* m, const wcfd_real* values, const wcfd_real* u,
+                            const wcfd_real* b, const wcfd_real* rua, const wcfd_real* corr, const wcfd_real* p,
+                            const wcfd_real* flux, void* workspace, double* out, void* stream);
 size_t wcfd_convergence_workspace_bytes(void);
 
 /* The same report for every case of a batch arena (the per-case form of
@@ -351,9 +364,9 @@ size_t wcfd_convergence_workspace_bytes(void);
  * own workgroups (bt's cell and workgroup ranges; finished cases, which have
  * none, report 0), so a case's report does not depend on the other cases.
  * workspace: wcfd_batch_convergence_workspace_bytes(bt->n_blocks). */
-int wcfd_batch_convergence_report(const wcfd_mesh* m, const wcfd_batch* bt, const float* values,
-                                  const float* u, const float* b, const float* rua,
-                                  const float* corr, const float* p, const float* flux,
+int wcfd_batch_convergence_report(const wcfd_mesh* m, const wcfd_batch* bt, const wcfd_real* values,
+                                  const wcfd_real* u, const wcfd_real* b, const wcfd_real* rua,
+                                  const wcfd_real* corr, const wcfd_real* p, const wcfd_real* flux,
                                   void* workspace, double* out, void* stream);
 size_t wcfd_batch_convergence_workspace_bytes(int32_t n_blocks);
 
@@ -370,23 +383,23 @@ size_t wcfd_batch_convergence_workspace_bytes(int32_t n_blocks);
  * streams or host threads) race and must be serialised by the caller. */
 #define WCFD_NOALLOC_MAX_BLOCKS 16384
 int wcfd_batch_convergence_report_noalloc(const wcfd_mesh* m, const wcfd_batch* bt,
-                                          const float* values, const float* u, const float* b,
-                                          const float* rua, const float* corr, const float* p,
-                                          const float* flux, double* out, void* stream);
+                                          const wcfd_real* values, const wcfd_real* u, const wcfd_real* b,
+                                          const wcfd_real* rua, const wcfd_real* corr, const wcfd_real* p,
+                                          const wcfd_real* flux, double* out, void* stream);
 
 /* ------------------------------------------------------------------ Krylov */
 
 /* y[q] = A x[q] for q < nvec, CSR (row_ptr, col, values) of size n. */
-int wcfd_spmv(int32_t n, const int32_t* row_ptr, const int32_t* col, const float* values,
-              int nvec, const float* x, float* y, void* stream);
+int wcfd_spmv(int32_t n, const int32_t* row_ptr, const int32_t* col, const wcfd_real* values,
+              int nvec, const wcfd_real* x, wcfd_real* y, void* stream);
 
 /* float64 dot product of two float32 vectors, result to out[0] (device).
  * Workspace: wcfd_reduce_workspace_bytes(0) bytes. */
-int wcfd_dot_f64(int32_t n, const float* a, const float* b, double* out, void* workspace,
+int wcfd_dot_f64(int32_t n, const wcfd_real* a, const wcfd_real* b, double* out, void* workspace,
                  void* stream);
 
 /* y[q] = A x[q] for the ELL matrix `values` [7][C] of mesh m. */
-int wcfd_spmv_ell(const wcfd_mesh* m, const float* values, int nvec, const float* x, float* y,
+int wcfd_spmv_ell(const wcfd_mesh* m, const wcfd_real* values, int nvec, const wcfd_real* x, wcfd_real* y,
                   void* stream);
 
 size_t wcfd_krylov_workspace_bytes(int32_t n, int nvec);
@@ -404,8 +417,8 @@ typedef struct wcfd_solve_info {
  * x is updated in place (warm start).  Stops when ||r|| <= max(tol ||b||,
  * atol), checked before the first iteration and every check_every
  * iterations, or at maxiter.  variant: WCFD_CG_*.  info is [host]. */
-int wcfd_cg(int32_t n, const int32_t* row_ptr, const int32_t* col, const float* values,
-            const int32_t* diag_pos, int variant, const float* b, float* x, double atol,
+int wcfd_cg(int32_t n, const int32_t* row_ptr, const int32_t* col, const wcfd_real* values,
+            const int32_t* diag_pos, int variant, const wcfd_real* b, wcfd_real* x, double atol,
             double tol, int maxiter, int check_every, void* workspace, wcfd_solve_info* info,
             void* stream);
 
@@ -413,8 +426,8 @@ int wcfd_cg(int32_t n, const int32_t* row_ptr, const int32_t* col, const float* 
  * Equation.solve_Axb for the momentum system (equation.py:195-200;
  * incompressible.py:116).  nvec right-hand sides b[nvec][n] share the matrix
  * and all scalars (the reference's single interleaved 3C solve). */
-int wcfd_bicgstab(int32_t n, const int32_t* row_ptr, const int32_t* col, const float* values,
-                  const int32_t* diag_pos, int nvec, const float* b, float* x, double atol,
+int wcfd_bicgstab(int32_t n, const int32_t* row_ptr, const int32_t* col, const wcfd_real* values,
+                  const int32_t* diag_pos, int nvec, const wcfd_real* b, wcfd_real* x, double atol,
                   double tol, int maxiter, int check_every, void* workspace,
                   wcfd_solve_info* info, void* stream);
 
@@ -435,10 +448,10 @@ int wcfd_set_krylov_fold(int max_inline);           /* max_inline >= 0 */
 int wcfd_get_krylov_fold(void);
 
 /* wcfd_cg / wcfd_bicgstab on the ELL matrix `values` [7][C] of mesh m. */
-int wcfd_cg_ell(const wcfd_mesh* m, const float* values, int variant, const float* b, float* x,
+int wcfd_cg_ell(const wcfd_mesh* m, const wcfd_real* values, int variant, const wcfd_real* b, wcfd_real* x,
                 double atol, double tol, int maxiter, int check_every, void* workspace,
                 wcfd_solve_info* info, void* stream);
-int wcfd_bicgstab_ell(const wcfd_mesh* m, const float* values, int nvec, const float* b, float* x,
+int wcfd_bicgstab_ell(const wcfd_mesh* m, const wcfd_real* values, int nvec, const wcfd_real* b, wcfd_real* x,
                       double atol, double tol, int maxiter, int check_every, void* workspace,
                       wcfd_solve_info* info, void* stream);
 
@@ -474,18 +487,18 @@ typedef struct wcfd_simple_params {
 
 /* Device state of one case. */
 typedef struct wcfd_simple_state {
-  float* cell_values;  /* [4][C] */
-  float* cell_grad;    /* [4][3][C] */
-  float* face_values;  /* [4][F] */
-  float* mass_flux;    /* [F] */
-  float* hbya;         /* [3][C] momentum solution / HbyA */
-  float* p_x;          /* [C] pressure solution (warm start) */
-  float* rua;          /* [C] */
-  float* rua_face;     /* [F] */
-  float* mom_values;   /* [7][C] ELL */
-  float* mom_rhs;      /* [3][C] */
-  float* p_values;     /* [7][C] ELL */
-  float* p_rhs;        /* [C] */
+  wcfd_real* cell_values;  /* [4][C] */
+  wcfd_real* cell_grad;    /* [4][3][C] */
+  wcfd_real* face_values;  /* [4][F] */
+  wcfd_real* mass_flux;    /* [F] */
+  wcfd_real* hbya;         /* [3][C] momentum solution / HbyA */
+  wcfd_real* p_x;          /* [C] pressure solution (warm start) */
+  wcfd_real* rua;          /* [C] */
+  wcfd_real* rua_face;     /* [F] */
+  wcfd_real* mom_values;   /* [7][C] ELL */
+  wcfd_real* mom_rhs;      /* [3][C] */
+  wcfd_real* p_values;     /* [7][C] ELL */
+  wcfd_real* p_rhs;        /* [C] */
   double* resid;       /* [4] (batch: [K][4]) continuity numerator, denominator,
                           ||u^n - u^(n-1)||^2, ||u^n||^2 */
 } wcfd_simple_state;
@@ -511,11 +524,11 @@ int wcfd_simple_iteration(const wcfd_mesh* m, wcfd_bc bc, const wcfd_simple_para
  * own Krylov scalars, stopping test and iteration count; `info` arrays are
  * [host] [K].  Workspaces are sized for (C, K, G). */
 size_t wcfd_batch_krylov_workspace_bytes(int32_t n, int32_t n_cases, int32_t n_blocks);
-int wcfd_batch_cg_ell(const wcfd_mesh* m, const wcfd_batch* b, const float* values, int variant,
-                      const float* rhs, float* x, double atol, double tol, int maxiter,
+int wcfd_batch_cg_ell(const wcfd_mesh* m, const wcfd_batch* b, const wcfd_real* values, int variant,
+                      const wcfd_real* rhs, wcfd_real* x, double atol, double tol, int maxiter,
                       int check_every, void* workspace, wcfd_solve_info* info, void* stream);
-int wcfd_batch_bicgstab_ell(const wcfd_mesh* m, const wcfd_batch* b, const float* values,
-                            int nvec, const float* rhs, float* x, double atol, double tol,
+int wcfd_batch_bicgstab_ell(const wcfd_mesh* m, const wcfd_batch* b, const wcfd_real* values,
+                            int nvec, const wcfd_real* rhs, wcfd_real* x, double atol, double tol,
                             int maxiter, int check_every, void* workspace,
                             wcfd_solve_info* info, void* stream);
 

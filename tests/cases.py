@@ -32,10 +32,15 @@ def apply_bcs(boundary, kind):
         raise ValueError(kind)
 
 
-def oracle_case(nodes, cells, kind, dtype=np.float32, ref_cell=0):
+def oracle_case(nodes, cells, kind, dtype=np.float32, ref_cell=0, geom_dtype=None):
+    """``geom_dtype``: the mesh geometry's type (default: float32 with float32
+    fields, float64 with float64 fields); float32 geometry with float64
+    fields is the reference's float64 model (mesh.py:22-33)."""
     from oracle.mesh import OracleMesh
     from oracle.fv import OracleFVM
-    om = OracleMesh(nodes, cells, geom_dtype=np.float32 if dtype == np.float32 else np.float64)
+    if geom_dtype is None:
+        geom_dtype = np.float32 if dtype == np.float32 else np.float64
+    om = OracleMesh(nodes, cells, geom_dtype=geom_dtype)
     om.define_boundary_walls()
     of = OracleFVM(om, dtype)
     if ref_cell is not None:
@@ -45,12 +50,12 @@ def oracle_case(nodes, cells, kind, dtype=np.float32, ref_cell=0):
     return of
 
 
-def device_case(nodes, cells, kind, ref_cell=0, nu=0.01, renumber=True):
+def device_case(nodes, cells, kind, ref_cell=0, nu=0.01, renumber=True, float_dtype=np.float32):
     from warpcfd_amd.preprocess import Mesh, define_boundary_walls
     from warpcfd_amd.FV import FVM
     m = Mesh(nodes, cells)
     define_boundary_walls(m)
-    model = FVM(m, renumber=renumber)
+    model = FVM(m, renumber=renumber, float_dtype=float_dtype)
     model.material.create_incompressible_newtonian_fluid(viscosity=nu, density=1.0)
     if ref_cell is not None:
         model.set_reference_pressure(ref_cell, 0.0)

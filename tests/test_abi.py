@@ -25,6 +25,21 @@ def test_library_exports_every_header_symbol():
     assert lib.wcfd_krylov_workspace_bytes(1000, 3) > 8 * 3 * 1000 * 4
     assert lib.wcfd_krylov_workspace_bytes(1000, 2) == 0
     assert lib.wcfd_simple_workspace_bytes(1000, 3000) > lib.wcfd_krylov_workspace_bytes(1000, 3)
+    assert lib.wcfd_real_bytes() == 4
+
+
+def test_double_library_exports_the_same_symbols():
+    """libwcfd64.so: the same entry points with double fields; its
+    workspaces hold the same vectors at twice the bytes."""
+    lib, lib64 = _lib.load(), _lib.load64()
+    assert lib64 is not lib
+    missing = [s for s in _lib.header_symbols() if not hasattr(lib64, s)]
+    assert not missing
+    assert lib64.wcfd_real_bytes() == 8
+    assert lib64.wcfd_abi_version() == lib.wcfd_abi_version()
+    n = 100000
+    assert lib64.wcfd_krylov_workspace_bytes(n, 3) > 1.9 * lib.wcfd_krylov_workspace_bytes(n, 3)
+    assert _lib.for_dtype('float64') is lib64 and _lib.for_dtype('float32') is lib
 
 
 def test_invalid_arguments_are_rejected_without_gpu_work():

@@ -18,7 +18,9 @@ class Boundary:
         self.num_outputs = len(fields)
         self.groups = groups
         self.boundary_ids = np.asarray(boundary_ids, np.int32)
-        self.boundary_values = np.zeros((len(self.boundary_ids), self.num_outputs), np.float32)
+        # float64 on the host (exact for the Python floats given); cast to the
+        # model's field type when uploaded
+        self.boundary_values = np.zeros((len(self.boundary_ids), self.num_outputs), np.float64)
         self.boundary_type = np.zeros(self.boundary_values.shape, np.uint8)
         self.float_dtype = float_type
         self._boundary_types = {'dirichlet': DIRICHLET, 'vonNeumann': NEUMANN}
@@ -91,17 +93,18 @@ class Boundary:
         t = self.boundary_type
         return bool(np.all(t[:, 0] == t[:, 1]) and np.all(t[:, 0] == t[:, 2]))
 
-    def host_arrays(self, layout):
-        """BC type (uint8) and value (float32) as [4][Fb] in device face
+    def host_arrays(self, layout, dtype=np.float32):
+        """BC type (uint8) and value (``dtype``) as [4][Fb] in device face
         order."""
         rows = layout.bnd_ref_row
         t = np.ascontiguousarray(self.boundary_type[rows].T).astype(np.uint8)
-        v = np.ascontiguousarray(self.boundary_values[rows].T.astype(np.float32))
+        v = np.ascontiguousarray(self.boundary_values[rows].T.astype(dtype))
         return t, v
 
-    def device_arrays(self, layout, device):
-        t, v = self.host_arrays(layout)
+    def device_arrays(self, layout, device, dtype=torch.float32):
+        np_dt = np.float64 if dtype == torch.float64 else np.float32
+        t, v = self.host_arrays(layout, np_dt)
         if t.size == 0:
             t = np.zeros((4, 1), np.uint8)
-            v = np.zeros((4, 1), np.float32)
+            v = np.zeros((4, 1), np_dt)
         return torch.from_numpy(t).to(device), torch.from_numpy(v).to(device)

@@ -40,14 +40,21 @@ def ptr(t):
 
 
 class DeviceMesh:
-    """Uploads a ``DeviceLayout`` and builds its ``wcfd_mesh`` descriptor."""
+    """Uploads a ``DeviceLayout`` and builds its ``wcfd_mesh`` descriptor.
+    ``dtype`` (torch.float32 or torch.float64) is the field type: the
+    geometry is uploaded in it and ``lib`` is the library built for it.  The
+    layout's geometry is float32 either way (the reference computes the mesh
+    in float32 numpy whatever its model's dtype, preprocess/mesh.py:22-33), so
+    a float64 model sees the same geometry values as a float32 one."""
 
-    def __init__(self, layout, device='cuda'):
+    def __init__(self, layout, device='cuda', dtype=torch.float32):
         self.device = require_gpu(device)
         self.layout = L = layout
-        self.lib = _lib.load()
+        self.dtype = dtype
+        self.lib = _lib.for_dtype(dtype)
         up = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a).astype(dt)).to(self.device)
-        i32, f32, i8 = np.int32, np.float32, np.int8
+        i32, i8 = np.int32, np.int8
+        f32 = np.float64 if dtype == torch.float64 else np.float32      # the field type
         self.owner = up(L.owner, i32)
         self.neigh = up(L.neigh if L.neigh.size else np.zeros(1), i32)
         self.psi = up(L.psi if L.psi.size else np.zeros(1), f32)

@@ -54,17 +54,17 @@ class IncompressibleSolver:
             raise NotImplementedError('u, v and w must share the boundary-condition type on every '
                                       'face (true for every helper of Boundary)')
         self.model = model
-        self.float_dtype = np.float32
+        self.float_dtype = model.float_dtype
         self.steady_state = steady_state
         self.correction = bool(correction)
         self.implicit_outflow = bool(implicit_outflow)
         self.p_relaxation_factor = p_relaxation_factor
         self.u_relaxation_factor = u_relaxation_factor
         self.material = model.material
-        self.lib = _lib.load()
+        self.lib = model.lib
         C, F, nnz = model.num_cells, model.dm.n_faces, model.dm.nnz
         dev = model._cv.device
-        z = lambda *s: torch.zeros(*s, dtype=torch.float32, device=dev)
+        z = lambda *s: torch.zeros(*s, dtype=model.tdtype, device=dev)
         self.HbyA = z(3, C)
         self.p_x = z(C)
         self.rUA = z(C)
@@ -77,7 +77,8 @@ class IncompressibleSolver:
         self.workspace = torch.zeros(self.lib.wcfd_simple_workspace_bytes(C, F), dtype=torch.uint8,
                                      device=dev)
         self.max_iter = 500
-        self.tol = 1e-5          # equation.py:36-39 (float32)
+        # equation.py:35-39: atol 1e-5 for float32 models, 1e-6 for float64
+        self.tol = 1e-5 if model.tdtype == torch.float32 else 1e-6
         self.rtol = None         # None: relative tolerance = atol, as the reference's call
         # not in the reference: also stop at rtol0 * ||r_0|| (the warm-start
         # residual); 0 = off

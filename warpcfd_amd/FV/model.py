@@ -23,11 +23,16 @@ from .field import Field
 class FVM:
     def __init__(self, mesh, output_variables=None, float_dtype=np.float32, int_dtype=np.int32,
                  device='cuda', renumber=True):
-        if np.dtype(float_dtype) != np.float32 and float_dtype is not torch.float32:
-            raise NotImplementedError('the device path computes in float32 (float64 reductions)')
+        # model.py:54-55: float32 (libwcfd.so) or float64 (libwcfd64.so); the
+        # reductions are float64 either way
+        if float_dtype in (torch.float32, torch.float64):
+            float_dtype = np.float32 if float_dtype == torch.float32 else np.float64
+        if np.dtype(float_dtype) not in (np.float32, np.float64):
+            raise ValueError(f'float_dtype must be float32 or float64, got {float_dtype}')
         self.material = Material()
         self.dimension = mesh.dimension
-        self.float_dtype = np.float32
+        self.float_dtype = np.dtype(float_dtype).type
+        self.tdtype = torch.float64 if self.float_dtype == np.float64 else torch.float32
         self.int_dtype = np.int32
         self.mesh = mesh
         self.skew_correction = True
@@ -69,16 +74,16 @@ class FVM:
         self.boundary.finalize()
         self.layout = layout if layout is not None else DeviceLayout(self.mesh,
                                                                      renumber=self._renumber)
-        self.dm = DeviceMesh(self.layout, dev)
-        self.bc_type, self.bc_value = self.boundary.device_arrays(self.layout, dev)
+        self.dm = DeviceMesh(self.layout, dev, dtype=self.tdtype)
+        self.bc_type, self.bc_value = self.boundary.device_arrays(self.layout, dev, self.tdtype)
         self.bc = _lib.WcfdBC(self.bc_type.data_ptr(), self.bc_value.data_ptr())
         C, F = self.num_cells, self.dm.n_faces        # device faces (with a null face)
-        z = lambda *s: torch.zeros(*s, dtype=torch.float32, device=dev)
+        z = lambda *s: torch.zeros(*s, dtype=self.tdtype, device=dev)
         self._cv = z(4, C)
         self._cg = z(12, C)
         self._fv = z(4, F)
         self._flux = z(F)
-        self.lib = _lib.load()
+        self.lib = self.dm.lib
         self._reduce_ws = torch.zeros(self.lib.wcfd_reduce_workspace_bytes(F), dtype=torch.uint8,
                                       device=dev)
         self.finalized = True
@@ -91,7 +96,7 @@ class FVM:
         """model.py:143-147 (IC in reference cell order)."""
         self._need()
         ic = torch.as_tensor(np.asarray(IC) if not torch.is_tensor(IC) else IC,
-                             dtype=torch.float32, device=self._cv.device)
+                             dtype=self.tdtype, device=self._cv.device)
         if output_indices is None:
             output_indices = list(range(ic.shape[-1]))
         for o in self._indices(output_indices):
@@ -137,7 +142,7 @@ class FVM:
     # ------------------------------------------------ reference-order views
     @property
     def cell_values(self):
-        """(C,4) float32 tensor in reference cell order (a copy)."""
+        """(C,4) tensor of the model's float type in reference cell order (a copy)."""
         self._need()
         return host_readable(self.dm.cells_to_reference(self._cv).T.contiguous())
 
@@ -162,7 +167,7 @@ class FVM:
     def face_gradients(self):
         """(F,4): the Neumann boundary gradients (the only entries the
         reference ever writes, conditions.py:17-18)."""
-        fg = np.zeros((self.num_faces, 4), np.float32)
+        fg = np.zeros((self.num_faces, 4), self.float_dtype)
         b = self.boundary
         n = b.boundary_type == 2
         rows, cols = np.nonzero(n)
@@ -208,14 +213,14 @@ class FVM:
     def divFlux(self, arr=None):
         """model.py:269-275; returns a (C,) tensor in reference order."""
         self._need()
-        d = torch.empty(self.num_cells, dtype=torch.float32, device=self._cv.device)
+        d = torch.empty(self.num_cells, dtype=self.tdtype, device=self._cv.device)
         _lib.check(self.lib.wcfd_divergence(self.dm.desc_ref, ptr(self._flux), ptr(d), self._s()),
                    'divFlux')
         out = self.dm.cells_to_reference(d)
         if arr is not None:
             arr.copy_(out)
             return arr
-        return out
+        return host_readable(out)
 
     # ------------------------------------------ check_convergence arguments
     def _ell_of(self, matrix):
@@ -250,7 +255,7 @@ class FVM:
             raise ValueError(f'vel_matrix has {n} rows; expected {C} or {3 * C}')
         L = self.layout
         old = L.new2old
-        vals = np.zeros((7, C), np.float32)
+        vals = np.zeros((7, C), self.float_dtype)
         vals[0] = np.asarray(A[old, old]).ravel()
         for k in range(6):
             nb = L.slot_nbr[k]
@@ -259,7 +264,7 @@ class FVM:
         return torch.from_numpy(vals).to(self._cv.device)
 
     def _cells_of(self, t, rows):
-        """A cell field as a contiguous float32 device tensor in device
+        """A cell field as a contiguous device tensor of the model's type in device
         order: (rows, C) or (C,) device-order tensors as they are; the
         reference's forms, (C * rows,) interleaved or (C, rows), reordered."""
         if t is None:
@@ -268,11 +273,11 @@ class FVM:
         t = torch.as_tensor(np.asarray(t) if not torch.is_tensor(t) else t).to(dev)
         shape = tuple(t.shape)
         if rows == 1 and shape == (C,):
-            return t.to(torch.float32).contiguous()
+            return t.to(self.tdtype).contiguous()
         if rows > 1 and shape == (rows, C):
-            return t.to(torch.float32).contiguous()
+            return t.to(self.tdtype).contiguous()
         if rows > 1 and shape in ((C * rows,), (C, rows)):
-            ref = t.to(torch.float32).reshape(C, rows).T.contiguous()
+            ref = t.to(self.tdtype).reshape(C, rows).T.contiguous()
             return self.dm.cells_to_device(ref).contiguous()
         raise ValueError(f'expected {(rows, C) if rows > 1 else (C,)} in device order or '
                          f'{(C * rows,)} / {(C, rows)} in reference order, got {shape}')
@@ -367,24 +372,24 @@ class FVM:
         o = self._indices(output_index)[0]
         g = self.cell_gradients[:, o]
         if isinstance(coeff, (float, int)):
-            return (torch.tensor(coeff, dtype=torch.float32, device=g.device) * g)
-        c = torch.as_tensor(coeff, dtype=torch.float32, device=g.device).reshape(-1, 1)
+            return (torch.tensor(coeff, dtype=self.tdtype, device=g.device) * g)
+        c = torch.as_tensor(coeff, dtype=self.tdtype, device=g.device).reshape(-1, 1)
         return c * g
 
     def relax(self, new_value, alpha, output_index):
         """model.py:256-262 (new_value in reference order)."""
         self._need()
         o = self._indices(output_index)[0]
-        nv = self.dm.cells_to_device(torch.as_tensor(new_value, dtype=torch.float32,
+        nv = self.dm.cells_to_device(torch.as_tensor(new_value, dtype=self.tdtype,
                                                      device=self._cv.device))
-        a = torch.tensor(alpha, dtype=torch.float32)
+        a = torch.tensor(alpha, dtype=self.tdtype)
         self._cv[o] = a.item() * nv + (1.0 - a).item() * self._cv[o]
 
     def replace_cell_values(self, output_indices, value):
         """model.py:318-325: value is interleaved (C*n,) in reference order."""
         self._need()
         idx = self._indices(output_indices)
-        v = torch.as_tensor(value, dtype=torch.float32, device=self._cv.device).reshape(-1, len(idx))
+        v = torch.as_tensor(value, dtype=self.tdtype, device=self._cv.device).reshape(-1, len(idx))
         for k, o in enumerate(idx):
             self._cv[o] = self.dm.cells_to_device(v[:, k].contiguous())
 
@@ -392,7 +397,7 @@ class FVM:
         """model.py:329-349."""
         self._need()
         idx = self._indices(field_idx)
-        v = torch.as_tensor(value, dtype=torch.float32, device=self._cv.device)
+        v = torch.as_tensor(value, dtype=self.tdtype, device=self._cv.device)
         for k, o in enumerate(idx):
             if v.numel() == 1:
                 self._cv[o] += scale * v.reshape(())

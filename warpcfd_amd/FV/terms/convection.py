@@ -30,10 +30,10 @@ def face_c2f(fv):
         own = fp.adjacent_cells[fi_ref[real], 0]
         k0, k1 = fp.cell_face_index[fi_ref[real], 0], fp.cell_face_index[fi_ref[real], 1]
         c2f = cp.cell_centroid_to_face_centroid
-        a = np.zeros((2, 3, L.n_internal), np.float32)
+        a = np.zeros((2, 3, L.n_internal), fv.float_dtype)
         a[:, :, real] = np.stack([c2f[own, k0].T, c2f[own, k1].T])
         if a.size == 0:
-            a = np.zeros((2, 3, 1), np.float32)
+            a = np.zeros((2, 3, 1), fv.float_dtype)
         fv._face_c2f = torch.from_numpy(np.ascontiguousarray(a)).to(fv._cv.device)
     return fv._face_c2f
 

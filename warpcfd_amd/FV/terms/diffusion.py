@@ -43,7 +43,7 @@ class DiffusionTerm(Term):
             nu = float(viscosity)
         else:
             v = torch.as_tensor(np.asarray(viscosity) if not torch.is_tensor(viscosity)
-                                else viscosity, dtype=torch.float32).reshape(-1)
+                                else viscosity, dtype=fv.tdtype).reshape(-1)
             if v.numel() == 1:
                 nu = float(v.item())
             elif v.numel() == fv.num_faces:

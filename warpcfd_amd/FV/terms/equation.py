@@ -59,18 +59,18 @@ class Equation:
         self.sparse_rows = fvm.num_cells * self.num_outputs
         self.linear_solver = _solver_name(solver)
         self.max_iter = 500
-        self.tol = 1.e-5
+        self.tol = 1.e-5 if fvm.tdtype == torch.float32 else 1.e-6   # equation.py:35-39
         self.check_every = 10
         self.initial = True
-        self.float_dtype = np.float32
+        self.float_dtype = fvm.float_dtype
         self.int_dtype = np.int32
         self.fvm = fvm
         self.initial_gradient = False
         C, O = fvm.num_cells, self.num_outputs
         dev = fvm._cv.device
-        self.values = torch.zeros((O, 7, C), dtype=torch.float32, device=dev)
-        self._rhs = torch.zeros((O, C), dtype=torch.float32, device=dev)
-        self._x = torch.zeros((O, C), dtype=torch.float32, device=dev)
+        self.values = torch.zeros((O, 7, C), dtype=fvm.tdtype, device=dev)
+        self._rhs = torch.zeros((O, C), dtype=fvm.tdtype, device=dev)
+        self._x = torch.zeros((O, C), dtype=fvm.tdtype, device=dev)
         self._ws = None
         self.last_info = None
 
@@ -83,7 +83,7 @@ class Equation:
         """(C * O,) interleaved, reference cell order -> (O, C) device."""
         C, O = self.fvm.num_cells, self.num_outputs
         v = torch.as_tensor(np.asarray(v) if not torch.is_tensor(v) else v,
-                            dtype=torch.float32).to(self.values.device).reshape(C, O)
+                            dtype=self.fvm.tdtype).to(self.values.device).reshape(C, O)
         return self.fvm.dm.cells_to_device(v.T.contiguous()).contiguous()
 
     @property
@@ -98,7 +98,7 @@ class Equation:
     def output_gradient(self):
         if not hasattr(self, '_output_gradient'):
             self._output_gradient = torch.zeros((self.fvm.num_cells, self.fvm.dimension),
-                                                dtype=torch.float32, device=self.values.device)
+                                                dtype=self.fvm.tdtype, device=self.values.device)
         return self._output_gradient
 
     @property

@@ -11,7 +11,7 @@ class GradTerm(Term):
         super().__init__(fv, field, implicit=False, need_global_index=True, cell_based=False)
         # one row per direction: the explicit weights have fv.dimension outputs
         import torch
-        self._w = torch.zeros((fv.dimension, fv.num_cells), dtype=torch.float32,
+        self._w = torch.zeros((fv.dimension, fv.num_cells), dtype=fv.tdtype,
                               device=fv._cv.device)
 
     def calculate_weights(self, fv, **kwargs):

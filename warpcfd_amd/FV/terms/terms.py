@@ -47,7 +47,7 @@ class Term:
         self.num_outputs = len(self.fields)
         self.set_implicit(implicit)
         self.cell_based = cell_based
-        self.float_dtype = np.float32
+        self.float_dtype = fv.float_dtype
         self.int_dtype = np.int32
         self._scale = 1.0
         self._fv = fv
@@ -58,7 +58,7 @@ class Term:
             shape = (O, 2, C) if cell_based else (O, 3, 6, C)
         else:
             shape = (O, C)
-        self._w = torch.zeros(shape, dtype=torch.float32, device=dev)
+        self._w = torch.zeros(shape, dtype=fv.tdtype, device=dev)
 
     # ------------------------------------------------------------- scale
     def set_implicit(self, implicit):

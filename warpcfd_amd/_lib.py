@@ -10,6 +10,9 @@ import re
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_NAME = 'libwcfd.so'
 LIB_PATH = os.path.join(_HERE, LIB_NAME)
+# the same library built with double fields (-DWCFD_REAL=double)
+LIB64_NAME = 'libwcfd64.so'
+LIB64_PATH = os.path.join(_HERE, LIB64_NAME)
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), 'include', 'wcfd.h')
 
 WCFD_OK, WCFD_EINVAL, WCFD_EHIP, WCFD_ENOTCONV, WCFD_ENAN, WCFD_ESCRATCH = 0, 1, 2, 3, 4, 5
@@ -84,6 +87,7 @@ _SIGS = {
     'wcfd_last_hip_error': (ctypes.c_int, []),
     'wcfd_error_string': (ctypes.c_char_p, [ctypes.c_int]),
     'wcfd_abi_version': (ctypes.c_int, []),
+    'wcfd_real_bytes': (ctypes.c_int, []),
     'wcfd_set_boundary_conditions': (ctypes.c_int, [_PM, WcfdBC, c_p, ctypes.c_int, c_p]),
     'wcfd_face_interpolation': (ctypes.c_int, [_PM, WcfdBC, c_p, c_p, c_p, ctypes.c_int,
                                                ctypes.c_int, c_p, c_p, c_p, c_p]),
@@ -161,7 +165,7 @@ _SIGS = {
                                                    ctypes.POINTER(SimpleInfo), c_p]),
 }
 
-_lib = None
+_libs = {}
 
 
 class WcfdError(RuntimeError):
@@ -175,23 +179,50 @@ def header_symbols(path=HEADER_PATH):
 
 
 def load(path=LIB_PATH):
-    """Load libwcfd.so and declare every signature; raises if it is missing."""
-    global _lib
-    if _lib is not None:
-        return _lib
+    """Load libwcfd.so (or the library at ``path``) and declare every
+    signature; raises if it is missing.  Each library is loaded once."""
+    if path in _libs:
+        return _libs[path]
     # torch first: it loads its own HIP runtime; libwcfd.so must bind to that
     # same runtime (loaded the other way round, HIP calls fail with
     # hipErrorNoDevice on tensors torch allocated)
     import torch  # noqa: F401
     if not os.path.exists(path):
-        raise WcfdError(f'{LIB_NAME} is not built ({path}); run __graft_entry__.build() first')
+        raise WcfdError(f'{os.path.basename(path)} is not built ({path}); run '
+                        f'__graft_entry__.build() first')
     lib = ctypes.CDLL(path)
     for name, (res, args) in _SIGS.items():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    _lib = lib
+    _libs[path] = lib
     return lib
+
+
+def load64():
+    """libwcfd64.so: the same entry points with double fields."""
+    return load(LIB64_PATH)
+
+
+def for_dtype(dtype):
+    """The library whose field type is ``dtype`` (numpy or torch float32 /
+    float64)."""
+    import numpy as np
+    import torch
+    if dtype in (torch.float32, torch.float64):
+        dtype = np.float32 if dtype == torch.float32 else np.float64
+    dt = np.dtype(dtype)
+    if dt == np.float32:
+        return load()
+    if dt == np.float64:
+        return load64()
+    raise ValueError(f'no library computes in {dt}: float32 or float64')
+
+
+def real_dtype(lib):
+    """torch dtype of a loaded library's fields."""
+    import torch
+    return torch.float64 if lib.wcfd_real_bytes() == 8 else torch.float32
 
 
 def check(code, what=''):

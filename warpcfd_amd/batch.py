@@ -54,7 +54,7 @@ class CaseSpec:
         L = layout if layout is not None else DeviceLayout(model.mesh, renumber=model._renumber)
         if L.slot_dnb is None:
             raise ValueError('the case band does not fit int16 neighbour offsets')
-        self.bc_type, self.bc_value = b.host_arrays(L)
+        self.bc_type, self.bc_value = b.host_arrays(L, np.float64)   # cast on upload
         self.layout = L.arena_view()
         rho = float(model.material.density)
         self.density = rho
@@ -160,9 +160,13 @@ class CaseArena:
                  chunk=256, block=256, density=1.0, alpha_u=0.7, alpha_p=0.3, atol=0.0,
                  tol=1e-6, rtol0=0.0, maxiter=2000, check_every=10, check_every_u=0, p_refine=0,
                  maxiter_u=0,
-                 cg_variant=_lib.CG_SINGLE_REDUCTION, implicit_outflow=True, correction=False):
+                 cg_variant=_lib.CG_SINGLE_REDUCTION, implicit_outflow=True, correction=False,
+                 dtype=torch.float32):
+        """``dtype``: field type, torch.float32 (libwcfd.so) or torch.float64
+        (libwcfd64.so, every field, matrix and geometry array in double)."""
         self.device = require_gpu(device)
-        self.lib = _lib.load()
+        self.dtype = dtype
+        self.lib = _lib.for_dtype(dtype)
         Cc, Fic, Fbc, K = int(cell_cap), int(internal_cap), int(boundary_cap), int(max_cases)
         if min(Cc, Fic, Fbc, K) <= 0:
             raise ValueError('capacities must be positive')
@@ -177,7 +181,7 @@ class CaseArena:
                          'faces': (Fic + per - 1) // per + (Fbc + per - 1) // per + 2 * K,
                          'bfaces': (Fbc + per - 1) // per + K}
         dev = self.device
-        f32, i32 = torch.float32, torch.int32
+        f32, i32 = dtype, torch.int32
         z = lambda *s, dt=f32: torch.zeros(*s, dtype=dt, device=dev)
         F = self.F
         # mesh
@@ -692,7 +696,8 @@ class CaseArena:
 
     # ------------------------------------------------------------ results
     def cell_values(self, k):
-        """(C, 4) float32 numpy array of case ``k`` in its reference cell order."""
+        """(C, 4) numpy array (the arena's field type) of case ``k`` in its
+        reference cell order."""
         sl = self.slots[k]
         s, c0 = sl['spec'], sl['c0']
         dev = self.cv[:, c0:c0 + s.n_cells].cpu().numpy()

@@ -9,6 +9,10 @@
 
 namespace wcfd {
 
+// Field type of this build (float, or double for libwcfd64.so).
+using real = wcfd_real;
+static_assert(sizeof(real) == 4 || sizeof(real) == 8, "real is float or double");
+
 constexpr int kBlock = 256;           // threads per workgroup for streaming kernels
 constexpr int kWave = 64;             // CDNA wavefront
 constexpr int kMaxWaves = 16;         // LDS sizing: workgroups of up to 1024 threads
@@ -164,49 +168,68 @@ __device__ inline void wave_sum_n(double (&v)[ND]) {
 // a 32-bit byte offset per lane.  The compiler forms such an address as a
 // scalar base plus one offset VGPR (global_load ... vOff, s[base]), so the
 // loads of several arrays at the same row share one offset register instead
-// of a 64-bit address pair each.  Offsets must stay below 2^32 bytes: the
-// ELL entry points reject meshes with 7 n floats beyond that.
+// of a 64-bit address pair each.  Offsets must stay below 2^32 bytes:
+// mesh_ok() refuses meshes whose field arrays reach that.
+//
+// The kernels form every offset of a field array as index * 4 -- the byte
+// offset of a 4-byte element, shared with the int32 tables at the same
+// index.  An 8-byte element type (real = double in libwcfd64.so) scales it
+// by 2 here; every other element type takes the offset as a byte offset.
+template <class T>
+__device__ __forceinline__ uint32_t scaled(uint32_t byte_off) {
+  if constexpr (sizeof(T) == 8)
+    return byte_off * 2u;
+  else
+    return byte_off;
+}
 template <class T>
 __device__ __forceinline__ T ld_off(const T* base, uint32_t byte_off) {
-  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + scaled<T>(byte_off));
 }
 template <class T>
 __device__ __forceinline__ void st_off(T* base, uint32_t byte_off, T v) {
 #if WCFD_NT_STORE
-  __builtin_nontemporal_store(v, reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off));
+  __builtin_nontemporal_store(
+      v, reinterpret_cast<T*>(reinterpret_cast<char*>(base) + scaled<T>(byte_off)));
 #else
-  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off) = v;
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + scaled<T>(byte_off)) = v;
 #endif
 }
 // an ordinary store at a uniform base plus a 32-bit byte offset
 template <class T>
 __device__ __forceinline__ void st_plain(T* base, uint32_t byte_off, T v) {
-  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off) = v;
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + scaled<T>(byte_off)) = v;
 }
 // matrix stream (read once per iteration)
 template <class T>
 __device__ __forceinline__ T ld_mat(const T* base, uint32_t byte_off) {
 #if WCFD_NT_MATRIX
   return __builtin_nontemporal_load(
-      reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off));
+      reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + scaled<T>(byte_off)));
 #else
   return ld_off(base, byte_off);
 #endif
 }
 
-// Raw buffer view of a float array for gathers: the descriptor sits in
+// Raw buffer view of a real array for gathers: the descriptor sits in
 // scalar registers and each load takes one 32-bit byte offset per lane
 // (buffer_load_dword vOff, s[desc]), so the gathers of several vectors at
 // the same neighbour share one offset register.  Offsets at or beyond
-// `bytes` read 0.  Word 3 is the gfx9 raw dword-buffer format.
+// `bytes` read 0.  Word 3 is the gfx9 raw dword-buffer format.  Sizes and
+// offsets are given for 4-byte elements (index * 4) and scaled as above.
 constexpr int kBufWord3 = 0x00020000;
 struct FBuf {
   __amdgpu_buffer_rsrc_t r;
-  __device__ __forceinline__ FBuf(const float* p, uint32_t bytes)
-      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), (short)0, (int)bytes,
-                                            kBufWord3)) {}
-  __device__ __forceinline__ float operator()(uint32_t off) const {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+  __device__ __forceinline__ FBuf(const real* p, uint32_t bytes)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<real*>(p), (short)0,
+                                            (int)scaled<real>(bytes), kBufWord3)) {}
+  __device__ __forceinline__ real operator()(uint32_t off) const { return load<real>(off); }
+  template <class T>
+  __device__ __forceinline__ T load(uint32_t off) const {
+    if constexpr (sizeof(T) == 8)
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, scaled<T>(off), 0, 0));
+    else
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
   }
 };
 
@@ -345,9 +368,9 @@ struct BatchDev {
   const int32_t* cell_case;  // [C] case of each cell, -1 for unused rows
   const int32_t* face_case;  // [F]
   const uint8_t* active;     // [K]
-  const float* nu;           // [K] or NULL
+  const real* nu;           // [K] or NULL
   const int32_t* ref_cell;   // [K] or NULL
-  const float* ref_value;    // [K]
+  const real* ref_value;    // [K]
   Items cells, faces, bfaces;  // compacted launches (cells, all faces, boundary faces)
   __device__ bool cell_skip(int c) const {
     if (!cell_case) return false;
@@ -551,9 +574,9 @@ struct CsrMat {
   int n;
   const int32_t* rp;
   const int32_t* col;
-  const float* val;
+  const real* val;
   const int32_t* dpos;
-  __device__ float diag(int i) const { return val[dpos[i]]; }
+  __device__ real diag(int i) const { return val[dpos[i]]; }
   template <class F>
   __device__ void row(int i, F&& f) const {
     for (int j = rp[i]; j < rp[i + 1]; ++j) f(val[j], col[j]);
@@ -574,13 +597,13 @@ struct EllMat {
   static constexpr int kSlots = NS;
   static constexpr int kRow = NS + 1;   // diagonal + slots
   int n;
-  const float* val;
+  const real* val;
   const int32_t* nbr;
   const int16_t* dnb;
-  __device__ float diag(int i) const { return val[i]; }
+  __device__ real diag(int i) const { return val[i]; }
   // the row as 7 (coefficient, column) pairs in row() order, for kernels
   // that issue every load of a row before they wait on anything
-  __device__ void load(int i, float (&a)[kRow], int (&c)[kRow]) const {
+  __device__ void load(int i, real (&a)[kRow], int (&c)[kRow]) const {
     a[0] = val[i];
     c[0] = i;
 #pragma unroll
@@ -589,9 +612,9 @@ struct EllMat {
       a[k + 1] = val[(k + 1) * n + i];
     }
   }
-  // The same row with the columns as byte offsets of float vectors
+  // The same row with the columns as byte offsets of real vectors
   // (o[j] = 4 c[j]); every load is a uniform base plus a 32-bit offset.
-  __device__ void load_off(int i, float (&a)[kRow], uint32_t (&o)[kRow]) const {
+  __device__ void load_off(int i, real (&a)[kRow], uint32_t (&o)[kRow]) const {
     const uint32_t oi = (uint32_t)i * 4u;
     a[0] = ld_mat(val, oi);
     o[0] = oi;

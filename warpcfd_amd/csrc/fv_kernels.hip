@@ -20,7 +20,7 @@ thread_local int g_last_hip_error = 0;
 
 // ------------------------------------------------------------------ BCs
 __global__ void k_set_bc(BatchDev bd, int F, int Fi, int Fb, const uint8_t* __restrict__ type,
-                         const float* __restrict__ value, float* __restrict__ fv, int mask) {
+                         const real* __restrict__ value, real* __restrict__ fv, int mask) {
   const int j = item_index(bd.bfaces);
   if (j >= Fb) return;
   int f = Fi + j;
@@ -39,7 +39,7 @@ __global__ void k_set_bc(BatchDev bd, int F, int Fi, int Fb, const uint8_t* __re
 // cell array) and streams the face arrays at the face's byte offset; for
 // k_face_interp that form measured 3% slower at 8.6 M cells, so it keeps
 // plain indexing.
-__device__ inline FBuf cell_plane(const wcfd_mesh& m, const float* a, int q) {
+__device__ inline FBuf cell_plane(const wcfd_mesh& m, const real* a, int q) {
   return FBuf(a + (size_t)q * m.n_cells, (uint32_t)m.n_cells * 4u);
 }
 
@@ -50,21 +50,21 @@ __device__ inline FBuf cell_plane(const wcfd_mesh& m, const float* a, int q) {
 // time (2-3 % faster at 4.4-8.6 M cells; two faces per thread this way
 // needs 104 VGPRs and lost 25 %).
 __device__ inline void face_interp_internal(const wcfd_mesh& m, int f,
-                                            const float* __restrict__ cv,
-                                            const float* __restrict__ cg, float* __restrict__ fv,
-                                            int mask, int skew_mask, float* __restrict__ flux,
-                                            const float* __restrict__ rua,
-                                            float* __restrict__ rua_f) {
+                                            const real* __restrict__ cv,
+                                            const real* __restrict__ cg, real* __restrict__ fv,
+                                            int mask, int skew_mask, real* __restrict__ flux,
+                                            const real* __restrict__ rua,
+                                            real* __restrict__ rua_f) {
   const int C = m.n_cells, F = m.n_faces, Fi = m.n_internal;
   const int o = m.owner[f], n = m.neigh[f];
-  const float psi = m.psi[f];
-  const float s0 = m.skew[f], s1 = m.skew[Fi + f], s2 = m.skew[2 * Fi + f];
+  const real psi = m.psi[f];
+  const real s0 = m.skew[f], s1 = m.skew[Fi + f], s2 = m.skew[2 * Fi + f];
   // loads of every field
-  float co[4], cn[4], go[4][3], gn[4][3], old[3];
+  real co[4], cn[4], go[4][3], gn[4][3], old[3];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     if ((mask >> q) & 1) {
-      const float* gq = cg + (size_t)(3 * q) * C;
+      const real* gq = cg + (size_t)(3 * q) * C;
       co[q] = cv[q * C + o];
       cn[q] = cv[q * C + n];
       if ((skew_mask >> q) & 1) {
@@ -78,7 +78,7 @@ __device__ inline void face_interp_internal(const wcfd_mesh& m, int f,
       old[q] = fv[q * F + f];
     }
   }
-  float nr[3], ar = 0.f, ro = 0.f, rn = 0.f;
+  real nr[3], ar = 0.f, ro = 0.f, rn = 0.f;
   if (flux) {
     nr[0] = m.normal[f];
     nr[1] = m.normal[F + f];
@@ -90,16 +90,16 @@ __device__ inline void face_interp_internal(const wcfd_mesh& m, int f,
     rn = rua[n];
   }
   // arithmetic
-  const float psi2 = 1.0f - psi;
-  float val[4];
+  const real psi2 = 1.0f - psi;
+  real val[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     if ((mask >> q) & 1) {
-      float v = co[q] * psi + psi2 * cn[q];
+      real v = co[q] * psi + psi2 * cn[q];
       if ((skew_mask >> q) & 1) {
-        const float g0 = go[q][0] * psi + psi2 * gn[q][0];
-        const float g1 = go[q][1] * psi + psi2 * gn[q][1];
-        const float g2 = go[q][2] * psi + psi2 * gn[q][2];
+        const real g0 = go[q][0] * psi + psi2 * gn[q][0];
+        const real g1 = go[q][1] * psi + psi2 * gn[q][1];
+        const real g2 = go[q][2] * psi + psi2 * gn[q][2];
         v = v + (g0 * s0 + g1 * s1 + g2 * s2);
       }
       val[q] = v;
@@ -116,20 +116,20 @@ __device__ inline void face_interp_internal(const wcfd_mesh& m, int f,
 }
 
 __device__ inline void face_interp_boundary(const wcfd_mesh& m, wcfd_bc bc, int f,
-                                            const float* __restrict__ cv, float* __restrict__ fv,
-                                            int mask, float* __restrict__ flux,
-                                            const float* __restrict__ rua,
-                                            float* __restrict__ rua_f, int apply_dirichlet) {
+                                            const real* __restrict__ cv, real* __restrict__ fv,
+                                            int mask, real* __restrict__ flux,
+                                            const real* __restrict__ rua,
+                                            real* __restrict__ rua_f, int apply_dirichlet) {
   const int C = m.n_cells, F = m.n_faces, Fi = m.n_internal, Fb = m.n_boundary;
-  float val[4];
+  real val[4];
   const int j = f - Fi;
   const int P = m.owner[f];
-  const float d = m.dmag[f];
+  const real d = m.dmag[f];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const bool want = ((mask >> q) & 1) || (q < 3 && flux);
     if (!want) continue;
-    float v;
+    real v;
     const uint8_t t = bc.type[q * Fb + j];
     if (((mask >> q) & 1) && t == WCFD_NEUMANN) {
       v = d * bc.value[q * Fb + j] + cv[q * C + P];
@@ -144,15 +144,15 @@ __device__ inline void face_interp_boundary(const wcfd_mesh& m, wcfd_bc bc, int 
   }
   if (rua_f) rua_f[f] = rua[P];
   if (flux) {
-    const float n0 = m.normal[f], n1 = m.normal[F + f], n2 = m.normal[2 * F + f];
+    const real n0 = m.normal[f], n1 = m.normal[F + f], n2 = m.normal[2 * F + f];
     flux[f] = (val[0] * n0 + val[1] * n1 + val[2] * n2) * m.area[f];
   }
 }
 
-__global__ void k_face_interp(wcfd_mesh m, BatchDev bd, wcfd_bc bc, const float* __restrict__ cv,
-                              const float* __restrict__ cg, float* __restrict__ fv,
-                              int mask, int skew_mask, float* __restrict__ flux,
-                              const float* __restrict__ rua, float* __restrict__ rua_f,
+__global__ void k_face_interp(wcfd_mesh m, BatchDev bd, wcfd_bc bc, const real* __restrict__ cv,
+                              const real* __restrict__ cg, real* __restrict__ fv,
+                              int mask, int skew_mask, real* __restrict__ flux,
+                              const real* __restrict__ rua, real* __restrict__ rua_f,
                               int apply_dirichlet) {
   const int f = item_index(bd.faces);
   if (f >= m.n_faces || bd.face_skip(f)) return;
@@ -187,13 +187,13 @@ __device__ inline void item_range(const Items& it, int n, int& a, int& b) {
 }
 
 __global__ __launch_bounds__(kBlock) void k_face_interp_lds(
-    wcfd_mesh m, BatchDev bd, wcfd_bc bc, const float* __restrict__ cv,
-    const float* __restrict__ cg, float* __restrict__ fv, int mask, int skew_mask,
-    float* __restrict__ flux, const float* __restrict__ rua, float* __restrict__ rua_f,
+    wcfd_mesh m, BatchDev bd, wcfd_bc bc, const real* __restrict__ cv,
+    const real* __restrict__ cg, real* __restrict__ fv, int mask, int skew_mask,
+    real* __restrict__ flux, const real* __restrict__ rua, real* __restrict__ rua_f,
     int apply_dirichlet) {
-  __shared__ float s_cv[4][kStage];
-  __shared__ float s_cg[4][3][kStage];
-  __shared__ float s_rua[kStage];
+  __shared__ real s_cv[4][kStage];
+  __shared__ real s_cg[4][3][kStage];
+  __shared__ real s_rua[kStage];
   const int C = m.n_cells, F = m.n_faces, Fi = m.n_internal;
   int a, b;
   item_range(bd.faces, F, a, b);
@@ -210,8 +210,8 @@ __global__ __launch_bounds__(kBlock) void k_face_interp_lds(
   const bool in = live && f < Fi;
   // this face's indices and its neighbour's data
   int o = 0, n = 0;
-  float psi = 0.f, s0 = 0.f, s1 = 0.f, s2 = 0.f;
-  float cn[4], gn[4][3], old[3], nr[3], ar = 0.f, rn = 0.f;
+  real psi = 0.f, s0 = 0.f, s1 = 0.f, s2 = 0.f;
+  real cn[4], gn[4][3], old[3], nr[3], ar = 0.f, rn = 0.f;
   if (in) {
     o = m.owner[f];
     n = m.neigh[f];
@@ -224,7 +224,7 @@ __global__ __launch_bounds__(kBlock) void k_face_interp_lds(
       if ((mask >> q) & 1) {
         cn[q] = cv[q * C + n];
         if ((skew_mask >> q) & 1) {
-          const float* gq = cg + (size_t)(3 * q) * C;
+          const real* gq = cg + (size_t)(3 * q) * C;
 #pragma unroll
           for (int d = 0; d < 3; ++d) gn[q][d] = gq[d * C + n];
         }
@@ -248,7 +248,7 @@ __global__ __launch_bounds__(kBlock) void k_face_interp_lds(
       if ((mask >> q) & 1) {
         s_cv[q][i] = cv[q * C + c];
         if ((skew_mask >> q) & 1) {
-          const float* gq = cg + (size_t)(3 * q) * C;
+          const real* gq = cg + (size_t)(3 * q) * C;
 #pragma unroll
           for (int d = 0; d < 3; ++d) s_cg[q][d][i] = gq[d * C + c];
         }
@@ -260,29 +260,29 @@ __global__ __launch_bounds__(kBlock) void k_face_interp_lds(
   if (in) {
     const int j = o - o_lo;
     const bool st = j >= 0 && j < R;
-    float co[4], go[4][3], ro = 0.f;
+    real co[4], go[4][3], ro = 0.f;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       if ((mask >> q) & 1) {
         co[q] = st ? s_cv[q][j] : cv[q * C + o];
         if ((skew_mask >> q) & 1) {
-          const float* gq = cg + (size_t)(3 * q) * C;
+          const real* gq = cg + (size_t)(3 * q) * C;
 #pragma unroll
           for (int d = 0; d < 3; ++d) go[q][d] = st ? s_cg[q][d][j] : gq[d * C + o];
         }
       }
     }
     if (rua_f) ro = st ? s_rua[j] : rua[o];
-    const float psi2 = 1.0f - psi;
-    float val[4];
+    const real psi2 = 1.0f - psi;
+    real val[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       if ((mask >> q) & 1) {
-        float v = co[q] * psi + psi2 * cn[q];
+        real v = co[q] * psi + psi2 * cn[q];
         if ((skew_mask >> q) & 1) {
-          const float g0 = go[q][0] * psi + psi2 * gn[q][0];
-          const float g1 = go[q][1] * psi + psi2 * gn[q][1];
-          const float g2 = go[q][2] * psi + psi2 * gn[q][2];
+          const real g0 = go[q][0] * psi + psi2 * gn[q][0];
+          const real g1 = go[q][1] * psi + psi2 * gn[q][1];
+          const real g2 = go[q][2] * psi + psi2 * gn[q][2];
           v = v + (g0 * s0 + g1 * s1 + g2 * s2);
         }
         val[q] = v;
@@ -316,12 +316,12 @@ __device__ inline void slot_faces(const wcfd_mesh& m, uint32_t c4, uint32_t (&f4
 
 // model_kernels.py:72-90: grad_c = sum_k phi_f A_f n_{c,k} / V_c.
 __device__ inline void gg_field(const wcfd_mesh& m, uint32_t c4, const uint32_t (&f4)[6],
-                                const FBuf& phi_f, float V, float (&g)[3]) {
+                                const FBuf& phi_f, real V, real (&g)[3]) {
   const size_t C = m.n_cells;
-  float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+  real a0 = 0.f, a1 = 0.f, a2 = 0.f;
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
-    const float phi = phi_f(f4[k]);
+    const real phi = phi_f(f4[k]);
     a0 = a0 + (phi * ld_off(m.slot_sn + (0 * 6 + k) * C, c4)) / V;
     a1 = a1 + (phi * ld_off(m.slot_sn + (1 * 6 + k) * C, c4)) / V;
     a2 = a2 + (phi * ld_off(m.slot_sn + (2 * 6 + k) * C, c4)) / V;
@@ -329,22 +329,22 @@ __device__ inline void gg_field(const wcfd_mesh& m, uint32_t c4, const uint32_t 
   g[0] = a0; g[1] = a1; g[2] = a2;
 }
 
-__device__ inline FBuf face_plane(const wcfd_mesh& m, const float* a, int q) {
+__device__ inline FBuf face_plane(const wcfd_mesh& m, const real* a, int q) {
   return FBuf(a + (size_t)q * m.n_faces, (uint32_t)m.n_faces * 4u);
 }
 
-__global__ void k_green_gauss(wcfd_mesh m, Items items, const float* __restrict__ fv,
-                              float* __restrict__ cg, int mask, int zero_others) {
+__global__ void k_green_gauss(wcfd_mesh m, Items items, const real* __restrict__ fv,
+                              real* __restrict__ cg, int mask, int zero_others) {
   const int c = item_index(items);
   const int C = m.n_cells;
   if (c >= C) return;
   const uint32_t c4 = (uint32_t)c * 4u;
   uint32_t f4[6];
   slot_faces(m, c4, f4);
-  const float V = ld_off(m.volume, c4);
+  const real V = ld_off(m.volume, c4);
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    float g[3] = {0.f, 0.f, 0.f};
+    real g[3] = {0.f, 0.f, 0.f};
     if ((mask >> q) & 1) {
       gg_field(m, c4, f4, face_plane(m, fv, q), V, g);
     } else if (!zero_others) {
@@ -355,7 +355,7 @@ __global__ void k_green_gauss(wcfd_mesh m, Items items, const float* __restrict_
   }
 }
 
-__global__ void k_mass_flux(wcfd_mesh m, const float* __restrict__ fv, float* __restrict__ flux) {
+__global__ void k_mass_flux(wcfd_mesh m, const real* __restrict__ fv, real* __restrict__ flux) {
   const int f = xcd_item();
   const int F = m.n_faces;
   if (f >= F) return;
@@ -363,19 +363,19 @@ __global__ void k_mass_flux(wcfd_mesh m, const float* __restrict__ fv, float* __
             m.area[f];
 }
 
-__device__ inline float cell_div(const wcfd_mesh& m, uint32_t c4, const uint32_t (&f4)[6],
+__device__ inline real cell_div(const wcfd_mesh& m, uint32_t c4, const uint32_t (&f4)[6],
                                  const FBuf& flux) {
   const size_t C = m.n_cells;
-  float acc = 0.f;
+  real acc = 0.f;
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
-    const float fl = flux(f4[k]);
+    const real fl = flux(f4[k]);
     acc = acc + (ld_off(m.slot_side + k * C, c4 >> 2) == 0 ? fl : -fl);
   }
   return acc;
 }
 
-__global__ void k_divergence(wcfd_mesh m, const float* __restrict__ flux, float* __restrict__ div) {
+__global__ void k_divergence(wcfd_mesh m, const real* __restrict__ flux, real* __restrict__ div) {
   const int c = xcd_item();
   const int C = m.n_cells;
   if (c >= C) return;
@@ -387,17 +387,17 @@ __global__ void k_divergence(wcfd_mesh m, const float* __restrict__ flux, float*
 
 // ------------------------------------------------------ momentum assembly
 __global__ void k_assemble_momentum(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
-                                    const float* __restrict__ cv, const float* __restrict__ fv,
-                                    const float* __restrict__ flux, float* __restrict__ cg,
-                                    int compute_grad, float nu_scalar,
-                                    float inv_density, float alpha, float dt, int flags,
-                                    float* __restrict__ values, float* __restrict__ rhs) {
+                                    const real* __restrict__ cv, const real* __restrict__ fv,
+                                    const real* __restrict__ flux, real* __restrict__ cg,
+                                    int compute_grad, real nu_scalar,
+                                    real inv_density, real alpha, real dt, int flags,
+                                    real* __restrict__ values, real* __restrict__ rhs) {
   const int correction = flags & WCFD_SCHEME_CORRECTED;
   const bool implicit_outflow = (flags & WCFD_SCHEME_IMPLICIT_OUTFLOW) != 0;
   const int c = item_index(bd.cells);
   const int C = m.n_cells, Fi = m.n_internal, Fb = m.n_boundary;
   if (c >= C || bd.cell_skip(c)) return;
-  const float nu = bd.nu ? bd.nu[bd.cell_case[c]] : nu_scalar;
+  const real nu = bd.nu ? bd.nu[bd.cell_case[c]] : nu_scalar;
   const uint32_t c4 = (uint32_t)c * 4u, Fi4 = (uint32_t)Fi * 4u;
   const size_t Cz = C;
   uint32_t f4[6];
@@ -407,14 +407,14 @@ __global__ void k_assemble_momentum(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
   for (int k = 0; k < 6; ++k) side[k] = ld_off(m.slot_side + k * Cz, (uint32_t)c);
   const FBuf Fl_(flux, (uint32_t)m.n_faces * 4u);
   const FBuf area(m.area, (uint32_t)m.n_faces * 4u), dmag(m.dmag, (uint32_t)m.n_faces * 4u);
-  const float V = ld_off(m.volume, c4);
+  const real V = ld_off(m.volume, c4);
   // (storing the gradients at the end, after every load, measured 1.5 %
   // slower here, unlike the face and corrector kernels)
-  float gp[3];
+  real gp[3];
   if (compute_grad) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      float g[3];
+      real g[3];
       gg_field(m, c4, f4, face_plane(m, fv, q), V, g);
       st_plain(cg + (3 * q + 0) * Cz, c4, g[0]);
       st_plain(cg + (3 * q + 1) * Cz, c4, g[1]);
@@ -426,15 +426,15 @@ __global__ void k_assemble_momentum(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
     gp[1] = ld_off(cg + 10 * Cz, c4);
     gp[2] = ld_off(cg + 11 * Cz, c4);
   }
-  float u[3];
+  real u[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q) u[q] = ld_off(cv + q * Cz, c4);
-  float diag = 0.f;
-  float b[3] = {0.f, 0.f, 0.f};
-  float off[6];
+  real diag = 0.f;
+  real b[3] = {0.f, 0.f, 0.f};
+  real off[6];
   // DdtTerm first (incompressible.py:110): diag += V/dt, b += phi V/dt
   if (dt > 0.f) {
-    const float vdt = V / dt;
+    const real vdt = V / dt;
     diag = diag + vdt;
 #pragma unroll
     for (int q = 0; q < 3; ++q) b[q] = b[q] + (-(-u[q] * vdt));
@@ -446,7 +446,7 @@ __global__ void k_assemble_momentum(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
   unsigned outflow = 0u;
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
-    const float Fl = Fl_(f4[k]);
+    const real Fl = Fl_(f4[k]);
     off[k] = 0.f;
     if (f4[k] < Fi4) {
       const bool up = Fl > 0.f;
@@ -465,10 +465,10 @@ __global__ void k_assemble_momentum(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
     if (f4[k] >= Fi4) {
-      const float Fl = Fl_(f4[k]);
+      const real Fl = Fl_(f4[k]);
       if ((outflow >> k) & 1u) {
         const int j = (int)((f4[k] - Fi4) >> 2);
-        const float dm = dmag(f4[k]);
+        const real dm = dmag(f4[k]);
 #pragma unroll
         for (int q = 0; q < 3; ++q) b[q] = b[q] + (-(Fl * (dm * bc.value[q * Fb + j])));
       } else {
@@ -481,17 +481,17 @@ __global__ void k_assemble_momentum(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
     if (f4[k] < Fi4) {
-      const float w = lap_factor_b(m, dmag, f4[k], correction) * (nu * area(f4[k]));
+      const real w = lap_factor_b(m, dmag, f4[k], correction) * (nu * area(f4[k]));
       diag = diag + w;
       off[k] = off[k] + (-w);
     } else {
       const int j = (int)((f4[k] - Fi4) >> 2);
       if (bc.type[j] == WCFD_NEUMANN) {
-        const float A = area(f4[k]);
+        const real A = area(f4[k]);
 #pragma unroll
         for (int q = 0; q < 3; ++q) b[q] = b[q] + (nu * bc.value[q * Fb + j]) * A;
       } else {
-        const float w = (nu * area(f4[k])) / dmag(f4[k]);
+        const real w = (nu * area(f4[k])) / dmag(f4[k]);
         diag = diag + w;
 #pragma unroll
         for (int q = 0; q < 3; ++q) b[q] = b[q] + w * face_plane(m, fv, q)(f4[k]);
@@ -503,7 +503,7 @@ __global__ void k_assemble_momentum(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
   for (int q = 0; q < 3; ++q) b[q] = b[q] + (gp[q] * V) * (-inv_density);
   // implicit under-relaxation (equation_kernels.py:128-152)
   if (alpha > 0.f) {
-    const float r = (1.0f - alpha) / alpha;
+    const real r = (1.0f - alpha) / alpha;
 #pragma unroll
     for (int q = 0; q < 3; ++q) b[q] = b[q] + r * diag * u[q];
     diag = diag / alpha;
@@ -518,12 +518,12 @@ __global__ void k_assemble_momentum(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
 
 // ------------------------------------------------------ pressure assembly
 __global__ void k_assemble_pressure(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
-                                    const float* __restrict__ fv,
-                                    const float* __restrict__ flux,
-                                    const float* __restrict__ rua_f, float* __restrict__ cg,
+                                    const real* __restrict__ fv,
+                                    const real* __restrict__ flux,
+                                    const real* __restrict__ rua_f, real* __restrict__ cg,
                                     int compute_grad_p, int correction, int ref_cell,
-                                    float ref_value, float* __restrict__ values,
-                                    float* __restrict__ rhs, float* __restrict__ div_out) {
+                                    real ref_value, real* __restrict__ values,
+                                    real* __restrict__ rhs, real* __restrict__ div_out) {
   const int c = item_index(bd.cells);
   const int C = m.n_cells, Fi = m.n_internal, Fb = m.n_boundary;
   if (c >= C || bd.cell_skip(c)) return;
@@ -538,22 +538,22 @@ __global__ void k_assemble_pressure(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
   slot_faces(m, c4, f4);
   const FBuf pf = face_plane(m, fv, 3);
   if (compute_grad_p) {
-    float g[3];
+    real g[3];
     gg_field(m, c4, f4, pf, ld_off(m.volume, c4), g);
     st_plain(cg + 9 * Cz, c4, g[0]);
     st_plain(cg + 10 * Cz, c4, g[1]);
     st_plain(cg + 11 * Cz, c4, g[2]);
   }
-  const float div = cell_div(m, c4, f4, FBuf(flux, F4));
+  const real div = cell_div(m, c4, f4, FBuf(flux, F4));
   if (div_out) st_plain(div_out, c4, div);
   const FBuf ruaf(rua_f, F4), area(m.area, F4), dmag(m.dmag, F4);
-  float diag = 0.f, b = 0.f;
-  float off[6];
+  real diag = 0.f, b = 0.f;
+  real off[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
     off[k] = 0.f;
     if (f4[k] < Fi4) {
-      const float w = lap_factor_b(m, dmag, f4[k], correction) * (ruaf(f4[k]) * area(f4[k]));
+      const real w = lap_factor_b(m, dmag, f4[k], correction) * (ruaf(f4[k]) * area(f4[k]));
       diag = diag + (-w);
       off[k] = w;
     } else {
@@ -561,7 +561,7 @@ __global__ void k_assemble_pressure(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
       if (bc.type[3 * Fb + j] == WCFD_NEUMANN) {
         b = b + (-((ruaf(f4[k]) * bc.value[3 * Fb + j]) * area(f4[k])));
       } else {
-        const float w = (ruaf(f4[k]) * area(f4[k])) / dmag(f4[k]);
+        const real w = (ruaf(f4[k]) * area(f4[k])) / dmag(f4[k]);
         diag = diag + (-w);
         b = b + (-(w * pf(f4[k])));
       }
@@ -581,14 +581,14 @@ __global__ void k_assemble_pressure(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
 }
 
 // ------------------------------------------------------------ rUA, HbyA
-__global__ void k_rua_hbya(wcfd_mesh m, BatchDev bd, const float* __restrict__ values,
-                           float density, const float* __restrict__ cg, float* __restrict__ hbya,
-                           float* __restrict__ cv, float* __restrict__ rua,
-                           float* __restrict__ u_prev) {
+__global__ void k_rua_hbya(wcfd_mesh m, BatchDev bd, const real* __restrict__ values,
+                           real density, const real* __restrict__ cg, real* __restrict__ hbya,
+                           real* __restrict__ cv, real* __restrict__ rua,
+                           real* __restrict__ u_prev) {
   const int c = item_index(bd.cells);
   const int C = m.n_cells;
   if (c >= C || bd.cell_skip(c)) return;
-  const float r = m.volume[c] / (values[c] * density);
+  const real r = m.volume[c] / (values[c] * density);
   rua[c] = r;
   if (u_prev) {   // u of the previous outer iteration, for the velocity change
 #pragma unroll
@@ -596,25 +596,25 @@ __global__ void k_rua_hbya(wcfd_mesh m, BatchDev bd, const float* __restrict__ v
   }
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    const float h = hbya[q * C + c] + cg[(9 + q) * C + c] * r;
+    const real h = hbya[q * C + c] + cg[(9 + q) * C + c] * r;
     hbya[q * C + c] = h;
     cv[q * C + c] = h;
   }
 }
 
-__global__ void k_relax_p(BatchDev bd, int C, const float* __restrict__ pn, float alpha,
-                          float* __restrict__ cv) {
+__global__ void k_relax_p(BatchDev bd, int C, const real* __restrict__ pn, real alpha,
+                          real* __restrict__ cv) {
   const int c = item_index(bd.cells);
   if (c >= C || bd.cell_skip(c)) return;
-  float* p = cv + (size_t)3 * C;
+  real* p = cv + (size_t)3 * C;
   p[c] = alpha > 0.f ? alpha * pn[c] + (1.0f - alpha) * p[c] : pn[c];
 }
 
 // face_interpolation('p') + the corrected conservative flux F_c per face.
-__global__ void k_face_p_fc(wcfd_mesh m, BatchDev bd, wcfd_bc bc, const float* __restrict__ cv,
-                            const float* __restrict__ cg, float* __restrict__ fv,
-                            const float* __restrict__ flux, const float* __restrict__ rua_f,
-                            float* __restrict__ fc) {
+__global__ void k_face_p_fc(wcfd_mesh m, BatchDev bd, wcfd_bc bc, const real* __restrict__ cv,
+                            const real* __restrict__ cg, real* __restrict__ fv,
+                            const real* __restrict__ flux, const real* __restrict__ rua_f,
+                            real* __restrict__ fc) {
   const int f = item_index(bd.faces);
   const int F = m.n_faces, Fi = m.n_internal, Fb = m.n_boundary;
   if (f >= F || bd.face_skip(f)) return;
@@ -623,14 +623,14 @@ __global__ void k_face_p_fc(wcfd_mesh m, BatchDev bd, wcfd_bc bc, const float* _
   const FBuf p = cell_plane(m, cv, 3);
   if (f < Fi) {
     const uint32_t o4 = (uint32_t)ld_off(m.owner, f4) * 4u, n4 = (uint32_t)ld_off(m.neigh, f4) * 4u;
-    const float psi = ld_off(m.psi, f4);
-    const float psi2 = 1.0f - psi;
+    const real psi = ld_off(m.psi, f4);
+    const real psi2 = 1.0f - psi;
     const FBuf g0b = cell_plane(m, cg, 9), g1b = cell_plane(m, cg, 10), g2b = cell_plane(m, cg, 11);
-    const float po = p(o4), pn = p(n4);
-    float v = po * psi + psi2 * pn;
-    float g0 = g0b(o4) * psi + psi2 * g0b(n4);
-    float g1 = g1b(o4) * psi + psi2 * g1b(n4);
-    float g2 = g2b(o4) * psi + psi2 * g2b(n4);
+    const real po = p(o4), pn = p(n4);
+    real v = po * psi + psi2 * pn;
+    real g0 = g0b(o4) * psi + psi2 * g0b(n4);
+    real g1 = g1b(o4) * psi + psi2 * g1b(n4);
+    real g2 = g2b(o4) * psi + psi2 * g2b(n4);
     v = v + (g0 * ld_off(m.skew, f4) + g1 * ld_off(m.skew + Fiz, f4) +
              g2 * ld_off(m.skew + 2 * Fiz, f4));
     st_plain(fv + 3 * Fz, f4, v);
@@ -644,7 +644,7 @@ __global__ void k_face_p_fc(wcfd_mesh m, BatchDev bd, wcfd_bc bc, const float* _
     if (t == WCFD_NEUMANN)
       st_plain(fv + 3 * Fz, f4, ld_off(m.dmag, f4) * bc.value[3 * Fb + j] + p(p4));
     if (fc) {
-      float F_ = ld_off(flux, f4);
+      real F_ = ld_off(flux, f4);
       if (t == WCFD_DIRICHLET)
         F_ = F_ - (ld_off(rua_f, f4) * ld_off(m.area, f4) / ld_off(m.dmag, f4)) *
                       (bc.value[3 * Fb + j] - p(p4));
@@ -657,10 +657,10 @@ __global__ void k_face_p_fc(wcfd_mesh m, BatchDev bd, wcfd_bc bc, const float* _
 // the float64 continuity partial sums of the corrected flux: one pair per
 // workgroup of the segmentation (one case, or a batch), folded per case by
 // k_fold_resid.
-__global__ void k_grad_p_correct(wcfd_mesh m, Seg sg, const float* __restrict__ fv,
-                                 float* __restrict__ cg, float* __restrict__ cv,
-                                 float* __restrict__ hbya, const float* __restrict__ rua,
-                                 const float* __restrict__ fc, const float* __restrict__ u_prev,
+__global__ void k_grad_p_correct(wcfd_mesh m, Seg sg, const real* __restrict__ fv,
+                                 real* __restrict__ cg, real* __restrict__ cv,
+                                 real* __restrict__ hbya, const real* __restrict__ rua,
+                                 const real* __restrict__ fc, const real* __restrict__ u_prev,
                                  double* __restrict__ partials) {
   const size_t Cz = m.n_cells;
   const BlockRows br = block_rows(sg);
@@ -674,16 +674,16 @@ __global__ void k_grad_p_correct(wcfd_mesh m, Seg sg, const float* __restrict__ 
     slot_faces(m, c4, f4);
     // every load (the cell's gradient sums, rUA, HbyA, the previous
     // velocity, the corrected face fluxes) before the first store
-    float g[3];
+    real g[3];
     gg_field(m, c4, f4, pf, ld_off(m.volume, c4), g);
-    const float r = ld_off(rua, c4);
-    float hb[3], up[3];
+    const real r = ld_off(rua, c4);
+    real hb[3], up[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       hb[q] = ld_off(hbya + q * Cz, c4);
       up[q] = (fc && u_prev) ? ld_off(u_prev + q * Cz, c4) : 0.f;
     }
-    float fk[6];
+    real fk[6];
     int sd[6];
     if (fc) {
 #pragma unroll
@@ -692,7 +692,7 @@ __global__ void k_grad_p_correct(wcfd_mesh m, Seg sg, const float* __restrict__ 
         sd[k] = ld_off(m.slot_side + k * Cz, (uint32_t)c);
       }
     }
-    float h[3];
+    real h[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       h[q] = hb[q] - r * g[q];
@@ -714,7 +714,7 @@ __global__ void k_grad_p_correct(wcfd_mesh m, Seg sg, const float* __restrict__ 
       acc[1] += den;
     }
 #pragma unroll
-    for (int i = 0; i < 9; ++i) st_plain(cg + i * Cz, c4, 0.f);
+    for (int i = 0; i < 9; ++i) st_plain(cg + i * Cz, c4, (real)0);
     st_plain(cg + 9 * Cz, c4, g[0]);
     st_plain(cg + 10 * Cz, c4, g[1]);
     st_plain(cg + 11 * Cz, c4, g[2]);
@@ -759,7 +759,7 @@ __global__ void k_fold_resid(Seg sg, const double* __restrict__ partials,
 }
 
 // ELL (slot-major) values -> values on the CSR pattern (row_ptr, col).
-__global__ void k_ell_to_csr(wcfd_mesh m, const float* __restrict__ ell, float* __restrict__ csr) {
+__global__ void k_ell_to_csr(wcfd_mesh m, const real* __restrict__ ell, real* __restrict__ csr) {
   const int c = xcd_item();
   const int C = m.n_cells;
   if (c >= C) return;
@@ -772,7 +772,7 @@ __global__ void k_ell_to_csr(wcfd_mesh m, const float* __restrict__ ell, float* 
 }
 
 // ========================================================== host launchers
-int launch_set_bc(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc, float* fv, int mask,
+int launch_set_bc(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc, real* fv, int mask,
                   hipStream_t s) {
   const unsigned g = items_grid(bd.bfaces, m.n_boundary);
   if (g == 0) return WCFD_OK;
@@ -781,9 +781,9 @@ int launch_set_bc(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc, float* fv,
   return check_launch();
 }
 
-int launch_face_interp(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc, const float* cv,
-                       const float* cg, float* fv, int mask, int skew_mask, float* flux,
-                       const float* rua, float* rua_f, int apply_dirichlet, hipStream_t s) {
+int launch_face_interp(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc, const real* cv,
+                       const real* cg, real* fv, int mask, int skew_mask, real* flux,
+                       const real* rua, real* rua_f, int apply_dirichlet, hipStream_t s) {
   const unsigned g = items_grid(bd.faces, m.n_faces);
   if (g == 0) return WCFD_OK;
 #if WCFD_FACE_LDS
@@ -796,7 +796,7 @@ int launch_face_interp(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc, const
   return check_launch();
 }
 
-int launch_green_gauss(const wcfd_mesh& m, const BatchDev& bd, const float* fv, float* cg,
+int launch_green_gauss(const wcfd_mesh& m, const BatchDev& bd, const real* fv, real* cg,
                        int mask, int zero_others, hipStream_t s) {
   const unsigned g = items_grid(bd.cells, m.n_cells);
   if (g == 0) return WCFD_OK;
@@ -804,10 +804,10 @@ int launch_green_gauss(const wcfd_mesh& m, const BatchDev& bd, const float* fv, 
   return check_launch();
 }
 
-int launch_assemble_momentum(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc, const float* cv,
-                             const float* fv, const float* flux, float* cg, int compute_grad,
-                             float nu, float inv_density, float alpha, float dt, int flags,
-                             float* values, float* rhs, hipStream_t s) {
+int launch_assemble_momentum(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc, const real* cv,
+                             const real* fv, const real* flux, real* cg, int compute_grad,
+                             real nu, real inv_density, real alpha, real dt, int flags,
+                             real* values, real* rhs, hipStream_t s) {
   const unsigned g = items_grid(bd.cells, m.n_cells);
   if (g == 0) return WCFD_OK;
   WCFD_LAUNCH(K_ASM_MOM, s, k_assemble_momentum, g, m, bd, bc, cv, fv, flux, cg, compute_grad, nu,
@@ -815,10 +815,10 @@ int launch_assemble_momentum(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc,
   return check_launch();
 }
 
-int launch_assemble_pressure(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc, const float* fv,
-                             const float* flux, const float* rua_f, float* cg, int compute_grad_p,
-                             int correction, int ref_cell, float ref_value, float* values,
-                             float* rhs, float* div_out, hipStream_t s) {
+int launch_assemble_pressure(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc, const real* fv,
+                             const real* flux, const real* rua_f, real* cg, int compute_grad_p,
+                             int correction, int ref_cell, real ref_value, real* values,
+                             real* rhs, real* div_out, hipStream_t s) {
   const unsigned g = items_grid(bd.cells, m.n_cells);
   if (g == 0) return WCFD_OK;
   WCFD_LAUNCH(K_ASM_P, s, k_assemble_pressure, g, m, bd, bc, fv, flux, rua_f, cg, compute_grad_p,
@@ -826,8 +826,8 @@ int launch_assemble_pressure(const wcfd_mesh& m, const BatchDev& bd, wcfd_bc bc,
   return check_launch();
 }
 
-int launch_rua_hbya(const wcfd_mesh& m, const BatchDev& bd, const float* values, float density,
-                    const float* cg, float* hbya, float* cv, float* rua, float* u_prev,
+int launch_rua_hbya(const wcfd_mesh& m, const BatchDev& bd, const real* values, real density,
+                    const real* cg, real* hbya, real* cv, real* rua, real* u_prev,
                     hipStream_t s) {
   const unsigned g = items_grid(bd.cells, m.n_cells);
   if (g == 0) return WCFD_OK;
@@ -835,7 +835,7 @@ int launch_rua_hbya(const wcfd_mesh& m, const BatchDev& bd, const float* values,
   return check_launch();
 }
 
-int launch_relax_p(const wcfd_mesh& m, const BatchDev& bd, const float* pn, float alpha, float* cv,
+int launch_relax_p(const wcfd_mesh& m, const BatchDev& bd, const real* pn, real alpha, real* cv,
                    hipStream_t s) {
   const unsigned g = items_grid(bd.cells, m.n_cells);
   if (g == 0) return WCFD_OK;
@@ -844,11 +844,11 @@ int launch_relax_p(const wcfd_mesh& m, const BatchDev& bd, const float* pn, floa
 }
 
 int launch_correct_velocity(const wcfd_mesh& m, const BatchDev& bd, const Seg& sg, int block,
-                            const wcfd_batch* bt, wcfd_bc bc, float* cv, float* cg, float* fv,
-                            const float* flux, const float* rua, const float* rua_f, float* hbya,
-                            const float* u_prev, double* resid, float* fc_scratch,
+                            const wcfd_batch* bt, wcfd_bc bc, real* cv, real* cg, real* fv,
+                            const real* flux, const real* rua, const real* rua_f, real* hbya,
+                            const real* u_prev, double* resid, real* fc_scratch,
                             double* partials, hipStream_t s) {
-  float* fc = resid ? fc_scratch : nullptr;
+  real* fc = resid ? fc_scratch : nullptr;
   const unsigned g = items_grid(bd.faces, m.n_faces);
   if (g > 0)
     WCFD_LAUNCH(K_FACE_P_FC, s, k_face_p_fc, g, m, bd, bc, cv, cg, fv, flux, rua_f, fc);
@@ -868,10 +868,15 @@ int launch_correct_velocity(const wcfd_mesh& m, const BatchDev& bd, const Seg& s
 // ============================================================== C ABI
 using namespace wcfd;
 
+// Kernels address field arrays with 32-bit byte offsets per lane, and the
+// widest buffer view spans three components of a face array: meshes whose
+// 3 max(C, F) reals reach 2^32 bytes are refused.
 bool wcfd::mesh_ok(const wcfd_mesh* m) {
-  return m && m->n_cells > 0 && m->n_faces >= m->n_internal && m->n_internal >= 0 &&
+  if (!m || m->n_cells <= 0) return false;
+  const uint64_t widest = (uint64_t)(m->n_faces > m->n_cells ? m->n_faces : m->n_cells);
+  return m->n_faces >= m->n_internal && m->n_internal >= 0 &&
          m->n_boundary == m->n_faces - m->n_internal && m->owner && m->slot_face && m->volume &&
-         slots_ok(m);
+         slots_ok(m) && 3u * widest * sizeof(real) < (1ull << 32);
 }
 
 extern "C" {
@@ -894,51 +899,53 @@ const char* wcfd_error_string(int code) {
 
 int wcfd_abi_version(void) { return 3; }
 
+int wcfd_real_bytes(void) { return (int)sizeof(real); }
+
 int wcfd_monitor_update(double* record, double continuity, double velocity_change,
                         double stop_c, double stop_u, double floor_u) {
   if (!record) return -WCFD_EINVAL;
   return monitor_update(record, continuity, velocity_change, stop_c, stop_u, floor_u) ? 1 : 0;
 }
 
-int wcfd_set_boundary_conditions(const wcfd_mesh* m, wcfd_bc bc, float* face_values,
+int wcfd_set_boundary_conditions(const wcfd_mesh* m, wcfd_bc bc, real* face_values,
                                  int field_mask, void* stream) {
   if (!mesh_ok(m) || !face_values || !bc.type || !bc.value) return WCFD_EINVAL;
   return launch_set_bc(*m, BatchDev{}, bc, face_values, field_mask, (hipStream_t)stream);
 }
 
-int wcfd_face_interpolation(const wcfd_mesh* m, wcfd_bc bc, const float* cell_values,
-                            const float* cell_grad, float* face_values, int field_mask,
-                            int skew_mask, float* mass_flux, const float* rua_cell,
-                            float* rua_face, void* stream) {
+int wcfd_face_interpolation(const wcfd_mesh* m, wcfd_bc bc, const real* cell_values,
+                            const real* cell_grad, real* face_values, int field_mask,
+                            int skew_mask, real* mass_flux, const real* rua_cell,
+                            real* rua_face, void* stream) {
   if (!mesh_ok(m) || !cell_values || !cell_grad || !face_values) return WCFD_EINVAL;
   if ((rua_cell == nullptr) != (rua_face == nullptr)) return WCFD_EINVAL;
   return launch_face_interp(*m, BatchDev{}, bc, cell_values, cell_grad, face_values, field_mask & 15,
                             skew_mask & 15, mass_flux, rua_cell, rua_face, 0, (hipStream_t)stream);
 }
 
-int wcfd_green_gauss(const wcfd_mesh* m, const float* face_values, float* cell_grad, int field_mask,
+int wcfd_green_gauss(const wcfd_mesh* m, const real* face_values, real* cell_grad, int field_mask,
                      int zero_others, void* stream) {
   if (!mesh_ok(m) || !face_values || !cell_grad) return WCFD_EINVAL;
   return launch_green_gauss(*m, BatchDev{}, face_values, cell_grad, field_mask & 15, zero_others,
                             (hipStream_t)stream);
 }
 
-int wcfd_mass_flux(const wcfd_mesh* m, const float* face_values, float* mass_flux, void* stream) {
+int wcfd_mass_flux(const wcfd_mesh* m, const real* face_values, real* mass_flux, void* stream) {
   if (!mesh_ok(m) || !face_values || !mass_flux) return WCFD_EINVAL;
   WCFD_LAUNCH(K_MASS_FLUX, (hipStream_t)stream, k_mass_flux, grid_for(m->n_faces), *m, face_values, mass_flux);
   return check_launch();
 }
 
-int wcfd_divergence(const wcfd_mesh* m, const float* mass_flux, float* div, void* stream) {
+int wcfd_divergence(const wcfd_mesh* m, const real* mass_flux, real* div, void* stream) {
   if (!mesh_ok(m) || !mass_flux || !div) return WCFD_EINVAL;
   WCFD_LAUNCH(K_DIVERGENCE, (hipStream_t)stream, k_divergence, grid_for(m->n_cells), *m, mass_flux, div);
   return check_launch();
 }
 
-int wcfd_assemble_momentum(const wcfd_mesh* m, wcfd_bc bc, const float* cell_values,
-                           const float* face_values, const float* mass_flux, float* cell_grad,
+int wcfd_assemble_momentum(const wcfd_mesh* m, wcfd_bc bc, const real* cell_values,
+                           const real* face_values, const real* mass_flux, real* cell_grad,
                            int compute_gradients, float nu, float inv_density, float alpha_u,
-                           float dt, int scheme_flags, float* values, float* rhs, void* stream) {
+                           float dt, int scheme_flags, real* values, real* rhs, void* stream) {
   if (!mesh_ok(m) || !cell_values || !face_values || !mass_flux || !cell_grad || !values || !rhs)
     return WCFD_EINVAL;
   return launch_assemble_momentum(*m, BatchDev{}, bc, cell_values, face_values, mass_flux, cell_grad,
@@ -946,10 +953,10 @@ int wcfd_assemble_momentum(const wcfd_mesh* m, wcfd_bc bc, const float* cell_val
                                   values, rhs, (hipStream_t)stream);
 }
 
-int wcfd_assemble_pressure(const wcfd_mesh* m, wcfd_bc bc, const float* face_values,
-                           const float* mass_flux, const float* rua_face, float* cell_grad,
+int wcfd_assemble_pressure(const wcfd_mesh* m, wcfd_bc bc, const real* face_values,
+                           const real* mass_flux, const real* rua_face, real* cell_grad,
                            int compute_grad_p, int correction, int ref_cell, float ref_value,
-                           float* values, float* rhs, float* div_out, void* stream) {
+                           real* values, real* rhs, real* div_out, void* stream) {
   if (!mesh_ok(m) || !face_values || !mass_flux || !rua_face || !values || !rhs) return WCFD_EINVAL;
   if (compute_grad_p && !cell_grad) return WCFD_EINVAL;
   if (ref_cell >= m->n_cells) return WCFD_EINVAL;
@@ -958,20 +965,20 @@ int wcfd_assemble_pressure(const wcfd_mesh* m, wcfd_bc bc, const float* face_val
                                   div_out, (hipStream_t)stream);
 }
 
-int wcfd_rua_hbya(const wcfd_mesh* m, const float* values, float density, const float* cell_grad,
-                  float* hbya, float* cell_values, float* rua, void* stream) {
+int wcfd_rua_hbya(const wcfd_mesh* m, const real* values, float density, const real* cell_grad,
+                  real* hbya, real* cell_values, real* rua, void* stream) {
   if (!mesh_ok(m) || !values || !cell_grad || !hbya || !cell_values || !rua) return WCFD_EINVAL;
   return launch_rua_hbya(*m, BatchDev{}, values, density, cell_grad, hbya, cell_values, rua, nullptr,
                          (hipStream_t)stream);
 }
 
-int wcfd_relax_pressure(const wcfd_mesh* m, const float* p_new, float alpha, float* cell_values,
+int wcfd_relax_pressure(const wcfd_mesh* m, const real* p_new, float alpha, real* cell_values,
                         void* stream) {
   if (!mesh_ok(m) || !p_new || !cell_values) return WCFD_EINVAL;
   return launch_relax_p(*m, BatchDev{}, p_new, alpha, cell_values, (hipStream_t)stream);
 }
 
-int wcfd_ell_to_csr(const wcfd_mesh* m, const float* ell_values, float* csr_values, void* stream) {
+int wcfd_ell_to_csr(const wcfd_mesh* m, const real* ell_values, real* csr_values, void* stream) {
   if (!mesh_ok(m) || !m->slot_pos || !m->diag_pos || !ell_values || !csr_values) return WCFD_EINVAL;
   hipStream_t s = (hipStream_t)stream;
   WCFD_LAUNCH(K_ELL_TO_CSR, s, k_ell_to_csr, grid_for(m->n_cells), *m, ell_values, csr_values);
@@ -983,13 +990,13 @@ int wcfd_ell_to_csr(const wcfd_mesh* m, const float* ell_values, float* csr_valu
 size_t wcfd_reduce_workspace_bytes(int32_t n_faces) {
   Carver c(nullptr);
   c.take<double>(4 * (size_t)cell_reduce_blocks(n_faces));
-  c.take<float>(n_faces > 0 ? n_faces : 1);
+  c.take<real>(n_faces > 0 ? n_faces : 1);
   return c.off;
 }
 
-int wcfd_correct_velocity(const wcfd_mesh* m, wcfd_bc bc, float* cell_values, float* cell_grad,
-                          float* face_values, const float* mass_flux, const float* rua,
-                          const float* rua_face, float* hbya, double* resid, void* workspace,
+int wcfd_correct_velocity(const wcfd_mesh* m, wcfd_bc bc, real* cell_values, real* cell_grad,
+                          real* face_values, const real* mass_flux, const real* rua,
+                          const real* rua_face, real* hbya, double* resid, void* workspace,
                           void* stream) {
   if (!mesh_ok(m) || !cell_values || !cell_grad || !face_values || !rua || !hbya || !rua_face)
     return WCFD_EINVAL;
@@ -997,7 +1004,7 @@ int wcfd_correct_velocity(const wcfd_mesh* m, wcfd_bc bc, float* cell_values, fl
   if (cell_reduce_blocks(m->n_cells) > cell_reduce_blocks(m->n_faces)) return WCFD_EINVAL;
   Carver w(workspace);
   double* partials = w.take<double>(4 * (size_t)cell_reduce_blocks(m->n_faces));
-  float* fc = w.take<float>(m->n_faces);
+  real* fc = w.take<real>(m->n_faces);
   const Seg sg = single_seg(m->n_cells, kBlock, (int)cell_reduce_blocks(m->n_cells));
   return launch_correct_velocity(*m, BatchDev{}, sg, kBlock, nullptr, bc, cell_values, cell_grad,
                                  face_values, mass_flux, rua, rua_face, hbya, nullptr, resid, fc,
@@ -1058,9 +1065,9 @@ __device__ inline void conv_block(double (&v)[kConvN], double* __restrict__ out,
 // Rows are segmented per case as in the Krylov kernels (Seg): workgroup vb
 // covers its chunk of one case's cells, so each case gets its own partials.
 __global__ __launch_bounds__(kBlock) void k_conv_cells(
-    wcfd_mesh m, Seg sg, const float* __restrict__ val, const float* __restrict__ u,
-    const float* __restrict__ b, const float* __restrict__ rua, const float* __restrict__ corr,
-    const float* __restrict__ p, const float* __restrict__ flux, double* __restrict__ part) {
+    wcfd_mesh m, Seg sg, const real* __restrict__ val, const real* __restrict__ u,
+    const real* __restrict__ b, const real* __restrict__ rua, const real* __restrict__ corr,
+    const real* __restrict__ p, const real* __restrict__ flux, double* __restrict__ part) {
   const int C = m.n_cells;
   const BlockRows br = block_rows(sg);
   double acc[kConvN];
@@ -1088,16 +1095,16 @@ __global__ __launch_bounds__(kBlock) void k_conv_cells(
     if (corr) {
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        const float c = rua ? rua[i] * corr[(size_t)k * C + i] : corr[(size_t)k * C + i];
+        const real c = rua ? rua[i] * corr[(size_t)k * C + i] : corr[(size_t)k * C + i];
         acc[6] = max_nan(acc[6], fabs((double)c));
       }
     }
     if (p) acc[7] = max_nan(acc[7], fabs((double)p[i]));
     if (flux) {
       const int ns = m.n_slots ? m.n_slots : 6;
-      float d = 0.f;
+      real d = 0.f;
       for (int s = 0; s < ns; ++s) {
-        const float fl = flux[m.slot_face[(size_t)s * C + i]];
+        const real fl = flux[m.slot_face[(size_t)s * C + i]];
         d = m.slot_side[(size_t)s * C + i] == 0 ? d + fl : d - fl;
       }
       acc[8] += fabs((double)d);
@@ -1134,9 +1141,9 @@ __global__ __launch_bounds__(kBlock) void k_conv_fold(Seg sg, const double* __re
   }
 }
 
-int launch_conv(const wcfd_mesh& m, const Seg& sg, const float* values, const float* u,
-                const float* b, const float* rua, const float* corr, const float* p,
-                const float* flux, double* part, double* out, hipStream_t s) {
+int launch_conv(const wcfd_mesh& m, const Seg& sg, const real* values, const real* u,
+                const real* b, const real* rua, const real* corr, const real* p,
+                const real* flux, double* part, double* out, hipStream_t s) {
   if (sg.G > 0)
     WCFD_LAUNCH(K_CONVERGENCE, s, k_conv_cells, sg.G, m, sg, values, u, b, rua, corr, p, flux,
                 part);
@@ -1144,8 +1151,8 @@ int launch_conv(const wcfd_mesh& m, const Seg& sg, const float* values, const fl
   return check_launch();
 }
 
-bool conv_args_ok(const wcfd_mesh* m, const float* values, const float* u, const float* b,
-                  const float* rua, const float* corr, const float* flux, const void* workspace,
+bool conv_args_ok(const wcfd_mesh* m, const real* values, const real* u, const real* b,
+                  const real* rua, const real* corr, const real* flux, const void* workspace,
                   const double* out) {
   if (!mesh_ok(m) || !workspace || !out) return false;
   if (values && (!u || !b || !m->slot_nbr)) return false;
@@ -1163,9 +1170,9 @@ size_t wcfd_convergence_workspace_bytes(void) {
   return sizeof(double) * (size_t)kConvN * (size_t)kConvMaxBlocks;
 }
 
-int wcfd_convergence_report(const wcfd_mesh* m, const float* values, const float* u,
-                            const float* b, const float* rua, const float* corr, const float* p,
-                            const float* flux, void* workspace, double* out, void* stream) {
+int wcfd_convergence_report(const wcfd_mesh* m, const real* values, const real* u,
+                            const real* b, const real* rua, const real* corr, const real* p,
+                            const real* flux, void* workspace, double* out, void* stream) {
   if (!conv_args_ok(m, values, u, b, rua, corr, flux, workspace, out)) return WCFD_EINVAL;
   const Seg sg = single_seg(m->n_cells, kBlock, kConvMaxBlocks);
   return launch_conv(*m, sg, values, u, b, rua, corr, p, flux, (double*)workspace, out,
@@ -1176,9 +1183,9 @@ size_t wcfd_batch_convergence_workspace_bytes(int32_t n_blocks) {
   return sizeof(double) * (size_t)kConvN * (size_t)(n_blocks > 0 ? n_blocks : 1);
 }
 
-int wcfd_batch_convergence_report(const wcfd_mesh* m, const wcfd_batch* bt, const float* values,
-                                  const float* u, const float* b, const float* rua,
-                                  const float* corr, const float* p, const float* flux,
+int wcfd_batch_convergence_report(const wcfd_mesh* m, const wcfd_batch* bt, const real* values,
+                                  const real* u, const real* b, const real* rua,
+                                  const real* corr, const real* p, const real* flux,
                                   void* workspace, double* out, void* stream) {
   if (!conv_args_ok(m, values, u, b, rua, corr, flux, workspace, out)) return WCFD_EINVAL;
   if (!batch_ok(m, bt)) return WCFD_EINVAL;
@@ -1226,9 +1233,9 @@ int scratch_address(double** out) {
 extern "C" {
 
 int wcfd_batch_convergence_report_noalloc(const wcfd_mesh* m, const wcfd_batch* bt,
-                                          const float* values, const float* u, const float* b,
-                                          const float* rua, const float* corr, const float* p,
-                                          const float* flux, double* out, void* stream) {
+                                          const real* values, const real* u, const real* b,
+                                          const real* rua, const real* corr, const real* p,
+                                          const real* flux, double* out, void* stream) {
   // the workspace check of conv_args_ok is satisfied by a placeholder: the
   // partials go to the static array
   if (!conv_args_ok(m, values, u, b, rua, corr, flux, out, out)) return WCFD_EINVAL;

@@ -58,21 +58,21 @@ __device__ inline double safe_div(double a, double b) { return b != 0.0 ? a / b 
 
 // ------------------------------------------------------------ small kernels
 template <class M>
-__global__ void k_inv_diag(M A, Items items, int n, float* __restrict__ inv) {
+__global__ void k_inv_diag(M A, Items items, int n, real* __restrict__ inv) {
   const int i = item_index(items);
   if (i >= n) return;
-  const float d = A.diag(i);
+  const real d = A.diag(i);
   inv[i] = d != 0.0f ? 1.0f / d : 1.0f;
 }
 
 template <class M, int NV>
-__global__ void k_spmv(M A, int n, const float* __restrict__ x, float* __restrict__ y) {
+__global__ void k_spmv(M A, int n, const real* __restrict__ x, real* __restrict__ y) {
   const int i = xcd_item();
   if (i >= n) return;
-  float acc[NV];
+  real acc[NV];
 #pragma unroll
   for (int q = 0; q < NV; ++q) acc[q] = 0.f;
-  A.row(i, [&](float a, int c) {
+  A.row(i, [&](real a, int c) {
 #pragma unroll
     for (int q = 0; q < NV; ++q) acc[q] += a * x[(size_t)q * n + c];
   });
@@ -80,7 +80,7 @@ __global__ void k_spmv(M A, int n, const float* __restrict__ x, float* __restric
   for (int q = 0; q < NV; ++q) y[(size_t)q * n + i] = acc[q];
 }
 
-__global__ __launch_bounds__(kMaxKrylovBlock) void k_dot(Seg sg, const float* __restrict__ a, const float* __restrict__ b,
+__global__ __launch_bounds__(kMaxKrylovBlock) void k_dot(Seg sg, const real* __restrict__ a, const real* __restrict__ b,
                       double* __restrict__ part) {
   const BlockRows br = block_rows(sg);
   double acc[1] = {0.0};
@@ -194,29 +194,29 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_prefold(Seg sg, double* __r
 // accumulated in float64 and rounded once (the residual of mixed-precision
 // iterative refinement); otherwise in float32, as the reference's solver.
 template <bool R64, class M>
-__device__ inline float residual_row(const M& A, int i, const float* __restrict__ b,
-                                     const float* __restrict__ x) {
+__device__ inline real residual_row(const M& A, int i, const real* __restrict__ b,
+                                     const real* __restrict__ x) {
   if constexpr (R64) {
     double ax = 0.0;
-    A.row(i, [&](float a, int c) { ax += (double)a * (double)x[c]; });
-    return (float)((double)b[i] - ax);
+    A.row(i, [&](real a, int c) { ax += (double)a * (double)x[c]; });
+    return (real)((double)b[i] - ax);
   } else {
-    float ax = 0.f;
-    A.row(i, [&](float a, int c) { ax += a * x[c]; });
+    real ax = 0.f;
+    A.row(i, [&](real a, int c) { ax += a * x[c]; });
     return b[i] - ax;
   }
 }
 
 template <class M, bool R64>
-__global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_init(M A, Seg sg, const float* __restrict__ b, const float* __restrict__ x,
-                          float* __restrict__ r, const float* __restrict__ inv,
+__global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_init(M A, Seg sg, const real* __restrict__ b, const real* __restrict__ x,
+                          real* __restrict__ r, const real* __restrict__ inv,
                           double* __restrict__ PA) {
   const BlockRows br = block_rows(sg);
   double acc[3] = {0.0, 0.0, 0.0};
   for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
-    const float ri = residual_row<R64>(A, i, b, x);
+    const real ri = residual_row<R64>(A, i, b, x);
     r[i] = ri;
-    const float zi = ri * inv[i];
+    const real zi = ri * inv[i];
     acc[0] += (double)ri * (double)zi;
     acc[1] += (double)ri * (double)ri;
     acc[2] += (double)b[i] * (double)b[i];
@@ -225,9 +225,9 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_init(M A, Seg sg, const 
 }
 
 template <class M, bool PF>
-__global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_1(M A, Seg sg, int it, const float* __restrict__ r,
-                       const float* __restrict__ inv, const float* __restrict__ p_in,
-                       float* __restrict__ p_out, float* __restrict__ q,
+__global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_1(M A, Seg sg, int it, const real* __restrict__ r,
+                       const real* __restrict__ inv, const real* __restrict__ p_in,
+                       real* __restrict__ p_out, real* __restrict__ q,
                        const double* __restrict__ PA, double* __restrict__ PB, KState* st) {
   const BlockRows br = block_rows(sg);
   KState* S = st + br.k;
@@ -240,7 +240,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_1(M A, Seg sg, int it, c
     if (it) ef.load(fp, br.nf);
   const int i0 = br.begin + (int)threadIdx.x;
   const bool has = pf && i0 < br.end;
-  float a[M::kRow], rc[M::kRow], ic[M::kRow], pc[M::kRow];
+  real a[M::kRow], rc[M::kRow], ic[M::kRow], pc[M::kRow];
   if constexpr (M::kEll) {
     if (has) {
       uint32_t o[M::kRow];
@@ -270,18 +270,18 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_1(M A, Seg sg, int it, c
     if (br.vb == br.b0 && threadIdx.x == 0) S->rz[it & 1] = rz;
     first = (beta_d == 0.0);
   }
-  const float beta = (float)beta_d;
+  const real beta = (real)beta_d;
   double acc[1] = {0.0};
   if (pf) {
     if (has) {
-      float qi = 0.f;
+      real qi = 0.f;
 #pragma unroll
       for (int j = 0; j < M::kRow; ++j) {
-        const float z = rc[j] * ic[j];
+        const real z = rc[j] * ic[j];
         qi += a[j] * (first ? z : z + beta * pc[j]);
       }
-      const float zi = rc[0] * ic[0];
-      const float pi = first ? zi : zi + beta * pc[0];
+      const real zi = rc[0] * ic[0];
+      const real pi = first ? zi : zi + beta * pc[0];
       const uint32_t o0 = (uint32_t)i0 * 4u;
       st_off(p_out, o0, pi);
       st_off(q, o0, qi);
@@ -289,14 +289,14 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_1(M A, Seg sg, int it, c
     }
   } else {
     for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
-      float qi = 0.f;
-      A.row(i, [&](float a, int c) {
-        const float z = r[c] * inv[c];
-        const float pc = first ? z : z + beta * p_in[c];
+      real qi = 0.f;
+      A.row(i, [&](real a, int c) {
+        const real z = r[c] * inv[c];
+        const real pc = first ? z : z + beta * p_in[c];
         qi += a * pc;
       });
-      const float zi = r[i] * inv[i];
-      const float pi = first ? zi : zi + beta * p_in[i];
+      const real zi = r[i] * inv[i];
+      const real pi = first ? zi : zi + beta * p_in[i];
       p_out[i] = pi;
       q[i] = qi;
       acc[0] += (double)pi * (double)qi;
@@ -306,9 +306,9 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_1(M A, Seg sg, int it, c
 }
 
 template <bool PF>
-__global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_2(Seg sg, int it, float* __restrict__ x, float* __restrict__ r,
-                       const float* __restrict__ inv, const float* __restrict__ p,
-                       const float* __restrict__ q, const double* __restrict__ PB,
+__global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_2(Seg sg, int it, real* __restrict__ x, real* __restrict__ r,
+                       const real* __restrict__ inv, const real* __restrict__ p,
+                       const real* __restrict__ q, const double* __restrict__ PB,
                        double* __restrict__ PA, KState* st) {
   const BlockRows br = block_rows(sg);
   KState* S = st + br.k;
@@ -320,7 +320,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_2(Seg sg, int it, float*
   if constexpr (pf) ef.load(fp, br.nf);
   const int i0 = br.begin + (int)threadIdx.x;
   const bool has = pf && i0 < br.end;
-  float xv = 0.f, pv = 0.f, rv = 0.f, qv = 0.f, iv = 0.f;
+  real xv = 0.f, pv = 0.f, rv = 0.f, qv = 0.f, iv = 0.f;
   if (has) {
     xv = x[i0];
     pv = p[i0];
@@ -336,23 +336,23 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_2(Seg sg, int it, float*
   } else {
     pq = fold(PB + br.b0, br.nf);
   }
-  const float alpha = (float)safe_div(S->rz[it & 1], pq);
+  const real alpha = (real)safe_div(S->rz[it & 1], pq);
   double acc[2] = {0.0, 0.0};
   if (pf) {
     if (has) {
       x[i0] = xv + alpha * pv;
-      const float ri = rv - alpha * qv;
+      const real ri = rv - alpha * qv;
       r[i0] = ri;
-      const float zi = ri * iv;
+      const real zi = ri * iv;
       acc[0] += (double)ri * (double)zi;
       acc[1] += (double)ri * (double)ri;
     }
   } else {
     for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
       x[i] = x[i] + alpha * p[i];
-      const float ri = r[i] - alpha * q[i];
+      const real ri = r[i] - alpha * q[i];
       r[i] = ri;
-      const float zi = ri * inv[i];
+      const real zi = ri * inv[i];
       acc[0] += (double)ri * (double)zi;
       acc[1] += (double)ri * (double)ri;
     }
@@ -371,12 +371,12 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_2(Seg sg, int it, float*
 // fly.  Partials ping-pong between PA and PC: kernel i reads the buffer
 // kernel i-1 wrote.
 template <class M, bool R64>
-__global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_init1(M A, Seg sg, const float* __restrict__ b, const float* __restrict__ x,
-                            float* __restrict__ r, double* __restrict__ PC) {
+__global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_init1(M A, Seg sg, const real* __restrict__ b, const real* __restrict__ x,
+                            real* __restrict__ r, double* __restrict__ PC) {
   const BlockRows br = block_rows(sg);
   double acc[3] = {0.0, 0.0, 0.0};
   for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
-    const float ri = residual_row<R64>(A, i, b, x);
+    const real ri = residual_row<R64>(A, i, b, x);
     r[i] = ri;
     acc[1] += (double)ri * (double)ri;
     acc[2] += (double)b[i] * (double)b[i];
@@ -385,15 +385,15 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_init1(M A, Seg sg, cons
 }
 
 template <class M>
-__global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_init2(M A, Seg sg, const float* __restrict__ r, const float* __restrict__ inv,
-                            float* __restrict__ w, double* __restrict__ PA) {
+__global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_init2(M A, Seg sg, const real* __restrict__ r, const real* __restrict__ inv,
+                            real* __restrict__ w, double* __restrict__ PA) {
   const BlockRows br = block_rows(sg);
   double acc[3] = {0.0, 0.0, 0.0};
   for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
-    float wi = 0.f;
-    A.row(i, [&](float a, int c) { wi += a * (r[c] * inv[c]); });
+    real wi = 0.f;
+    A.row(i, [&](real a, int c) { wi += a * (r[c] * inv[c]); });
     w[i] = wi;
-    const float ui = r[i] * inv[i];
+    const real ui = r[i] * inv[i];
     acc[0] += (double)r[i] * (double)ui;
     acc[1] += (double)wi * (double)ui;
     acc[2] += (double)r[i] * (double)r[i];
@@ -402,11 +402,11 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_init2(M A, Seg sg, cons
 }
 
 template <class M, bool PF, bool FIRST>
-__global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int it, float* __restrict__ x,
-                           const float* __restrict__ r_in, float* __restrict__ r_out,
-                           const float* __restrict__ w_in, float* __restrict__ w_out,
-                           const float* __restrict__ s_in, float* __restrict__ s_out,
-                           float* __restrict__ p, const float* __restrict__ inv,
+__global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int it, real* __restrict__ x,
+                           const real* __restrict__ r_in, real* __restrict__ r_out,
+                           const real* __restrict__ w_in, real* __restrict__ w_out,
+                           const real* __restrict__ s_in, real* __restrict__ s_out,
+                           real* __restrict__ p, const real* __restrict__ inv,
                            const double* __restrict__ P_in, double* __restrict__ P_out,
                            KState* st) {
   const BlockRows br = block_rows(sg);
@@ -427,8 +427,8 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int i
   }
   const int i0 = br.begin + (int)threadIdx.x;
   const bool has = pf && i0 < br.end;
-  float a[M::kRow], wc[M::kRow], sc[M::kRow], ic[M::kRow], rc[M::kRow];
-  float p0 = 0.f, x0 = 0.f;
+  real a[M::kRow], wc[M::kRow], sc[M::kRow], ic[M::kRow], rc[M::kRow];
+  real p0 = 0.f, x0 = 0.f;
   const uint32_t o0 = (uint32_t)i0 * 4u;
   if constexpr (M::kEll) {
     if (has) {
@@ -443,8 +443,8 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int i
         ic[j] = I(o[j]);
         rc[j] = R(o[j]);
       }
-      p0 = FIRST ? 0.f : ld_off((const float*)p, o0);
-      x0 = ld_off((const float*)x, o0);
+      p0 = FIRST ? 0.f : ld_off((const real*)p, o0);
+      x0 = ld_off((const real*)x, o0);
     }
   }
   double alpha_d, beta_d = 0.0;
@@ -464,49 +464,49 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int i
       S->alpha[it & 1] = alpha_d;
     }
   }
-  const float alpha = (float)alpha_d, beta = (float)beta_d;
+  const real alpha = (real)alpha_d, beta = (real)beta_d;
   double acc[3] = {0.0, 0.0, 0.0};
   if (pf) {
     if (has) {
-      float wn = 0.f;
+      real wn = 0.f;
 #pragma unroll
       for (int j = 0; j < M::kRow; ++j) {
-        const float scj = FIRST ? wc[j] : wc[j] + beta * sc[j];
+        const real scj = FIRST ? wc[j] : wc[j] + beta * sc[j];
         wn += a[j] * (ic[j] * (rc[j] - alpha * scj));
       }
-      const float ii = ic[0];
-      const float ui = rc[0] * ii;
-      const float pi = FIRST ? ui : ui + beta * p0;
-      const float si = FIRST ? wc[0] : wc[0] + beta * sc[0];
+      const real ii = ic[0];
+      const real ui = rc[0] * ii;
+      const real pi = FIRST ? ui : ui + beta * p0;
+      const real si = FIRST ? wc[0] : wc[0] + beta * sc[0];
       st_off(x, o0, x0 + alpha * pi);
-      const float rn = rc[0] - alpha * si;
+      const real rn = rc[0] - alpha * si;
       st_off(p, o0, pi);
       st_off(s_out, o0, si);
       st_off(r_out, o0, rn);
       st_off(w_out, o0, wn);
-      const float un = rn * ii;
+      const real un = rn * ii;
       acc[0] += (double)rn * (double)un;
       acc[1] += (double)wn * (double)un;
       acc[2] += (double)rn * (double)rn;
     }
   } else {
     for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
-      float wn = 0.f;
-      A.row(i, [&](float a, int c) {
-        const float sc = FIRST ? w_in[c] : w_in[c] + beta * s_in[c];
+      real wn = 0.f;
+      A.row(i, [&](real a, int c) {
+        const real sc = FIRST ? w_in[c] : w_in[c] + beta * s_in[c];
         wn += a * (inv[c] * (r_in[c] - alpha * sc));
       });
-      const float ii = inv[i];
-      const float ui = r_in[i] * ii;
-      const float pi = FIRST ? ui : ui + beta * p[i];
-      const float si = FIRST ? w_in[i] : w_in[i] + beta * s_in[i];
+      const real ii = inv[i];
+      const real ui = r_in[i] * ii;
+      const real pi = FIRST ? ui : ui + beta * p[i];
+      const real si = FIRST ? w_in[i] : w_in[i] + beta * s_in[i];
       x[i] = x[i] + alpha * pi;
-      const float rn = r_in[i] - alpha * si;
+      const real rn = r_in[i] - alpha * si;
       p[i] = pi;
       s_out[i] = si;
       r_out[i] = rn;
       w_out[i] = wn;
-      const float un = rn * ii;
+      const real un = rn * ii;
       acc[0] += (double)rn * (double)un;
       acc[1] += (double)wn * (double)un;
       acc[2] += (double)rn * (double)rn;
@@ -523,11 +523,11 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int i
 // the fold completes while the gathers are in flight.  FIRST (iteration 0:
 // no fold, s = p = 0) is a separate instance, without the branches.
 template <class M, bool FIRST>
-__global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter2(M A, Seg sg, int it, float* __restrict__ x,
-                           const float* __restrict__ r_in, float* __restrict__ r_out,
-                           const float* __restrict__ w_in, float* __restrict__ w_out,
-                           const float* __restrict__ s_in, float* __restrict__ s_out,
-                           float* __restrict__ p, const float* __restrict__ inv,
+__global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter2(M A, Seg sg, int it, real* __restrict__ x,
+                           const real* __restrict__ r_in, real* __restrict__ r_out,
+                           const real* __restrict__ w_in, real* __restrict__ w_out,
+                           const real* __restrict__ s_in, real* __restrict__ s_out,
+                           real* __restrict__ p, const real* __restrict__ inv,
                            const double* __restrict__ P_in, double* __restrict__ P_out,
                            KState* st) {
   const BlockRows br = block_rows(sg);
@@ -545,8 +545,8 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter2(M A, Seg sg, int 
     rz_prev = S->rz[pv];
     alpha_prev = S->alpha[pv];
   }
-  float a[NR][M::kRow], wc[NR][M::kRow], sc[NR][M::kRow], ic[NR][M::kRow], rc[NR][M::kRow];
-  float p0[NR], x0[NR];
+  real a[NR][M::kRow], wc[NR][M::kRow], sc[NR][M::kRow], ic[NR][M::kRow], rc[NR][M::kRow];
+  real p0[NR], x0[NR];
   bool has[NR];
   uint32_t o0[NR];
 #pragma unroll
@@ -568,8 +568,8 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter2(M A, Seg sg, int 
         ic[q][j] = I(o[j]);
         rc[q][j] = Rv(o[j]);
       }
-      p0[q] = FIRST ? 0.f : ld_off((const float*)p, o0[q]);
-      x0[q] = ld_off((const float*)x, o0[q]);
+      p0[q] = FIRST ? 0.f : ld_off((const real*)p, o0[q]);
+      x0[q] = ld_off((const real*)x, o0[q]);
     }
   }
   double alpha_d, beta_d = 0.0;
@@ -586,28 +586,28 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter2(M A, Seg sg, int 
       S->alpha[it & 1] = alpha_d;
     }
   }
-  const float alpha = (float)alpha_d, beta = (float)beta_d;
+  const real alpha = (real)alpha_d, beta = (real)beta_d;
   double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int q = 0; q < NR; ++q) {
     if (!has[q]) continue;
-    float wn = 0.f;
+    real wn = 0.f;
 #pragma unroll
     for (int j = 0; j < M::kRow; ++j) {
-      const float scj = FIRST ? wc[q][j] : wc[q][j] + beta * sc[q][j];
+      const real scj = FIRST ? wc[q][j] : wc[q][j] + beta * sc[q][j];
       wn += a[q][j] * (ic[q][j] * (rc[q][j] - alpha * scj));
     }
-    const float ii = ic[q][0];
-    const float ui = rc[q][0] * ii;
-    const float pi = FIRST ? ui : ui + beta * p0[q];
-    const float si = FIRST ? wc[q][0] : wc[q][0] + beta * sc[q][0];
+    const real ii = ic[q][0];
+    const real ui = rc[q][0] * ii;
+    const real pi = FIRST ? ui : ui + beta * p0[q];
+    const real si = FIRST ? wc[q][0] : wc[q][0] + beta * sc[q][0];
     st_off(x, o0[q], x0[q] + alpha * pi);
-    const float rn = rc[q][0] - alpha * si;
+    const real rn = rc[q][0] - alpha * si;
     st_off(p, o0[q], pi);
     st_off(s_out, o0[q], si);
     st_off(r_out, o0[q], rn);
     st_off(w_out, o0[q], wn);
-    const float un = rn * ii;
+    const real un = rn * ii;
     acc[0] += (double)rn * (double)un;
     acc[1] += (double)wn * (double)un;
     acc[2] += (double)rn * (double)rn;
@@ -619,24 +619,24 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter2(M A, Seg sg, int 
 // partial slots: PA[3][G] = {r0.r, r.r, b.b} (init, k_bi_3), PB[1][G] = {r0.v}
 // (k_bi_1), PC[3][G] = {t.s, t.t, s.s} (k_bi_2)
 template <class M, int NV>
-__global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_init(M A, Seg sg, const float* __restrict__ b, const float* __restrict__ x,
-                          float* __restrict__ r, float* __restrict__ r0, double* __restrict__ PA) {
+__global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_init(M A, Seg sg, const real* __restrict__ b, const real* __restrict__ x,
+                          real* __restrict__ r, real* __restrict__ r0, double* __restrict__ PA) {
   const BlockRows br = block_rows(sg);
   const int n = sg.n;
   double acc[3] = {0.0, 0.0, 0.0};
   for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
-    float ax[NV];
+    real ax[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) ax[q] = 0.f;
-    A.row(i, [&](float a, int c) {
+    A.row(i, [&](real a, int c) {
 #pragma unroll
       for (int q = 0; q < NV; ++q) ax[q] += a * x[(size_t)q * n + c];
     });
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
       const size_t k = (size_t)q * n + i;
-      const float bi = b[k];
-      const float ri = bi - ax[q];
+      const real bi = b[k];
+      const real ri = bi - ax[q];
       r[k] = ri;
       r0[k] = ri;
       acc[0] += (double)ri * (double)ri;
@@ -648,10 +648,10 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_init(M A, Seg sg, const 
 }
 
 template <class M, int NV, bool PF>
-__global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_1(M A, Seg sg, int it, const float* __restrict__ r,
-                       const float* __restrict__ inv, const float* __restrict__ p_in,
-                       const float* __restrict__ v_in, float* __restrict__ p_out,
-                       float* __restrict__ v_out, const float* __restrict__ r0,
+__global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_1(M A, Seg sg, int it, const real* __restrict__ r,
+                       const real* __restrict__ inv, const real* __restrict__ p_in,
+                       const real* __restrict__ v_in, real* __restrict__ p_out,
+                       real* __restrict__ v_out, const real* __restrict__ r0,
                        const double* __restrict__ PA, double* __restrict__ PB, KState* st) {
   const BlockRows br = block_rows(sg);
   KState* S = st + br.k;
@@ -665,7 +665,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_1(M A, Seg sg, int it, c
     if (it) ef.load(fp, br.nf);
   const int i0 = br.begin + (int)threadIdx.x;
   const bool has = pf && i0 < br.end;
-  float a[M::kRow], ic[M::kRow], rr[NV][M::kRow], pp[NV][M::kRow], vv[NV][M::kRow], r0i[NV];
+  real a[M::kRow], ic[M::kRow], rr[NV][M::kRow], pp[NV][M::kRow], vv[NV][M::kRow], r0i[NV];
   if constexpr (M::kEll) {
     if (has) {
       uint32_t o[M::kRow];
@@ -704,19 +704,19 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_1(M A, Seg sg, int it, c
     if (br.vb == br.b0 && threadIdx.x == 0) S->rho[it & 1] = rho;
     first = (beta_d == 0.0);
   }
-  const float beta = (float)beta_d;
-  const float omega = (float)omega_d;
+  const real beta = (real)beta_d;
+  const real omega = (real)omega_d;
   double acc[1] = {0.0};
   if (pf) {
     if (has) {
-      float v[NV];
+      real v[NV];
 #pragma unroll
       for (int q = 0; q < NV; ++q) v[q] = 0.f;
 #pragma unroll
       for (int j = 0; j < M::kRow; ++j) {
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
-          const float pc = first ? rr[q][j] : rr[q][j] + beta * (pp[q][j] - omega * vv[q][j]);
+          const real pc = first ? rr[q][j] : rr[q][j] + beta * (pp[q][j] - omega * vv[q][j]);
           v[q] += a[j] * (pc * ic[j]);
         }
       }
@@ -724,7 +724,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_1(M A, Seg sg, int it, c
 #pragma unroll
       for (int q = 0; q < NV; ++q) {
         const size_t qn = (size_t)q * n;
-        const float pi = first ? rr[q][0] : rr[q][0] + beta * (pp[q][0] - omega * vv[q][0]);
+        const real pi = first ? rr[q][0] : rr[q][0] + beta * (pp[q][0] - omega * vv[q][0]);
         st_off(p_out + qn, o0, pi);
         st_off(v_out + qn, o0, v[q]);
         acc[0] += (double)r0i[q] * (double)v[q];
@@ -732,22 +732,22 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_1(M A, Seg sg, int it, c
     }
   } else {
     for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
-      float v[NV];
+      real v[NV];
 #pragma unroll
       for (int q = 0; q < NV; ++q) v[q] = 0.f;
-      A.row(i, [&](float a, int c) {
-        const float ic = inv[c];
+      A.row(i, [&](real a, int c) {
+        const real ic = inv[c];
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
           const size_t k = (size_t)q * n + c;
-          const float pc = first ? r[k] : r[k] + beta * (p_in[k] - omega * v_in[k]);
+          const real pc = first ? r[k] : r[k] + beta * (p_in[k] - omega * v_in[k]);
           v[q] += a * (pc * ic);
         }
       });
 #pragma unroll
       for (int q = 0; q < NV; ++q) {
         const size_t k = (size_t)q * n + i;
-        const float pi = first ? r[k] : r[k] + beta * (p_in[k] - omega * v_in[k]);
+        const real pi = first ? r[k] : r[k] + beta * (p_in[k] - omega * v_in[k]);
         p_out[k] = pi;
         v_out[k] = v[q];
         acc[0] += (double)r0[k] * (double)v[q];
@@ -758,9 +758,9 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_1(M A, Seg sg, int it, c
 }
 
 template <class M, int NV, bool PF>
-__global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_2(M A, Seg sg, int it, const float* __restrict__ r,
-                       const float* __restrict__ inv, const float* __restrict__ v,
-                       float* __restrict__ s, float* __restrict__ t,
+__global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_2(M A, Seg sg, int it, const real* __restrict__ r,
+                       const real* __restrict__ inv, const real* __restrict__ v,
+                       real* __restrict__ s, real* __restrict__ t,
                        const double* __restrict__ PB, double* __restrict__ PC, KState* st) {
   const BlockRows br = block_rows(sg);
   KState* S = st + br.k;
@@ -773,7 +773,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_2(M A, Seg sg, int it, c
   if constexpr (pf) ef.load(fp, br.nf);
   const int i0 = br.begin + (int)threadIdx.x;
   const bool has = pf && i0 < br.end;
-  float a[M::kRow], ic[M::kRow], rr[NV][M::kRow], vv[NV][M::kRow];
+  real a[M::kRow], ic[M::kRow], rr[NV][M::kRow], vv[NV][M::kRow];
   if constexpr (M::kEll) {
     if (has) {
       uint32_t o[M::kRow];
@@ -802,18 +802,18 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_2(M A, Seg sg, int it, c
   }
   const double alpha_d = safe_div(S->rho[it & 1], r0v);
   if (br.vb == br.b0 && threadIdx.x == 0) S->alpha[it & 1] = alpha_d;
-  const float alpha = (float)alpha_d;
+  const real alpha = (real)alpha_d;
   double acc[3] = {0.0, 0.0, 0.0};
   if (pf) {
     if (has) {
-      float tv[NV];
+      real tv[NV];
 #pragma unroll
       for (int q = 0; q < NV; ++q) tv[q] = 0.f;
 #pragma unroll
       for (int j = 0; j < M::kRow; ++j) {
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
-          const float sc = rr[q][j] - alpha * vv[q][j];
+          const real sc = rr[q][j] - alpha * vv[q][j];
           tv[q] += a[j] * (sc * ic[j]);
         }
       }
@@ -821,7 +821,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_2(M A, Seg sg, int it, c
 #pragma unroll
       for (int q = 0; q < NV; ++q) {
         const size_t qn = (size_t)q * n;
-        const float si = rr[q][0] - alpha * vv[q][0];
+        const real si = rr[q][0] - alpha * vv[q][0];
         st_off(s + qn, o0, si);
         st_off(t + qn, o0, tv[q]);
         acc[0] += (double)tv[q] * (double)si;
@@ -831,22 +831,22 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_2(M A, Seg sg, int it, c
     }
   } else {
     for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
-      float tv[NV];
+      real tv[NV];
 #pragma unroll
       for (int q = 0; q < NV; ++q) tv[q] = 0.f;
-      A.row(i, [&](float a, int c) {
-        const float ic = inv[c];
+      A.row(i, [&](real a, int c) {
+        const real ic = inv[c];
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
           const size_t k = (size_t)q * n + c;
-          const float sc = r[k] - alpha * v[k];
+          const real sc = r[k] - alpha * v[k];
           tv[q] += a * (sc * ic);
         }
       });
 #pragma unroll
       for (int q = 0; q < NV; ++q) {
         const size_t k = (size_t)q * n + i;
-        const float si = r[k] - alpha * v[k];
+        const real si = r[k] - alpha * v[k];
         s[k] = si;
         t[k] = tv[q];
         acc[0] += (double)tv[q] * (double)si;
@@ -859,10 +859,10 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_2(M A, Seg sg, int it, c
 }
 
 template <int NV, bool PF>
-__global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_3(Seg sg, int it, float* __restrict__ x, float* __restrict__ r,
-                       const float* __restrict__ inv, const float* __restrict__ p,
-                       const float* __restrict__ s, const float* __restrict__ t,
-                       const float* __restrict__ r0, const double* __restrict__ PC,
+__global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_3(Seg sg, int it, real* __restrict__ x, real* __restrict__ r,
+                       const real* __restrict__ inv, const real* __restrict__ p,
+                       const real* __restrict__ s, const real* __restrict__ t,
+                       const real* __restrict__ r0, const double* __restrict__ PC,
                        double* __restrict__ PA, KState* st) {
   const BlockRows br = block_rows(sg);
   KState* S = st + br.k;
@@ -875,7 +875,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_3(Seg sg, int it, float*
   if constexpr (pf) ef.load(pp, br.nf);
   const int i0 = br.begin + (int)threadIdx.x;
   const bool has = pf && i0 < br.end;
-  float iv = 0.f, pv[NV], sv[NV], tv[NV], xv[NV], r0v[NV];
+  real iv = 0.f, pv[NV], sv[NV], tv[NV], xv[NV], r0v[NV];
   if (has) {
     iv = inv[i0];
 #pragma unroll
@@ -899,18 +899,18 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_3(Seg sg, int it, float*
     S->omega[it & 1] = omega_d;
     S->ss = ss;
   }
-  const float alpha = (float)S->alpha[it & 1];
-  const float omega = (float)omega_d;
+  const real alpha = (real)S->alpha[it & 1];
+  const real omega = (real)omega_d;
   double acc[2] = {0.0, 0.0};
   if (pf) {
     if (has) {
 #pragma unroll
       for (int q = 0; q < NV; ++q) {
         const size_t k = (size_t)q * n + i0;
-        const float y = pv[q] * iv;
-        const float z = sv[q] * iv;
+        const real y = pv[q] * iv;
+        const real z = sv[q] * iv;
         x[k] = xv[q] + (alpha * y + omega * z);
-        const float rk = sv[q] - omega * tv[q];
+        const real rk = sv[q] - omega * tv[q];
         r[k] = rk;
         acc[0] += (double)r0v[q] * (double)rk;
         acc[1] += (double)rk * (double)rk;
@@ -920,14 +920,14 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_3(Seg sg, int it, float*
     return;
   }
   for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
-    const float ii = inv[i];
+    const real ii = inv[i];
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
       const size_t k = (size_t)q * n + i;
-      const float y = p[k] * ii;
-      const float z = s[k] * ii;
+      const real y = p[k] * ii;
+      const real z = s[k] * ii;
       x[k] = x[k] + (alpha * y + omega * z);
-      const float rk = s[k] - omega * t[k];
+      const real rk = s[k] - omega * t[k];
       r[k] = rk;
       acc[0] += (double)r0[k] * (double)rk;
       acc[1] += (double)rk * (double)rk;
@@ -938,7 +938,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_3(Seg sg, int it, float*
 
 // ------------------------------------------------------------- host side
 struct KWork {
-  float *r, *r0, *p0, *p1, *v0, *v1, *s, *t, *inv;
+  real *r, *r0, *p0, *p1, *v0, *v1, *s, *t, *inv;
   double *PA, *PB, *PC;
   KState* st;
 };
@@ -953,15 +953,15 @@ static KWork carve_krylov(void* ws, int32_t n, int nvec, int n_cases, int max_bl
   w.PB = c.take<double>((size_t)max_blocks);
   w.PC = c.take<double>(3 * (size_t)max_blocks);
   const size_t nv = (size_t)n * nvec;
-  w.r = c.take<float>(nv);
-  w.r0 = c.take<float>(nv);
-  w.p0 = c.take<float>(nv);
-  w.p1 = c.take<float>(nv);
-  w.v0 = c.take<float>(nv);
-  w.v1 = c.take<float>(nv);
-  w.s = c.take<float>(nv);
-  w.t = c.take<float>(nv);
-  w.inv = c.take<float>(n);
+  w.r = c.take<real>(nv);
+  w.r0 = c.take<real>(nv);
+  w.p0 = c.take<real>(nv);
+  w.p1 = c.take<real>(nv);
+  w.v0 = c.take<real>(nv);
+  w.v1 = c.take<real>(nv);
+  w.s = c.take<real>(nv);
+  w.t = c.take<real>(nv);
+  w.inv = c.take<real>(n);
   return w;
 }
 
@@ -972,8 +972,8 @@ size_t krylov_workspace_bytes_batch(int32_t n, int nvec, int n_cases, int max_bl
   c.take<double>((size_t)max_blocks);
   c.take<double>(3 * (size_t)max_blocks);
   const size_t nv = (size_t)n * nvec;
-  for (int i = 0; i < 8; ++i) c.take<float>(nv);
-  c.take<float>(n);
+  for (int i = 0; i < 8; ++i) c.take<real>(nv);
+  c.take<real>(n);
   return c.off;
 }
 
@@ -1170,13 +1170,13 @@ static int prepare(KState* st, const Seg& sg, const Tol& tl, hipStream_t s) {
 }
 
 template <class M>
-static int cg_impl(const M& A, const KLaunch& L, const float* b, float* x, const Tol& tl,
+static int cg_impl(const M& A, const KLaunch& L, const real* b, real* x, const Tol& tl,
                    bool r64, int maxiter, int check_every, KWork& w, wcfd_solve_info* info,
                    hipStream_t s) {
   const Seg& sg = L.sg;
   const int n = sg.n, B = L.B, G = sg.G, K = sg.n_cases;
   const bool pf = row_per_thread<M>(L);
-  float* q = w.s;
+  real* q = w.s;
   int e = prepare(w.st, sg, tl, s);
   if (e) return e;
   if (const unsigned gi = items_grid(L.rows, n))
@@ -1187,7 +1187,7 @@ static int cg_impl(const M& A, const KLaunch& L, const float* b, float* x, const
     WCFD_LAUNCH_B(K_CG_INIT, s, (k_cg_init<M, false>), G, B, A, sg, b, x, w.r, w.inv, w.PA);
   WCFD_LAUNCH(K_FOLD, s, k_init_fold, K, sg, w.PA, w.PA, w.st, 0, maxiter);
   if ((e = check_launch())) return e;
-  float* pb[2] = {w.p0, w.p1};
+  real* pb[2] = {w.p0, w.p1};
   const int ce = check_every > 0 ? check_every : 1;
   auto group = [&](int it0) {
     for (int it = it0; it < it0 + ce; ++it) {
@@ -1215,7 +1215,7 @@ static int cg_impl(const M& A, const KLaunch& L, const float* b, float* x, const
 }
 
 template <class M>
-static int cgs_impl(const M& A, const KLaunch& L, const float* b, float* x, const Tol& tl,
+static int cgs_impl(const M& A, const KLaunch& L, const real* b, real* x, const Tol& tl,
                     bool r64, int maxiter, int check_every, KWork& w, wcfd_solve_info* info,
                     hipStream_t s) {
   const Seg& sg = L.sg;
@@ -1223,9 +1223,9 @@ static int cgs_impl(const M& A, const KLaunch& L, const float* b, float* x, cons
   const int rows = rows_per_thread<M>(L);
   int e = prepare(w.st, sg, tl, s);
   if (e) return e;
-  float* rb[2] = {w.r, w.r0};
-  float* wb[2] = {w.v0, w.v1};
-  float* sb[2] = {w.s, w.t};
+  real* rb[2] = {w.r, w.r0};
+  real* wb[2] = {w.v0, w.v1};
+  real* sb[2] = {w.s, w.t};
   double* pb[2] = {w.PA, w.PC};   // iteration i reads pb[i&1], writes pb[(i+1)&1]
   if (const unsigned gi = items_grid(L.rows, n))
     WCFD_LAUNCH(K_INV_DIAG, s, k_inv_diag<M>, gi, A, L.rows, n, w.inv);
@@ -1275,7 +1275,7 @@ static int cgs_impl(const M& A, const KLaunch& L, const float* b, float* x, cons
 }
 
 template <class M, int NV>
-static int bicg_impl(const M& A, const KLaunch& L, const float* b, float* x, const Tol& tl,
+static int bicg_impl(const M& A, const KLaunch& L, const real* b, real* x, const Tol& tl,
                      int maxiter, int check_every, KWork& w, wcfd_solve_info* info,
                      hipStream_t s) {
   const Seg& sg = L.sg;
@@ -1288,8 +1288,8 @@ static int bicg_impl(const M& A, const KLaunch& L, const float* b, float* x, con
   WCFD_LAUNCH_B(K_BI_INIT, s, (k_bi_init<M, NV>), G, B, A, sg, b, x, w.r, w.r0, w.PA);
   WCFD_LAUNCH(K_FOLD, s, k_init_fold, K, sg, w.PA, w.PA, w.st, 1, maxiter);
   if ((e = check_launch())) return e;
-  float* pb[2] = {w.p0, w.p1};
-  float* vb[2] = {w.v0, w.v1};
+  real* pb[2] = {w.p0, w.p1};
+  real* vb[2] = {w.v0, w.v1};
   const int ce = check_every > 0 ? check_every : 1;
   auto group = [&](int it0) {
     for (int it = it0; it < it0 + ce; ++it) {
@@ -1331,7 +1331,7 @@ static int budget_maxiter(const Tol& tl, int maxiter, int check_every) {
 }
 
 template <class M>
-static int cg_dispatch(const M& A, const KLaunch& L, int variant, const float* b, float* x,
+static int cg_dispatch(const M& A, const KLaunch& L, int variant, const real* b, real* x,
                        const Tol& tl, int maxiter, int check_every, KWork& w,
                        wcfd_solve_info* info, hipStream_t s) {
   if (variant != WCFD_CG_SINGLE_REDUCTION && variant != WCFD_CG_STANDARD) return WCFD_EINVAL;
@@ -1361,7 +1361,7 @@ static int cg_dispatch(const M& A, const KLaunch& L, int variant, const float* b
 }
 
 template <class M>
-static int bicg_dispatch(const M& A, const KLaunch& L, int nvec, const float* b, float* x,
+static int bicg_dispatch(const M& A, const KLaunch& L, int nvec, const real* b, real* x,
                          const Tol& tl, int maxiter, int check_every, KWork& w,
                          wcfd_solve_info* info, hipStream_t s) {
   maxiter = budget_maxiter(tl, maxiter, check_every);
@@ -1372,7 +1372,7 @@ static int bicg_dispatch(const M& A, const KLaunch& L, int nvec, const float* b,
 
 // ELL matrix of a mesh, with or without int16 neighbour offsets
 template <class F>
-static int with_ell(const wcfd_mesh& m, const float* values, F&& f) {
+static int with_ell(const wcfd_mesh& m, const real* values, F&& f) {
   // tetrahedra and wedges read 4 or 5 slots (int16 offsets only; the int32
   // form reads all six, the unused ones zero)
   if (m.slot_dnb) {
@@ -1404,7 +1404,7 @@ static KWork work_of(void* ws, const wcfd_batch* bt, int n, int nvec) {
             : carve_krylov(ws, n, nvec, 1, single_max_blocks(n));
 }
 
-int solve_cg(const CsrView& A, int variant, const float* b, float* x, const Tol& tl,
+int solve_cg(const CsrView& A, int variant, const real* b, real* x, const Tol& tl,
              int maxiter, int check_every, void* workspace, wcfd_solve_info* info, hipStream_t s) {
   const CsrMat M{A.n, A.row_ptr, A.col, A.values, A.diag_pos};
   KWork w = work_of(workspace, nullptr, A.n, 1);
@@ -1412,7 +1412,7 @@ int solve_cg(const CsrView& A, int variant, const float* b, float* x, const Tol&
                      info, s);
 }
 
-int solve_bicgstab(const CsrView& A, int nvec, const float* b, float* x, const Tol& tl,
+int solve_bicgstab(const CsrView& A, int nvec, const real* b, real* x, const Tol& tl,
                    int maxiter, int check_every, void* workspace, wcfd_solve_info* info,
                    hipStream_t s) {
   const CsrMat M{A.n, A.row_ptr, A.col, A.values, A.diag_pos};
@@ -1421,8 +1421,8 @@ int solve_bicgstab(const CsrView& A, int nvec, const float* b, float* x, const T
                        info, s);
 }
 
-int solve_cg_ell(const wcfd_mesh& m, const wcfd_batch* bt, const float* values, int variant,
-                 const float* b, float* x, const Tol& tl, int maxiter, int check_every,
+int solve_cg_ell(const wcfd_mesh& m, const wcfd_batch* bt, const real* values, int variant,
+                 const real* b, real* x, const Tol& tl, int maxiter, int check_every,
                  void* workspace, wcfd_solve_info* info, hipStream_t s) {
   const int n = m.n_cells;
   const KLaunch L = launch_of(bt, n);
@@ -1432,8 +1432,8 @@ int solve_cg_ell(const wcfd_mesh& m, const wcfd_batch* bt, const float* values, 
   });
 }
 
-int solve_bicgstab_ell(const wcfd_mesh& m, const wcfd_batch* bt, const float* values, int nvec,
-                       const float* b, float* x, const Tol& tl, int maxiter,
+int solve_bicgstab_ell(const wcfd_mesh& m, const wcfd_batch* bt, const real* values, int nvec,
+                       const real* b, real* x, const Tol& tl, int maxiter,
                        int check_every, void* workspace, wcfd_solve_info* info, hipStream_t s) {
   const int n = m.n_cells;
   const KLaunch L = launch_of(bt, n);
@@ -1455,8 +1455,8 @@ using namespace wcfd;
 
 extern "C" {
 
-int wcfd_spmv(int32_t n, const int32_t* row_ptr, const int32_t* col, const float* values, int nvec,
-              const float* x, float* y, void* stream) {
+int wcfd_spmv(int32_t n, const int32_t* row_ptr, const int32_t* col, const real* values, int nvec,
+              const real* x, real* y, void* stream) {
   if (n <= 0 || !row_ptr || !col || !values || !x || !y) return WCFD_EINVAL;
   hipStream_t s = (hipStream_t)stream;
   const CsrMat A{n, row_ptr, col, values, nullptr};
@@ -1469,7 +1469,7 @@ int wcfd_spmv(int32_t n, const int32_t* row_ptr, const int32_t* col, const float
   return check_launch();
 }
 
-int wcfd_spmv_ell(const wcfd_mesh* m, const float* values, int nvec, const float* x, float* y,
+int wcfd_spmv_ell(const wcfd_mesh* m, const real* values, int nvec, const real* x, real* y,
                   void* stream) {
   if (!m || m->n_cells <= 0 || !m->slot_nbr || !slots_ok(m) || !values || !x || !y)
     return WCFD_EINVAL;
@@ -1484,7 +1484,7 @@ int wcfd_spmv_ell(const wcfd_mesh* m, const float* values, int nvec, const float
   });
 }
 
-int wcfd_dot_f64(int32_t n, const float* a, const float* b, double* out, void* workspace,
+int wcfd_dot_f64(int32_t n, const real* a, const real* b, double* out, void* workspace,
                  void* stream) {
   if (n <= 0 || !a || !b || !out || !workspace) return WCFD_EINVAL;
   Carver c(workspace);
@@ -1531,8 +1531,8 @@ size_t wcfd_batch_krylov_workspace_bytes(int32_t n, int32_t n_cases, int32_t n_b
   return krylov_workspace_bytes_batch(n, 3, n_cases, n_blocks);
 }
 
-int wcfd_cg(int32_t n, const int32_t* row_ptr, const int32_t* col, const float* values,
-            const int32_t* diag_pos, int variant, const float* b, float* x, double atol, double tol,
+int wcfd_cg(int32_t n, const int32_t* row_ptr, const int32_t* col, const real* values,
+            const int32_t* diag_pos, int variant, const real* b, real* x, double atol, double tol,
             int maxiter, int check_every, void* workspace, wcfd_solve_info* info, void* stream) {
   if (n <= 0 || !row_ptr || !col || !values || !diag_pos || !b || !x || !workspace)
     return WCFD_EINVAL;
@@ -1541,8 +1541,8 @@ int wcfd_cg(int32_t n, const int32_t* row_ptr, const int32_t* col, const float* 
                   (hipStream_t)stream);
 }
 
-int wcfd_bicgstab(int32_t n, const int32_t* row_ptr, const int32_t* col, const float* values,
-                  const int32_t* diag_pos, int nvec, const float* b, float* x, double atol,
+int wcfd_bicgstab(int32_t n, const int32_t* row_ptr, const int32_t* col, const real* values,
+                  const int32_t* diag_pos, int nvec, const real* b, real* x, double atol,
                   double tol, int maxiter, int check_every, void* workspace, wcfd_solve_info* info,
                   void* stream) {
   if (n <= 0 || !row_ptr || !col || !values || !diag_pos || !b || !x || !workspace)
@@ -1552,7 +1552,7 @@ int wcfd_bicgstab(int32_t n, const int32_t* row_ptr, const int32_t* col, const f
                         (hipStream_t)stream);
 }
 
-int wcfd_cg_ell(const wcfd_mesh* m, const float* values, int variant, const float* b, float* x,
+int wcfd_cg_ell(const wcfd_mesh* m, const real* values, int variant, const real* b, real* x,
                 double atol, double tol, int maxiter, int check_every, void* workspace,
                 wcfd_solve_info* info, void* stream) {
   if (!m || m->n_cells <= 0 || !m->slot_nbr || !slots_ok(m) || !values || !b || !x ||
@@ -1562,7 +1562,7 @@ int wcfd_cg_ell(const wcfd_mesh* m, const float* values, int variant, const floa
                       workspace, info, (hipStream_t)stream);
 }
 
-int wcfd_bicgstab_ell(const wcfd_mesh* m, const float* values, int nvec, const float* b, float* x,
+int wcfd_bicgstab_ell(const wcfd_mesh* m, const real* values, int nvec, const real* b, real* x,
                       double atol, double tol, int maxiter, int check_every, void* workspace,
                       wcfd_solve_info* info, void* stream) {
   if (!m || m->n_cells <= 0 || !m->slot_nbr || !slots_ok(m) || !values || !b || !x ||
@@ -1572,8 +1572,8 @@ int wcfd_bicgstab_ell(const wcfd_mesh* m, const float* values, int nvec, const f
                             workspace, info, (hipStream_t)stream);
 }
 
-int wcfd_batch_cg_ell(const wcfd_mesh* m, const wcfd_batch* bt, const float* values, int variant,
-                      const float* b, float* x, double atol, double tol, int maxiter,
+int wcfd_batch_cg_ell(const wcfd_mesh* m, const wcfd_batch* bt, const real* values, int variant,
+                      const real* b, real* x, double atol, double tol, int maxiter,
                       int check_every, void* workspace, wcfd_solve_info* info, void* stream) {
   if (!batch_ok(m, bt) || !m->slot_nbr || !slots_ok(m) || !values || !b || !x || !workspace)
     return WCFD_EINVAL;
@@ -1581,8 +1581,8 @@ int wcfd_batch_cg_ell(const wcfd_mesh* m, const wcfd_batch* bt, const float* val
                       info, (hipStream_t)stream);
 }
 
-int wcfd_batch_bicgstab_ell(const wcfd_mesh* m, const wcfd_batch* bt, const float* values,
-                            int nvec, const float* b, float* x, double atol, double tol,
+int wcfd_batch_bicgstab_ell(const wcfd_mesh* m, const wcfd_batch* bt, const real* values,
+                            int nvec, const real* b, real* x, double atol, double tol,
                             int maxiter, int check_every, void* workspace, wcfd_solve_info* info,
                             void* stream) {
   if (!batch_ok(m, bt) || !m->slot_nbr || !slots_ok(m) || !values || !b || !x || !workspace)

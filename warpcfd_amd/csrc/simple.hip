@@ -24,8 +24,8 @@ namespace wcfd {
 struct SimpleWork {
   void* krylov;
   double* partials;
-  float* fc;
-  float* u_prev;
+  real* fc;
+  real* u_prev;
 };
 
 static SimpleWork carve_simple(void* ws, int32_t C, int32_t F, int K, int G, bool batch) {
@@ -34,8 +34,8 @@ static SimpleWork carve_simple(void* ws, int32_t C, int32_t F, int K, int G, boo
   w.krylov = c.take<char>(batch ? krylov_workspace_bytes_batch(C, 3, K, G)
                                 : krylov_workspace_bytes(C, 3));
   w.partials = c.take<double>(4 * (size_t)(batch ? G : cell_reduce_blocks(C)));
-  w.fc = c.take<float>(F);
-  w.u_prev = c.take<float>(3 * (size_t)C);
+  w.fc = c.take<real>(F);
+  w.u_prev = c.take<real>(3 * (size_t)C);
   return w;
 }
 
@@ -43,8 +43,8 @@ static size_t simple_bytes(int32_t C, int32_t F, int K, int G, bool batch) {
   Carver c(nullptr);
   c.take<char>(batch ? krylov_workspace_bytes_batch(C, 3, K, G) : krylov_workspace_bytes(C, 3));
   c.take<double>(4 * (size_t)(batch ? G : cell_reduce_blocks(C)));
-  c.take<float>(F);
-  c.take<float>(3 * (size_t)C);
+  c.take<real>(F);
+  c.take<real>(3 * (size_t)C);
   return c.off;
 }
 

@@ -42,22 +42,22 @@ __device__ inline size_t widx(int o, int j, int k, int c, int C) {
 // upwind, convectionScheme.py:82).  Boundary faces: F_b phi_b explicit, or,
 // with implicit_outflow on a Neumann outflow face, F_b implicit plus
 // F_b |x_f - x_P| g explicit (not in the reference).
-__global__ void k_convection_weights(wcfd_mesh m, wcfd_bc bc, const float* __restrict__ cg,
-                                     const float* __restrict__ fv,
-                                     const float* __restrict__ flux,
-                                     const float* __restrict__ fc2f, int scheme,
-                                     int implicit_outflow, Outs outs, float* __restrict__ w) {
+__global__ void k_convection_weights(wcfd_mesh m, wcfd_bc bc, const real* __restrict__ cg,
+                                     const real* __restrict__ fv,
+                                     const real* __restrict__ flux,
+                                     const real* __restrict__ fc2f, int scheme,
+                                     int implicit_outflow, Outs outs, real* __restrict__ w) {
   const int c = xcd_item();
   const int C = m.n_cells, F = m.n_faces, Fi = m.n_internal, Fb = m.n_boundary;
   if (c >= C) return;
   for (int k = 0; k < 6; ++k) {
     const int f = m.slot_face[k * C + c];
-    const float Fl = flux[f];
+    const real Fl = flux[f];
     if (f >= Fi) {
       const int j = f - Fi;
       for (int o = 0; o < outs.n; ++o) {
         const int go = outs.g[o];
-        float w0 = 0.f, w2;
+        real w0 = 0.f, w2;
         if (implicit_outflow && Fl > 0.f && bc.type[go * Fb + j] == WCFD_NEUMANN) {
           w0 = Fl;
           w2 = Fl * (m.dmag[f] * bc.value[go * Fb + j]);
@@ -71,10 +71,10 @@ __global__ void k_convection_weights(wcfd_mesh m, wcfd_bc bc, const float* __res
       continue;
     }
     const int on = m.owner[f], nn = m.neigh[f];
-    const float psi = m.psi[f];
-    const float psi2 = 1.0f - psi;
+    const real psi = m.psi[f];
+    const real psi2 = 1.0f - psi;
     const bool up = Fl > 0.f;
-    float wo, wn;
+    real wo, wn;
     if (scheme == WCFD_CONV_CENTRAL) {
       wo = psi;
       wn = psi2;
@@ -83,28 +83,28 @@ __global__ void k_convection_weights(wcfd_mesh m, wcfd_bc bc, const float* __res
       wn = up ? 0.f : 1.f;
     }
     const bool own = m.slot_side[k * C + c] == 0;
-    const float r0 = m.skew[f], r1 = m.skew[Fi + f], r2 = m.skew[2 * Fi + f];
+    const real r0 = m.skew[f], r1 = m.skew[Fi + f], r2 = m.skew[2 * Fi + f];
     for (int o = 0; o < outs.n; ++o) {
-      float wx = 0.f;
+      real wx = 0.f;
       if (scheme == WCFD_CONV_UPWIND_LINEAR) {
         const int go = outs.g[o];
         const int u = up ? on : nn;
-        const float* d = fc2f + (size_t)(up ? 0 : 3) * Fi;
-        const float* gu = cg + (size_t)(3 * go) * C;
+        const real* d = fc2f + (size_t)(up ? 0 : 3) * Fi;
+        const real* gu = cg + (size_t)(3 * go) * C;
         wx = gu[u] * d[f] + gu[C + u] * d[Fi + f] + gu[2 * C + u] * d[2 * Fi + f];
       }
-      const float* g = cg + (size_t)(3 * o) * C;
-      const float g0 = g[on] * psi + psi2 * g[nn];
-      const float g1 = g[C + on] * psi + psi2 * g[C + nn];
-      const float g2 = g[2 * C + on] * psi + psi2 * g[2 * C + nn];
+      const real* g = cg + (size_t)(3 * o) * C;
+      const real g0 = g[on] * psi + psi2 * g[nn];
+      const real g1 = g[C + on] * psi + psi2 * g[C + nn];
+      const real g2 = g[2 * C + on] * psi + psi2 * g[2 * C + nn];
       wx = wx + (g0 * r0 + g1 * r1 + g2 * r2);
-      float w0, w1, w2;
+      real w0, w1, w2;
       if (own) {
         w0 = wo * Fl;
         w1 = wn * Fl;
         w2 = wx * Fl;
       } else {
-        const float mf = -Fl;
+        const real mf = -Fl;
         w0 = wn * mf;
         w1 = wo * mf;
         w2 = wx * mf;
@@ -123,18 +123,18 @@ __global__ void k_convection_weights(wcfd_mesh m, wcfd_bc bc, const float* __res
 // bc_mode 0: the faces' boundary conditions; bc_mode 1: every boundary face
 // gets the explicit nu * bc_value * A_f on output 0 (the von_neumann option;
 // the reference's dirchlet option builds the same kernel, diffusion.py:33).
-__global__ void k_diffusion_weights(wcfd_mesh m, wcfd_bc bc, const float* __restrict__ cg,
-                                    const float* __restrict__ fv,
-                                    const float* __restrict__ nu_face, float nu_scalar,
-                                    int correction, int bc_mode, float bc_value, Outs outs,
-                                    float* __restrict__ w) {
+__global__ void k_diffusion_weights(wcfd_mesh m, wcfd_bc bc, const real* __restrict__ cg,
+                                    const real* __restrict__ fv,
+                                    const real* __restrict__ nu_face, real nu_scalar,
+                                    int correction, int bc_mode, real bc_value, Outs outs,
+                                    real* __restrict__ w) {
   const int c = xcd_item();
   const int C = m.n_cells, F = m.n_faces, Fi = m.n_internal, Fb = m.n_boundary;
   if (c >= C) return;
   for (int k = 0; k < 6; ++k) {
     const int f = m.slot_face[k * C + c];
-    const float nu = nu_face ? nu_face[f] : nu_scalar;
-    const float A = m.area[f];
+    const real nu = nu_face ? nu_face[f] : nu_scalar;
+    const real A = m.area[f];
     if (f >= Fi) {
       const int j = f - Fi;
       if (bc_mode == 1) {
@@ -145,12 +145,12 @@ __global__ void k_diffusion_weights(wcfd_mesh m, wcfd_bc bc, const float* __rest
       }
       for (int o = 0; o < outs.n; ++o) {
         const int go = outs.g[o];
-        float w0, w2;
+        real w0, w2;
         if (bc.type[go * Fb + j] == WCFD_NEUMANN) {
           w0 = 0.f;
           w2 = nu * bc.value[go * Fb + j] * A;
         } else {
-          const float wt = nu * A / m.dmag[f];
+          const real wt = nu * A / m.dmag[f];
           w0 = -wt;
           w2 = wt * fv[(size_t)go * F + f];
         }
@@ -160,16 +160,16 @@ __global__ void k_diffusion_weights(wcfd_mesh m, wcfd_bc bc, const float* __rest
       }
       continue;
     }
-    const float wt = nu * A;
-    const float f1 = lap_factor(m, f, correction);
-    const float f0 = -f1;
-    float n2[3] = {0.f, 0.f, 0.f};
-    float psi = 0.f, psi2 = 0.f;
+    const real wt = nu * A;
+    const real f1 = lap_factor(m, f, correction);
+    const real f0 = -f1;
+    real n2[3] = {0.f, 0.f, 0.f};
+    real psi = 0.f, psi2 = 0.f;
     int on = 0, nn = 0;
     if (correction) {
-      const float d0 = m.dist[f], d1 = m.dist[Fi + f], d2 = m.dist[2 * Fi + f];
-      const float nf0 = m.normal[f], nf1 = m.normal[F + f], nf2 = m.normal[2 * F + f];
-      const float inv_nd = 1.0f / (nf0 * d0 + nf1 * d1 + nf2 * d2);
+      const real d0 = m.dist[f], d1 = m.dist[Fi + f], d2 = m.dist[2 * Fi + f];
+      const real nf0 = m.normal[f], nf1 = m.normal[F + f], nf2 = m.normal[2 * F + f];
+      const real inv_nd = 1.0f / (nf0 * d0 + nf1 * d1 + nf2 * d2);
       n2[0] = nf0 - d0 * inv_nd;
       n2[1] = nf1 - d1 * inv_nd;
       n2[2] = nf2 - d2 * inv_nd;
@@ -180,15 +180,15 @@ __global__ void k_diffusion_weights(wcfd_mesh m, wcfd_bc bc, const float* __rest
     }
     const bool own = m.slot_side[k * C + c] == 0;
     for (int o = 0; o < outs.n; ++o) {
-      float fx = 0.f;
+      real fx = 0.f;
       if (correction) {
-        const float* g = cg + (size_t)(3 * outs.g[o]) * C;
-        const float g0 = psi * g[on] + psi2 * g[nn];
-        const float g1 = psi * g[C + on] + psi2 * g[C + nn];
-        const float g2 = psi * g[2 * C + on] + psi2 * g[2 * C + nn];
+        const real* g = cg + (size_t)(3 * outs.g[o]) * C;
+        const real g0 = psi * g[on] + psi2 * g[nn];
+        const real g1 = psi * g[C + on] + psi2 * g[C + nn];
+        const real g2 = psi * g[2 * C + on] + psi2 * g[2 * C + nn];
         fx = g0 * n2[0] + g1 * n2[1] + g2 * n2[2];
       }
-      float w0, w1, w2;
+      real w0, w1, w2;
       if (own) {
         w0 = f0 * wt;
         w1 = f1 * wt;
@@ -209,12 +209,12 @@ __global__ void k_diffusion_weights(wcfd_mesh m, wcfd_bc bc, const float* __rest
 // ddt.py:20-33 (backwards Euler): w[o] = (V/dt, -phi V/dt).  The reference
 // indexes its (C, O, 2) weights by the global field (ddt.py:31-32); here by
 // the local output, which is the same for u, v, w.
-__global__ void k_ddt_weights(wcfd_mesh m, const float* __restrict__ cv, float dt, Outs outs,
-                              float* __restrict__ w) {
+__global__ void k_ddt_weights(wcfd_mesh m, const real* __restrict__ cv, real dt, Outs outs,
+                              real* __restrict__ w) {
   const int c = xcd_item();
   const int C = m.n_cells;
   if (c >= C) return;
-  const float vdt = m.volume[c] / dt;
+  const real vdt = m.volume[c] / dt;
   for (int o = 0; o < outs.n; ++o) {
     w[((size_t)o * 2 + 0) * C + c] = vdt;
     w[((size_t)o * 2 + 1) * C + c] = -cv[(size_t)outs.g[o] * C + c] * vdt;
@@ -222,12 +222,12 @@ __global__ void k_ddt_weights(wcfd_mesh m, const float* __restrict__ cv, float d
 }
 
 // grad.py:22-27: w[d][c] = (grad phi)_c[d] V_c (explicit, component-major)
-__global__ void k_grad_weights(wcfd_mesh m, const float* __restrict__ cg, int field,
-                               float* __restrict__ w) {
+__global__ void k_grad_weights(wcfd_mesh m, const real* __restrict__ cg, int field,
+                               real* __restrict__ w) {
   const int c = xcd_item();
   const int C = m.n_cells;
   if (c >= C) return;
-  const float V = m.volume[c];
+  const real V = m.volume[c];
 #pragma unroll
   for (int d = 0; d < 3; ++d) w[(size_t)d * C + c] = cg[(size_t)(3 * field + d) * C + c] * V;
 }
@@ -237,8 +237,8 @@ __global__ void k_grad_weights(wcfd_mesh m, const float* __restrict__ cg, int fi
 // scale w0 over all six faces, off-diagonal += scale w1 on internal faces;
 // calculate_RHS_values_kernel (84-101): b -= scale w2 on boundary faces only
 // (T2).
-__global__ void k_add_face_term(wcfd_mesh m, const float* __restrict__ w, int O, float scale,
-                                float* __restrict__ values, float* __restrict__ rhs) {
+__global__ void k_add_face_term(wcfd_mesh m, const real* __restrict__ w, int O, real scale,
+                                real* __restrict__ values, real* __restrict__ rhs) {
   const int c = xcd_item();
   const int C = m.n_cells, Fi = m.n_internal;
   if (c >= C) return;
@@ -246,12 +246,12 @@ __global__ void k_add_face_term(wcfd_mesh m, const float* __restrict__ w, int O,
 #pragma unroll
   for (int k = 0; k < 6; ++k) fk[k] = m.slot_face[k * C + c];
   for (int o = 0; o < O; ++o) {
-    float* v = values + (size_t)o * 7 * C;
-    float d = v[c];
+    real* v = values + (size_t)o * 7 * C;
+    real d = v[c];
 #pragma unroll
     for (int k = 0; k < 6; ++k) d = d + scale * w[widx(o, 0, k, c, C)];
     v[c] = d;
-    float b = rhs[(size_t)o * C + c];
+    real b = rhs[(size_t)o * C + c];
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
       if (fk[k] < Fi)
@@ -264,32 +264,32 @@ __global__ void k_add_face_term(wcfd_mesh m, const float* __restrict__ w, int O,
 }
 
 // add_Implicit_cell_based_term_kernel (equation_kernels.py:104-122)
-__global__ void k_add_cell_term(int C, const float* __restrict__ w, int O, float scale,
-                                float* __restrict__ values, float* __restrict__ rhs) {
+__global__ void k_add_cell_term(int C, const real* __restrict__ w, int O, real scale,
+                                real* __restrict__ values, real* __restrict__ rhs) {
   const int c = xcd_item();
   if (c >= C) return;
   for (int o = 0; o < O; ++o) {
-    float* v = values + (size_t)o * 7 * C;
+    real* v = values + (size_t)o * 7 * C;
     v[c] = v[c] + scale * w[((size_t)o * 2 + 0) * C + c];
     rhs[(size_t)o * C + c] = rhs[(size_t)o * C + c] + (-scale * w[((size_t)o * 2 + 1) * C + c]);
   }
 }
 
 // rhs += scale * arr (equation.py:120-121, 169-170)
-__global__ void k_add_rhs(int n, const float* __restrict__ arr, float scale,
-                          float* __restrict__ rhs) {
+__global__ void k_add_rhs(int n, const real* __restrict__ arr, real scale,
+                          real* __restrict__ rhs) {
   const int i = xcd_item();
   if (i < n) rhs[i] = rhs[i] + arr[i] * scale;
 }
 
 // implicit_relaxation_kernel (equation_kernels.py:128-152)
-__global__ void k_relax_equation(int C, const float* __restrict__ cv, Outs outs, float alpha,
-                                 float* __restrict__ values, float* __restrict__ rhs) {
+__global__ void k_relax_equation(int C, const real* __restrict__ cv, Outs outs, real alpha,
+                                 real* __restrict__ values, real* __restrict__ rhs) {
   const int c = xcd_item();
   if (c >= C) return;
   for (int o = 0; o < outs.n; ++o) {
-    float* v = values + (size_t)o * 7 * C;
-    const float a = v[c];
+    real* v = values + (size_t)o * 7 * C;
+    const real a = v[c];
     rhs[(size_t)o * C + c] =
         rhs[(size_t)o * C + c] + (1.0f - alpha) / alpha * a * cv[(size_t)outs.g[o] * C + c];
     v[c] = a / alpha;
@@ -299,8 +299,8 @@ __global__ void k_relax_equation(int C, const float* __restrict__ cv, Outs outs,
 // replace_row_kernel (equation_kernels.py:155-175): an identity row.  The
 // reference reads the right-hand side from a length-1 array at index row_id
 // (T7); the given value is used here for any row.
-__global__ void k_replace_row(int C, int o, int c, float value, float* __restrict__ values,
-                              float* __restrict__ rhs) {
+__global__ void k_replace_row(int C, int o, int c, real value, real* __restrict__ values,
+                              real* __restrict__ rhs) {
   const int k = threadIdx.x;
   if (k < 7) values[((size_t)o * 7 + k) * C + c] = k == 0 ? 1.0f : 0.0f;
   if (k == 0) rhs[(size_t)o * C + c] = value;
@@ -323,10 +323,10 @@ using namespace wcfd;
 
 extern "C" {
 
-int wcfd_convection_weights(const wcfd_mesh* m, wcfd_bc bc, const float* cell_grad,
-                            const float* face_values, const float* mass_flux,
-                            const float* face_c2f, int scheme, int flags, const int32_t* outputs,
-                            int n_outputs, float* weights, void* stream) {
+int wcfd_convection_weights(const wcfd_mesh* m, wcfd_bc bc, const real* cell_grad,
+                            const real* face_values, const real* mass_flux,
+                            const real* face_c2f, int scheme, int flags, const int32_t* outputs,
+                            int n_outputs, real* weights, void* stream) {
   Outs o;
   if (!mesh_ok(m) || !cell_grad || !face_values || !mass_flux || !weights || !bc.type ||
       !bc.value || !outs_of(outputs, n_outputs, o))
@@ -342,10 +342,10 @@ int wcfd_convection_weights(const wcfd_mesh* m, wcfd_bc bc, const float* cell_gr
   return check_launch();
 }
 
-int wcfd_diffusion_weights(const wcfd_mesh* m, wcfd_bc bc, const float* cell_grad,
-                           const float* face_values, const float* viscosity_faces,
+int wcfd_diffusion_weights(const wcfd_mesh* m, wcfd_bc bc, const real* cell_grad,
+                           const real* face_values, const real* viscosity_faces,
                            float viscosity, int correction, int bc_mode, float bc_value,
-                           const int32_t* outputs, int n_outputs, float* weights, void* stream) {
+                           const int32_t* outputs, int n_outputs, real* weights, void* stream) {
   Outs o;
   if (!mesh_ok(m) || !face_values || !weights || !outs_of(outputs, n_outputs, o))
     return WCFD_EINVAL;
@@ -360,8 +360,8 @@ int wcfd_diffusion_weights(const wcfd_mesh* m, wcfd_bc bc, const float* cell_gra
   return check_launch();
 }
 
-int wcfd_ddt_weights(const wcfd_mesh* m, const float* cell_values, float dt,
-                     const int32_t* outputs, int n_outputs, float* weights, void* stream) {
+int wcfd_ddt_weights(const wcfd_mesh* m, const real* cell_values, float dt,
+                     const int32_t* outputs, int n_outputs, real* weights, void* stream) {
   Outs o;
   if (!mesh_ok(m) || !cell_values || !weights || !(dt > 0.f) ||
       !outs_of(outputs, n_outputs, o))
@@ -372,7 +372,7 @@ int wcfd_ddt_weights(const wcfd_mesh* m, const float* cell_values, float dt,
   return check_launch();
 }
 
-int wcfd_grad_weights(const wcfd_mesh* m, const float* cell_grad, int field, float* weights,
+int wcfd_grad_weights(const wcfd_mesh* m, const real* cell_grad, int field, real* weights,
                       void* stream) {
   if (!mesh_ok(m) || !cell_grad || !weights || field < 0 || field > 3) return WCFD_EINVAL;
   hipStream_t s = (hipStream_t)stream;
@@ -381,8 +381,8 @@ int wcfd_grad_weights(const wcfd_mesh* m, const float* cell_grad, int field, flo
   return check_launch();
 }
 
-int wcfd_equation_add_face_term(const wcfd_mesh* m, const float* weights, int n_outputs,
-                                float scale, float* values, float* rhs, void* stream) {
+int wcfd_equation_add_face_term(const wcfd_mesh* m, const real* weights, int n_outputs,
+                                float scale, real* values, real* rhs, void* stream) {
   if (!mesh_ok(m) || !weights || !values || !rhs || n_outputs < 1 || n_outputs > 4)
     return WCFD_EINVAL;
   hipStream_t s = (hipStream_t)stream;
@@ -391,8 +391,8 @@ int wcfd_equation_add_face_term(const wcfd_mesh* m, const float* weights, int n_
   return check_launch();
 }
 
-int wcfd_equation_add_cell_term(const wcfd_mesh* m, const float* weights, int n_outputs,
-                                float scale, float* values, float* rhs, void* stream) {
+int wcfd_equation_add_cell_term(const wcfd_mesh* m, const real* weights, int n_outputs,
+                                float scale, real* values, real* rhs, void* stream) {
   if (!mesh_ok(m) || !weights || !values || !rhs || n_outputs < 1 || n_outputs > 4)
     return WCFD_EINVAL;
   hipStream_t s = (hipStream_t)stream;
@@ -401,15 +401,15 @@ int wcfd_equation_add_cell_term(const wcfd_mesh* m, const float* weights, int n_
   return check_launch();
 }
 
-int wcfd_equation_add_rhs(int32_t n, const float* arr, float scale, float* rhs, void* stream) {
+int wcfd_equation_add_rhs(int32_t n, const real* arr, float scale, real* rhs, void* stream) {
   if (n <= 0 || !arr || !rhs) return WCFD_EINVAL;
   hipStream_t s = (hipStream_t)stream;
   WCFD_LAUNCH(K_EQUATION, s, k_add_rhs, grid_for(n), n, arr, scale, rhs);
   return check_launch();
 }
 
-int wcfd_equation_relax(const wcfd_mesh* m, const float* cell_values, const int32_t* outputs,
-                        int n_outputs, float alpha, float* values, float* rhs, void* stream) {
+int wcfd_equation_relax(const wcfd_mesh* m, const real* cell_values, const int32_t* outputs,
+                        int n_outputs, float alpha, real* values, real* rhs, void* stream) {
   Outs o;
   if (!mesh_ok(m) || !cell_values || !values || !rhs || !(alpha > 0.f) ||
       !outs_of(outputs, n_outputs, o))
@@ -421,7 +421,7 @@ int wcfd_equation_relax(const wcfd_mesh* m, const float* cell_values, const int3
 }
 
 int wcfd_equation_replace_row(const wcfd_mesh* m, int n_outputs, int output, int32_t cell,
-                              float value, float* values, float* rhs, void* stream) {
+                              float value, real* values, real* rhs, void* stream) {
   if (!mesh_ok(m) || !values || !rhs || n_outputs < 1 || n_outputs > 4 || output < 0 ||
       output >= n_outputs || cell < 0 || cell >= m->n_cells)
     return WCFD_EINVAL;

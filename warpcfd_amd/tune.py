@@ -40,14 +40,14 @@ def _laplacian_ell(dm):
     sf = dm.slot_face.long()
     w = dm.area[sf] / dm.dmag[sf]
     w = torch.where(dm.slot_pos >= 0, w, torch.zeros_like(w))
-    ell = torch.empty(7, C, dtype=torch.float32, device=w.device)
+    ell = torch.empty(7, C, dtype=dm.dtype, device=w.device)
     ell[1:] = w
     ell[0] = -(w.sum(dim=0)) * 1.001
     return ell.contiguous()
 
 
-def install(grid, block, fold=FOLDS[0]):
-    lib = _lib.load()
+def install(grid, block, fold=FOLDS[0], lib=None):
+    lib = lib if lib is not None else _lib.load()
     _lib.check(lib.wcfd_set_krylov_launch(grid, block), 'set_krylov_launch')
     _lib.check(lib.wcfd_set_krylov_fold(fold), 'set_krylov_fold')
 
@@ -55,20 +55,20 @@ def install(grid, block, fold=FOLDS[0]):
 def time_solvers(dm, grid, block, iters=60, reps=2, cg_variant=0, fold=FOLDS[0]):
     """Milliseconds of `iters` CG iterations and `iters` 3-vector BiCGSTAB
     iterations at launch shape (grid, block) and inline-fold limit ``fold``."""
-    lib = _lib.load()
-    install(grid, block, fold)
+    lib = dm.lib
+    install(grid, block, fold, lib)
     C = dm.n_cells
     dev = dm.device
     A = _laplacian_ell(dm)
     g = torch.Generator(device='cpu').manual_seed(0)
-    b = torch.randn(C, generator=g).to(dev)
-    b3 = torch.randn(3, C, generator=g).to(dev)
+    b = torch.randn(C, generator=g).to(dev, dm.dtype)
+    b3 = torch.randn(3, C, generator=g).to(dev, dm.dtype)
     ws = torch.zeros(lib.wcfd_krylov_workspace_bytes(C, 3), dtype=torch.uint8, device=dev)
     info = _lib.SolveInfo()
     best = [float('inf'), float('inf')]
     for _ in range(reps):
         for j, nv in enumerate((1, 3)):
-            x = torch.zeros(nv, C, dtype=torch.float32, device=dev)
+            x = torch.zeros(nv, C, dtype=dm.dtype, device=dev)
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
             torch.cuda.synchronize()
@@ -116,10 +116,10 @@ def autotune_krylov(dm, grids=GRIDS, blocks=BLOCKS, iters=60, cg_weight=1.0, bic
     """Pick and install the fastest (grid, block, fold) for mesh ``dm``
     (a DeviceMesh).  The weights reflect a SIMPLE iteration, where the
     pressure CG runs far more iterations than the momentum BiCGSTAB."""
-    key = (dm.n_cells, cg_variant)
+    key = (dm.n_cells, cg_variant, dm.lib.wcfd_real_bytes())
     if key in _cache:
         c = _cache[key]
-        install(c['grid'], c['block'], c['fold'])
+        install(c['grid'], c['block'], c['fold'], dm.lib)
         return c
     results = []
     for grid, blk, fold in krylov_candidates(dm.n_cells, grids, blocks):
@@ -127,7 +127,7 @@ def autotune_krylov(dm, grids=GRIDS, blocks=BLOCKS, iters=60, cg_weight=1.0, bic
         results.append({'grid': grid, 'block': blk, 'fold': fold, 'cg_ms': cg_ms,
                         'bicg_ms': bi_ms, 'score': cg_weight * cg_ms + bicg_weight * bi_ms})
     best = min(results, key=lambda r: r['score'])
-    install(best['grid'], best['block'], best['fold'])
+    install(best['grid'], best['block'], best['fold'], dm.lib)
     out = dict(best, iters=iters, candidates=results)
     _cache[key] = out
     return out
