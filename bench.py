@@ -72,6 +72,19 @@ def apply_bcs(b, kind):
         b.slip_wall('+Z'); b.slip_wall('-Z')
 
 
+def fp64_error(fields32, run64):
+    """rel-L2 of the float32 fields against the double build's, per field
+    and for the velocity vector; ``run64()`` returns the double build's
+    (C, 4) fields after the same iterations."""
+    import numpy as np
+    a = np.asarray(fields32, np.float64)
+    b = np.asarray(run64(), np.float64)
+    rel = lambda x, y: float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300))
+    out = {name: rel(a[:, q], b[:, q]) for q, name in enumerate('uvwp')}
+    out['velocity'] = rel(a[:, :3], b[:, :3])
+    return out
+
+
 def calibration_copy(mb):
     """One elementwise kernel of known traffic (reads and writes ``mb`` MiB)
     for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings."""
@@ -234,10 +247,11 @@ def bench_batch64(args):
     arena, and the same 64 cases one after another through the plain
     single-case solver.  One step = one solve of the whole batch."""
     print(json.dumps(run_batch64(args, sequential=not args.no_sequential,
-                                 profile=not args.no_profile, cpu=not args.no_cpu_baseline)))
+                                 profile=not args.no_profile, cpu=not args.no_cpu_baseline,
+                                 fp64_check=not args.no_fp64)))
 
 
-def run_batch64(args, sequential=True, profile=True, cpu=True):
+def run_batch64(args, sequential=True, profile=True, cpu=True, fp64_check=False):
     """bench_batch64's measurement; returns the result dict."""
     import numpy as np
     import torch
@@ -360,6 +374,30 @@ def run_batch64(args, sequential=True, profile=True, cpu=True):
                'max_rel_l2_velocity_vs_batched': float(np.max(dev))}
         print(f'# sequential: {tot:.2f} s', file=sys.stderr, flush=True)
 
+    fp64 = None
+    if fp64_check:
+        import copy
+        specs64 = []
+        for k, sp in enumerate(specs):
+            s64 = copy.copy(sp)
+            s64.stop, s64.stop_u, s64.floor_u = 0.0, 0.0, 0.0    # run exactly its iterations
+            s64.max_outer = iters[k]
+            specs64.append(s64)
+        t = time.perf_counter()
+        a64 = CaseArena.for_cases(specs64, dtype=torch.float64, **kw)
+        a64.run(max_steps=max(iters) + 1)
+        errs = []
+        for k in range(len(specs)):
+            a = arena.cell_values(k).astype(np.float64)
+            b = a64.cell_values(k).astype(np.float64)
+            errs.append(float(np.linalg.norm(a[:, :3] - b[:, :3]) / np.linalg.norm(b[:, :3])))
+        fp64 = {'velocity_rel_l2': {'max': max(errs), 'median': float(np.median(errs))},
+                'seconds': round(time.perf_counter() - t, 1),
+                'note': 'every case through a double-field arena (libwcfd64.so) for the same '
+                        'number of SIMPLE iterations its float32 solve ran'}
+        print(f'# fp64 check: {fp64}', file=sys.stderr, flush=True)
+        del a64
+
     cpu_b = None
     if cpu:
         cpu_b = cpu_solve_baseline(draws, stop, pol['rtol0'], stop_u)
@@ -400,6 +438,7 @@ def run_batch64(args, sequential=True, profile=True, cpu=True):
                                             'p99': round(float(np.percentile(lat, 99)), 3)}},
         'sequential': seq,
         'speedup_vs_sequential': round(seq['total_s'] / batched_s, 2) if seq else None,
+        'error_vs_fp64_build': fp64,
         'host_preprocessing_s': round(prep_s, 1),
         'roofline': roofline,
         'cpu_baseline': cpu_b,
@@ -656,12 +695,38 @@ def bench_cylinder(args):
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     hist = sv.history[-args.steps:]
+    fields32 = model.cell_values.numpy()
     roofline = kernels = None
     if not args.no_profile:
         def run():
             for _ in range(min(args.steps, 5)):
                 sv.outer_iteration(dt=dt)
         roofline, kernels = profile_roofline(run, C, F, Fi, Fb, nnz, args.cg == 'single')
+    fp64 = None
+    if not args.no_fp64:
+        import numpy as np
+        from warpcfd_amd.FV import FVM
+        from warpcfd_amd.workloads import cylinder_bcs
+
+        def run64():
+            from warpcfd_amd.tune import install
+            m64 = FVM(model.mesh, float_dtype=np.float64)
+            m64.material.create_incompressible_newtonian_fluid(
+                viscosity=model.material.viscosity, density=1.0)
+            cylinder_bcs(m64.boundary)
+            m64.finalize(layout=model.layout)
+            if tuned:
+                install(tuned['grid'], tuned['block'], tuned['fold'], m64.lib)
+            s64 = IncompressibleSolver(m64, None, None, steady_state=False, implicit_outflow=True)
+            s64.set_iterations(1, 2, 1)
+            s64.cg_variant, s64.p_refine, s64.tol = sv.cg_variant, sv.p_refine, sv.tol
+            for _ in range(args.warmup + args.steps):
+                s64.outer_iteration(dt=dt)
+            return m64.cell_values.numpy()
+        fp64 = fp64_error(fields32, run64)
+        fp64['note'] = (f'the same workload through the double-field build (libwcfd64.so), same '
+                        f'Krylov settings, after the same {args.warmup + args.steps} time steps '
+                        f'from rest')
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_cylinder_baseline(C, args.p_refine)
@@ -689,6 +754,7 @@ def bench_cylinder(args):
         'roofline': roofline,
         'cpu_baseline': cpu,
         'continuity_residual': hist[-1]['continuity'],
+        'error_vs_fp64_build': fp64,
         'krylov_iters_per_step': {'momentum': float(np.mean([h_['u_iters'] for h_ in hist])),
                                   'pressure_last_corrector': float(np.mean([h_['p_iters'] for h_ in hist]))},
         'krylov_launch': tuned,
@@ -849,6 +915,8 @@ def main():
     ap.add_argument('--no-extras', action='store_true',
                     help='channel: skip the batch64 and served-sample sub-benchmarks')
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-fp64', action='store_true',
+                    help='skip the comparison with the double-field build')
     ap.add_argument('--no-profile', action='store_true')
     ap.add_argument('--no-tune', action='store_true')
     ap.add_argument('--cg', default='single', choices=['standard', 'single'],
@@ -929,6 +997,7 @@ def main():
     t1 = time.perf_counter()
     elapsed = t1 - t0
     hist = solver.history[-args.steps:]
+    fields32 = model.cell_values.numpy()
 
     roofline = None
     kernels = None
@@ -956,6 +1025,29 @@ def main():
             if 'bytes_per_launch' in v:
                 tk = traffic(args.traffic_from, sym.get(k, 'k_' + k), args.traffic_copy_mb)
                 v['traffic'] = tk['bytes_per_launch']
+
+    fp64 = None
+    if not args.no_fp64:
+        def run64():
+            from warpcfd_amd.tune import install
+            m64 = FVM(m, float_dtype=np.float64)
+            m64.material.create_incompressible_newtonian_fluid(viscosity=0.01, density=1.0)
+            if wl['kind'] == 'cavity':
+                m64.set_reference_pressure(0, 0.0)
+            apply_bcs(m64.boundary, wl['kind'])
+            m64.finalize(layout=model.layout)
+            if tuned:
+                install(tuned['grid'], tuned['block'], tuned['fold'], m64.lib)
+            s64 = IncompressibleSolver(m64, 0.7, 0.3, implicit_outflow=(wl['kind'] == 'channel'))
+            s64.cg_variant = solver.cg_variant
+            s64.tol = solver.tol           # the same Krylov settings as the float32 run
+            for _ in range(args.warmup + args.steps):
+                s64.outer_iteration()
+            return m64.cell_values.numpy()
+        fp64 = fp64_error(fields32, run64)
+        fp64['note'] = (f'the same workload through the double-field build (libwcfd64.so), same '
+                        f'Krylov settings and launch shape, after the same '
+                        f'{args.warmup + args.steps} SIMPLE iterations from rest')
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -991,6 +1083,7 @@ def main():
         'continuity_residual': hist[-1]['continuity'],
         'krylov_iters_per_step': {'momentum': float(np.mean([h['u_iters'] for h in hist])),
                                   'pressure': float(np.mean([h['p_iters'] for h in hist]))},
+        'error_vs_fp64_build': fp64,
         'krylov_launch': tuned,
         'host_preprocessing_s': round(prep_s, 1),
         'kernels': kernels,
