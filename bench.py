@@ -750,7 +750,8 @@ def bench_cylinder(args):
                                f'pressure-residual refinement ({args.p_refine} sweep)',
                    'cells': C, 'faces': F, 'nnz': nnz, 'reductions': 'f64',
                    'krylov': {'atol': 1e-5, 'maxiter': 500, 'check_every': 10,
-                              'pressure_cg': args.cg, 'p_refine': args.p_refine}},
+                              'pressure_cg': args.cg, 'p_refine': args.p_refine,
+                              'cg_stop_test': args.krylov_check}},
         'roofline': roofline,
         'cpu_baseline': cpu,
         'continuity_residual': hist[-1]['continuity'],
@@ -925,12 +926,19 @@ def main():
     ap.add_argument('--krylov-launch', default=None,
                     help='GRIDxBLOCK[xFOLD]: fixed Krylov launch shape (grid 0: one row per '
                          'thread; FOLD: inline-fold limit, default 2048) instead of autotuning')
+    ap.add_argument('--krylov-check', default='inline', choices=['inline', 'kernel'],
+                    help='single-case CG stop test: inside the first iteration kernel of each '
+                         'check group (default) or in a check kernel after it (bitwise the same)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py runs on one GPU only: this build has no multi-GPU path '
               '(cases shard across GPUs with no communication; see the design notes).',
               file=sys.stderr)
         sys.exit(2)
+    if args.krylov_check != 'inline':
+        from warpcfd_amd import _lib
+        for lib in (_lib.load(), _lib.load64()):
+            _lib.check(lib.wcfd_set_krylov_inline_check(0), 'set_krylov_inline_check')
     if args.workload == 'batch64':
         return bench_batch64(args)
     if args.workload == 'serve':
@@ -1077,7 +1085,8 @@ def main():
         'data': 'synthetic',
         'config': {'workload': wl['desc'], 'cells': C, 'faces': F, 'nnz': nnz,
                    'reductions': 'f64', 'krylov': {'atol': 1e-5, 'maxiter': 500, 'check_every': 10,
-                                                   'pressure_cg': args.cg}},
+                                                   'pressure_cg': args.cg,
+                                                   'cg_stop_test': args.krylov_check}},
         'roofline': roofline,
         'cpu_baseline': cpu,
         'continuity_residual': hist[-1]['continuity'],

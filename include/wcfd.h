@@ -22,7 +22,8 @@
  *    error is kept per thread and readable with wcfd_last_hip_error().
  *  - `stream` is a hipStream_t passed as void* (0 = the null stream).
  *  - Functions keep no global mutable state, except the opt-in profiler and
- *    the Krylov launch shape and fold limit (wcfd_set_krylov_launch / _fold),
+ *    the Krylov launch shape, fold limit and stop-test placement
+ *    (wcfd_set_krylov_launch / _fold / _inline_check),
  *    which are process-wide: two callers that tune for different meshes must
  *    not interleave their set and solve calls.  The fold limit never changes
  *    results; the launch shape sets the order of the float64 partial sums.
@@ -446,6 +447,17 @@ int wcfd_set_krylov_launch(int grid, int block);
 int wcfd_get_krylov_launch(int* grid, int* block); /* [host] outputs */
 int wcfd_set_krylov_fold(int max_inline);           /* max_inline >= 0 */
 int wcfd_get_krylov_fold(void);
+
+/* Where the stop test of a single-case, single-reduction CG solve is made:
+ * on (1, the default) the first iteration kernel of every check group makes
+ * it in its prologue, from the partials it folds anyway; off (0) a check
+ * kernel follows every group.  The iterations run, the solution and the
+ * reported residual are bitwise the same either way; on saves one launch
+ * per group.  Batched solves and fixed launch budgets always use the check
+ * kernel.  Process-wide.  No counterpart in the reference (warp.optim.linear
+ * tests the residual on the host every check_every iterations). */
+int wcfd_set_krylov_inline_check(int on);           /* 0 or 1 */
+int wcfd_get_krylov_inline_check(void);
 
 /* wcfd_cg / wcfd_bicgstab on the ELL matrix `values` [7][C] of mesh m. */
 int wcfd_cg_ell(const wcfd_mesh* m, const wcfd_real* values, int variant, const wcfd_real* b, wcfd_real* x,

@@ -164,6 +164,35 @@ __global__ void k_check(Seg sg, const double* __restrict__ part, KState* st, int
   }
 }
 
+// The stop test of k_check made inside an iteration kernel (single-case
+// single-reduction CG): the kernel of iteration `it` = m * check_every folds
+// ||r_it||^2 with its other partials, and its case's first workgroup sets the
+// flags and writes the record; the whole case stops before updating when the
+// test passes.  Every workgroup of the case folds the same partials in the
+// same order as k_check does, so each reaches the same decision, and the
+// iterations run, the result and the record are those of a separate check.
+__device__ inline void publish_flags(KFlags* out, const KFlags& f) {
+  if (out) {
+    *out = f;
+    __threadfence_system();   // visible to the host once the kernel has completed
+  }
+}
+
+// true: stop (the case's flags are set); called by every thread of the case
+__device__ inline bool inline_check(const BlockRows& br, KState* S, double rr, int it,
+                                    int maxiter, KFlags* out) {
+  const bool stop = !(rr > S->f.thr2) || it >= maxiter;
+  if (br.vb == br.b0 && threadIdx.x == 0) {
+    S->f.rr = rr;
+    if (stop) {
+      S->f.done = 1;
+      S->f.iters = it;
+    }
+    publish_flags(out, S->f);
+  }
+  return stop;
+}
+
 // Prefold, one workgroup per case (launched with the iteration kernels'
 // workgroup size): the ND partial rows of a case are folded in the order of
 // fold() and the sums replace the case's first partial of each row.  Every
@@ -401,24 +430,31 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_init2(M A, Seg sg, cons
   store_partials<3>(acc, PA, sg.G, br.vb);
 }
 
-template <class M, bool PF, bool FIRST>
+template <class M, bool PF, bool FIRST, bool CHECK = false>
 __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int it, real* __restrict__ x,
                            const real* __restrict__ r_in, real* __restrict__ r_out,
                            const real* __restrict__ w_in, real* __restrict__ w_out,
                            const real* __restrict__ s_in, real* __restrict__ s_out,
                            real* __restrict__ p, const real* __restrict__ inv,
                            const double* __restrict__ P_in, double* __restrict__ P_out,
-                           KState* st) {
+                           KState* st, int maxiter, KFlags* out) {
+  static_assert(!(FIRST && CHECK), "iteration 0 is checked by the init fold");
   const BlockRows br = block_rows(sg);
   KState* S = st + br.k;
-  if (S->f.done) return;
+  if (S->f.done) {
+    if (CHECK && br.vb == br.b0 && threadIdx.x == 0) publish_flags(out, S->f);
+    return;
+  }
   const int G = sg.G;
   // one row per thread on an ELL matrix: issue the row's loads before the
   // fold, and the fold's own loads (and the case's scalars) before those
   constexpr bool pf = PF && M::kEll;
+  constexpr int ND = CHECK ? 3 : 2;   // r.u, w.u (and r.r for the stop test)
   const int pv = (it - 1) & 1;
-  const double* pp[2] = {P_in + br.b0, P_in + G + br.b0};
-  EarlyFold<2> ef;
+  const double* pp[ND];
+#pragma unroll
+  for (int i = 0; i < ND; ++i) pp[i] = P_in + (size_t)i * G + br.b0;
+  EarlyFold<ND> ef;
   double rz_prev = 0.0, alpha_prev = 0.0;
   if constexpr (!FIRST) {
     if constexpr (pf) ef.load(pp, br.nf);
@@ -451,11 +487,13 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int i
   if constexpr (FIRST) {
     alpha_d = S->alpha[0];
   } else {
-    double gd[2];
+    double gd[ND];
     if constexpr (pf)
       ef.finish(pp, br.nf, gd);
     else
-      fold_n<2>(pp, br.nf, gd);
+      fold_n<ND>(pp, br.nf, gd);
+    if constexpr (CHECK)
+      if (inline_check(br, S, gd[2], it, maxiter, out)) return;
     const double g = gd[0], d = gd[1];
     beta_d = safe_div(g, rz_prev);
     alpha_d = safe_div(g, d - beta_d * safe_div(g, alpha_prev));
@@ -522,23 +560,30 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int i
 // partials and the case's scalars are loaded before the rows (EarlyFold), so
 // the fold completes while the gathers are in flight.  FIRST (iteration 0:
 // no fold, s = p = 0) is a separate instance, without the branches.
-template <class M, bool FIRST>
+template <class M, bool FIRST, bool CHECK = false>
 __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter2(M A, Seg sg, int it, real* __restrict__ x,
                            const real* __restrict__ r_in, real* __restrict__ r_out,
                            const real* __restrict__ w_in, real* __restrict__ w_out,
                            const real* __restrict__ s_in, real* __restrict__ s_out,
                            real* __restrict__ p, const real* __restrict__ inv,
                            const double* __restrict__ P_in, double* __restrict__ P_out,
-                           KState* st) {
+                           KState* st, int maxiter, KFlags* out) {
+  static_assert(!(FIRST && CHECK), "iteration 0 is checked by the init fold");
   const BlockRows br = block_rows(sg);
   KState* S = st + br.k;
-  if (S->f.done) return;
+  if (S->f.done) {
+    if (CHECK && br.vb == br.b0 && threadIdx.x == 0) publish_flags(out, S->f);
+    return;
+  }
   const int G = sg.G;
   static_assert(M::kEll, "the two-row form reads ELL rows");
   constexpr int NR = 2;
+  constexpr int ND = CHECK ? 3 : 2;
   const int pv = (it - 1) & 1;
-  const double* pp[2] = {P_in + br.b0, P_in + G + br.b0};
-  EarlyFold<2> ef;
+  const double* pp[ND];
+#pragma unroll
+  for (int i = 0; i < ND; ++i) pp[i] = P_in + (size_t)i * G + br.b0;
+  EarlyFold<ND> ef;
   double rz_prev = 0.0, alpha_prev = 0.0;
   if constexpr (!FIRST) {
     ef.load(pp, br.nf);
@@ -576,8 +621,10 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter2(M A, Seg sg, int 
   if constexpr (FIRST) {
     alpha_d = S->alpha[0];
   } else {
-    double gd[2];
+    double gd[ND];
     ef.finish(pp, br.nf, gd);
+    if constexpr (CHECK)
+      if (inline_check(br, S, gd[2], it, maxiter, out)) return;
     const double g = gd[0], d = gd[1];
     beta_d = safe_div(g, rz_prev);
     alpha_d = safe_div(g, d - beta_d * safe_div(g, alpha_prev));
@@ -1120,8 +1167,61 @@ static int run_groups(const Seg& sg, int B, KState* st, RrPart&& rr_part, int ma
   return finish(last, K, info);
 }
 
+// run_groups for one case whose iteration kernels make the stop test
+// themselves: iterations 0 .. ce-1, then units of [iteration it with the
+// test of r_it (its record goes to a pinned slot, an event follows),
+// iterations it+1 .. it+ce-1], it = ce, 2 ce, ...  The unit with it >=
+// maxiter holds the test alone.  The host keeps `depth` units in flight and
+// stops launching once a polled record reports the case done, as in
+// run_groups; the iterations run and the result are the same, and no check
+// kernel is launched.
+template <class LaunchIter>
+static int run_groups_inline(int maxiter, int ce, LaunchIter&& iteration,
+                             wcfd_solve_info* info, hipStream_t s) {
+  HostSync* hs = host_sync(1);
+  if (!hs) return WCFD_EHIP;
+  const size_t prof_from = prof_mark();
+  int e;
+  for (int it = 0; it < ce; ++it) iteration(it, false, nullptr);
+  if ((e = check_launch())) return e;
+  const int depth = 2;
+  int launched = 0, polled = 0, it = ce;
+  bool more = true;
+  const KFlags* last = nullptr;
+  while (true) {
+    if (more && launched - polled < depth) {
+      const int sl = launched % kSlots;
+      iteration(it, true, hs->dslot + (size_t)sl * hs->cap);
+      if ((e = check_launch())) return e;
+      if ((e = check_hip(hipEventRecord(hs->ev[sl], s)))) return e;
+      ++launched;
+      if (it >= maxiter) {
+        more = false;   // this test stops the case
+      } else {
+        for (int j = it + 1; j < it + ce; ++j) iteration(j, false, nullptr);
+        if ((e = check_launch())) return e;
+      }
+      it += ce;
+      continue;
+    }
+    if (polled == launched) break;   // cannot happen: the last test sets the flag
+    const int sl = polled % kSlots;
+    if ((e = check_hip(hipEventSynchronize(hs->ev[sl])))) return e;
+    last = hs->slot + (size_t)sl * hs->cap;
+    ++polled;
+    if (all_done(last, 1)) break;
+  }
+  if (launched > polled) {   // queued tests still write their slots
+    if ((e = check_hip(hipEventRecord(hs->tail, s)))) return e;
+    hs->tail_pending = true;
+  }
+  if (last) prof_retire(prof_from, last[0].iters);
+  return finish(last, 1, info);
+}
+
 namespace {
 std::atomic<int> g_krylov_grid{kKrylovGrid};
+std::atomic<int> g_inline_check{1};
 std::atomic<int> g_krylov_block{kBlock};
 std::atomic<int> g_krylov_fold{kMaxInlineFold};
 }  // namespace
@@ -1237,37 +1337,51 @@ static int cgs_impl(const M& A, const KLaunch& L, const real* b, real* x, const 
   WCFD_LAUNCH(K_FOLD, s, k_init_fold, K, sg, w.PA, w.PC, w.st, 2, maxiter);
   if ((e = check_launch())) return e;
   const int ce = check_every > 0 ? check_every : 1;
-  auto group = [&](int it0) {
-    for (int it = it0; it < it0 + ce; ++it) {
-      prof_set_iter(it);
-      const int cur = it & 1;
+  // one iteration; `check`: the kernel first makes the stop test of the
+  // iterations before it (writing the flag record to `out`)
+  auto iteration = [&](int it, bool check, KFlags* out) {
+    prof_set_iter(it);
+    const int cur = it & 1;
 #define WCFD_CGS_ITER(KERNEL)                                                             \
   WCFD_LAUNCH_B(K_CG_1, s, KERNEL, G, B, A, sg, it, x, rb[cur], rb[1 - cur], wb[cur],       \
-                wb[1 - cur], sb[cur], sb[1 - cur], w.p0, w.inv, pb[cur], pb[1 - cur], w.st)
-      const bool first = it == 0;
-      if (rows == 1) {
-        if (first)
-          WCFD_CGS_ITER((k_cgs_iter<M, true, true>));
-        else
-          WCFD_CGS_ITER((k_cgs_iter<M, true, false>));
-      } else if constexpr (M::kEll) {
-        if (rows == 2 && first)
-          WCFD_CGS_ITER((k_cgs_iter2<M, true>));
-        else if (rows == 2)
-          WCFD_CGS_ITER((k_cgs_iter2<M, false>));
-        else if (first)
-          WCFD_CGS_ITER((k_cgs_iter<M, false, true>));
-        else
-          WCFD_CGS_ITER((k_cgs_iter<M, false, false>));
-      } else if (first) {
+                wb[1 - cur], sb[cur], sb[1 - cur], w.p0, w.inv, pb[cur], pb[1 - cur], w.st,  \
+                maxiter, out)
+    const bool first = it == 0;
+    if (rows == 1) {
+      if (first)
+        WCFD_CGS_ITER((k_cgs_iter<M, true, true>));
+      else if (check)
+        WCFD_CGS_ITER((k_cgs_iter<M, true, false, true>));
+      else
+        WCFD_CGS_ITER((k_cgs_iter<M, true, false>));
+    } else if constexpr (M::kEll) {
+      if (rows == 2 && first)
+        WCFD_CGS_ITER((k_cgs_iter2<M, true>));
+      else if (rows == 2 && check)
+        WCFD_CGS_ITER((k_cgs_iter2<M, false, true>));
+      else if (rows == 2)
+        WCFD_CGS_ITER((k_cgs_iter2<M, false>));
+      else if (first)
         WCFD_CGS_ITER((k_cgs_iter<M, false, true>));
-      } else {
+      else if (check)
+        WCFD_CGS_ITER((k_cgs_iter<M, false, false, true>));
+      else
         WCFD_CGS_ITER((k_cgs_iter<M, false, false>));
-      }
-#undef WCFD_CGS_ITER
-      prefold<3>(L, pb[1 - cur], w.st, s);
+    } else if (first) {
+      WCFD_CGS_ITER((k_cgs_iter<M, false, true>));
+    } else if (check) {
+      WCFD_CGS_ITER((k_cgs_iter<M, false, false, true>));
+    } else {
+      WCFD_CGS_ITER((k_cgs_iter<M, false, false>));
     }
+#undef WCFD_CGS_ITER
+    prefold<3>(L, pb[1 - cur], w.st, s);
     prof_set_iter(-1);
+  };
+  if (K == 1 && tl.fixed_groups <= 0 && g_inline_check.load())
+    return run_groups_inline(maxiter, ce, iteration, info, s);
+  auto group = [&](int it0) {
+    for (int it = it0; it < it0 + ce; ++it) iteration(it, false, nullptr);
   };
   // the r.r partials of the last iteration (it_end - 1) are in pb[it_end & 1], slot 2
   auto rr_part = [&](int it_end) { return (const double*)(pb[it_end & 1] + 2 * G); };
@@ -1512,6 +1626,14 @@ int wcfd_set_krylov_fold(int max_inline) {
 }
 
 int wcfd_get_krylov_fold(void) { return g_krylov_fold.load(); }
+
+int wcfd_set_krylov_inline_check(int on) {
+  if (on != 0 && on != 1) return WCFD_EINVAL;
+  g_inline_check = on;
+  return WCFD_OK;
+}
+
+int wcfd_get_krylov_inline_check(void) { return g_inline_check.load(); }
 
 int wcfd_get_krylov_launch(int* grid, int* block) {
   if (!grid || !block) return WCFD_EINVAL;
