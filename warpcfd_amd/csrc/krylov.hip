@@ -65,6 +65,25 @@ __global__ void k_inv_diag(M A, Items items, int n, real* __restrict__ inv) {
   inv[i] = d != 0.0f ? 1.0f / d : 1.0f;
 }
 
+// The Jacobi-scaled matrix A M of an ELL matrix (M = diag(inv)): column j
+// of every row times inv[j], in the matrix's own [slots + 1][n] layout.  The
+// single-reduction CG forms w = A M r from it, so an iteration gathers three
+// vectors per neighbour instead of four.  Same rows as k_inv_diag; the
+// gathers of inv are those the iteration kernels made before.
+template <class M>
+__global__ void k_scale_cols(M A, Items items, int n, const real* __restrict__ inv,
+                             real* __restrict__ As) {
+  static_assert(M::kEll, "ELL matrices only");
+  const int i = item_index(items);
+  if (i >= n) return;
+  real a[M::kRow];
+  uint32_t o[M::kRow];
+  A.load_off(i, a, o);
+  const FBuf I(inv, (uint32_t)n * 4u);
+#pragma unroll
+  for (int j = 0; j < M::kRow; ++j) As[(size_t)j * n + i] = a[j] * I(o[j]);
+}
+
 template <class M, int NV>
 __global__ void k_spmv(M A, int n, const real* __restrict__ x, real* __restrict__ y) {
   const int i = xcd_item();
@@ -449,6 +468,8 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int i
   // one row per thread on an ELL matrix: issue the row's loads before the
   // fold, and the fold's own loads (and the case's scalars) before those
   constexpr bool pf = PF && M::kEll;
+  // an ELL matrix arrives Jacobi-scaled (k_scale_cols): w = (A M) r
+  constexpr bool CS = M::kEll;
   constexpr int ND = CHECK ? 3 : 2;   // r.u, w.u (and r.r for the stop test)
   const int pv = (it - 1) & 1;
   const double* pp[ND];
@@ -476,7 +497,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int i
       for (int j = 0; j < M::kRow; ++j) {
         wc[j] = W(o[j]);
         sc[j] = FIRST ? 0.f : Sv(o[j]);
-        ic[j] = I(o[j]);
+        ic[j] = (CS && j > 0) ? 0.f : I(o[j]);
         rc[j] = R(o[j]);
       }
       p0 = FIRST ? 0.f : ld_off((const real*)p, o0);
@@ -510,7 +531,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int i
 #pragma unroll
       for (int j = 0; j < M::kRow; ++j) {
         const real scj = FIRST ? wc[j] : wc[j] + beta * sc[j];
-        wn += a[j] * (ic[j] * (rc[j] - alpha * scj));
+        wn += CS ? a[j] * (rc[j] - alpha * scj) : a[j] * (ic[j] * (rc[j] - alpha * scj));
       }
       const real ii = ic[0];
       const real ui = rc[0] * ii;
@@ -532,7 +553,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int i
       real wn = 0.f;
       A.row(i, [&](real a, int c) {
         const real sc = FIRST ? w_in[c] : w_in[c] + beta * s_in[c];
-        wn += a * (inv[c] * (r_in[c] - alpha * sc));
+        wn += CS ? a * (r_in[c] - alpha * sc) : a * (inv[c] * (r_in[c] - alpha * sc));
       });
       const real ii = inv[i];
       const real ui = r_in[i] * ii;
@@ -553,7 +574,8 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int i
   store_partials<3>(acc, P_out, G, br.vb);
 }
 
-// Two rows per thread (chunk <= 2 B): rows tid and tid + B of the chunk,
+// Two rows per thread (chunk <= 2 B), on the Jacobi-scaled ELL matrix A M
+// (k_scale_cols): rows tid and tid + B of the chunk,
 // the grid-stride order, with all their loads issued before the fold; the
 // same sums as the grid-stride form.  Kept apart from k_cgs_iter so that
 // the one-row form keeps its register count (and occupancy).  The fold's
@@ -610,7 +632,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter2(M A, Seg sg, int 
       for (int j = 0; j < M::kRow; ++j) {
         wc[q][j] = W(o[j]);
         sc[q][j] = FIRST ? 0.f : Sv(o[j]);
-        ic[q][j] = I(o[j]);
+        ic[q][j] = j > 0 ? 0.f : I(o[j]);   // A arrives Jacobi-scaled: only inv[i]
         rc[q][j] = Rv(o[j]);
       }
       p0[q] = FIRST ? 0.f : ld_off((const real*)p, o0[q]);
@@ -642,7 +664,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter2(M A, Seg sg, int 
 #pragma unroll
     for (int j = 0; j < M::kRow; ++j) {
       const real scj = FIRST ? wc[q][j] : wc[q][j] + beta * sc[q][j];
-      wn += a[q][j] * (ic[q][j] * (rc[q][j] - alpha * scj));
+      wn += a[q][j] * (rc[q][j] - alpha * scj);
     }
     const real ii = ic[q][0];
     const real ui = rc[q][0] * ii;
@@ -986,6 +1008,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_3(Seg sg, int it, real* 
 // ------------------------------------------------------------- host side
 struct KWork {
   real *r, *r0, *p0, *p1, *v0, *v1, *s, *t, *inv;
+  real* As;   // [7][n]: the Jacobi-scaled ELL matrix of a single-reduction CG solve
   double *PA, *PB, *PC;
   KState* st;
 };
@@ -1009,6 +1032,7 @@ static KWork carve_krylov(void* ws, int32_t n, int nvec, int n_cases, int max_bl
   w.s = c.take<real>(nv);
   w.t = c.take<real>(nv);
   w.inv = c.take<real>(n);
+  w.As = c.take<real>(7 * (size_t)n);
   return w;
 }
 
@@ -1021,6 +1045,7 @@ size_t krylov_workspace_bytes_batch(int32_t n, int nvec, int n_cases, int max_bl
   const size_t nv = (size_t)n * nvec;
   for (int i = 0; i < 8; ++i) c.take<real>(nv);
   c.take<real>(n);
+  c.take<real>(7 * (size_t)n);
   return c.off;
 }
 
@@ -1327,8 +1352,13 @@ static int cgs_impl(const M& A, const KLaunch& L, const real* b, real* x, const 
   real* wb[2] = {w.v0, w.v1};
   real* sb[2] = {w.s, w.t};
   double* pb[2] = {w.PA, w.PC};   // iteration i reads pb[i&1], writes pb[(i+1)&1]
-  if (const unsigned gi = items_grid(L.rows, n))
+  // the iteration kernels read an ELL matrix Jacobi-scaled (A M)
+  M As = A;
+  if (const unsigned gi = items_grid(L.rows, n)) {
     WCFD_LAUNCH(K_INV_DIAG, s, k_inv_diag<M>, gi, A, L.rows, n, w.inv);
+    if constexpr (M::kEll) WCFD_LAUNCH(K_INV_DIAG, s, k_scale_cols<M>, gi, A, L.rows, n, w.inv, w.As);
+  }
+  if constexpr (M::kEll) As.val = w.As;
   if (r64)
     WCFD_LAUNCH_B(K_CG_INIT, s, (k_cgs_init1<M, true>), G, B, A, sg, b, x, rb[0], w.PC);
   else
@@ -1343,7 +1373,7 @@ static int cgs_impl(const M& A, const KLaunch& L, const real* b, real* x, const 
     prof_set_iter(it);
     const int cur = it & 1;
 #define WCFD_CGS_ITER(KERNEL)                                                             \
-  WCFD_LAUNCH_B(K_CG_1, s, KERNEL, G, B, A, sg, it, x, rb[cur], rb[1 - cur], wb[cur],       \
+  WCFD_LAUNCH_B(K_CG_1, s, KERNEL, G, B, As, sg, it, x, rb[cur], rb[1 - cur], wb[cur],      \
                 wb[1 - cur], sb[cur], sb[1 - cur], w.p0, w.inv, pb[cur], pb[1 - cur], w.st,  \
                 maxiter, out)
     const bool first = it == 0;
