@@ -1,15 +1,24 @@
-"""HBM traffic of a kernel from rocprofv3 counter runs.
+"""Per-dispatch figures of a kernel from rocprofv3 runs.
 
-Two profiled runs of the same program, one with ``--pmc FETCH_SIZE`` and one
-with ``--pmc WRITE_SIZE`` (rocprofv3 collects counters per dispatch, in KiB;
-``TCC`` channels summed), give the bytes each dispatch moved between the L2
-and memory.  The program also runs a plain device copy of known size (the
-calibration copy: ``bench.py --calibration-copy-mb``); the ratio of its
-known bytes to its counter reading scales every other reading.
+HBM traffic: two profiled runs of the same program, one with ``--pmc
+FETCH_SIZE`` and one with ``--pmc WRITE_SIZE`` (rocprofv3 collects counters
+per dispatch, in KiB; ``TCC`` channels summed), give the bytes each dispatch
+moved between the L2 and memory.  The program also runs a plain device copy
+of known size (the calibration copy: ``bench.py --calibration-copy-mb``);
+the ratio of its known bytes to its counter reading scales every other
+reading.
+
+Durations: a ``--kernel-trace`` run gives each dispatch's begin and end.
+
+A Krylov solve keeps launching until the host has seen its stop, and those
+launches exit at entry; both summaries count them apart (a dispatch far
+below the kernel's typical bytes or duration), as the library profiler does.
 
     rocprofv3 --pmc FETCH_SIZE --output-format csv -d DIR/fetch -o run -- python3 bench.py ...
     rocprofv3 --pmc WRITE_SIZE --output-format csv -d DIR/write -o run -- python3 bench.py ...
-    python -m warpcfd_amd.pmc DIR k_cgs_iter 512
+    python -m warpcfd_amd.pmc traffic DIR k_cgs_iter 512
+    rocprofv3 --kernel-trace --stats --output-format csv -d TDIR -o run -- python3 bench.py ...
+    python -m warpcfd_amd.pmc trace TDIR k_cgs_iter
 """
 import csv
 import glob
@@ -47,13 +56,24 @@ def per_dispatch(run_dir, counter):
     return acc
 
 
+# a dispatch reading or writing less than this share of the kernel's largest
+# dispatch exited at entry
+IDLE_BYTES_FRAC = 0.05
+# a dispatch shorter than this share of the kernel's 90th-percentile duration
+# exited at entry (about 1.6 us against 7 us and more for an iteration)
+IDLE_TIME_FRAC = 0.4
+
+
 def kernel_mean_bytes(run_dir, counter, kernel_substr):
-    """Mean bytes per dispatch of the kernels whose name contains
-    ``kernel_substr``, and the number of dispatches averaged."""
+    """Mean bytes per working dispatch of the kernels whose name contains
+    ``kernel_substr``, the number of dispatches averaged and the number left
+    out as exits at entry."""
     vals = [v for k, v in per_dispatch(run_dir, counter).values() if kernel_substr in k]
     if not vals:
-        return None, 0
-    return 1024.0 * sum(vals) / len(vals), len(vals)
+        return None, 0, 0
+    cut = IDLE_BYTES_FRAC * max(vals)
+    work = [v for v in vals if v >= cut]
+    return 1024.0 * sum(work) / len(work), len(work), len(vals) - len(work)
 
 
 def calibration_factor(run_dir, counter, copy_bytes):
@@ -78,9 +98,9 @@ def traffic(pmc_dir, kernel_substr, copy_mb):
     total = 0.0
     for sub, counter in (('fetch', 'FETCH_SIZE'), ('write', 'WRITE_SIZE')):
         d = os.path.join(pmc_dir, sub)
-        raw, n = kernel_mean_bytes(d, counter, kernel_substr)
+        raw, n, idle = kernel_mean_bytes(d, counter, kernel_substr)
         f = calibration_factor(d, counter, copy_bytes)
-        out[counter] = {'raw_bytes': raw, 'dispatches': n, 'scale': f,
+        out[counter] = {'raw_bytes': raw, 'dispatches': n, 'exit_at_entry': idle, 'scale': f,
                         'bytes': raw * f if (raw is not None and f) else None}
         if out[counter]['bytes'] is None:
             total = None
@@ -90,6 +110,43 @@ def traffic(pmc_dir, kernel_substr, copy_mb):
     return out
 
 
+def trace_summary(trace_dir, kernel_substr):
+    """Durations of the dispatches whose kernel name contains
+    ``kernel_substr`` in a ``--kernel-trace`` run: all of them (what
+    ``--stats`` averages) and the working ones (exits at entry left out),
+    per kernel instance and together."""
+    paths = glob.glob(os.path.join(trace_dir, '**', '*kernel_trace.csv'), recursive=True)
+    if not paths:
+        raise FileNotFoundError(f'no *kernel_trace.csv under {trace_dir}')
+    per = {}
+    for p in paths:
+        with open(p) as f:
+            for r in csv.DictReader(f):
+                k = r.get('Kernel_Name', '')
+                if kernel_substr in k:
+                    t = (int(r['End_Timestamp']) - int(r['Start_Timestamp'])) * 1e-3
+                    per.setdefault(k, []).append(t)
+    if not per:
+        return None
+    every = sorted(t for ts in per.values() for t in ts)
+    cut = IDLE_TIME_FRAC * every[min(len(every) - 1, int(0.9 * len(every)))]
+
+    def stats(ts):
+        work = [t for t in ts if t >= cut]
+        return {'dispatches': len(ts), 'avg_us': round(sum(ts) / len(ts), 3),
+                'working': len(work),
+                'working_avg_us': round(sum(work) / len(work), 3) if work else None,
+                'exit_at_entry': len(ts) - len(work)}
+    out = {'kernel': kernel_substr, 'exit_at_entry_below_us': round(cut, 3),
+           'instances': {k: stats(ts) for k, ts in sorted(per.items())}}
+    out.update(stats(every))
+    return out
+
+
 if __name__ == '__main__':
     import json
-    print(json.dumps(traffic(sys.argv[1], sys.argv[2], float(sys.argv[3])), indent=1))
+    if sys.argv[1] == 'trace':
+        print(json.dumps(trace_summary(sys.argv[2], sys.argv[3]), indent=1))
+    else:
+        a = sys.argv[2:] if sys.argv[1] == 'traffic' else sys.argv[1:]
+        print(json.dumps(traffic(a[0], a[1], float(a[2])), indent=1))
