@@ -233,6 +233,11 @@ struct FBuf {
   }
 };
 
+// a * b + c rounded once, in the field type: kernels that must round alike
+// spell their multiply-adds out instead of leaving them to contraction
+__device__ __forceinline__ float mad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double mad(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 // ------------------------------------------------------ XCD-aware mapping
 // MI355X deals workgroups to its 8 XCDs round-robin (workgroup b runs on XCD
 // b % 8) and each XCD has its own L2.  xcd_remap() renumbers workgroups so

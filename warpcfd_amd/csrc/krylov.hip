@@ -409,6 +409,34 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_2(Seg sg, int it, real* 
 }
 
 // ------------------------------------------- CG with one reduction per iteration
+// One row of the single-reduction iteration on the Jacobi-scaled matrix
+// (a = row of A M; wc, sc, rc = w, s, r at the row's columns, the row
+// itself first; ii = inv[i]): the next w, p, s and r of the row.  Every
+// multiply-add is explicit, so every kernel form rounds the same way.
+template <int KR, bool FIRST>
+__device__ __forceinline__ void cg_row(const real (&a)[KR], const real (&wc)[KR],
+                                       const real (&sc)[KR], const real (&rc)[KR], real ii,
+                                       real p0, real alpha, real beta, real& wn, real& pi,
+                                       real& si, real& rn) {
+  wn = 0.f;
+#pragma unroll
+  for (int j = 0; j < KR; ++j) {
+    const real scj = FIRST ? wc[j] : mad(beta, sc[j], wc[j]);
+    wn = mad(a[j], mad(-alpha, scj, rc[j]), wn);
+  }
+  const real ui = rc[0] * ii;
+  pi = FIRST ? ui : mad(beta, p0, ui);
+  si = FIRST ? wc[0] : mad(beta, sc[0], wc[0]);
+  rn = mad(-alpha, si, rc[0]);
+}
+
+// the row's shares of r.u, w.u and r.r (u = M r), accumulated in float64
+__device__ __forceinline__ void cg_dots(double (&acc)[3], real rn, real wn, real ii) {
+  const real un = rn * ii;
+  acc[0] = __builtin_fma((double)rn, (double)un, acc[0]);
+  acc[1] = __builtin_fma((double)wn, (double)un, acc[1]);
+  acc[2] = __builtin_fma((double)rn, (double)rn, acc[2]);
+}
 // Chronopoulos-Gear form of the same Jacobi-preconditioned CG: the two inner
 // products of an iteration (gamma = r.u, delta = w.u, u = M r, w = A u) are
 // taken together, so one fused kernel per iteration updates p, s (= A p), x
@@ -439,12 +467,9 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_init2(M A, Seg sg, cons
   double acc[3] = {0.0, 0.0, 0.0};
   for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
     real wi = 0.f;
-    A.row(i, [&](real a, int c) { wi += a * (r[c] * inv[c]); });
+    A.row(i, [&](real a, int c) { wi = mad(a, r[c] * inv[c], wi); });
     w[i] = wi;
-    const real ui = r[i] * inv[i];
-    acc[0] += (double)r[i] * (double)ui;
-    acc[1] += (double)wi * (double)ui;
-    acc[2] += (double)r[i] * (double)r[i];
+    cg_dots(acc, r[i], wi, inv[i]);
   }
   store_partials<3>(acc, PA, sg.G, br.vb);
 }
@@ -526,49 +551,36 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int i
   const real alpha = (real)alpha_d, beta = (real)beta_d;
   double acc[3] = {0.0, 0.0, 0.0};
   if (pf) {
-    if (has) {
-      real wn = 0.f;
-#pragma unroll
-      for (int j = 0; j < M::kRow; ++j) {
-        const real scj = FIRST ? wc[j] : wc[j] + beta * sc[j];
-        wn += CS ? a[j] * (rc[j] - alpha * scj) : a[j] * (ic[j] * (rc[j] - alpha * scj));
-      }
+    if (has) {   // pf: an ELL matrix, so Jacobi-scaled
+      real wn, pi, si, rn;
       const real ii = ic[0];
-      const real ui = rc[0] * ii;
-      const real pi = FIRST ? ui : ui + beta * p0;
-      const real si = FIRST ? wc[0] : wc[0] + beta * sc[0];
-      st_off(x, o0, x0 + alpha * pi);
-      const real rn = rc[0] - alpha * si;
+      cg_row<M::kRow, FIRST>(a, wc, sc, rc, ii, p0, alpha, beta, wn, pi, si, rn);
+      st_off(x, o0, mad(alpha, pi, x0));
       st_off(p, o0, pi);
       st_off(s_out, o0, si);
       st_off(r_out, o0, rn);
       st_off(w_out, o0, wn);
-      const real un = rn * ii;
-      acc[0] += (double)rn * (double)un;
-      acc[1] += (double)wn * (double)un;
-      acc[2] += (double)rn * (double)rn;
+      cg_dots(acc, rn, wn, ii);
     }
   } else {
     for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) {
       real wn = 0.f;
       A.row(i, [&](real a, int c) {
-        const real sc = FIRST ? w_in[c] : w_in[c] + beta * s_in[c];
-        wn += CS ? a * (r_in[c] - alpha * sc) : a * (inv[c] * (r_in[c] - alpha * sc));
+        const real sc = FIRST ? w_in[c] : mad(beta, s_in[c], w_in[c]);
+        wn = CS ? mad(a, mad(-alpha, sc, r_in[c]), wn)
+                : mad(a, inv[c] * mad(-alpha, sc, r_in[c]), wn);
       });
       const real ii = inv[i];
       const real ui = r_in[i] * ii;
-      const real pi = FIRST ? ui : ui + beta * p[i];
-      const real si = FIRST ? w_in[i] : w_in[i] + beta * s_in[i];
-      x[i] = x[i] + alpha * pi;
-      const real rn = r_in[i] - alpha * si;
+      const real pi = FIRST ? ui : mad(beta, p[i], ui);
+      const real si = FIRST ? w_in[i] : mad(beta, s_in[i], w_in[i]);
+      x[i] = mad(alpha, pi, x[i]);
+      const real rn = mad(-alpha, si, r_in[i]);
       p[i] = pi;
       s_out[i] = si;
       r_out[i] = rn;
       w_out[i] = wn;
-      const real un = rn * ii;
-      acc[0] += (double)rn * (double)un;
-      acc[1] += (double)wn * (double)un;
-      acc[2] += (double)rn * (double)rn;
+      cg_dots(acc, rn, wn, ii);
     }
   }
   store_partials<3>(acc, P_out, G, br.vb);
@@ -660,26 +672,15 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter2(M A, Seg sg, int 
 #pragma unroll
   for (int q = 0; q < NR; ++q) {
     if (!has[q]) continue;
-    real wn = 0.f;
-#pragma unroll
-    for (int j = 0; j < M::kRow; ++j) {
-      const real scj = FIRST ? wc[q][j] : wc[q][j] + beta * sc[q][j];
-      wn += a[q][j] * (rc[q][j] - alpha * scj);
-    }
+    real wn, pi, si, rn;
     const real ii = ic[q][0];
-    const real ui = rc[q][0] * ii;
-    const real pi = FIRST ? ui : ui + beta * p0[q];
-    const real si = FIRST ? wc[q][0] : wc[q][0] + beta * sc[q][0];
-    st_off(x, o0[q], x0[q] + alpha * pi);
-    const real rn = rc[q][0] - alpha * si;
+    cg_row<M::kRow, FIRST>(a[q], wc[q], sc[q], rc[q], ii, p0[q], alpha, beta, wn, pi, si, rn);
+    st_off(x, o0[q], mad(alpha, pi, x0[q]));
     st_off(p, o0[q], pi);
     st_off(s_out, o0[q], si);
     st_off(r_out, o0[q], rn);
     st_off(w_out, o0[q], wn);
-    const real un = rn * ii;
-    acc[0] += (double)rn * (double)un;
-    acc[1] += (double)wn * (double)un;
-    acc[2] += (double)rn * (double)rn;
+    cg_dots(acc, rn, wn, ii);
   }
   store_partials<3>(acc, P_out, G, br.vb);
 }
