@@ -570,10 +570,12 @@ def run_serve(args, n_requests, mix=SERVE_MIX, sequential=True, error_sample=0, 
 
 
 def bench_serve(args):
-    """The served stream of the configured mix (cavities N in [32,512],
-    channels 50k-500k cells), ``--requests`` Poisson arrivals.  One step =
-    the whole stream."""
-    res = run_serve(args, args.requests, SERVE_MIX, sequential=not args.no_sequential,
+    """A served stream of ``--requests`` Poisson arrivals: the configured mix
+    (cavities N in [32,512], channels 50k-500k cells) or, with ``--serve-mix
+    sample``, the reduced mix of the default line's served sample (cavities
+    N in [32,96], channels 50k-150k cells).  One step = the whole stream."""
+    mix = SERVE_SAMPLE_MIX if args.serve_mix == 'sample' else SERVE_MIX
+    res = run_serve(args, args.requests, mix, sequential=not args.no_sequential,
                     error_sample=args.error_sample)
     served = res['served']
     out = {
@@ -904,6 +906,11 @@ def main():
                     help='batch: replay a HIP-graph-captured iteration (Krylov launch budget '
                          'measured every 100 iterations)')
     ap.add_argument('--requests', type=int, default=200, help='serve: requests in the stream')
+    ap.add_argument('--serve-sample-requests', type=int, default=200,
+                    help='default line: requests in the served sample')
+    ap.add_argument('--serve-mix', default='configured', choices=['configured', 'sample'],
+                    help='serve: the configured mix (cavities up to 512^2, channels up to 500k '
+                         'cells) or the reduced mix of the default line\'s served sample')
     ap.add_argument('--error-sample', type=int, default=6,
                     help='serve: served results checked against long runs')
     ap.add_argument('--rate', type=float, default=20.0, help='serve: arrivals per second')
@@ -1055,7 +1062,8 @@ def main():
         fp64 = fp64_error(fields32, run64)
         fp64['note'] = (f'the same workload through the double-field build (libwcfd64.so), same '
                         f'Krylov settings and launch shape, after the same '
-                        f'{args.warmup + args.steps} SIMPLE iterations from rest')
+                        f'{args.warmup + args.steps} SIMPLE iterations from rest; each build '
+                        f'stops its own Krylov solves at the absolute 1e-5')
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -1106,12 +1114,12 @@ def channel_extras(args):
     """The other half of the metric, measured in the default run beside the
     channel line: the 64-cavity batch solved batched (the configured batch,
     timed as ``--workload batch64`` times it, without its one-after-another
-    leg), and a served sample -- 24 Poisson requests of a reduced mix
-    (cavities N in [32,96], channels 50k-150k cells) to the batch stop,
-    continuous batching against one at a time on the same sample, with a
-    velocity-error check of 4 results against long runs.  The configured
-    served stream (N up to 512, 200 requests) runs for minutes and is
-    measured by ``--workload serve``."""
+    leg), and a served sample -- ``--serve-sample-requests`` (200) Poisson
+    requests of a reduced mix (cavities N in [32,96], channels 50k-150k
+    cells) to the batch stop, continuous batching against one at a time on
+    the same stream, with a velocity-error check of 12 results against long
+    runs.  The configured mix (N up to 512) runs for minutes and is measured
+    by ``--workload serve``."""
     out = {}
     t = time.perf_counter()
     try:
@@ -1126,8 +1134,8 @@ def channel_extras(args):
     print(f'# batch64 sub-benchmark: {time.perf_counter() - t:.1f} s', file=sys.stderr, flush=True)
     t = time.perf_counter()
     try:
-        out['served_sample'] = run_serve(args, 24, SERVE_SAMPLE_MIX, sequential=True,
-                                         error_sample=4, label=' sample')
+        out['served_sample'] = run_serve(args, args.serve_sample_requests, SERVE_SAMPLE_MIX,
+                                         sequential=True, error_sample=12, label=' sample')
     except Exception as exc:
         out['served_sample'] = {'error': repr(exc)}
     print(f'# served sample: {time.perf_counter() - t:.1f} s', file=sys.stderr, flush=True)

@@ -65,25 +65,6 @@ __global__ void k_inv_diag(M A, Items items, int n, real* __restrict__ inv) {
   inv[i] = d != 0.0f ? 1.0f / d : 1.0f;
 }
 
-// The Jacobi-scaled matrix A M of an ELL matrix (M = diag(inv)): column j
-// of every row times inv[j], in the matrix's own [slots + 1][n] layout.  The
-// single-reduction CG forms w = A M r from it, so an iteration gathers three
-// vectors per neighbour instead of four.  Same rows as k_inv_diag; the
-// gathers of inv are those the iteration kernels made before.
-template <class M>
-__global__ void k_scale_cols(M A, Items items, int n, const real* __restrict__ inv,
-                             real* __restrict__ As) {
-  static_assert(M::kEll, "ELL matrices only");
-  const int i = item_index(items);
-  if (i >= n) return;
-  real a[M::kRow];
-  uint32_t o[M::kRow];
-  A.load_off(i, a, o);
-  const FBuf I(inv, (uint32_t)n * 4u);
-#pragma unroll
-  for (int j = 0; j < M::kRow; ++j) As[(size_t)j * n + i] = a[j] * I(o[j]);
-}
-
 template <class M, int NV>
 __global__ void k_spmv(M A, int n, const real* __restrict__ x, real* __restrict__ y) {
   const int i = xcd_item();
@@ -409,21 +390,25 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cg_2(Seg sg, int it, real* 
 }
 
 // ------------------------------------------- CG with one reduction per iteration
-// One row of the single-reduction iteration on the Jacobi-scaled matrix
-// (a = row of A M; wc, sc, rc = w, s, r at the row's columns, the row
-// itself first; ii = inv[i]): the next w, p, s and r of the row.  Every
-// multiply-add is explicit, so every kernel form rounds the same way.
+// One row of the single-reduction iteration (a = the row of A; wc, sc, rc,
+// ic = w, s, r, inv at the row's columns, the row itself first): the next
+// w = A M r, p, s and r of the row.  M is applied to the vector, as the
+// reference's Jacobi preconditioner does; a matrix scaled once by M would
+// round A M once per solve, a fixed perturbation of the operator that
+// breaks its zero row sums (measured: 10x further from the float64 build).
+// Every multiply-add is explicit, so every kernel form rounds the same way.
 template <int KR, bool FIRST>
 __device__ __forceinline__ void cg_row(const real (&a)[KR], const real (&wc)[KR],
-                                       const real (&sc)[KR], const real (&rc)[KR], real ii,
-                                       real p0, real alpha, real beta, real& wn, real& pi,
-                                       real& si, real& rn) {
+                                       const real (&sc)[KR], const real (&rc)[KR],
+                                       const real (&ic)[KR], real p0, real alpha, real beta,
+                                       real& wn, real& pi, real& si, real& rn) {
   wn = 0.f;
 #pragma unroll
   for (int j = 0; j < KR; ++j) {
     const real scj = FIRST ? wc[j] : mad(beta, sc[j], wc[j]);
-    wn = mad(a[j], mad(-alpha, scj, rc[j]), wn);
+    wn = mad(a[j], ic[j] * mad(-alpha, scj, rc[j]), wn);
   }
+  const real ii = ic[0];
   const real ui = rc[0] * ii;
   pi = FIRST ? ui : mad(beta, p0, ui);
   si = FIRST ? wc[0] : mad(beta, sc[0], wc[0]);
@@ -493,8 +478,6 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int i
   // one row per thread on an ELL matrix: issue the row's loads before the
   // fold, and the fold's own loads (and the case's scalars) before those
   constexpr bool pf = PF && M::kEll;
-  // an ELL matrix arrives Jacobi-scaled (k_scale_cols): w = (A M) r
-  constexpr bool CS = M::kEll;
   constexpr int ND = CHECK ? 3 : 2;   // r.u, w.u (and r.r for the stop test)
   const int pv = (it - 1) & 1;
   const double* pp[ND];
@@ -522,7 +505,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int i
       for (int j = 0; j < M::kRow; ++j) {
         wc[j] = W(o[j]);
         sc[j] = FIRST ? 0.f : Sv(o[j]);
-        ic[j] = (CS && j > 0) ? 0.f : I(o[j]);
+        ic[j] = I(o[j]);
         rc[j] = R(o[j]);
       }
       p0 = FIRST ? 0.f : ld_off((const real*)p, o0);
@@ -551,10 +534,10 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int i
   const real alpha = (real)alpha_d, beta = (real)beta_d;
   double acc[3] = {0.0, 0.0, 0.0};
   if (pf) {
-    if (has) {   // pf: an ELL matrix, so Jacobi-scaled
+    if (has) {
       real wn, pi, si, rn;
       const real ii = ic[0];
-      cg_row<M::kRow, FIRST>(a, wc, sc, rc, ii, p0, alpha, beta, wn, pi, si, rn);
+      cg_row<M::kRow, FIRST>(a, wc, sc, rc, ic, p0, alpha, beta, wn, pi, si, rn);
       st_off(x, o0, mad(alpha, pi, x0));
       st_off(p, o0, pi);
       st_off(s_out, o0, si);
@@ -567,8 +550,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int i
       real wn = 0.f;
       A.row(i, [&](real a, int c) {
         const real sc = FIRST ? w_in[c] : mad(beta, s_in[c], w_in[c]);
-        wn = CS ? mad(a, mad(-alpha, sc, r_in[c]), wn)
-                : mad(a, inv[c] * mad(-alpha, sc, r_in[c]), wn);
+        wn = mad(a, inv[c] * mad(-alpha, sc, r_in[c]), wn);
       });
       const real ii = inv[i];
       const real ui = r_in[i] * ii;
@@ -586,8 +568,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter(M A, Seg sg, int i
   store_partials<3>(acc, P_out, G, br.vb);
 }
 
-// Two rows per thread (chunk <= 2 B), on the Jacobi-scaled ELL matrix A M
-// (k_scale_cols): rows tid and tid + B of the chunk,
+// Two rows per thread (chunk <= 2 B): rows tid and tid + B of the chunk,
 // the grid-stride order, with all their loads issued before the fold; the
 // same sums as the grid-stride form.  Kept apart from k_cgs_iter so that
 // the one-row form keeps its register count (and occupancy).  The fold's
@@ -644,7 +625,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter2(M A, Seg sg, int 
       for (int j = 0; j < M::kRow; ++j) {
         wc[q][j] = W(o[j]);
         sc[q][j] = FIRST ? 0.f : Sv(o[j]);
-        ic[q][j] = j > 0 ? 0.f : I(o[j]);   // A arrives Jacobi-scaled: only inv[i]
+        ic[q][j] = I(o[j]);
         rc[q][j] = Rv(o[j]);
       }
       p0[q] = FIRST ? 0.f : ld_off((const real*)p, o0[q]);
@@ -674,7 +655,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter2(M A, Seg sg, int 
     if (!has[q]) continue;
     real wn, pi, si, rn;
     const real ii = ic[q][0];
-    cg_row<M::kRow, FIRST>(a[q], wc[q], sc[q], rc[q], ii, p0[q], alpha, beta, wn, pi, si, rn);
+    cg_row<M::kRow, FIRST>(a[q], wc[q], sc[q], rc[q], ic[q], p0[q], alpha, beta, wn, pi, si, rn);
     st_off(x, o0[q], mad(alpha, pi, x0[q]));
     st_off(p, o0[q], pi);
     st_off(s_out, o0[q], si);
@@ -1009,7 +990,6 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_bi_3(Seg sg, int it, real* 
 // ------------------------------------------------------------- host side
 struct KWork {
   real *r, *r0, *p0, *p1, *v0, *v1, *s, *t, *inv;
-  real* As;   // [7][n]: the Jacobi-scaled ELL matrix of a single-reduction CG solve
   double *PA, *PB, *PC;
   KState* st;
 };
@@ -1033,7 +1013,6 @@ static KWork carve_krylov(void* ws, int32_t n, int nvec, int n_cases, int max_bl
   w.s = c.take<real>(nv);
   w.t = c.take<real>(nv);
   w.inv = c.take<real>(n);
-  w.As = c.take<real>(7 * (size_t)n);
   return w;
 }
 
@@ -1046,7 +1025,6 @@ size_t krylov_workspace_bytes_batch(int32_t n, int nvec, int n_cases, int max_bl
   const size_t nv = (size_t)n * nvec;
   for (int i = 0; i < 8; ++i) c.take<real>(nv);
   c.take<real>(n);
-  c.take<real>(7 * (size_t)n);
   return c.off;
 }
 
@@ -1353,13 +1331,8 @@ static int cgs_impl(const M& A, const KLaunch& L, const real* b, real* x, const 
   real* wb[2] = {w.v0, w.v1};
   real* sb[2] = {w.s, w.t};
   double* pb[2] = {w.PA, w.PC};   // iteration i reads pb[i&1], writes pb[(i+1)&1]
-  // the iteration kernels read an ELL matrix Jacobi-scaled (A M)
-  M As = A;
-  if (const unsigned gi = items_grid(L.rows, n)) {
+  if (const unsigned gi = items_grid(L.rows, n))
     WCFD_LAUNCH(K_INV_DIAG, s, k_inv_diag<M>, gi, A, L.rows, n, w.inv);
-    if constexpr (M::kEll) WCFD_LAUNCH(K_INV_DIAG, s, k_scale_cols<M>, gi, A, L.rows, n, w.inv, w.As);
-  }
-  if constexpr (M::kEll) As.val = w.As;
   if (r64)
     WCFD_LAUNCH_B(K_CG_INIT, s, (k_cgs_init1<M, true>), G, B, A, sg, b, x, rb[0], w.PC);
   else
@@ -1374,7 +1347,7 @@ static int cgs_impl(const M& A, const KLaunch& L, const real* b, real* x, const 
     prof_set_iter(it);
     const int cur = it & 1;
 #define WCFD_CGS_ITER(KERNEL)                                                             \
-  WCFD_LAUNCH_B(K_CG_1, s, KERNEL, G, B, As, sg, it, x, rb[cur], rb[1 - cur], wb[cur],      \
+  WCFD_LAUNCH_B(K_CG_1, s, KERNEL, G, B, A, sg, it, x, rb[cur], rb[1 - cur], wb[cur],       \
                 wb[1 - cur], sb[cur], sb[1 - cur], w.p0, w.inv, pb[cur], pb[1 - cur], w.st,  \
                 maxiter, out)
     const bool first = it == 0;
