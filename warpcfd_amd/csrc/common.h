@@ -39,6 +39,22 @@ constexpr int kMaxInlineFold = 2048;    // default: largest per-case fold done i
 #ifndef WCFD_NT_MATRIX
 #define WCFD_NT_MATRIX 0
 #endif
+// The cell kernels that gather face data through their slots (assembly,
+// the pressure correction) read and write their cell-ordered streams (slot
+// tables, the cell's own values, the outputs) with the nontemporal hint, so
+// that these do not push out of the L2 the face data that a face's
+// neighbour side reads again one band of cells later.  Measured at 8.64 M
+// cells: FETCH_SIZE of the momentum assembly 2281 -> 1782 MB, 2 % faster.
+#ifndef WCFD_NT_CELL
+#define WCFD_NT_CELL 1
+#endif
+// the same for the face-ordered streams of the internal faces in the face
+// kernels (indices, face geometry, the face outputs), which keeps the
+// gathered cell data in the L2: face interpolation at 8.64 M cells 3.6 %
+// faster, FETCH_SIZE 1640 -> 1488 MB
+#ifndef WCFD_NT_FACE
+#define WCFD_NT_FACE 1
+#endif
 // Largest workgroup of the row-parallel Krylov / reduction kernels: their
 // __launch_bounds__, which lets a wave hold up to 256 VGPRs (a whole row of
 // gathers in flight) without spilling.
@@ -199,6 +215,35 @@ __device__ __forceinline__ void st_off(T* base, uint32_t byte_off, T v) {
 template <class T>
 __device__ __forceinline__ void st_plain(T* base, uint32_t byte_off, T v) {
   *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + scaled<T>(byte_off)) = v;
+}
+// a cell-ordered array read or written once in a launch, with the
+// nontemporal hint when NT
+template <bool NT, class T>
+__device__ __forceinline__ T ld_cell(const T* base, uint32_t byte_off) {
+  if constexpr (NT)
+    return __builtin_nontemporal_load(
+        reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + scaled<T>(byte_off)));
+  else
+    return ld_off(base, byte_off);
+}
+template <bool NT, class T>
+__device__ __forceinline__ void st_cell(T* base, uint32_t byte_off, T v) {
+  if constexpr (NT)
+    __builtin_nontemporal_store(
+        v, reinterpret_cast<T*>(reinterpret_cast<char*>(base) + scaled<T>(byte_off)));
+  else
+    st_plain(base, byte_off, v);
+}
+// the same at a plain pointer
+template <bool NT, class T>
+__device__ __forceinline__ T ld_once(const T* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <bool NT, class T>
+__device__ __forceinline__ void st_once(T* p, T v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
 }
 // matrix stream (read once per iteration)
 template <class T>

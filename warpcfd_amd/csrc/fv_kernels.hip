@@ -55,10 +55,12 @@ __device__ inline void face_interp_internal(const wcfd_mesh& m, int f,
                                             int mask, int skew_mask, real* __restrict__ flux,
                                             const real* __restrict__ rua,
                                             real* __restrict__ rua_f) {
+  constexpr bool NT = WCFD_NT_FACE;
   const int C = m.n_cells, F = m.n_faces, Fi = m.n_internal;
-  const int o = m.owner[f], n = m.neigh[f];
-  const real psi = m.psi[f];
-  const real s0 = m.skew[f], s1 = m.skew[Fi + f], s2 = m.skew[2 * Fi + f];
+  const int o = ld_once<NT>(m.owner + f), n = ld_once<NT>(m.neigh + f);
+  const real psi = ld_once<NT>(m.psi + f);
+  const real s0 = ld_once<NT>(m.skew + f), s1 = ld_once<NT>(m.skew + Fi + f),
+             s2 = ld_once<NT>(m.skew + 2 * Fi + f);
   // loads of every field
   real co[4], cn[4], go[4][3], gn[4][3], old[3];
 #pragma unroll
@@ -75,15 +77,15 @@ __device__ inline void face_interp_internal(const wcfd_mesh& m, int f,
         }
       }
     } else if (q < 3 && flux) {
-      old[q] = fv[q * F + f];
+      old[q] = ld_once<NT>(fv + q * F + f);
     }
   }
   real nr[3], ar = 0.f, ro = 0.f, rn = 0.f;
   if (flux) {
-    nr[0] = m.normal[f];
-    nr[1] = m.normal[F + f];
-    nr[2] = m.normal[2 * F + f];
-    ar = m.area[f];
+    nr[0] = ld_once<NT>(m.normal + f);
+    nr[1] = ld_once<NT>(m.normal + F + f);
+    nr[2] = ld_once<NT>(m.normal + 2 * F + f);
+    ar = ld_once<NT>(m.area + f);
   }
   if (rua_f) {
     ro = rua[o];
@@ -110,9 +112,9 @@ __device__ inline void face_interp_internal(const wcfd_mesh& m, int f,
   // stores
 #pragma unroll
   for (int q = 0; q < 4; ++q)
-    if ((mask >> q) & 1) fv[q * F + f] = val[q];
-  if (rua_f) rua_f[f] = psi * ro + psi2 * rn;
-  if (flux) flux[f] = (val[0] * nr[0] + val[1] * nr[1] + val[2] * nr[2]) * ar;
+    if ((mask >> q) & 1) st_once<NT>(fv + q * F + f, val[q]);
+  if (rua_f) st_once<NT>(rua_f + f, psi * ro + psi2 * rn);
+  if (flux) st_once<NT>(flux + f, (val[0] * nr[0] + val[1] * nr[1] + val[2] * nr[2]) * ar);
 }
 
 __device__ inline void face_interp_boundary(const wcfd_mesh& m, wcfd_bc bc, int f,
@@ -307,14 +309,16 @@ __global__ __launch_bounds__(kBlock) void k_face_interp_lds(
 // many face arrays are read (rather than a 64-bit address per array and
 // face).
 
-// the six faces of cell c as byte offsets
+// the six faces of cell c as byte offsets (NT: nontemporal table loads)
+template <bool NT = false>
 __device__ inline void slot_faces(const wcfd_mesh& m, uint32_t c4, uint32_t (&f4)[6]) {
   const size_t C = m.n_cells;
 #pragma unroll
-  for (int k = 0; k < 6; ++k) f4[k] = (uint32_t)ld_off(m.slot_face + k * C, c4) * 4u;
+  for (int k = 0; k < 6; ++k) f4[k] = (uint32_t)ld_cell<NT>(m.slot_face + k * C, c4) * 4u;
 }
 
 // model_kernels.py:72-90: grad_c = sum_k phi_f A_f n_{c,k} / V_c.
+template <bool NT = false>
 __device__ inline void gg_field(const wcfd_mesh& m, uint32_t c4, const uint32_t (&f4)[6],
                                 const FBuf& phi_f, real V, real (&g)[3]) {
   const size_t C = m.n_cells;
@@ -322,9 +326,9 @@ __device__ inline void gg_field(const wcfd_mesh& m, uint32_t c4, const uint32_t 
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
     const real phi = phi_f(f4[k]);
-    a0 = a0 + (phi * ld_off(m.slot_sn + (0 * 6 + k) * C, c4)) / V;
-    a1 = a1 + (phi * ld_off(m.slot_sn + (1 * 6 + k) * C, c4)) / V;
-    a2 = a2 + (phi * ld_off(m.slot_sn + (2 * 6 + k) * C, c4)) / V;
+    a0 = a0 + (phi * ld_cell<NT>(m.slot_sn + (0 * 6 + k) * C, c4)) / V;
+    a1 = a1 + (phi * ld_cell<NT>(m.slot_sn + (1 * 6 + k) * C, c4)) / V;
+    a2 = a2 + (phi * ld_cell<NT>(m.slot_sn + (2 * 6 + k) * C, c4)) / V;
   }
   g[0] = a0; g[1] = a1; g[2] = a2;
 }
@@ -363,6 +367,7 @@ __global__ void k_mass_flux(wcfd_mesh m, const real* __restrict__ fv, real* __re
             m.area[f];
 }
 
+template <bool NT = false>
 __device__ inline real cell_div(const wcfd_mesh& m, uint32_t c4, const uint32_t (&f4)[6],
                                  const FBuf& flux) {
   const size_t C = m.n_cells;
@@ -370,7 +375,7 @@ __device__ inline real cell_div(const wcfd_mesh& m, uint32_t c4, const uint32_t 
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
     const real fl = flux(f4[k]);
-    acc = acc + (ld_off(m.slot_side + k * C, c4 >> 2) == 0 ? fl : -fl);
+    acc = acc + (ld_cell<NT>(m.slot_side + k * C, c4 >> 2) == 0 ? fl : -fl);
   }
   return acc;
 }
@@ -392,6 +397,7 @@ __global__ void k_assemble_momentum(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
                                     int compute_grad, real nu_scalar,
                                     real inv_density, real alpha, real dt, int flags,
                                     real* __restrict__ values, real* __restrict__ rhs) {
+  constexpr bool NT = WCFD_NT_CELL;
   const int correction = flags & WCFD_SCHEME_CORRECTED;
   const bool implicit_outflow = (flags & WCFD_SCHEME_IMPLICIT_OUTFLOW) != 0;
   const int c = item_index(bd.cells);
@@ -401,13 +407,13 @@ __global__ void k_assemble_momentum(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
   const uint32_t c4 = (uint32_t)c * 4u, Fi4 = (uint32_t)Fi * 4u;
   const size_t Cz = C;
   uint32_t f4[6];
-  slot_faces(m, c4, f4);
+  slot_faces<NT>(m, c4, f4);
   int side[6];
 #pragma unroll
-  for (int k = 0; k < 6; ++k) side[k] = ld_off(m.slot_side + k * Cz, (uint32_t)c);
+  for (int k = 0; k < 6; ++k) side[k] = ld_cell<NT>(m.slot_side + k * Cz, (uint32_t)c);
   const FBuf Fl_(flux, (uint32_t)m.n_faces * 4u);
   const FBuf area(m.area, (uint32_t)m.n_faces * 4u), dmag(m.dmag, (uint32_t)m.n_faces * 4u);
-  const real V = ld_off(m.volume, c4);
+  const real V = ld_cell<NT>(m.volume, c4);
   // (storing the gradients at the end, after every load, measured 1.5 %
   // slower here, unlike the face and corrector kernels)
   real gp[3];
@@ -415,20 +421,20 @@ __global__ void k_assemble_momentum(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       real g[3];
-      gg_field(m, c4, f4, face_plane(m, fv, q), V, g);
-      st_plain(cg + (3 * q + 0) * Cz, c4, g[0]);
-      st_plain(cg + (3 * q + 1) * Cz, c4, g[1]);
-      st_plain(cg + (3 * q + 2) * Cz, c4, g[2]);
+      gg_field<NT>(m, c4, f4, face_plane(m, fv, q), V, g);
+      st_cell<NT>(cg + (3 * q + 0) * Cz, c4, g[0]);
+      st_cell<NT>(cg + (3 * q + 1) * Cz, c4, g[1]);
+      st_cell<NT>(cg + (3 * q + 2) * Cz, c4, g[2]);
       if (q == 3) { gp[0] = g[0]; gp[1] = g[1]; gp[2] = g[2]; }
     }
   } else {
-    gp[0] = ld_off(cg + 9 * Cz, c4);
-    gp[1] = ld_off(cg + 10 * Cz, c4);
-    gp[2] = ld_off(cg + 11 * Cz, c4);
+    gp[0] = ld_cell<NT>(cg + 9 * Cz, c4);
+    gp[1] = ld_cell<NT>(cg + 10 * Cz, c4);
+    gp[2] = ld_cell<NT>(cg + 11 * Cz, c4);
   }
   real u[3];
 #pragma unroll
-  for (int q = 0; q < 3; ++q) u[q] = ld_off(cv + q * Cz, c4);
+  for (int q = 0; q < 3; ++q) u[q] = ld_cell<NT>(cv + q * Cz, c4);
   real diag = 0.f;
   real b[3] = {0.f, 0.f, 0.f};
   real off[6];
@@ -509,11 +515,11 @@ __global__ void k_assemble_momentum(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
     diag = diag / alpha;
   }
   // slot-major (ELL) matrix: diagonal, then the 6 slot coefficients
-  st_plain(values, c4, diag);
+  st_cell<NT>(values, c4, diag);
 #pragma unroll
-  for (int k = 0; k < 6; ++k) st_plain(values + (k + 1) * Cz, c4, off[k]);
+  for (int k = 0; k < 6; ++k) st_cell<NT>(values + (k + 1) * Cz, c4, off[k]);
 #pragma unroll
-  for (int q = 0; q < 3; ++q) st_plain(rhs + q * Cz, c4, b[q]);
+  for (int q = 0; q < 3; ++q) st_cell<NT>(rhs + q * Cz, c4, b[q]);
 }
 
 // ------------------------------------------------------ pressure assembly
@@ -532,20 +538,21 @@ __global__ void k_assemble_pressure(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
     ref_cell = bd.ref_cell[k];
     ref_value = bd.ref_value[k];
   }
+  constexpr bool NT = WCFD_NT_CELL;
   const uint32_t c4 = (uint32_t)c * 4u, Fi4 = (uint32_t)Fi * 4u, F4 = (uint32_t)m.n_faces * 4u;
   const size_t Cz = C;
   uint32_t f4[6];
-  slot_faces(m, c4, f4);
+  slot_faces<NT>(m, c4, f4);
   const FBuf pf = face_plane(m, fv, 3);
   if (compute_grad_p) {
     real g[3];
-    gg_field(m, c4, f4, pf, ld_off(m.volume, c4), g);
-    st_plain(cg + 9 * Cz, c4, g[0]);
-    st_plain(cg + 10 * Cz, c4, g[1]);
-    st_plain(cg + 11 * Cz, c4, g[2]);
+    gg_field<NT>(m, c4, f4, pf, ld_cell<NT>(m.volume, c4), g);
+    st_cell<NT>(cg + 9 * Cz, c4, g[0]);
+    st_cell<NT>(cg + 10 * Cz, c4, g[1]);
+    st_cell<NT>(cg + 11 * Cz, c4, g[2]);
   }
-  const real div = cell_div(m, c4, f4, FBuf(flux, F4));
-  if (div_out) st_plain(div_out, c4, div);
+  const real div = cell_div<NT>(m, c4, f4, FBuf(flux, F4));
+  if (div_out) st_cell<NT>(div_out, c4, div);
   const FBuf ruaf(rua_f, F4), area(m.area, F4), dmag(m.dmag, F4);
   real diag = 0.f, b = 0.f;
   real off[6];
@@ -574,10 +581,10 @@ __global__ void k_assemble_pressure(wcfd_mesh m, BatchDev bd, wcfd_bc bc,
     for (int k = 0; k < 6; ++k) off[k] = 0.f;
     b = ref_value;
   }
-  st_plain(values, c4, diag);
+  st_cell<NT>(values, c4, diag);
 #pragma unroll
-  for (int k = 0; k < 6; ++k) st_plain(values + (k + 1) * Cz, c4, off[k]);
-  st_plain(rhs, c4, b);
+  for (int k = 0; k < 6; ++k) st_cell<NT>(values + (k + 1) * Cz, c4, off[k]);
+  st_cell<NT>(rhs, c4, b);
 }
 
 // ------------------------------------------------------------ rUA, HbyA
@@ -618,12 +625,14 @@ __global__ void k_face_p_fc(wcfd_mesh m, BatchDev bd, wcfd_bc bc, const real* __
   const int f = item_index(bd.faces);
   const int F = m.n_faces, Fi = m.n_internal, Fb = m.n_boundary;
   if (f >= F || bd.face_skip(f)) return;
+  constexpr bool NT = WCFD_NT_FACE;
   const uint32_t f4 = (uint32_t)f * 4u;
   const size_t Fz = F, Fiz = Fi;
   const FBuf p = cell_plane(m, cv, 3);
   if (f < Fi) {
-    const uint32_t o4 = (uint32_t)ld_off(m.owner, f4) * 4u, n4 = (uint32_t)ld_off(m.neigh, f4) * 4u;
-    const real psi = ld_off(m.psi, f4);
+    const uint32_t o4 = (uint32_t)ld_cell<NT>(m.owner, f4) * 4u,
+                   n4 = (uint32_t)ld_cell<NT>(m.neigh, f4) * 4u;
+    const real psi = ld_cell<NT>(m.psi, f4);
     const real psi2 = 1.0f - psi;
     const FBuf g0b = cell_plane(m, cg, 9), g1b = cell_plane(m, cg, 10), g2b = cell_plane(m, cg, 11);
     const real po = p(o4), pn = p(n4);
@@ -631,12 +640,12 @@ __global__ void k_face_p_fc(wcfd_mesh m, BatchDev bd, wcfd_bc bc, const real* __
     real g0 = g0b(o4) * psi + psi2 * g0b(n4);
     real g1 = g1b(o4) * psi + psi2 * g1b(n4);
     real g2 = g2b(o4) * psi + psi2 * g2b(n4);
-    v = v + (g0 * ld_off(m.skew, f4) + g1 * ld_off(m.skew + Fiz, f4) +
-             g2 * ld_off(m.skew + 2 * Fiz, f4));
-    st_plain(fv + 3 * Fz, f4, v);
+    v = v + (g0 * ld_cell<NT>(m.skew, f4) + g1 * ld_cell<NT>(m.skew + Fiz, f4) +
+             g2 * ld_cell<NT>(m.skew + 2 * Fiz, f4));
+    st_cell<NT>(fv + 3 * Fz, f4, v);
     if (fc)
-      st_plain(fc, f4, ld_off(flux, f4) - (ld_off(rua_f, f4) * ld_off(m.area, f4) /
-                                           ld_off(m.dmag, f4)) * (pn - po));
+      st_cell<NT>(fc, f4, ld_cell<NT>(flux, f4) - (ld_cell<NT>(rua_f, f4) * ld_cell<NT>(m.area, f4) /
+                                                   ld_cell<NT>(m.dmag, f4)) * (pn - po));
   } else {
     const int j = f - Fi;
     const uint32_t p4 = (uint32_t)ld_off(m.owner, f4) * 4u;
@@ -668,20 +677,21 @@ __global__ void k_grad_p_correct(wcfd_mesh m, Seg sg, const real* __restrict__ f
   const FBuf pf = face_plane(m, fv, 3);
   const FBuf fcb(fc, (uint32_t)m.n_faces * 4u);
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  constexpr bool NT = WCFD_NT_CELL;
   for (int c = br.begin + (int)threadIdx.x; c < br.end; c += blockDim.x) {
     const uint32_t c4 = (uint32_t)c * 4u;
     uint32_t f4[6];
-    slot_faces(m, c4, f4);
+    slot_faces<NT>(m, c4, f4);
     // every load (the cell's gradient sums, rUA, HbyA, the previous
     // velocity, the corrected face fluxes) before the first store
     real g[3];
-    gg_field(m, c4, f4, pf, ld_off(m.volume, c4), g);
-    const real r = ld_off(rua, c4);
+    gg_field<NT>(m, c4, f4, pf, ld_cell<NT>(m.volume, c4), g);
+    const real r = ld_cell<NT>(rua, c4);
     real hb[3], up[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      hb[q] = ld_off(hbya + q * Cz, c4);
-      up[q] = (fc && u_prev) ? ld_off(u_prev + q * Cz, c4) : 0.f;
+      hb[q] = ld_cell<NT>(hbya + q * Cz, c4);
+      up[q] = (fc && u_prev) ? ld_cell<NT>(u_prev + q * Cz, c4) : 0.f;
     }
     real fk[6];
     int sd[6];
@@ -689,7 +699,7 @@ __global__ void k_grad_p_correct(wcfd_mesh m, Seg sg, const real* __restrict__ f
 #pragma unroll
       for (int k = 0; k < 6; ++k) {
         fk[k] = fcb(f4[k]);
-        sd[k] = ld_off(m.slot_side + k * Cz, (uint32_t)c);
+        sd[k] = ld_cell<NT>(m.slot_side + k * Cz, (uint32_t)c);
       }
     }
     real h[3];
@@ -714,14 +724,14 @@ __global__ void k_grad_p_correct(wcfd_mesh m, Seg sg, const real* __restrict__ f
       acc[1] += den;
     }
 #pragma unroll
-    for (int i = 0; i < 9; ++i) st_plain(cg + i * Cz, c4, (real)0);
-    st_plain(cg + 9 * Cz, c4, g[0]);
-    st_plain(cg + 10 * Cz, c4, g[1]);
-    st_plain(cg + 11 * Cz, c4, g[2]);
+    for (int i = 0; i < 9; ++i) st_cell<NT>(cg + i * Cz, c4, (real)0);
+    st_cell<NT>(cg + 9 * Cz, c4, g[0]);
+    st_cell<NT>(cg + 10 * Cz, c4, g[1]);
+    st_cell<NT>(cg + 11 * Cz, c4, g[2]);
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      st_plain(hbya + q * Cz, c4, h[q]);
-      st_plain(cv + q * Cz, c4, h[q]);
+      st_cell<NT>(hbya + q * Cz, c4, h[q]);
+      st_cell<NT>(cv + q * Cz, c4, h[q]);
     }
   }
   if (fc) store_partials<4>(acc, partials, sg.G, br.vb);
