@@ -78,7 +78,7 @@ enum KernelId {
   K_SET_BC, K_FACE_INTERP, K_GREEN_GAUSS, K_MASS_FLUX, K_DIVERGENCE, K_ASM_MOM, K_ASM_P,
   K_RUA_HBYA, K_RELAX_P, K_FACE_P_FC, K_GRAD_P_CORR, K_INV_DIAG, K_SPMV, K_DOT, K_CG_INIT,
   K_CG_1, K_CG_2, K_BI_INIT, K_BI_1, K_BI_2, K_BI_3, K_FOLD, K_ELL_TO_CSR, K_TERM_WEIGHTS,
-  K_EQUATION, K_CONVERGENCE, K_COUNT
+  K_EQUATION, K_CONVERGENCE, K_CG_XCD, K_COUNT
 };
 bool prof_active();
 // a start/stop event pair for one launch of kernel `id` (false: not

@@ -33,6 +33,7 @@
 #include <math.h>
 
 #include <atomic>
+#include <mutex>
 #include <vector>
 
 namespace wcfd {
@@ -666,6 +667,349 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter2(M A, Seg sg, int 
   store_partials<3>(acc, P_out, G, br.vb);
 }
 
+// ------------------------------------- XCD-local persistent single-reduction CG
+// One cooperative launch runs the whole solve.  Every case is placed on ONE
+// XCD: its workgroups all run on that XCD's CUs and share its L2, so they can
+// synchronise once per iteration with a barrier in memory (about 1 us for 32
+// workgroups, measured on MI355X) instead of a kernel boundary (about 4-5 us
+// at these sizes).  The coherence rules this relies on, measured on MI355X
+// (every combination over 5e9 reads):
+//   - a plain store, followed by s_waitcnt vmcnt(0) before the barrier's
+//     arrival, is seen by an agent-scope (sc1) load of another workgroup of
+//     the same XCD after the barrier;
+//   - plain loads can return stale data from the CU's L1 even after
+//     buffer_inv sc0, and buffer_inv sc1 (the L2 as well) costs 8 us;
+// so every value written inside the launch (x, p, r, s, w, the partials) is
+// read back with sc1 loads, and the data written before it (the matrix and
+// the Jacobi inverse) with ordinary ones.
+//
+// A workgroup reads its XCD from the XCC_ID register, takes a slot on that
+// XCD with an atomic counter, and works out its part with the plan every
+// workgroup computes alike from the solve's state: the cases still to solve,
+// in order, each on the XCD with the fewest slots used so far, taking
+// ceil(nb / rv) slots for its nb virtual workgroups (at most rv virtual
+// workgroups each).  A virtual workgroup is the launch path's workgroup: the
+// same rows, threads, per-row arithmetic, partials and fold order, so the
+// solve is bitwise the one k_cgs_iter / k_cgs_iter2 make at the same launch
+// shape (one row per thread).  Nothing depends on the dispatch order for its
+// correctness: a slot beyond the plan, a plan that does not fit, or a barrier
+// wait longer than wait_ticks sets the abort flag, every workgroup leaves at
+// its next barrier, the commit kernel leaves x as it was and the host runs the
+// launch path instead.
+constexpr int kSyncPad = 32;     // ints between sync words (128 B)
+constexpr int kSyncXcc = 0;      // [0, 8): slots taken per XCD
+constexpr int kSyncAbort = 8;    // abort flag
+constexpr int kSyncCase = 9;     // [9, 9 + K): barrier counter of case k
+constexpr unsigned kHwregXccId = 20u | (0u << 6) | ((4u - 1u) << 11);   // XCC_ID[3:0]
+constexpr int kAuxSc1 = 16;      // buffer-load cache policy: agent scope (sc1)
+
+inline size_t xcd_sync_ints(int n_cases) { return (size_t)(kSyncCase + n_cases) * kSyncPad; }
+
+struct XcdPlan {
+  int per_xcc;           // workgroup slots per XCD in this launch
+  int rv;                // most virtual workgroups one workgroup takes
+  int check_every;
+  long long wait_ticks;  // longest barrier wait (s_memrealtime, 100 MHz) before aborting
+};
+
+template <class T>
+__device__ __forceinline__ T ld_coh(const T* p) {
+  return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ T ld_coh_off(const T* base, uint32_t byte_off) {
+  return ld_coh(reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
+                                           scaled<T>(byte_off)));
+}
+
+// FBuf whose loads are agent-scope (sc1): they skip the CU's L1
+struct FBufC {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ FBufC(const real* p, uint32_t bytes)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<real*>(p), (short)0,
+                                            (int)scaled<real>(bytes), kBufWord3)) {}
+  template <class T = real>
+  __device__ __forceinline__ T operator()(uint32_t off) const {
+    if constexpr (sizeof(T) == 8)
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, scaled<T>(off), 0, kAuxSc1));
+    else
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, kAuxSc1));
+  }
+};
+
+// EarlyFold with agent-scope loads; the same terms in the same order, so the
+// same sums
+template <int ND, int E = kFoldUnroll>
+struct EarlyFoldC {
+  double v[E][ND];
+  __device__ __forceinline__ void load(const double* const (&p)[ND], int nb) {
+    const int step = blockDim.x;
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const int b = (int)threadIdx.x + u * step;
+#pragma unroll
+      for (int i = 0; i < ND; ++i) v[u][i] = b < nb ? ld_coh(p[i] + b) : 0.0;
+    }
+  }
+  __device__ __forceinline__ void finish(const double* const (&p)[ND], int nb,
+                                         double (&out)[ND]) {
+    __shared__ double lds[ND * kMaxWaves];
+    __shared__ double res[ND];
+    const int step = blockDim.x;
+    double s[ND];
+#pragma unroll
+    for (int i = 0; i < ND; ++i) s[i] = 0.0;
+#pragma unroll
+    for (int u = 0; u < E; ++u)
+      if ((int)threadIdx.x + u * step < nb) {
+#pragma unroll
+        for (int i = 0; i < ND; ++i) s[i] += v[u][i];
+      }
+    for (int b = (int)threadIdx.x + E * step; b < nb; b += step) {
+#pragma unroll
+      for (int i = 0; i < ND; ++i) s[i] += ld_coh(p[i] + b);
+    }
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    wave_sum_n<ND>(s);
+#pragma unroll
+    for (int i = 0; i < ND; ++i)
+      if (lane == 0) lds[i * kMaxWaves + wid] = s[i];
+    __syncthreads();
+    if (threadIdx.x < ND) {
+      double t = 0.0;
+      for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += lds[threadIdx.x * kMaxWaves + w];
+      res[threadIdx.x] = t;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < ND; ++i) out[i] = res[i];
+  }
+};
+
+// The part of one workgroup: case k, virtual workgroups [v_lo, v_hi), and
+// the number of workgroups in the case's group.
+struct XcdPart {
+  int k, v_lo, v_hi, members;
+};
+
+// The plan (see above) evaluated for (xcc, slot).  false: the plan does not
+// fit the launch.  part.k = -1: the slot has no work.
+__device__ inline bool xcd_part(const Seg& sg, const KState* st, const XcdPlan& pl, int xcc,
+                                int slot, XcdPart& part) {
+  int load[kNumXcd];
+#pragma unroll
+  for (int j = 0; j < (int)kNumXcd; ++j) load[j] = 0;
+  part.k = -1;
+  for (int k = 0; k < sg.n_cases; ++k) {
+    if (st[k].f.done) continue;
+    int b0, nb;
+    case_blocks(sg, k, b0, nb);
+    if (nb <= 0) continue;
+    const int m = (nb + pl.rv - 1) / pl.rv;
+    int x = 0, lx = load[0];
+#pragma unroll
+    for (int j = 1; j < (int)kNumXcd; ++j)
+      if (load[j] < lx) {
+        lx = load[j];
+        x = j;
+      }
+    if (lx + m > pl.per_xcc) return false;
+    if (x == xcc && slot >= lx && slot < lx + m) {
+      const int r = slot - lx;
+      part.k = k;
+      part.v_lo = b0 + (int)((int64_t)r * nb / m);
+      part.v_hi = b0 + (int)((int64_t)(r + 1) * nb / m);
+      part.members = m;
+    }
+#pragma unroll
+    for (int j = 0; j < (int)kNumXcd; ++j)
+      if (j == x) load[j] = lx + m;
+  }
+  return true;
+}
+
+__device__ inline void xcd_abort(int* sync) {
+  __hip_atomic_store(sync + kSyncAbort * kSyncPad, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Arrive at the case's barrier for the (n+1)-th time and wait for its other
+// workgroups (thread 0); false on abort or when the wait passes wait_ticks.
+__device__ inline bool xcd_barrier(int* cnt, int target, int* sync, long long wait_ticks) {
+  __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const long long t0 = (long long)__builtin_amdgcn_s_memrealtime();
+  for (int spins = 0; ld_coh(cnt) < target; ++spins) {
+    if (ld_coh(sync + kSyncAbort * kSyncPad)) return false;
+    if ((long long)__builtin_amdgcn_s_memrealtime() - t0 > wait_ticks || spins > (1 << 20)) {
+      xcd_abort(sync);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return true;
+}
+
+// Rows of virtual workgroup v of case k (one row per thread: chunk <= B).
+// FIRST: iteration 0.  x_in: x before the solve (read at iteration 0);
+// x_out: the iterate (read back with sc1 loads afterwards).
+template <class M, bool FIRST>
+__device__ __forceinline__ void xcd_rows(const M& A, const Seg& sg, int k, int b0, int v,
+                                         const real* __restrict__ x_in, real* x_out,
+                                         const real* r_in, real* r_out, const real* w_in,
+                                         real* w_out, const real* s_in, real* s_out, real* p,
+                                         const real* __restrict__ inv, real alpha, real beta,
+                                         double* P_out) {
+  const int e = sg.cell_end ? sg.cell_end[k] : (sg.cell_off ? sg.cell_off[k + 1] : sg.n);
+  const int begin = (sg.cell_off ? sg.cell_off[k] : 0) + (v - b0) * sg.chunk;
+  const int end = begin + sg.chunk < e ? begin + sg.chunk : e;
+  const int i = begin + (int)threadIdx.x;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (i < end) {
+    real a[M::kRow], wc[M::kRow], sc[M::kRow], ic[M::kRow], rc[M::kRow];
+    uint32_t o[M::kRow];
+    A.load_off(i, a, o);
+    const uint32_t nb4 = (uint32_t)A.n * 4u;
+    const FBufC W(w_in, nb4), Sv(s_in, nb4), R(r_in, nb4);
+    const FBuf I(inv, nb4);
+#pragma unroll
+    for (int j = 0; j < M::kRow; ++j) {
+      wc[j] = W(o[j]);
+      sc[j] = FIRST ? 0.f : Sv(o[j]);
+      ic[j] = I(o[j]);
+      rc[j] = R(o[j]);
+    }
+    const uint32_t o0 = (uint32_t)i * 4u;
+    const real p0 = FIRST ? 0.f : ld_coh_off((const real*)p, o0);
+    const real x0 = FIRST ? ld_off(x_in, o0) : ld_coh_off((const real*)x_out, o0);
+    real wn, pi, si, rn;
+    cg_row<M::kRow, FIRST>(a, wc, sc, rc, ic, p0, alpha, beta, wn, pi, si, rn);
+    st_plain(x_out, o0, mad(alpha, pi, x0));
+    st_plain(p, o0, pi);
+    st_plain(s_out, o0, si);
+    st_plain(r_out, o0, rn);
+    st_plain(w_out, o0, wn);
+    cg_dots(acc, rn, wn, ic[0]);
+  }
+  store_partials<3>(acc, P_out, sg.G, v);
+}
+
+template <class M>
+__global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_xcd(
+    M A, Seg sg, XcdPlan pl, const real* __restrict__ x_in, real* x_out, real* r0, real* r1,
+    real* w0, real* w1, real* s0, real* s1, real* p, const real* __restrict__ inv, double* P0,
+    double* P1, KState* st, int maxiter, int* sync) {
+  __shared__ int sh[4];
+  __shared__ int s_go;
+  if (threadIdx.x == 0) {
+    const int xcc = (int)(__builtin_amdgcn_s_getreg(kHwregXccId) & (kNumXcd - 1));
+    const int slot = __hip_atomic_fetch_add(sync + (kSyncXcc + xcc) * kSyncPad, 1,
+                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    XcdPart part;
+    bool ok = slot < pl.per_xcc && xcd_part(sg, st, pl, xcc, slot, part);
+    if (!ok) {
+      xcd_abort(sync);
+      part.k = -1;
+    }
+    sh[0] = part.k;
+    sh[1] = part.v_lo;
+    sh[2] = part.v_hi;
+    sh[3] = part.members;
+  }
+  __syncthreads();
+  const int k = sh[0];
+  if (k < 0) return;
+  const int v_lo = sh[1], v_hi = sh[2], members = sh[3];
+  KState* S = st + k;
+  int b0, nb;
+  case_blocks(sg, k, b0, nb);
+  const int G = sg.G;
+  const int ce = pl.check_every > 0 ? pl.check_every : 1;
+  const double thr2 = S->f.thr2;
+  double rz_prev = S->rz[0], alpha_prev = S->alpha[0];
+  const bool leader = v_lo == b0 && threadIdx.x == 0;   // records the case's flags
+  int* cnt = sync + (kSyncCase + k) * kSyncPad;
+  real* rb[2] = {r0, r1};
+  real* wb[2] = {w0, w1};
+  real* sb[2] = {s0, s1};
+  double* pb[2] = {P0, P1};   // iteration i reads pb[i & 1], writes pb[(i + 1) & 1]
+  for (int it = 0;; ++it) {
+    const int cur = it & 1;
+    if (it == 0) {
+      const real alpha = (real)alpha_prev;
+      for (int v = v_lo; v < v_hi; ++v)
+        xcd_rows<M, true>(A, sg, k, b0, v, x_in, x_out, rb[0], rb[1], wb[0], wb[1], sb[0],
+                          sb[1], p, inv, alpha, (real)0, pb[1]);
+    } else {
+      const double* pp[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) pp[i] = pb[cur] + (size_t)i * G + b0;
+      EarlyFoldC<3> ef;
+      ef.load(pp, nb);   // the prefold kernel does not run here: fold all of them
+      double gd[3];
+      ef.finish(pp, nb, gd);
+      if (it % ce == 0) {
+        // the stop test of k_check / inline_check, on every workgroup alike
+        const bool stop = !(gd[2] > thr2) || it >= maxiter;
+        if (leader) {
+          S->f.rr = gd[2];
+          if (stop) {
+            S->f.done = 1;
+            S->f.iters = it;
+          }
+        }
+        if (stop) break;
+      }
+      const double g = gd[0], d = gd[1];
+      const double beta_d = safe_div(g, rz_prev);
+      const double alpha_d = safe_div(g, d - beta_d * safe_div(g, alpha_prev));
+      rz_prev = g;
+      alpha_prev = alpha_d;
+      const real alpha = (real)alpha_d, beta = (real)beta_d;
+      for (int v = v_lo; v < v_hi; ++v)
+        xcd_rows<M, false>(A, sg, k, b0, v, x_in, x_out, rb[cur], rb[1 - cur], wb[cur],
+                           wb[1 - cur], sb[cur], sb[1 - cur], p, inv, alpha, beta,
+                           pb[1 - cur]);
+    }
+    // every store of this workgroup has reached the L2 before it arrives
+    asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (threadIdx.x == 0) s_go = xcd_barrier(cnt, members * (it + 1), sync, pl.wait_ticks);
+    __syncthreads();
+    if (!s_go) return;
+  }
+}
+
+// Zero the sync words of a launch (one workgroup).
+__global__ void k_xcd_reset(int* sync, int n_words) {
+  for (int i = threadIdx.x; i < n_words; i += blockDim.x) sync[i * kSyncPad] = 0;
+}
+
+// After k_cgs_xcd, over the launch path's virtual workgroups: unless the
+// launch aborted, copy the iterate of every case that ran (f.iters > 0: a
+// case stops only at a check, iteration check_every or later) into x.
+__global__ __launch_bounds__(kMaxKrylovBlock) void k_xcd_commit(Seg sg, const KState* st,
+                                                                 const int* sync, const real* xs,
+                                                                 real* x) {
+  if (sync[kSyncAbort * kSyncPad]) return;
+  const BlockRows br = block_rows(sg);
+  if (st[br.k].f.iters <= 0) return;
+  for (int i = br.begin + (int)threadIdx.x; i < br.end; i += blockDim.x) x[i] = xs[i];
+}
+
+// The flag records of every case to `out` (pinned host memory), and the
+// abort flag as record n_cases (its `done`).
+__global__ void k_xcd_publish(const KState* st, int n_cases, const int* sync, KFlags* out) {
+  for (int k = threadIdx.x; k <= n_cases; k += blockDim.x) {
+    if (k < n_cases) {
+      out[k] = st[k].f;
+    } else {
+      KFlags f{};
+      f.done = sync[kSyncAbort * kSyncPad];
+      out[k] = f;
+    }
+  }
+  __threadfence_system();
+}
+
 // ------------------------------------------------------------- BiCGSTAB
 // partial slots: PA[3][G] = {r0.r, r.r, b.b} (init, k_bi_3), PB[1][G] = {r0.v}
 // (k_bi_1), PC[3][G] = {t.s, t.t, s.s} (k_bi_2)
@@ -992,6 +1336,7 @@ struct KWork {
   real *r, *r0, *p0, *p1, *v0, *v1, *s, *t, *inv;
   double *PA, *PB, *PC;
   KState* st;
+  int* sync;   // k_cgs_xcd's slot, abort and barrier words
 };
 
 // State and partials first, at offsets that depend only on (n_cases,
@@ -1003,6 +1348,7 @@ static KWork carve_krylov(void* ws, int32_t n, int nvec, int n_cases, int max_bl
   w.PA = c.take<double>(3 * (size_t)max_blocks);
   w.PB = c.take<double>((size_t)max_blocks);
   w.PC = c.take<double>(3 * (size_t)max_blocks);
+  w.sync = c.take<int>(xcd_sync_ints(n_cases));
   const size_t nv = (size_t)n * nvec;
   w.r = c.take<real>(nv);
   w.r0 = c.take<real>(nv);
@@ -1022,6 +1368,7 @@ size_t krylov_workspace_bytes_batch(int32_t n, int nvec, int n_cases, int max_bl
   c.take<double>(3 * (size_t)max_blocks);
   c.take<double>((size_t)max_blocks);
   c.take<double>(3 * (size_t)max_blocks);
+  c.take<int>(xcd_sync_ints(n_cases));
   const size_t nv = (size_t)n * nvec;
   for (int i = 0; i < 8; ++i) c.take<real>(nv);
   c.take<real>(n);
@@ -1228,6 +1575,8 @@ std::atomic<int> g_krylov_grid{kKrylovGrid};
 std::atomic<int> g_inline_check{1};
 std::atomic<int> g_krylov_block{kBlock};
 std::atomic<int> g_krylov_fold{kMaxInlineFold};
+std::atomic<int> g_persistent{0};
+std::atomic<long long> g_xcd_solves{0}, g_xcd_fallbacks{0};
 }  // namespace
 
 KrylovLaunch krylov_launch() { return {g_krylov_grid.load(), g_krylov_block.load()}; }
@@ -1238,6 +1587,7 @@ struct KLaunch {
   Seg sg;
   int B;
   Items rows;
+  int max_case_blocks;   // most virtual workgroups of one case
 };
 
 // One row per thread (every workgroup's chunk fits its threads): the
@@ -1392,6 +1742,126 @@ static int cgs_impl(const M& A, const KLaunch& L, const real* b, real* x, const 
   return run_groups(sg, B, w.st, rr_part, maxiter, ce, tl.fixed_groups, group, info, s);
 }
 
+// Workgroup slots per XCD for k_cgs_xcd<M> at B threads per workgroup on the
+// current device: co-resident workgroups per CU times CUs per XCD (0: no
+// cooperative launch).
+template <class M>
+static int xcd_slots(int B) {
+  struct Entry {
+    int dev, B, slots;
+  };
+  static std::mutex mu;
+  static std::vector<Entry> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  std::lock_guard<std::mutex> lk(mu);
+  for (const Entry& e : cache)
+    if (e.dev == dev && e.B == B) return e.slots;
+  int coop = 0, cus = 0, occ = 0, slots = 0;
+  if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) == hipSuccess &&
+      coop &&
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_cgs_xcd<M>, B, 0) == hipSuccess)
+    slots = occ * (cus / (int)kNumXcd);
+  (void)hipGetLastError();
+  cache.push_back({dev, B, slots});
+  return slots;
+}
+
+constexpr int kXcdMaxRv = 64;   // most virtual workgroups per workgroup
+
+// Smallest rv whose plan is sure to fit `slots` workgroups per XCD, and the
+// slots per XCD the launch then needs: one case takes ceil(nb / rv) slots
+// of one XCD; several are placed greedily on the least-used XCD, which
+// bounds every XCD's use by (sum of the cases' slots) / 8 + the largest
+// case's slots.
+static bool xcd_plan_for(const KLaunch& L, int slots, XcdPlan& pl) {
+  const int G = L.sg.G, K = L.sg.n_cases;
+  const int widest = L.max_case_blocks > 0 ? L.max_case_blocks : G;
+  for (int rv = 1; rv <= kXcdMaxRv; ++rv) {
+    const int64_t m_case = (widest + rv - 1) / rv;
+    const int64_t need =
+        K == 1 ? m_case
+               : ((int64_t)(G + rv - 1) / rv + K + kNumXcd - 1) / kNumXcd + m_case;
+    if (need <= slots) {
+      pl.per_xcc = (int)need;
+      pl.rv = rv;
+      return true;
+    }
+  }
+  return false;
+}
+
+// The single-reduction CG solve as one k_cgs_xcd launch, with the init
+// kernels of cgs_impl before it.  ran = false: the solve is not eligible,
+// or the launch aborted with x untouched; the caller runs cgs_impl.
+template <class M>
+static int cgs_xcd_impl(const M& A, const KLaunch& L, const real* b, real* x, const Tol& tl,
+                        bool r64, int maxiter, int check_every, KWork& w,
+                        wcfd_solve_info* info, hipStream_t s, bool& ran) {
+  ran = false;
+  if constexpr (!M::kEll) {
+    return WCFD_OK;
+  } else {
+    const Seg& sg = L.sg;
+    const int n = sg.n, B = L.B, G = sg.G, K = sg.n_cases;
+    if (tl.fixed_groups > 0 || sg.chunk > B) return WCFD_OK;
+    XcdPlan pl{};
+    const int slots = xcd_slots<M>(B);
+    if (slots <= 0 || !xcd_plan_for(L, slots, pl)) return WCFD_OK;
+    pl.check_every = check_every > 0 ? check_every : 1;
+    pl.wait_ticks = 10000000;   // 100 ms at 100 MHz
+    HostSync* hs = host_sync(K + 1);
+    if (!hs) return WCFD_EHIP;
+    int e = prepare(w.st, sg, tl, s);
+    if (e) return e;
+    real* rb[2] = {w.r, w.r0};
+    real* wb[2] = {w.v0, w.v1};
+    real* sb[2] = {w.s, w.t};
+    double* pb[2] = {w.PA, w.PC};
+    if (const unsigned gi = items_grid(L.rows, n))
+      WCFD_LAUNCH(K_INV_DIAG, s, k_inv_diag<M>, gi, A, L.rows, n, w.inv);
+    if (r64)
+      WCFD_LAUNCH_B(K_CG_INIT, s, (k_cgs_init1<M, true>), G, B, A, sg, b, x, rb[0], w.PC);
+    else
+      WCFD_LAUNCH_B(K_CG_INIT, s, (k_cgs_init1<M, false>), G, B, A, sg, b, x, rb[0], w.PC);
+    WCFD_LAUNCH_B(K_CG_INIT, s, k_cgs_init2<M>, G, B, A, sg, rb[0], w.inv, wb[0], w.PA);
+    WCFD_LAUNCH(K_FOLD, s, k_init_fold, K, sg, w.PA, w.PC, w.st, 2, maxiter);
+    hipLaunchKernelGGL(k_xcd_reset, dim3(1), dim3(kWave), 0, s, w.sync, kSyncCase + K);
+    if ((e = check_launch())) return e;
+    M a_ = A;
+    Seg sg_ = sg;
+    const real* x_in = x;
+    real* x_out = w.p1;
+    int maxiter_ = maxiter;
+    void* args[] = {&a_,    &sg_,   &pl,    &x_in,  &x_out, &rb[0], &rb[1],
+                    &wb[0], &wb[1], &sb[0], &sb[1], &w.p0,  &w.inv, &pb[0],
+                    &pb[1], &w.st,  &maxiter_, &w.sync};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const bool timed = prof_active() && prof_events(K_CG_XCD, &e0, &e1);
+    if (timed && (e = check_hip(hipEventRecord(e0, s)))) return e;
+    if ((e = check_hip(hipLaunchCooperativeKernel((const void*)k_cgs_xcd<M>,
+                                                  dim3(kNumXcd * pl.per_xcc), dim3(B), args, 0,
+                                                  s))))
+      return e;
+    if (timed && (e = check_hip(hipEventRecord(e1, s)))) return e;
+    WCFD_LAUNCH_B(K_FOLD, s, k_xcd_commit, G, B, sg, w.st, w.sync, w.p1, x);
+    hipLaunchKernelGGL(k_xcd_publish, dim3(1), dim3(kWave), 0, s, w.st, K, w.sync, hs->dslot);
+    if ((e = check_launch())) return e;
+    if ((e = check_hip(hipEventRecord(hs->ev[0], s)))) return e;
+    if ((e = check_hip(hipEventSynchronize(hs->ev[0])))) return e;
+    const KFlags* f = hs->slot;
+    if (f[K].done) {   // aborted: x is as it was
+      g_persistent = 0;
+      ++g_xcd_fallbacks;
+      return WCFD_OK;
+    }
+    ++g_xcd_solves;
+    ran = true;
+    return finish(f, K, info);
+  }
+}
+
 template <class M, int NV>
 static int bicg_impl(const M& A, const KLaunch& L, const real* b, real* x, const Tol& tl,
                      int maxiter, int check_every, KWork& w, wcfd_solve_info* info,
@@ -1463,9 +1933,15 @@ static int cg_dispatch(const M& A, const KLaunch& L, int variant, const real* b,
   std::vector<wcfd_solve_info> tmp((info && sweeps > 1) ? K : 0);
   for (int sw = 0; sw < sweeps; ++sw) {
     wcfd_solve_info* out = (info && sweeps > 1) ? tmp.data() : info;
-    const int e = variant == WCFD_CG_SINGLE_REDUCTION
-                      ? cgs_impl(A, L, b, x, tl, r64, maxiter, check_every, w, out, s)
-                      : cg_impl(A, L, b, x, tl, r64, maxiter, check_every, w, out, s);
+    bool ran = false;
+    int e = WCFD_OK;
+    if (variant == WCFD_CG_SINGLE_REDUCTION && g_persistent.load())
+      e = cgs_xcd_impl(A, L, b, x, tl, r64, maxiter, check_every, w, out, s, ran);
+    if (e) return e;
+    if (!ran)
+      e = variant == WCFD_CG_SINGLE_REDUCTION
+              ? cgs_impl(A, L, b, x, tl, r64, maxiter, check_every, w, out, s)
+              : cg_impl(A, L, b, x, tl, r64, maxiter, check_every, w, out, s);
     if (e) return e;
     if (info && sweeps > 1) {
       for (int k = 0; k < K; ++k) {
@@ -1506,7 +1982,7 @@ static KLaunch single_launch(int n) {
   const KrylovLaunch kl = krylov_launch();
   Seg sg = single_seg(n, kl.block, kl.grid > 0 ? kl.grid : kMaxKrylovBlocks);
   sg.prefold = sg.G > g_krylov_fold.load() ? 1 : 0;
-  return {sg, kl.block, Items{}};
+  return {sg, kl.block, Items{}, sg.G};
 }
 
 static KLaunch launch_of(const wcfd_batch* bt, int n) {
@@ -1514,7 +1990,7 @@ static KLaunch launch_of(const wcfd_batch* bt, int n) {
   Seg sg = batch_seg(*bt, n);
   const int widest = bt->max_case_blocks > 0 ? bt->max_case_blocks : bt->n_blocks;
   sg.prefold = widest > g_krylov_fold.load() ? 1 : 0;
-  return {sg, bt->block, Items{bt->cell_blk, bt->n_cell_blk}};
+  return {sg, bt->block, Items{bt->cell_blk, bt->n_cell_blk}, widest};
 }
 
 static KWork work_of(void* ws, const wcfd_batch* bt, int n, int nvec) {
@@ -1638,6 +2114,21 @@ int wcfd_set_krylov_inline_check(int on) {
 }
 
 int wcfd_get_krylov_inline_check(void) { return g_inline_check.load(); }
+
+int wcfd_set_krylov_persistent(int on) {
+  if (on != 0 && on != 1) return WCFD_EINVAL;
+  g_persistent = on;
+  return WCFD_OK;
+}
+
+int wcfd_get_krylov_persistent(void) { return g_persistent.load(); }
+
+int wcfd_krylov_persistent_stats(int64_t* solves, int64_t* fallbacks) {
+  if (!solves || !fallbacks) return WCFD_EINVAL;
+  *solves = g_xcd_solves.load();
+  *fallbacks = g_xcd_fallbacks.load();
+  return WCFD_OK;
+}
 
 int wcfd_get_krylov_launch(int* grid, int* block) {
   if (!grid || !block) return WCFD_EINVAL;

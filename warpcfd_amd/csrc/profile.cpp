@@ -36,7 +36,7 @@ const char* kNames[K_COUNT] = {
     "set_bc", "face_interp", "green_gauss", "mass_flux", "divergence", "assemble_momentum",
     "assemble_pressure", "rua_hbya", "relax_p", "face_p_fc", "grad_p_correct", "inv_diag", "spmv",
     "dot", "cg_init", "cg_1", "cg_2", "bicgstab_init", "bicgstab_1", "bicgstab_2", "bicgstab_3", "fold", "ell_to_csr", "term_weights", "equation",
-    "convergence"};
+    "convergence", "cg_xcd"};
 }  // namespace
 
 bool prof_active() { return g_on.load(std::memory_order_relaxed); }
