@@ -722,6 +722,32 @@ __device__ __forceinline__ T ld_coh_off(const T* base, uint32_t byte_off) {
                                            scaled<T>(byte_off)));
 }
 
+template <class T>
+__device__ __forceinline__ void st_coh_off(T* base, uint32_t byte_off, T v) {
+  __hip_atomic_store(reinterpret_cast<T*>(reinterpret_cast<char*>(base) + scaled<T>(byte_off)), v,
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// store_partials with agent-scope stores of the sums
+template <int ND>
+__device__ inline void store_partials_coh(double (&v)[ND], double* part, int G, int vb) {
+  __shared__ double lds[ND * kMaxWaves];
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  wave_sum_n<ND>(v);
+#pragma unroll
+  for (int i = 0; i < ND; ++i)
+    if (lane == 0) lds[i * kMaxWaves + wid] = v[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+      double t = 0.0;
+      for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += lds[i * kMaxWaves + w];
+      __hip_atomic_store(part + i * G + vb, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
 // FBuf whose loads are agent-scope (sc1): they skip the CU's L1
 struct FBufC {
   __amdgpu_buffer_rsrc_t r;
@@ -736,6 +762,9 @@ struct FBufC {
       return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, kAuxSc1));
   }
 };
+
+// one of a ping-pong pair
+__device__ __forceinline__ real* wb_sel(real* a, real* b, int which) { return which ? b : a; }
 
 // EarlyFold with agent-scope loads; the same terms in the same order, so the
 // same sums
@@ -848,55 +877,19 @@ __device__ inline bool xcd_barrier(int* cnt, int target, int* sync, long long wa
   return true;
 }
 
-// Rows of virtual workgroup v of case k (one row per thread: chunk <= B).
-// FIRST: iteration 0.  x_in: x before the solve (read at iteration 0);
-// x_out: the iterate (read back with sc1 loads afterwards).
-template <class M, bool FIRST>
-__device__ __forceinline__ void xcd_rows(const M& A, const Seg& sg, int k, int b0, int v,
-                                         const real* __restrict__ x_in, real* x_out,
-                                         const real* r_in, real* r_out, const real* w_in,
-                                         real* w_out, const real* s_in, real* s_out, real* p,
-                                         const real* __restrict__ inv, real alpha, real beta,
-                                         double* P_out) {
-  const int e = sg.cell_end ? sg.cell_end[k] : (sg.cell_off ? sg.cell_off[k + 1] : sg.n);
-  const int begin = (sg.cell_off ? sg.cell_off[k] : 0) + (v - b0) * sg.chunk;
-  const int end = begin + sg.chunk < e ? begin + sg.chunk : e;
-  const int i = begin + (int)threadIdx.x;
-  double acc[3] = {0.0, 0.0, 0.0};
-  if (i < end) {
-    real a[M::kRow], wc[M::kRow], sc[M::kRow], ic[M::kRow], rc[M::kRow];
-    uint32_t o[M::kRow];
-    A.load_off(i, a, o);
-    const uint32_t nb4 = (uint32_t)A.n * 4u;
-    const FBufC W(w_in, nb4), Sv(s_in, nb4), R(r_in, nb4);
-    const FBuf I(inv, nb4);
-#pragma unroll
-    for (int j = 0; j < M::kRow; ++j) {
-      wc[j] = W(o[j]);
-      sc[j] = FIRST ? 0.f : Sv(o[j]);
-      ic[j] = I(o[j]);
-      rc[j] = R(o[j]);
-    }
-    const uint32_t o0 = (uint32_t)i * 4u;
-    const real p0 = FIRST ? 0.f : ld_coh_off((const real*)p, o0);
-    const real x0 = FIRST ? ld_off(x_in, o0) : ld_coh_off((const real*)x_out, o0);
-    real wn, pi, si, rn;
-    cg_row<M::kRow, FIRST>(a, wc, sc, rc, ic, p0, alpha, beta, wn, pi, si, rn);
-    st_plain(x_out, o0, mad(alpha, pi, x0));
-    st_plain(p, o0, pi);
-    st_plain(s_out, o0, si);
-    st_plain(r_out, o0, rn);
-    st_plain(w_out, o0, wn);
-    cg_dots(acc, rn, wn, ic[0]);
-  }
-  store_partials<3>(acc, P_out, sg.G, v);
-}
-
-template <class M>
+// The persistent kernel: every workgroup takes RV (1 or 2) of its case's
+// virtual workgroups, one row per thread each, and keeps the rows'
+// coefficients, column offsets, Jacobi inverses at the columns, x and p in
+// registers for the whole solve; an iteration reads only the gathers of w,
+// s and r (sc1) and the case's partials, and writes w, s, r and the
+// partials.  The fold's loads are issued before the gathers (as EarlyFold
+// does in k_cgs_iter), so both are in flight together.
+template <class M, int RV>
 __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_xcd(
     M A, Seg sg, XcdPlan pl, const real* __restrict__ x_in, real* x_out, real* r0, real* r1,
-    real* w0, real* w1, real* s0, real* s1, real* p, const real* __restrict__ inv, double* P0,
+    real* w0, real* w1, real* s0, real* s1, const real* __restrict__ inv, double* P0,
     double* P1, KState* st, int maxiter, int* sync) {
+  constexpr int KR = M::kRow;
   __shared__ int sh[4];
   __shared__ int s_go;
   if (threadIdx.x == 0) {
@@ -904,7 +897,8 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_xcd(
     const int slot = __hip_atomic_fetch_add(sync + (kSyncXcc + xcc) * kSyncPad, 1,
                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     XcdPart part;
-    bool ok = slot < pl.per_xcc && xcd_part(sg, st, pl, xcc, slot, part);
+    const bool ok = slot < pl.per_xcc && xcd_part(sg, st, pl, xcc, slot, part) &&
+                    (part.k < 0 || part.v_hi - part.v_lo <= RV);
     if (!ok) {
       xcd_abort(sync);
       part.k = -1;
@@ -915,9 +909,13 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_xcd(
     sh[3] = part.members;
   }
   __syncthreads();
-  const int k = sh[0];
+  // Values broadcast through LDS are the same in every lane; readfirstlane
+  // says so to the compiler, so that the exits below are scalar branches.
+  const int k = __builtin_amdgcn_readfirstlane(sh[0]);
   if (k < 0) return;
-  const int v_lo = sh[1], v_hi = sh[2], members = sh[3];
+  const int v_lo = __builtin_amdgcn_readfirstlane(sh[1]);
+  const int nv = __builtin_amdgcn_readfirstlane(sh[2]) - v_lo;
+  const int members = __builtin_amdgcn_readfirstlane(sh[3]);
   KState* S = st + k;
   int b0, nb;
   case_blocks(sg, k, b0, nb);
@@ -927,55 +925,113 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_xcd(
   double rz_prev = S->rz[0], alpha_prev = S->alpha[0];
   const bool leader = v_lo == b0 && threadIdx.x == 0;   // records the case's flags
   int* cnt = sync + (kSyncCase + k) * kSyncPad;
-  real* rb[2] = {r0, r1};
-  real* wb[2] = {w0, w1};
-  real* sb[2] = {s0, s1};
-  double* pb[2] = {P0, P1};   // iteration i reads pb[i & 1], writes pb[(i + 1) & 1]
+  // the rows, cached
+  const int e = sg.cell_end ? sg.cell_end[k] : (sg.cell_off ? sg.cell_off[k + 1] : sg.n);
+  const int c0 = sg.cell_off ? sg.cell_off[k] : 0;
+  const uint32_t nb4 = (uint32_t)A.n * 4u;
+  real a[RV][KR], ic[RV][KR], xr[RV], pr[RV];
+  uint32_t o[RV][KR];
+  bool has[RV];
+#pragma unroll
+  for (int q = 0; q < RV; ++q) {
+    const int begin = c0 + (v_lo + q - b0) * sg.chunk;
+    const int end = begin + sg.chunk < e ? begin + sg.chunk : e;
+    const int i = begin + (int)threadIdx.x;
+    has[q] = q < nv && i < end;
+    xr[q] = pr[q] = 0.f;
+    if (has[q]) {
+      A.load_off(i, a[q], o[q]);
+      const FBuf I(inv, nb4);
+#pragma unroll
+      for (int j = 0; j < KR; ++j) ic[q][j] = I(o[q][j]);
+      xr[q] = ld_off(x_in, o[q][0]);
+    }
+  }
   for (int it = 0;; ++it) {
     const int cur = it & 1;
-    if (it == 0) {
-      const real alpha = (real)alpha_prev;
-      for (int v = v_lo; v < v_hi; ++v)
-        xcd_rows<M, true>(A, sg, k, b0, v, x_in, x_out, rb[0], rb[1], wb[0], wb[1], sb[0],
-                          sb[1], p, inv, alpha, (real)0, pb[1]);
-    } else {
-      const double* pp[3];
+    const real* W = wb_sel(w0, w1, cur);
+    const real* Sv = wb_sel(s0, s1, cur);
+    const real* R = wb_sel(r0, r1, cur);
+    real* w_out = wb_sel(w0, w1, 1 - cur);
+    real* s_out = wb_sel(s0, s1, 1 - cur);
+    real* r_out = wb_sel(r0, r1, 1 - cur);
+    double* P_in = cur ? P1 : P0;
+    double* P_out = cur ? P0 : P1;
+    const double* pp[3];
 #pragma unroll
-      for (int i = 0; i < 3; ++i) pp[i] = pb[cur] + (size_t)i * G + b0;
-      EarlyFoldC<3> ef;
-      ef.load(pp, nb);   // the prefold kernel does not run here: fold all of them
+    for (int i = 0; i < 3; ++i) pp[i] = P_in + (size_t)i * G + b0;
+    EarlyFoldC<3> ef;
+    if (it > 0) ef.load(pp, nb);   // the prefold kernel does not run here: fold all of them
+    real wc[RV][KR], sc[RV][KR], rc[RV][KR];
+#pragma unroll
+    for (int q = 0; q < RV; ++q)
+      if (has[q]) {
+#pragma unroll
+        for (int j = 0; j < KR; ++j) {
+          wc[q][j] = ld_coh_off(W, o[q][j]);
+          sc[q][j] = it > 0 ? ld_coh_off(Sv, o[q][j]) : (real)0;
+          rc[q][j] = ld_coh_off(R, o[q][j]);
+        }
+      }
+    // The stop test is a flag, and the loop's only exits are at the top
+    // level of its body: an exit nested in the check branch was compiled so
+    // that a check iteration that went on kept the previous rz and alpha.
+    real alpha = (real)alpha_prev, beta = 0;
+    int stop = 0;
+    if (it > 0) {
       double gd[3];
       ef.finish(pp, nb, gd);
-      if (it % ce == 0) {
-        // the stop test of k_check / inline_check, on every workgroup alike
-        const bool stop = !(gd[2] > thr2) || it >= maxiter;
-        if (leader) {
-          S->f.rr = gd[2];
-          if (stop) {
-            S->f.done = 1;
-            S->f.iters = it;
-          }
+      const bool check = it % ce == 0;
+      // the stop test of k_check / inline_check, on every workgroup alike
+      stop = check && (!(gd[2] > thr2) || it >= maxiter) ? 1 : 0;
+      if (leader && check) {
+        S->f.rr = gd[2];
+        if (stop) {
+          S->f.done = 1;
+          S->f.iters = it;
         }
-        if (stop) break;
       }
       const double g = gd[0], d = gd[1];
       const double beta_d = safe_div(g, rz_prev);
       const double alpha_d = safe_div(g, d - beta_d * safe_div(g, alpha_prev));
       rz_prev = g;
       alpha_prev = alpha_d;
-      const real alpha = (real)alpha_d, beta = (real)beta_d;
-      for (int v = v_lo; v < v_hi; ++v)
-        xcd_rows<M, false>(A, sg, k, b0, v, x_in, x_out, rb[cur], rb[1 - cur], wb[cur],
-                           wb[1 - cur], sb[cur], sb[1 - cur], p, inv, alpha, beta,
-                           pb[1 - cur]);
+      alpha = (real)alpha_d;
+      beta = (real)beta_d;
+    }
+    if (__builtin_amdgcn_readfirstlane(stop)) break;
+#pragma unroll
+    for (int q = 0; q < RV; ++q) {
+      if (q >= nv) continue;   // uniform across the workgroup
+      double acc[3] = {0.0, 0.0, 0.0};
+      if (has[q]) {
+        real wn, pi, si, rn;
+        if (it == 0)
+          cg_row<KR, true>(a[q], wc[q], sc[q], rc[q], ic[q], pr[q], alpha, beta, wn, pi, si, rn);
+        else
+          cg_row<KR, false>(a[q], wc[q], sc[q], rc[q], ic[q], pr[q], alpha, beta, wn, pi, si, rn);
+        xr[q] = mad(alpha, pi, xr[q]);
+        pr[q] = pi;
+        st_coh_off(s_out, o[q][0], si);
+        st_coh_off(r_out, o[q][0], rn);
+        st_coh_off(w_out, o[q][0], wn);
+        cg_dots(acc, rn, wn, ic[q][0]);
+      }
+      store_partials_coh<3>(acc, P_out, G, v_lo + q);
+      // thread 0 has read the wave sums from LDS before the next
+      // store_partials overwrites them
+      __syncthreads();
     }
     // every store of this workgroup has reached the L2 before it arrives
     asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory");
     __syncthreads();
     if (threadIdx.x == 0) s_go = xcd_barrier(cnt, members * (it + 1), sync, pl.wait_ticks);
     __syncthreads();
-    if (!s_go) return;
+    if (!__builtin_amdgcn_readfirstlane(s_go)) return;
   }
+#pragma unroll
+  for (int q = 0; q < RV; ++q)
+    if (has[q]) st_plain(x_out, o[q][0], xr[q]);
 }
 
 // Zero the sync words of a launch (one workgroup).
@@ -1742,10 +1798,10 @@ static int cgs_impl(const M& A, const KLaunch& L, const real* b, real* x, const 
   return run_groups(sg, B, w.st, rr_part, maxiter, ce, tl.fixed_groups, group, info, s);
 }
 
-// Workgroup slots per XCD for k_cgs_xcd<M> at B threads per workgroup on the
-// current device: co-resident workgroups per CU times CUs per XCD (0: no
+// Workgroup slots per XCD for k_cgs_xcd<M, RV> at B threads per workgroup on
+// the current device: co-resident workgroups per CU times CUs per XCD (0: no
 // cooperative launch).
-template <class M>
+template <class M, int RV>
 static int xcd_slots(int B) {
   struct Entry {
     int dev, B, slots;
@@ -1761,35 +1817,28 @@ static int xcd_slots(int B) {
   if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) == hipSuccess &&
       coop &&
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_cgs_xcd<M>, B, 0) == hipSuccess)
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_cgs_xcd<M, RV>, B, 0) == hipSuccess)
     slots = occ * (cus / (int)kNumXcd);
   (void)hipGetLastError();
   cache.push_back({dev, B, slots});
   return slots;
 }
 
-constexpr int kXcdMaxRv = 64;   // most virtual workgroups per workgroup
-
-// Smallest rv whose plan is sure to fit `slots` workgroups per XCD, and the
-// slots per XCD the launch then needs: one case takes ceil(nb / rv) slots
-// of one XCD; several are placed greedily on the least-used XCD, which
-// bounds every XCD's use by (sum of the cases' slots) / 8 + the largest
-// case's slots.
-static bool xcd_plan_for(const KLaunch& L, int slots, XcdPlan& pl) {
+// Whether virtual workgroups taken rv at a time fit `slots` workgroups per
+// XCD, and the slots per XCD the launch then needs: one case takes
+// ceil(nb / rv) slots of one XCD; several are placed greedily on the
+// least-used XCD, which bounds every XCD's use by (sum of the cases' slots)
+// / 8 + the largest case's slots.
+static bool xcd_plan_fits(const KLaunch& L, int rv, int slots, XcdPlan& pl) {
   const int G = L.sg.G, K = L.sg.n_cases;
   const int widest = L.max_case_blocks > 0 ? L.max_case_blocks : G;
-  for (int rv = 1; rv <= kXcdMaxRv; ++rv) {
-    const int64_t m_case = (widest + rv - 1) / rv;
-    const int64_t need =
-        K == 1 ? m_case
-               : ((int64_t)(G + rv - 1) / rv + K + kNumXcd - 1) / kNumXcd + m_case;
-    if (need <= slots) {
-      pl.per_xcc = (int)need;
-      pl.rv = rv;
-      return true;
-    }
-  }
-  return false;
+  const int64_t m_case = (widest + rv - 1) / rv;
+  const int64_t need =
+      K == 1 ? m_case : ((int64_t)(G + rv - 1) / rv + K + kNumXcd - 1) / kNumXcd + m_case;
+  if (slots <= 0 || need > slots) return false;
+  pl.per_xcc = (int)need;
+  pl.rv = rv;
+  return true;
 }
 
 // The single-reduction CG solve as one k_cgs_xcd launch, with the init
@@ -1807,8 +1856,9 @@ static int cgs_xcd_impl(const M& A, const KLaunch& L, const real* b, real* x, co
     const int n = sg.n, B = L.B, G = sg.G, K = sg.n_cases;
     if (tl.fixed_groups > 0 || sg.chunk > B) return WCFD_OK;
     XcdPlan pl{};
-    const int slots = xcd_slots<M>(B);
-    if (slots <= 0 || !xcd_plan_for(L, slots, pl)) return WCFD_OK;
+    if (!xcd_plan_fits(L, 1, xcd_slots<M, 1>(B), pl) &&
+        !xcd_plan_fits(L, 2, xcd_slots<M, 2>(B), pl))
+      return WCFD_OK;   // more rows than one XCD's registers hold: per-iteration kernels
     pl.check_every = check_every > 0 ? check_every : 1;
     pl.wait_ticks = 10000000;   // 100 ms at 100 MHz
     HostSync* hs = host_sync(K + 1);
@@ -1834,15 +1884,15 @@ static int cgs_xcd_impl(const M& A, const KLaunch& L, const real* b, real* x, co
     const real* x_in = x;
     real* x_out = w.p1;
     int maxiter_ = maxiter;
-    void* args[] = {&a_,    &sg_,   &pl,    &x_in,  &x_out, &rb[0], &rb[1],
-                    &wb[0], &wb[1], &sb[0], &sb[1], &w.p0,  &w.inv, &pb[0],
-                    &pb[1], &w.st,  &maxiter_, &w.sync};
+    void* args[] = {&a_,    &sg_,   &pl,    &x_in,  &x_out, &rb[0],    &rb[1], &wb[0],
+                    &wb[1], &sb[0], &sb[1], &w.inv, &pb[0], &pb[1], &w.st, &maxiter_,
+                    &w.sync};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     const bool timed = prof_active() && prof_events(K_CG_XCD, &e0, &e1);
     if (timed && (e = check_hip(hipEventRecord(e0, s)))) return e;
-    if ((e = check_hip(hipLaunchCooperativeKernel((const void*)k_cgs_xcd<M>,
-                                                  dim3(kNumXcd * pl.per_xcc), dim3(B), args, 0,
-                                                  s))))
+    const void* kern = pl.rv == 1 ? (const void*)k_cgs_xcd<M, 1> : (const void*)k_cgs_xcd<M, 2>;
+    if ((e = check_hip(hipLaunchCooperativeKernel(kern, dim3(kNumXcd * pl.per_xcc), dim3(B),
+                                                  args, 0, s))))
       return e;
     if (timed && (e = check_hip(hipEventRecord(e1, s)))) return e;
     WCFD_LAUNCH_B(K_FOLD, s, k_xcd_commit, G, B, sg, w.st, w.sync, w.p1, x);
