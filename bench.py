@@ -685,8 +685,11 @@ def bench_cylinder(args):
         tuned = install_launch(args.krylov_launch)
     elif not args.no_tune:
         from warpcfd_amd.tune import autotune_krylov
-        tt = autotune_krylov(dm, cg_variant=sv.cg_variant)
-        tuned = {'grid': tt['grid'], 'block': tt['block'], 'fold': tt['fold']}
+        # the pressure CG runs about 40 iterations per momentum BiCGSTAB
+        # iteration in these steps: weigh the two timings accordingly
+        tt = autotune_krylov(dm, cg_variant=sv.cg_variant, bicg_weight=0.025)
+        tuned = {'grid': tt['grid'], 'block': tt['block'], 'fold': tt['fold'],
+                 'persistent_rows': tt['persistent']}
     C, F, Fi, Fb, nnz = dm.n_cells, dm.n_faces, dm.n_internal, dm.n_boundary, dm.nnz
     for _ in range(args.warmup):
         sv.outer_iteration(dt=dt)
@@ -933,6 +936,9 @@ def main():
     ap.add_argument('--krylov-launch', default=None,
                     help='GRIDxBLOCK[xFOLD]: fixed Krylov launch shape (grid 0: one row per '
                          'thread; FOLD: inline-fold limit, default 2048) instead of autotuning')
+    ap.add_argument('--persistent-rows', type=int, default=0,
+                    help='persistent single-reduction CG (one launch per solve, each case on one '
+                         'XCD) for solves whose widest case has at most this many rows; 0: off')
     ap.add_argument('--krylov-check', default='inline', choices=['inline', 'kernel'],
                     help='single-case CG stop test: inside the first iteration kernel of each '
                          'check group (default) or in a check kernel after it (bitwise the same)')
@@ -942,6 +948,11 @@ def main():
               '(cases shard across GPUs with no communication; see the design notes).',
               file=sys.stderr)
         sys.exit(2)
+    if args.persistent_rows:
+        from warpcfd_amd import _lib
+        for lib in (_lib.load(), _lib.load64()):
+            _lib.check(lib.wcfd_set_krylov_persistent(args.persistent_rows),
+                       'set_krylov_persistent')
     if args.krylov_check != 'inline':
         from warpcfd_amd import _lib
         for lib in (_lib.load(), _lib.load64()):
@@ -996,10 +1007,16 @@ def main():
         tuned = install_launch(args.krylov_launch)
     elif not args.no_tune:
         from warpcfd_amd.tune import autotune_krylov
-        t = autotune_krylov(dm, cg_variant=solver.cg_variant)
+        # a SIMPLE iteration of the channel runs about 420 pressure CG
+        # iterations and 10 momentum BiCGSTAB iterations: weigh the two
+        # timings accordingly
+        t = autotune_krylov(dm, cg_variant=solver.cg_variant, bicg_weight=0.025)
         tuned = {'grid': t['grid'], 'block': t['block'], 'fold': t['fold'],
+                 'persistent_rows': t['persistent'],
                  'cg_ms_per_60_it': round(t['cg_ms'], 3),
-                 'candidates': len(t['candidates'])}
+                 'candidates': len(t['candidates']),
+                 'refined': [(r['grid'], r['block'], r['fold'], r['persistent'],
+                              round(r['cg_ms'], 4)) for r in t['refined']]}
     C, F, Fi, Fb, nnz = dm.n_cells, dm.n_faces, dm.n_internal, dm.n_boundary, dm.nnz
 
     for _ in range(args.warmup):

@@ -459,19 +459,23 @@ int wcfd_get_krylov_fold(void);
 int wcfd_set_krylov_inline_check(int on);           /* 0 or 1 */
 int wcfd_get_krylov_inline_check(void);
 
-/* Persistent single-reduction CG (1; 0, the default: one launch per
- * iteration).  One cooperative launch runs the whole solve, every case on
- * the CUs of ONE XCD (its workgroups share that XCD's L2 and synchronise
- * through a barrier in memory once per iteration), for single-case and
- * batched solves with one row per thread (chunk <= block) and no fixed launch
- * budget; other solves take the per-iteration kernels.  The iterations, the
- * solution and the reported residual are bitwise those of the per-iteration
- * kernels at the same launch shape.  A launch whose workgroups do not land
- * as planned, or whose barrier waits longer than 100 ms, aborts before x is
- * written; the solve then runs on the per-iteration kernels and the setting
- * turns itself off.  Process-wide.  No counterpart in the reference
- * (warp.optim.linear launches per iteration). */
-int wcfd_set_krylov_persistent(int on);             /* 0 or 1 */
+/* Persistent single-reduction CG for solves whose widest case has at most
+ * max_rows rows (its virtual workgroups times the chunk); 0, the default:
+ * one launch per iteration always.  One cooperative launch runs the whole
+ * solve, every case on the CUs of ONE XCD (its workgroups share that XCD's
+ * L2 and meet at a barrier in memory once per iteration), for single-case and
+ * batched solves with one row per thread (chunk <= block), at most two
+ * virtual workgroups per workgroup, and no fixed launch budget; other solves
+ * take the per-iteration kernels.  The iterations, the solution and the
+ * reported residual are bitwise those of the per-iteration kernels at the
+ * same launch shape.  Measured on MI355X: an iteration costs about 4 us plus
+ * 0.2 us per thousand rows (one XCD's gather rate), against 5.5-6 us per
+ * launch, so it pays below about 8 000 rows per case.  A launch whose
+ * workgroups do not land as planned, or whose barrier waits longer than
+ * 100 ms, aborts before x is written; the solve then runs on the
+ * per-iteration kernels and the setting turns itself off.  Process-wide.  No
+ * counterpart in the reference (warp.optim.linear launches per iteration). */
+int wcfd_set_krylov_persistent(int max_rows);       /* max_rows >= 0 */
 int wcfd_get_krylov_persistent(void);
 /* Solves run by the persistent kernel and solves that fell back after an
  * abort, since the library was loaded. [host] outputs */

@@ -50,12 +50,15 @@ def _stats(lib):
     return s.value, f.value
 
 
+ON = 1 << 30   # wcfd_set_krylov_persistent: every eligible solve
+
+
 def _cg(lib, model, A, ell, b, persistent, atol=1e-7, tol=1e-7, maxiter=2000, check_every=10,
         csr=False):
     import torch
     from warpcfd_amd._lib import SolveInfo, CG_SINGLE_REDUCTION
     from warpcfd_amd.FV.device import ptr, stream_ptr
-    assert lib.wcfd_set_krylov_persistent(persistent) == 0
+    assert lib.wcfd_set_krylov_persistent(ON if persistent else 0) == 0
     L = model.layout
     n = L.num_cells
     dev = ell.device
@@ -159,7 +162,7 @@ def test_persistent_batch_bitwise(restore, chunk, block):
     lib = restore
     out = []
     for persistent in (0, 1):
-        assert lib.wcfd_set_krylov_persistent(persistent) == 0
+        assert lib.wcfd_set_krylov_persistent(ON if persistent else 0) == 0
         specs = [CaseSpec(_cavity(n, re), stop=0.0, max_outer=100)
                  for n, re in ((24, 100.0), (40, 400.0), (64, 1000.0), (33, 250.0))]
         specs.append(channel_spec(stop=0.0, max_outer=100))
@@ -189,7 +192,7 @@ def test_persistent_simple_run_bitwise(restore, float_dtype):
         model = device_case(nodes, cells, 'channel', float_dtype=float_dtype)
         lib = model.lib
         assert lib.wcfd_set_krylov_launch(0, 256) == 0
-        assert lib.wcfd_set_krylov_persistent(persistent) == 0
+        assert lib.wcfd_set_krylov_persistent(ON if persistent else 0) == 0
         s0, f0 = _stats(lib)
         sv = IncompressibleSolver(model, 0.7, 0.3, implicit_outflow=True)
         sv.cg_variant = _lib.CG_SINGLE_REDUCTION
@@ -200,3 +203,31 @@ def test_persistent_simple_run_bitwise(restore, float_dtype):
                     [(h['p_iters'], h['u_iters']) for h in hist]))
     assert out[0][1] == out[1][1]
     assert np.array_equal(out[0][0], out[1][0])
+
+
+def test_persistent_row_threshold(restore):
+    """wcfd_set_krylov_persistent(max_rows) takes the persistent kernel for
+    solves whose widest case has at most max_rows rows (virtual workgroups
+    times chunk: 1 920 rows over 8 workgroups of 256 threads take chunks of
+    240, so 8 x 240 = 1 920 here) and the per-iteration kernels above."""
+    lib = restore
+    model, A, ell, b = _system((16, 12, 10), 45)   # 1 920 rows
+    assert lib.wcfd_set_krylov_launch(0, 256) == 0
+    ref = _cg(lib, model, A, ell, b, 0)
+    for limit, ran in ((1920, True), (1919, False), (0, False)):
+        s0, f0 = _stats(lib)
+        x, info = _cg(lib, model, A, ell, b, 0)
+        assert lib.wcfd_set_krylov_persistent(limit) == 0
+        import torch
+        from warpcfd_amd._lib import SolveInfo, CG_SINGLE_REDUCTION
+        from warpcfd_amd.FV.device import ptr, stream_ptr
+        n = model.layout.num_cells
+        ws = torch.zeros(lib.wcfd_krylov_workspace_bytes(n, 1), dtype=torch.uint8, device=ell.device)
+        xt = torch.zeros(n, dtype=torch.float32, device=ell.device)
+        si = SolveInfo()
+        assert lib.wcfd_cg_ell(model.dm.desc_ref, ptr(ell), CG_SINGLE_REDUCTION, ptr(b), ptr(xt),
+                               1e-7, 1e-7, 2000, 10, ptr(ws), ctypes.byref(si), stream_ptr()) == 0
+        torch.cuda.synchronize()
+        s1, f1 = _stats(lib)
+        assert f1 == f0 and (s1 > s0) == ran, (limit, ran)
+        assert np.array_equal(xt.cpu().numpy(), ref[0]) and si.iterations == ref[1][0]

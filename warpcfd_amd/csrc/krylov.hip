@@ -701,7 +701,6 @@ constexpr int kSyncXcc = 0;      // [0, 8): slots taken per XCD
 constexpr int kSyncAbort = 8;    // abort flag
 constexpr int kSyncCase = 9;     // [9, 9 + K): barrier counter of case k
 constexpr unsigned kHwregXccId = 20u | (0u << 6) | ((4u - 1u) << 11);   // XCC_ID[3:0]
-constexpr int kAuxSc1 = 16;      // buffer-load cache policy: agent scope (sc1)
 
 inline size_t xcd_sync_ints(int n_cases) { return (size_t)(kSyncCase + n_cases) * kSyncPad; }
 
@@ -721,47 +720,6 @@ __device__ __forceinline__ T ld_coh_off(const T* base, uint32_t byte_off) {
   return ld_coh(reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
                                            scaled<T>(byte_off)));
 }
-
-template <class T>
-__device__ __forceinline__ void st_coh_off(T* base, uint32_t byte_off, T v) {
-  __hip_atomic_store(reinterpret_cast<T*>(reinterpret_cast<char*>(base) + scaled<T>(byte_off)), v,
-                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// store_partials with agent-scope stores of the sums
-template <int ND>
-__device__ inline void store_partials_coh(double (&v)[ND], double* part, int G, int vb) {
-  __shared__ double lds[ND * kMaxWaves];
-  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-  wave_sum_n<ND>(v);
-#pragma unroll
-  for (int i = 0; i < ND; ++i)
-    if (lane == 0) lds[i * kMaxWaves + wid] = v[i];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int i = 0; i < ND; ++i) {
-      double t = 0.0;
-      for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += lds[i * kMaxWaves + w];
-      __hip_atomic_store(part + i * G + vb, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
-}
-
-// FBuf whose loads are agent-scope (sc1): they skip the CU's L1
-struct FBufC {
-  __amdgpu_buffer_rsrc_t r;
-  __device__ __forceinline__ FBufC(const real* p, uint32_t bytes)
-      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<real*>(p), (short)0,
-                                            (int)scaled<real>(bytes), kBufWord3)) {}
-  template <class T = real>
-  __device__ __forceinline__ T operator()(uint32_t off) const {
-    if constexpr (sizeof(T) == 8)
-      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, scaled<T>(off), 0, kAuxSc1));
-    else
-      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, kAuxSc1));
-  }
-};
 
 // one of a ping-pong pair
 __device__ __forceinline__ real* wb_sel(real* a, real* b, int which) { return which ? b : a; }
@@ -1012,12 +970,12 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_xcd(
           cg_row<KR, false>(a[q], wc[q], sc[q], rc[q], ic[q], pr[q], alpha, beta, wn, pi, si, rn);
         xr[q] = mad(alpha, pi, xr[q]);
         pr[q] = pi;
-        st_coh_off(s_out, o[q][0], si);
-        st_coh_off(r_out, o[q][0], rn);
-        st_coh_off(w_out, o[q][0], wn);
+        st_plain(s_out, o[q][0], si);
+        st_plain(r_out, o[q][0], rn);
+        st_plain(w_out, o[q][0], wn);
         cg_dots(acc, rn, wn, ic[q][0]);
       }
-      store_partials_coh<3>(acc, P_out, G, v_lo + q);
+      store_partials<3>(acc, P_out, G, v_lo + q);
       // thread 0 has read the wave sums from LDS before the next
       // store_partials overwrites them
       __syncthreads();
@@ -1631,7 +1589,7 @@ std::atomic<int> g_krylov_grid{kKrylovGrid};
 std::atomic<int> g_inline_check{1};
 std::atomic<int> g_krylov_block{kBlock};
 std::atomic<int> g_krylov_fold{kMaxInlineFold};
-std::atomic<int> g_persistent{0};
+std::atomic<int> g_persistent{0};   // most rows per case of a persistent solve (0: off)
 std::atomic<long long> g_xcd_solves{0}, g_xcd_fallbacks{0};
 }  // namespace
 
@@ -1855,6 +1813,8 @@ static int cgs_xcd_impl(const M& A, const KLaunch& L, const real* b, real* x, co
     const Seg& sg = L.sg;
     const int n = sg.n, B = L.B, G = sg.G, K = sg.n_cases;
     if (tl.fixed_groups > 0 || sg.chunk > B) return WCFD_OK;
+    const int widest = L.max_case_blocks > 0 ? L.max_case_blocks : G;
+    if ((int64_t)widest * sg.chunk > (int64_t)g_persistent.load()) return WCFD_OK;
     XcdPlan pl{};
     if (!xcd_plan_fits(L, 1, xcd_slots<M, 1>(B), pl) &&
         !xcd_plan_fits(L, 2, xcd_slots<M, 2>(B), pl))
@@ -1901,7 +1861,7 @@ static int cgs_xcd_impl(const M& A, const KLaunch& L, const real* b, real* x, co
     if ((e = check_hip(hipEventRecord(hs->ev[0], s)))) return e;
     if ((e = check_hip(hipEventSynchronize(hs->ev[0])))) return e;
     const KFlags* f = hs->slot;
-    if (f[K].done) {   // aborted: x is as it was
+    if (f[K].done) {   // aborted: x is as it was; the setting turns itself off
       g_persistent = 0;
       ++g_xcd_fallbacks;
       return WCFD_OK;
@@ -1985,7 +1945,7 @@ static int cg_dispatch(const M& A, const KLaunch& L, int variant, const real* b,
     wcfd_solve_info* out = (info && sweeps > 1) ? tmp.data() : info;
     bool ran = false;
     int e = WCFD_OK;
-    if (variant == WCFD_CG_SINGLE_REDUCTION && g_persistent.load())
+    if (variant == WCFD_CG_SINGLE_REDUCTION && g_persistent.load() > 0)
       e = cgs_xcd_impl(A, L, b, x, tl, r64, maxiter, check_every, w, out, s, ran);
     if (e) return e;
     if (!ran)
@@ -2165,9 +2125,9 @@ int wcfd_set_krylov_inline_check(int on) {
 
 int wcfd_get_krylov_inline_check(void) { return g_inline_check.load(); }
 
-int wcfd_set_krylov_persistent(int on) {
-  if (on != 0 && on != 1) return WCFD_EINVAL;
-  g_persistent = on;
+int wcfd_set_krylov_persistent(int max_rows) {
+  if (max_rows < 0) return WCFD_EINVAL;
+  g_persistent = max_rows;
   return WCFD_OK;
 }
 

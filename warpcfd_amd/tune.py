@@ -46,17 +46,22 @@ def _laplacian_ell(dm):
     return ell.contiguous()
 
 
-def install(grid, block, fold=FOLDS[0], lib=None):
+def install(grid, block, fold=FOLDS[0], lib=None, persistent=None):
+    """Launch shape, inline-fold limit and, unless None, the persistent-CG
+    row limit (0: off)."""
     lib = lib if lib is not None else _lib.load()
     _lib.check(lib.wcfd_set_krylov_launch(grid, block), 'set_krylov_launch')
     _lib.check(lib.wcfd_set_krylov_fold(fold), 'set_krylov_fold')
+    if persistent is not None:
+        _lib.check(lib.wcfd_set_krylov_persistent(persistent), 'set_krylov_persistent')
 
 
-def time_solvers(dm, grid, block, iters=60, reps=2, cg_variant=0, fold=FOLDS[0]):
+def time_solvers(dm, grid, block, iters=60, reps=2, cg_variant=0, fold=FOLDS[0], persistent=0):
     """Milliseconds of `iters` CG iterations and `iters` 3-vector BiCGSTAB
-    iterations at launch shape (grid, block) and inline-fold limit ``fold``."""
+    iterations at launch shape (grid, block), inline-fold limit ``fold`` and
+    persistent-CG row limit ``persistent``."""
     lib = dm.lib
-    install(grid, block, fold, lib)
+    install(grid, block, fold, lib, persistent)
     C = dm.n_cells
     dev = dm.device
     A = _laplacian_ell(dm)
@@ -111,24 +116,63 @@ def krylov_candidates(C, grids=GRIDS, blocks=BLOCKS):
     return out
 
 
+# Largest case for which the persistent CG is tried: one XCD's registers hold
+# about 40 000 - 49 000 rows at 256 threads, and above about 8 000 rows the
+# per-iteration kernels measured faster (one XCD gathers at an eighth of the
+# chip's rate).
+PERSISTENT_MAX_ROWS = 16384
+
+
+def persistent_candidates(C, blocks=BLOCKS):
+    """(grid, block, fold, persistent rows) shapes that run the persistent
+    single-reduction CG for a case of C rows: one row per thread, every
+    block size, when the case is small enough to be worth trying."""
+    if C > PERSISTENT_MAX_ROWS:
+        return []
+    return [(0, blk, FOLDS[0], C) for blk in blocks]
+
+
 def autotune_krylov(dm, grids=GRIDS, blocks=BLOCKS, iters=60, cg_weight=1.0, bicg_weight=0.1,
-                    cg_variant=0):
-    """Pick and install the fastest (grid, block, fold) for mesh ``dm``
-    (a DeviceMesh).  The weights reflect a SIMPLE iteration, where the
-    pressure CG runs far more iterations than the momentum BiCGSTAB."""
+                    cg_variant=0, refine=6, refine_iters=300, refine_reps=3):
+    """Pick and install the fastest (grid, block, fold, persistent) for mesh
+    ``dm`` (a DeviceMesh).  The weights reflect a SIMPLE iteration, where the
+    pressure CG runs far more iterations than the momentum BiCGSTAB.  Every
+    candidate is timed on ``iters`` iterations; the ``refine`` fastest are
+    then timed again on ``refine_iters`` iterations, ``refine_reps`` rounds
+    in turn, and the smallest median score wins, so that the pick does not
+    follow the timing noise of one short solve.  The persistent CG is a
+    candidate only with the single-reduction form."""
     key = (dm.n_cells, cg_variant, dm.lib.wcfd_real_bytes())
     if key in _cache:
         c = _cache[key]
-        install(c['grid'], c['block'], c['fold'], dm.lib)
+        install(c['grid'], c['block'], c['fold'], dm.lib, c['persistent'])
         return c
-    results = []
-    for grid, blk, fold in krylov_candidates(dm.n_cells, grids, blocks):
-        cg_ms, bi_ms = time_solvers(dm, grid, blk, iters, cg_variant=cg_variant, fold=fold)
-        results.append({'grid': grid, 'block': blk, 'fold': fold, 'cg_ms': cg_ms,
-                        'bicg_ms': bi_ms, 'score': cg_weight * cg_ms + bicg_weight * bi_ms})
-    best = min(results, key=lambda r: r['score'])
-    install(best['grid'], best['block'], best['fold'], dm.lib)
-    out = dict(best, iters=iters, candidates=results)
+    cands = [c + (0,) for c in krylov_candidates(dm.n_cells, grids, blocks)]
+    if cg_variant == _lib.CG_SINGLE_REDUCTION:
+        cands += persistent_candidates(dm.n_cells, blocks)
+
+    def score(grid, blk, fold, pers, n_it, reps):
+        cg_ms, bi_ms = time_solvers(dm, grid, blk, n_it, reps=reps, cg_variant=cg_variant,
+                                    fold=fold, persistent=pers)
+        return {'grid': grid, 'block': blk, 'fold': fold, 'persistent': pers, 'cg_ms': cg_ms,
+                'bicg_ms': bi_ms, 'score': cg_weight * cg_ms + bicg_weight * bi_ms}
+
+    results = [score(*c, iters, 2) for c in cands]
+    top = sorted(results, key=lambda r: r['score'])[:max(1, refine)]
+    rounds = {i: [] for i in range(len(top))}
+    for _ in range(refine_reps):
+        for i, r in enumerate(top):
+            rounds[i].append(score(r['grid'], r['block'], r['fold'], r['persistent'],
+                                   refine_iters, 1))
+    refined = []
+    for i, r in enumerate(top):
+        med = sorted(rounds[i], key=lambda x: x['score'])[len(rounds[i]) // 2]
+        refined.append(dict(med, cg_ms=med['cg_ms'] * iters / refine_iters,
+                            bicg_ms=med['bicg_ms'] * iters / refine_iters,
+                            score=med['score'] * iters / refine_iters))
+    best = min(refined, key=lambda r: r['score'])
+    install(best['grid'], best['block'], best['fold'], dm.lib, best['persistent'])
+    out = dict(best, iters=iters, candidates=results, refined=refined)
     _cache[key] = out
     return out
 
