@@ -1,5 +1,6 @@
-"""A case read from a mesh file runs like the same case built from arrays,
-and its exported fields read back unchanged (GPU)."""
+"""A case read from a mesh file runs like the same case built from arrays and
+like the oracle on the file's points and cells, and its exported fields read
+back unchanged (GPU)."""
 import numpy as np
 import pytest
 
@@ -42,6 +43,17 @@ def test_case_from_file_runs_bitwise_and_exports(gpu, tmp_path, fmt):
     b = _solve(d)
     va, vb = a.cell_values.cpu().numpy(), b.cell_values.cpu().numpy()
     assert np.array_equal(va, vb)
+    # the oracle (the reference's algorithm, float64 numpy) on what the file holds
+    from oracle.simple import OracleSolver
+    from cases import oracle_case, rel_l2
+    of = oracle_case(d.points, d.cells, 'cavity')
+    os_ = OracleSolver(of, 0.7, 0.3, nu=0.01)
+    for _ in range(15):
+        os_.outer_iteration()
+    err_u = rel_l2(vb[:, :3], of.cell_values[:, :3])
+    err_p = rel_l2(vb[:, 3], of.cell_values[:, 3])
+    print(f'{fmt}: 15 SIMPLE iterations, velocity rel-L2 vs oracle {err_u:.2e}, p {err_p:.2e}')
+    assert err_u <= 1e-4 and err_p <= 1e-4
     out = str(tmp_path / 'result.vtu')
     export_vtu(b, out)
     r = read_vtu(out)

@@ -37,11 +37,11 @@ def channel_bcs(b):
         b.no_slip_wall(w)
 
 
-def cavity_model(n, re=100.0, device='cuda'):
+def cavity_model(n, re=100.0, device='cuda', float_dtype=np.float32):
     nodes, cells = cavity_mesh(int(n))
     m = Mesh(nodes, cells)
     define_boundary_walls(m)
-    model = FVM(m, device=device)
+    model = FVM(m, device=device, float_dtype=float_dtype)
     model.material.create_incompressible_newtonian_fluid(viscosity=1.0 / float(re), density=1.0)
     model.set_reference_pressure(0, 0.0)
     cavity_bcs(model.boundary)
@@ -49,12 +49,12 @@ def cavity_model(n, re=100.0, device='cuda'):
 
 
 def channel_model(counts=(200, 40, 40), lengths=(10.0, 2.0, 2.0), re=200.0, jitter=0.1, seed=0,
-                  device='cuda'):
+                  device='cuda', float_dtype=np.float32):
     """Re on the channel height: nu = U H / Re with U = 1, H = lengths[1]."""
     nodes, cells = channel_mesh(*counts, lengths=lengths, jitter=jitter, seed=seed)
     m = Mesh(nodes, cells)
     define_boundary_walls(m)
-    model = FVM(m, device=device)
+    model = FVM(m, device=device, float_dtype=float_dtype)
     model.material.create_incompressible_newtonian_fluid(viscosity=lengths[1] / float(re),
                                                          density=1.0)
     channel_bcs(model.boundary)
