@@ -452,12 +452,14 @@ SERVE_SAMPLE_MIX = {'cavity_n': (32, 96), 'channel_cells': (50_000, 150_000)}
 
 
 def _long_run_errors(sample, pol, chunk, block):
-    """Velocity error of served results against long runs of the same cases:
-    every sampled case runs again, all of them together in one arena, for a
-    fixed 4 x its served iterations (at least served + 3000, no stop), and
-    the served u, v, w are compared with that longer solve (rel-L2)."""
+    """Velocity error of served results against long runs of the same cases
+    in the double-field build: every sampled case runs again, all of them
+    together in one float64 arena (libwcfd64.so), for a fixed 4 x its served
+    iterations (at least served + 3000, no stop), and the served float32 u,
+    v, w are compared with that longer float64 solve (rel-L2)."""
     import copy
     import numpy as np
+    import torch
     from warpcfd_amd.batch import CaseArena
     specs = []
     for r in sample:
@@ -465,7 +467,7 @@ def _long_run_errors(sample, pol, chunk, block):
         sp.stop, sp.stop_u, sp.floor_u = 0.0, 0.0, 0.0     # never stop: run to max_outer
         sp.max_outer = max(4 * r.iterations, r.iterations + 3000)
         specs.append(sp)
-    arena = CaseArena.for_cases(specs, chunk=chunk, block=block, **pol)
+    arena = CaseArena.for_cases(specs, chunk=chunk, block=block, dtype=torch.float64, **pol)
     arena.run(max_steps=max(sp.max_outer for sp in specs) + 1)
     out = []
     for k, r in enumerate(sample):
@@ -484,7 +486,7 @@ def run_serve(args, n_requests, mix=SERVE_MIX, sequential=True, error_sample=0, 
     stop; ``--serve-continuity-only`` drops the second test), with continuous
     batching on an arena sized by an HBM budget and, if ``sequential``, one
     at a time through the same kernels; ``error_sample`` served results are
-    checked against long runs.  Returns the result dict."""
+    checked against float64 long runs.  Returns the result dict."""
     import math
     import numpy as np
     import torch
@@ -542,7 +544,7 @@ def run_serve(args, n_requests, mix=SERVE_MIX, sequential=True, error_sample=0, 
             rng = np.random.default_rng(args.seed + 1)
             pick = sorted(rng.choice(len(ok), size=min(error_sample, len(ok)), replace=False))
             errors = _long_run_errors([ok[i] for i in pick], pol, args.chunk, args.block)
-            print(f'# velocity error vs long runs{label}: {errors}', file=sys.stderr, flush=True)
+            print(f'# velocity error vs float64 long runs{label}: {errors}', file=sys.stderr, flush=True)
     finally:
         prep.close()
     stop_desc = f'continuity <= {stop:g}' + ('' if math.isinf(stop_u) else
@@ -562,7 +564,7 @@ def run_serve(args, n_requests, mix=SERVE_MIX, sequential=True, error_sample=0, 
         'one_at_a_time': seq,
         'speedup_vs_one_at_a_time': (round(served['solves_per_s'] / seq['solves_per_s'], 2)
                                      if seq else None),
-        'velocity_error_vs_long_runs': errors,
+        'velocity_error_vs_float64_long_runs': errors,
         'latency_note': 'time to converged = arrival to convergence, including host '
                         'preparation (mesh, layout; worker processes) and queueing; '
                         'resident time = admission to convergence',
@@ -915,7 +917,7 @@ def main():
                     help='serve: the configured mix (cavities up to 512^2, channels up to 500k '
                          'cells) or the reduced mix of the default line\'s served sample')
     ap.add_argument('--error-sample', type=int, default=6,
-                    help='serve: served results checked against long runs')
+                    help="serve: served results checked against float64 long runs")
     ap.add_argument('--rate', type=float, default=20.0, help='serve: arrivals per second')
     ap.add_argument('--hbm-gb', type=float, default=32.0, help='serve: arena HBM budget')
     ap.add_argument('--slots', type=int, default=64, help='serve: case slots')
