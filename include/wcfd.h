@@ -480,6 +480,11 @@ int wcfd_get_krylov_persistent(void);
 /* Solves run by the persistent kernel and solves that fell back after an
  * abort, since the library was loaded. [host] outputs */
 int wcfd_krylov_persistent_stats(int64_t* solves, int64_t* fallbacks);
+/* Test hook for the abort path (no counterpart in the reference): launch the
+ * persistent kernel with `slots` fewer workgroup slots per XCD than its plan
+ * needs, so that every workgroup finds the plan does not fit, sets the abort
+ * flag and returns at entry; the solve then falls back.  0 (the default): off. */
+int wcfd_debug_krylov_persistent_shrink(int slots);
 
 /* wcfd_cg / wcfd_bicgstab on the ELL matrix `values` [7][C] of mesh m. */
 int wcfd_cg_ell(const wcfd_mesh* m, const wcfd_real* values, int variant, const wcfd_real* b, wcfd_real* x,

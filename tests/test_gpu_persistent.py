@@ -231,3 +231,28 @@ def test_persistent_row_threshold(restore):
         s1, f1 = _stats(lib)
         assert f1 == f0 and (s1 > s0) == ran, (limit, ran)
         assert np.array_equal(xt.cpu().numpy(), ref[0]) and si.iterations == ref[1][0]
+
+
+def test_persistent_abort_falls_back(restore):
+    """The abort path, forced from the host: the persistent launch gets fewer
+    slots per XCD than its plan needs, so every workgroup aborts at entry
+    (no barrier is waited on).  x is left as it was, the solve reruns on the
+    per-iteration kernels -- bitwise their result -- the fallback counter
+    rises and the setting turns itself off."""
+    lib = restore
+    model, A, ell, b = _system((16, 12, 10), 46)
+    assert lib.wcfd_set_krylov_launch(0, 64) == 0
+    ref = _cg(lib, model, A, ell, b, 0)
+    s0, f0 = _stats(lib)
+    assert lib.wcfd_debug_krylov_persistent_shrink(1 << 20) == 0
+    try:
+        got = _cg(lib, model, A, ell, b, 1)
+    finally:
+        assert lib.wcfd_debug_krylov_persistent_shrink(0) == 0
+    s1, f1 = _stats(lib)
+    assert (s1, f1) == (s0, f0 + 1)
+    assert lib.wcfd_get_krylov_persistent() == 0
+    assert got[1] == ref[1] and np.array_equal(got[0], ref[0])
+    # the next solve with the setting on again runs the persistent kernel
+    again = _cg(lib, model, A, ell, b, 1)
+    assert _stats(lib)[0] == s1 + 1 and np.array_equal(again[0], ref[0])

@@ -1591,6 +1591,7 @@ std::atomic<int> g_krylov_block{kBlock};
 std::atomic<int> g_krylov_fold{kMaxInlineFold};
 std::atomic<int> g_persistent{0};   // most rows per case of a persistent solve (0: off)
 std::atomic<long long> g_xcd_solves{0}, g_xcd_fallbacks{0};
+std::atomic<int> g_xcd_shrink{0};   // test hook: slots per XCD taken from the plan's launch
 }  // namespace
 
 KrylovLaunch krylov_launch() { return {g_krylov_grid.load(), g_krylov_block.load()}; }
@@ -1821,6 +1822,12 @@ static int cgs_xcd_impl(const M& A, const KLaunch& L, const real* b, real* x, co
       return WCFD_OK;   // more rows than one XCD's registers hold: per-iteration kernels
     pl.check_every = check_every > 0 ? check_every : 1;
     pl.wait_ticks = 10000000;   // 100 ms at 100 MHz
+    if (const int cut = g_xcd_shrink.load()) {
+      // the plan's own bound is unchanged; the launch has fewer slots, so the
+      // plan does not fit and every workgroup aborts at entry
+      pl.per_xcc = pl.per_xcc > cut ? pl.per_xcc - cut : 0;
+      if (pl.per_xcc == 0) pl.per_xcc = 1;
+    }
     HostSync* hs = host_sync(K + 1);
     if (!hs) return WCFD_EHIP;
     int e = prepare(w.st, sg, tl, s);
@@ -2132,6 +2139,12 @@ int wcfd_set_krylov_persistent(int max_rows) {
 }
 
 int wcfd_get_krylov_persistent(void) { return g_persistent.load(); }
+
+int wcfd_debug_krylov_persistent_shrink(int slots) {
+  if (slots < 0) return WCFD_EINVAL;
+  g_xcd_shrink = slots;
+  return WCFD_OK;
+}
 
 int wcfd_krylov_persistent_stats(int64_t* solves, int64_t* fallbacks) {
   if (!solves || !fallbacks) return WCFD_EINVAL;
