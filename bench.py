@@ -689,7 +689,8 @@ def bench_cylinder(args):
         from warpcfd_amd.tune import autotune_krylov
         # the pressure CG runs about 40 iterations per momentum BiCGSTAB
         # iteration in these steps: weigh the two timings accordingly
-        tt = autotune_krylov(dm, cg_variant=sv.cg_variant, bicg_weight=0.025)
+        tt = autotune_krylov(dm, cg_variant=sv.cg_variant, bicg_weight=0.025,
+                             try_persistent=False)
         tuned = {'grid': tt['grid'], 'block': tt['block'], 'fold': tt['fold'],
                  'persistent_rows': tt['persistent']}
     C, F, Fi, Fb, nnz = dm.n_cells, dm.n_faces, dm.n_internal, dm.n_boundary, dm.nnz
@@ -1012,7 +1013,8 @@ def main():
         # a SIMPLE iteration of the channel runs about 420 pressure CG
         # iterations and 10 momentum BiCGSTAB iterations: weigh the two
         # timings accordingly
-        t = autotune_krylov(dm, cg_variant=solver.cg_variant, bicg_weight=0.025)
+        t = autotune_krylov(dm, cg_variant=solver.cg_variant, bicg_weight=0.025,
+                            try_persistent=False)
         tuned = {'grid': t['grid'], 'block': t['block'], 'fold': t['fold'],
                  'persistent_rows': t['persistent'],
                  'cg_ms_per_60_it': round(t['cg_ms'], 3),
@@ -1093,6 +1095,23 @@ def main():
                                            'momentum': float(np.mean([h['u_iters'] for h in hist])),
                                            'pressure': float(np.mean([h['p_iters'] for h in hist]))}})
 
+    persistent = None
+    if not args.no_extras and args.cg == 'single':
+        def make_solver():
+            m2 = FVM(m, float_dtype=np.float32)
+            m2.material.create_incompressible_newtonian_fluid(viscosity=0.01, density=1.0)
+            if wl['kind'] == 'cavity':
+                m2.set_reference_pressure(0, 0.0)
+            apply_bcs(m2.boundary, wl['kind'])
+            m2.finalize(layout=model.layout)
+            s2 = IncompressibleSolver(m2, 0.7, 0.3, implicit_outflow=(wl['kind'] == 'channel'))
+            s2.cg_variant = solver.cg_variant
+            s2.tol = solver.tol
+            return m2, s2
+        try:
+            persistent = persistent_cg_line(args, make_solver, C)
+        except Exception as exc:             # the main line is printed either way
+            persistent = {'error': repr(exc)}
     extras = {}
     if not args.no_extras and args.workload == 'channel' and dims == (200, 40, 40) \
             and args.cell_type == 'hexahedron':
@@ -1121,12 +1140,67 @@ def main():
                                   'pressure': float(np.mean([h['p_iters'] for h in hist]))},
         'error_vs_fp64_build': fp64,
         'krylov_launch': tuned,
+        'persistent_cg': persistent,
         'host_preprocessing_s': round(prep_s, 1),
         'kernels': kernels,
         'kernels_note': KERNELS_NOTE,
     }
     out.update(extras)
     print(json.dumps(out))
+
+
+def persistent_cg_line(args, make_solver, C):
+    """The same steps with the persistent pressure CG (one cooperative launch
+    per solve, the case spread over every XCD, bitwise the per-iteration
+    kernels at the installed launch shape), from rest on a fresh model: SIMPLE
+    iter/s over the same warm-up and timed window, and the launch's own
+    roofline on the bytes it must move.  Its threads keep the matrix, the
+    Jacobi values at the columns, x and p in registers, so a launch reads
+    those once (52 B per row: 7 coefficients, 6 int16 offsets, inv, x in and
+    out) and an iteration moves w, s and r in and out (24 B per row)."""
+    import torch
+    from warpcfd_amd import _lib
+    lib = _lib.load()
+    _lib.check(lib.wcfd_set_krylov_persistent(int(C)), 'set_krylov_persistent')
+    try:
+        model, sv = make_solver()
+        for _ in range(args.warmup):
+            sv.outer_iteration()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            sv.outer_iteration()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        n0 = len(sv.history)
+        with _lib.Profiler(capacity=200000) as prof:
+            for _ in range(args.steps):
+                sv.outer_iteration()
+            torch.cuda.synchronize()
+        ran = lib.wcfd_get_krylov_persistent() > 0
+    finally:
+        _lib.check(lib.wcfd_set_krylov_persistent(0), 'set_krylov_persistent')
+    its = sum(h['p_iters'] for h in sv.history[n0:])
+    ms, launches = prof.result.get('cg_xcd', (0.0, 0))
+    out = {'value': round(args.steps / dt, 3), 'unit': 'iter/s',
+           'ms_per_step': round(1e3 * dt / args.steps, 3), 'persistent_kernel_ran': ran,
+           'note': 'same workload, launch shape and Krylov settings as the main line, from rest; '
+                   'every pressure solve one cooperative launch over every XCD, bitwise the '
+                   'per-iteration kernels (tests/test_gpu_persistent.py)'}
+    if launches and its:
+        by = C * (52.0 * launches + 24.0 * its) / launches
+        gbs = by / (1e-3 * ms / launches) / 1e9
+        out.update({'launches': launches, 'cg_iterations': its,
+                    'us_per_launch': round(1e3 * ms / launches, 2),
+                    'us_per_cg_iteration': round(1e3 * ms / its, 3),
+                    'roofline': {'bound': 'hbm', 'kernel': 'cg_xcd', 'achieved': round(gbs, 1),
+                                 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                                 'frac': round(gbs / HBM_PEAK_GBS, 4),
+                                 'bytes_per_launch': round(by),
+                                 'note': 'the bytes the launch must move (52 B per row once, '
+                                         '24 B per row per iteration); the barrier between '
+                                         'iterations, not HBM, bounds it'}})
+    return out
 
 
 def run_channel_to_stop(stop=1e-6, max_iters=600, float32_tail=200):

@@ -7,8 +7,9 @@ at a barrier in memory once per iteration.  Its workgroups take the launch
 path's virtual workgroups (the same rows, threads, partials and fold order),
 so the iterations run, the solution and the reported residual must be
 bitwise those of the per-iteration kernels at the same launch shape.
-Covered: workgroups of 64 to 512 threads, one or two virtual workgroups
-per workgroup, a solve too large for one XCD (not eligible), prefolded partials (which the persistent kernel folds itself),
+Covered: single cases on one XCD and spread over every XCD (grid mode, a
+barrier across XCDs), workgroups of 64 to 512 threads, one to three virtual
+workgroups per workgroup, the full 320k-cell channel, prefolded partials (which the persistent kernel folds itself),
 convergence, running out of iterations with and without a multiple of
 check_every, maxiter 0, a zero right side, the CSR entry point (not eligible:
 the per-iteration kernels), batched solves in an arena of several cases, and
@@ -97,14 +98,15 @@ def _same(lib, model, A, ell, b, ran=True, **kw):
     return xb, ib
 
 
-# 9 600 rows: one virtual workgroup per workgroup; 46 080 rows at 256
-# threads: more virtual workgroups than one XCD has slots for one each, so
-# (on MI355X, 104 and 144 VGPRs) two per workgroup; 81 920 rows: more than one
-# XCD's registers hold, not eligible
+# Single cases above 8 192 rows spread over every XCD (grid mode): 9 600,
+# 46 080 and 81 920 rows at 64 to 512 threads, one or more virtual
+# workgroups per workgroup; 1 920 rows (the limit tests below) sit on one XCD.
 @pytest.mark.parametrize('shape,grid,block,ran', [
     ((30, 20, 16), 0, 64, True), ((30, 20, 16), 0, 256, True), ((30, 20, 16), 0, 512, True),
-    ((30, 20, 16), 40, 512, True), ((40, 36, 32), 0, 256, None),
-    ((64, 40, 32), 0, 256, False)])
+    ((30, 20, 16), 40, 512, True), ((40, 36, 32), 0, 256, True),
+    ((64, 40, 32), 0, 256, True), ((64, 40, 32), 0, 512, True),
+    # grid-stride shapes (chunk > block): 2, 3 and 4 rows per thread
+    ((30, 20, 16), 38, 128, True), ((64, 40, 32), 64, 512, True), ((30, 20, 16), 24, 128, True)])
 @pytest.mark.parametrize('fold', [2048, 0])
 def test_persistent_bitwise_equals_launch_path(restore, shape, grid, block, ran, fold):
     lib = restore
@@ -256,3 +258,33 @@ def test_persistent_abort_falls_back(restore):
     # the next solve with the setting on again runs the persistent kernel
     again = _cg(lib, model, A, ell, b, 1)
     assert _stats(lib)[0] == s1 + 1 and np.array_equal(again[0], ref[0])
+
+
+@pytest.mark.parametrize('grid,block', [(0, 512), (0, 256), (256, 512)])
+def test_persistent_channel_320k_bitwise(restore, grid, block):
+    """Three SIMPLE iterations of the full 320k-cell channel (reference Krylov
+    settings), the pressure CG in grid mode (0 x 512: three virtual
+    workgroups of one row per thread per workgroup; 256 x 512, the tuner's
+    shape: one virtual workgroup of three rows per thread): bitwise the same
+    fields and Krylov counts as the per-iteration kernels."""
+    from warpcfd_amd import _lib
+    from warpcfd_amd.workloads import channel_model
+    from warpcfd_amd.FV.implicit_Solvers import IncompressibleSolver
+    lib = restore
+    out = []
+    for persistent in (0, 1):
+        assert lib.wcfd_set_krylov_launch(grid, block) == 0
+        assert lib.wcfd_set_krylov_persistent(ON if persistent else 0) == 0
+        model = channel_model()
+        model.finalize()
+        sv = IncompressibleSolver(model, 0.7, 0.3, implicit_outflow=True)
+        sv.cg_variant = _lib.CG_SINGLE_REDUCTION
+        s0, f0 = _stats(lib)
+        hist = [sv.outer_iteration() for _ in range(3)]
+        s1, f1 = _stats(lib)
+        assert f1 == f0 and (s1 > s0) == bool(persistent)
+        out.append((model.cell_values.cpu().numpy(), [(h['p_iters'], h['u_iters']) for h in hist]))
+        del sv, model
+    print(f'{grid} x {block}: Krylov counts {out[1][1]}')
+    assert out[0][1] == out[1][1]
+    assert np.array_equal(out[0][0], out[1][0])

@@ -699,7 +699,10 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_iter2(M A, Seg sg, int 
 constexpr int kSyncPad = 32;     // ints between sync words (128 B)
 constexpr int kSyncXcc = 0;      // [0, 8): slots taken per XCD
 constexpr int kSyncAbort = 8;    // abort flag
-constexpr int kSyncCase = 9;     // [9, 9 + K): barrier counter of case k
+constexpr int kSyncLbar = 9;     // [9, 17): arrivals per XCD (grid mode)
+constexpr int kSyncLrel = 17;    // [17, 25): releases per XCD (grid mode)
+constexpr int kSyncGbar = 25;    // arrivals of the XCDs' leaders (grid mode)
+constexpr int kSyncCase = 26;    // [26, 26 + K): barrier counter of case k
 constexpr unsigned kHwregXccId = 20u | (0u << 6) | ((4u - 1u) << 11);   // XCC_ID[3:0]
 
 inline size_t xcd_sync_ints(int n_cases) { return (size_t)(kSyncCase + n_cases) * kSyncPad; }
@@ -719,6 +722,35 @@ template <class T>
 __device__ __forceinline__ T ld_coh_off(const T* base, uint32_t byte_off) {
   return ld_coh(reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
                                            scaled<T>(byte_off)));
+}
+
+template <class T>
+__device__ __forceinline__ void st_coh_off(T* base, uint32_t byte_off, T v) {
+  __hip_atomic_store(reinterpret_cast<T*>(reinterpret_cast<char*>(base) + scaled<T>(byte_off)), v,
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// store_partials whose sums are stored at agent scope (COH) or plainly
+template <int ND, bool COH>
+__device__ inline void store_partials_sc(double (&v)[ND], double* part, int G, int vb) {
+  __shared__ double lds[ND * kMaxWaves];
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  wave_sum_n<ND>(v);
+#pragma unroll
+  for (int i = 0; i < ND; ++i)
+    if (lane == 0) lds[i * kMaxWaves + wid] = v[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+      double t = 0.0;
+      for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += lds[i * kMaxWaves + w];
+      if constexpr (COH)
+        __hip_atomic_store(part + i * G + vb, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      else
+        part[i * G + vb] = t;
+    }
+  }
 }
 
 // one of a ping-pong pair
@@ -835,6 +867,38 @@ __device__ inline bool xcd_barrier(int* cnt, int target, int* sync, long long wa
   return true;
 }
 
+// The barrier of a case spread over every XCD (thread 0): each workgroup
+// arrives at its XCD's counter; the XCD's slot-0 workgroup waits for its
+// XCD, arrives at the grid counter, waits for the other XCDs' leaders and
+// releases its XCD; the others wait for that release.  round = n + 1 for
+// the n-th barrier.  false on abort or a wait past wait_ticks.
+__device__ inline bool xcd_wait_ge(const int* p, int target, int* sync, long long wait_ticks) {
+  const long long t0 = (long long)__builtin_amdgcn_s_memrealtime();
+  for (int spins = 0; ld_coh(p) < target; ++spins) {
+    if (ld_coh(sync + kSyncAbort * kSyncPad)) return false;
+    if ((long long)__builtin_amdgcn_s_memrealtime() - t0 > wait_ticks || spins > (1 << 20)) {
+      xcd_abort(sync);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return true;
+}
+
+__device__ inline bool grid_barrier(int* sync, int xcc, int slot, int per_xcc, int round,
+                                    long long wait_ticks) {
+  int* lbar = sync + (kSyncLbar + xcc) * kSyncPad;
+  int* lrel = sync + (kSyncLrel + xcc) * kSyncPad;
+  __hip_atomic_fetch_add(lbar, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (slot != 0) return xcd_wait_ge(lrel, round, sync, wait_ticks);
+  if (!xcd_wait_ge(lbar, per_xcc * round, sync, wait_ticks)) return false;
+  int* gbar = sync + kSyncGbar * kSyncPad;
+  __hip_atomic_fetch_add(gbar, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (!xcd_wait_ge(gbar, (int)kNumXcd * round, sync, wait_ticks)) return false;
+  __hip_atomic_fetch_add(lrel, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return true;
+}
+
 // The persistent kernel: every workgroup takes RV (1 or 2) of its case's
 // virtual workgroups, one row per thread each, and keeps the rows'
 // coefficients, column offsets, Jacobi inverses at the columns, x and p in
@@ -842,21 +906,44 @@ __device__ inline bool xcd_barrier(int* cnt, int target, int* sync, long long wa
 // s and r (sc1) and the case's partials, and writes w, s, r and the
 // partials.  The fold's loads are issued before the gathers (as EarlyFold
 // does in k_cgs_iter), so both are in flight together.
-template <class M, int RV>
+// GRID: one case (the single-case solve) spread over every XCD: XCD x takes
+// the x-th eighth of the case's virtual workgroups as a contiguous band and
+// its slots split the band; the case's workgroups meet at grid_barrier, and
+// every value another workgroup reads is stored at agent scope (sc1, which
+// other XCDs' sc1 loads see; measured on MI355X).  Otherwise each case sits
+// on one XCD, as planned by xcd_part, and meets at its own counter.
+// RR: rows per thread of a virtual workgroup (its chunk <= RR x block: rows
+// tid, tid + block, ... as the per-iteration kernels stride), RV / RR virtual
+// workgroups per workgroup; RV rows per thread in all, cached in registers.
+template <class M, int RV, bool GRID, int RR = 1>
 __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_xcd(
     M A, Seg sg, XcdPlan pl, const real* __restrict__ x_in, real* x_out, real* r0, real* r1,
     real* w0, real* w1, real* s0, real* s1, const real* __restrict__ inv, double* P0,
     double* P1, KState* st, int maxiter, int* sync) {
   constexpr int KR = M::kRow;
-  __shared__ int sh[4];
+  __shared__ int sh[6];
   __shared__ int s_go;
   if (threadIdx.x == 0) {
     const int xcc = (int)(__builtin_amdgcn_s_getreg(kHwregXccId) & (kNumXcd - 1));
     const int slot = __hip_atomic_fetch_add(sync + (kSyncXcc + xcc) * kSyncPad, 1,
                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     XcdPart part;
-    const bool ok = slot < pl.per_xcc && xcd_part(sg, st, pl, xcc, slot, part) &&
-                    (part.k < 0 || part.v_hi - part.v_lo <= RV);
+    bool ok = slot < pl.per_xcc;
+    if (GRID) {
+      // one case: its band on this XCD, split over the XCD's slots
+      part.k = st[0].f.done ? -1 : 0;
+      int b0g, nbg;
+      case_blocks(sg, 0, b0g, nbg);
+      const int lo = b0g + (int)((int64_t)xcc * nbg / kNumXcd);
+      const int nbx = b0g + (int)((int64_t)(xcc + 1) * nbg / kNumXcd) - lo;
+      part.v_lo = lo + (int)((int64_t)slot * nbx / pl.per_xcc);
+      part.v_hi = lo + (int)((int64_t)(slot + 1) * nbx / pl.per_xcc);
+      part.members = (int)kNumXcd * pl.per_xcc;
+      ok = ok && sg.n_cases == 1;
+    } else {
+      ok = ok && xcd_part(sg, st, pl, xcc, slot, part);
+    }
+    ok = ok && (part.k < 0 || part.v_hi - part.v_lo <= RV / RR);
     if (!ok) {
       xcd_abort(sync);
       part.k = -1;
@@ -865,6 +952,8 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_xcd(
     sh[1] = part.v_lo;
     sh[2] = part.v_hi;
     sh[3] = part.members;
+    sh[4] = xcc;
+    sh[5] = slot;
   }
   __syncthreads();
   // Values broadcast through LDS are the same in every lane; readfirstlane
@@ -874,6 +963,8 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_xcd(
   const int v_lo = __builtin_amdgcn_readfirstlane(sh[1]);
   const int nv = __builtin_amdgcn_readfirstlane(sh[2]) - v_lo;
   const int members = __builtin_amdgcn_readfirstlane(sh[3]);
+  const int my_xcc = __builtin_amdgcn_readfirstlane(sh[4]);
+  const int my_slot = __builtin_amdgcn_readfirstlane(sh[5]);
   KState* S = st + k;
   int b0, nb;
   case_blocks(sg, k, b0, nb);
@@ -881,7 +972,9 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_xcd(
   const int ce = pl.check_every > 0 ? pl.check_every : 1;
   const double thr2 = S->f.thr2;
   double rz_prev = S->rz[0], alpha_prev = S->alpha[0];
-  const bool leader = v_lo == b0 && threadIdx.x == 0;   // records the case's flags
+  // records the case's flags: the workgroup holding the case's first virtual
+  // workgroup (in grid mode a workgroup may hold none: XCD 0, slot 0)
+  const bool leader = threadIdx.x == 0 && (GRID ? my_xcc == 0 && my_slot == 0 : v_lo == b0);
   int* cnt = sync + (kSyncCase + k) * kSyncPad;
   // the rows, cached
   const int e = sg.cell_end ? sg.cell_end[k] : (sg.cell_off ? sg.cell_off[k + 1] : sg.n);
@@ -892,10 +985,11 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_xcd(
   bool has[RV];
 #pragma unroll
   for (int q = 0; q < RV; ++q) {
-    const int begin = c0 + (v_lo + q - b0) * sg.chunk;
+    const int vq = q / RR, rq = q % RR;   // virtual workgroup, row of the thread in it
+    const int begin = c0 + (v_lo + vq - b0) * sg.chunk;
     const int end = begin + sg.chunk < e ? begin + sg.chunk : e;
-    const int i = begin + (int)threadIdx.x;
-    has[q] = q < nv && i < end;
+    const int i = begin + (int)threadIdx.x + rq * (int)blockDim.x;
+    has[q] = vq < nv && i < end;
     xr[q] = pr[q] = 0.f;
     if (has[q]) {
       A.load_off(i, a[q], o[q]);
@@ -958,10 +1052,10 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_xcd(
       beta = (real)beta_d;
     }
     if (__builtin_amdgcn_readfirstlane(stop)) break;
+    double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int q = 0; q < RV; ++q) {
-      if (q >= nv) continue;   // uniform across the workgroup
-      double acc[3] = {0.0, 0.0, 0.0};
+      if (q / RR >= nv) continue;   // uniform across the workgroup
       if (has[q]) {
         real wn, pi, si, rn;
         if (it == 0)
@@ -970,20 +1064,31 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_xcd(
           cg_row<KR, false>(a[q], wc[q], sc[q], rc[q], ic[q], pr[q], alpha, beta, wn, pi, si, rn);
         xr[q] = mad(alpha, pi, xr[q]);
         pr[q] = pi;
-        st_plain(s_out, o[q][0], si);
-        st_plain(r_out, o[q][0], rn);
-        st_plain(w_out, o[q][0], wn);
+        if constexpr (GRID) {
+          st_coh_off(s_out, o[q][0], si);
+          st_coh_off(r_out, o[q][0], rn);
+          st_coh_off(w_out, o[q][0], wn);
+        } else {
+          st_plain(s_out, o[q][0], si);
+          st_plain(r_out, o[q][0], rn);
+          st_plain(w_out, o[q][0], wn);
+        }
         cg_dots(acc, rn, wn, ic[q][0]);
       }
-      store_partials<3>(acc, P_out, G, v_lo + q);
-      // thread 0 has read the wave sums from LDS before the next
-      // store_partials overwrites them
-      __syncthreads();
+      if (q % RR == RR - 1) {   // the virtual workgroup's last row of the thread
+        store_partials_sc<3, GRID>(acc, P_out, G, v_lo + q / RR);
+        // thread 0 has read the wave sums from LDS before the next
+        // store_partials overwrites them
+        __syncthreads();
+        acc[0] = acc[1] = acc[2] = 0.0;
+      }
     }
     // every store of this workgroup has reached the L2 before it arrives
     asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory");
     __syncthreads();
-    if (threadIdx.x == 0) s_go = xcd_barrier(cnt, members * (it + 1), sync, pl.wait_ticks);
+    if (threadIdx.x == 0)
+      s_go = GRID ? grid_barrier(sync, my_xcc, my_slot, pl.per_xcc, it + 1, pl.wait_ticks)
+                  : xcd_barrier(cnt, members * (it + 1), sync, pl.wait_ticks);
     __syncthreads();
     if (!__builtin_amdgcn_readfirstlane(s_go)) return;
   }
@@ -1760,7 +1865,7 @@ static int cgs_impl(const M& A, const KLaunch& L, const real* b, real* x, const 
 // Workgroup slots per XCD for k_cgs_xcd<M, RV> at B threads per workgroup on
 // the current device: co-resident workgroups per CU times CUs per XCD (0: no
 // cooperative launch).
-template <class M, int RV>
+template <class M, int RV, bool GRID, int RR = 1>
 static int xcd_slots(int B) {
   struct Entry {
     int dev, B, slots;
@@ -1776,7 +1881,7 @@ static int xcd_slots(int B) {
   if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) == hipSuccess &&
       coop &&
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_cgs_xcd<M, RV>, B, 0) == hipSuccess)
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_cgs_xcd<M, RV, GRID, RR>, B, 0) == hipSuccess)
     slots = occ * (cus / (int)kNumXcd);
   (void)hipGetLastError();
   cache.push_back({dev, B, slots});
@@ -1788,6 +1893,8 @@ static int xcd_slots(int B) {
 // ceil(nb / rv) slots of one XCD; several are placed greedily on the
 // least-used XCD, which bounds every XCD's use by (sum of the cases' slots)
 // / 8 + the largest case's slots.
+constexpr int64_t kXcdLocalRows = 8192;   // the measured crossover (design notes)
+
 static bool xcd_plan_fits(const KLaunch& L, int rv, int slots, XcdPlan& pl) {
   const int G = L.sg.G, K = L.sg.n_cases;
   const int widest = L.max_case_blocks > 0 ? L.max_case_blocks : G;
@@ -1813,13 +1920,46 @@ static int cgs_xcd_impl(const M& A, const KLaunch& L, const real* b, real* x, co
   } else {
     const Seg& sg = L.sg;
     const int n = sg.n, B = L.B, G = sg.G, K = sg.n_cases;
-    if (tl.fixed_groups > 0 || sg.chunk > B) return WCFD_OK;
+    const int rr = (sg.chunk + B - 1) / B;   // rows per thread of a virtual workgroup
+    if (tl.fixed_groups > 0 || rr > 4 || (rr > 1 && K != 1)) return WCFD_OK;
     const int widest = L.max_case_blocks > 0 ? L.max_case_blocks : G;
     if ((int64_t)widest * sg.chunk > (int64_t)g_persistent.load()) return WCFD_OK;
     XcdPlan pl{};
-    if (!xcd_plan_fits(L, 1, xcd_slots<M, 1>(B), pl) &&
-        !xcd_plan_fits(L, 2, xcd_slots<M, 2>(B), pl))
-      return WCFD_OK;   // more rows than one XCD's registers hold: per-iteration kernels
+    // A case of more than kXcdLocalRows rows alone in the solve spreads over
+    // every XCD (one XCD gathers at an eighth of the chip's rate); smaller
+    // cases, and every case of a batch, sit on one XCD each.
+    bool grid = K == 1 && ((int64_t)widest * sg.chunk > kXcdLocalRows || rr > 1);
+    if (!grid && !xcd_plan_fits(L, 1, xcd_slots<M, 1, false>(B), pl) &&
+        !xcd_plan_fits(L, 2, xcd_slots<M, 2, false>(B), pl))
+      grid = K == 1;   // one XCD's registers do not hold it
+    if (grid) {
+      const int per_band = (G + (int)kNumXcd - 1) / (int)kNumXcd;
+      pl.rv = 0;
+      if (rr == 1) {
+        // one row per thread: 1-4 virtual workgroups per workgroup
+        const int slots[5] = {0, xcd_slots<M, 1, true>(B), xcd_slots<M, 2, true>(B),
+                              xcd_slots<M, 3, true>(B), xcd_slots<M, 4, true>(B)};
+        for (int rv = 1; rv <= 4 && !pl.rv; ++rv) {
+          const int need = (per_band + rv - 1) / rv;
+          if (need <= slots[rv]) {
+            pl.rv = rv;
+            pl.per_xcc = need;
+          }
+        }
+      } else {
+        // rr rows per thread: one virtual workgroup per workgroup
+        const int slots = rr == 2 ? xcd_slots<M, 2, true, 2>(B)
+                          : rr == 3 ? xcd_slots<M, 3, true, 3>(B)
+                                    : xcd_slots<M, 4, true, 4>(B);
+        if (per_band <= slots) {
+          pl.rv = rr;
+          pl.per_xcc = per_band;
+        }
+      }
+      if (!pl.rv) return WCFD_OK;   // more rows than the chip's registers hold
+    } else if (!pl.rv) {
+      return WCFD_OK;
+    }
     pl.check_every = check_every > 0 ? check_every : 1;
     pl.wait_ticks = 10000000;   // 100 ms at 100 MHz
     if (const int cut = g_xcd_shrink.load()) {
@@ -1857,7 +1997,15 @@ static int cgs_xcd_impl(const M& A, const KLaunch& L, const real* b, real* x, co
     hipEvent_t e0 = nullptr, e1 = nullptr;
     const bool timed = prof_active() && prof_events(K_CG_XCD, &e0, &e1);
     if (timed && (e = check_hip(hipEventRecord(e0, s)))) return e;
-    const void* kern = pl.rv == 1 ? (const void*)k_cgs_xcd<M, 1> : (const void*)k_cgs_xcd<M, 2>;
+    const void* kern =
+        !grid ? (pl.rv == 1 ? (const void*)k_cgs_xcd<M, 1, false> : (const void*)k_cgs_xcd<M, 2, false>)
+        : rr == 2 ? (const void*)k_cgs_xcd<M, 2, true, 2>
+        : rr == 3 ? (const void*)k_cgs_xcd<M, 3, true, 3>
+        : rr == 4 ? (const void*)k_cgs_xcd<M, 4, true, 4>
+        : pl.rv == 1 ? (const void*)k_cgs_xcd<M, 1, true>
+        : pl.rv == 2 ? (const void*)k_cgs_xcd<M, 2, true>
+        : pl.rv == 3 ? (const void*)k_cgs_xcd<M, 3, true>
+                     : (const void*)k_cgs_xcd<M, 4, true>;
     if ((e = check_hip(hipLaunchCooperativeKernel(kern, dim3(kNumXcd * pl.per_xcc), dim3(B),
                                                   args, 0, s))))
       return e;

@@ -116,10 +116,10 @@ def krylov_candidates(C, grids=GRIDS, blocks=BLOCKS):
     return out
 
 
-# Largest case for which the persistent CG is tried: one XCD's registers hold
-# about 40 000 - 49 000 rows at 256 threads, and above about 8 000 rows the
-# per-iteration kernels measured faster (one XCD gathers at an eighth of the
-# chip's rate).
+# Largest case for which the persistent CG at one row per thread is among
+# the screened candidates (on one XCD up to 8 192 rows, spread over every XCD
+# above); larger cases meet it in the refinement stage, at the launch shapes
+# that screened fastest.
 PERSISTENT_MAX_ROWS = 16384
 
 
@@ -133,22 +133,25 @@ def persistent_candidates(C, blocks=BLOCKS):
 
 
 def autotune_krylov(dm, grids=GRIDS, blocks=BLOCKS, iters=60, cg_weight=1.0, bicg_weight=0.1,
-                    cg_variant=0, refine=6, refine_iters=300, refine_reps=3):
+                    cg_variant=0, refine=6, refine_iters=300, refine_reps=3,
+                    try_persistent=True):
     """Pick and install the fastest (grid, block, fold, persistent) for mesh
     ``dm`` (a DeviceMesh).  The weights reflect a SIMPLE iteration, where the
     pressure CG runs far more iterations than the momentum BiCGSTAB.  Every
     candidate is timed on ``iters`` iterations; the ``refine`` fastest are
     then timed again on ``refine_iters`` iterations, ``refine_reps`` rounds
-    in turn, and the smallest median score wins, so that the pick does not
-    follow the timing noise of one short solve.  The persistent CG is a
-    candidate only with the single-reduction form."""
-    key = (dm.n_cells, cg_variant, dm.lib.wcfd_real_bytes())
+    in turn (each also with the persistent CG when ``try_persistent``), and
+    the smallest median score wins, so that the pick does not follow the
+    timing noise of one short solve.  The persistent CG is a candidate only
+    with the single-reduction form."""
+    key = (dm.n_cells, cg_variant, dm.lib.wcfd_real_bytes(), bool(try_persistent))
     if key in _cache:
         c = _cache[key]
         install(c['grid'], c['block'], c['fold'], dm.lib, c['persistent'])
         return c
     cands = [c + (0,) for c in krylov_candidates(dm.n_cells, grids, blocks)]
-    if cg_variant == _lib.CG_SINGLE_REDUCTION:
+    try_persistent = try_persistent and cg_variant == _lib.CG_SINGLE_REDUCTION
+    if try_persistent:
         cands += persistent_candidates(dm.n_cells, blocks)
 
     def score(grid, blk, fold, pers, n_it, reps):
@@ -159,6 +162,16 @@ def autotune_krylov(dm, grids=GRIDS, blocks=BLOCKS, iters=60, cg_weight=1.0, bic
 
     results = [score(*c, iters, 2) for c in cands]
     top = sorted(results, key=lambda r: r['score'])[:max(1, refine)]
+    if try_persistent:
+        # each refined launch shape also with the persistent CG (a single case
+        # larger than one XCD serves spreads over every XCD; the library takes
+        # the per-iteration kernels where the shape is not eligible)
+        seen = {(r['grid'], r['block'], r['fold'], r['persistent']) for r in top}
+        for r in list(top):
+            key2 = (r['grid'], r['block'], r['fold'], dm.n_cells)
+            if r['persistent'] == 0 and key2 not in seen:
+                seen.add(key2)
+                top.append(dict(r, persistent=dm.n_cells))
     rounds = {i: [] for i in range(len(top))}
     for _ in range(refine_reps):
         for i, r in enumerate(top):
