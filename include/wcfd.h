@@ -602,6 +602,12 @@ int wcfd_profile_end(double* total_ms, int32_t* counts, int n_ids); /* [host] ar
  * launches at or beyond the iteration count it ran); this returns, per
  * kernel id, how many launches of the last profiled region were left out. */
 int wcfd_profile_skipped(int32_t* counts, int n_ids); /* [host] array */
+/* The launches recorded so far (before wcfd_profile_end), in launch order:
+ * kernel id (-1 - id for a launch that exited at entry) and the begin and end
+ * of each, in ms from the first launch's begin.  Returns the number written
+ * (at most capacity), or minus an error code. */
+int wcfd_profile_timeline(int32_t* ids, double* start_ms, double* end_ms,
+                          int capacity); /* [host] arrays */
 
 #ifdef __cplusplus
 }

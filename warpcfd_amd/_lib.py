@@ -148,6 +148,9 @@ _SIGS = {
     'wcfd_get_krylov_persistent': (ctypes.c_int, []),
     'wcfd_krylov_persistent_stats': (ctypes.c_int, [ctypes.POINTER(ctypes.c_int64),
                                                      ctypes.POINTER(ctypes.c_int64)]),
+    'wcfd_profile_timeline': (ctypes.c_int, [ctypes.POINTER(ctypes.c_int32),
+                                              ctypes.POINTER(ctypes.c_double),
+                                              ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
     'wcfd_debug_krylov_persistent_shrink': (ctypes.c_int, [ctypes.c_int]),
     'wcfd_simple_workspace_bytes': (ctypes.c_size_t, [c_i32, c_i32]),
     'wcfd_kernel_count': (ctypes.c_int, []),

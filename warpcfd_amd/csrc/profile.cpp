@@ -119,6 +119,27 @@ int wcfd_profile_end(double* total_ms, int32_t* counts, int n_ids) {
   return WCFD_OK;
 }
 
+int wcfd_profile_timeline(int32_t* ids, double* start_ms, double* end_ms, int capacity) {
+  if (capacity < 0 || (capacity > 0 && (!ids || !start_ms || !end_ms))) return WCFD_EINVAL;
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (g_recs.empty()) return 0;
+  const hipEvent_t base = g_recs.front().a;
+  int n = 0;
+  for (auto& r : g_recs) {
+    if (n >= capacity) break;
+    int rc = check_hip(hipEventSynchronize(r.b));
+    if (rc) return -rc;
+    float a = 0.f, b = 0.f;
+    if ((rc = check_hip(hipEventElapsedTime(&a, base, r.a)))) return -rc;
+    if ((rc = check_hip(hipEventElapsedTime(&b, base, r.b)))) return -rc;
+    ids[n] = r.dead ? -1 - r.id : r.id;
+    start_ms[n] = a;
+    end_ms[n] = b;
+    ++n;
+  }
+  return n;
+}
+
 int wcfd_profile_skipped(int32_t* counts, int n_ids) {
   if (n_ids < K_COUNT || !counts) return WCFD_EINVAL;
   std::lock_guard<std::mutex> lk(g_mu);
