@@ -594,8 +594,9 @@ struct EarlyFold {
   }
 };
 
-// Block sum of ND values; thread 0 stores them as partials part[i*G + vb].
-template <int ND>
+// Block sum of ND values; thread 0 stores them as partials part[i*G + vb]
+// (COH: at agent scope, for readers on other XCDs inside the same launch).
+template <int ND, bool COH = false>
 __device__ inline void store_partials(double (&v)[ND], double* __restrict__ part, int G, int vb) {
   __shared__ double lds[ND * kMaxWaves];
   const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
@@ -609,7 +610,10 @@ __device__ inline void store_partials(double (&v)[ND], double* __restrict__ part
     for (int i = 0; i < ND; ++i) {
       double t = 0.0;
       for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += lds[i * kMaxWaves + w];
-      part[i * G + vb] = t;
+      if constexpr (COH)
+        __hip_atomic_store(part + i * G + vb, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      else
+        part[i * G + vb] = t;
     }
   }
 }

@@ -730,29 +730,6 @@ __device__ __forceinline__ void st_coh_off(T* base, uint32_t byte_off, T v) {
                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// store_partials whose sums are stored at agent scope (COH) or plainly
-template <int ND, bool COH>
-__device__ inline void store_partials_sc(double (&v)[ND], double* part, int G, int vb) {
-  __shared__ double lds[ND * kMaxWaves];
-  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-  wave_sum_n<ND>(v);
-#pragma unroll
-  for (int i = 0; i < ND; ++i)
-    if (lane == 0) lds[i * kMaxWaves + wid] = v[i];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int i = 0; i < ND; ++i) {
-      double t = 0.0;
-      for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += lds[i * kMaxWaves + w];
-      if constexpr (COH)
-        __hip_atomic_store(part + i * G + vb, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      else
-        part[i * G + vb] = t;
-    }
-  }
-}
-
 // one of a ping-pong pair
 __device__ __forceinline__ real* wb_sel(real* a, real* b, int which) { return which ? b : a; }
 
@@ -1076,7 +1053,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_xcd(
         cg_dots(acc, rn, wn, ic[q][0]);
       }
       if (q % RR == RR - 1) {   // the virtual workgroup's last row of the thread
-        store_partials_sc<3, GRID>(acc, P_out, G, v_lo + q / RR);
+        store_partials<3, GRID>(acc, P_out, G, v_lo + q / RR);
         // thread 0 has read the wave sums from LDS before the next
         // store_partials overwrites them
         __syncthreads();
