@@ -462,15 +462,19 @@ int wcfd_get_krylov_inline_check(void);
 /* Persistent single-reduction CG for solves whose widest case has at most
  * max_rows rows (its virtual workgroups times the chunk); 0, the default:
  * one launch per iteration always.  One cooperative launch runs the whole
- * solve, every case on the CUs of ONE XCD (its workgroups share that XCD's
- * L2 and meet at a barrier in memory once per iteration), for single-case and
- * batched solves with one row per thread (chunk <= block), at most two
- * virtual workgroups per workgroup, and no fixed launch budget; other solves
- * take the per-iteration kernels.  The iterations, the solution and the
- * reported residual are bitwise those of the per-iteration kernels at the
- * same launch shape.  Measured on MI355X: an iteration costs about 4 us plus
- * 0.2 us per thousand rows (one XCD's gather rate), against 5.5-6 us per
- * launch, so it pays below about 8 000 rows per case.  A launch whose
+ * solve.  Every case of a batch, and a single case of at most 8 192 rows,
+ * sits on the CUs of ONE XCD (its workgroups share that XCD's L2 and meet at
+ * a barrier in memory once per iteration), one row per thread (chunk <=
+ * block), one or two virtual workgroups per workgroup.  A larger single case
+ * spreads over every XCD in contiguous bands (a two-level barrier, agent-scope
+ * stores), one to four rows per thread.  Fixed launch budgets and other
+ * shapes take the per-iteration kernels.  The iterations, the solution and
+ * the reported residual are bitwise those of the per-iteration kernels at
+ * the same launch shape.  Measured on MI355X: on one XCD an iteration costs
+ * about 4 us plus 0.2 us per thousand rows (one XCD's gather rate), against
+ * 5.5-6 us per launch, so it pays below about 8 000 rows per case; the
+ * 320 k-row channel over every XCD takes 8.06 us per iteration, as many as a
+ * launch, and saves the kernel boundaries.  A launch whose
  * workgroups do not land as planned, or whose barrier waits longer than
  * 100 ms, aborts before x is written; the solve then runs on the
  * per-iteration kernels and the setting turns itself off.  Process-wide.  No
