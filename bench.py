@@ -20,8 +20,9 @@ begin and end during a profiled pass of the same number of steps) and
     python bench.py [--steps K] [--warmup W] [--workload channel|cavity|batch64]
 
 ``--workload batch64`` solves the 64-cavity batch (one step = the whole
-batch to continuity <= 1e-6) in one arena and, for comparison, the same
-cases one after another through the plain solver; its value is solves/s.
+batch to continuity <= 1e-6) split over four arenas stepped side by side
+(``--arenas``) and, for comparison, the same cases one after another
+through the plain solver; its value is solves/s.
 """
 import argparse
 import json
@@ -29,6 +30,21 @@ import re
 import os
 import sys
 import time
+
+# Hardware queues HIP gives this process (read when HIP starts, so before
+# any import that starts it).  The batch and the served stream step four
+# arenas on four streams; with HIP's default of four queues the default
+# stream takes one and two of those streams can land on one queue, which
+# serialises them (the 64-cavity batch: 39.0 s against 28.1 s with 8
+# queues, measured on MI355X).  Six or more busy queues measured slower, so
+# the arena count stays at four.  A larger value set by the caller is kept.
+try:
+    _queues = int(os.environ.get('GPU_MAX_HW_QUEUES', '4'))
+except ValueError:
+    _queues = 4
+if _queues < 8:
+    os.environ['GPU_MAX_HW_QUEUES'] = '8'
+del _queues
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -289,14 +305,25 @@ def run_batch64(args, sequential=True, profile=True, cpu=True, fp64_check=False)
     torch.cuda.synchronize()
     del warm
 
-    arena = CaseArena.for_cases(specs, **kw)
+    split = args.arenas > 1 and not args.graph
+    if split:
+        # the cases over args.arenas arenas, each stepped by its own host
+        # thread on its own stream (bitwise the results of one arena)
+        from warpcfd_amd.batch import SplitBatch
+        arena = SplitBatch(specs, n_arenas=args.arenas, **kw)
+    else:
+        arena = CaseArena.for_cases(specs, **kw)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     done_at = {}
     steps = 0
     cur = None
     n_capture = 0
-    while len(done_at) < len(specs) and steps < args.max_outer + 1:
+    if split:
+        per_arena = arena.run(max_steps=args.max_outer + 1, poll_every=args.poll_every)
+        steps = max(per_arena)
+        done_at = dict(arena.done_at)
+    while not split and len(done_at) < len(specs) and steps < args.max_outer + 1:
         if args.graph:
             # HIP-graph replay with a Krylov launch budget measured every 100 iterations
             stale = cur is not None and arena._graph['blocks'] != arena.n_blocks
@@ -416,7 +443,9 @@ def run_batch64(args, sequential=True, profile=True, cpu=True, fp64_check=False)
         'vs_baseline': None,
         'dtype': 'f32',
         'data': 'synthetic',
-        'config': {'workload': f'{len(specs)} lid-driven cavities in one arena, N log-uniform in '
+        'config': {'workload': f'{len(specs)} lid-driven cavities in '
+                               f'{f"{args.arenas} arenas on {args.arenas} streams" if split else "one arena"}'
+                               f', N log-uniform in '
                                f'[32,256], Re uniform in [100,1000] (seed {args.seed}), steady '
                                f'SIMPLE (alpha_u 0.7, alpha_p 0.3) to continuity <= {stop:g} and '
                                f'estimated velocity error <= {stop_u:g}', 'cases': len(specs),
@@ -431,6 +460,11 @@ def run_batch64(args, sequential=True, profile=True, cpu=True, fp64_check=False)
                    'reductions': 'f64'},
         'arena_tuning': tuned,
         'batched': {'total_s': round(batched_s, 3), 'batch_iterations': steps,
+                    'arenas': ({'count': len(arena.arenas), 'cases': [len(g) for g in arena.groups],
+                                'batch_iterations': per_arena,
+                                'note': 'cases split by cell count over arenas stepped by '
+                                        'separate host threads on separate streams'}
+                               if split else None),
                     'hip_graph': ({'captures': n_capture, 'last_budget': cur, 'recheck_every': 100}
                                   if args.graph else None),
                     'converged': n_conv, 'outer_iterations_max': int(max(iters)),
@@ -517,7 +551,7 @@ def run_serve(args, n_requests, mix=SERVE_MIX, sequential=True, error_sample=0, 
         print(f'# serve{label}: {len(stream)} requests, {cells} cells, arrivals over '
               f'{stream[-1].arrival:.2f} s', file=sys.stderr, flush=True)
         srv = ContinuousBatchServer(hbm_bytes=int(args.hbm_gb * (1 << 30)), max_cases=args.slots,
-                                    chunk=args.chunk, block=args.block,
+                                    chunk=args.chunk, block=args.block, lanes=args.lanes,
                                     keep_fields=error_sample > 0, **pol)
 
         def report(r):
@@ -556,7 +590,7 @@ def run_serve(args, n_requests, mix=SERVE_MIX, sequential=True, error_sample=0, 
                     f'({mix["channel_cells"][0] // 1000}k-{mix["channel_cells"][1] // 1000}k cells, '
                     f'Re in [50,400]), steady SIMPLE to {stop_desc}',
         'requests': len(stream), 'cells': cells, 'hbm_budget_gb': args.hbm_gb,
-        'slots': args.slots, 'host_workers': args.workers,
+        'slots': args.slots, 'lanes': args.lanes, 'host_workers': args.workers,
         'krylov': {'atol': 0.0, 'tol': 0.0, 'rtol0': args.rtol0, 'maxiter': 2000,
                    'check_every': 10, 'check_every_momentum': args.check_u,
                    'pressure_cg': 'single'},
@@ -901,6 +935,10 @@ def main():
     ap.add_argument('--check-u', type=int, default=2,
                     help='batch: check interval of the momentum solve')
     ap.add_argument('--no-sequential', action='store_true')
+    ap.add_argument('--arenas', type=int, default=4,
+                    help='batch64: split the cases over this many arenas, each stepped by its '
+                         'own host thread and stream (4: the hardware queues HIP gives a '
+                         'process by default); 1: one arena')
     ap.add_argument('--calibration-copy-mb', type=float, default=0.0,
                     help='run one elementwise copy of this size (rocprofv3 counter calibration)')
     ap.add_argument('--traffic-from', default=None,
@@ -922,6 +960,9 @@ def main():
     ap.add_argument('--rate', type=float, default=20.0, help='serve: arrivals per second')
     ap.add_argument('--hbm-gb', type=float, default=32.0, help='serve: arena HBM budget')
     ap.add_argument('--slots', type=int, default=64, help='serve: case slots')
+    ap.add_argument('--lanes', type=int, default=4,
+                    help='serve: arenas (budget and slots split evenly), each served by its own '
+                         'host thread and stream; 1: one arena')
     ap.add_argument('--workers', type=int, default=8, help='serve: host preparation threads')
     ap.add_argument('--serve-continuity-only', action='store_true',
                     help='serve: stop at continuity <= 1e-6 alone (without the velocity-error '

@@ -161,3 +161,18 @@ def test_krylov_tuner_candidates():
     assert (500, 512, 4096) in cand and (500, 512, 0) in cand
     # a case smaller than a workgroup: no two-row grids, no grids far beyond its rows
     assert {g for g, _, _ in krylov_candidates(100)} <= {0, 128}
+
+
+def test_split_groups_deal_back_and_forth():
+    """Cases in decreasing size dealt 0, 1, ..., n-1, n-1, ..., 0, 0, ...:
+    here every group's total is 12; empty groups are dropped; ties keep the
+    index order."""
+    from warpcfd_amd.batch import split_groups
+    sizes = [5, 9, 1, 9, 3, 7, 2]
+    g = split_groups(sizes, 3)
+    assert g == [[1, 6, 2], [3, 4], [5, 0]]
+    assert [sum(sizes[k] for k in x) for x in g] == [12, 12, 12]
+    assert split_groups([1, 2], 4) == [[1], [0]]
+    assert split_groups([4, 4, 4, 4], 2) == [[0, 3], [1, 2]]
+    assert split_groups(sizes, 1) == [[1, 3, 5, 0, 4, 6, 2]]
+    assert split_groups([], 3) == []

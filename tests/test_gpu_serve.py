@@ -43,3 +43,26 @@ def test_served_equals_alone_and_sequential(gpu):
         q = by_id[r.rid]
         assert r.iterations == q.iterations and np.array_equal(r.fields, q.fields), r.describe()
     prep.close()
+
+
+def test_lanes_equal_one_lane(gpu):
+    """Two and three lanes (arenas served by their own threads on their own
+    streams, requests to the least loaded lane with space): every request's
+    fields and SIMPLE iterations bitwise those of the one-lane server, and
+    more than one lane takes requests."""
+    from warpcfd_amd.serve import ContinuousBatchServer, Preparer
+    prep = Preparer(workers=2, stop=1e-6, stop_u=5e-5, floor_u=1e-7, max_outer=3000)
+    one = ContinuousBatchServer(hbm_bytes=3 << 20, max_cases=3, keep_fields=True, **KW)
+    ref = {r.rid: r for r in one.serve(_stream(), prep)}
+    for lanes in (2, 3):
+        print(f'{lanes} lanes', flush=True)
+        srv = ContinuousBatchServer(hbm_bytes=lanes * (3 << 20), max_cases=3 * lanes,
+                                    keep_fields=True, lanes=lanes, **KW)
+        got = srv.serve(_stream(), prep)
+        assert len(got) == 6 and all(r.error is None and r.converged for r in got)
+        for r in got:
+            q = ref[r.rid]
+            assert r.iterations == q.iterations, r.describe()
+            assert np.array_equal(r.fields, q.fields), r.describe()
+        assert len({j for _, j, _, _ in srv.log}) > 1
+    prep.close()

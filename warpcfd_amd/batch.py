@@ -711,3 +711,110 @@ class CaseArena:
 
     def iterations(self, k):
         return int(self.outer_iters[k].item())
+
+
+def split_groups(sizes, n_groups):
+    """Case indices in ``n_groups`` groups of about equal total size: the
+    cases in decreasing size (ties by index) dealt out back and forth
+    (groups 0, 1, ..., n-1, n-1, ..., 1, 0, 0, 1, ...), so every group gets
+    one of the largest cases.  Groups left empty are dropped."""
+    n = max(1, int(n_groups))
+    order = sorted(range(len(sizes)), key=lambda k: (-sizes[k], k))
+    groups = [[] for _ in range(n)]
+    for i, k in enumerate(order):
+        r, j = divmod(i, n)
+        groups[j if r % 2 == 0 else n - 1 - j].append(k)
+    return [g for g in groups if g]
+
+
+class SplitBatch:
+    """Steady cases split over several arenas, each stepped by its own host
+    thread on its own stream, so that the arenas' kernels run side by side.
+
+    One arena advances its cases in lockstep: each batched SIMPLE iteration
+    waits for the slowest Krylov solve among them, and once few cases remain
+    every launch costs its fixed overhead rather than HBM time.  Over several
+    arenas, a group whose solves end early starts its next iteration while
+    the others still iterate, and the launches of different groups overlap
+    on the GPU.  A case's arithmetic depends only on its own rows and on the
+    workgroup shape (``chunk``, ``block``, the same for every arena), so its
+    results are bitwise those of one arena holding every case.
+
+    ``specs``: CaseSpec list; ``n_arenas``: groups (``split_groups`` by cell
+    count); ``arena_kw``: CaseArena keywords.  Results are addressed by the
+    index of the case in ``specs``.  The library keeps one pinned polling
+    buffer and its events per host thread, so each worker thread owns one
+    such set.  The arenas overlap only on streams of their own hardware
+    queues: with HIP's default of four queues per process, four arenas
+    measured slower (two of their streams shared a queue) than with eight
+    (``GPU_MAX_HW_QUEUES``, read when HIP starts).
+    """
+
+    def __init__(self, specs, n_arenas=4, **arena_kw):
+        self.groups = split_groups([s.n_cells for s in specs], n_arenas)
+        self.arenas = [CaseArena.for_cases([specs[k] for k in g], **arena_kw)
+                       for g in self.groups]
+        self.where = {k: (j, i) for j, g in enumerate(self.groups) for i, k in enumerate(g)}
+        self.streams = [torch.cuda.Stream(device=a.device) for a in self.arenas]
+        self.done_at = {}
+        self.steps = [0] * len(self.arenas)
+
+    def run(self, max_steps=100000, poll_every=4):
+        """Step every arena on its own thread and stream until none of its
+        cases is active (at most ``max_steps`` batched iterations each).
+        ``done_at[k]``: seconds from the start to the poll that saw case k
+        stop.  Returns the batched iterations of each arena."""
+        import threading
+        import time
+        torch.cuda.synchronize()          # arenas were filled on the default stream
+        t0 = time.perf_counter()
+        errors = []
+
+        def work(j):
+            a, g = self.arenas[j], self.groups[j]
+            try:
+                with torch.cuda.stream(self.streams[j]):
+                    n = 0
+                    while n < max_steps:
+                        a.step()
+                        n += 1
+                        if n % poll_every == 0 or n == max_steps:
+                            now = time.perf_counter() - t0
+                            for i in a.poll():
+                                self.done_at.setdefault(g[i], now)
+                            if not any(a.h_active[i] for i in range(len(g))):
+                                break
+                    self.steps[j] = n
+                    self.streams[j].synchronize()
+            except BaseException as e:    # re-raised on the calling thread
+                errors.append(e)
+
+        threads = [threading.Thread(target=work, args=(j,)) for j in range(len(self.arenas))]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        if errors:
+            raise errors[0]
+        torch.cuda.synchronize()
+        return list(self.steps)
+
+    def _at(self, k):
+        j, i = self.where[k]
+        return self.arenas[j], i
+
+    def cell_values(self, k):
+        a, i = self._at(k)
+        return a.cell_values(i)
+
+    def converged(self, k):
+        a, i = self._at(k)
+        return a.converged(i)
+
+    def iterations(self, k):
+        a, i = self._at(k)
+        return a.iterations(i)
+
+    def continuity(self, k):
+        a, i = self._at(k)
+        return a.continuity(i)

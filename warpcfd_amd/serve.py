@@ -158,23 +158,41 @@ class Preparer:
 
 
 class ContinuousBatchServer:
-    """Serves a request stream with continuous batching on one arena.
+    """Serves a request stream with continuous batching.
 
     ``hbm_bytes`` sets the arena capacity (cells and faces); ``max_cases``
     the number of case slots.  Admission is first come first served, with
     smaller prepared requests allowed to take space the head of the queue
     does not fit in (backfill).  Every other arena keyword (Krylov policy,
-    relaxation, chunking) is passed through."""
+    relaxation, chunking) is passed through.
 
-    def __init__(self, hbm_bytes=16 << 30, max_cases=64, keep_fields=False, **arena_kw):
-        self.arena = CaseArena.for_budget(hbm_bytes, max_cases, **arena_kw)
+    ``lanes`` > 1 splits the budget and the slots over that many arenas, each
+    served by its own host thread on its own stream: a lane admits queued
+    requests into its arena, steps it and releases what finished, so one
+    lane's long Krylov solves do not hold up the others' iterations, and the
+    lanes' kernels overlap on the GPU (as ``batch.SplitBatch``).  A request
+    goes to the least loaded lane (resident cells) that has space for it.
+    Results are bitwise the same in any lane."""
+
+    def __init__(self, hbm_bytes=16 << 30, max_cases=64, keep_fields=False, lanes=1,
+                 **arena_kw):
+        n = max(1, int(lanes))
+        self.arenas = [CaseArena.for_budget(hbm_bytes // n, max(1, max_cases // n), **arena_kw)
+                       for _ in range(n)]
+        self.arena = self.arenas[0]
         self.keep_fields = keep_fields
         self.log = []
+        if n > 1:
+            import torch
+            self.streams = [torch.cuda.Stream(device=a.device) for a in self.arenas]
 
     def serve(self, requests, preparer, clock_start=None, max_steps=10_000_000, on_done=None):
         """Serve ``requests`` (``Request`` objects, arrival times in seconds
         after ``clock_start``); returns them in completion order.
-        ``on_done(request)`` is called as each one finishes."""
+        ``on_done(request)`` is called as each one finishes (from a lane's
+        thread when there are several lanes)."""
+        if len(self.arenas) > 1:
+            return self._serve_lanes(requests, preparer, clock_start, max_steps, on_done)
         a = self.arena
         t0 = clock_start if clock_start is not None else time.perf_counter()
         pending = sorted(requests, key=lambda r: r.arrival)
@@ -226,11 +244,123 @@ class ContinuousBatchServer:
                     done.append(r)
                     if on_done:
                         on_done(r)
-                    self.log.append((r.t_done, len(resident), a.cells.largest()))
+                    self.log.append((r.t_done, 0, len(resident), a.cells.largest()))
             else:
                 # idle: wait for the next arrival or preparation
                 nxt = pending[0].arrival - (time.perf_counter() - t0) if pending else 0.0
                 time.sleep(min(max(nxt, 0.0), 0.01) if nxt > 0 else 0.001)
+        return done
+
+    def _finish(self, a, k, r, t0):
+        r.t_done = time.perf_counter() - t0
+        r.iterations = a.iterations(k)
+        r.converged = a.converged(k)
+        if self.keep_fields:
+            r.fields = a.cell_values(k)
+        a.release(k)
+
+    def _serve_lanes(self, requests, preparer, clock_start, max_steps, on_done):
+        """serve() over several lanes.  The calling thread takes arrivals and
+        moves prepared requests to the shared queue; each lane's thread, on
+        its own stream, admits from that queue (under the lock, as admission
+        rewrites the arena's descriptors, which only its own stream may
+        touch), steps and polls its arena, and releases finished cases.  A
+        lane leaves a request to a less loaded lane unless that lane has
+        found no space for it."""
+        import threading
+        import torch
+        t0 = clock_start if clock_start is not None else time.perf_counter()
+        pending = sorted(requests, key=lambda r: r.arrival)
+        futures = []
+        lock = threading.Lock()
+        queue = []               # prepared, waiting for space (arrival order)
+        done = []
+        L = len(self.arenas)
+        load = [0] * L           # resident cells per lane
+        full = {}                # request id -> lanes that found no space for it
+        closing = threading.Event()
+        errors = []
+
+        def lane(j):
+            a = self.arenas[j]
+            resident = {}
+            steps = 0
+            try:
+                with torch.cuda.stream(self.streams[j]):
+                    while steps < max_steps:
+                        with lock:
+                            for r in list(queue):
+                                if not a.fits(r.spec):
+                                    full.setdefault(r.rid, set()).add(j)
+                                    continue
+                                skip = full.get(r.rid, ())
+                                if any(load[m] < load[j] and m not in skip
+                                       for m in range(L) if m != j):
+                                    continue          # a less loaded lane may take it
+                                k = a.admit(r.spec)
+                                r.t_admit = time.perf_counter() - t0
+                                resident[k] = r
+                                load[j] += r.spec.n_cells
+                                queue.remove(r)
+                            idle = not resident
+                            last = closing.is_set() and not queue
+                        if idle:
+                            if last:
+                                break
+                            time.sleep(0.0005)
+                            continue
+                        a.step()
+                        steps += 1
+                        for k in a.poll():
+                            r = resident.pop(k)
+                            self._finish(a, k, r, t0)
+                            with lock:
+                                load[j] -= r.spec.n_cells
+                                done.append(r)
+                                self.log.append((r.t_done, j, len(resident), a.cells.largest()))
+                            if on_done:
+                                on_done(r)
+                    self.streams[j].synchronize()
+            except BaseException as e:    # re-raised on the calling thread
+                errors.append(e)
+                closing.set()
+
+        torch.cuda.synchronize()          # the arenas were built on the default stream
+        threads = [threading.Thread(target=lane, args=(j,)) for j in range(L)]
+        for t in threads:
+            t.start()
+        try:
+            while (pending or futures) and not errors:
+                now = time.perf_counter() - t0
+                while pending and pending[0].arrival <= now:
+                    r = pending.pop(0)
+                    futures.append((r, preparer.submit(r)))
+                still = []
+                for r, f in futures:
+                    if not f.done():
+                        still.append((r, f))
+                        continue
+                    f.result()
+                    s = r.spec
+                    if all(s.n_cells > a.Cc or s.n_internal > a.Fic or s.n_boundary > a.Fbc
+                           for a in self.arenas):
+                        r.error = 'larger than the arena'       # can never be admitted
+                        r.t_done = time.perf_counter() - t0
+                        with lock:
+                            done.append(r)
+                        continue
+                    with lock:
+                        queue.append(r)
+                        queue.sort(key=lambda q: q.arrival)
+                futures = still
+                nxt = pending[0].arrival - (time.perf_counter() - t0) if pending else 0.0
+                time.sleep(min(max(nxt, 0.0005), 0.002))
+        finally:
+            closing.set()
+            for t in threads:
+                t.join()
+        if errors:
+            raise errors[0]
         return done
 
 
