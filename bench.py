@@ -558,8 +558,13 @@ def run_serve(args, n_requests, mix=SERVE_MIX, sequential=True, error_sample=0, 
             print(f'#   done {r.describe()}: {r.iterations} iterations, '
                   f'{r.t_done - r.arrival:.2f} s after arrival', file=sys.stderr, flush=True)
 
+        def progress(lane, steps, resident):
+            print(f'#   lane {lane}: {steps} iterations, {resident} resident, '
+                  f'{time.perf_counter() - t0:.1f} s', file=sys.stderr, flush=True)
+
         t0 = time.perf_counter()
-        done = srv.serve(stream, prep, clock_start=t0, on_done=report)
+        done = srv.serve(stream, prep, clock_start=t0, on_done=report, on_step=progress,
+                         step_every=2000)
         torch.cuda.synchronize()
         served = summarize(done, max(r.t_done for r in done))
         print(f'# continuous batching{label}: {served}', file=sys.stderr, flush=True)

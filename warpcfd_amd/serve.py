@@ -186,13 +186,17 @@ class ContinuousBatchServer:
             import torch
             self.streams = [torch.cuda.Stream(device=a.device) for a in self.arenas]
 
-    def serve(self, requests, preparer, clock_start=None, max_steps=10_000_000, on_done=None):
+    def serve(self, requests, preparer, clock_start=None, max_steps=10_000_000, on_done=None,
+              on_step=None, step_every=1000):
         """Serve ``requests`` (``Request`` objects, arrival times in seconds
         after ``clock_start``); returns them in completion order.
-        ``on_done(request)`` is called as each one finishes (from a lane's
-        thread when there are several lanes)."""
+        ``on_done(request)`` is called as each one finishes, and
+        ``on_step(lane, steps, resident)`` after every ``step_every`` batched
+        iterations of a lane (from the lane's thread when there are several
+        lanes)."""
         if len(self.arenas) > 1:
-            return self._serve_lanes(requests, preparer, clock_start, max_steps, on_done)
+            return self._serve_lanes(requests, preparer, clock_start, max_steps, on_done,
+                                     on_step, step_every)
         a = self.arena
         t0 = clock_start if clock_start is not None else time.perf_counter()
         pending = sorted(requests, key=lambda r: r.arrival)
@@ -233,6 +237,8 @@ class ContinuousBatchServer:
             if resident:
                 a.step()
                 steps += 1
+                if on_step and steps % step_every == 0:
+                    on_step(0, steps, len(resident))
                 for k in a.poll():
                     r = resident.pop(k)
                     r.t_done = time.perf_counter() - t0
@@ -259,7 +265,8 @@ class ContinuousBatchServer:
             r.fields = a.cell_values(k)
         a.release(k)
 
-    def _serve_lanes(self, requests, preparer, clock_start, max_steps, on_done):
+    def _serve_lanes(self, requests, preparer, clock_start, max_steps, on_done, on_step,
+                     step_every):
         """serve() over several lanes.  The calling thread takes arrivals and
         moves prepared requests to the shared queue; each lane's thread, on
         its own stream, admits from that queue (under the lock, as admission
@@ -311,6 +318,8 @@ class ContinuousBatchServer:
                             continue
                         a.step()
                         steps += 1
+                        if on_step and steps % step_every == 0:
+                            on_step(j, steps, len(resident))
                         for k in a.poll():
                             r = resident.pop(k)
                             self._finish(a, k, r, t0)
