@@ -1,6 +1,8 @@
 """Host logic of the batched path (no GPU): the arena's range allocator,
-the batch workload draws and the oracle's warm-start-relative Krylov stop."""
+the batch workload draws, the oracle's warm-start-relative Krylov stop, the
+split of a batch over arenas and the worker threads' failure path."""
 import numpy as np
+import pytest
 import scipy.sparse as sp
 
 from oracle import krylov
@@ -176,3 +178,34 @@ def test_split_groups_deal_back_and_forth():
     assert split_groups([4, 4, 4, 4], 2) == [[0, 3], [1, 2]]
     assert split_groups(sizes, 1) == [[1, 3, 5, 0, 4, 6, 2]]
     assert split_groups([], 3) == []
+
+
+def test_run_workers_stops_the_others_and_raises():
+    """One worker fails: the abort event stops the others at their next
+    check, and the failure is raised on the calling thread."""
+    import time
+    from warpcfd_amd.batch import run_workers
+    seen = {}
+
+    def work(j, abort):
+        if j == 1:
+            time.sleep(0.05)
+            raise RuntimeError('worker 1 failed')
+        n = 0
+        while not abort.is_set() and n < 10_000:
+            time.sleep(0.001)
+            n += 1
+        seen[j] = n
+
+    t = time.perf_counter()
+    with pytest.raises(RuntimeError, match='worker 1 failed'):
+        run_workers(work, 3)
+    assert time.perf_counter() - t < 5.0
+    assert sorted(seen) == [0, 2] and all(n < 10_000 for n in seen.values())
+
+
+def test_run_workers_all_succeed():
+    from warpcfd_amd.batch import run_workers
+    out = []
+    run_workers(lambda j, abort: out.append(j), 4)
+    assert sorted(out) == [0, 1, 2, 3]

@@ -713,6 +713,30 @@ class CaseArena:
         return int(self.outer_iters[k].item())
 
 
+def run_workers(work, n):
+    """Run ``work(j, abort)`` for j in [0, n) on n threads and wait for all of
+    them.  The first exception sets ``abort`` (a threading.Event the workers
+    check between steps, so the others stop early) and is raised here."""
+    import threading
+    abort = threading.Event()
+    errors = []
+
+    def body(j):
+        try:
+            work(j, abort)
+        except BaseException as e:    # re-raised on the calling thread
+            errors.append(e)
+            abort.set()
+
+    threads = [threading.Thread(target=body, args=(j,)) for j in range(n)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    if errors:
+        raise errors[0]
+
+
 def split_groups(sizes, n_groups):
     """Case indices in ``n_groups`` groups of about equal total size: the
     cases in decreasing size (ties by index) dealt out back and forth
@@ -764,18 +788,16 @@ class SplitBatch:
         cases is active (at most ``max_steps`` batched iterations each).
         ``done_at[k]``: seconds from the start to the poll that saw case k
         stop.  Returns the batched iterations of each arena."""
-        import threading
         import time
         torch.cuda.synchronize()          # arenas were filled on the default stream
         t0 = time.perf_counter()
-        errors = []
 
-        def work(j):
+        def work(j, abort):
             a, g = self.arenas[j], self.groups[j]
-            try:
-                with torch.cuda.stream(self.streams[j]):
-                    n = 0
-                    while n < max_steps:
+            with torch.cuda.stream(self.streams[j]):
+                n = 0
+                try:
+                    while n < max_steps and not abort.is_set():
                         a.step()
                         n += 1
                         if n % poll_every == 0 or n == max_steps:
@@ -784,18 +806,11 @@ class SplitBatch:
                                 self.done_at.setdefault(g[i], now)
                             if not any(a.h_active[i] for i in range(len(g))):
                                 break
+                finally:
                     self.steps[j] = n
                     self.streams[j].synchronize()
-            except BaseException as e:    # re-raised on the calling thread
-                errors.append(e)
 
-        threads = [threading.Thread(target=work, args=(j,)) for j in range(len(self.arenas))]
-        for t in threads:
-            t.start()
-        for t in threads:
-            t.join()
-        if errors:
-            raise errors[0]
+        run_workers(work, len(self.arenas))
         torch.cuda.synchronize()
         return list(self.steps)
 

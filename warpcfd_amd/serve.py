@@ -286,6 +286,7 @@ class ContinuousBatchServer:
         load = [0] * L           # resident cells per lane
         full = {}                # request id -> lanes that found no space for it
         closing = threading.Event()
+        abort = threading.Event()        # a lane failed: every lane stops
         errors = []
 
         def lane(j):
@@ -294,7 +295,7 @@ class ContinuousBatchServer:
             steps = 0
             try:
                 with torch.cuda.stream(self.streams[j]):
-                    while steps < max_steps:
+                    while steps < max_steps and not abort.is_set():
                         with lock:
                             for r in list(queue):
                                 if not a.fits(r.spec):
@@ -332,6 +333,7 @@ class ContinuousBatchServer:
                     self.streams[j].synchronize()
             except BaseException as e:    # re-raised on the calling thread
                 errors.append(e)
+                abort.set()
                 closing.set()
 
         torch.cuda.synchronize()          # the arenas were built on the default stream
