@@ -81,3 +81,31 @@ def test_trace_summary_splits_exits_at_entry(tmp_path):
     assert (inst['working'], inst['exit_at_entry']) == (1, 1)
     assert abs(s['avg_us'] - sum(durs['void wcfd::k_cgs_iter2<E, false, false>']
                                  + durs['void wcfd::k_cgs_iter2<E, false, true>']) / 10) < 1e-3
+
+
+def test_overlap_summary_counts_concurrent_kernels(tmp_path):
+    """Three dispatches, [0, 10], [5, 15] and [20, 30] ns: 30 ns of kernel
+    time in a 25 ns union of busy intervals, 1.2 kernels running on average
+    whenever one runs; a second trace file in another folder is merged."""
+    import csv
+    from warpcfd_amd.pmc import overlap_summary
+    for sub, ivs in (('a', [(0, 10), (20, 30)]), ('b', [(5, 15)])):
+        d = tmp_path / sub
+        d.mkdir()
+        with open(d / 'run_kernel_trace.csv', 'w', newline='') as f:
+            w = csv.writer(f)
+            w.writerow(['Kernel_Name', 'Start_Timestamp', 'End_Timestamp'])
+            for b, e in ivs:
+                w.writerow(['k', b, e])
+    s = overlap_summary(str(tmp_path))
+    assert s['dispatches'] == 3
+    assert s['mean_concurrent'] == 1.2
+    # serial kernels: the ratio is exactly 1
+    d = tmp_path / 'c'
+    d.mkdir()
+    with open(d / 'run_kernel_trace.csv', 'w', newline='') as f:
+        w = csv.writer(f)
+        w.writerow(['Kernel_Name', 'Start_Timestamp', 'End_Timestamp'])
+        for b, e in [(100, 110), (120, 125)]:
+            w.writerow(['k', b, e])
+    assert overlap_summary(str(d))['mean_concurrent'] == 1.0

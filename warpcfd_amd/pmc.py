@@ -19,6 +19,12 @@ below the kernel's typical bytes or duration), as the library profiler does.
     python -m warpcfd_amd.pmc traffic DIR k_cgs_iter 512
     rocprofv3 --kernel-trace --stats --output-format csv -d TDIR -o run -- python3 bench.py ...
     python -m warpcfd_amd.pmc trace TDIR k_cgs_iter
+    python -m warpcfd_amd.pmc overlap TDIR
+
+Overlap (kernels from several streams running at once): the sum of every
+dispatch's duration against the union of the intervals in which at least one
+kernel ran; their ratio is the mean number of kernels running whenever one
+is.
 """
 import csv
 import glob
@@ -143,10 +149,39 @@ def trace_summary(trace_dir, kernel_substr):
     return out
 
 
+def overlap_summary(trace_dir):
+    """Dispatch count, traced span, summed kernel durations and the union of
+    the busy intervals (seconds) of a ``--kernel-trace`` run, and their
+    ratio: the mean number of kernels running whenever at least one runs."""
+    paths = glob.glob(os.path.join(trace_dir, '**', '*kernel_trace.csv'), recursive=True)
+    if not paths:
+        raise FileNotFoundError(f'no *kernel_trace.csv under {trace_dir}')
+    iv = []
+    for p in paths:
+        with open(p) as f:
+            for r in csv.DictReader(f):
+                iv.append((int(r['Start_Timestamp']), int(r['End_Timestamp'])))
+    iv.sort()
+    busy = sum(e - b for b, e in iv)
+    union, cur_b, cur_e = 0, iv[0][0], iv[0][1]
+    for b, e in iv[1:]:
+        if b > cur_e:
+            union += cur_e - cur_b
+            cur_b, cur_e = b, e
+        elif e > cur_e:
+            cur_e = e
+    union += cur_e - cur_b
+    return {'dispatches': len(iv), 'span_s': round((max(e for _, e in iv) - iv[0][0]) * 1e-9, 3),
+            'kernel_time_s': round(busy * 1e-9, 3), 'busy_union_s': round(union * 1e-9, 3),
+            'mean_concurrent': round(busy / union, 3) if union else None}
+
+
 if __name__ == '__main__':
     import json
     if sys.argv[1] == 'trace':
         print(json.dumps(trace_summary(sys.argv[2], sys.argv[3]), indent=1))
+    elif sys.argv[1] == 'overlap':
+        print(json.dumps(overlap_summary(sys.argv[2]), indent=1))
     else:
         a = sys.argv[2:] if sys.argv[1] == 'traffic' else sys.argv[1:]
         print(json.dumps(traffic(a[0], a[1], float(a[2])), indent=1))
