@@ -171,7 +171,8 @@ class ContinuousBatchServer:
     requests into its arena, steps it and releases what finished, so one
     lane's long Krylov solves do not hold up the others' iterations, and the
     lanes' kernels overlap on the GPU (as ``batch.SplitBatch``).  A request
-    goes to the least loaded lane (resident cells) that has space for it.
+    goes to the least loaded lane (resident cells) that has space for it, so
+    it must fit one lane's share of the budget (larger ones are rejected).
     Results are bitwise the same in any lane."""
 
     def __init__(self, hbm_bytes=16 << 30, max_cases=64, keep_fields=False, lanes=1,
