@@ -737,8 +737,11 @@ def bench_cylinder(args):
         sv.outer_iteration(dt=dt)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for i in range(args.steps):
         sv.outer_iteration(dt=dt)
+        if (i + 1) % 500 == 0:        # long runs (the configured 2 000 steps) report progress
+            print(f'# PISO step {i + 1}: {time.perf_counter() - t0:.1f} s', file=sys.stderr,
+                  flush=True)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     hist = sv.history[-args.steps:]
@@ -767,8 +770,10 @@ def bench_cylinder(args):
             s64 = IncompressibleSolver(m64, None, None, steady_state=False, implicit_outflow=True)
             s64.set_iterations(1, 2, 1)
             s64.cg_variant, s64.p_refine, s64.tol = sv.cg_variant, sv.p_refine, sv.tol
-            for _ in range(args.warmup + args.steps):
+            for i in range(args.warmup + args.steps):
                 s64.outer_iteration(dt=dt)
+                if (i + 1) % 500 == 0:
+                    print(f'# float64 PISO step {i + 1}', file=sys.stderr, flush=True)
             return m64.cell_values.numpy()
         fp64 = fp64_error(fields32, run64)
         fp64['note'] = (f'the same workload through the double-field build (libwcfd64.so), same '
