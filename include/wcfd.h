@@ -203,6 +203,16 @@ int wcfd_face_interpolation(const wcfd_mesh* m, wcfd_bc bc, const wcfd_real* cel
                             int skew_mask, wcfd_real* mass_flux, const wcfd_real* rua_cell,
                             wcfd_real* rua_face, void* stream);
 
+/* FVM.face_interpolation(upwind=True) (model.py:210-229 with
+ * interpolation_functions.py:18-28): an internal face takes its owner's value
+ * where mass_flux (per face, read) is > 0, else its neighbour's, plus the
+ * same skew correction as the linear scheme for the fields in skew_mask;
+ * Neumann boundary faces as wcfd_face_interpolation. */
+int wcfd_face_interpolation_upwind(const wcfd_mesh* m, wcfd_bc bc, const wcfd_real* cell_values,
+                                   const wcfd_real* cell_grad, const wcfd_real* mass_flux,
+                                   wcfd_real* face_values, int field_mask, int skew_mask,
+                                   void* stream);
+
 /* FVM.calculate_gradients (model.py:232-239; model_kernels.py:72-90):
  * Green-Gauss gradient of the fields in field_mask; when zero_others != 0 the
  * gradients of the other fields are zeroed, as the reference's zero_() does. */

@@ -103,8 +103,11 @@ class OracleFVM:
             self.face_values[b.boundary_ids[d], j] = b.values[d, j]
             self.face_gradients[b.boundary_ids[nmn], j] = b.values[nmn, j]
 
-    def face_interpolation(self, outputs=(0, 1, 2, 3)):
-        """model.py:210-229; interpolation_kernels.py:5-54 (linear + skew)."""
+    def face_interpolation(self, outputs=(0, 1, 2, 3), upwind=False):
+        """model.py:210-229; interpolation_kernels.py:5-54 (linear + skew).
+        ``upwind``: interpolation_functions.py:18-28, the owner's value where
+        the face's mass flux is > 0, else the neighbour's (a zero flux takes
+        the neighbour), plus the same skew correction."""
         m = self.m
         fi = m.internal_face_ids
         own = m.adjacent_cells[fi, 0]
@@ -116,8 +119,12 @@ class OracleFVM:
         # skewness_functions.py:7-20
         rk_dash = self.centroids[own] * psi + self.centroids[nb] * psi2
         rk = self.face_centroid[fi] - rk_dash
+        up = self.mass_fluxes[fi] > 0
         for o in outputs:
-            val = cv[own, o] * psi[:, 0] + psi2[:, 0] * cv[nb, o]
+            if upwind:
+                val = np.where(up, cv[own, o], cv[nb, o])
+            else:
+                val = cv[own, o] * psi[:, 0] + psi2[:, 0] * cv[nb, o]
             gi = cg[own, o] * psi + psi2 * cg[nb, o]
             skew = gi[:, 0] * rk[:, 0] + gi[:, 1] * rk[:, 1] + gi[:, 2] * rk[:, 2]
             self.face_values[fi, o] = val + skew

@@ -70,6 +70,29 @@ def test_face_interpolation_and_flux(gpu):
     assert rel_l2(model.mass_fluxes.cpu().numpy(), of.mass_fluxes) <= KERNEL_TOL
 
 
+def test_face_interpolation_upwind(gpu):
+    """face_interpolation(upwind=True) (interpolation_functions.py:18-28)
+    against the oracle: random cell values and gradients (so the skew term
+    is live on the jittered mesh), the same mass fluxes on both sides, of
+    both signs and with some exact zeros (which take the neighbour)."""
+    torch = _torch()
+    of, model = _pair()
+    _set_state(model, of, seed=4)
+    rng = np.random.default_rng(8)
+    flux = rng.standard_normal(of.mass_fluxes.shape).astype(np.float32)
+    flux[::5] = 0.0
+    of.mass_fluxes[:] = flux
+    dev = model._cv.device
+    model._flux.copy_(model.dm.faces_to_device(torch.from_numpy(flux).to(dev)))
+    of.set_boundary_conditions(); of.face_interpolation(upwind=True)
+    model.set_boundary_conditions(); model.face_interpolation(upwind=True)
+    got = model.face_values.cpu().numpy()
+    assert rel_l2(got, of.face_values) <= KERNEL_TOL
+    # the linear scheme gives other face values on the same state
+    model.face_interpolation()
+    assert not np.array_equal(model.face_values.cpu().numpy(), got)
+
+
 def test_green_gauss_and_divergence(gpu):
     of, model = _pair()
     _set_state(model, of, seed=3)

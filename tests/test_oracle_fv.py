@@ -251,3 +251,54 @@ def test_relax_and_replace_row():
     assert A[0, 0] == 1.0 and np.all(A[0, 1:] == 0)
     assert np.any(A[1:, 0] != 0)                  # T7: the column stays, A is non-symmetric
     assert peq.rhs[0] == 0.25
+
+
+@pytest.mark.parametrize('jitter', [0.0, 0.12])
+def test_upwind_face_values_take_the_upwind_cell(jitter):
+    """interpolation_functions.py:18-28: with zero gradients (no skew term) an
+    internal face takes its owner's value where its mass flux is > 0 and its
+    neighbour's where it is < 0 or exactly 0; boundary faces are untouched by
+    the scheme.  Random cell values, fluxes of every sign and some zeros."""
+    nodes, cells = structured_hex((5, 4, 3), (1.0, 0.8, 0.6), jitter=jitter, seed=4)
+    f = _fvm(nodes, cells)
+    rng = np.random.default_rng(5)
+    f.cell_values[:] = rng.standard_normal(f.cell_values.shape)
+    f.cell_gradients[:] = 0
+    flux = rng.standard_normal(f.mass_fluxes.shape)
+    flux[::7] = 0.0
+    f.mass_fluxes[:] = flux
+    f.face_interpolation(upwind=True)
+    m = f.m
+    fi = m.internal_face_ids
+    own, nb = m.adjacent_cells[fi, 0], m.adjacent_cells[fi, 1]
+    for o in range(4):
+        want = np.where(flux[fi] > 0, f.cell_values[own, o], f.cell_values[nb, o])
+        np.testing.assert_array_equal(f.face_values[fi, o], want)
+    assert np.any(flux[fi] == 0) and np.any(flux[fi] < 0) and np.any(flux[fi] > 0)
+
+
+def test_upwind_and_linear_share_the_skew_term():
+    """On a jittered grid with non-zero gradients, upwind minus linear face
+    values equal upwind minus linear interpolants of the cell values alone:
+    the skew correction (skewness_functions.py:7-20) is the same term in
+    both schemes."""
+    nodes, cells = structured_hex((5, 4, 3), (1.0, 0.8, 0.6), jitter=0.12, seed=6)
+    f = _fvm(nodes, cells)
+    rng = np.random.default_rng(7)
+    f.cell_values[:] = rng.standard_normal(f.cell_values.shape)
+    f.cell_gradients[:] = rng.standard_normal(f.cell_gradients.shape)
+    f.mass_fluxes[:] = rng.standard_normal(f.mass_fluxes.shape)
+    f.face_interpolation(upwind=True)
+    up = f.face_values.copy()
+    f.face_interpolation()
+    lin = f.face_values.copy()
+    m = f.m
+    fi = m.internal_face_ids
+    own, nb = m.adjacent_cells[fi, 0], m.adjacent_cells[fi, 1]
+    psi, psi2 = f.psi[fi], f.psi2[fi]
+    for o in range(4):
+        cv = f.cell_values[:, o]
+        d_val = np.where(f.mass_fluxes[fi] > 0, cv[own], cv[nb]) - (cv[own] * psi + psi2 * cv[nb])
+        np.testing.assert_allclose(up[fi, o] - lin[fi, o], d_val, atol=1e-12)
+    skew = lin[fi, 0] - (f.cell_values[own, 0] * psi + psi2 * f.cell_values[nb, 0])
+    assert np.abs(skew).max() > 1e-3          # the skew term is really there

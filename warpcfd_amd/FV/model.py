@@ -182,16 +182,20 @@ class FVM:
                                                          15, self._s()), 'set_boundary_conditions')
 
     def face_interpolation(self, output_indices=None, upwind=False):
-        """model.py:210-229 (linear + skew correction; ``upwind`` is not used
-        by the reference solver and is not provided)."""
+        """model.py:210-229: linear + skew correction, or with ``upwind`` the
+        upwind cell's value by the sign of the model's mass fluxes
+        (interpolation_functions.py:18-28) + the same skew correction."""
         self._need()
-        if upwind:
-            raise NotImplementedError('upwind face interpolation is not on the solver path')
         idx = self._indices(output_indices)
         mask = self._mask(idx)
         skew = mask if self.skew_correction else 0
         if self._uvw_grad_zero:
             skew &= 8
+        if upwind:
+            _lib.check(self.lib.wcfd_face_interpolation_upwind(
+                self.dm.desc_ref, self.bc, ptr(self._cv), ptr(self._cg), ptr(self._flux),
+                ptr(self._fv), mask, skew, self._s()), 'face_interpolation(upwind=True)')
+            return
         _lib.check(self.lib.wcfd_face_interpolation(self.dm.desc_ref, self.bc, ptr(self._cv),
                                                     ptr(self._cg), ptr(self._fv), mask, skew, None,
                                                     None, None, self._s()), 'face_interpolation')

@@ -91,6 +91,8 @@ _SIGS = {
     'wcfd_set_boundary_conditions': (ctypes.c_int, [_PM, WcfdBC, c_p, ctypes.c_int, c_p]),
     'wcfd_face_interpolation': (ctypes.c_int, [_PM, WcfdBC, c_p, c_p, c_p, ctypes.c_int,
                                                ctypes.c_int, c_p, c_p, c_p, c_p]),
+    'wcfd_face_interpolation_upwind': (ctypes.c_int, [_PM, WcfdBC, c_p, c_p, c_p, c_p,
+                                                      ctypes.c_int, ctypes.c_int, c_p]),
     'wcfd_green_gauss': (ctypes.c_int, [_PM, c_p, c_p, ctypes.c_int, ctypes.c_int, c_p]),
     'wcfd_mass_flux': (ctypes.c_int, [_PM, c_p, c_p, c_p]),
     'wcfd_divergence': (ctypes.c_int, [_PM, c_p, c_p, c_p]),

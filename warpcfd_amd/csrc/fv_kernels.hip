@@ -165,6 +165,43 @@ __global__ void k_face_interp(wcfd_mesh m, BatchDev bd, wcfd_bc bc, const real* 
   }
 }
 
+// face_interpolation(upwind=True) (model.py:210-229 with
+// interpolation_functions.py:18-28): an internal face takes its owner's
+// value where its mass flux is > 0 and its neighbour's otherwise, plus the
+// linear scheme's skew correction (the same expression as
+// face_interp_internal); boundary faces as in the linear scheme (Neumann
+// faces only).  Off the solver's path, so one plain face per thread.
+__global__ void k_face_interp_upwind(wcfd_mesh m, wcfd_bc bc, const real* __restrict__ cv,
+                                     const real* __restrict__ cg,
+                                     const real* __restrict__ flux, real* __restrict__ fv,
+                                     int mask, int skew_mask) {
+  const int f = xcd_item();
+  const int C = m.n_cells, F = m.n_faces, Fi = m.n_internal;
+  if (f >= F) return;
+  if (f >= Fi) {
+    face_interp_boundary(m, bc, f, cv, fv, mask, nullptr, nullptr, nullptr, 0);
+    return;
+  }
+  const int o = m.owner[f], n = m.neigh[f];
+  const real psi = m.psi[f];
+  const real psi2 = 1.0f - psi;
+  const real s0 = m.skew[f], s1 = m.skew[Fi + f], s2 = m.skew[2 * Fi + f];
+  const bool up = flux[f] > 0.f;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (!((mask >> q) & 1)) continue;
+    real v = up ? cv[q * C + o] : cv[q * C + n];
+    if ((skew_mask >> q) & 1) {
+      const real* gq = cg + (size_t)(3 * q) * C;
+      const real g0 = gq[o] * psi + psi2 * gq[n];
+      const real g1 = gq[C + o] * psi + psi2 * gq[C + n];
+      const real g2 = gq[2 * C + o] * psi + psi2 * gq[2 * C + n];
+      v = v + (g0 * s0 + g1 * s1 + g2 * s2);
+    }
+    fv[q * F + f] = v;
+  }
+}
+
 // Owner data of a workgroup's internal faces staged in LDS.  Internal faces
 // are numbered by owner (per case in a batch, and the compacted item ranges
 // never span two cases), so the owners of one workgroup's faces are one cell
@@ -931,6 +968,17 @@ int wcfd_face_interpolation(const wcfd_mesh* m, wcfd_bc bc, const real* cell_val
   if ((rua_cell == nullptr) != (rua_face == nullptr)) return WCFD_EINVAL;
   return launch_face_interp(*m, BatchDev{}, bc, cell_values, cell_grad, face_values, field_mask & 15,
                             skew_mask & 15, mass_flux, rua_cell, rua_face, 0, (hipStream_t)stream);
+}
+
+int wcfd_face_interpolation_upwind(const wcfd_mesh* m, wcfd_bc bc, const real* cell_values,
+                                   const real* cell_grad, const real* mass_flux,
+                                   real* face_values, int field_mask, int skew_mask,
+                                   void* stream) {
+  if (!mesh_ok(m) || !cell_values || !cell_grad || !mass_flux || !face_values) return WCFD_EINVAL;
+  if (m->n_faces <= 0) return WCFD_OK;
+  WCFD_LAUNCH(K_FACE_INTERP, (hipStream_t)stream, k_face_interp_upwind, grid_for(m->n_faces), *m,
+              bc, cell_values, cell_grad, mass_flux, face_values, field_mask & 15, skew_mask & 15);
+  return check_launch();
 }
 
 int wcfd_green_gauss(const wcfd_mesh* m, const real* face_values, real* cell_grad, int field_mask,
