@@ -93,6 +93,20 @@ def test_face_interpolation_upwind(gpu):
     assert not np.array_equal(model.face_values.cpu().numpy(), got)
 
 
+def test_init_global_arrays_zeroes_the_fields(gpu):
+    """init_global_arrays (model.py:155-176), which finalize calls, gives
+    zeroed cell values, gradients, face values and mass fluxes again after
+    a state has been set and the face pass run."""
+    of, model = _pair()
+    _set_state(model, of, seed=5)
+    model.set_boundary_conditions(); model.face_interpolation(); model.calculate_mass_flux()
+    assert np.abs(model.cell_values.cpu().numpy()).max() > 0
+    model.init_global_arrays()
+    for arr in (model.cell_values, model.cell_gradients, model.face_values, model.mass_fluxes):
+        a = arr.cpu().numpy()
+        assert a.size and not a.any()
+
+
 def test_green_gauss_and_divergence(gpu):
     of, model = _pair()
     _set_state(model, of, seed=3)

@@ -77,16 +77,23 @@ class FVM:
         self.dm = DeviceMesh(self.layout, dev, dtype=self.tdtype)
         self.bc_type, self.bc_value = self.boundary.device_arrays(self.layout, dev, self.tdtype)
         self.bc = _lib.WcfdBC(self.bc_type.data_ptr(), self.bc_value.data_ptr())
+        self.lib = self.dm.lib
+        self.init_global_arrays()
+        self.finalized = True
+
+    def init_global_arrays(self):
+        """model.py:155-176: zeroed cell values and gradients, face values
+        and mass fluxes, and the reduction workspace (field-major, device
+        cell and face order; face gradients are computed on demand)."""
+        dev = self.dm.device
         C, F = self.num_cells, self.dm.n_faces        # device faces (with a null face)
         z = lambda *s: torch.zeros(*s, dtype=self.tdtype, device=dev)
         self._cv = z(4, C)
         self._cg = z(12, C)
         self._fv = z(4, F)
         self._flux = z(F)
-        self.lib = self.dm.lib
         self._reduce_ws = torch.zeros(self.lib.wcfd_reduce_workspace_bytes(F), dtype=torch.uint8,
                                       device=dev)
-        self.finalized = True
 
     def _need(self):
         if not self.finalized:
