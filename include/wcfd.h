@@ -58,6 +58,8 @@ typedef WCFD_REAL wcfd_real;
 #define WCFD_ENOTCONV 3
 #define WCFD_ENAN 4
 #define WCFD_ESCRATCH 5 /* input larger than a no-allocation entry point's static scratch */
+#define WCFD_EMESH 6     /* a face shared by more than two cells (wcfd_match_faces) */
+#define WCFD_ECAPACITY 7 /* an output array too small for the result (wcfd_match_faces) */
 
 #define WCFD_DIRICHLET 1
 #define WCFD_NEUMANN 2
@@ -622,6 +624,26 @@ int wcfd_profile_skipped(int32_t* counts, int n_ids); /* [host] array */
  * (at most capacity), or minus an error code. */
 int wcfd_profile_timeline(int32_t* ids, double* start_ms, double* end_ms,
                           int capacity); /* [host] arrays */
+
+/* Face matching of the mesh preprocessing, on the host (no GPU work; host
+ * scratch of about 20 bytes per slot plus 4 per node, freed on return).
+ * Replaces the unique-face numbering and the pairing of shared faces in
+ * Mesh.get_mesh_properties (warp_cfd/preprocess/mesh.py:108-148 and
+ * 220-302, np.unique over the sorted node ids and the owner/neighbour pair
+ * search).  face_nodes [host] (n_cells, faces_per_cell, nodes_per_face)
+ * node ids, nodes_per_face <= 4, one face set (every face of the set has
+ * that many nodes).  Writes face_of_slot [host] (n_cells * faces_per_cell):
+ * the face id of each (cell, local face) slot; unique_nodes [host]
+ * (capacity, nodes_per_face): each face's node ids in ascending order,
+ * faces numbered in the lexicographic order of those rows; face_slots
+ * [host] (capacity, 2): the face's slots c * faces_per_cell + k in
+ * ascending order, -1 in the second for a boundary face.  Returns the
+ * number of faces, or minus WCFD_EINVAL (bad sizes, a negative node id),
+ * WCFD_EMESH (a face held by three or more slots) or WCFD_ECAPACITY (more
+ * than `capacity` faces). */
+int64_t wcfd_match_faces(const int32_t* face_nodes, int64_t n_cells, int32_t faces_per_cell,
+                         int32_t nodes_per_face, int32_t* face_of_slot, int32_t* unique_nodes,
+                         int32_t* face_slots, int64_t capacity);
 
 #ifdef __cplusplus
 }

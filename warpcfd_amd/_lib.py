@@ -16,6 +16,7 @@ LIB64_PATH = os.path.join(_HERE, LIB64_NAME)
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), 'include', 'wcfd.h')
 
 WCFD_OK, WCFD_EINVAL, WCFD_EHIP, WCFD_ENOTCONV, WCFD_ENAN, WCFD_ESCRATCH = 0, 1, 2, 3, 4, 5
+WCFD_EMESH, WCFD_ECAPACITY = 6, 7
 NOALLOC_MAX_BLOCKS = 16384
 DIRICHLET, NEUMANN = 1, 2
 SCHEME_CORRECTED, SCHEME_IMPLICIT_OUTFLOW = 1, 2
@@ -153,6 +154,9 @@ _SIGS = {
     'wcfd_profile_timeline': (ctypes.c_int, [ctypes.POINTER(ctypes.c_int32),
                                               ctypes.POINTER(ctypes.c_double),
                                               ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
+    'wcfd_match_faces': (ctypes.c_int64, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32,
+                                          ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_int64]),
     'wcfd_debug_krylov_persistent_shrink': (ctypes.c_int, [ctypes.c_int]),
     'wcfd_simple_workspace_bytes': (ctypes.c_size_t, [c_i32, c_i32]),
     'wcfd_kernel_count': (ctypes.c_int, []),

@@ -10,7 +10,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SOURCES = ['csrc/fv_kernels.hip', 'csrc/krylov.hip', 'csrc/simple.hip', 'csrc/terms.hip',
-           'csrc/profile.cpp']
+           'csrc/profile.cpp', 'csrc/mesh.cpp']
 OUT = os.path.join(HERE, 'libwcfd.so')
 OUT64 = os.path.join(HERE, 'libwcfd64.so')
 ARCH = os.environ.get('WCFD_OFFLOAD_ARCH', 'gfx950')

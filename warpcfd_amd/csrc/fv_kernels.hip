@@ -940,6 +940,8 @@ const char* wcfd_error_string(int code) {
     case WCFD_ESCRATCH:
       return "input exceeds the static scratch of the no-allocation entry point "
              "(WCFD_NOALLOC_MAX_BLOCKS workgroups)";
+    case WCFD_EMESH: return "a face is shared by more than two cells";
+    case WCFD_ECAPACITY: return "output array too small";
     default: return "unknown error";
   }
 }

@@ -63,6 +63,30 @@ class _Props:
     pass
 
 
+def match_faces(local):
+    """Unique faces of one face set and the slots holding each, by
+    wcfd_match_faces (host code in libwcfd.so).  local: (C, Fs, Ns) node
+    ids.  Returns (unique (K, Ns) ascending node ids in lexicographic row
+    order, face id per slot (C, Fs), slots (K, 2) c * Fs + k ascending, -1
+    in the second for a boundary face): the numbering of np.unique over the
+    sorted rows (mesh.py:108-148) and the pairs of mesh.py:263-288."""
+    from .. import _lib
+    lib = _lib.load()
+    local = np.ascontiguousarray(local, dtype=np.int32)
+    C, Fs, Ns = local.shape
+    cap = C * Fs
+    face_of = np.empty((C, Fs), np.int32)
+    uniq = np.empty((cap, Ns), np.int32)
+    slots = np.empty((cap, 2), np.int32)
+    k = lib.wcfd_match_faces(local.ctypes.data, C, Fs, Ns, face_of.ctypes.data,
+                             uniq.ctypes.data, slots.ctypes.data, cap)
+    if k < 0:
+        if -k == _lib.WCFD_EMESH:
+            raise AssertionError('a face is shared by more than two cells')
+        _lib.check(-k, 'match_faces')
+    return uniq[:k].copy(), face_of, slots[:k].astype(np.int64)
+
+
 def _fan(coords):
     v0 = coords[..., 0, :]
     parts = [0.5 * np.cross(coords[..., i, :] - v0, coords[..., i + 1, :] - v0)
@@ -146,12 +170,15 @@ class Mesh:
     def get_mesh_properties(self):
         C = self.cells.shape[0]
         N = max(t.shape[1] for t in self._sets)
-        parts, shift = [], 0
+        Fc = self.faces_per_cell
+        parts, shift, col, pairs = [], 0, 0, []
         for t in self._sets:                                          # mesh.py:108-148
             local = self.cells[:, t]                                  # (C,Fs,Ns)
-            keyed = np.sort(local, axis=2).reshape(-1, t.shape[1])
-            uq, inv = np.unique(keyed, axis=0, return_inverse=True)
-            cf = inv.reshape(C, t.shape[0]).astype(np.int32)
+            Fs = t.shape[0]
+            uq, cf, slots = match_faces(local)
+            # the face's slots as flat indices of cell_faces (C, Fc)
+            pairs.append(np.where(slots >= 0, (slots // Fs) * Fc + col + slots % Fs, -1))
+            col += Fs
             a, n = _fan(self.nodes[local])
             fc = self.nodes[uq].mean(axis=1)
             c2f_s = fc[cf] - self.cell_centroids[:, None, :]
@@ -168,17 +195,11 @@ class Mesh:
         c2f = np.concatenate([p[5] for p in parts], axis=1)
         num_nodes = np.concatenate([p[6] for p in parts])
         K = uniq.shape[0]
-        Fc = cell_faces.shape[1]
 
-        # pair the two (cell, local face) slots of every shared face
-        flat = cell_faces.ravel()
-        order = np.lexsort((np.arange(flat.size), flat))
-        counts = np.bincount(flat, minlength=K)
-        if counts.max() > 2:
-            raise AssertionError('a face is shared by more than two cells')
-        first = np.concatenate([[0], np.cumsum(counts)[:-1]])
-        fi = np.nonzero(counts == 2)[0].astype(np.int32)
-        a, b = order[first[fi]], order[first[fi] + 1]      # slot a has the lower cell id
+        # the two (cell, local face) slots of every shared face
+        pairs = np.concatenate(pairs, axis=0)
+        fi = np.nonzero(pairs[:, 1] >= 0)[0].astype(np.int32)
+        a, b = pairs[fi, 0], pairs[fi, 1]                  # slot a has the lower cell id
         ca, ka, cb, kb = a // Fc, a % Fc, b // Fc, b % Fc
 
         neighbors = np.full((C, Fc), -1, np.int32)
@@ -192,8 +213,8 @@ class Mesh:
         cfi[fi, 0], cfi[fi, 1] = ka, kb
         is_b = np.ones(K, np.uint8)
         is_b[fi] = 0
-        bf = np.nonzero(counts == 1)[0]
-        sb = order[first[bf]]
+        bf = np.nonzero(pairs[:, 1] < 0)[0]
+        sb = pairs[bf, 0]
         adjacent[bf, 0] = sb // Fc
         cfi[bf, 0] = sb % Fc
         dist = np.full((K, 3), -1.0, np.float32)
