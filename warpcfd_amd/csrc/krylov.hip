@@ -898,7 +898,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_xcd(
     real* w0, real* w1, real* s0, real* s1, const real* __restrict__ inv, double* P0,
     double* P1, KState* st, int maxiter, int* sync) {
   constexpr int KR = M::kRow;
-  __shared__ int sh[6];
+  __shared__ int sh[8];
   __shared__ int s_go;
   if (threadIdx.x == 0) {
     const int xcc = (int)(__builtin_amdgcn_s_getreg(kHwregXccId) & (kNumXcd - 1));
@@ -917,6 +917,8 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_xcd(
       part.v_hi = lo + (int)((int64_t)(slot + 1) * nbx / pl.per_xcc);
       part.members = (int)kNumXcd * pl.per_xcc;
       ok = ok && sg.n_cases == 1;
+      sh[6] = lo;          // the XCD's band of virtual workgroups
+      sh[7] = lo + nbx;
     } else {
       ok = ok && xcd_part(sg, st, pl, xcc, slot, part);
     }
@@ -960,6 +962,16 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_xcd(
   real a[RV][KR], ic[RV][KR], xr[RV], pr[RV];
   uint32_t o[RV][KR];
   bool has[RV];
+  // GRID: rows of the XCD's band, [band_lo, band_hi) (its last one may end
+  // past the case: no column reaches there)
+  const uint32_t band_lo = GRID ? (uint32_t)(c0 + (sh[6] - b0) * sg.chunk) : 0u;
+  const uint32_t band_hi = GRID ? (uint32_t)(c0 + (sh[7] - b0) * sg.chunk) : 0u;
+  // GRID: the row is read by another XCD.  The pattern is symmetric (mesh
+  // adjacency), so that is when one of the row's own columns lies outside
+  // the band.  Those rows are stored at agent scope (other XCDs' agent-scope
+  // loads see them); the rest are read only on this XCD and stored plainly,
+  // as the one-XCD form stores every row.
+  bool remote[RV];
 #pragma unroll
   for (int q = 0; q < RV; ++q) {
     const int vq = q / RR, rq = q % RR;   // virtual workgroup, row of the thread in it
@@ -968,8 +980,16 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_xcd(
     const int i = begin + (int)threadIdx.x + rq * (int)blockDim.x;
     has[q] = vq < nv && i < end;
     xr[q] = pr[q] = 0.f;
+    remote[q] = false;
     if (has[q]) {
       A.load_off(i, a[q], o[q]);
+      if constexpr (GRID) {
+#pragma unroll
+        for (int j = 1; j < KR; ++j) {
+          const uint32_t cj = o[q][j] >> 2;
+          remote[q] = remote[q] || cj < band_lo || cj >= band_hi;
+        }
+      }
       const FBuf I(inv, nb4);
 #pragma unroll
       for (int j = 0; j < KR; ++j) ic[q][j] = I(o[q][j]);
@@ -1041,7 +1061,7 @@ __global__ __launch_bounds__(kMaxKrylovBlock) void k_cgs_xcd(
           cg_row<KR, false>(a[q], wc[q], sc[q], rc[q], ic[q], pr[q], alpha, beta, wn, pi, si, rn);
         xr[q] = mad(alpha, pi, xr[q]);
         pr[q] = pi;
-        if constexpr (GRID) {
+        if (GRID && remote[q]) {
           st_coh_off(s_out, o[q][0], si);
           st_coh_off(r_out, o[q][0], rn);
           st_coh_off(w_out, o[q][0], wn);
