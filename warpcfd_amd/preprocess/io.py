@@ -84,8 +84,18 @@ def _sections(text):
     return out
 
 
+def _parse(s, dtype, count=None):
+    """Whitespace-separated numbers of s, parsed in C (np.fromstring's text
+    mode); a token that is not a number ends the parse, which the count
+    check reports."""
+    a = np.fromstring(s, dtype=dtype, sep=' ')
+    if count is not None and a.size != count:
+        raise ValueError(f'expected {count} numbers, read {a.size}')
+    return a
+
+
 def _ints(s):
-    return np.array(s.split(), np.int64)
+    return _parse(s, np.int64)
 
 
 def read_msh(path):
@@ -154,7 +164,7 @@ def _finish(tags, coords, blocks, names):
 def _read_msh2(sec, names):
     lines = sec['Nodes'][0].splitlines()
     n = int(lines[0])
-    a = np.array(' '.join(lines[1:n + 1]).split(), np.float64).reshape(n, 4)
+    a = _parse(' '.join(lines[1:n + 1]), np.float64, 4 * n).reshape(n, 4)
     tags, coords = a[:, 0].astype(np.int64), a[:, 1:]
     lines = sec['Elements'][0].splitlines()
     rows = lines[1:int(lines[0]) + 1]
@@ -165,7 +175,7 @@ def _read_msh2(sec, names):
     keys = []
     for wd in np.unique(width):
         sel = np.nonzero(width == wd)[0]
-        arr = np.array(' '.join(rows[i] for i in sel).split(), np.int64).reshape(sel.size, -1)
+        arr = _parse(' '.join(rows[i] for i in sel), np.int64).reshape(sel.size, -1)
         for et in np.unique(arr[:, 1]):
             m = arr[:, 1] == et
             ntags = arr[m, 2]
@@ -210,9 +220,9 @@ def _read_msh4(sec, names):
     for _ in range(n_blocks):
         ed, et, para, nn = (int(x) for x in lines[row].split())
         row += 1
-        tags.append(np.array(lines[row:row + nn], np.int64) if nn else np.zeros(0, np.int64))
+        tags.append(_parse(' '.join(lines[row:row + nn]), np.int64, nn))
         row += nn
-        xyz = np.array(' '.join(lines[row:row + nn]).split(), np.float64).reshape(nn, -1)[:, :3]
+        xyz = _parse(' '.join(lines[row:row + nn]), np.float64).reshape(nn, -1)[:, :3]
         coords.append(xyz)
         row += nn
     tags = np.concatenate(tags)
@@ -225,7 +235,7 @@ def _read_msh4(sec, names):
         ed, et, etype, ne = (int(x) for x in lines[row].split())
         row += 1
         if ne:
-            block = np.array(' '.join(lines[row:row + ne]).split(), np.int64).reshape(ne, -1)
+            block = _parse(' '.join(lines[row:row + ne]), np.int64).reshape(ne, -1)
             blocks.append((etype, ent_phys.get((ed, et)), block[:, 1:]))
         row += ne
     return _finish(tags, coords, blocks, names)
