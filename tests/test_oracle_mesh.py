@@ -196,3 +196,20 @@ def test_cavity_groups():
     om = OracleMesh(nodes, cells)
     for k, v in om.define_boundary_walls().items():
         assert np.array_equal(v, walls[k])
+
+
+def test_product_connectivity_matches_oracle_full_channel():
+    """The native face matching at the full 200 x 40 x 40 channel (320 000
+    cells, 977 600 faces): face numbering, owner/neighbour pairs and the
+    per-cell tables bit-exact against the oracle's np.unique restatement."""
+    from warpcfd_amd.preprocess.grid import channel_mesh
+    nodes, cells = channel_mesh(200, 40, 40)
+    om = OracleMesh(nodes, cells)
+    pm = Mesh(nodes, cells)
+    fp, cp = pm.face_properties, pm.cell_properties
+    assert fp.num_faces == 977600
+    for k, v in {'adjacent_cells': fp.adjacent_cells, 'cell_face_index': fp.cell_face_index,
+                 'neighbors': cp.neighbors, 'face_side': cp.face_side,
+                 'is_boundary': fp.is_boundary, 'unique_faces': fp.unique_faces,
+                 'cell_faces': cp.faces}.items():
+        assert np.array_equal(getattr(om, k), v), k
